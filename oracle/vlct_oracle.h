@@ -1,0 +1,144 @@
+/*
+ * vlct_oracle.h -- CPU oracle for the Enzo-E VL+CT MHD step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C, double-precision restatement
+ * of the reference algorithm (src/Enzo/hydro-mhd/EnzoMethodMHDVlct.cpp and the
+ * toolkit/riemann files it drives), written operation by operation so that the
+ * floating-point evaluation order follows the reference expressions.  It is
+ * the checker for the HIP library, never the product: only tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py may load it.
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   - end to end: the reference's golden L1 errors for the inclined MHD and HD
+ *     linear waves (input/vlct/run_MHD_linear_wave_test.py:178-188,
+ *     input/vlct/run_HD_linear_wave_test.py:70-80) and the RJ2a shock tube
+ *     (input/vlct/run_MHD_shock_tube_test.py:82-87) at rtol 1e-13 /
+ *     atol 7e-14 (input/vlct/testing_utils.py:275-292);
+ *   - per function: closed forms, invariants and brute-force cases.
+ *
+ * Layout: every array is dense, row-major (z, y, x), x fastest, exactly as a
+ * Cello field with ghost zones (src/Cello/view_CelloView.hpp:507-512).
+ */
+#ifndef VLCT_ORACLE_H
+#define VLCT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cell-centred integration quantities (Cello field names) */
+enum {
+  O_RHO = 0, O_VX, O_VY, O_VZ, O_ETOT, O_BX, O_BY, O_BZ, O_NCC
+};
+
+enum { O_HLLE = 0, O_HLLC = 1, O_HLLD = 2 };
+enum { O_NN = 0, O_PLM = 1, O_PLM_ATHENA = 2 };
+
+typedef struct {
+  double gamma;
+  double density_floor;
+  double pressure_floor;
+  double theta_limiter;
+  int riemann;      /* O_HLLE / O_HLLC / O_HLLD                        */
+  int recon[2];     /* reconstructor per stage                         */
+  int nstages;      /* 2 = "vl", 1 = "euler"                            */
+  int mhd;          /* 1 = constrained_transport, 0 = no_bfield         */
+  double courant;
+} o_config;
+
+/* One Cello block: ghost-extended fields owned by the caller. */
+typedef struct {
+  int mx, my, mz;          /* cell-centred extents including ghosts   */
+  double cw[3];            /* cell widths along x, y, z               */
+  double *cc[O_NCC];       /* (mz, my, mx) each                       */
+  double *bfi[3];          /* bfieldi_x (mz,my,mx+1), _y (mz,my+1,mx),
+                              _z (mz+1,my,mx)                          */
+  double *pressure;        /* permanent "pressure" field (mz,my,mx)   */
+} o_block;
+
+/* ---- whole-method entry points ---------------------------------------- */
+
+/* EnzoMethodMHDVlct::timestep (EnzoMethodMHDVlct.cpp:551-588): writes the
+ * pressure field over the whole block and returns courant * dt_block.      */
+double o_vlct_timestep(const o_config *cfg, o_block *b);
+
+/* EnzoMethodMHDVlct::compute (EnzoMethodMHDVlct.cpp:356-500) for one block
+ * and one timestep dt.  Returns 0 on success.                              */
+int o_vlct_compute(const o_config *cfg, o_block *b, double dt);
+
+/* EnzoComputePressure::compute_pressure (EnzoComputePressure.cpp:82-198)
+ * over [stale, m-stale) into p.                                            */
+void o_compute_pressure(const o_config *cfg, const o_block *b, double *p,
+                        int stale_depth);
+
+/* ---- single-face Riemann solvers (LUT order in/out) -------------------- */
+/* MHD LUT: density, bfield_i, bfield_j, bfield_k, velocity_i, velocity_j,
+ * velocity_k, pressure|total_energy (EnzoRiemannImpl.hpp:40-56).
+ * Hydro LUT: density, velocity_i..k, pressure|total_energy (:21-36).      */
+void o_hlld_face(const double wl[8], const double wr[8], double gamma,
+                 double flux[8]);
+void o_hlle_mhd_face(const double wl[8], const double wr[8], double gamma,
+                     double flux[8]);
+void o_hllc_face(const double wl[5], const double wr[5], double gamma,
+                 double flux[5]);
+
+/* slope limiters (EnzoReconstructorPLM.hpp:287-346) */
+double o_limiter_enzo(double vm1, double v, double vp1, double theta);
+double o_limiter_athena(double vm1, double v, double vp1);
+
+/* ---- per-operation kernels (exposed for unit tests) -------------------- */
+/* Each takes dense arrays; see vlct_oracle.c for the region semantics.     */
+void o_reconstruct(int kind, double theta, const double *w, double *wl,
+                   double *wr, int mz, int my, int mx, int dim,
+                   int stale_depth, int use_floor, double floor_val);
+void o_riemann_solve(const o_config *cfg, double *const *pl,
+                     double *const *pr, double **flux, int mz, int my,
+                     int mx, int dim, int stale_depth);
+void o_update_bfield(const double cw[3], int dim, double *const *edge,
+                     const double *bcur, double *bout, double dt, int mz,
+                     int my, int mx, int stale_depth);
+void o_compute_center_bfield(int dim, double *bc, const double *bi, int mz,
+                             int my, int mx, int stale_depth);
+
+/* ---- multi-block harness ------------------------------------------------ */
+
+enum { O_BC_PERIODIC = 0, O_BC_OUTFLOW = 1 };
+
+/* A uniform grid of equally sized blocks covering [lower, upper].          */
+typedef struct {
+  int nbx, nby, nbz;       /* blocks per axis                           */
+  int nx, ny, nz;          /* active cells per block                    */
+  int g;                   /* ghost depth                               */
+  double lower[3], upper[3];
+  int bc[3];               /* boundary type per axis                    */
+  o_block *blocks;         /* nbx*nby*nbz, x fastest                    */
+} o_grid;
+
+int  o_grid_alloc(o_grid *gr);
+void o_grid_free(o_grid *gr);
+/* Fill ghost zones of every block from its neighbours / boundaries.        */
+void o_grid_refresh(const o_config *cfg, o_grid *gr);
+/* One Cello cycle: refresh, dt (min over blocks, clipped to t_stop), compute
+ * on every block.  Returns the dt taken.                                   */
+double o_grid_cycle(const o_config *cfg, o_grid *gr, double t, double t_stop,
+                    int nthreads);
+/* Run from t0 to t_stop; returns number of cycles.                         */
+int o_grid_evolve(const o_config *cfg, o_grid *gr, double t0, double t_stop,
+                  int nthreads, double *dt_log, int dt_log_cap);
+
+/* Initial conditions written by the harness (recipes of the reference's
+ * initializers): see vlct_harness.c.                                       */
+int o_init_inclined_wave(const o_config *cfg, o_grid *gr, const char *wave,
+                         double alpha, double beta, double amplitude,
+                         double lambda, int positive_vel);
+int o_init_shock_tube_rj2a(const o_config *cfg, o_grid *gr, int aligned_ax);
+
+/* Copy the active region of one field of the grid into a dense global array
+ * (nz_tot, ny_tot, nx_tot).  field: 0..7 cc fields, 8 = derived pressure.  */
+void o_grid_gather(const o_config *cfg, const o_grid *gr, int field,
+                   double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
