@@ -1,0 +1,512 @@
+// vlct_kernels.hpp -- gfx950 kernels of one VL+CT stage, the CFL reduction
+// and the ghost refresh, launched over a whole list of blocks at once.
+//
+// A stage (EnzoMHDIntegratorStageCommands::compute_update_stage,
+// src/Enzo/hydro-mhd/EnzoMHDIntegratorStageCommands.cpp:102-203) is four
+// launches, each covering every block of the batch (grid.y = block):
+//   k_flux  : primitives -> reconstruction -> Riemann flux on the three low
+//             faces of every cell (one HLLD/HLLE/HLLC solve per face)
+//   k_edge  : upwinded edge EMFs (Gardiner & Stone 2009) of constrained
+//             transport, the upwind weights taken from the sign of the
+//             density flux
+//   k_faces : face-centred B update from the edge EMFs
+//   k_cells : cell-centred B from faces, flux divergence, conserved update,
+//             density and energy floors
+// All index regions follow the reference's stale-depth bookkeeping; scratch
+// arrays are cell-shaped (a face or edge value is stored at the index of
+// its low cell).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vlct_physics.hpp"
+
+namespace enzoe {
+
+// Device descriptor of one block: permanent fields + the scratch the stage
+// kernels use for it.  FluxT is double (FP64 path) or float (mixed path).
+template <typename FluxT>
+struct BlockDesc {
+  double *U[S_N];      // density, velocity_x..z, total_energy, bfield_x..z
+  double *bfi[3];      // bfieldi_x..z (start of step / final output)
+  double *pressure;    // permanent pressure field (may be null)
+  double *Ut[S_N];     // half-step state (temp integration map)
+  double *bft[3];      // half-step face B
+  FluxT *F[3][S_N];    // fluxes along x, y, z; slot = field
+  double *E[3];        // edge EMFs along x, y, z
+  int mx, my, mz;
+  double cw[3];
+};
+
+struct StageParams {
+  int stage;           // 0 or 1
+  int is_final;        // writes into the permanent fields
+  int stale;           // stale depth at the start of the stage
+  int cur_stale;       // stale + immediate staling of the reconstructor
+  double dt;           // timestep of this stage (dt/2 for the VL half step)
+  double gamma;
+  double density_floor;
+  double pressure_floor;
+  double theta;
+};
+
+__device__ __forceinline__ long idx3(int k, int j, int i, int ny, int nx) {
+  return ((long)k * ny + j) * (long)nx + i;
+}
+
+// Decodes a linear thread index into (k,j,i) inside the box [lo, lo+n).
+__device__ __forceinline__ bool box_coords(long t, int lo, int nx, int ny,
+                                           int nz, int &k, int &j, int &i) {
+  long nxy = (long)nx * ny;
+  if (t >= nxy * nz) return false;
+  k = (int)(t / nxy);
+  long r = t - (long)k * nxy;
+  j = (int)(r / nx);
+  i = (int)(r - (long)j * nx);
+  k += lo; j += lo; i += lo;
+  return true;
+}
+
+// Cell primitives w = (rho, vx, vy, vz, p, bx, by, bz) from the integration
+// quantities (EnzoPhysicsFluidProps.cpp:64-138, EnzoComputePressure.cpp:
+// 177-189), in FP64.
+template <bool MHD>
+__device__ __forceinline__ void load_prim(double *const U[S_N], long c,
+                                          double gm1, double w[S_N]) {
+  double rho = U[S_RHO][c];
+  double vx = U[S_VX][c], vy = U[S_VY][c], vz = U[S_VZ][c];
+  double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+  double me = 0.;
+  w[S_RHO] = rho; w[S_VX] = vx; w[S_VY] = vy; w[S_VZ] = vz;
+  if (MHD) {
+    double bx = U[S_BX][c], by = U[S_BY][c], bz = U[S_BZ][c];
+    me = 0.5 * (bx * bx + by * by + bz * bz);
+    w[S_BX] = bx; w[S_BY] = by; w[S_BZ] = bz;
+  }
+  w[S_E] = gm1 * (rho * (U[S_E][c] - ke) - me);
+}
+
+// ---------------------------------------------------------------- k_flux
+
+// One thread per cell p of the box [cur_stale, m - cur_stale); for each
+// dim d it solves the face between p and p+e_d when that face lies in the
+// flux region of the stage (EnzoRiemannImpl.hpp:317-338 with
+// stale_depth = cur_stale).
+template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD>
+__global__ void __launch_bounds__(256)
+k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  const int mx = b.mx, my = b.my, mz = b.mz;
+  const int cs = sp.cur_stale;
+  int k, j, i;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!box_coords(t, cs, mx - 2 * cs, my - 2 * cs, mz - 2 * cs, k, j, i))
+    return;
+  double *const *U = (sp.stage == 0) ? b.U : b.Ut;
+  const double *bfi_rec[3] = {(sp.stage == 0) ? b.bfi[0] : b.bft[0],
+                              (sp.stage == 0) ? b.bfi[1] : b.bft[1],
+                              (sp.stage == 0) ? b.bfi[2] : b.bft[2]};
+  const double gm1 = sp.gamma - 1.0;
+  const T gamma = (T)sp.gamma;
+  const T theta = (T)sp.theta;
+  const T dfloor = (T)sp.density_floor, pfloor = (T)sp.pressure_floor;
+  const int p[3] = {i, j, k};
+  const int m[3] = {mx, my, mz};
+  const long c = idx3(k, j, i, my, mx);
+  const long str[3] = {1, (long)mx, (long)mx * my};
+  constexpr int NK = MHD ? 8 : 5;
+
+#pragma unroll 1
+  for (int d = 0; d < 3; d++) {
+    if (p[d] >= m[d] - 1 - cs) continue;  // face index range [cs, m_d-1-cs)
+    T wl[S_N], wr[S_N];
+    if (RECON == RC_NN) {
+      double a[S_N], bq[S_N];
+      load_prim<MHD>(U, c, gm1, a);
+      load_prim<MHD>(U, c + str[d], gm1, bq);
+#pragma unroll
+      for (int q = 0; q < S_N; q++) { wl[q] = (T)a[q]; wr[q] = (T)bq[q]; }
+    } else {
+      double w0[S_N], w1[S_N], w2[S_N], w3[S_N];  // cells p-1 .. p+2
+      load_prim<MHD>(U, c - str[d], gm1, w0);
+      load_prim<MHD>(U, c, gm1, w1);
+      load_prim<MHD>(U, c + str[d], gm1, w2);
+      load_prim<MHD>(U, c + 2 * str[d], gm1, w3);
+#pragma unroll
+      for (int q = 0; q < S_N; q++) {
+        if (!MHD && q >= S_BX) continue;
+        T a0 = (T)w0[q], a1 = (T)w1[q], a2 = (T)w2[q], a3 = (T)w3[q];
+        T dvl = (RECON == RC_PLM) ? limiter_enzo(a0, a1, a2, theta)
+                                  : limiter_athena(a0, a1, a2);
+        T dvr = (RECON == RC_PLM) ? limiter_enzo(a1, a2, a3, theta)
+                                  : limiter_athena(a1, a2, a3);
+        T lv = a1 + dvl * T(0.5);
+        T rv = a2 - dvr * T(0.5);
+        if (q == S_RHO) { lv = dmax(lv, dfloor); rv = dmax(rv, dfloor); }
+        if (q == S_E) { lv = dmax(lv, pfloor); rv = dmax(rv, pfloor); }
+        wl[q] = lv;
+        wr[q] = rv;
+      }
+    }
+    const int dj = (d + 1) % 3, dk = (d + 2) % 3;
+    T fo[8];
+    if (MHD) {
+      // correct_reconstructed_bfield (EnzoBfieldMethodCT.cpp:122-166):
+      // the face value of bfieldi (index p_d + 1 along d)
+      int bnx = mx + (d == 0), bny = my + (d == 1);
+      T bface = (T)bfi_rec[d][idx3(k + (d == 2), j + (d == 1), i + (d == 0),
+                                   bny, bnx)];
+      T l8[8], r8[8];
+      l8[L_D] = wl[S_RHO]; r8[L_D] = wr[S_RHO];
+      l8[L_BI] = bface; r8[L_BI] = bface;
+      l8[L_BJ] = wl[S_BX + dj]; r8[L_BJ] = wr[S_BX + dj];
+      l8[L_BK] = wl[S_BX + dk]; r8[L_BK] = wr[S_BX + dk];
+      l8[L_VI] = wl[S_VX + d]; r8[L_VI] = wr[S_VX + d];
+      l8[L_VJ] = wl[S_VX + dj]; r8[L_VJ] = wr[S_VX + dj];
+      l8[L_VK] = wl[S_VX + dk]; r8[L_VK] = wr[S_VX + dk];
+      l8[L_E] = wl[S_E]; r8[L_E] = wr[S_E];
+      if (RIEMANN == RS_HLLD) {
+        hlld(l8, r8, gamma, fo);
+      } else {
+        hlle_mhd(l8, r8, gamma, fo);
+      }
+      b.F[d][S_RHO][c] = (FluxT)fo[L_D];
+      b.F[d][S_VX + d][c] = (FluxT)fo[L_VI];
+      b.F[d][S_VX + dj][c] = (FluxT)fo[L_VJ];
+      b.F[d][S_VX + dk][c] = (FluxT)fo[L_VK];
+      b.F[d][S_E][c] = (FluxT)fo[L_E];
+      b.F[d][S_BX + dj][c] = (FluxT)fo[L_BJ];
+      b.F[d][S_BX + dk][c] = (FluxT)fo[L_BK];
+    } else {
+      T l5[5], r5[5];
+      l5[H_D] = wl[S_RHO]; r5[H_D] = wr[S_RHO];
+      l5[H_VI] = wl[S_VX + d]; r5[H_VI] = wr[S_VX + d];
+      l5[H_VJ] = wl[S_VX + dj]; r5[H_VJ] = wr[S_VX + dj];
+      l5[H_VK] = wl[S_VX + dk]; r5[H_VK] = wr[S_VX + dk];
+      l5[H_E] = wl[S_E]; r5[H_E] = wr[S_E];
+      hllc(l5, r5, gamma, fo);
+      b.F[d][S_RHO][c] = (FluxT)fo[H_D];
+      b.F[d][S_VX + d][c] = (FluxT)fo[H_VI];
+      b.F[d][S_VX + dj][c] = (FluxT)fo[H_VJ];
+      b.F[d][S_VX + dk][c] = (FluxT)fo[H_VK];
+      b.F[d][S_E][c] = (FluxT)fo[H_E];
+    }
+    (void)NK;
+  }
+}
+
+// ---------------------------------------------------------------- k_edge
+
+// identify_upwind weight from the density flux (EnzoBfieldMethodCT.cpp:
+// 170-216)
+__device__ __forceinline__ double upwind_weight(double f) {
+  return (f > 0) ? 1.0 : ((f < 0) ? 0.0 : 0.5);
+}
+
+// One thread per position p of the box [cs, m - cs); computes the edge EMF
+// along each axis i at (cell p_i, faces p_j + 1/2, p_k + 1/2) when the edge
+// lies in [cs+1, m_i-cs-1) x [cs, m_j-cs-1) x [cs, m_k-cs-1)
+// (EnzoBfieldMethodCT::compute_edge_efield / compute_edge_<true>,
+// EnzoBfieldMethodCT.cpp:384-557; centre E from compute_center_efield,
+// :267-292, on the current-stage state).
+template <typename FluxT>
+__global__ void __launch_bounds__(256)
+k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  const int mx = b.mx, my = b.my, mz = b.mz;
+  const int cs = sp.cur_stale;
+  int k, j, i;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!box_coords(t, cs, mx - 2 * cs, my - 2 * cs, mz - 2 * cs, k, j, i))
+    return;
+  double *const *U = (sp.stage == 0) ? b.U : b.Ut;
+  const int p[3] = {i, j, k};
+  const int m[3] = {mx, my, mz};
+  const long c = idx3(k, j, i, my, mx);
+  const long str[3] = {1, (long)mx, (long)mx * my};
+#pragma unroll 1
+  for (int ax = 0; ax < 3; ax++) {
+    const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
+    if (p[ax] < cs + 1 || p[ax] >= m[ax] - cs - 1) continue;
+    if (p[aj] >= m[aj] - cs - 1 || p[ak] >= m[ak] - cs - 1) continue;
+    const long sj = str[aj], sk = str[ak];
+    // centre E_i = -v_j B_k + v_k B_j at the four cells around the edge
+    const double *vj = U[S_VX + aj], *vk = U[S_VX + ak];
+    const double *bj = U[S_BX + aj], *bk = U[S_BX + ak];
+    auto ecen = [&](long cc) { return (-vj[cc] * bk[cc] + vk[cc] * bj[cc]); };
+    const double Ec = ecen(c), Ec_jp1 = ecen(c + sj), Ec_kp1 = ecen(c + sk),
+                 Ec_jkp1 = ecen(c + sj + sk);
+    // face E: j-faces carry -F_j(B_k), k-faces +F_k(B_j)
+    const FluxT *Fj = b.F[aj][S_BX + ak];
+    const FluxT *Fk = b.F[ak][S_BX + aj];
+    const double Ej = (double)Fj[c], Ej_kp1 = (double)Fj[c + sk];
+    const double Ek = (double)Fk[c], Ek_jp1 = (double)Fk[c + sj];
+    const double Wj = upwind_weight((double)b.F[aj][S_RHO][c]);
+    const double Wj_kp1 = upwind_weight((double)b.F[aj][S_RHO][c + sk]);
+    const double Wk = upwind_weight((double)b.F[ak][S_RHO][c]);
+    const double Wk_jp1 = upwind_weight((double)b.F[ak][S_RHO][c + sj]);
+
+    double dEdj_r = Wk_jp1 * (Ec_jp1 + Ej) + (1 - Wk_jp1) * (Ec_jkp1 + Ej_kp1);
+    double dEdj_l = Wk * (-Ej - Ec) + (1 - Wk) * (-Ej_kp1 - Ec_kp1);
+    double dEdk_r = Wj_kp1 * (Ec_kp1 - Ek) + (1 - Wj_kp1) * (Ec_jkp1 - Ek_jp1);
+    double dEdk_l = Wj * (Ek - Ec) + (1 - Wj) * (Ek_jp1 - Ec_jp1);
+    double Ej_sum = Ej + Ej_kp1;
+    Ej_sum *= -1;
+    double Ek_sum = Ek + Ek_jp1;
+    b.E[ax][c] =
+        0.25 * (Ej_sum + Ek_sum + (dEdj_l - dEdj_r) + (dEdk_l - dEdk_r));
+  }
+}
+
+// --------------------------------------------------------------- k_faces
+
+// One thread per position p of the box [cs, m - cs + 1) (faces reach one
+// index further); updates face X = p_i of bfieldi_i when
+// X in [cs+1, m_i-cs) and p_j, p_k in [cs+1, m-cs-1)
+// (EnzoBfieldMethodCT::update_bfield, EnzoBfieldMethodCT.cpp:617-687).  The
+// starting value is always the start-of-step face field.
+template <typename FluxT>
+__global__ void __launch_bounds__(256)
+k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  const int mx = b.mx, my = b.my, mz = b.mz;
+  const int cs = sp.cur_stale;
+  int k, j, i;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lo = cs + 1;
+  if (!box_coords(t, lo, mx - cs - lo, my - cs - lo, mz - cs - lo, k, j, i))
+    return;
+  const double dt = sp.dt;
+  const int p[3] = {i, j, k};
+  const int m[3] = {mx, my, mz};
+  const long str[3] = {1, (long)mx, (long)mx * my};
+#pragma unroll 1
+  for (int ax = 0; ax < 3; ax++) {
+    const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
+    if (p[aj] >= m[aj] - cs - 1 || p[ak] >= m[ak] - cs - 1) continue;
+    // p[ax] in [cs+1, m_ax - cs) by construction of the box
+    const double dtdj = dt / b.cw[aj];
+    const double dtdk = dt / b.cw[ak];
+    // edges at i-position X - 1/2: cell-shaped index of p - e_i
+    const long e = idx3(k, j, i, my, mx) - str[ax];
+    const double *Ej = b.E[aj], *Ek = b.E[ak];
+    double E_k_term = dtdj * (Ek[e] - Ek[e - str[aj]]);
+    double E_j_term = dtdk * (Ej[e] - Ej[e - str[ak]]);
+    const int bnx = mx + (ax == 0), bny = my + (ax == 1);
+    const long cb = idx3(k, j, i, bny, bnx);
+    double *out = sp.is_final ? b.bfi[ax] : b.bft[ax];
+    out[cb] = b.bfi[ax][cb] - E_k_term + E_j_term;
+  }
+}
+
+// --------------------------------------------------------------- k_cells
+
+// One thread per cell of [lo, m - lo) with lo = cur_stale for the final
+// stage (cell-centred B is rebuilt there, EnzoBfieldMethodCT.cpp:702-728)
+// and cur_stale + 1 otherwise.  Cells in [cs+1, m-cs-1) also get the flux
+// divergence (EnzoIntegrationQuanUpdate::accumulate_flux_component,
+// EnzoIntegrationQuanUpdate.cpp:105-147), the conserved update with the
+// density floor (update_quantities, :183-238) and the energy floor
+// (EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync, :162-290).
+template <typename FluxT, bool MHD>
+__global__ void __launch_bounds__(256)
+k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  const int mx = b.mx, my = b.my, mz = b.mz;
+  const int cs = sp.cur_stale;
+  const int lo = sp.is_final ? cs : cs + 1;
+  int k, j, i;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!box_coords(t, lo, mx - 2 * lo, my - 2 * lo, mz - 2 * lo, k, j, i))
+    return;
+  const long c = idx3(k, j, i, my, mx);
+  double *const *Uo = sp.is_final ? b.U : b.Ut;
+  double bc[3] = {0., 0., 0.};
+  if (MHD) {
+    double *const *bf = sp.is_final ? b.bfi : b.bft;
+    bc[0] = 0.5 * (bf[0][idx3(k, j, i, my, mx + 1)] +
+                   bf[0][idx3(k, j, i + 1, my, mx + 1)]);
+    bc[1] = 0.5 * (bf[1][idx3(k, j, i, my + 1, mx)] +
+                   bf[1][idx3(k, j + 1, i, my + 1, mx)]);
+    bc[2] = 0.5 * (bf[2][idx3(k, j, i, my, mx)] +
+                   bf[2][idx3(k + 1, j, i, my, mx)]);
+    Uo[S_BX][c] = bc[0];
+    Uo[S_BY][c] = bc[1];
+    Uo[S_BZ][c] = bc[2];
+  }
+  const int ul = cs + 1;
+  if (k < ul || j < ul || i < ul || k >= mz - ul || j >= my - ul ||
+      i >= mx - ul)
+    return;
+  const double dt = sp.dt;
+  const long str[3] = {1, (long)mx, (long)mx * my};
+  const int fq[5] = {S_RHO, S_VX, S_VY, S_VZ, S_E};
+  double dU[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) dU[q] = 0.;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double dtdx = dt / b.cw[d];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const FluxT *F = b.F[d][fq[q]];
+      double fr = (double)F[c], fl = (double)F[c - str[d]];
+      dU[q] -= dtdx * (fr - fl);
+    }
+  }
+  // the update base is always the start-of-step state
+  double old_rho = b.U[S_RHO][c];
+  double new_rho = b.U[S_RHO][c] + dU[0];
+  new_rho = dmax(new_rho, sp.density_floor);
+  double inv_new_rho = 1. / new_rho;
+  double v[3];
+#pragma unroll
+  for (int q = 1; q < 4; q++)
+    v[q - 1] = (b.U[fq[q]][c] * old_rho + dU[q]) * inv_new_rho;
+  double etot = (b.U[S_E][c] * old_rho + dU[4]) * inv_new_rho;
+  // energy floor
+  double inv_rho = 1. / new_rho;
+  double eint_floor = sp.pressure_floor * (1. / (sp.gamma - 1.)) * inv_rho;
+  double v2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  double non_thermal_e = 0.5 * v2;
+  if (MHD) {
+    double b2 = (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
+    non_thermal_e += (0.5 * b2 * inv_rho);
+  }
+  double etot_floor = eint_floor + non_thermal_e;
+  etot = dmax(etot, etot_floor);
+  Uo[S_RHO][c] = new_rho;
+  Uo[S_VX][c] = v[0];
+  Uo[S_VY][c] = v[1];
+  Uo[S_VZ][c] = v[2];
+  Uo[S_E][c] = etot;
+}
+
+// ----------------------------------------------------------------- k_dt
+
+// CFL timestep of every cell (ghosts included): writes the pressure field
+// and min-reduces dx_i/(|v_i|+c_f) into *result (as the bit pattern of a
+// non-negative double, so an integer atomicMin is an exact double min; a
+// NaN local value never wins, as std::min in the reference ignores it).
+// EnzoMHDIntegratorStageCommands::timestep
+// (EnzoMHDIntegratorStageCommands.cpp:299-366).
+template <typename FluxT, bool MHD>
+__global__ void __launch_bounds__(256)
+k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
+     unsigned long long *result) {
+  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  const int mx = b.mx, my = b.my, mz = b.mz;
+  const long n = (long)mx * my * mz;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double local = 1.7976931348623157e308;
+  if (c < n) {
+    double d = b.U[S_RHO][c];
+    double vx = b.U[S_VX][c], vy = b.U[S_VY][c], vz = b.U[S_VZ][c];
+    double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+    double me = 0.;
+    double bx = 0., by = 0., bz = 0.;
+    if (MHD) {
+      bx = b.U[S_BX][c]; by = b.U[S_BY][c]; bz = b.U[S_BZ][c];
+      me = 0.5 * (bx * bx + by * by + bz * bz);
+    }
+    double p = (gamma - 1.0) * (d * (b.U[S_E][c] - ke) - me);
+    if (b.pressure) b.pressure[c] = p;
+    double sig;
+    if (MHD) {
+      double B2 = ((bx * bx) + ((by * by) + (bz * bz)));
+      double cs2 = gamma * p / d;
+      double va2 = B2 / d;
+      sig = sqrt(va2 + cs2);
+    } else {
+      sig = sqrt(gamma * p / d);
+    }
+    double l = dmin3(b.cw[0] / (fabs(vx) + sig), b.cw[1] / (fabs(vy) + sig),
+                     b.cw[2] / (fabs(vz) + sig));
+    local = dmin(local, l);
+  }
+  // wave64 butterfly min, then one atomic per wavefront
+  unsigned long long bits = __double_as_longlong(local);
+  if (!(local >= 0.0)) bits = 0x7FF8000000000000ull;  // NaN never wins
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(bits, off, 64);
+    bits = (o < bits) ? o : bits;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMin(result, bits);
+}
+
+static __global__ void k_dt_init(unsigned long long *result) {
+  *result = 0x7FEFFFFFFFFFFFFFull;  // numeric_limits<double>::max()
+}
+
+// -------------------------------------------------------------- refresh
+
+// Ghost-zone copy for a uniform grid of equally sized blocks.  One thread
+// per element of one array (cell- or face-centred along `fc`); interior
+// elements return at once.
+struct RefreshGrid {
+  int nb[3];     // blocks per axis
+  int n[3];      // active cells per block
+  int g;         // ghost depth
+  int bc[3];     // 0 periodic, 1 outflow
+};
+
+__device__ __forceinline__ bool refresh_axis(const RefreshGrid &G, int a,
+                                             int blk, int idx, int centred,
+                                             int &sblk, int &sidx) {
+  const int n = G.n[a], nb = G.nb[a], g = G.g;
+  const int N = n * nb;
+  if (!centred) {
+    if (idx >= g && idx < g + n) { sblk = blk; sidx = idx; return false; }
+    int gi = blk * n + idx - g;
+    if (G.bc[a] == 0) {
+      gi = ((gi % N) + N) % N;
+    } else {
+      gi = (gi < 0) ? 0 : ((gi >= N) ? N - 1 : gi);
+    }
+    sblk = gi / n;
+    sidx = gi % n + g;
+    return true;
+  }
+  if (idx >= g && idx <= g + n) { sblk = blk; sidx = idx; return false; }
+  int F = blk * n + idx - g;
+  if (G.bc[a] == 0) {
+    F = ((F % N) + N) % N;
+    sblk = F / n;
+    sidx = F % n + g;
+  } else if (F < 0) {
+    sblk = 0; sidx = g;
+  } else {
+    sblk = nb - 1; sidx = g + n;
+  }
+  return true;
+}
+
+// arrays: table[b] = device pointer of this field for block b
+static __global__ void __launch_bounds__(256)
+k_refresh(double *const *__restrict__ table, RefreshGrid G, int fc) {
+  const int bidx = blockIdx.y;
+  const int bx = bidx % G.nb[0];
+  const int by = (bidx / G.nb[0]) % G.nb[1];
+  const int bz = bidx / (G.nb[0] * G.nb[1]);
+  const int ax = G.n[0] + 2 * G.g + (fc == 0);
+  const int ay = G.n[1] + 2 * G.g + (fc == 1);
+  const int az = G.n[2] + 2 * G.g + (fc == 2);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nxy = (long)ax * ay;
+  if (t >= nxy * az) return;
+  const int k = (int)(t / nxy);
+  const long r = t - (long)k * nxy;
+  const int j = (int)(r / ax);
+  const int i = (int)(r - (long)j * ax);
+  int sb[3], si[3];
+  bool ghost = refresh_axis(G, 0, bx, i, fc == 0, sb[0], si[0]);
+  ghost |= refresh_axis(G, 1, by, j, fc == 1, sb[1], si[1]);
+  ghost |= refresh_axis(G, 2, bz, k, fc == 2, sb[2], si[2]);
+  if (!ghost) return;
+  const int sbid = (sb[2] * G.nb[1] + sb[1]) * G.nb[0] + sb[0];
+  table[bidx][t] = table[sbid][idx3(si[2], si[1], si[0], ay, ax)];
+}
+
+}  // namespace enzoe

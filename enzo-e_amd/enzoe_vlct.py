@@ -1,0 +1,367 @@
+"""Python binding of the gfx950 VL+CT library (``libenzoe_vlct.so``).
+
+Thin ctypes layer over the C ABI declared in ``include/enzoe_vlct.h`` plus
+the few HIP runtime calls needed to own device memory.  The product path is
+always the HIP library: importing this module on a machine without the built
+library or without a HIP device raises instead of falling back to the CPU.
+
+Blocks are stored the way Cello stores them (src/Cello/data_FieldData.cpp:
+319-395): one device allocation per block holding every field at its own
+offset, each field a dense (z, y, x) array with ghost zones.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libenzoe_vlct.so")
+
+ENZOE_PREC_DOUBLE = 0
+ENZOE_PREC_MIXED = 1
+BC = {"periodic": 0, "outflow": 1}
+
+CC_FIELDS = ["density", "velocity_x", "velocity_y", "velocity_z",
+             "total_energy", "bfield_x", "bfield_y", "bfield_z"]
+FACE_FIELDS = ["bfieldi_x", "bfieldi_y", "bfieldi_z"]
+
+
+class EnzoeError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("mhd_choice", ctypes.c_char_p),
+                ("riemann_solver", ctypes.c_char_p),
+                ("reconstruct_method", ctypes.c_char_p),
+                ("time_scheme", ctypes.c_char_p),
+                ("theta_limiter", ctypes.c_double),
+                ("courant", ctypes.c_double),
+                ("gamma", ctypes.c_double),
+                ("density_floor", ctypes.c_double),
+                ("pressure_floor", ctypes.c_double),
+                ("precision", ctypes.c_int)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class Block(ctypes.Structure):
+    _fields_ = [("mx", ctypes.c_int), ("my", ctypes.c_int), ("mz", ctypes.c_int),
+                ("ghost", ctypes.c_int),
+                ("cell_width", ctypes.c_double * 3),
+                ("density", ctypes.c_void_p),
+                ("velocity", ctypes.c_void_p * 3),
+                ("total_energy", ctypes.c_void_p),
+                ("bfield", ctypes.c_void_p * 3),
+                ("bfieldi", ctypes.c_void_p * 3),
+                ("pressure", ctypes.c_void_p)]
+
+
+_lib = None
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise EnzoeError("HIP runtime (libamdhip64.so) not found")
+        _hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        _hip.hipFree.argtypes = [ctypes.c_void_p]
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+        _hip.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        _hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        _hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        _hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float),
+                                             ctypes.c_void_p, ctypes.c_void_p]
+        _hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        _hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        _hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    return _hip
+
+
+def hip_check(rc, what):
+    if rc != 0:
+        raise EnzoeError("%s failed with hipError %d" % (what, rc))
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    rc = hip().hipGetDeviceCount(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def lib():
+    """The HIP library.  Raises if it was not built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EnzoeError("libenzoe_vlct.so is not built (run "
+                             "__graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.enzoe_version.restype = ctypes.c_char_p
+        L.enzoe_vlct_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.enzoe_vlct_destroy.argtypes = [vp]
+        L.enzoe_vlct_courant.argtypes = [vp]
+        L.enzoe_vlct_courant.restype = ctypes.c_double
+        L.enzoe_vlct_timestep.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                          vp, _dp]
+        L.enzoe_vlct_compute.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                         ctypes.c_double, vp]
+        L.enzoe_grid_refresh.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int), vp]
+        L.enzoe_set_profiling.argtypes = [ctypes.c_int]
+        L.enzoe_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        P8 = ctypes.c_void_p * 8
+        L.enzoe_reconstruct_interface.argtypes = [
+            ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            P8, P8, P8, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, vp]
+        L.enzoe_reconstructor_staling.argtypes = [
+            ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.enzoe_riemann_solve.argtypes = [
+            ctypes.c_char_p, ctypes.c_int, ctypes.c_double, P8, P8, P8,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.enzoe_ct_update_bfield.argtypes = [
+            _dp, ctypes.c_int, ctypes.c_void_p * 3, vp, vp, ctypes.c_double,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.enzoe_ct_compute_center_bfield.argtypes = [
+            ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        raise EnzoeError("%s returned status %d" % (what, rc))
+
+
+# ------------------------------------------------------------ device memory
+
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        hip_check(hip().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 8)),
+                  "hipMalloc(%d)" % self.nbytes)
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            hip().hipFree(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def to_device(dst_ptr, arr):
+    a = np.ascontiguousarray(arr, dtype=np.float64)
+    hip_check(hip().hipMemcpy(ctypes.c_void_p(dst_ptr), a.ctypes.data_as(ctypes.c_void_p),
+                              a.nbytes, 1), "hipMemcpy H2D")
+
+
+def from_device(src_ptr, shape):
+    a = np.empty(shape, dtype=np.float64)
+    hip_check(hip().hipMemcpy(a.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(src_ptr),
+                              a.nbytes, 2), "hipMemcpy D2H")
+    return a
+
+
+def field_shapes(mx, my, mz, mhd):
+    shapes = {}
+    names = CC_FIELDS if mhd else CC_FIELDS[:5]
+    for f in names:
+        shapes[f] = (mz, my, mx)
+    shapes["pressure"] = (mz, my, mx)
+    if mhd:
+        shapes["bfieldi_x"] = (mz, my, mx + 1)
+        shapes["bfieldi_y"] = (mz, my + 1, mx)
+        shapes["bfieldi_z"] = (mz + 1, my, mx)
+    return shapes
+
+
+class DeviceBlock:
+    """One block's permanent fields in one device allocation."""
+
+    def __init__(self, mx, my, mz, ghost, cell_width, mhd=True):
+        self.mx, self.my, self.mz, self.ghost = mx, my, mz, ghost
+        self.mhd = mhd
+        self.shapes = field_shapes(mx, my, mz, mhd)
+        self.offsets = {}
+        off = 0
+        for name, shp in self.shapes.items():
+            self.offsets[name] = off
+            n = int(np.prod(shp)) * 8
+            off += (n + 255) // 256 * 256  # 256-byte aligned fields
+        self.buf = DeviceBuffer(off)
+        hip_check(hip().hipMemset(self.buf.ptr, 0, off), "hipMemset")
+        base = self.buf.ptr.value
+        b = Block()
+        b.mx, b.my, b.mz, b.ghost = mx, my, mz, ghost
+        for a in range(3):
+            b.cell_width[a] = cell_width[a]
+        b.density = base + self.offsets["density"]
+        for a, c in enumerate("xyz"):
+            b.velocity[a] = base + self.offsets["velocity_" + c]
+            if mhd:
+                b.bfield[a] = base + self.offsets["bfield_" + c]
+                b.bfieldi[a] = base + self.offsets["bfieldi_" + c]
+        b.total_energy = base + self.offsets["total_energy"]
+        b.pressure = base + self.offsets["pressure"]
+        self.desc = b
+
+    def ptr(self, name):
+        return self.buf.ptr.value + self.offsets[name]
+
+    def upload(self, arrays):
+        for name in self.shapes:
+            if name in arrays:
+                assert arrays[name].shape == self.shapes[name], name
+                to_device(self.ptr(name), arrays[name])
+
+    def download(self, names=None):
+        names = names or list(self.shapes)
+        return {n: from_device(self.ptr(n), self.shapes[n]) for n in names}
+
+
+# ------------------------------------------------------------------- method
+
+class VlctMethod:
+    """EnzoMethodMHDVlct on the GPU (through the C ABI)."""
+
+    def __init__(self, mhd=True, riemann="hlld", reconstruct="plm",
+                 time_scheme="vl", theta=1.5, courant=None, gamma=5.0 / 3.0,
+                 density_floor=1e-200, pressure_floor=1e-200,
+                 precision=ENZOE_PREC_DOUBLE):
+        p = Params()
+        p.mhd_choice = b"constrained_transport" if mhd else b"no_bfield"
+        p.riemann_solver = riemann.encode() if riemann else None
+        p.reconstruct_method = reconstruct.encode() if reconstruct else None
+        p.time_scheme = time_scheme.encode() if time_scheme else None
+        p.theta_limiter = theta
+        p.courant = -1.0 if courant is None else courant
+        p.gamma = gamma
+        p.density_floor = density_floor
+        p.pressure_floor = pressure_floor
+        p.precision = precision
+        self.mhd = mhd
+        self.handle = ctypes.c_void_p()
+        check(lib().enzoe_vlct_create(ctypes.byref(p), ctypes.byref(self.handle)),
+              "enzoe_vlct_create")
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and self.handle.value and _lib:
+            _lib.enzoe_vlct_destroy(self.handle)
+            self.handle = None
+
+    @property
+    def courant(self):
+        return lib().enzoe_vlct_courant(self.handle)
+
+    @staticmethod
+    def _blocks(blocks):
+        arr = (Block * len(blocks))(*[b.desc for b in blocks])
+        return arr
+
+    def timestep(self, blocks, stream=None):
+        dt = ctypes.c_double(0)
+        check(lib().enzoe_vlct_timestep(self.handle, self._blocks(blocks), len(blocks),
+                                        stream, ctypes.byref(dt)), "enzoe_vlct_timestep")
+        return dt.value
+
+    def compute(self, blocks, dt, stream=None):
+        check(lib().enzoe_vlct_compute(self.handle, self._blocks(blocks), len(blocks),
+                                       dt, stream), "enzoe_vlct_compute")
+
+    def refresh(self, blocks, nblocks, bc, stream=None):
+        bcs = (ctypes.c_int * 3)(*[BC[b] if isinstance(b, str) else b for b in bc])
+        check(lib().enzoe_grid_refresh(self.handle, self._blocks(blocks), nblocks[0],
+                                       nblocks[1], nblocks[2], bcs, stream),
+              "enzoe_grid_refresh")
+
+
+def synchronize():
+    hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def profiling(on):
+    lib().enzoe_set_profiling(1 if on else 0)
+
+
+def profile_report():
+    cap, nl = 64, 64
+    names = ctypes.create_string_buffer(cap * nl)
+    ms = (ctypes.c_double * cap)()
+    cnt = (ctypes.c_int * cap)()
+    n = lib().enzoe_profile_report(names, nl, ms, cnt, cap)
+    out = {}
+    for i in range(n):
+        nm = names.raw[i * nl:(i + 1) * nl].split(b"\0", 1)[0].decode()
+        out[nm] = (ms[i], cnt[i])
+    return out
+
+
+class DeviceGrid:
+    """A uniform grid of equally sized blocks resident on the device."""
+
+    def __init__(self, method, nblocks, ncells, cell_width, ghost=3,
+                 bc=("periodic",) * 3):
+        self.method = method
+        self.nblocks = tuple(nblocks)
+        self.ncells = tuple(ncells)
+        self.ghost = ghost
+        self.bc = tuple(bc)
+        m = [ncells[a] + 2 * ghost for a in range(3)]
+        nb = nblocks[0] * nblocks[1] * nblocks[2]
+        self.blocks = [DeviceBlock(m[0], m[1], m[2], ghost, cell_width[i] if
+                                   isinstance(cell_width[0], (list, tuple)) else cell_width,
+                                   mhd=method.mhd) for i in range(nb)]
+
+    def upload_from(self, arrays_per_block):
+        for blk, arrs in zip(self.blocks, arrays_per_block):
+            blk.upload(arrs)
+
+    def download(self, names=None):
+        return [b.download(names) for b in self.blocks]
+
+    def refresh(self):
+        self.method.refresh(self.blocks, self.nblocks, self.bc)
+
+    def cycle(self, t, t_stop):
+        """One Cello cycle: dt on current data, refresh, compute."""
+        dt = self.method.timestep(self.blocks)
+        if t < t_stop < t + dt:
+            dt = t_stop - t
+        rem = t_stop - t
+        dt = dt if dt < rem else rem
+        self.refresh()
+        self.method.compute(self.blocks, dt)
+        return dt
+
+    def evolve(self, t0, t_stop, max_cycles=1 << 30):
+        t, n, log = t0, 0, []
+        while t < t_stop and n < max_cycles:
+            dt = self.cycle(t, t_stop)
+            log.append(dt)
+            t = t + dt
+            n += 1
+        synchronize()
+        return n, log

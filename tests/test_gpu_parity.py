@@ -1,0 +1,217 @@
+"""Parity of the gfx950 library (through the C ABI) with the CPU oracle.
+
+FP64 bar: <= 1e-12 relative per cell against the oracle (which restates the
+reference operation by operation), on every cell of every field including
+the ghost zones the method writes; the end-to-end runs must also reproduce
+the reference's golden L1 values (rtol 1e-13, atol 7e-14,
+input/vlct/testing_utils.py:275-292).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from _states import random_periodic_grid
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _ev():
+    import enzoe_vlct as ev
+    return ev
+
+
+def method_for(cfg_o, precision=0, courant=None):
+    ev = _ev()
+    rec = {0: "nn", 1: "plm", 2: "plm_athena"}[cfg_o.recon[1]]
+    rs = {0: "hlle", 1: "hllc", 2: "hlld"}[cfg_o.riemann]
+    return ev.VlctMethod(mhd=bool(cfg_o.mhd), riemann=rs, reconstruct=rec,
+                         theta=cfg_o.theta_limiter,
+                         courant=cfg_o.courant if courant is None else courant,
+                         gamma=cfg_o.gamma, density_floor=cfg_o.density_floor,
+                         pressure_floor=cfg_o.pressure_floor,
+                         precision=precision)
+
+
+def device_copy(grid_o, method):
+    ev = _ev()
+    nb = grid_o.nblocks[0] * grid_o.nblocks[1] * grid_o.nblocks[2]
+    cws = [tuple(grid_o.block(i).cw) for i in range(nb)]
+    bcs = tuple("periodic" if grid_o.g.bc[a] == 0 else "outflow" for a in range(3))
+    dg = ev.DeviceGrid(method, grid_o.nblocks, grid_o.ncells, cws, bc=bcs)
+    dg.upload_from([grid_o.block_arrays(i) for i in range(nb)])
+    return dg
+
+
+def max_rel(a, b):
+    scale = np.maximum(np.abs(b), 1e-300)
+    d = np.abs(a - b)
+    return float(np.max(d / np.maximum(scale, np.abs(b).max() * 1e-300)))
+
+
+def compare_grids(dg, grid_o, rel=REL, fields=None):
+    out = {}
+    nb = len(dg.blocks)
+    for i in range(nb):
+        got = dg.blocks[i].download()
+        ref = grid_o.block_arrays(i)
+        for name in (fields or got):
+            if name not in ref:
+                continue
+            a, b = got[name], ref[name]
+            # relative to the field's magnitude scale (per cell)
+            denom = np.maximum(np.abs(b), 1e-12 * max(1.0, float(np.abs(b).max())))
+            r = float(np.max(np.abs(a - b) / denom))
+            out[name] = max(out.get(name, 0.0), r)
+    bad = {k: v for k, v in out.items() if not v <= rel}
+    assert not bad, "relative differences above %g: %s" % (rel, bad)
+    return out
+
+
+CASES = [
+    dict(mhd=True, riemann="hlld", recon=("nn", "plm"), theta=2.0),
+    dict(mhd=True, riemann="hlld", recon=("nn", "plm"), theta=1.5),
+    dict(mhd=True, riemann="hlld", recon=("nn", "plm_athena"), theta=1.5),
+    dict(mhd=True, riemann="hlle", recon=("nn", "plm"), theta=2.0),
+    dict(mhd=True, riemann="hlld", recon=("nn", "nn"), theta=1.5),
+    dict(mhd=False, riemann="hllc", recon=("nn", "plm"), theta=2.0),
+    dict(mhd=False, riemann="hllc", recon=("nn", "plm_athena"), theta=2.0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s-%g" % (
+    "mhd" if c["mhd"] else "hd", c["riemann"], c["recon"][1], c["theta"]))
+def test_cycles_match_oracle(case):
+    print("case", case, flush=True)
+    grid = random_periodic_grid(mhd=case["mhd"], seed=11, n=(14, 12, 10),
+                                riemann=case["riemann"], recon=case["recon"],
+                                theta=case["theta"], vamp=0.6, bamp=0.08)
+    method = method_for(grid.cfg)
+    dg = device_copy(grid, method)
+    t = 0.0
+    for _ in range(3):
+        dt_o = grid.cycle(t, 10.0)
+        dt_g = dg.cycle(t, 10.0)
+        assert dt_g == pytest.approx(dt_o, rel=1e-14, abs=0)
+        t += dt_o
+    _ev().synchronize()
+    diffs = compare_grids(dg, grid)
+    print("max rel diff per field:", diffs)
+
+
+def test_timestep_writes_pressure_and_matches_oracle():
+    grid = random_periodic_grid(mhd=True, seed=3, n=(10, 9, 8))
+    method = method_for(grid.cfg)
+    dg = device_copy(grid, method)
+    dt_g = method.timestep(dg.blocks)
+    dt_o = po.lib().o_vlct_timestep(ctypes.byref(grid.cfg), ctypes.byref(grid.block(0)))
+    assert dt_g == dt_o
+    p = dg.blocks[0].download(["pressure"])["pressure"]
+    np.testing.assert_allclose(p, grid.block_arrays(0)["pressure"], rtol=REL, atol=0)
+
+
+def _wave(wave, res, t_stop, mhd=True, nblocks=(1, 1, 1)):
+    grid = po.linear_wave_grid(wave, res, mhd=mhd, nblocks=nblocks)
+    fields = po.MHD_L1_FIELDS if mhd else po.HD_L1_FIELDS
+    init = [grid.gather(f) for f in fields]
+    method = method_for(grid.cfg)
+    dg = device_copy(grid, method)
+    n_g, log_g = dg.evolve(0.0, t_stop)
+    n_o, log_o = grid.evolve(0.0, t_stop)
+    assert n_g == n_o
+    np.testing.assert_allclose(log_g, log_o, rtol=1e-13)
+    # copy the device state back into the oracle grid to use its gather
+    nb = nblocks[0] * nblocks[1] * nblocks[2]
+    ref_final = [grid.gather(f) for f in fields]
+    diffs = compare_grids(dg, grid)
+    for i in range(nb):
+        arrs = dg.blocks[i].download()
+        view = grid.block_arrays(i)
+        for k, v in arrs.items():
+            if k in view:
+                view[k][...] = v
+    final = [grid.gather(f) for f in fields]
+    for a, b in zip(final, ref_final):
+        np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-14)
+    return po.l1_norm(init, final, res=res), diffs
+
+
+@pytest.mark.parametrize("wave,res,t_stop,ref", [
+    ("fast", 16, 0.5, 1.6388526155394664e-07),
+    ("alfven", 16, 1.0, 1.927245356389947e-07),
+    ("mhd_entropy", 16, 1.0, 1.0021263485338544e-07),
+])
+def test_mhd_linear_wave_golden_on_gpu(wave, res, t_stop, ref):
+    print("wave", wave, res, flush=True)
+    l1, diffs = _wave(wave, res, t_stop)
+    print("L1", repr(l1), "ref", ref, "per-field max rel", diffs)
+    assert np.isclose(l1, ref, rtol=1e-13, atol=7e-14)
+
+
+def test_hd_sound_wave_multiblock_golden_on_gpu():
+    l1, diffs = _wave("sound", 16, 1.0, mhd=False, nblocks=(2, 2, 2))
+    print("L1", repr(l1), diffs)
+    assert np.isclose(l1, 1.3704437791196907e-07, rtol=1e-13, atol=7e-14)
+
+
+def test_rj2a_shock_tube_golden_on_gpu():
+    table = po.load_reference_table(
+        os.path.join(GOLDEN, "rj2a_shock_tube_t0.2_res256.csv"))
+    grid = po.rj2a_grid(0)
+    method = method_for(grid.cfg)
+    dg = device_copy(grid, method)
+    dg.evolve(0.0, 0.2)
+    arrs = dg.blocks[0].download()
+    view = grid.block_arrays(0)
+    for k, v in arrs.items():
+        if k in view:
+            view[k][...] = v
+    l1 = po.table_l1(grid, table, 0)
+    print("rj2a L1", repr(l1))
+    assert np.isclose(l1, 0.012523489882320429, rtol=1e-13, atol=7e-14)
+
+
+def test_div_b_stays_at_roundoff_on_gpu():
+    grid = random_periodic_grid(mhd=True, seed=5, n=(16, 16, 16), ext=(1, 1, 1),
+                                vamp=0.8, bamp=0.1)
+    method = method_for(grid.cfg)
+    dg = device_copy(grid, method)
+    t = 0.0
+    for _ in range(6):
+        t += dg.cycle(t, 10.0)
+    a = dg.blocks[0].download()
+    dx = 1.0 / 16
+    div = ((a["bfieldi_x"][:, :, 1:] - a["bfieldi_x"][:, :, :-1]) / dx +
+           (a["bfieldi_y"][:, 1:, :] - a["bfieldi_y"][:, :-1, :]) / dx +
+           (a["bfieldi_z"][1:, :, :] - a["bfieldi_z"][:-1, :, :]) / dx)[3:-3, 3:-3, 3:-3]
+    print("max |div B| * dx =", np.abs(div).max() * dx)
+    assert np.abs(div).max() * dx < 1e-13
+
+
+def test_mixed_precision_close_to_fp64():
+    """FP32 reconstruction + Riemann, FP64 accumulation/CT/dt: the L1 error
+    against the FP64 result on a smooth problem stays below 1e-5 relative."""
+    grid = random_periodic_grid(mhd=True, seed=9, n=(16, 16, 16), ext=(1, 1, 1),
+                                vamp=0.3, bamp=0.05)
+    m64 = method_for(grid.cfg)
+    m32 = method_for(grid.cfg, precision=1)
+    d64 = device_copy(grid, m64)
+    d32 = device_copy(grid, m32)
+    t = 0.0
+    for _ in range(5):
+        dt = m64.timestep(d64.blocks)
+        d64.refresh(); d32.refresh()
+        m64.compute(d64.blocks, dt)
+        m32.compute(d32.blocks, dt)
+        t += dt
+    a, b = d64.blocks[0].download(), d32.blocks[0].download()
+    for f in ("density", "velocity_x", "total_energy", "bfield_y"):
+        x, y = a[f][3:-3, 3:-3, 3:-3], b[f][3:-3, 3:-3, 3:-3]
+        rel = np.abs(x - y).sum() / np.abs(x).sum()
+        print(f, "L1 rel", rel)
+        assert rel < 1e-5
