@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 VL+CT step: zone-updates/s on one MI355X.
+
+Workload (default): plain FP64 VL+CT on the 3D Orszag-Tang vortex in one
+periodic 256^3 block with 3 ghost zones (gamma = 5/3, PLM with theta = 2,
+HLLD, CT, CFL 0.4).  A step is one Cello cycle of the method on the
+block: the CFL reduction (timestep), the periodic ghost refresh and the
+two-stage VL+CT compute.  The state lives in HBM for the whole run.
+
+Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "enzo-e_amd"))
+
+import numpy as np  # noqa: E402
+
+import enzoe_vlct as ev  # noqa: E402
+import problems  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per zone-update of a fused FP64 MHD VL+CT step:
+# stage 1 reads and writes the 11-word state (8 cell + 3 face words), stage 2
+# reads the half-step state and the start-of-step state and writes the new
+# state: 55 words = 440 B.
+STEP_BYTES_PER_ZONE_FP64 = 440.0
+# Algorithmic bytes per cell of one flux launch: it must read the 11-word
+# state of the cell and write 7 flux components on each of its 3 low faces.
+FLUX_BYTES_PER_CELL = {0: (11 + 21) * 8.0, 1: 11 * 8.0 + 21 * 4.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="active cells per axis")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=128,
+                    help="edge of the Orszag-Tang block timed on the CPU")
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    return ap.parse_args()
+
+
+def make_state(n, method):
+    blk, cw = problems.orszag_tang(n)
+    grid = ev.DeviceGrid(method, (1, 1, 1), (n, n, n), [cw])
+    grid.upload_from([blk])
+    grid.refresh()
+    ev.synchronize()
+    return grid
+
+
+def step(grid):
+    m = grid.method
+    dt = m.timestep(grid.blocks)
+    grid.refresh()
+    m.compute(grid.blocks, dt)
+    return dt
+
+
+def cpu_baseline(n, nsteps):
+    """The oracle (plain-C restatement of the reference loops, OpenMP over
+    the z loops) on an n^3 Orszag-Tang block: zone-updates/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    cfg = po.make_config(riemann="hlld", recon=("nn", "plm"), theta=2.0,
+                         mhd=True, courant=0.4)
+    blk, cw = problems.orszag_tang(n)
+    grid = po.Grid(cfg, (1, 1, 1), (n, n, n), (0, 0, 0), (1.0, 1.0, 1.0))
+    view = grid.block_arrays(0)
+    for k, v in blk.items():
+        view[k][...] = v
+    grid.refresh()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    grid.cycle(0.0, 1e9, nthreads=threads)  # warm the scratch
+    t0 = time.perf_counter()
+    t = 0.0
+    for _ in range(nsteps):
+        t += grid.cycle(t, 1e9, nthreads=threads)
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * nsteps / el, "unit": "zone-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": "oracle (C restatement of the reference loops, OpenMP "
+                      "over z) on the same Orszag-Tang problem at %d^3, %d "
+                      "steps, %.1f s" % (n, nsteps, el)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU "
+              "path); got --gpus %d" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    if ev.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    prec = ev.ENZOE_PREC_DOUBLE if a.precision == "fp64" else ev.ENZOE_PREC_MIXED
+    method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
+                           theta=2.0, courant=0.4, gamma=5.0 / 3.0,
+                           precision=prec)
+    grid = make_state(a.n, method)
+    for _ in range(a.warmup):
+        step(grid)
+    ev.synchronize()
+
+    ev.profiling(True)
+    ev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step(grid)
+    ev.synchronize()
+    el = time.perf_counter() - t0
+    prof = ev.profile_report()
+    ev.profiling(False)
+
+    zones = a.n ** 3
+    value = zones * a.steps / el
+    m = a.n + 6
+    # dominant kernel: the flux kernel of the full (PLM) stage
+    dom = "k_flux_plm"
+    dom_ms = prof.get(dom, (float("nan"), 0))[0]
+    cells = (m - 4) ** 3  # flux box of the PLM stage: [2, m-2)^3
+    dom_bytes = FLUX_BYTES_PER_CELL[prec] * cells
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    gpu_step_ms = sum(v[0] * v[1] for v in prof.values()) / a.steps
+    step_gbs = STEP_BYTES_PER_ZONE_FP64 * zones / (gpu_step_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "zone-updates/s/GPU (VL+CT MHD step)",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else "f32-riemann/f64-update",
+        "data": "synthetic (3D Orszag-Tang vortex, seeded 1e-3 v_z noise)",
+        "config": {"workload": "orszag_tang_%d3_fp64_vlct_plm_hlld" % a.n
+                   if prec == 0 else "orszag_tang_%d3_mixed_vlct_plm_hlld" % a.n,
+                   "block": [a.n, a.n, a.n], "ghost": 3, "riemann": "hlld",
+                   "reconstruct": "plm", "theta": 2.0, "courant": 0.4,
+                   "precision": a.precision},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": dom_bytes,
+            "avg_launch_ms": dom_ms,
+        },
+        "step_roofline": {
+            "bytes_per_zone": STEP_BYTES_PER_ZONE_FP64,
+            "gpu_ms_per_step": gpu_step_ms,
+            "achieved_GBs": step_gbs,
+            "frac": step_gbs / HBM_PEAK_GBS,
+        },
+        "kernels_ms": {k: round(v[0], 4) for k, v in sorted(prof.items())},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

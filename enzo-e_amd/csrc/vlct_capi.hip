@@ -35,32 +35,65 @@ struct ProfEntry {
 };
 bool g_profiling = false;
 std::map<std::string, ProfEntry> g_prof;
+// launches recorded but not yet resolved: events are read only when a
+// report is asked for, so profiling adds no host synchronisation
+struct PendingTiming {
+  const char *name;
+  hipEvent_t a, b;
+};
+std::vector<PendingTiming> g_pending;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t pool_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void resolve_pending() {
+  for (PendingTiming &p : g_pending) {
+    (void)hipEventSynchronize(p.b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    ProfEntry &e = g_prof[p.name];
+    e.total_ms += ms;
+    e.count += 1;
+    g_event_pool.push_back(p.a);
+    g_event_pool.push_back(p.b);
+  }
+  g_pending.clear();
+}
 
 struct KernelTimer {
   hipStream_t s;
   const char *name;
-  hipEvent_t a{}, b{};
+  hipEvent_t a = nullptr, b = nullptr;
   KernelTimer(hipStream_t s_, const char *n) : s(s_), name(n) {
     if (g_profiling) {
-      (void)hipEventCreate(&a);
-      (void)hipEventCreate(&b);
+      a = pool_event();
+      b = pool_event();
       (void)hipEventRecord(a, s);
     }
   }
   ~KernelTimer() {
     if (g_profiling) {
       (void)hipEventRecord(b, s);
-      (void)hipEventSynchronize(b);
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, a, b);
-      ProfEntry &e = g_prof[name];
-      e.total_ms += ms;
-      e.count += 1;
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
+      g_pending.push_back({name, a, b});
+      if (g_pending.size() > 4096) resolve_pending();
     }
   }
 };
+
+__global__ void k_calibration_copy(double *__restrict__ dst,
+                                   const double *__restrict__ src, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
 
 inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -457,7 +490,10 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     maxcells = std::max<long>(maxcells, (long)blocks[i].mx * blocks[i].my * blocks[i].mz);
   {
     KernelTimer kt(s, "k_dt");
-    dim3 grid(nblk(maxcells, 256), n);
+    // enough workgroups to fill the device; each thread strides over cells
+    const unsigned wg = (unsigned)std::max<long>(1, std::min<long>(nblk(maxcells, 256),
+                                                                   std::max<long>(1, 2048 / n)));
+    dim3 grid(wg, n);
     if (m->mhd)
       hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s,
                          (const BlockDesc<double> *)m->d_ptrtab, m->gamma, m->d_dt);
@@ -522,25 +558,42 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
     last_tab = tab;
   }
   KernelTimer kt(s, "k_refresh");
-  for (int q = 0; q < nf; q++) {
-    const int fc = (q >= 8) ? q - 8 : -1;
-    const long ncells = (long)(blocks[0].mx + (fc == 0)) *
-                        (blocks[0].my + (fc == 1)) * (blocks[0].mz + (fc == 2));
-    hipLaunchKernelGGL(k_refresh, dim3(nblk(ncells, 256), nb), dim3(256), 0, s,
-                       ((double *const *)d_tab) + (size_t)q * nb, G, fc);
+  // largest ghost-element count over the field kinds (a face field has one
+  // more interior layer and one more layer of extent along its axis)
+  long nghost = 0;
+  for (int fc = -1; fc < 3; fc++) {
+    long tot = 1, in = 1;
+    for (int a = 0; a < 3; a++) {
+      tot *= (G.n[a] + 2 * G.g + (fc == a));
+      in *= (G.n[a] + (fc == a));
+    }
+    nghost = std::max(nghost, tot - in);
   }
+  hipLaunchKernelGGL(k_refresh, dim3(nblk(nghost, 256), nb, nf), dim3(256), 0, s,
+                     (double *const *)d_tab, G);
   if (hipGetLastError() != hipSuccess) return ENZOE_ERR_HIP;
   return ENZOE_OK;
 }
 
 int enzoe_set_profiling(int on) {
+  resolve_pending();
   g_profiling = on != 0;
   if (on) g_prof.clear();
   return ENZOE_OK;
 }
 
+int enzoe_calibration_copy(double *dst, const double *src, size_t n,
+                           enzoe_stream_t stream) {
+  if (!dst || !src) return ENZOE_ERR_ARG;
+  KernelTimer kt((hipStream_t)stream, "k_calibration_copy");
+  hipLaunchKernelGGL(k_calibration_copy, dim3(nblk((long)n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, dst, src, (long)n);
+  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+}
+
 int enzoe_profile_report(char *names, int name_len, double *ms, int *count,
                          int cap) {
+  resolve_pending();
   int i = 0;
   for (auto &kv : g_prof) {
     if (i >= cap) break;

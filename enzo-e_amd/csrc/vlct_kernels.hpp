@@ -386,21 +386,21 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
 // ----------------------------------------------------------------- k_dt
 
 // CFL timestep of every cell (ghosts included): writes the pressure field
-// and min-reduces dx_i/(|v_i|+c_f) into *result (as the bit pattern of a
-// non-negative double, so an integer atomicMin is an exact double min; a
-// NaN local value never wins, as std::min in the reference ignores it).
-// EnzoMHDIntegratorStageCommands::timestep
-// (EnzoMHDIntegratorStageCommands.cpp:299-366).
+// and min-reduces dx_i/(|v_i|+c_f) into *result, stored as the bit pattern
+// of a non-negative double so that an integer atomicMin is an exact double
+// min (a NaN local value never wins, as std::min in the reference ignores
+// it).  EnzoMHDIntegratorStageCommands::timestep
+// (EnzoMHDIntegratorStageCommands.cpp:299-366).  Grid-stride loop; wave64
+// butterfly, then LDS across the workgroup, then one atomic per workgroup.
 template <typename FluxT, bool MHD>
 __global__ void __launch_bounds__(256)
 k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
      unsigned long long *result) {
   const BlockDesc<FluxT> &b = blocks[blockIdx.y];
-  const int mx = b.mx, my = b.my, mz = b.mz;
-  const long n = (long)mx * my * mz;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)b.mx * b.my * b.mz;
   double local = 1.7976931348623157e308;
-  if (c < n) {
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (long)gridDim.x * blockDim.x) {
     double d = b.U[S_RHO][c];
     double vx = b.U[S_VX][c], vy = b.U[S_VY][c], vz = b.U[S_VZ][c];
     double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
@@ -425,7 +425,6 @@ k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
                      b.cw[2] / (fabs(vz) + sig));
     local = dmin(local, l);
   }
-  // wave64 butterfly min, then one atomic per wavefront
   unsigned long long bits = __double_as_longlong(local);
   if (!(local >= 0.0)) bits = 0x7FF8000000000000ull;  // NaN never wins
 #pragma unroll
@@ -433,7 +432,15 @@ k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
     unsigned long long o = __shfl_xor(bits, off, 64);
     bits = (o < bits) ? o : bits;
   }
-  if ((threadIdx.x & 63) == 0) atomicMin(result, bits);
+  __shared__ unsigned long long wmin[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wmin[wave] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = (wmin[w] < m) ? wmin[w] : m;
+    atomicMin(result, m);
+  }
 }
 
 static __global__ void k_dt_init(unsigned long long *result) {
@@ -483,30 +490,78 @@ __device__ __forceinline__ bool refresh_axis(const RefreshGrid &G, int a,
   return true;
 }
 
-// arrays: table[b] = device pointer of this field for block b
+// Decodes the t-th ghost element of an (az, ay, ax) array whose interior is
+// [lo, hi) along each axis: z slabs first, then y slabs of the interior z
+// range, then x slabs of the interior (z, y) range.  Returns false past the
+// last ghost element.
+__device__ __forceinline__ bool ghost_coords(long t, const int ext[3],
+                                             const int lo[3], const int hi[3],
+                                             int &k, int &j, int &i) {
+  const int ax = ext[0], ay = ext[1], az = ext[2];
+  const int nxi = hi[0] - lo[0], nyi = hi[1] - lo[1], nzi = hi[2] - lo[2];
+  const long plane = (long)ay * ax;
+  const long cz = (long)(az - nzi) * plane;
+  if (t < cz) {
+    int pz = (int)(t / plane);
+    long r = t - (long)pz * plane;
+    k = pz < lo[2] ? pz : hi[2] + (pz - lo[2]);
+    j = (int)(r / ax);
+    i = (int)(r - (long)j * ax);
+    return true;
+  }
+  t -= cz;
+  const long row = (long)(ay - nyi) * ax;
+  const long cy = (long)nzi * row;
+  if (t < cy) {
+    int kk = (int)(t / row);
+    long r = t - (long)kk * row;
+    int py = (int)(r / ax);
+    k = lo[2] + kk;
+    j = py < lo[1] ? py : hi[1] + (py - lo[1]);
+    i = (int)(r - (long)py * ax);
+    return true;
+  }
+  t -= cy;
+  const int gx = ax - nxi;
+  const long cx = (long)nzi * nyi * gx;
+  if (t >= cx) return false;
+  long q = t / gx;
+  int px = (int)(t - q * gx);
+  k = lo[2] + (int)(q / nyi);
+  j = lo[1] + (int)(q % nyi);
+  i = px < lo[0] ? px : hi[0] + (px - lo[0]);
+  return true;
+}
+
+// table[f * nb + b] = device pointer of field f of block b; fields 0..7 are
+// cell-centred, 8..10 the face-centred bfieldi_x..z.  grid: x = ghost
+// elements, y = block, z = field.
 static __global__ void __launch_bounds__(256)
-k_refresh(double *const *__restrict__ table, RefreshGrid G, int fc) {
+k_refresh(double *const *__restrict__ table, RefreshGrid G) {
+  const int nb = G.nb[0] * G.nb[1] * G.nb[2];
   const int bidx = blockIdx.y;
+  const int f = blockIdx.z;
+  const int fc = (f >= 8) ? f - 8 : -1;
   const int bx = bidx % G.nb[0];
   const int by = (bidx / G.nb[0]) % G.nb[1];
   const int bz = bidx / (G.nb[0] * G.nb[1]);
-  const int ax = G.n[0] + 2 * G.g + (fc == 0);
-  const int ay = G.n[1] + 2 * G.g + (fc == 1);
-  const int az = G.n[2] + 2 * G.g + (fc == 2);
+  int ext[3], lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    ext[a] = G.n[a] + 2 * G.g + (fc == a);
+    lo[a] = G.g;
+    hi[a] = G.g + G.n[a] + (fc == a);
+  }
+  int k, j, i;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long nxy = (long)ax * ay;
-  if (t >= nxy * az) return;
-  const int k = (int)(t / nxy);
-  const long r = t - (long)k * nxy;
-  const int j = (int)(r / ax);
-  const int i = (int)(r - (long)j * ax);
+  if (!ghost_coords(t, ext, lo, hi, k, j, i)) return;
   int sb[3], si[3];
-  bool ghost = refresh_axis(G, 0, bx, i, fc == 0, sb[0], si[0]);
-  ghost |= refresh_axis(G, 1, by, j, fc == 1, sb[1], si[1]);
-  ghost |= refresh_axis(G, 2, bz, k, fc == 2, sb[2], si[2]);
-  if (!ghost) return;
+  refresh_axis(G, 0, bx, i, fc == 0, sb[0], si[0]);
+  refresh_axis(G, 1, by, j, fc == 1, sb[1], si[1]);
+  refresh_axis(G, 2, bz, k, fc == 2, sb[2], si[2]);
   const int sbid = (sb[2] * G.nb[1] + sb[1]) * G.nb[0] + sb[0];
-  table[bidx][t] = table[sbid][idx3(si[2], si[1], si[0], ay, ax)];
+  double *const *tab = table + (size_t)f * nb;
+  tab[bidx][idx3(k, j, i, ext[1], ext[0])] =
+      tab[sbid][idx3(si[2], si[1], si[0], ext[1], ext[0])];
 }
 
 }  // namespace enzoe

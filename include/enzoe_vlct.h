@@ -167,12 +167,19 @@ int enzoe_ct_compute_center_bfield(int dim, double *bfieldc,
 
 /* ---- diagnostics ---------------------------------------------------------- */
 
-/* Average duration (ms) of the last `enzoe_vlct_compute` launches of each
- * kernel, measured with HIP events on the compute stream when profiling is
- * on (enzoe_set_profiling(1)).  names/ms arrays of capacity cap.            */
+/* Per-kernel average duration (ms) of the launches made while profiling was
+ * on (enzoe_set_profiling(1)), measured with HIP events recorded on the
+ * launch stream around each kernel; the events are read when the report is
+ * asked for, so profiling adds no host synchronisation.  names holds cap
+ * strings of name_len bytes.  Returns the number of kernels reported.       */
 int enzoe_set_profiling(int on);
 int enzoe_profile_report(char *names, int name_len, double *ms, int *count,
                          int cap);
+
+/* Plain device copy of n doubles (a kernel of known traffic, n*8 bytes read
+ * and written), used to calibrate hardware byte counters.                   */
+int enzoe_calibration_copy(double *dst, const double *src, size_t n,
+                           enzoe_stream_t stream);
 
 /* Library version string. */
 const char *enzoe_version(void);

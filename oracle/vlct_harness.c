@@ -175,7 +175,7 @@ double o_grid_cycle(const o_config *cfg, o_grid *gr, double t, double t_stop,
   dt *= 1.0; /* Method::courant_global */
   o_grid_refresh(cfg, gr);
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) if (nb > 1)
 #endif
   for (int i = 0; i < nb; i++) o_vlct_compute(cfg, &gr->blocks[i], dt);
   (void)nthreads;
