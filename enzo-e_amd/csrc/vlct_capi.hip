@@ -1,6 +1,7 @@
 // vlct_capi.hip -- C ABI of the gfx950 VL+CT library (declared in
-// include/enzoe_vlct.h).  Owns the method configuration, the device scratch
-// pool and the launch sequence of one VL+CT step over a list of blocks.
+// include/enzoe_vlct.h): method configuration, timestep, compute, refresh,
+// profiling.  Built with -ffp-contract=off: the FP64 path rounds exactly
+// like the reference's CPU build.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -11,36 +12,14 @@
 #include <string>
 #include <vector>
 
-#include "../../include/enzoe_vlct.h"
-#include "vlct_kernels.hpp"
+#include "vlct_launch.hpp"
 
 using namespace enzoe;
 
-namespace {
+namespace enzoe {
 
-const char *kVersion = "enzoe-vlct 0.1 (gfx950)";
-
-std::string lower(const char *s, const char *dflt) {
-  std::string r = s ? s : dflt;
-  std::transform(r.begin(), r.end(), r.begin(),
-                 [](unsigned char c) { return (char)std::tolower(c); });
-  return r;
-}
-
-// ---------------------------------------------------------------- profiling
-
-struct ProfEntry {
-  double total_ms = 0;
-  long count = 0;
-};
 bool g_profiling = false;
 std::map<std::string, ProfEntry> g_prof;
-// launches recorded but not yet resolved: events are read only when a
-// report is asked for, so profiling adds no host synchronisation
-struct PendingTiming {
-  const char *name;
-  hipEvent_t a, b;
-};
 std::vector<PendingTiming> g_pending;
 std::vector<hipEvent_t> g_event_pool;
 
@@ -69,279 +48,23 @@ void resolve_pending() {
   g_pending.clear();
 }
 
-struct KernelTimer {
-  hipStream_t s;
-  const char *name;
-  hipEvent_t a = nullptr, b = nullptr;
-  KernelTimer(hipStream_t s_, const char *n) : s(s_), name(n) {
-    if (g_profiling) {
-      a = pool_event();
-      b = pool_event();
-      (void)hipEventRecord(a, s);
-    }
-  }
-  ~KernelTimer() {
-    if (g_profiling) {
-      (void)hipEventRecord(b, s);
-      g_pending.push_back({name, a, b});
-      if (g_pending.size() > 4096) resolve_pending();
-    }
-  }
-};
+}  // namespace enzoe
+
+namespace {
+
+const char *kVersion = "enzoe-vlct 0.2 (gfx950)";
+
+std::string lower(const char *s, const char *dflt) {
+  std::string r = s ? s : dflt;
+  std::transform(r.begin(), r.end(), r.begin(),
+                 [](unsigned char c) { return (char)std::tolower(c); });
+  return r;
+}
 
 __global__ void k_calibration_copy(double *__restrict__ dst,
                                    const double *__restrict__ src, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
-}
-
-inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
-
-}  // namespace
-
-// ------------------------------------------------------------------- method
-
-struct enzoe_vlct_method {
-  int mhd = 1;
-  int riemann = RS_HLLD;
-  int recon[2] = {RC_NN, RC_PLM};
-  int nstages = 2;
-  int precision = ENZOE_PREC_DOUBLE;
-  double theta = 1.5, courant = 0.3, gamma = 5.0 / 3.0;
-  double dfloor = 1e-200, pfloor = 1e-200;
-
-  // scratch pool (grown on demand)
-  void *scratch = nullptr;
-  size_t scratch_bytes = 0;
-  // descriptor table
-  void *d_desc = nullptr;
-  size_t desc_bytes = 0;
-  std::vector<char> last_desc;
-  unsigned long long *d_dt = nullptr;
-  void *d_ptrtab = nullptr;
-  size_t ptrtab_bytes = 0;
-};
-
-namespace {
-
-int total_staling(int r) { return r == RC_NN ? 1 : 2; }
-int immediate_staling(int r) { return r == RC_NN ? 0 : 1; }
-
-int parse_recon(const std::string &n, int *out) {
-  if (n == "nn") { *out = RC_NN; return 0; }
-  if (n == "plm" || n == "plm_enzo") { *out = RC_PLM; return 0; }
-  if (n == "plm_athena") { *out = RC_PLM_ATHENA; return 0; }
-  return ENZOE_ERR_CONFIG;
-}
-
-// bytes of scratch per block
-template <typename FluxT>
-size_t scratch_per_block(int mx, int my, int mz, int mhd) {
-  size_t ncc = (size_t)mx * my * mz;
-  size_t nq = mhd ? 8 : 5;
-  size_t bytes = 0;
-  bytes += nq * ncc * sizeof(double);                  // Ut
-  if (mhd) {
-    bytes += ((size_t)mz * my * (mx + 1) + (size_t)mz * (my + 1) * mx +
-              (size_t)(mz + 1) * my * mx) * sizeof(double);  // bft
-    bytes += 3 * ncc * sizeof(double);                 // E
-  }
-  bytes += 3 * (mhd ? 7 : 5) * ncc * sizeof(FluxT);    // F
-  return bytes;
-}
-
-int ensure_bytes(void **ptr, size_t *have, size_t need) {
-  if (*have >= need) return 0;
-  // work queued earlier may still read the old buffer
-  if (*ptr) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(*ptr);
-  }
-  *ptr = nullptr;
-  *have = 0;
-  if (hipMalloc(ptr, need) != hipSuccess) return ENZOE_ERR_ALLOC;
-  *have = need;
-  return 0;
-}
-
-int check_block(const enzoe_vlct_method *m, const enzoe_block &b) {
-  if (b.mx < 7 || b.my < 7 || b.mz < 7) return ENZOE_ERR_ARG;
-  // EnzoMethodMHDVlct.cpp:124-133: ghost depth >= sum of staling rates
-  int req = 0;
-  for (int s = 0; s < m->nstages; s++) req += total_staling(m->recon[s]);
-  if (b.ghost < req) return ENZOE_ERR_ARG;
-  if (!b.density || !b.velocity[0] || !b.velocity[1] || !b.velocity[2] ||
-      !b.total_energy)
-    return ENZOE_ERR_ARG;
-  if (m->mhd) {
-    for (int d = 0; d < 3; d++)
-      if (!b.bfield[d] || !b.bfieldi[d]) return ENZOE_ERR_ARG;
-  }
-  return 0;
-}
-
-// Fills descriptors for blocks [b0, b1) using the scratch pool.
-template <typename FluxT>
-void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
-               int b1, std::vector<BlockDesc<FluxT>> &out) {
-  out.clear();
-  char *base = (char *)m->scratch;
-  for (int bi = b0; bi < b1; bi++) {
-    const enzoe_block &b = blocks[bi];
-    BlockDesc<FluxT> d;
-    std::memset(&d, 0, sizeof(d));
-    d.U[S_RHO] = b.density;
-    for (int a = 0; a < 3; a++) d.U[S_VX + a] = b.velocity[a];
-    d.U[S_E] = b.total_energy;
-    if (m->mhd)
-      for (int a = 0; a < 3; a++) {
-        d.U[S_BX + a] = b.bfield[a];
-        d.bfi[a] = b.bfieldi[a];
-      }
-    d.pressure = b.pressure;
-    d.mx = b.mx; d.my = b.my; d.mz = b.mz;
-    for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
-    const size_t ncc = (size_t)b.mx * b.my * b.mz;
-    const int nq = m->mhd ? 8 : 5;
-    for (int q = 0; q < S_N; q++) {
-      if (!m->mhd && q >= S_BX) continue;
-      d.Ut[q] = (double *)base;
-      base += ncc * sizeof(double);
-    }
-    (void)nq;
-    if (m->mhd) {
-      d.bft[0] = (double *)base; base += (size_t)b.mz * b.my * (b.mx + 1) * sizeof(double);
-      d.bft[1] = (double *)base; base += (size_t)b.mz * (b.my + 1) * b.mx * sizeof(double);
-      d.bft[2] = (double *)base; base += (size_t)(b.mz + 1) * b.my * b.mx * sizeof(double);
-      for (int a = 0; a < 3; a++) {
-        d.E[a] = (double *)base;
-        base += ncc * sizeof(double);
-      }
-    }
-    for (int dd = 0; dd < 3; dd++) {
-      for (int q = 0; q < S_N; q++) {
-        if (!m->mhd && q >= S_BX) continue;
-        if (m->mhd && q == S_BX + dd) continue;  // normal-B flux is 0
-        d.F[dd][q] = (FluxT *)base;
-        base += ncc * sizeof(FluxT);
-      }
-    }
-    out.push_back(d);
-  }
-}
-
-template <typename FluxT, int RECON, int RIEMANN, bool MHD>
-void launch_flux(const BlockDesc<FluxT> *d, int nb, long nthreads,
-                 StageParams sp, hipStream_t s) {
-  dim3 grid(nblk(nthreads, 256), nb);
-  hipLaunchKernelGGL((k_flux<FluxT, FluxT, RECON, RIEMANN, MHD>), grid,
-                     dim3(256), 0, s, d, sp);
-}
-
-template <typename FluxT, bool MHD>
-void dispatch_flux(const enzoe_vlct_method *m, int recon,
-                   const BlockDesc<FluxT> *d, int nb, long nthreads,
-                   StageParams sp, hipStream_t s) {
-  if (MHD) {
-    if (m->riemann == RS_HLLD) {
-      if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLD, MHD>(d, nb, nthreads, sp, s);
-      else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLD, MHD>(d, nb, nthreads, sp, s);
-      else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLD, MHD>(d, nb, nthreads, sp, s);
-    } else {
-      if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLE, MHD>(d, nb, nthreads, sp, s);
-      else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLE, MHD>(d, nb, nthreads, sp, s);
-      else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD>(d, nb, nthreads, sp, s);
-    }
-  } else {
-    if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLC, MHD>(d, nb, nthreads, sp, s);
-    else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLC, MHD>(d, nb, nthreads, sp, s);
-    else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLC, MHD>(d, nb, nthreads, sp, s);
-  }
-}
-
-long cube(int mx, int my, int mz, int lo, int hi_trim) {
-  long a = mx - lo - hi_trim, b = my - lo - hi_trim, c = mz - lo - hi_trim;
-  if (a <= 0 || b <= 0 || c <= 0) return 0;
-  return a * b * c;
-}
-
-// One VL+CT step (EnzoMethodMHDVlct::compute) on blocks of equal shape.
-template <typename FluxT, bool MHD>
-int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
-                  double dt, hipStream_t s) {
-  const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
-  const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd);
-  // chunk size: as many blocks as fit a scratch budget of ~4 GiB (at least 1)
-  const size_t budget = (size_t)4 << 30;
-  int chunk = (int)std::max<size_t>(1, std::min<size_t>(n, budget / per));
-  if (int rc = ensure_bytes(&m->scratch, &m->scratch_bytes, per * chunk)) return rc;
-
-  std::vector<BlockDesc<FluxT>> desc;
-  for (int c0 = 0; c0 < n; c0 += chunk) {
-    const int c1 = std::min(n, c0 + chunk);
-    const int nb = c1 - c0;
-    fill_desc<FluxT>(m, blocks, c0, c1, desc);
-    const size_t dbytes = desc.size() * sizeof(BlockDesc<FluxT>);
-    // upload the descriptor table only when it changed
-    const char *raw = (const char *)desc.data();
-    bool same = (m->last_desc.size() == dbytes) &&
-                std::memcmp(m->last_desc.data(), raw, dbytes) == 0;
-    if (!same) {
-      if (int rc = ensure_bytes(&m->d_desc, &m->desc_bytes, dbytes)) return rc;
-      if (hipMemcpyAsync(m->d_desc, raw, dbytes, hipMemcpyHostToDevice, s) !=
-          hipSuccess)
-        return ENZOE_ERR_HIP;
-      // the host copy must stay valid until the copy has been consumed
-      if (hipStreamSynchronize(s) != hipSuccess) return ENZOE_ERR_HIP;
-      m->last_desc.assign(raw, raw + dbytes);
-    }
-    const BlockDesc<FluxT> *d = (const BlockDesc<FluxT> *)m->d_desc;
-
-    int stale = 0;
-    for (int stage = 0; stage < m->nstages; stage++) {
-      const bool is_final = (stage + 1) == m->nstages;
-      const int recon = m->recon[stage];
-      StageParams sp;
-      sp.stage = stage;
-      sp.is_final = is_final ? 1 : 0;
-      sp.stale = stale;
-      sp.cur_stale = stale + immediate_staling(recon);
-      // EnzoMethodMHDVlct.cpp:462: dt/2 for the partial step
-      sp.dt = (!is_final) ? dt / 2. : dt;
-      sp.gamma = m->gamma;
-      sp.density_floor = m->dfloor;
-      sp.pressure_floor = m->pfloor;
-      sp.theta = m->theta;
-      const int cs = sp.cur_stale;
-      {
-        KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm");
-        dispatch_flux<FluxT, MHD>(m, recon, d, nb, cube(mx, my, mz, cs, cs), sp, s);
-      }
-      if (MHD) {
-        {
-          KernelTimer kt(s, "k_edge");
-          hipLaunchKernelGGL((k_edge<FluxT>), dim3(nblk(cube(mx, my, mz, cs, cs), 256), nb),
-                             dim3(256), 0, s, d, sp);
-        }
-        {
-          KernelTimer kt(s, "k_faces");
-          hipLaunchKernelGGL((k_faces<FluxT>),
-                             dim3(nblk(cube(mx, my, mz, cs + 1, cs), 256), nb), dim3(256),
-                             0, s, d, sp);
-        }
-      }
-      {
-        KernelTimer kt(s, "k_cells");
-        const int lo = is_final ? cs : cs + 1;
-        hipLaunchKernelGGL((k_cells<FluxT, MHD>),
-                           dim3(nblk(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
-                           s, d, sp);
-      }
-      stale += total_staling(recon);
-    }
-  }
-  if (hipGetLastError() != hipSuccess) return ENZOE_ERR_HIP;
-  return 0;
 }
 
 }  // namespace
@@ -449,8 +172,7 @@ int enzoe_vlct_compute(enzoe_vlct_t m, const enzoe_block *blocks, int n,
       rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, s)
                   : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, s);
     } else {
-      rc = m->mhd ? compute_group<float, true>(m, blocks + b0, b1 - b0, dt, s)
-                  : compute_group<float, false>(m, blocks + b0, b1 - b0, dt, s);
+      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, s);
     }
     if (rc) return rc;
     b0 = b1;

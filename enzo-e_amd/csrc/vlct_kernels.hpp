@@ -68,132 +68,169 @@ __device__ __forceinline__ bool box_coords(long t, int lo, int nx, int ny,
   return true;
 }
 
+// Loads through the global address space (the pointers come from a
+// descriptor table in memory, so the compiler cannot infer it by itself and
+// would otherwise emit flat accesses).
+typedef const __attribute__((address_space(1))) double *gdp;
+template <typename P>
+__device__ __forceinline__ P gld(const P *p, int i) {
+  return ((const __attribute__((address_space(1))) P *)p)[i];
+}
+
 // Cell primitives w = (rho, vx, vy, vz, p, bx, by, bz) from the integration
 // quantities (EnzoPhysicsFluidProps.cpp:64-138, EnzoComputePressure.cpp:
 // 177-189), in FP64.
 template <bool MHD>
-__device__ __forceinline__ void load_prim(double *const U[S_N], long c,
+__device__ __forceinline__ void load_prim(double *const U[S_N], int c,
                                           double gm1, double w[S_N]) {
-  double rho = U[S_RHO][c];
-  double vx = U[S_VX][c], vy = U[S_VY][c], vz = U[S_VZ][c];
+  double rho = gld(U[S_RHO], c);
+  double vx = gld(U[S_VX], c), vy = gld(U[S_VY], c), vz = gld(U[S_VZ], c);
   double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
   double me = 0.;
   w[S_RHO] = rho; w[S_VX] = vx; w[S_VY] = vy; w[S_VZ] = vz;
   if (MHD) {
-    double bx = U[S_BX][c], by = U[S_BY][c], bz = U[S_BZ][c];
+    double bx = gld(U[S_BX], c), by = gld(U[S_BY], c), bz = gld(U[S_BZ], c);
     me = 0.5 * (bx * bx + by * by + bz * bz);
     w[S_BX] = bx; w[S_BY] = by; w[S_BZ] = bz;
+  } else {
+    w[S_BX] = 0.; w[S_BY] = 0.; w[S_BZ] = 0.;
   }
-  w[S_E] = gm1 * (rho * (U[S_E][c] - ke) - me);
+  w[S_E] = gm1 * (rho * (gld(U[S_E], c) - ke) - me);
 }
 
 // ---------------------------------------------------------------- k_flux
 
-// One thread per cell p of the box [cur_stale, m - cur_stale); for each
-// dim d it solves the face between p and p+e_d when that face lies in the
-// flux region of the stage (EnzoRiemannImpl.hpp:317-338 with
-// stale_depth = cur_stale).
+struct FluxConsts {
+  double gm1, gamma, theta, dfloor, pfloor;
+};
+
+// Flux through the face between cell c and c + e_D (D fixed at compile
+// time so the i/j/k permutation costs nothing): reconstruction of both
+// sides, the face-centred normal B (EnzoBfieldMethodCT.cpp:122-166) and the
+// Riemann solve, written at index c of the flux arrays of dim D.
+template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD, int D>
+__device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
+                                          double *const *U,
+                                          const double *bfi, int k, int j,
+                                          int i, int c, const double wc[S_N],
+                                          const FluxConsts &fc) {
+  constexpr int DJ = (D + 1) % 3, DK = (D + 2) % 3;
+  const int s = (D == 0) ? 1 : ((D == 1) ? b.mx : b.mx * b.my);
+  T wl[S_N], wr[S_N];
+  if (RECON == RC_NN) {
+    double a[S_N];
+    load_prim<MHD>(U, c + s, fc.gm1, a);
+#pragma unroll
+    for (int q = 0; q < S_N; q++) { wl[q] = (T)wc[q]; wr[q] = (T)a[q]; }
+  } else {
+    double w0[S_N], w2[S_N], w3[S_N];  // cells c-1, c+1, c+2 along D
+    load_prim<MHD>(U, c - s, fc.gm1, w0);
+    load_prim<MHD>(U, c + s, fc.gm1, w2);
+    load_prim<MHD>(U, c + 2 * s, fc.gm1, w3);
+    const T theta = (T)fc.theta;
+#pragma unroll
+    for (int q = 0; q < S_N; q++) {
+      if (!MHD && q >= S_BX) continue;
+      if (MHD && q == S_BX + D) continue;  // replaced by the face value
+      T a0 = (T)w0[q], a1 = (T)wc[q], a2 = (T)w2[q], a3 = (T)w3[q];
+      T dvl = (RECON == RC_PLM) ? limiter_enzo(a0, a1, a2, theta)
+                                : limiter_athena(a0, a1, a2);
+      T dvr = (RECON == RC_PLM) ? limiter_enzo(a1, a2, a3, theta)
+                                : limiter_athena(a1, a2, a3);
+      T lv = a1 + dvl * T(0.5);
+      T rv = a2 - dvr * T(0.5);
+      if (q == S_RHO) {
+        lv = dmax(lv, (T)fc.dfloor);
+        rv = dmax(rv, (T)fc.dfloor);
+      }
+      if (q == S_E) {
+        lv = dmax(lv, (T)fc.pfloor);
+        rv = dmax(rv, (T)fc.pfloor);
+      }
+      wl[q] = lv;
+      wr[q] = rv;
+    }
+  }
+  const T gamma = (T)fc.gamma;
+  T fo[8];
+  if (MHD) {
+    const int bnx = b.mx + (D == 0), bny = b.my + (D == 1);
+    const T bface =
+        (T)gld(bfi, (k + (D == 2)) * bny * bnx + (j + (D == 1)) * bnx + i + (D == 0));
+    T l8[8], r8[8];
+    l8[L_D] = wl[S_RHO]; r8[L_D] = wr[S_RHO];
+    l8[L_BI] = bface; r8[L_BI] = bface;
+    l8[L_BJ] = wl[S_BX + DJ]; r8[L_BJ] = wr[S_BX + DJ];
+    l8[L_BK] = wl[S_BX + DK]; r8[L_BK] = wr[S_BX + DK];
+    l8[L_VI] = wl[S_VX + D]; r8[L_VI] = wr[S_VX + D];
+    l8[L_VJ] = wl[S_VX + DJ]; r8[L_VJ] = wr[S_VX + DJ];
+    l8[L_VK] = wl[S_VX + DK]; r8[L_VK] = wr[S_VX + DK];
+    l8[L_E] = wl[S_E]; r8[L_E] = wr[S_E];
+    if (RIEMANN == RS_HLLD) {
+      hlld(l8, r8, gamma, fo);
+    } else {
+      hlle_mhd(l8, r8, gamma, fo);
+    }
+    b.F[D][S_RHO][c] = (FluxT)fo[L_D];
+    b.F[D][S_VX + D][c] = (FluxT)fo[L_VI];
+    b.F[D][S_VX + DJ][c] = (FluxT)fo[L_VJ];
+    b.F[D][S_VX + DK][c] = (FluxT)fo[L_VK];
+    b.F[D][S_E][c] = (FluxT)fo[L_E];
+    b.F[D][S_BX + DJ][c] = (FluxT)fo[L_BJ];
+    b.F[D][S_BX + DK][c] = (FluxT)fo[L_BK];
+  } else {
+    T l5[5], r5[5];
+    l5[H_D] = wl[S_RHO]; r5[H_D] = wr[S_RHO];
+    l5[H_VI] = wl[S_VX + D]; r5[H_VI] = wr[S_VX + D];
+    l5[H_VJ] = wl[S_VX + DJ]; r5[H_VJ] = wr[S_VX + DJ];
+    l5[H_VK] = wl[S_VX + DK]; r5[H_VK] = wr[S_VX + DK];
+    l5[H_E] = wl[S_E]; r5[H_E] = wr[S_E];
+    hllc(l5, r5, gamma, fo);
+    b.F[D][S_RHO][c] = (FluxT)fo[H_D];
+    b.F[D][S_VX + D][c] = (FluxT)fo[H_VI];
+    b.F[D][S_VX + DJ][c] = (FluxT)fo[H_VJ];
+    b.F[D][S_VX + DK][c] = (FluxT)fo[H_VK];
+    b.F[D][S_E][c] = (FluxT)fo[H_E];
+  }
+}
+
+// One thread per cell p of the box [cur_stale, m - cur_stale); it solves
+// the three faces between p and p + e_d that lie in the flux region of the
+// stage (EnzoRiemannImpl.hpp:317-338 with stale_depth = cur_stale: face
+// index range [cs, m_d - 1 - cs) along d).
 template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD>
 __global__ void __launch_bounds__(256)
 k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const BlockDesc<FluxT> &b = blocks[blockIdx.y];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
-  int k, j, i;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!box_coords(t, cs, mx - 2 * cs, my - 2 * cs, mz - 2 * cs, k, j, i))
-    return;
+  const int nx = mx - 2 * cs, ny = my - 2 * cs;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nxy = nx * ny;
+  if (t >= nxy * (mz - 2 * cs)) return;
+  int k = t / nxy;
+  int r = t - k * nxy;
+  int j = r / nx;
+  int i = r - j * nx + cs;
+  k += cs;
+  j += cs;
   double *const *U = (sp.stage == 0) ? b.U : b.Ut;
-  const double *bfi_rec[3] = {(sp.stage == 0) ? b.bfi[0] : b.bft[0],
-                              (sp.stage == 0) ? b.bfi[1] : b.bft[1],
-                              (sp.stage == 0) ? b.bfi[2] : b.bft[2]};
-  const double gm1 = sp.gamma - 1.0;
-  const T gamma = (T)sp.gamma;
-  const T theta = (T)sp.theta;
-  const T dfloor = (T)sp.density_floor, pfloor = (T)sp.pressure_floor;
-  const int p[3] = {i, j, k};
-  const int m[3] = {mx, my, mz};
-  const long c = idx3(k, j, i, my, mx);
-  const long str[3] = {1, (long)mx, (long)mx * my};
-  constexpr int NK = MHD ? 8 : 5;
-
-#pragma unroll 1
-  for (int d = 0; d < 3; d++) {
-    if (p[d] >= m[d] - 1 - cs) continue;  // face index range [cs, m_d-1-cs)
-    T wl[S_N], wr[S_N];
-    if (RECON == RC_NN) {
-      double a[S_N], bq[S_N];
-      load_prim<MHD>(U, c, gm1, a);
-      load_prim<MHD>(U, c + str[d], gm1, bq);
-#pragma unroll
-      for (int q = 0; q < S_N; q++) { wl[q] = (T)a[q]; wr[q] = (T)bq[q]; }
-    } else {
-      double w0[S_N], w1[S_N], w2[S_N], w3[S_N];  // cells p-1 .. p+2
-      load_prim<MHD>(U, c - str[d], gm1, w0);
-      load_prim<MHD>(U, c, gm1, w1);
-      load_prim<MHD>(U, c + str[d], gm1, w2);
-      load_prim<MHD>(U, c + 2 * str[d], gm1, w3);
-#pragma unroll
-      for (int q = 0; q < S_N; q++) {
-        if (!MHD && q >= S_BX) continue;
-        T a0 = (T)w0[q], a1 = (T)w1[q], a2 = (T)w2[q], a3 = (T)w3[q];
-        T dvl = (RECON == RC_PLM) ? limiter_enzo(a0, a1, a2, theta)
-                                  : limiter_athena(a0, a1, a2);
-        T dvr = (RECON == RC_PLM) ? limiter_enzo(a1, a2, a3, theta)
-                                  : limiter_athena(a1, a2, a3);
-        T lv = a1 + dvl * T(0.5);
-        T rv = a2 - dvr * T(0.5);
-        if (q == S_RHO) { lv = dmax(lv, dfloor); rv = dmax(rv, dfloor); }
-        if (q == S_E) { lv = dmax(lv, pfloor); rv = dmax(rv, pfloor); }
-        wl[q] = lv;
-        wr[q] = rv;
-      }
-    }
-    const int dj = (d + 1) % 3, dk = (d + 2) % 3;
-    T fo[8];
-    if (MHD) {
-      // correct_reconstructed_bfield (EnzoBfieldMethodCT.cpp:122-166):
-      // the face value of bfieldi (index p_d + 1 along d)
-      int bnx = mx + (d == 0), bny = my + (d == 1);
-      T bface = (T)bfi_rec[d][idx3(k + (d == 2), j + (d == 1), i + (d == 0),
-                                   bny, bnx)];
-      T l8[8], r8[8];
-      l8[L_D] = wl[S_RHO]; r8[L_D] = wr[S_RHO];
-      l8[L_BI] = bface; r8[L_BI] = bface;
-      l8[L_BJ] = wl[S_BX + dj]; r8[L_BJ] = wr[S_BX + dj];
-      l8[L_BK] = wl[S_BX + dk]; r8[L_BK] = wr[S_BX + dk];
-      l8[L_VI] = wl[S_VX + d]; r8[L_VI] = wr[S_VX + d];
-      l8[L_VJ] = wl[S_VX + dj]; r8[L_VJ] = wr[S_VX + dj];
-      l8[L_VK] = wl[S_VX + dk]; r8[L_VK] = wr[S_VX + dk];
-      l8[L_E] = wl[S_E]; r8[L_E] = wr[S_E];
-      if (RIEMANN == RS_HLLD) {
-        hlld(l8, r8, gamma, fo);
-      } else {
-        hlle_mhd(l8, r8, gamma, fo);
-      }
-      b.F[d][S_RHO][c] = (FluxT)fo[L_D];
-      b.F[d][S_VX + d][c] = (FluxT)fo[L_VI];
-      b.F[d][S_VX + dj][c] = (FluxT)fo[L_VJ];
-      b.F[d][S_VX + dk][c] = (FluxT)fo[L_VK];
-      b.F[d][S_E][c] = (FluxT)fo[L_E];
-      b.F[d][S_BX + dj][c] = (FluxT)fo[L_BJ];
-      b.F[d][S_BX + dk][c] = (FluxT)fo[L_BK];
-    } else {
-      T l5[5], r5[5];
-      l5[H_D] = wl[S_RHO]; r5[H_D] = wr[S_RHO];
-      l5[H_VI] = wl[S_VX + d]; r5[H_VI] = wr[S_VX + d];
-      l5[H_VJ] = wl[S_VX + dj]; r5[H_VJ] = wr[S_VX + dj];
-      l5[H_VK] = wl[S_VX + dk]; r5[H_VK] = wr[S_VX + dk];
-      l5[H_E] = wl[S_E]; r5[H_E] = wr[S_E];
-      hllc(l5, r5, gamma, fo);
-      b.F[d][S_RHO][c] = (FluxT)fo[H_D];
-      b.F[d][S_VX + d][c] = (FluxT)fo[H_VI];
-      b.F[d][S_VX + dj][c] = (FluxT)fo[H_VJ];
-      b.F[d][S_VX + dk][c] = (FluxT)fo[H_VK];
-      b.F[d][S_E][c] = (FluxT)fo[H_E];
-    }
-    (void)NK;
-  }
+  const double *const *bfr = (sp.stage == 0) ? b.bfi : b.bft;
+  FluxConsts fc;
+  fc.gm1 = sp.gamma - 1.0;
+  fc.gamma = sp.gamma;
+  fc.theta = sp.theta;
+  fc.dfloor = sp.density_floor;
+  fc.pfloor = sp.pressure_floor;
+  const int c = (k * my + j) * mx + i;
+  double wc[S_N];
+  load_prim<MHD>(U, c, fc.gm1, wc);
+  if (i < mx - 1 - cs)
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 0>(b, U, bfr[0], k, j, i, c, wc, fc);
+  if (j < my - 1 - cs)
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 1>(b, U, bfr[1], k, j, i, c, wc, fc);
+  if (k < mz - 1 - cs)
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 2>(b, U, bfr[2], k, j, i, c, wc, fc);
 }
 
 // ---------------------------------------------------------------- k_edge

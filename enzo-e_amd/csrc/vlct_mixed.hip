@@ -1,0 +1,18 @@
+// vlct_mixed.hip -- the mixed-precision VL+CT step: FP32 reconstruction and
+// Riemann fluxes (fast FP32 reciprocal and square root), FP64 primitives,
+// flux divergence, CT EMF / face-B update and dt.  Built with
+// -ffp-contract=fast: this path is checked against the FP64 result by its
+// L1 error, not bit by bit.
+#include <hip/hip_runtime.h>
+
+#include "vlct_launch.hpp"
+
+namespace enzoe {
+
+int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
+                  double dt, hipStream_t s) {
+  return m->mhd ? compute_group<float, true>(m, blocks, n, dt, s)
+                : compute_group<float, false>(m, blocks, n, dt, s);
+}
+
+}  // namespace enzoe

@@ -24,6 +24,18 @@ enum : int { H_D = 0, H_VI, H_VJ, H_VK, H_E };
 enum : int { RS_HLLE = 0, RS_HLLC = 1, RS_HLLD = 2 };
 enum : int { RC_NN = 0, RC_PLM = 1, RC_PLM_ATHENA = 2 };
 
+// Division and square root: exact IEEE operations in FP64 (reference
+// parity); hardware reciprocal / square root (about 1 ulp) in FP32, where
+// the mixed-precision path is judged by its L1 error instead.
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float qdiv(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+__device__ __forceinline__ double qsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float qsqrt(float a) {
+  return __builtin_amdgcn_sqrtf(a);
+}
+
 template <typename T> __device__ __forceinline__ T dmin(T a, T b) {
   return (b < a) ? b : a;  // std::min
 }
@@ -56,7 +68,7 @@ __device__ __forceinline__ T limiter_athena(T vm1, T v, T vp1) {
   T dv_r = (vp1 - v);
   T temp = dv_l * dv_r;
   if (temp <= T(0)) return T(0);
-  return T(2) * temp / (dv_l + dv_r);
+  return qdiv(T(2) * temp, dv_l + dv_r);
 }
 
 // EnzoEOSIdeal::fast_magnetosonic_speed<-1> (EnzoEOSIdeal.hpp:111-140);
@@ -65,12 +77,12 @@ template <typename T>
 __device__ __forceinline__ T fast_ms_speed(T d, T p, T bi, T bj, T bk,
                                            T gamma) {
   const T B2 = ((bi * bi) + ((bj * bj) + (bk * bk)));
-  const T cs2 = gamma * p / d;
-  const T inv_density = T(1) / d;
+  const T cs2 = qdiv(gamma * p, d);
+  const T inv_density = qdiv(T(1), d);
   const T va2 = B2 * inv_density;
   const T va2_cos2 = (bi * bi) * inv_density;
   const T sum = cs2 + va2;
-  return sqrt(T(0.5) * (va2 + cs2 + sqrt(sum * sum - T(4) * cs2 * va2_cos2)));
+  return qsqrt(T(0.5) * (va2 + cs2 + qsqrt(sum * sum - T(4) * cs2 * va2_cos2)));
 }
 
 template <typename T> struct Cons1D { T d, mx, my, mz, e, by, bz; };
@@ -82,7 +94,7 @@ template <typename T>
 __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
                                      T flx[8]) {
   const T SMALL_NUMBER = T(1.0e-8);
-  const T igm1 = T(1) / (gamma - T(1));
+  const T igm1 = qdiv(T(1), gamma - T(1));
   T spd[5];
   Cons1D<T> ul, ur, ulst, uldst, urdst, urst, fl, fr;
 
@@ -147,22 +159,22 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
 
   T sdl = spd[0] - wli[L_VI];
   T sdr = spd[4] - wri[L_VI];
-  spd[2] = (sdr * ur.mx - sdl * ul.mx + (ptl - ptr)) /
-           (sdr * ur.d - sdl * ul.d);
+  spd[2] = qdiv(sdr * ur.mx - sdl * ul.mx + (ptl - ptr),
+                sdr * ur.d - sdl * ul.d);
 
   T sdml = spd[0] - spd[2];
   T sdmr = spd[4] - spd[2];
-  T sdml_inv = T(1) / sdml;
-  T sdmr_inv = T(1) / sdmr;
+  T sdml_inv = qdiv(T(1), sdml);
+  T sdmr_inv = qdiv(T(1), sdmr);
   ulst.d = ul.d * sdl * sdml_inv;
   urst.d = ur.d * sdr * sdmr_inv;
-  T ulst_d_inv = T(1) / ulst.d;
-  T urst_d_inv = T(1) / urst.d;
-  T sqrtdl = sqrt(ulst.d);
-  T sqrtdr = sqrt(urst.d);
+  T ulst_d_inv = qdiv(T(1), ulst.d);
+  T urst_d_inv = qdiv(T(1), urst.d);
+  T sqrtdl = qsqrt(ulst.d);
+  T sqrtdr = qsqrt(urst.d);
 
-  spd[1] = spd[2] - fabs(bxi) / sqrtdl;
-  spd[3] = spd[2] + fabs(bxi) / sqrtdr;
+  spd[1] = spd[2] - qdiv(fabs(bxi), sqrtdl);
+  spd[3] = spd[2] + qdiv(fabs(bxi), sqrtdr);
 
   T ptstl = ptl + ul.d * sdl * (spd[2] - wli[L_VI]);
   T ptstr = ptr + ur.d * sdr * (spd[2] - wri[L_VI]);
@@ -175,10 +187,10 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     ulst.by = ul.by;
     ulst.bz = ul.bz;
   } else {
-    T tmp = bxi * (sdl - sdml) / (ul.d * sdl * sdml - bxsq);
+    T tmp = qdiv(bxi * (sdl - sdml), ul.d * sdl * sdml - bxsq);
     ulst.my = ulst.d * (wli[L_VJ] - ul.by * tmp);
     ulst.mz = ulst.d * (wli[L_VK] - ul.bz * tmp);
-    tmp = (ul.d * (sdl * sdl) - bxsq) / (ul.d * sdl * sdml - bxsq);
+    tmp = qdiv(ul.d * (sdl * sdl) - bxsq, ul.d * sdl * sdml - bxsq);
     ulst.by = ul.by * tmp;
     ulst.bz = ul.bz * tmp;
   }
@@ -196,10 +208,10 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     urst.by = ur.by;
     urst.bz = ur.bz;
   } else {
-    T tmp = bxi * (sdr - sdmr) / (ur.d * sdr * sdmr - bxsq);
+    T tmp = qdiv(bxi * (sdr - sdmr), ur.d * sdr * sdmr - bxsq);
     urst.my = urst.d * (wri[L_VJ] - ur.by * tmp);
     urst.mz = urst.d * (wri[L_VK] - ur.bz * tmp);
-    tmp = (ur.d * (sdr * sdr) - bxsq) / (ur.d * sdr * sdmr - bxsq);
+    tmp = qdiv(ur.d * (sdr * sdr) - bxsq, ur.d * sdr * sdmr - bxsq);
     urst.by = ur.by * tmp;
     urst.bz = ur.bz * tmp;
   }
@@ -214,7 +226,7 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     uldst = ulst;
     urdst = urst;
   } else {
-    T invsumd = T(1) / (sqrtdl + sqrtdr);
+    T invsumd = qdiv(T(1), sqrtdl + sqrtdr);
     T bxsig = (bxi > T(0) ? T(1) : T(-1));
     uldst.d = ulst.d;
     urdst.d = urst.d;
@@ -243,7 +255,7 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
                          ((urst.mz * urst_d_inv) - (ulst.mz * ulst_d_inv)));
     uldst.bz = urdst.bz = tmp;
 
-    tmp = spd[2] * bxi + (uldst.my * uldst.by + uldst.mz * uldst.bz) / uldst.d;
+    tmp = spd[2] * bxi + qdiv(uldst.my * uldst.by + uldst.mz * uldst.bz, uldst.d);
     uldst.e = ulst.e - sqrtdl * bxsig * (vbstl - tmp);
     urdst.e = urst.e + sqrtdr * bxsig * (vbstr - tmp);
   }
@@ -300,7 +312,7 @@ __device__ __forceinline__ void mhd_cons_flux(const T w[8], T gamma, T cons[8],
   for (int i = 4; i < 8; i++) cons[i] = w[i] * w[L_D];
   {
     T density = w[L_D];
-    T internal_edens = w[L_E] / (gamma - T(1));
+    T internal_edens = qdiv(w[L_E], gamma - T(1));
     const T vi = w[L_VI], vj = w[L_VJ], vk = w[L_VK];
     const T kinetic = T(0.5) * density * ((vi * vi) + ((vj * vj) + (vk * vk)));
     T magnetic = T(0.5) * ((w[L_BI] * w[L_BI]) +
@@ -334,9 +346,9 @@ __device__ __forceinline__ void hlle_mhd(const T wl[8], const T wr[8],
   T c_r = fast_ms_speed(wr[L_D], wr[L_E], wr[L_BI], wr[L_BJ], wr[L_BK], gamma);
   T left_speed = (wl[L_VI] - c_l);
   T right_speed = (wr[L_VI] + c_r);
-  T sqrtrho_l = sqrt(wl[L_D]);
-  T sqrtrho_r = sqrt(wr[L_D]);
-  T inv_sqrtrho_tot = T(1) / (sqrtrho_l + sqrtrho_r);
+  T sqrtrho_l = qsqrt(wl[L_D]);
+  T sqrtrho_r = qsqrt(wr[L_D]);
+  T inv_sqrtrho_tot = qdiv(T(1), sqrtrho_l + sqrtrho_r);
   T vi_roe = (sqrtrho_l * wl[L_VI] + sqrtrho_r * wr[L_VI]) * inv_sqrtrho_tot;
   T vj_roe = (sqrtrho_l * wl[L_VJ] + sqrtrho_r * wr[L_VJ]) * inv_sqrtrho_tot;
   T vk_roe = (sqrtrho_l * wl[L_VK] + sqrtrho_r * wr[L_VK]) * inv_sqrtrho_tot;
@@ -347,8 +359,8 @@ __device__ __forceinline__ void hlle_mhd(const T wl[8], const T wr[8],
                       ((wl[L_BJ] * wl[L_BJ]) + (wl[L_BK] * wl[L_BK])));
   ptot_r += T(0.5) * ((wr[L_BI] * wr[L_BI]) +
                       ((wr[L_BJ] * wr[L_BJ]) + (wr[L_BK] * wr[L_BK])));
-  T h_l = (cl[L_E] + ptot_l) / wl[L_D];
-  T h_r = (cr[L_E] + ptot_r) / wr[L_D];
+  T h_l = qdiv(cl[L_E] + ptot_l, wl[L_D]);
+  T h_r = qdiv(cr[L_E] + ptot_r, wr[L_D]);
   T h_roe = (sqrtrho_l * h_l + sqrtrho_r * h_r) * inv_sqrtrho_tot;
   // roe_cfast_
   T rho_roe = sqrtrho_l * sqrtrho_r;
@@ -361,20 +373,21 @@ __device__ __forceinline__ void hlle_mhd(const T wl[8], const T wr[8],
   T dbk = wl[L_BK] - wr[L_BK];
   T x_prime =
       ((dbj * dbj + dbk * dbk) * T(0.5) * (gamma_prime - T(1)) * inv_sqrtrho_tot);
-  T y_prime = ((gamma_prime - T(1)) * (wl[L_D] + wr[L_D]) * T(0.5) / rho_roe);
+  T y_prime = qdiv((gamma_prime - T(1)) * (wl[L_D] + wr[L_D]) * T(0.5), rho_roe);
   T tilde_a2 =
-      (gamma_prime * (h_roe - T(0.5) * v_roe2 - b_roe2 / rho_roe) - x_prime);
-  T tilde_vai2 = bi_roe * bi_roe / rho_roe;
-  T tilde_va2 = (tilde_vai2 + (gamma_prime - y_prime) *
-                                  (bj_roe * bj_roe + bk_roe * bk_roe) / rho_roe);
+      (gamma_prime * (h_roe - T(0.5) * v_roe2 - qdiv(b_roe2, rho_roe)) - x_prime);
+  T tilde_vai2 = qdiv(bi_roe * bi_roe, rho_roe);
+  T tilde_va2 = (tilde_vai2 + qdiv((gamma_prime - y_prime) *
+                                       (bj_roe * bj_roe + bk_roe * bk_roe),
+                                   rho_roe));
   T s = tilde_a2 + tilde_va2;
-  T c_roe = sqrt(T(0.5) * (tilde_a2 + tilde_va2 +
-                           sqrt(s * s - T(4) * tilde_a2 * tilde_vai2)));
+  T c_roe = qsqrt(T(0.5) * (tilde_a2 + tilde_va2 +
+                            qsqrt(s * s - T(4) * tilde_a2 * tilde_vai2)));
   T bp = fmax(vi_roe + c_roe, right_speed);
   T bm = fmin(vi_roe - c_roe, left_speed);
   bp = fmax(bp, T(0));
   bm = fmin(bm, T(0));
-  T inv_speed_diff = T(1) / (bp - bm);
+  T inv_speed_diff = qdiv(T(1), bp - bm);
 #pragma unroll
   for (int f = 0; f < 8; f++) {
     flux[f] = ((bp * fl[f] - bm * fr[f] + (cr[f] - cl[f]) * bp * bm) *
@@ -398,7 +411,7 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T gamma,
       c[s][H_D] = w[s][H_D];
       for (int i = 1; i < 5; i++) c[s][i] = w[s][i] * w[s][H_D];
       T density = w[s][H_D];
-      T internal_edens = w[s][H_E] / (gamma - T(1));
+      T internal_edens = qdiv(w[s][H_E], gamma - T(1));
       const T vi = w[s][H_VI], vj = w[s][H_VJ], vk = w[s][H_VK];
       const T kinetic =
           T(0.5) * density * ((vi * vi) + ((vj * vj) + (vk * vk)));
@@ -407,22 +420,22 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T gamma,
   }
   T cs_l, cs_r;
   {
-    T c_l = sqrt(gamma * pressure_l / wl[H_D]);
-    T c_r = sqrt(gamma * pressure_r / wr[H_D]);
+    T c_l = qsqrt(qdiv(gamma * pressure_l, wl[H_D]));
+    T c_r = qsqrt(qdiv(gamma * pressure_r, wr[H_D]));
     T left_speed = (wl[H_VI] - c_l);
     T right_speed = (wr[H_VI] + c_r);
-    T sqrtrho_l = sqrt(wl[H_D]);
-    T sqrtrho_r = sqrt(wr[H_D]);
-    T inv_sqrtrho_tot = T(1) / (sqrtrho_l + sqrtrho_r);
+    T sqrtrho_l = qsqrt(wl[H_D]);
+    T sqrtrho_r = qsqrt(wr[H_D]);
+    T inv_sqrtrho_tot = qdiv(T(1), sqrtrho_l + sqrtrho_r);
     T vi_roe = (sqrtrho_l * wl[H_VI] + sqrtrho_r * wr[H_VI]) * inv_sqrtrho_tot;
     T vj_roe = (sqrtrho_l * wl[H_VJ] + sqrtrho_r * wr[H_VJ]) * inv_sqrtrho_tot;
     T vk_roe = (sqrtrho_l * wl[H_VK] + sqrtrho_r * wr[H_VK]) * inv_sqrtrho_tot;
     T v_roe2 = vi_roe * vi_roe + vj_roe * vj_roe + vk_roe * vk_roe;
-    T h_l = (cons_l[H_E] + pressure_l) / wl[H_D];
-    T h_r = (cons_r[H_E] + pressure_r) / wr[H_D];
+    T h_l = qdiv(cons_l[H_E] + pressure_l, wl[H_D]);
+    T h_r = qdiv(cons_r[H_E] + pressure_r, wr[H_D]);
     T h_roe = (sqrtrho_l * h_l + sqrtrho_r * h_r) * inv_sqrtrho_tot;
     T temp = h_roe - T(0.5) * v_roe2;
-    T c_roe = sqrt((gamma - T(1)) * dmax(temp, T(0)));
+    T c_roe = qsqrt((gamma - T(1)) * dmax(temp, T(0)));
     cs_r = fmax(vi_roe + c_roe, right_speed);
     cs_l = fmin(vi_roe - c_roe, left_speed);
   }
@@ -432,18 +445,18 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T gamma,
   T tr = (pressure_r - (cs_r - wr[H_VI]) * wr[H_D] * wr[H_VI]);
   T dl = wl[H_D] * (cs_l - wl[H_VI]);
   T dr = -wr[H_D] * (cs_r - wr[H_VI]);
-  T q1 = T(1) / (dl + dr);
+  T q1 = qdiv(T(1), dl + dr);
   T cw = (tr - tl) * q1;
   T cp = (dl * tr + dr * tl) * q1;
   T sl, sr, sm;
   if (cw >= T(0)) {
-    sl = cw / (cw - bm);
+    sl = qdiv(cw, cw - bm);
     sr = T(0);
-    sm = -bm / (cw - bm);
+    sm = qdiv(-bm, cw - bm);
   } else {
     sl = T(0);
-    sr = -cw / (bp - cw);
-    sm = bp / (bp - cw);
+    sr = qdiv(-cw, bp - cw);
+    sm = qdiv(bp, bp - cw);
   }
   cp = dmax(cp, T(0));
   T momentumi_l = cons_l[H_VI];
