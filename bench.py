@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=128,
                     help="edge of the Orszag-Tang block timed on the CPU")
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the plain copy kernel (HBM counter scale)")
     return ap.parse_args()
 
 
@@ -64,6 +66,52 @@ def step(grid):
     grid.refresh()
     m.compute(grid.blocks, dt)
     return dt
+
+
+COPY_WORDS = 1 << 26  # 512 MiB read + 512 MiB written per copy launch
+
+
+def copy_bandwidth(reps=5):
+    """Achievable HBM copy rate: a plain one-word-per-thread copy kernel of
+    known size (also the scale for the FETCH_SIZE / WRITE_SIZE counters)."""
+    nbytes = COPY_WORDS * 8
+    src, dst = ev.DeviceBuffer(nbytes), ev.DeviceBuffer(nbytes)
+    ev.hip_check(ev.hip().hipMemset(src.ptr, 0, ctypes.c_size_t(nbytes)),
+                 "hipMemset")
+    L = ev.lib()
+    L.enzoe_calibration_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_size_t, ctypes.c_void_p]
+    ev.check(L.enzoe_calibration_copy(dst.ptr, src.ptr, COPY_WORDS, None),
+             "copy")
+    ev.synchronize()
+    ev.profiling(True)
+    for _ in range(reps):
+        ev.check(L.enzoe_calibration_copy(dst.ptr, src.ptr, COPY_WORDS, None),
+                 "copy")
+    ev.synchronize()
+    ms = ev.profile_report()["k_calibration_copy"][0]
+    ev.profiling(False)
+    src.free()
+    dst.free()
+    return {"kernel": "k_calibration_copy", "bytes_read": nbytes,
+            "bytes_written": nbytes, "avg_ms": ms,
+            "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    kernel's reading), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 4 and p[0] == "traffic" and p[1] == kernel \
+                    and p[2] == workload:
+                return float(p[3])
+    return None
 
 
 def cpu_baseline(n, nsteps):
@@ -102,6 +150,9 @@ def main():
     if ev.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if a.copy_calibration:
+        print(json.dumps(copy_bandwidth()))
+        return
     prec = ev.ENZOE_PREC_DOUBLE if a.precision == "fp64" else ev.ENZOE_PREC_MIXED
     method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
                            theta=2.0, courant=0.4, gamma=5.0 / 3.0,
@@ -121,6 +172,7 @@ def main():
     prof = ev.profile_report()
     ev.profiling(False)
 
+    copy = copy_bandwidth()
     zones = a.n ** 3
     value = zones * a.steps / el
     m = a.n + 6
@@ -133,6 +185,7 @@ def main():
     gpu_step_ms = sum(v[0] * v[1] for v in prof.values()) / a.steps
     step_gbs = STEP_BYTES_PER_ZONE_FP64 * zones / (gpu_step_ms * 1e-3) / 1e9
 
+    workload = "orszag_tang_%d3_%s_vlct_plm_hlld" % (a.n, a.precision)
     out = {
         "metric": "zone-updates/s/GPU (VL+CT MHD step)",
         "value": value,
@@ -146,8 +199,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == 0 else "f32-riemann/f64-update",
         "data": "synthetic (3D Orszag-Tang vortex, seeded 1e-3 v_z noise)",
-        "config": {"workload": "orszag_tang_%d3_fp64_vlct_plm_hlld" % a.n
-                   if prec == 0 else "orszag_tang_%d3_mixed_vlct_plm_hlld" % a.n,
+        "config": {"workload": workload,
                    "block": [a.n, a.n, a.n], "ghost": 3, "riemann": "hlld",
                    "reconstruct": "plm", "theta": 2.0, "courant": 0.4,
                    "precision": a.precision},
@@ -158,7 +210,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, workload),
             "bytes_per_launch": dom_bytes,
             "avg_launch_ms": dom_ms,
         },
@@ -169,6 +221,7 @@ def main():
             "frac": step_gbs / HBM_PEAK_GBS,
         },
         "kernels_ms": {k: round(v[0], 4) for k, v in sorted(prof.items())},
+        "copy_kernel_GBs": copy["GBs"],
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)

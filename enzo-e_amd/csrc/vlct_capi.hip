@@ -134,10 +134,8 @@ int enzoe_vlct_create(const enzoe_vlct_params *p, enzoe_vlct_t *out) {
   m->precision = p->precision;
   // CT requires exactly 2 partial timesteps (EnzoBfieldMethod.cpp:23-27)
   if (m->mhd && m->nstages != 2) { delete m; return ENZOE_ERR_CONFIG; }
-  if (hipMalloc((void **)&m->d_dt, sizeof(unsigned long long)) != hipSuccess) {
-    delete m;
-    return ENZOE_ERR_ALLOC;
-  }
+  // no device call here: device memory is taken on first use, so a
+  // configuration can be validated on a host without a GPU
   *out = m;
   return ENZOE_OK;
 }
@@ -186,6 +184,11 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   hipStream_t s = (hipStream_t)stream;
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
+  if (!m->d_dt &&
+      hipMalloc((void **)&m->d_dt, sizeof(unsigned long long)) != hipSuccess) {
+    m->d_dt = nullptr;
+    return ENZOE_ERR_ALLOC;
+  }
   hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
   // descriptors for the dt kernel: only U, pressure, shape and widths used
   std::vector<BlockDesc<double>> desc(n);

@@ -1,11 +1,21 @@
 // vlct_ops.hip -- single-operation entry points of the C ABI.  Each runs one
-// reference plugin call (a reconstructor, a Riemann solver, one CT step) on
-// device arrays in the reference's own face-trimmed layout, so the C++
-// drop-in classes can be used one call at a time exactly like the CPU ones.
+// reference plugin call (a reconstructor, a Riemann solver, one CT step, one
+// quantity-update step) on device views with arbitrary strides, so the C++
+// drop-in classes can pass the reference's own views (stale-trimmed,
+// face-trimmed with cell-centred strides, 4-D backing arrays) unchanged.
+//
+// Index convention of every kernel: a position p = (x, y, z) is taken in the
+// coordinates of each view (element (0,0,0) of the view is its corner); the
+// stale-depth regions are those of the reference loops, which all address
+// their views through `get(key, stale_depth)` / `subarray(stale_slc, ...)`
+// (src/Cello/view_ViewMap.hpp:310-327) -- an offset common to every view.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cctype>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "../../include/enzoe_vlct.h"
@@ -22,136 +32,440 @@ std::string lower(const char *s) {
   return r;
 }
 
+// View of one array, axes in (x, y, z) order.
+struct KArr {
+  double *p;
+  long s[3];
+  int n[3];
+};
+
+// Same-shape views of a map, indexed by slot (ENZOE_SLOT_*).
+struct KMap {
+  double *p[ENZOE_MAP_SLOTS];
+  long s[3];
+  int n[3];
+};
+
+// Iteration box [lo, lo + n) in (x, y, z).
+struct Box {
+  int lo[3];
+  int n[3];
+};
+
+__device__ __forceinline__ long off(const long s[3], const int p[3]) {
+  return p[2] * s[2] + p[1] * s[1] + p[0] * s[0];
+}
+
+__device__ __forceinline__ bool box_pos(const Box &b, int p[3]) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long nxy = (long)b.n[0] * b.n[1];
+  if (t >= nxy * b.n[2]) return false;
+  int z = (int)(t / nxy);
+  long r = t - (long)z * nxy;
+  int y = (int)(r / b.n[0]);
+  int x = (int)(r - (long)y * b.n[0]);
+  p[0] = x + b.lo[0];
+  p[1] = y + b.lo[1];
+  p[2] = z + b.lo[2];
+  return true;
+}
+
+// ------------------------------------------------------------- host helpers
+
+bool to_karr(const enzoe_farray *a, KArr &k) {
+  if (!a || !a->data) return false;
+  k.p = a->data;
+  for (int d = 0; d < 3; d++) {
+    k.n[d] = a->shape[2 - d];
+    k.s[d] = (long)a->stride[2 - d];
+    if (k.n[d] < 0) return false;
+  }
+  return true;
+}
+
+// required: bitmask of slots that must be present
+bool to_kmap(const enzoe_fmap *m, unsigned required, KMap &k) {
+  if (!m) return false;
+  for (int q = 0; q < ENZOE_MAP_SLOTS; q++) {
+    k.p[q] = m->data[q];
+    if ((required >> q & 1u) && !k.p[q]) return false;
+  }
+  for (int d = 0; d < 3; d++) {
+    k.n[d] = m->shape[2 - d];
+    k.s[d] = (long)m->stride[2 - d];
+    if (k.n[d] < 0) return false;
+  }
+  return true;
+}
+
+unsigned present(const KMap &k) {
+  unsigned r = 0;
+  for (int q = 0; q < ENZOE_MAP_SLOTS; q++)
+    if (k.p[q]) r |= 1u << q;
+  return r;
+}
+
+bool same_shape(const int a[3], const int b[3]) {
+  return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+}
+
+// true if `face` is `cell` with the extent along dim changed by `delta`
+bool shifted(const int face[3], const int cell[3], int dim, int delta) {
+  for (int d = 0; d < 3; d++)
+    if (face[d] != cell[d] + (d == dim ? delta : 0)) return false;
+  return true;
+}
+
+Box make_box(const int lo[3], const int hi[3], long &count) {
+  Box b;
+  count = 1;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = lo[d];
+    b.n[d] = std::max(0, hi[d] - lo[d]);
+    count *= b.n[d];
+  }
+  return b;
+}
+
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
-struct Ptr8 { double *p[8]; };
+int launched() {
+  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+}
 
-// Reconstruction on face-trimmed arrays (EnzoReconstructorNN.cpp:14-49,
-// EnzoReconstructorPLM.hpp:166-248): one thread per cell of the
-// stale-trimmed box.
+constexpr unsigned kMhdSlots = 0xFFu;
+constexpr unsigned kHydroSlots = 0x1Fu;
+constexpr unsigned kUpdateSlots = 0x1Fu;  // density, velocity, total_energy
+
+// ------------------------------------------------------------------ kernels
+
+// EnzoReconstructorNN::reconstruct_interface (EnzoReconstructorNN.cpp:14-49):
+// over cells p of [s, n-s) with p_dim < n_dim - s - 1:
+//   wl(p) = w(p), wr(p) = w(p + e_dim).
+// EnzoReconstructorPLM<L>::reconstruct_interface
+// (EnzoReconstructorPLM.hpp:166-248): over cells c with c_dim in
+// [s+1, n_dim-s-1): dv = L(w(c-1), w(c), w(c+1)); wr(c - e_dim) =
+// floor(w - dv/2), wl(c) = floor(w + dv/2), floors on density and pressure.
 template <int RECON>
-__global__ void k_op_reconstruct(Ptr8 w, Ptr8 wl, Ptr8 wr, int nkeys, int mz,
-                                 int my, int mx, int dim, int s, double theta,
+__global__ void k_op_reconstruct(KMap w, KMap wl, KMap wr, unsigned slots,
+                                 Box box, int dim, int s, double theta,
                                  double dfloor, double pfloor) {
-  const int di = dim == 0, dj = dim == 1, dk = dim == 2;
-  const int fy = my - dj, fx = mx - di;
-  int k, j, i;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!box_coords(t, s, mx - 2 * s, my - 2 * s, mz - 2 * s, k, j, i)) return;
-  const long cs = (dim == 0) ? 1 : (dim == 1 ? (long)mx : (long)mx * my);
-  const long c = idx3(k, j, i, my, mx);
-  const int p[3] = {i, j, k};
-  const int m[3] = {mx, my, mz};
-  for (int q = 0; q < nkeys; q++) {
-    const double *W = w.p[q];
-    if (RECON == RC_NN) {
-      if (p[dim] >= m[dim] - s - 1) continue;
-      long f = idx3(k, j, i, fy, fx);
-      wl.p[q][f] = W[c];
-      wr.p[q][f] = W[c + cs];
-    } else {
-      if (p[dim] < s + 1 || p[dim] >= m[dim] - s - 1) continue;
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long c = off(w.s, p);
+  const long sd = w.s[dim];
+  if (RECON == RC_NN) {
+    if (p[dim] >= w.n[dim] - s - 1) return;
+    const long fl = off(wl.s, p), fr = off(wr.s, p);
+    for (int q = 0; q < ENZOE_MAP_SLOTS; q++) {
+      if (!(slots >> q & 1u)) continue;
+      wl.p[q][fl] = w.p[q][c];
+      wr.p[q][fr] = w.p[q][c + sd];
+    }
+  } else {
+    if (p[dim] < s + 1 || p[dim] >= w.n[dim] - s - 1) return;
+    const long fl = off(wl.s, p), fr = off(wr.s, p) - wr.s[dim];
+    for (int q = 0; q < ENZOE_MAP_SLOTS; q++) {
+      if (!(slots >> q & 1u)) continue;
+      const double *W = w.p[q];
       double val = W[c];
-      double dv = (RECON == RC_PLM) ? limiter_enzo(W[c - cs], val, W[c + cs], theta)
-                                    : limiter_athena(W[c - cs], val, W[c + cs]);
+      double dv = (RECON == RC_PLM) ? limiter_enzo(W[c - sd], val, W[c + sd], theta)
+                                    : limiter_athena(W[c - sd], val, W[c + sd]);
       double half_dv = dv * 0.5;
       double rv = val - half_dv, lv = val + half_dv;
-      if (q == S_RHO) { rv = dmax(rv, dfloor); lv = dmax(lv, dfloor); }
-      if (q == S_E) { rv = dmax(rv, pfloor); lv = dmax(lv, pfloor); }
-      wr.p[q][idx3(k - dk, j - dj, i - di, fy, fx)] = rv;
-      wl.p[q][idx3(k, j, i, fy, fx)] = lv;
+      if (q == ENZOE_SLOT_DENSITY) { rv = dmax(rv, dfloor); lv = dmax(lv, dfloor); }
+      if (q == ENZOE_SLOT_ENERGY) { rv = dmax(rv, pfloor); lv = dmax(lv, pfloor); }
+      wr.p[q][fr] = rv;
+      wl.p[q][fl] = lv;
     }
   }
 }
 
-// Riemann solve on face-trimmed arrays (EnzoRiemannImpl.hpp:317-338).
+// EnzoRiemannImpl<K>::solve_ (EnzoRiemannImpl.hpp:317-338): one face per
+// thread over [s, n-s) of the flux view; states permuted into the i/j/k
+// order of the kernels (EnzoRiemannLUT, KernelConfig).
 template <int RIEMANN, bool MHD>
-__global__ void k_op_riemann(Ptr8 pl, Ptr8 pr, Ptr8 fx8, int mz, int my,
-                             int mx, int dim, int s, double gamma) {
-  const int fz = mz - (dim == 2), fy = my - (dim == 1), fx = mx - (dim == 0);
-  int k, j, i;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!box_coords(t, 0, fx, fy, fz, k, j, i)) return;
-  if (k < s || j < s || i < s || k >= fz - s || j >= fy - s || i >= fx - s)
-    return;
-  const long f = idx3(k, j, i, fy, fx);
+__global__ void k_op_riemann(KMap pl, KMap pr, KMap fx8, Box box, int dim,
+                             double gamma) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long a = off(pl.s, p), b = off(pr.s, p), f = off(fx8.s, p);
   const int dj = (dim + 1) % 3, dk = (dim + 2) % 3;
   if (MHD) {
+    const int src[8] = {S_RHO, S_BX + dim, S_BX + dj, S_BX + dk,
+                        S_VX + dim, S_VX + dj, S_VX + dk, S_E};
     double l[8], r[8], o[8];
-    l[L_D] = pl.p[S_RHO][f]; r[L_D] = pr.p[S_RHO][f];
-    l[L_BI] = pl.p[S_BX + dim][f]; r[L_BI] = pr.p[S_BX + dim][f];
-    l[L_BJ] = pl.p[S_BX + dj][f]; r[L_BJ] = pr.p[S_BX + dj][f];
-    l[L_BK] = pl.p[S_BX + dk][f]; r[L_BK] = pr.p[S_BX + dk][f];
-    l[L_VI] = pl.p[S_VX + dim][f]; r[L_VI] = pr.p[S_VX + dim][f];
-    l[L_VJ] = pl.p[S_VX + dj][f]; r[L_VJ] = pr.p[S_VX + dj][f];
-    l[L_VK] = pl.p[S_VX + dk][f]; r[L_VK] = pr.p[S_VX + dk][f];
-    l[L_E] = pl.p[S_E][f]; r[L_E] = pr.p[S_E][f];
+    for (int q = 0; q < 8; q++) { l[q] = pl.p[src[q]][a]; r[q] = pr.p[src[q]][b]; }
     if (RIEMANN == RS_HLLD) hlld(l, r, gamma, o);
     else hlle_mhd(l, r, gamma, o);
-    fx8.p[S_RHO][f] = o[L_D];
-    fx8.p[S_BX + dim][f] = o[L_BI];
-    fx8.p[S_BX + dj][f] = o[L_BJ];
-    fx8.p[S_BX + dk][f] = o[L_BK];
-    fx8.p[S_VX + dim][f] = o[L_VI];
-    fx8.p[S_VX + dj][f] = o[L_VJ];
-    fx8.p[S_VX + dk][f] = o[L_VK];
-    fx8.p[S_E][f] = o[L_E];
+    for (int q = 0; q < 8; q++) fx8.p[src[q]][f] = o[q];
   } else {
+    const int src[5] = {S_RHO, S_VX + dim, S_VX + dj, S_VX + dk, S_E};
     double l[5], r[5], o[5];
-    l[H_D] = pl.p[S_RHO][f]; r[H_D] = pr.p[S_RHO][f];
-    l[H_VI] = pl.p[S_VX + dim][f]; r[H_VI] = pr.p[S_VX + dim][f];
-    l[H_VJ] = pl.p[S_VX + dj][f]; r[H_VJ] = pr.p[S_VX + dj][f];
-    l[H_VK] = pl.p[S_VX + dk][f]; r[H_VK] = pr.p[S_VX + dk][f];
-    l[H_E] = pl.p[S_E][f]; r[H_E] = pr.p[S_E][f];
+    for (int q = 0; q < 5; q++) { l[q] = pl.p[src[q]][a]; r[q] = pr.p[src[q]][b]; }
     hllc(l, r, gamma, o);
-    fx8.p[S_RHO][f] = o[H_D];
-    fx8.p[S_VX + dim][f] = o[H_VI];
-    fx8.p[S_VX + dj][f] = o[H_VJ];
-    fx8.p[S_VX + dk][f] = o[H_VK];
-    fx8.p[S_E][f] = o[H_E];
+    for (int q = 0; q < 5; q++) fx8.p[src[q]][f] = o[q];
   }
 }
 
-struct Ptr3 { double *p[3]; };
-
-// EnzoBfieldMethodCT::update_bfield on the reference's edge-array layout.
-__global__ void k_op_update_bfield(Ptr3 edge, const double *bcur, double *bout,
-                                   double dtdj, double dtdk, int mz, int my,
-                                   int mx, int ax, int s) {
-  const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
-  const int m[3] = {mx, my, mz};
-  int nb[3] = {mx, my, mz};
-  nb[ax] += 1;
-  int k, j, i;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!box_coords(t, 0, nb[0], nb[1], nb[2], k, j, i)) return;
-  int p[3] = {i, j, k};
-  if (p[ax] < s + 1 || p[ax] >= m[ax] - s) return;
-  if (p[aj] < s + 1 || p[aj] >= m[aj] - s - 1) return;
-  if (p[ak] < s + 1 || p[ak] >= m[ak] - s - 1) return;
-  int nEj[3] = {mx, my, mz}; nEj[ak] -= 1; nEj[ax] -= 1;
-  int nEk[3] = {mx, my, mz}; nEk[ax] -= 1; nEk[aj] -= 1;
-  int pe[3] = {p[0], p[1], p[2]}; pe[ax] -= 1;
-  int pjm[3] = {pe[0], pe[1], pe[2]}; pjm[aj] -= 1;
-  int pkm[3] = {pe[0], pe[1], pe[2]}; pkm[ak] -= 1;
-  auto at = [](const int q[3], const int n[3]) {
-    return ((long)q[2] * n[1] + q[1]) * (long)n[0] + q[0];
-  };
-  double E_k_term = dtdj * (edge.p[ak][at(pe, nEk)] - edge.p[ak][at(pjm, nEk)]);
-  double E_j_term = dtdk * (edge.p[aj][at(pe, nEj)] - edge.p[aj][at(pkm, nEj)]);
-  long cb = at(p, nb);
-  bout[cb] = bcur[cb] - E_k_term + E_j_term;
+// EnzoBfieldMethodCT::correct_reconstructed_bfield
+// (EnzoBfieldMethodCT.cpp:122-166): face f of the trimmed states takes
+// bfieldi(f + e_dim), over [s, n-s) of the interior-face view.
+__global__ void k_op_correct_bfield(KArr bfi, KMap wl, KMap wr, Box box,
+                                    int dim) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  double v = bfi.p[off(bfi.s, p) + bfi.s[dim]];
+  wl.p[S_BX + dim][off(wl.s, p)] = v;
+  wr.p[S_BX + dim][off(wr.s, p)] = v;
 }
 
-__global__ void k_op_center_bfield(double *bc, const double *bi, int mz,
-                                   int my, int mx, int dim, int s) {
-  int k, j, i;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!box_coords(t, s, mx - 2 * s, my - 2 * s, mz - 2 * s, k, j, i)) return;
-  const int bny = my + (dim == 1), bnx = mx + (dim == 0);
-  double l = bi[idx3(k, j, i, bny, bnx)];
-  double r = bi[idx3(k + (dim == 2), j + (dim == 1), i + (dim == 0), bny, bnx)];
-  bc[idx3(k, j, i, my, mx)] = 0.5 * (l + r);
+// EnzoBfieldMethodCT::identify_upwind (EnzoBfieldMethodCT.cpp:170-216)
+__global__ void k_op_upwind(KMap flux, KArr w, Box box) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  w.p[off(w.s, p)] = upwind_weight(flux.p[S_RHO][off(flux.s, p)]);
+}
+
+// EnzoBfieldMethodCT::compute_center_efield (EnzoBfieldMethodCT.cpp:267-292)
+__global__ void k_op_center_efield(KArr E, KMap U, Box box, int dim) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const int aj = (dim + 1) % 3, ak = (dim + 2) % 3;
+  const long c = off(U.s, p);
+  E.p[off(E.s, p)] = (-U.p[S_VX + aj][c] * U.p[S_BX + ak][c] +
+                      U.p[S_VX + ak][c] * U.p[S_BX + aj][c]);
+}
+
+// EnzoBfieldMethodCT::compute_edge_efield + compute_edge_<true>
+// (EnzoBfieldMethodCT.cpp:384-557): edge EMF along ax at (cell p_ax, faces
+// p_j + 1/2, p_k + 1/2) over p_ax in [s+1, n_ax-s-1), p_j, p_k in
+// [s, n-s-1) of the centre-E view.
+struct W3 {
+  KArr w[3];
+};
+
+__global__ void k_op_edge_efield(KArr Ec, KArr edge, KMap jflux, KMap kflux,
+                                 W3 W, Box box, int ax) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
+  const double *Fj = jflux.p[S_BX + ak];  // -E_ax on j faces
+  const double *Fk = kflux.p[S_BX + aj];  // +E_ax on k faces
+  const long ej = off(jflux.s, p), ek = off(kflux.s, p), ec = off(Ec.s, p);
+  const long wj = off(W.w[aj].s, p), wk = off(W.w[ak].s, p);
+  const double *Wj = W.w[aj].p, *Wk = W.w[ak].p;
+  const double vEj = Fj[ej], vEj_kp1 = Fj[ej + jflux.s[ak]];
+  const double vEk = Fk[ek], vEk_jp1 = Fk[ek + kflux.s[aj]];
+  const double vEc = Ec.p[ec], vEc_jp1 = Ec.p[ec + Ec.s[aj]],
+               vEc_kp1 = Ec.p[ec + Ec.s[ak]],
+               vEc_jkp1 = Ec.p[ec + Ec.s[aj] + Ec.s[ak]];
+  const double vWj = Wj[wj], vWj_kp1 = Wj[wj + W.w[aj].s[ak]];
+  const double vWk = Wk[wk], vWk_jp1 = Wk[wk + W.w[ak].s[aj]];
+  double dEdj_r = vWk_jp1 * (vEc_jp1 + vEj) + (1 - vWk_jp1) * (vEc_jkp1 + vEj_kp1);
+  double dEdj_l = vWk * (-vEj - vEc) + (1 - vWk) * (-vEj_kp1 - vEc_kp1);
+  double dEdk_r = vWj_kp1 * (vEc_kp1 - vEk) + (1 - vWj_kp1) * (vEc_jkp1 - vEk_jp1);
+  double dEdk_l = vWj * (vEk - vEc) + (1 - vWj) * (vEk_jp1 - vEc_jp1);
+  double Ej_sum = vEj + vEj_kp1;
+  Ej_sum *= -1;
+  double Ek_sum = vEk + vEk_jp1;
+  edge.p[off(edge.s, p)] =
+      0.25 * (Ej_sum + Ek_sum + (dEdj_l - dEdj_r) + (dEdk_l - dEdk_r));
+}
+
+// EnzoBfieldMethodCT::update_bfield (EnzoBfieldMethodCT.cpp:617-687): face
+// X = p_ax in [s+1, nb_ax-s-1), p_j, p_k in [s+1, nb-s-1) of the face view;
+// the edges are taken at ax-position X-1.
+__global__ void k_op_update_bfield(KArr Ej, KArr Ek, KArr bcur, KArr bout,
+                                   Box box, int ax, double dtdj, double dtdk) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
+  int pe[3] = {p[0], p[1], p[2]};
+  pe[ax] -= 1;
+  const long k0 = off(Ek.s, pe), j0 = off(Ej.s, pe);
+  double E_k_term = dtdj * (Ek.p[k0] - Ek.p[k0 - Ek.s[aj]]);
+  double E_j_term = dtdk * (Ej.p[j0] - Ej.p[j0 - Ej.s[ak]]);
+  bout.p[off(bout.s, p)] = bcur.p[off(bcur.s, p)] - E_k_term + E_j_term;
+}
+
+// EnzoBfieldMethodCT::compute_center_bfield (EnzoBfieldMethodCT.cpp:702-728)
+__global__ void k_op_center_bfield(KArr bc, KArr bi, Box box, int dim) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long f = off(bi.s, p);
+  bc.p[off(bc.s, p)] = 0.5 * (bi.p[f] + bi.p[f + bi.s[dim]]);
+}
+
+// EnzoIntegrationQuanUpdate::clear_dUcons_map (EnzoIntegrationQuanUpdate.cpp:
+// 83-101), a kernel rather than a memset so it may sit in a captured graph.
+__global__ void k_op_fill(KMap m, unsigned slots, Box box, double v) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long c = off(m.s, p);
+  for (int q = 0; q < ENZOE_MAP_SLOTS; q++)
+    if (slots >> q & 1u) m.p[q][c] = v;
+}
+
+// EnzoIntegrationQuanUpdate::accumulate_flux_component
+// (EnzoIntegrationQuanUpdate.cpp:105-147): cells with p_dim in
+// [s+1, n_dim-s-1), [s, n-s) transverse: dU(c) -= dtdx (F(c) - F(c - e_dim)).
+__global__ void k_op_accumulate(KMap F, KMap dU, unsigned slots, Box box,
+                                int dim, double dtdx) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long c = off(dU.s, p), fr = off(F.s, p), fl = fr - F.s[dim];
+  for (int q = 0; q < ENZOE_MAP_SLOTS; q++)
+    if (slots >> q & 1u) dU.p[q][c] -= dtdx * (F.p[q][fr] - F.p[q][fl]);
+}
+
+// EnzoIntegrationQuanUpdate::update_quantities (:183-238) over [s+1, n-s-1)
+// and EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync
+// (EnzoPhysicsFluidProps.cpp:162-290, dual energy off) at stale s+1, which
+// uses the cell-centred B already written into the output map.
+__global__ void k_op_update(KMap U0, KMap dU, KMap Uo, Box box, int mhd,
+                            double gamma, double dfloor, double pfloor) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long a = off(U0.s, p), d = off(dU.s, p), o = off(Uo.s, p);
+  double old_rho = U0.p[S_RHO][a];
+  double new_rho = dmax(U0.p[S_RHO][a] + dU.p[S_RHO][d], dfloor);
+  double inv_new_rho = 1. / new_rho;
+  double v[4];
+  for (int q = 1; q < 5; q++)
+    v[q - 1] = (U0.p[q][a] * old_rho + dU.p[q][d]) * inv_new_rho;
+  Uo.p[S_RHO][o] = new_rho;
+  for (int q = 1; q < 5; q++) Uo.p[q][o] = v[q - 1];
+  double inv_rho = 1. / new_rho;
+  double eint_floor = pfloor * (1. / (gamma - 1.)) * inv_rho;
+  double non_thermal_e = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (mhd) {
+    double bx = Uo.p[S_BX][o], by = Uo.p[S_BY][o], bz = Uo.p[S_BZ][o];
+    non_thermal_e += (0.5 * (bx * bx + by * by + bz * bz) * inv_rho);
+  }
+  double etot_floor = eint_floor + non_thermal_e;
+  Uo.p[S_E][o] = dmax(v[3], etot_floor);
+}
+
+// p = (gamma-1) (rho (e_tot - v^2/2) - B^2/2)
+// (EnzoComputePressure::compute_pressure, EnzoComputePressure.cpp:177-189)
+__device__ __forceinline__ double pressure_at(const KMap &U, long c, int mhd,
+                                              double gm1) {
+  double vx = U.p[S_VX][c], vy = U.p[S_VY][c], vz = U.p[S_VZ][c];
+  double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+  double me_den = 0.;
+  if (mhd) {
+    double bx = U.p[S_BX][c], by = U.p[S_BY][c], bz = U.p[S_BZ][c];
+    me_den = 0.5 * (bx * bx + by * by + bz * bz);
+  }
+  return gm1 * (U.p[S_RHO][c] * (U.p[S_E][c] - ke) - me_den);
+}
+
+// EnzoPhysicsFluidProps::primitive_from_integration
+// (EnzoPhysicsFluidProps.cpp:64-138) over [s, n-s): copies density, velocity
+// and B, pressure into the pressure slot.
+__global__ void k_op_primitive(KMap U, KMap W, Box box, int mhd, double gm1) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long c = off(U.s, p), w = off(W.s, p);
+  const int nq = mhd ? 8 : 5;
+  for (int q = 0; q < nq; q++)
+    if (q != S_E) W.p[q][w] = U.p[q][c];
+  W.p[S_E][w] = pressure_at(U, c, mhd, gm1);
+}
+
+// EnzoPhysicsFluidProps::pressure_from_integration (:142-156)
+__global__ void k_op_pressure(KMap U, KArr P, Box box, int mhd, double gm1) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  P.p[off(P.s, p)] = pressure_at(U, off(U.s, p), mhd, gm1);
+}
+
+
+// EnzoMHDIntegratorStageCommands::timestep
+// (EnzoMHDIntegratorStageCommands.cpp:299-366): min over every cell of the
+// views of dx_i/(|v_i| + c), c = fast magnetosonic (MHD) or sound speed from
+// the given pressure; reduced as the bit pattern of a non-negative double
+// (integer atomicMin == exact double min; a NaN never wins, as std::min).
+template <bool MHD>
+__global__ void __launch_bounds__(256)
+k_op_dt(KMap U, KArr P, double gamma, double dx, double dy, double dz,
+        unsigned long long *result) {
+  const long nxy = (long)U.n[0] * U.n[1];
+  const long n = nxy * U.n[2];
+  double local = 1.7976931348623157e308;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    int p[3];
+    p[2] = (int)(t / nxy);
+    long r = t - (long)p[2] * nxy;
+    p[1] = (int)(r / U.n[0]);
+    p[0] = (int)(r - (long)p[1] * U.n[0]);
+    const long c = off(U.s, p);
+    double d = U.p[S_RHO][c], pr = P.p[off(P.s, p)];
+    double sig;
+    if (MHD) {
+      double bx = U.p[S_BX][c], by = U.p[S_BY][c], bz = U.p[S_BZ][c];
+      double B2 = ((bx * bx) + ((by * by) + (bz * bz)));
+      double cs2 = gamma * pr / d;
+      double va2 = B2 / d;
+      sig = sqrt(va2 + cs2);
+    } else {
+      sig = sqrt(gamma * pr / d);
+    }
+    double l = dmin3(dx / (fabs(U.p[S_VX][c]) + sig),
+                     dy / (fabs(U.p[S_VY][c]) + sig),
+                     dz / (fabs(U.p[S_VZ][c]) + sig));
+    local = dmin(local, l);
+  }
+  unsigned long long bits = __double_as_longlong(local);
+  if (!(local >= 0.0)) bits = 0x7FF8000000000000ull;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long v = __shfl_xor(bits, o, 64);
+    bits = (v < bits) ? v : bits;
+  }
+  __shared__ unsigned long long wmin[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wmin[wave] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = (wmin[w] < m) ? wmin[w] : m;
+    atomicMin(result, m);
+  }
+}
+
+// one 8-byte reduction target per device, taken on first use
+unsigned long long *dt_target() {
+  static std::mutex mtx;
+  static std::map<int, unsigned long long *> targets;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(mtx);
+  auto it = targets.find(dev);
+  if (it != targets.end()) return it->second;
+  unsigned long long *p = nullptr;
+  if (hipMalloc((void **)&p, sizeof(unsigned long long)) != hipSuccess)
+    return nullptr;
+  targets[dev] = p;
+  return p;
+}
+
+int recon_kind(const std::string &n) {
+  if (n == "nn") return RC_NN;
+  if (n == "plm" || n == "plm_enzo") return RC_PLM;
+  if (n == "plm_athena") return RC_PLM_ATHENA;
+  return -1;
+}
+
+bool valid_dim(int dim) { return dim >= 0 && dim <= 2; }
+
+void stale_box(const int n[3], int s, int lo[3], int hi[3]) {
+  for (int d = 0; d < 3; d++) {
+    lo[d] = s;
+    hi[d] = n[d] - s;
+  }
 }
 
 }  // namespace
@@ -160,115 +474,374 @@ extern "C" {
 
 int enzoe_reconstructor_staling(const char *name, int *total_rate,
                                 int *immediate_rate) {
-  std::string n = lower(name);
-  if (!total_rate || !immediate_rate) return ENZOE_ERR_ARG;
-  if (n == "nn") { *total_rate = 1; *immediate_rate = 0; return ENZOE_OK; }
-  if (n == "plm" || n == "plm_enzo" || n == "plm_athena") {
-    *total_rate = 2; *immediate_rate = 1; return ENZOE_OK;
-  }
-  return ENZOE_ERR_CONFIG;
+  int kind = recon_kind(lower(name));
+  if (kind < 0) return ENZOE_ERR_CONFIG;
+  // EnzoReconstructorNN.hpp:31-35, EnzoReconstructorPLM.hpp:153-157
+  if (total_rate) *total_rate = (kind == RC_NN) ? 1 : 2;
+  if (immediate_rate) *immediate_rate = (kind == RC_NN) ? 0 : 1;
+  return ENZOE_OK;
 }
 
 int enzoe_reconstruct_interface(const char *name, double theta_limiter,
                                 double density_floor, double pressure_floor,
-                                double *const prim[8], double *priml[8],
-                                double *primr[8], int nkeys, int mz, int my,
-                                int mx, int dim, int stale_depth,
-                                enzoe_stream_t stream) {
-  std::string n = lower(name);
-  if (!(theta_limiter >= 1.0 && theta_limiter <= 2.0)) return ENZOE_ERR_CONFIG;
-  if (!prim || !priml || !primr || (nkeys != 5 && nkeys != 8) || dim < 0 ||
-      dim > 2 || stale_depth < 0 || 2 * stale_depth + 2 >= std::min(mx, std::min(my, mz)))
+                                const enzoe_fmap *prim,
+                                const enzoe_fmap *priml,
+                                const enzoe_fmap *primr, int dim,
+                                int stale_depth, enzoe_stream_t stream) {
+  int kind = recon_kind(lower(name));
+  if (kind < 0) return ENZOE_ERR_CONFIG;
+  KMap w, wl, wr;
+  if (!valid_dim(dim) || stale_depth < 0 || !to_kmap(prim, 0, w))
     return ENZOE_ERR_ARG;
-  Ptr8 w{}, l{}, r{};
-  for (int q = 0; q < nkeys; q++) {
-    if (!prim[q] || !priml[q] || !primr[q]) return ENZOE_ERR_ARG;
-    w.p[q] = prim[q]; l.p[q] = priml[q]; r.p[q] = primr[q];
-  }
-  hipStream_t s = (hipStream_t)stream;
-  long nt = (long)(mx - 2 * stale_depth) * (my - 2 * stale_depth) * (mz - 2 * stale_depth);
-  if (n == "nn") {
-    hipLaunchKernelGGL(k_op_reconstruct<RC_NN>, dim3(nblk(nt)), dim3(256), 0, s, w, l, r,
-                       nkeys, mz, my, mx, dim, stale_depth, theta_limiter,
-                       density_floor, pressure_floor);
-  } else if (n == "plm" || n == "plm_enzo") {
-    hipLaunchKernelGGL(k_op_reconstruct<RC_PLM>, dim3(nblk(nt)), dim3(256), 0, s, w, l, r,
-                       nkeys, mz, my, mx, dim, stale_depth, theta_limiter,
-                       density_floor, pressure_floor);
-  } else if (n == "plm_athena") {
-    hipLaunchKernelGGL(k_op_reconstruct<RC_PLM_ATHENA>, dim3(nblk(nt)), dim3(256), 0, s, w,
-                       l, r, nkeys, mz, my, mx, dim, stale_depth, theta_limiter,
-                       density_floor, pressure_floor);
-  } else {
-    return ENZOE_ERR_CONFIG;
-  }
-  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+  const unsigned slots = present(w);
+  if (!slots || !to_kmap(priml, slots, wl) || !to_kmap(primr, slots, wr))
+    return ENZOE_ERR_ARG;
+  if (!shifted(wl.n, w.n, dim, -1) || !same_shape(wl.n, wr.n))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(w.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (kind == RC_NN)
+    hipLaunchKernelGGL(k_op_reconstruct<RC_NN>, dim3(nblk(count)), dim3(256), 0,
+                       st, w, wl, wr, slots, box, dim, stale_depth,
+                       theta_limiter, density_floor, pressure_floor);
+  else if (kind == RC_PLM)
+    hipLaunchKernelGGL(k_op_reconstruct<RC_PLM>, dim3(nblk(count)), dim3(256),
+                       0, st, w, wl, wr, slots, box, dim, stale_depth,
+                       theta_limiter, density_floor, pressure_floor);
+  else
+    hipLaunchKernelGGL(k_op_reconstruct<RC_PLM_ATHENA>, dim3(nblk(count)),
+                       dim3(256), 0, st, w, wl, wr, slots, box, dim,
+                       stale_depth, theta_limiter, density_floor,
+                       pressure_floor);
+  return launched();
 }
 
 int enzoe_riemann_solve(const char *solver, int mhd, double gamma,
-                        double *const priml[8], double *const primr[8],
-                        double *flux[8], int mz, int my, int mx, int dim,
-                        int stale_depth, enzoe_stream_t stream) {
-  std::string n = lower(solver);
-  if (!priml || !primr || !flux || dim < 0 || dim > 2 || stale_depth < 0)
+                        const enzoe_fmap *priml, const enzoe_fmap *primr,
+                        const enzoe_fmap *flux, int dim, int stale_depth,
+                        enzoe_stream_t stream) {
+  std::string s = lower(solver);
+  int kind;
+  // EnzoRiemann::construct_riemann (riemann/EnzoRiemann.cpp:26-68)
+  if (s == "hlld" && mhd) kind = RS_HLLD;
+  else if (s == "hlle" && mhd) kind = RS_HLLE;
+  else if (s == "hllc" && !mhd) kind = RS_HLLC;
+  else return ENZOE_ERR_CONFIG;
+  const unsigned need = mhd ? kMhdSlots : kHydroSlots;
+  KMap l, r, f;
+  if (!valid_dim(dim) || stale_depth < 0 || !to_kmap(priml, need, l) ||
+      !to_kmap(primr, need, r) || !to_kmap(flux, need, f))
     return ENZOE_ERR_ARG;
-  const int nk = mhd ? 8 : 5;
-  Ptr8 l{}, r{}, f{};
-  for (int q = 0; q < nk; q++) {
-    if (!priml[q] || !primr[q] || !flux[q]) return ENZOE_ERR_ARG;
-    l.p[q] = priml[q]; r.p[q] = primr[q]; f.p[q] = flux[q];
+  if (!same_shape(l.n, r.n) || !same_shape(l.n, f.n)) return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(f.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (kind == RS_HLLD)
+    hipLaunchKernelGGL((k_op_riemann<RS_HLLD, true>), dim3(nblk(count)),
+                       dim3(256), 0, st, l, r, f, box, dim, gamma);
+  else if (kind == RS_HLLE)
+    hipLaunchKernelGGL((k_op_riemann<RS_HLLE, true>), dim3(nblk(count)),
+                       dim3(256), 0, st, l, r, f, box, dim, gamma);
+  else
+    hipLaunchKernelGGL((k_op_riemann<RS_HLLC, false>), dim3(nblk(count)),
+                       dim3(256), 0, st, l, r, f, box, dim, gamma);
+  return launched();
+}
+
+int enzoe_ct_correct_reconstructed_bfield(int dim, const enzoe_farray *bfieldi,
+                                          const enzoe_fmap *priml,
+                                          const enzoe_fmap *primr,
+                                          int stale_depth,
+                                          enzoe_stream_t stream) {
+  if (!valid_dim(dim) || stale_depth < 0) return ENZOE_ERR_ARG;
+  KArr b;
+  KMap l, r;
+  const unsigned need = 1u << (ENZOE_SLOT_BFIELD_X + dim);
+  if (!to_karr(bfieldi, b) || !to_kmap(priml, need, l) ||
+      !to_kmap(primr, need, r))
+    return ENZOE_ERR_ARG;
+  // interior faces: CSlice(1,-1) of bfieldi along dim
+  if (!shifted(b.n, l.n, dim, 2) || !same_shape(l.n, r.n)) return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(l.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_correct_bfield, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, b, l, r, box, dim);
+  return launched();
+}
+
+int enzoe_ct_identify_upwind(int dim, const enzoe_fmap *flux,
+                             const enzoe_farray *weight, int stale_depth,
+                             enzoe_stream_t stream) {
+  KMap f;
+  KArr w;
+  if (!valid_dim(dim) || stale_depth < 0 ||
+      !to_kmap(flux, 1u << ENZOE_SLOT_DENSITY, f) || !to_karr(weight, w) ||
+      !same_shape(f.n, w.n))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(f.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_upwind, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, f, w, box);
+  return launched();
+}
+
+int enzoe_ct_compute_center_efield(int dim, const enzoe_farray *efield,
+                                   const enzoe_fmap *integration,
+                                   int stale_depth, enzoe_stream_t stream) {
+  KArr e;
+  KMap u;
+  if (!valid_dim(dim) || stale_depth < 0 || !to_karr(efield, e) ||
+      !to_kmap(integration, kMhdSlots, u) || !same_shape(e.n, u.n))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(e.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_center_efield, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, e, u, box, dim);
+  return launched();
+}
+
+int enzoe_ct_compute_edge_efield(int dim, const enzoe_farray *center_efield,
+                                 const enzoe_farray *edge_efield,
+                                 const enzoe_fmap *jflux,
+                                 const enzoe_fmap *kflux,
+                                 const enzoe_farray weight[3],
+                                 int stale_depth, enzoe_stream_t stream) {
+  if (!valid_dim(dim) || stale_depth < 0 || !weight) return ENZOE_ERR_ARG;
+  const int aj = (dim + 1) % 3, ak = (dim + 2) % 3;
+  KArr ec, ed;
+  KMap fj, fk;
+  W3 w;
+  if (!to_karr(center_efield, ec) || !to_karr(edge_efield, ed) ||
+      !to_kmap(jflux, 1u << (ENZOE_SLOT_BFIELD_X + ak), fj) ||
+      !to_kmap(kflux, 1u << (ENZOE_SLOT_BFIELD_X + aj), fk))
+    return ENZOE_ERR_ARG;
+  for (int d = 0; d < 3; d++)
+    if (!to_karr(&weight[d], w.w[d]) || !shifted(w.w[d].n, ec.n, d, -1))
+      return ENZOE_ERR_ARG;
+  // edge array: extents minus one along j and k (EnzoBfieldMethodCT.cpp:70-72)
+  for (int d = 0; d < 3; d++)
+    if (ed.n[d] != ec.n[d] - ((d == aj || d == ak) ? 1 : 0)) return ENZOE_ERR_ARG;
+  if (!shifted(fj.n, ec.n, aj, -1) || !shifted(fk.n, ec.n, ak, -1))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = stale_depth + (d == dim ? 1 : 0);
+    hi[d] = ec.n[d] - stale_depth - 1;
   }
-  hipStream_t s = (hipStream_t)stream;
-  long nt = (long)(mx - (dim == 0)) * (my - (dim == 1)) * (mz - (dim == 2));
-  if (n == "hlld" && mhd) {
-    hipLaunchKernelGGL((k_op_riemann<RS_HLLD, true>), dim3(nblk(nt)), dim3(256), 0, s, l,
-                       r, f, mz, my, mx, dim, stale_depth, gamma);
-  } else if (n == "hlle" && mhd) {
-    hipLaunchKernelGGL((k_op_riemann<RS_HLLE, true>), dim3(nblk(nt)), dim3(256), 0, s, l,
-                       r, f, mz, my, mx, dim, stale_depth, gamma);
-  } else if (n == "hllc" && !mhd) {
-    hipLaunchKernelGGL((k_op_riemann<RS_HLLC, false>), dim3(nblk(nt)), dim3(256), 0, s, l,
-                       r, f, mz, my, mx, dim, stale_depth, gamma);
-  } else {
-    // EnzoRiemann::construct_riemann: HLLD needs B, HLLC cannot take B,
-    // hydro HLLE and Davis "hll" are not available
-    return ENZOE_ERR_CONFIG;
-  }
-  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_edge_efield, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, ec, ed, fj, fk, w, box, dim);
+  return launched();
 }
 
 int enzoe_ct_update_bfield(const double cell_width[3], int dim,
-                           double *const edge[3], const double *bfieldi_cur,
-                           double *bfieldi_out, double dt, int mz, int my,
-                           int mx, int stale_depth, enzoe_stream_t stream) {
-  if (!cell_width || !edge || !bfieldi_cur || !bfieldi_out || dim < 0 || dim > 2)
+                           const enzoe_farray edge_efield[3],
+                           const enzoe_farray *cur_bfieldi,
+                           const enzoe_farray *out_bfieldi, double dt,
+                           int stale_depth, enzoe_stream_t stream) {
+  if (!cell_width || !edge_efield || !valid_dim(dim) || stale_depth < 0)
     return ENZOE_ERR_ARG;
-  Ptr3 e{};
-  for (int a = 0; a < 3; a++) {
-    if (!edge[a]) return ENZOE_ERR_ARG;
-    e.p[a] = edge[a];
-  }
   const int aj = (dim + 1) % 3, ak = (dim + 2) % 3;
+  KArr ej, ek, bc, bo;
+  if (!to_karr(&edge_efield[aj], ej) || !to_karr(&edge_efield[ak], ek) ||
+      !to_karr(cur_bfieldi, bc) || !to_karr(out_bfieldi, bo) ||
+      !same_shape(bc.n, bo.n))
+    return ENZOE_ERR_ARG;
+  // cell extents from the face array; edge[j] lacks one along k and dim,
+  // edge[k] lacks one along dim and j
+  int m[3] = {bc.n[0], bc.n[1], bc.n[2]};
+  m[dim] -= 1;
+  for (int d = 0; d < 3; d++) {
+    if (ej.n[d] != m[d] - ((d == ak || d == dim) ? 1 : 0)) return ENZOE_ERR_ARG;
+    if (ek.n[d] != m[d] - ((d == dim || d == aj) ? 1 : 0)) return ENZOE_ERR_ARG;
+  }
+  // dt/dj, dt/dk (EnzoBfieldMethodCT.cpp:627-628)
   double dtdj = dt / cell_width[aj];
   double dtdk = dt / cell_width[ak];
-  long nt = (long)(mx + (dim == 0)) * (my + (dim == 1)) * (mz + (dim == 2));
-  hipLaunchKernelGGL(k_op_update_bfield, dim3(nblk(nt)), dim3(256), 0,
-                     (hipStream_t)stream, e, bfieldi_cur, bfieldi_out, dtdj,
-                     dtdk, mz, my, mx, dim, stale_depth);
-  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = stale_depth + 1;
+    hi[d] = bc.n[d] - stale_depth - 1;
+  }
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_update_bfield, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, ej, ek, bc, bo, box, dim, dtdj, dtdk);
+  return launched();
 }
 
-int enzoe_ct_compute_center_bfield(int dim, double *bfieldc,
-                                   const double *bfieldi, int mz, int my,
-                                   int mx, int stale_depth,
-                                   enzoe_stream_t stream) {
-  if (!bfieldc || !bfieldi || dim < 0 || dim > 2 || stale_depth < 0)
+int enzoe_ct_compute_center_bfield(int dim, const enzoe_farray *bfieldc,
+                                   const enzoe_farray *bfieldi,
+                                   int stale_depth, enzoe_stream_t stream) {
+  KArr c, f;
+  if (!valid_dim(dim) || stale_depth < 0 || !to_karr(bfieldc, c) ||
+      !to_karr(bfieldi, f) || !shifted(f.n, c.n, dim, 1))
     return ENZOE_ERR_ARG;
-  long nt = (long)(mx - 2 * stale_depth) * (my - 2 * stale_depth) * (mz - 2 * stale_depth);
-  hipLaunchKernelGGL(k_op_center_bfield, dim3(nblk(nt)), dim3(256), 0,
-                     (hipStream_t)stream, bfieldc, bfieldi, mz, my, mx, dim,
-                     stale_depth);
-  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+  int lo[3], hi[3];
+  stale_box(c.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_center_bfield, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, c, f, box, dim);
+  return launched();
+}
+
+int enzoe_fill_map(const enzoe_fmap *map, double value,
+                   enzoe_stream_t stream) {
+  KMap m;
+  if (!to_kmap(map, 0, m)) return ENZOE_ERR_ARG;
+  const unsigned slots = present(m);
+  const int lo[3] = {0, 0, 0};
+  long count;
+  Box box = make_box(lo, m.n, count);
+  if (count == 0 || !slots) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_fill, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, m, slots, box, value);
+  return launched();
+}
+
+int enzoe_accumulate_flux_component(int dim, double dt, double cell_width,
+                                    const enzoe_fmap *flux,
+                                    const enzoe_fmap *dUcons, int stale_depth,
+                                    enzoe_stream_t stream) {
+  KMap f, d;
+  if (!valid_dim(dim) || stale_depth < 0 || !to_kmap(dUcons, 0, d))
+    return ENZOE_ERR_ARG;
+  const unsigned slots = present(d);
+  if (!to_kmap(flux, slots, f) || !shifted(f.n, d.n, dim, -1))
+    return ENZOE_ERR_ARG;
+  // dtdx_i = dt/cell_width (EnzoIntegrationQuanUpdate.cpp:111)
+  double dtdx = dt / cell_width;
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = stale_depth + (a == dim ? 1 : 0);
+    hi[a] = d.n[a] - stale_depth - (a == dim ? 1 : 0);
+  }
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0 || !slots) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_accumulate, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, f, d, slots, box, dim, dtdx);
+  return launched();
+}
+
+int enzoe_update_quantities(const enzoe_fmap *initial,
+                            const enzoe_fmap *dUcons, const enzoe_fmap *out,
+                            int mhd, double gamma, double density_floor,
+                            double pressure_floor, int stale_depth,
+                            enzoe_stream_t stream) {
+  KMap a, d, o;
+  const unsigned need_out = mhd ? kMhdSlots : kUpdateSlots;
+  if (stale_depth < 0 || !to_kmap(initial, kUpdateSlots, a) ||
+      !to_kmap(dUcons, kUpdateSlots, d) || !to_kmap(out, need_out, o) ||
+      !same_shape(a.n, d.n) || !same_shape(a.n, o.n))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(a.n, stale_depth + 1, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_update, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, a, d, o, box, mhd, gamma,
+                     density_floor, pressure_floor);
+  return launched();
+}
+
+int enzoe_primitive_from_integration(const enzoe_fmap *integration,
+                                     const enzoe_fmap *primitive, int mhd,
+                                     double gamma, int stale_depth,
+                                     enzoe_stream_t stream) {
+  KMap u, w;
+  const unsigned need = mhd ? kMhdSlots : kHydroSlots;
+  if (stale_depth < 0 || !to_kmap(integration, need, u) ||
+      !to_kmap(primitive, need, w) || !same_shape(u.n, w.n))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(u.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_primitive, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, u, w, box, mhd, gamma - 1.0);
+  return launched();
+}
+
+int enzoe_pressure_from_integration(const enzoe_fmap *integration,
+                                    const enzoe_farray *pressure, int mhd,
+                                    double gamma, int stale_depth,
+                                    enzoe_stream_t stream) {
+  KMap u;
+  KArr p;
+  const unsigned need = mhd ? kMhdSlots : kHydroSlots;
+  if (stale_depth < 0 || !to_kmap(integration, need, u) ||
+      !to_karr(pressure, p) || !same_shape(u.n, p.n))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(u.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_pressure, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, u, p, box, mhd, gamma - 1.0);
+  return launched();
+}
+
+int enzoe_integrator_timestep(const enzoe_fmap *integration,
+                              const enzoe_farray *pressure, int mhd,
+                              double gamma, double dx, double dy, double dz,
+                              enzoe_stream_t stream, double *dt_out) {
+  KMap u;
+  KArr p;
+  const unsigned need = mhd ? kMhdSlots : kHydroSlots;
+  if (!dt_out || !to_kmap(integration, need, u) || !to_karr(pressure, p) ||
+      !same_shape(u.n, p.n))
+    return ENZOE_ERR_ARG;
+  const long n = (long)u.n[0] * u.n[1] * u.n[2];
+  if (n == 0) {
+    *dt_out = 1.7976931348623157e308;
+    return ENZOE_OK;
+  }
+  unsigned long long *target = dt_target();
+  if (!target) return ENZOE_ERR_ALLOC;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, target);
+  const unsigned wg = (unsigned)std::min<long>(nblk(n), 1024);
+  if (mhd)
+    hipLaunchKernelGGL(k_op_dt<true>, dim3(wg), dim3(256), 0, st, u, p, gamma,
+                       dx, dy, dz, target);
+  else
+    hipLaunchKernelGGL(k_op_dt<false>, dim3(wg), dim3(256), 0, st, u, p, gamma,
+                       dx, dy, dz, target);
+  if (hipGetLastError() != hipSuccess) return ENZOE_ERR_HIP;
+  unsigned long long bits = 0;
+  if (hipMemcpyAsync(&bits, target, sizeof(bits), hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return ENZOE_ERR_HIP;
+  double dt;
+  std::memcpy(&dt, &bits, sizeof(dt));
+  *dt_out = dt;
+  return ENZOE_OK;
 }
 
 }  // extern "C"

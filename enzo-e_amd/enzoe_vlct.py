@@ -126,22 +126,6 @@ def lib():
         L.enzoe_set_profiling.argtypes = [ctypes.c_int]
         L.enzoe_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp,
                                            ctypes.POINTER(ctypes.c_int), ctypes.c_int]
-        P8 = ctypes.c_void_p * 8
-        L.enzoe_reconstruct_interface.argtypes = [
-            ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-            P8, P8, P8, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-            ctypes.c_int, ctypes.c_int, vp]
-        L.enzoe_reconstructor_staling.argtypes = [
-            ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
-        L.enzoe_riemann_solve.argtypes = [
-            ctypes.c_char_p, ctypes.c_int, ctypes.c_double, P8, P8, P8,
-            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
-        L.enzoe_ct_update_bfield.argtypes = [
-            _dp, ctypes.c_int, ctypes.c_void_p * 3, vp, vp, ctypes.c_double,
-            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
-        L.enzoe_ct_compute_center_bfield.argtypes = [
-            ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-            ctypes.c_int, vp]
         _lib = L
     return _lib
 

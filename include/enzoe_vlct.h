@@ -121,49 +121,160 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
                        enzoe_stream_t stream);
 
 /* ---- plugin objects used one call at a time -----------------------------
- * These mirror the reference's factories; each call runs on the device
- * arrays it is given and follows the reference's stale-depth regions.
- * Arrays are dense (z,y,x) of the cell-centred extents (mz,my,mx) unless
- * stated; a face array along `dim` has the cell-centred extents with the
- * `dim` extent reduced by one (index f = face between cells f and f+1).
- * Primitive / flux arrays are passed as 8 pointers in the order
- * density, velocity_x, velocity_y, velocity_z, pressure|total_energy,
- * bfield_x, bfield_y, bfield_z (hydro uses the first 5).                    */
+ * These mirror the reference's plugin methods; each call runs on device
+ * views and follows the reference's stale-depth regions exactly.  A view is
+ * the device analogue of CelloView<double,3> (src/Cello/view_CelloView.hpp:
+ * 507-512): a pointer to element (0,0,0), a (z,y,x) shape and (z,y,x)
+ * element strides, so trimmed views and slices of 4-D backing arrays are
+ * passed without copies.  A map (EnzoEFltArrayMap = ViewMap<double>,
+ * enzo_typedefs.hpp:36; view_ViewMap.hpp:43-395) is a set of same-shape,
+ * same-stride views addressed by slot instead of by key:                    */
+#define ENZOE_MAP_SLOTS 8
+#define ENZOE_SLOT_DENSITY 0     /* "density"                               */
+#define ENZOE_SLOT_VELOCITY_X 1  /* "velocity_x"                            */
+#define ENZOE_SLOT_VELOCITY_Y 2  /* "velocity_y"                            */
+#define ENZOE_SLOT_VELOCITY_Z 3  /* "velocity_z"                            */
+#define ENZOE_SLOT_ENERGY 4      /* "total_energy" (integration, dU, flux)
+                                    or "pressure" (primitive, priml/primr)   */
+#define ENZOE_SLOT_BFIELD_X 5    /* "bfield_x"                              */
+#define ENZOE_SLOT_BFIELD_Y 6    /* "bfield_y"                              */
+#define ENZOE_SLOT_BFIELD_Z 7    /* "bfield_z"                              */
+
+typedef struct enzoe_farray {
+  double *data;           /* element (0,0,0) of the view                      */
+  int shape[3];           /* (mz, my, mx)                                     */
+  long long stride[3];    /* (z, y, x) strides in elements                    */
+} enzoe_farray;
+
+typedef struct enzoe_fmap {
+  double *data[ENZOE_MAP_SLOTS]; /* NULL for a key the map does not hold     */
+  int shape[3];                  /* shared (mz, my, mx)                      */
+  long long stride[3];           /* shared (z, y, x) strides in elements     */
+} enzoe_fmap;
+
+/* Face-trimmed views along dim (reconstructed states, fluxes, CT weights)
+ * have the cell-centred shape minus one along dim; entry f is the face
+ * between cells f and f+1.  All functions return ENZOE_ERR_ARG when a view
+ * has the wrong shape or a required slot is NULL, and launch nothing.       */
 
 /* EnzoReconstructor::reconstruct_interface for "nn", "plm"/"plm_enzo",
  * "plm_athena" (toolkit/EnzoReconstructorNN.cpp:14-49,
- * EnzoReconstructorPLM.hpp:166-248). nkeys = 5 (hydro) or 8 (MHD).          */
+ * toolkit/EnzoReconstructorPLM.hpp:166-248; names as
+ * EnzoReconstructor::construct_reconstructor, EnzoReconstructor.cpp:14-44).
+ * Reconstructs every slot present in prim; PLM floors density and pressure.*/
 int enzoe_reconstruct_interface(const char *name, double theta_limiter,
                                 double density_floor, double pressure_floor,
-                                double *const prim[8], double *priml[8],
-                                double *primr[8], int nkeys, int mz, int my,
-                                int mx, int dim, int stale_depth,
-                                enzoe_stream_t stream);
+                                const enzoe_fmap *prim,
+                                const enzoe_fmap *priml,
+                                const enzoe_fmap *primr, int dim,
+                                int stale_depth, enzoe_stream_t stream);
 
-/* Staling rates of a reconstructor (EnzoReconstructor.hpp:86-94).           */
+/* EnzoReconstructor::total_staling_rate / immediate_staling_rate
+ * (EnzoReconstructor.hpp:86-94; NN 1/0, PLM 2/1).                            */
 int enzoe_reconstructor_staling(const char *name, int *total_rate,
                                 int *immediate_rate);
 
-/* EnzoRiemann::construct_riemann + solve (riemann/EnzoRiemann.cpp:26-68,
- * EnzoRiemannImpl.hpp:265-338) for "hlld", "hlle" (MHD) or "hllc" (hydro). */
+/* EnzoRiemann::construct_riemann + EnzoRiemannImpl::solve
+ * (riemann/EnzoRiemann.cpp:26-68, riemann/EnzoRiemannImpl.hpp:265-338):
+ * "hlld", "hlle" (MHD) or "hllc" (hydro) over [stale, n - stale) of the
+ * flux view.  Other names or combinations: ENZOE_ERR_CONFIG.                */
 int enzoe_riemann_solve(const char *solver, int mhd, double gamma,
-                        double *const priml[8], double *const primr[8],
-                        double *flux[8], int mz, int my, int mx, int dim,
-                        int stale_depth, enzoe_stream_t stream);
+                        const enzoe_fmap *priml, const enzoe_fmap *primr,
+                        const enzoe_fmap *flux, int dim, int stale_depth,
+                        enzoe_stream_t stream);
 
-/* EnzoBfieldMethodCT::update_bfield (toolkit/EnzoBfieldMethodCT.cpp:
- * 617-687).  edge[i] has the cell-centred extents minus one along the two
- * axes other than i.                                                        */
+/* EnzoBfieldMethodCT::correct_reconstructed_bfield
+ * (toolkit/EnzoBfieldMethodCT.cpp:122-166): the interior faces of bfieldi
+ * (full face-centred view, +1 along dim) into the bfield_dim slot of both
+ * reconstructed states.                                                     */
+int enzoe_ct_correct_reconstructed_bfield(int dim, const enzoe_farray *bfieldi,
+                                          const enzoe_fmap *priml,
+                                          const enzoe_fmap *primr,
+                                          int stale_depth,
+                                          enzoe_stream_t stream);
+
+/* EnzoBfieldMethodCT::identify_upwind (:170-216): weight 1 / 0 / 0.5 from
+ * the sign of the density flux; weight has the flux view's shape.           */
+int enzoe_ct_identify_upwind(int dim, const enzoe_fmap *flux,
+                             const enzoe_farray *weight, int stale_depth,
+                             enzoe_stream_t stream);
+
+/* EnzoBfieldMethodCT::compute_center_efield (:267-292):
+ * E_dim = -v_j B_k + v_k B_j of the integration map.                        */
+int enzoe_ct_compute_center_efield(int dim, const enzoe_farray *efield,
+                                   const enzoe_fmap *integration,
+                                   int stale_depth, enzoe_stream_t stream);
+
+/* EnzoBfieldMethodCT::compute_edge_efield (:480-557): upwinded edge EMF
+ * along dim from the centre E, the j-flux map (bfield_k slot) and the
+ * k-flux map (bfield_j slot) and the weights of the three dims.  The edge
+ * view has the cell shape minus one along j and k (:70-72).                 */
+int enzoe_ct_compute_edge_efield(int dim, const enzoe_farray *center_efield,
+                                 const enzoe_farray *edge_efield,
+                                 const enzoe_fmap *jflux,
+                                 const enzoe_fmap *kflux,
+                                 const enzoe_farray weight[3],
+                                 int stale_depth, enzoe_stream_t stream);
+
+/* EnzoBfieldMethodCT::update_bfield (:617-687): out = cur - dt/dj dE_k +
+ * dt/dk dE_j on the interior faces; bfieldi views are +1 along dim.         */
 int enzoe_ct_update_bfield(const double cell_width[3], int dim,
-                           double *const edge[3], const double *bfieldi_cur,
-                           double *bfieldi_out, double dt, int mz, int my,
-                           int mx, int stale_depth, enzoe_stream_t stream);
+                           const enzoe_farray edge_efield[3],
+                           const enzoe_farray *cur_bfieldi,
+                           const enzoe_farray *out_bfieldi, double dt,
+                           int stale_depth, enzoe_stream_t stream);
 
 /* EnzoBfieldMethodCT::compute_center_bfield (:702-728).                      */
-int enzoe_ct_compute_center_bfield(int dim, double *bfieldc,
-                                   const double *bfieldi, int mz, int my,
-                                   int mx, int stale_depth,
-                                   enzoe_stream_t stream);
+int enzoe_ct_compute_center_bfield(int dim, const enzoe_farray *bfieldc,
+                                   const enzoe_farray *bfieldi,
+                                   int stale_depth, enzoe_stream_t stream);
+
+/* EnzoIntegrationQuanUpdate::clear_dUcons_map
+ * (toolkit/EnzoIntegrationQuanUpdate.cpp:83-101): every present slot over
+ * the whole view set to value (a kernel, no memset).                        */
+int enzoe_fill_map(const enzoe_fmap *map, double value,
+                   enzoe_stream_t stream);
+
+/* EnzoIntegrationQuanUpdate::accumulate_flux_component (:105-147):
+ * dU(c) -= dt/dx (F(c+1/2) - F(c-1/2)) for every slot present in dUcons.   */
+int enzoe_accumulate_flux_component(int dim, double dt, double cell_width,
+                                    const enzoe_fmap *flux,
+                                    const enzoe_fmap *dUcons, int stale_depth,
+                                    enzoe_stream_t stream);
+
+/* EnzoIntegrationQuanUpdate::update_quantities (:183-238) followed by
+ * EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync
+ * (EnzoPhysicsFluidProps.cpp:162-290) at stale_depth + 1.  initial/dUcons
+ * need slots 0-4; out's bfield slots (MHD) must already hold the updated
+ * cell-centred B (the reference runs CT first for this reason).             */
+int enzoe_update_quantities(const enzoe_fmap *initial,
+                            const enzoe_fmap *dUcons, const enzoe_fmap *out,
+                            int mhd, double gamma, double density_floor,
+                            double pressure_floor, int stale_depth,
+                            enzoe_stream_t stream);
+
+/* EnzoPhysicsFluidProps::primitive_from_integration
+ * (EnzoPhysicsFluidProps.cpp:64-138) with the pressure of
+ * EnzoComputePressure::compute_pressure (EnzoComputePressure.cpp:177-189).  */
+int enzoe_primitive_from_integration(const enzoe_fmap *integration,
+                                     const enzoe_fmap *primitive, int mhd,
+                                     double gamma, int stale_depth,
+                                     enzoe_stream_t stream);
+
+/* EnzoMHDIntegratorStageCommands::timestep
+ * (EnzoMHDIntegratorStageCommands.cpp:299-366): min over every cell of the
+ * views of dx_i/(|v_i| + c_fast) (c_s for hydro) from the given pressure,
+ * without the courant factor.  Synchronises `stream`.                       */
+int enzoe_integrator_timestep(const enzoe_fmap *integration,
+                              const enzoe_farray *pressure, int mhd,
+                              double gamma, double dx, double dy, double dz,
+                              enzoe_stream_t stream, double *dt_out);
+
+/* EnzoPhysicsFluidProps::pressure_from_integration (:142-156).               */
+int enzoe_pressure_from_integration(const enzoe_fmap *integration,
+                                    const enzoe_farray *pressure, int mhd,
+                                    double gamma, int stale_depth,
+                                    enzoe_stream_t stream);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 

@@ -996,6 +996,40 @@ void o_update_bfield(const double cw[3], int i, double *const *edge,
       }
 }
 
+/* Exported single-operation entry points (same loops as above), for the
+ * unit parity tests of the per-operation device API. */
+void o_correct_reconstructed_bfield(const double *bfi, double *wl, double *wr,
+                                    int mz, int my, int mx, int dim, int s) {
+  correct_reconstructed_bfield(bfi, wl, wr, mz, my, mx, dim, s);
+}
+void o_identify_upwind(const double *rho_flux, double *w, int mz, int my,
+                       int mx, int dim, int s) {
+  identify_upwind(rho_flux, w, mz, my, mx, dim, s);
+}
+void o_compute_center_efield(int dim, double *E, double *const *U, int mz,
+                             int my, int mx, int s) {
+  center_efield(dim, E, U, mz, my, mx, s);
+}
+void o_compute_edge_efield(int dim, const double *Ec, double *edge,
+                           double *const *jflux, double *const *kflux,
+                           double *const *weight, int mz, int my, int mx,
+                           int s) {
+  edge_efield(dim, Ec, edge, jflux, kflux, weight, mz, my, mx, s);
+}
+void o_accumulate_flux_component(int dim, double dt, double cw, double **flux,
+                                 double **dU, int mz, int my, int mx, int s) {
+  accumulate_flux(dim, dt, cw, flux, dU, mz, my, mx, s);
+}
+void o_update_quantities(const o_config *cfg, double *const *U0,
+                         double *const *dU, double **Uo, int mz, int my,
+                         int mx, int s) {
+  update_quantities(cfg, U0, dU, Uo, mz, my, mx, s);
+}
+void o_primitive_from_integration(const o_config *cfg, double *const *U,
+                                  double **W, int mz, int my, int mx, int s) {
+  primitive_from_integration(cfg, U, W, mz, my, mx, s);
+}
+
 /* EnzoBfieldMethodCT::compute_center_bfield (EnzoBfieldMethodCT.cpp:702-728)
  * over cells [s, m-s): Bc = 0.5*(B_{c-1/2} + B_{c+1/2}). */
 void o_compute_center_bfield(int dim, double *bc, const double *bi, int mz,

@@ -100,6 +100,27 @@ void o_update_bfield(const double cw[3], int dim, double *const *edge,
 void o_compute_center_bfield(int dim, double *bc, const double *bi, int mz,
                              int my, int mx, int stale_depth);
 
+/* single-operation restatements, flux/face arrays face-trimmed along dim;
+ * flux and dU maps indexed by O_* slots (dU: density, vx, vy, vz, etot) */
+void o_correct_reconstructed_bfield(const double *bfi, double *wl, double *wr,
+                                    int mz, int my, int mx, int dim, int s);
+void o_identify_upwind(const double *rho_flux, double *w, int mz, int my,
+                       int mx, int dim, int s);
+void o_compute_center_efield(int dim, double *E, double *const *U, int mz,
+                             int my, int mx, int s);
+void o_compute_edge_efield(int dim, const double *Ec, double *edge,
+                           double *const *jflux, double *const *kflux,
+                           double *const *weight, int mz, int my, int mx,
+                           int s);
+void o_accumulate_flux_component(int dim, double dt, double cw, double **flux,
+                                 double **dU, int mz, int my, int mx, int s);
+void o_update_quantities(const o_config *cfg, double *const *U0,
+                         double *const *dU, double **Uo, int mz, int my,
+                         int mx, int s);
+void o_primitive_from_integration(const o_config *cfg, double *const *U,
+                                  double **W, int mz, int my, int mx, int s);
+
+
 /* ---- multi-block harness ------------------------------------------------ */
 
 enum { O_BC_PERIODIC = 0, O_BC_OUTFLOW = 1 };
