@@ -4,6 +4,8 @@
 // like the reference's CPU build.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cctype>
 #include <cstdio>
@@ -134,6 +136,10 @@ int enzoe_vlct_create(const enzoe_vlct_params *p, enzoe_vlct_t *out) {
   m->precision = p->precision;
   // CT requires exactly 2 partial timesteps (EnzoBfieldMethod.cpp:23-27)
   if (m->mhd && m->nstages != 2) { delete m; return ENZOE_ERR_CONFIG; }
+  {
+    const char *e = std::getenv("ENZOE_XCD_REMAP");
+    if (e && *e) m->xcd_remap = std::atoi(e) & 15;
+  }
   // no device call here: device memory is taken on first use, so a
   // configuration can be validated on a host without a GPU
   *out = m;

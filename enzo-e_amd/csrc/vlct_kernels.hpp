@@ -49,10 +49,32 @@ struct StageParams {
   double density_floor;
   double pressure_floor;
   double theta;
+  int xcd_remap;       // bit mask over k_flux(1) k_edge(2) k_faces(4)
+                       // k_cells(8): XCD-contiguous order, see wg_coords
 };
 
 __device__ __forceinline__ long idx3(int k, int j, int i, int ny, int nx) {
   return ((long)k * ny + j) * (long)nx + i;
+}
+
+
+// Workgroup coordinates of a (gx, gy) launch.  The hardware hands the
+// workgroups of a launch to the 8 XCDs round-robin in linear order, and each
+// XCD has its own L2; with xcd_remap the linear range [x T/8, (x+1) T/8)
+// goes to XCD x, so the neighbouring rows a stencil re-reads stay in one L2.
+// Needs T = gx * gy to be a multiple of 8: the launches pad gx to a multiple
+// of 8 and padding workgroups fall outside every box.
+__device__ __forceinline__ void wg_coords(int xcd_remap, int &bx, int &by) {
+  if (!xcd_remap) {
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    return;
+  }
+  const unsigned gx = gridDim.x, T = gx * gridDim.y;
+  const unsigned L = blockIdx.y * gx + blockIdx.x;
+  const unsigned lg = (L & 7u) * (T >> 3) + (L >> 3);
+  by = (int)(lg / gx);
+  bx = (int)(lg - (unsigned)by * gx);
 }
 
 // Decodes a linear thread index into (k,j,i) inside the box [lo, lo+n).
@@ -201,11 +223,13 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
 template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD>
 __global__ void __launch_bounds__(256)
 k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  int bx, by;
+  wg_coords(sp.xcd_remap & 1, bx, by);
+  const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
   const int nx = mx - 2 * cs, ny = my - 2 * cs;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = bx * blockDim.x + threadIdx.x;
   const int nxy = nx * ny;
   if (t >= nxy * (mz - 2 * cs)) return;
   int k = t / nxy;
@@ -250,11 +274,13 @@ __device__ __forceinline__ double upwind_weight(double f) {
 template <typename FluxT>
 __global__ void __launch_bounds__(256)
 k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  int bx, by;
+  wg_coords(sp.xcd_remap & 2, bx, by);
+  const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
   int k, j, i;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)bx * blockDim.x + threadIdx.x;
   if (!box_coords(t, cs, mx - 2 * cs, my - 2 * cs, mz - 2 * cs, k, j, i))
     return;
   double *const *U = (sp.stage == 0) ? b.U : b.Ut;
@@ -306,11 +332,13 @@ k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
 template <typename FluxT>
 __global__ void __launch_bounds__(256)
 k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  int bx, by;
+  wg_coords(sp.xcd_remap & 4, bx, by);
+  const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
   int k, j, i;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)bx * blockDim.x + threadIdx.x;
   const int lo = cs + 1;
   if (!box_coords(t, lo, mx - cs - lo, my - cs - lo, mz - cs - lo, k, j, i))
     return;
@@ -349,12 +377,14 @@ k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
 template <typename FluxT, bool MHD>
 __global__ void __launch_bounds__(256)
 k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  const BlockDesc<FluxT> &b = blocks[blockIdx.y];
+  int bx, by;
+  wg_coords(sp.xcd_remap & 8, bx, by);
+  const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
   const int lo = sp.is_final ? cs : cs + 1;
   int k, j, i;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)bx * blockDim.x + threadIdx.x;
   if (!box_coords(t, lo, mx - 2 * lo, my - 2 * lo, mz - 2 * lo, k, j, i))
     return;
   const long c = idx3(k, j, i, my, mx);

@@ -57,6 +57,8 @@ struct KernelTimer {
 };
 
 inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// workgroup count padded to a multiple of the 8 XCDs (see wg_coords)
+inline unsigned nblk8(long n, int bs) { return (nblk(n, bs) + 7u) & ~7u; }
 
 }  // namespace enzoe
 
@@ -68,6 +70,9 @@ struct enzoe_vlct_method {
   int recon[2] = {enzoe::RC_NN, enzoe::RC_PLM};
   int nstages = 2;
   int precision = ENZOE_PREC_DOUBLE;
+  // kernels launched in XCD-contiguous workgroup order (StageParams mask);
+  // measured on the 256^3 Orszag-Tang block it pays for k_edge only
+  int xcd_remap = 2;
   double theta = 1.5, courant = 0.3, gamma = 5.0 / 3.0;
   double dfloor = 1e-200, pfloor = 1e-200;
 
@@ -195,7 +200,7 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
 template <typename FluxT, int RECON, int RIEMANN, bool MHD>
 void launch_flux(const BlockDesc<FluxT> *d, int nb, long nthreads,
                  StageParams sp, hipStream_t s) {
-  dim3 grid(nblk(nthreads, 256), nb);
+  dim3 grid(nblk8(nthreads, 256), nb);
   hipLaunchKernelGGL((k_flux<FluxT, FluxT, RECON, RIEMANN, MHD>), grid,
                      dim3(256), 0, s, d, sp);
 }
@@ -274,6 +279,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       sp.density_floor = m->dfloor;
       sp.pressure_floor = m->pfloor;
       sp.theta = m->theta;
+      sp.xcd_remap = m->xcd_remap;
       const int cs = sp.cur_stale;
       {
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm");
@@ -282,13 +288,13 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       if (MHD) {
         {
           KernelTimer kt(s, "k_edge");
-          hipLaunchKernelGGL((k_edge<FluxT>), dim3(nblk(cube(mx, my, mz, cs, cs), 256), nb),
+          hipLaunchKernelGGL((k_edge<FluxT>), dim3(nblk8(cube(mx, my, mz, cs, cs), 256), nb),
                              dim3(256), 0, s, d, sp);
         }
         {
           KernelTimer kt(s, "k_faces");
           hipLaunchKernelGGL((k_faces<FluxT>),
-                             dim3(nblk(cube(mx, my, mz, cs + 1, cs), 256), nb), dim3(256),
+                             dim3(nblk8(cube(mx, my, mz, cs + 1, cs), 256), nb), dim3(256),
                              0, s, d, sp);
         }
       }
@@ -296,7 +302,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
         KernelTimer kt(s, "k_cells");
         const int lo = is_final ? cs : cs + 1;
         hipLaunchKernelGGL((k_cells<FluxT, MHD>),
-                           dim3(nblk(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
+                           dim3(nblk8(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
                            s, d, sp);
       }
       stale += total_staling(recon);

@@ -349,3 +349,148 @@ class DeviceGrid:
             n += 1
         synchronize()
         return n, log
+
+
+# ------------------------------------------------ single-operation views
+
+SLOTS = {"density": 0, "velocity_x": 1, "velocity_y": 2, "velocity_z": 3,
+         "total_energy": 4, "pressure": 4, "bfield_x": 5, "bfield_y": 6,
+         "bfield_z": 7}
+
+
+class FArray(ctypes.Structure):
+    """enzoe_farray: one device view (pointer, (z,y,x) shape, strides)."""
+    _fields_ = [("data", ctypes.c_void_p), ("shape", ctypes.c_int * 3),
+                ("stride", ctypes.c_longlong * 3)]
+
+
+class FMap(ctypes.Structure):
+    """enzoe_fmap: same-shape device views by slot."""
+    _fields_ = [("data", ctypes.c_void_p * 8), ("shape", ctypes.c_int * 3),
+                ("stride", ctypes.c_longlong * 3)]
+
+
+def farray(ptr, shape, stride=None):
+    """View of device memory at `ptr` (bytes) with (z,y,x) shape and element
+    strides (dense when omitted)."""
+    a = FArray()
+    a.data = ptr
+    if stride is None:
+        stride = (shape[1] * shape[2], shape[2], 1)
+    for i in range(3):
+        a.shape[i] = shape[i]
+        a.stride[i] = stride[i]
+    return a
+
+
+def fmap(ptrs, shape, stride=None):
+    """Map from {key or slot: device pointer} with a common shape/stride."""
+    m = FMap()
+    if stride is None:
+        stride = (shape[1] * shape[2], shape[2], 1)
+    for k, p in ptrs.items():
+        m.data[SLOTS[k] if isinstance(k, str) else k] = p
+    for i in range(3):
+        m.shape[i] = shape[i]
+        m.stride[i] = stride[i]
+    return m
+
+
+_ops_bound = False
+
+
+def ops():
+    """The library with the single-operation entry points bound."""
+    global _ops_bound
+    L = lib()
+    if not _ops_bound:
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        PM, PA = ctypes.POINTER(FMap), ctypes.POINTER(FArray)
+        L.enzoe_reconstruct_interface.argtypes = [ctypes.c_char_p, d, d, d, PM, PM, PM,
+                                                  i, i, vp]
+        L.enzoe_reconstructor_staling.argtypes = [ctypes.c_char_p,
+                                                  ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.enzoe_riemann_solve.argtypes = [ctypes.c_char_p, i, d, PM, PM, PM, i, i, vp]
+        L.enzoe_ct_correct_reconstructed_bfield.argtypes = [i, PA, PM, PM, i, vp]
+        L.enzoe_ct_identify_upwind.argtypes = [i, PM, PA, i, vp]
+        L.enzoe_ct_compute_center_efield.argtypes = [i, PA, PM, i, vp]
+        L.enzoe_ct_compute_edge_efield.argtypes = [i, PA, PA, PM, PM, PA, i, vp]
+        L.enzoe_ct_update_bfield.argtypes = [_dp, i, PA, PA, PA, d, i, vp]
+        L.enzoe_ct_compute_center_bfield.argtypes = [i, PA, PA, i, vp]
+        L.enzoe_fill_map.argtypes = [PM, d, vp]
+        L.enzoe_accumulate_flux_component.argtypes = [i, d, d, PM, PM, i, vp]
+        L.enzoe_update_quantities.argtypes = [PM, PM, PM, i, d, d, d, i, vp]
+        L.enzoe_primitive_from_integration.argtypes = [PM, PM, i, d, i, vp]
+        L.enzoe_pressure_from_integration.argtypes = [PM, PA, i, d, i, vp]
+        L.enzoe_integrator_timestep.argtypes = [PM, PA, i, d, d, d, d, vp, _dp]
+        _ops_bound = True
+    return L
+
+
+# ------------------------------------------------------ C++ drop-in layer
+
+HOST_LIB_PATH = os.path.join(HERE, "libenzoe_vlct_host.so")
+_host = None
+
+
+def host_lib():
+    """libenzoe_vlct_host.so: the C++ EnzoMethodMHDVlct and its plugins."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise EnzoeError("libenzoe_vlct_host.so is not built (run "
+                             "__graft_entry__.build())")
+        lib()
+        H = ctypes.CDLL(HOST_LIB_PATH)
+        vp = ctypes.c_void_p
+        H.enzoe_host_set_fluid_props.argtypes = [ctypes.c_double] * 3
+        H.enzoe_host_method_create.argtypes = [ctypes.c_char_p]
+        H.enzoe_host_method_create.restype = vp
+        H.enzoe_host_method_destroy.argtypes = [vp]
+        H.enzoe_host_method_name.argtypes = [vp]
+        H.enzoe_host_method_name.restype = ctypes.c_char_p
+        H.enzoe_host_method_courant.argtypes = [vp]
+        H.enzoe_host_method_courant.restype = ctypes.c_double
+        H.enzoe_host_method_compute.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                                ctypes.c_double, vp]
+        H.enzoe_host_method_compute.restype = ctypes.c_int
+        H.enzoe_host_method_timestep.argtypes = [vp, ctypes.POINTER(Block), vp]
+        H.enzoe_host_method_timestep.restype = ctypes.c_double
+        _host = H
+    return _host
+
+
+class HostMethod:
+    """EnzoMethodMHDVlct of the C++ drop-in layer, built from the text of a
+    "Method:mhd_vlct" parameter group."""
+
+    def __init__(self, parameters, gamma=5.0 / 3.0, density_floor=1e-200,
+                 pressure_floor=1e-200):
+        H = host_lib()
+        H.enzoe_host_set_fluid_props(gamma, density_floor, pressure_floor)
+        self.handle = H.enzoe_host_method_create(parameters.encode())
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _host:
+            _host.enzoe_host_method_destroy(self.handle)
+            self.handle = None
+
+    @property
+    def name(self):
+        return host_lib().enzoe_host_method_name(self.handle).decode()
+
+    @property
+    def courant(self):
+        return host_lib().enzoe_host_method_courant(self.handle)
+
+    def compute(self, blocks, dt, stream=None):
+        arr = (Block * len(blocks))(*[b.desc for b in blocks])
+        done = host_lib().enzoe_host_method_compute(self.handle, arr, len(blocks), dt,
+                                                    stream)
+        if done != len(blocks):
+            raise EnzoeError("compute_done called %d times for %d blocks"
+                             % (done, len(blocks)))
+
+    def timestep(self, block, stream=None):
+        return host_lib().enzoe_host_method_timestep(self.handle, ctypes.byref(block.desc),
+                                                     stream)

@@ -1,0 +1,571 @@
+// enzoe_host.hpp -- C++ drop-in layer with the reference's plugin surface for
+// the VL+CT method: the same class names, static factories, accepted names,
+// argument meaning, defaults and error behaviour as
+//   src/Enzo/hydro-mhd/EnzoMethodMHDVlct.{hpp,cpp}
+//   src/Enzo/hydro-mhd/EnzoMHDIntegratorStageCommands.{hpp,cpp}
+//   src/Enzo/hydro-mhd/toolkit/EnzoReconstructor*.{hpp,cpp}
+//   src/Enzo/hydro-mhd/riemann/EnzoRiemann*.{hpp,cpp}
+//   src/Enzo/hydro-mhd/toolkit/EnzoBfieldMethod{,CT}.{hpp,cpp}
+//   src/Enzo/hydro-mhd/toolkit/EnzoIntegrationQuanUpdate.{hpp,cpp}
+// Every array is device memory; every operation is one call of the gfx950
+// library through its C ABI (include/enzoe_vlct.h).  The Cello pieces the
+// method touches (CelloView, ViewMap, Block, ParameterGroup, the fluid
+// properties) are reduced to what the method uses.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/enzoe_vlct.h"
+
+using enzo_float = double;
+using str_vec_t = std::vector<std::string>;
+
+// ERROR / ASSERT of the reference (src/Cello/error_Error.hpp:52,264,
+// error_Error.cpp:86): print the message and exit(1).
+[[noreturn]] void enzoe_host_error(const char *function, const std::string &message);
+#define ENZOE_ERROR(F, M) enzoe_host_error((F), (M))
+#define ENZOE_ASSERT(F, M, COND) \
+  do {                           \
+    if (!(COND)) enzoe_host_error((F), (M)); \
+  } while (0)
+
+// Stream every plugin call of the calling thread launches on (default: the
+// null stream).  One stream per host thread, as one PE per Method instance.
+void enzoe_host_set_stream(enzoe_stream_t stream);
+enzoe_stream_t enzoe_host_stream();
+
+// ------------------------------------------------------------------ views
+
+// Slice [start, stop) of one axis; a negative bound counts from the end and
+// an absent bound is open (src/Cello/view_CSlice.hpp).
+class CSlice {
+ public:
+  CSlice() = default;
+  CSlice(std::nullptr_t, std::nullptr_t) {}
+  CSlice(int start, int stop) : has_start_(true), has_stop_(true), start_(start), stop_(stop) {}
+  CSlice(int start, std::nullptr_t) : has_start_(true), start_(start) {}
+  CSlice(std::nullptr_t, int stop) : has_stop_(true), stop_(stop) {}
+  // resolved [lo, hi) on an axis of extent n (ERROR when empty or outside)
+  void resolve(int n, int *lo, int *hi) const;
+
+ private:
+  bool has_start_ = false, has_stop_ = false;
+  int start_ = 0, stop_ = 0;
+};
+
+// CelloView<double, 3> over device memory (src/Cello/view_CelloView.hpp:
+// 356-931): shared ownership of the allocation, offset, (z,y,x) shape and
+// strides, row-major with x fastest.
+class EFlt3DArray {
+ public:
+  EFlt3DArray() = default;
+  // a new zero-filled device array, owned (shared) by the views
+  EFlt3DArray(int mz, int my, int mx);
+  // a view of device memory owned elsewhere
+  EFlt3DArray(double *data, int mz, int my, int mx);
+  EFlt3DArray(std::shared_ptr<void> owner, double *data,
+              const std::array<int, 3> &shape,
+              const std::array<long long, 3> &stride);
+
+  int shape(unsigned axis) const { return shape_[axis]; }
+  long long stride(unsigned axis) const { return stride_[axis]; }
+  double *data() const { return data_; }
+  bool is_null() const { return data_ == nullptr; }
+  std::size_t size() const {
+    return (std::size_t)shape_[0] * shape_[1] * shape_[2];
+  }
+  EFlt3DArray subarray(const CSlice &z, const CSlice &y, const CSlice &x) const;
+  enzoe_farray desc() const;
+
+ private:
+  std::shared_ptr<void> owner_;
+  double *data_ = nullptr;
+  std::array<int, 3> shape_{{0, 0, 0}};
+  std::array<long long, 3> stride_{{0, 0, 0}};
+};
+
+// EnzoPermutedCoordinates (EnzoPermutedCoordinates.hpp:16-104): i = dim,
+// j = (dim+1)%3, k = (dim+2)%3; slices given in (k, j, i) order.
+class EnzoPermutedCoordinates {
+ public:
+  explicit EnzoPermutedCoordinates(int i_axis);
+  int i_axis() const { return i_; }
+  int j_axis() const { return (i_ + 1) % 3; }
+  int k_axis() const { return (i_ + 2) % 3; }
+  EFlt3DArray get_subarray(const EFlt3DArray &a, const CSlice &k_slice,
+                           const CSlice &j_slice, const CSlice &i_slice) const;
+  EFlt3DArray left_edge_offset(const EFlt3DArray &a, int k_start, int j_start,
+                               int i_start) const;
+
+ private:
+  int i_;
+};
+
+// EnzoEFltArrayMap = ViewMap<enzo_float> (view_ViewMap.hpp:43-395): keys ->
+// same-shape views.  Built from (keys, shape) the views are slices of one
+// contiguous (nkeys, mz, my, mx) device array, as ViewCollec
+// (view_ViewCollec.hpp:126-170).
+class EnzoEFltArrayMap {
+ public:
+  EnzoEFltArrayMap() = default;
+  EnzoEFltArrayMap(std::string name, const str_vec_t &keys,
+                   const std::array<int, 3> &shape);
+  EnzoEFltArrayMap(std::string name, const str_vec_t &keys,
+                   const std::vector<EFlt3DArray> &arrays);
+
+  const std::string &name() const { return name_; }
+  std::size_t size() const { return keys_.size(); }
+  const str_vec_t &keys() const { return keys_; }
+  bool contains(const std::string &key) const;
+  EFlt3DArray at(const std::string &key) const;
+  EFlt3DArray operator[](const std::string &key) const { return at(key); }
+  // view trimmed by stale_depth on every side (view_ViewMap.hpp:310-327)
+  EFlt3DArray get(const std::string &key, int stale_depth = 0) const;
+  int array_shape(unsigned axis) const;
+  EnzoEFltArrayMap subarray_map(const CSlice &z, const CSlice &y,
+                                const CSlice &x,
+                                const std::string &name = "") const;
+  // the device descriptor: views by slot (ENZOE_SLOT_*); ERROR for a key
+  // without a slot or views of different shapes or strides
+  enzoe_fmap slot_map() const;
+
+ private:
+  std::string name_;
+  str_vec_t keys_;
+  std::vector<EFlt3DArray> arrays_;
+};
+
+// ------------------------------------------------------- fluid properties
+
+// The slice of EnzoPhysicsFluidProps the method reads: ideal-gas gamma
+// (EnzoEOSIdeal) and the density / pressure floors
+// (EnzoFluidFloorConfig); the dual-energy formalism is not provided.
+class EnzoPhysicsFluidProps {
+ public:
+  EnzoPhysicsFluidProps(double gamma, double density_floor,
+                        double pressure_floor);
+  double gamma() const { return gamma_; }
+  double density_floor() const { return density_floor_; }
+  double pressure_floor() const { return pressure_floor_; }
+  bool dual_energy_enabled() const { return false; }
+
+  // EnzoPhysicsFluidProps::primitive_from_integration (:64-138)
+  void primitive_from_integration(const EnzoEFltArrayMap &integration_map,
+                                  EnzoEFltArrayMap &primitive_map,
+                                  int stale_depth, const str_vec_t &passive_list,
+                                  bool ignore_grackle = false) const;
+  // EnzoPhysicsFluidProps::pressure_from_integration (:142-156)
+  void pressure_from_integration(const EnzoEFltArrayMap &integration_map,
+                                 const EFlt3DArray &pressure, int stale_depth,
+                                 bool ignore_grackle = false) const;
+
+ private:
+  double gamma_, density_floor_, pressure_floor_;
+};
+
+namespace enzo {
+// the process-wide fluid properties (Physics "fluid_props")
+EnzoPhysicsFluidProps *fluid_props();
+void set_fluid_props(const EnzoPhysicsFluidProps &props);
+}  // namespace enzo
+
+// ---------------------------------------------------------- Cello slices
+
+// "key = value" parameters of one Method group (Cello ParameterGroup);
+// values are given as text, separated by ';' or newlines.
+class ParameterGroup {
+ public:
+  explicit ParameterGroup(const std::string &text = "",
+                          std::string group_path = "Method:mhd_vlct");
+  void set(const std::string &name, const std::string &value);
+  const std::string *param(const std::string &name) const;
+  std::string value_string(const std::string &name, const std::string &dflt) const;
+  double value_float(const std::string &name, double dflt) const;
+  std::string full_name(const std::string &name) const;
+  const std::string &get_group_path() const { return path_; }
+
+ private:
+  std::string path_;
+  std::map<std::string, std::string> values_;
+};
+
+// The slice of a Cello Block the method touches: its permanent fields
+// (device), cell widths, dt, and the compute_done continuation.
+class Block {
+ public:
+  // device-resident fields, one allocation holding every permanent field
+  // (density, velocity_x..z, total_energy, pressure and for MHD
+  // bfield_x..z, bfieldi_x..z), ghost zones included
+  Block(const std::array<int, 3> &active_xyz, int ghost_depth,
+        const std::array<double, 3> &cell_width, bool mhd);
+  // fields owned by the caller
+  explicit Block(const enzoe_block &fields);
+
+  EFlt3DArray field_view(const std::string &name) const;
+  bool has_field(const std::string &name) const;
+  const double *CellWidth() const { return cell_width_.data(); }
+  int ghost_depth() const { return ghost_; }
+  double dt() const { return dt_; }
+  void set_dt(double dt) { dt_ = dt; }
+  bool is_leaf() const { return true; }
+  void compute_done() { ++compute_done_count_; }
+  int compute_done_count() const { return compute_done_count_; }
+  // the fields as the library's block descriptor
+  enzoe_block desc() const;
+
+ private:
+  void bind_(const enzoe_block &b);
+  std::shared_ptr<void> storage_;
+  std::map<std::string, EFlt3DArray> fields_;
+  std::array<double, 3> cell_width_{{0., 0., 0.}};
+  int ghost_ = 0;
+  double dt_ = 0.;
+  int compute_done_count_ = 0;
+};
+
+// Cello Method: compute(Block*) + timestep(Block*) + courant
+// (src/Cello/problem_Method.hpp:14-127).
+class Method {
+ public:
+  virtual ~Method() = default;
+  virtual void compute(Block *block) throw() = 0;
+  virtual std::string name() throw() = 0;
+  virtual double timestep(Block *block) throw() = 0;
+  double courant() const { return courant_; }
+  void set_courant(double courant) { courant_ = courant; }
+
+ protected:
+  double courant_ = 1.0;
+};
+
+// --------------------------------------------------------- reconstruction
+
+class EnzoReconstructor {
+ public:
+  // EnzoReconstructor::construct_reconstructor (EnzoReconstructor.cpp:14-44)
+  static EnzoReconstructor *construct_reconstructor(
+      const str_vec_t &active_primitive_keys, std::string name,
+      enzo_float theta_limiter);
+
+  explicit EnzoReconstructor(str_vec_t active_primitive_keys)
+      : active_key_names_(std::move(active_primitive_keys)) {}
+  virtual ~EnzoReconstructor() = default;
+
+  // EnzoReconstructor.hpp:61-64
+  virtual void reconstruct_interface(const EnzoEFltArrayMap &prim_map,
+                                     EnzoEFltArrayMap &priml_map,
+                                     EnzoEFltArrayMap &primr_map, const int dim,
+                                     const int stale_depth,
+                                     const str_vec_t &passive_list);
+  virtual int total_staling_rate() = 0;
+  virtual int immediate_staling_rate() = 0;
+
+ protected:
+  virtual const char *device_name_() const = 0;
+  virtual enzo_float theta_() const { return 1.5; }
+  str_vec_t active_key_names_;
+};
+
+class EnzoReconstructorNN : public EnzoReconstructor {
+ public:
+  explicit EnzoReconstructorNN(const str_vec_t &active_primitive_keys)
+      : EnzoReconstructor(active_primitive_keys) {}
+  // EnzoReconstructorNN.hpp:31-35
+  int total_staling_rate() override { return 1; }
+  int immediate_staling_rate() override { return 0; }
+
+ protected:
+  const char *device_name_() const override { return "nn"; }
+};
+
+// the two slope limiters of EnzoReconstructorPLM.hpp:257-346
+struct PLM_EnzoRKLimiter {
+  static constexpr const char *device_name = "plm";
+};
+struct PLM_AthenaLimiter {
+  static constexpr const char *device_name = "plm_athena";
+};
+
+template <class Limiter>
+class EnzoReconstructorPLM : public EnzoReconstructor {
+ public:
+  EnzoReconstructorPLM(const str_vec_t &active_primitive_keys,
+                       enzo_float theta_limiter)
+      : EnzoReconstructor(active_primitive_keys), theta_limiter_(theta_limiter) {}
+  // EnzoReconstructorPLM.hpp:153-157
+  int total_staling_rate() override { return 2; }
+  int immediate_staling_rate() override { return 1; }
+
+ protected:
+  const char *device_name_() const override { return Limiter::device_name; }
+  enzo_float theta_() const override { return theta_limiter_; }
+  enzo_float theta_limiter_;
+};
+
+using EnzoReconstructorPLMEnzoRKLim = EnzoReconstructorPLM<PLM_EnzoRKLimiter>;
+using EnzoReconstructorPLMAthenaLim = EnzoReconstructorPLM<PLM_AthenaLimiter>;
+
+// ---------------------------------------------------------------- Riemann
+
+class EnzoRiemann {
+ public:
+  // EnzoRiemann.hpp:44-60
+  struct FactoryArgs {
+    std::string solver;
+    bool mhd;
+    bool internal_energy;
+  };
+  // EnzoRiemann::construct_riemann (EnzoRiemann.cpp:26-68)
+  static EnzoRiemann *construct_riemann(const FactoryArgs &factory_args) noexcept;
+
+  EnzoRiemann(std::string solver, bool mhd);
+  virtual ~EnzoRiemann() = default;
+
+  // EnzoRiemann.hpp:125-129 (interface_velocity: dual energy only)
+  void solve(const EnzoEFltArrayMap &prim_map_l,
+             const EnzoEFltArrayMap &prim_map_r, EnzoEFltArrayMap &flux_map,
+             const int dim, const int stale_depth,
+             const str_vec_t &passive_list,
+             const EFlt3DArray *const interface_velocity) const;
+  // EnzoRiemann.hpp:134-141
+  const str_vec_t &integration_quantity_keys() const { return integration_keys_; }
+  const str_vec_t &primitive_quantity_keys() const { return primitive_keys_; }
+  bool mhd() const { return mhd_; }
+
+ private:
+  std::string solver_;
+  bool mhd_;
+  str_vec_t integration_keys_, primitive_keys_;
+};
+
+// ------------------------------------------------- constrained transport
+
+// EnzoBfieldMethod (toolkit/EnzoBfieldMethod.{hpp,cpp}): the partial
+// timestep state machine (:32-63).
+class EnzoBfieldMethod {
+ public:
+  explicit EnzoBfieldMethod(int num_partial_timesteps);
+  virtual ~EnzoBfieldMethod() = default;
+  void register_target_block(Block *block) noexcept;
+  void increment_partial_timestep() noexcept;
+  int num_partial_timesteps() const { return num_partial_timesteps_; }
+  int partial_timestep_index() const { return partial_timestep_index_; }
+
+  virtual void check_required_fields() const noexcept = 0;
+  virtual void correct_reconstructed_bfield(EnzoEFltArrayMap &l_map,
+                                            EnzoEFltArrayMap &r_map, int dim,
+                                            int stale_depth) noexcept = 0;
+  virtual void identify_upwind(const EnzoEFltArrayMap &flux_map, int dim,
+                               int stale_depth) noexcept = 0;
+  virtual void update_all_bfield_components(
+      EnzoEFltArrayMap &cur_integration_map, const EnzoEFltArrayMap &xflux_map,
+      const EnzoEFltArrayMap &yflux_map, const EnzoEFltArrayMap &zflux_map,
+      EnzoEFltArrayMap &out_centered_bfield_map, enzo_float dt,
+      int stale_depth) noexcept = 0;
+
+ protected:
+  virtual void register_target_block_(Block *target_block,
+                                      bool first_initialization) noexcept = 0;
+  void require_registered_block_() const noexcept;
+  Block *target_block_ = nullptr;
+
+ private:
+  int num_partial_timesteps_;
+  int partial_timestep_index_ = -1;
+};
+
+class EnzoBfieldMethodCT : public EnzoBfieldMethod {
+ public:
+  explicit EnzoBfieldMethodCT(int num_partial_timesteps);
+
+  void check_required_fields() const noexcept override {}
+  void correct_reconstructed_bfield(EnzoEFltArrayMap &l_map,
+                                    EnzoEFltArrayMap &r_map, int dim,
+                                    int stale_depth) noexcept override;
+  void identify_upwind(const EnzoEFltArrayMap &flux_map, int dim,
+                       int stale_depth) noexcept override;
+  void update_all_bfield_components(EnzoEFltArrayMap &cur_integration_map,
+                                    const EnzoEFltArrayMap &xflux_map,
+                                    const EnzoEFltArrayMap &yflux_map,
+                                    const EnzoEFltArrayMap &zflux_map,
+                                    EnzoEFltArrayMap &out_centered_bfield_map,
+                                    enzo_float dt,
+                                    int stale_depth) noexcept override;
+
+  // static physics methods of EnzoBfieldMethodCT.hpp
+  static void compute_center_efield(const int dim, EFlt3DArray &efield,
+                                    const EnzoEFltArrayMap &integration_map,
+                                    int stale_depth = 0);
+  static void compute_edge_efield(int dim, const EFlt3DArray &center_efield,
+                                  const EFlt3DArray &edge_efield,
+                                  const EnzoEFltArrayMap &jflux_map,
+                                  const EnzoEFltArrayMap &kflux_map,
+                                  const std::array<EFlt3DArray, 3> &weight_l,
+                                  int stale_depth = 0);
+  static void compute_all_edge_efields(const EnzoEFltArrayMap &integration_map,
+                                       const EnzoEFltArrayMap &xflux_map,
+                                       const EnzoEFltArrayMap &yflux_map,
+                                       const EnzoEFltArrayMap &zflux_map,
+                                       EFlt3DArray &center_efield,
+                                       std::array<EFlt3DArray, 3> &edge_efield_l,
+                                       const std::array<EFlt3DArray, 3> &weight_l,
+                                       int stale_depth = 0);
+  static void update_bfield(const enzo_float *cell_widths, int dim,
+                            const std::array<EFlt3DArray, 3> &efield_l,
+                            const EFlt3DArray &cur_interface_bfield,
+                            const EFlt3DArray &out_interface_bfield,
+                            enzo_float dt, int stale_depth = 0);
+  static void compute_center_bfield(int dim, const EFlt3DArray &bfieldc_comp,
+                                    const EFlt3DArray &bfieldi_comp,
+                                    int stale_depth = 0);
+
+ protected:
+  void register_target_block_(Block *target_block,
+                              bool first_initialization) noexcept override;
+
+ private:
+  std::array<EFlt3DArray, 3> bfieldi_l_, temp_bfieldi_l_, weight_l_,
+      edge_efield_l_;
+  EFlt3DArray center_efield_;
+  const enzo_float *cell_widths_ = nullptr;
+};
+
+// ---------------------------------------------------- quantity updating
+
+class EnzoIntegrationQuanUpdate {
+ public:
+  // EnzoIntegrationQuanUpdate.cpp:61-79
+  EnzoIntegrationQuanUpdate(const str_vec_t &integration_quantity_keys,
+                            const bool skip_B_update) throw();
+  void clear_dUcons_map(EnzoEFltArrayMap &dUcons_map, enzo_float value,
+                        const str_vec_t &passive_list) const noexcept;
+  void accumulate_flux_component(int dim, double dt, enzo_float cell_width,
+                                 const EnzoEFltArrayMap &flux_map,
+                                 EnzoEFltArrayMap &dUcons_map, int stale_depth,
+                                 const str_vec_t &passive_list) const noexcept;
+  void update_quantities(EnzoEFltArrayMap &initial_integration_map,
+                         const EnzoEFltArrayMap &dUcons_map,
+                         EnzoEFltArrayMap &out_integration_map,
+                         const int stale_depth,
+                         const str_vec_t &passive_list) const;
+  const str_vec_t &integration_keys() const { return integration_keys_; }
+
+ private:
+  str_vec_t integration_keys_;
+};
+
+// ------------------------------------------------------- stage commands
+
+// EnzoMHDIntegratorStageCommands.hpp: arguments of the integrator
+struct EnzoMHDIntegratorStageArgPack {
+  std::string rsolver;
+  str_vec_t recon_names;
+  double theta_limiter;
+  std::string mhd_choice;
+};
+
+class EnzoMHDIntegratorStageCommands {
+ public:
+  enum class bfield_choice { no_bfield, unsafe_const_uniform, constrained_transport };
+
+  explicit EnzoMHDIntegratorStageCommands(const EnzoMHDIntegratorStageArgPack &args);
+  ~EnzoMHDIntegratorStageCommands();
+  EnzoMHDIntegratorStageCommands(const EnzoMHDIntegratorStageCommands &) = delete;
+  EnzoMHDIntegratorStageCommands &operator=(const EnzoMHDIntegratorStageCommands &) = delete;
+
+  // EnzoMHDIntegratorStageCommands.cpp:102-203
+  void compute_update_stage(EnzoEFltArrayMap tstep_begin_integration_map,
+                            EnzoEFltArrayMap cur_stage_integration_map,
+                            EnzoEFltArrayMap out_integration_map,
+                            EnzoEFltArrayMap primitive_map,
+                            EnzoEFltArrayMap priml_map,
+                            EnzoEFltArrayMap primr_map,
+                            std::array<EnzoEFltArrayMap, 3> flux_maps_xyz,
+                            EnzoEFltArrayMap dUcons_map,
+                            const str_vec_t &passive_list,
+                            EnzoBfieldMethod *bfield_method,
+                            unsigned short stage_index, double cur_dt,
+                            int stale_depth,
+                            const std::array<enzo_float, 3> cell_widths_xyz) const noexcept;
+  // EnzoMHDIntegratorStageCommands.cpp:299-366 (no courant factor)
+  double timestep(EnzoEFltArrayMap integration_map, const EFlt3DArray &pressure,
+                  const double dx, const double dy, const double dz) const noexcept;
+
+  int staling_from_stage(int stage_index) const noexcept;
+  EnzoBfieldMethod *construct_bfield_method(int num_partial_timesteps) const noexcept;
+  str_vec_t integration_quantity_keys() const noexcept;
+  str_vec_t primitive_quantity_keys() const noexcept;
+  str_vec_t dUcons_map_keys() const noexcept;
+  bool is_pure_hydro() const noexcept { return mhd_choice_ == bfield_choice::no_bfield; }
+
+ private:
+  static bfield_choice parse_bfield_choice_(std::string choice) noexcept;
+  void check_valid_stage_index_(int stage_index) const noexcept;
+  void compute_flux_(const int dim, const double cur_dt,
+                     const enzo_float cell_width, EnzoEFltArrayMap &primitive_map,
+                     EnzoEFltArrayMap &priml_map, EnzoEFltArrayMap &primr_map,
+                     EnzoEFltArrayMap &flux_map, EnzoEFltArrayMap &dUcons_map,
+                     EnzoReconstructor &reconstructor,
+                     EnzoBfieldMethod *bfield_method, const int stale_depth,
+                     const str_vec_t &passive_list) const noexcept;
+
+  EnzoRiemann *riemann_solver_ = nullptr;
+  std::vector<std::unique_ptr<EnzoReconstructor>> reconstructors_;
+  EnzoIntegrationQuanUpdate *integration_quan_updater_ = nullptr;
+  bfield_choice mhd_choice_;
+};
+
+// EnzoVlctScratchSpace (EnzoMethodMHDVlct.hpp:244-273)
+struct EnzoVlctScratchSpace {
+  EnzoVlctScratchSpace(const std::array<int, 3> &field_shape,
+                       const str_vec_t &integration_keys,
+                       const str_vec_t &primitive_keys,
+                       const str_vec_t &dUcons_keys);
+  EnzoEFltArrayMap temp_integration_map, primitive_map, priml_map, primr_map,
+      xflux_map, yflux_map, zflux_map, dUcons_map;
+};
+
+// ----------------------------------------------------------------- method
+
+class EnzoMethodMHDVlct : public Method {
+ public:
+  // EnzoMethodMHDVlct.cpp:89-150.  Parameters (defaults as the reference):
+  // mhd_choice (required), riemann_solver ("hlld"), reconstruct_method
+  // ("plm"), theta_limiter (1.5), time_scheme ("vl"), courant (0.3 for
+  // vl, 1.0 for euler).  Device extensions: "device_fused" (true: one fused
+  // library launch sequence per compute; false: the stage-by-stage plugin
+  // composition below) and "precision" ("double" or "mixed", fused only).
+  EnzoMethodMHDVlct(ParameterGroup p, bool store_fluxes_for_corrections);
+  ~EnzoMethodMHDVlct() override;
+
+  // EnzoMethodMHDVlct.cpp:356-500; calls block->compute_done() once
+  void compute(Block *block) throw() override;
+  // several blocks sharing dt in one batched launch; compute_done is called
+  // on each block once the launch sequence is enqueued
+  void compute_blocks(const std::vector<Block *> &blocks) throw();
+  std::string name() throw() override { return "mhd_vlct"; }
+  // EnzoMethodMHDVlct.cpp:551-588
+  double timestep(Block *block) throw() override;
+
+  bool device_fused() const { return fused_; }
+
+ private:
+  EnzoEFltArrayMap get_integration_map_(Block *block) const noexcept;
+  EnzoVlctScratchSpace *get_scratch_ptr_(const std::array<int, 3> &field_shape) noexcept;
+  void compute_staged_(Block *block) noexcept;
+
+  std::string time_scheme_;
+  EnzoMHDIntegratorStageArgPack *integrator_arg_pack_ = nullptr;
+  EnzoMHDIntegratorStageCommands *integrator_ = nullptr;
+  EnzoVlctScratchSpace *scratch_space_ = nullptr;
+  EnzoBfieldMethod *bfield_method_ = nullptr;
+  str_vec_t integration_field_list_, primitive_field_list_;
+  bool fused_ = true;
+  int precision_ = ENZOE_PREC_DOUBLE;
+  enzoe_vlct_t device_ = nullptr;
+};

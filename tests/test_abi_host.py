@@ -1,0 +1,169 @@
+"""CPU checks of the two shared libraries: every function declared in
+include/*.h is exported, configurations are validated as the reference
+validates them (error codes at the C ABI; message + exit(1) in the C++
+drop-in layer, as the reference's ERROR/ASSERT), and the single-operation
+entry points reject malformed views before launching anything.  Nothing here
+touches a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "enzo-e_amd"))
+import enzoe_vlct as ev  # noqa: E402
+
+ERR_CONFIG, ERR_ARG = 1, 2
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(enzoe_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,path", [
+    ("enzoe_vlct.h", ev.LIB_PATH), ("enzoe_vlct_host.h", ev.HOST_LIB_PATH)])
+def test_library_exports_every_declared_symbol(header, path):
+    names = declared(header)
+    assert len(names) > 5
+    L = ctypes.CDLL(path)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def _create(**kw):
+    p = ev.Params()
+    p.mhd_choice = kw.get("mhd_choice", b"constrained_transport")
+    p.riemann_solver = kw.get("riemann", b"hlld")
+    p.reconstruct_method = kw.get("recon", b"plm")
+    p.time_scheme = kw.get("time_scheme", b"vl")
+    p.theta_limiter = kw.get("theta", 2.0)
+    p.courant = kw.get("courant", -1.0)
+    p.gamma = kw.get("gamma", 5.0 / 3.0)
+    p.density_floor = kw.get("dfloor", 1e-200)
+    p.pressure_floor = kw.get("pfloor", 1e-200)
+    p.precision = kw.get("precision", 0)
+    h = ctypes.c_void_p()
+    rc = ev.lib().enzoe_vlct_create(ctypes.byref(p), ctypes.byref(h))
+    return rc, h
+
+
+@pytest.mark.parametrize("kw", [
+    dict(riemann=b"hll"), dict(riemann=b"roe"), dict(riemann=b"hllc"),
+    dict(mhd_choice=b"no_bfield", riemann=b"hlld"),
+    dict(mhd_choice=b"no_bfield", riemann=b"hlle"),
+    dict(mhd_choice=b"unsafe_constant_uniform"), dict(mhd_choice=None),
+    dict(recon=b"ppm"), dict(time_scheme=b"rk3"), dict(theta=2.5), dict(theta=0.5),
+    dict(gamma=1.0), dict(dfloor=0.0), dict(precision=7),
+    dict(time_scheme=b"euler"),  # CT needs two partial timesteps
+])
+def test_abi_rejects_configurations_the_reference_rejects(kw):
+    rc, h = _create(**kw)
+    assert rc == ERR_CONFIG and not h.value
+
+
+def test_abi_accepts_and_defaults_courant():
+    rc, h = _create()
+    assert rc == 0
+    assert ev.lib().enzoe_vlct_courant(h) == 0.3  # EnzoMethodMHDVlct.cpp:96
+    ev.lib().enzoe_vlct_destroy(h)
+    rc, h = _create(mhd_choice=b"no_bfield", riemann=b"HLLC", recon=b"PLM_Athena",
+                    time_scheme=b"euler", courant=0.7)
+    assert rc == 0 and ev.lib().enzoe_vlct_courant(h) == 0.7
+    ev.lib().enzoe_vlct_destroy(h)
+
+
+def test_staling_rates():
+    L = ev.ops()
+    t, i = ctypes.c_int(), ctypes.c_int()
+    for name, rates in [(b"nn", (1, 0)), (b"plm", (2, 1)), (b"PLM_ENZO", (2, 1)),
+                        (b"plm_athena", (2, 1))]:
+        assert L.enzoe_reconstructor_staling(name, ctypes.byref(t), ctypes.byref(i)) == 0
+        assert (t.value, i.value) == rates
+    assert L.enzoe_reconstructor_staling(b"weno5", ctypes.byref(t), ctypes.byref(i)) \
+        == ERR_CONFIG
+
+
+def test_single_operations_validate_views_without_launching():
+    """Wrong shapes / missing keys return ERR_ARG (no device call is made, so
+    the fake addresses are never touched)."""
+    L = ev.ops()
+    fake = 1 << 40
+    keys = {k: fake + 8 * i for i, k in enumerate(
+        ["density", "velocity_x", "velocity_y", "velocity_z", "pressure",
+         "bfield_x", "bfield_y", "bfield_z"])}
+    m = (12, 11, 10)
+    prim = ev.fmap(keys, m)
+    face_x = ev.fmap(keys, (12, 11, 9), stride=(110, 10, 1))
+    wrong = ev.fmap(keys, (12, 10, 10))
+    assert L.enzoe_reconstruct_interface(b"plm", 2.0, 1e-200, 1e-200, prim, wrong,
+                                         wrong, 0, 0, None) == ERR_ARG
+    assert L.enzoe_reconstruct_interface(b"weno", 2.0, 1e-200, 1e-200, prim, face_x,
+                                         face_x, 0, 0, None) == ERR_CONFIG
+    assert L.enzoe_riemann_solve(b"hlld", 0, 5 / 3, face_x, face_x, face_x, 0, 0,
+                                 None) == ERR_CONFIG
+    partial = ev.fmap({"density": fake}, (12, 11, 9))
+    assert L.enzoe_riemann_solve(b"hlld", 1, 5 / 3, face_x, face_x, partial, 0, 0,
+                                 None) == ERR_ARG
+    bfi_bad = ev.farray(fake, (12, 11, 10))
+    assert L.enzoe_ct_correct_reconstructed_bfield(0, bfi_bad, face_x, face_x, 0,
+                                                   None) == ERR_ARG
+    assert L.enzoe_ct_identify_upwind(3, face_x, ev.farray(fake, (12, 11, 9)), 0,
+                                      None) == ERR_ARG
+    assert L.enzoe_accumulate_flux_component(0, 0.1, 0.1, wrong, prim, 1,
+                                             None) == ERR_ARG
+    assert L.enzoe_update_quantities(prim, prim, partial, 1, 5 / 3, 1e-200, 1e-200,
+                                     2, None) == ERR_ARG
+
+
+def _run_host(params):
+    code = ("import sys; sys.path.insert(0, %r); import enzoe_vlct as ev; "
+            "m = ev.HostMethod(%r); print('constructed', m.name, m.courant)"
+            % (os.path.join(ROOT, "enzo-e_amd"), params))
+    return subprocess.run([sys.executable, "-c", code], capture_output=True,
+                          text=True, timeout=120)
+
+
+def test_host_method_defaults():
+    r = _run_host("mhd_choice = constrained_transport")
+    assert r.returncode == 0, r.stderr
+    assert "constructed mhd_vlct 0.3" in r.stdout
+    r = _run_host('mhd_choice = "no_bfield"; riemann_solver = "HLLC"; '
+                  'time_scheme = "euler"; reconstruct_method = "plm_athena"')
+    assert r.returncode == 0, r.stderr
+    assert "constructed mhd_vlct 1.0" in r.stdout
+    assert "USE WITH CAUTION" in r.stderr
+
+
+@pytest.mark.parametrize("params,message", [
+    ("riemann_solver = hlld", "mhd_choice wasn't specified"),
+    ("mhd_choice = constrained_transport; riemann_solver = roe",
+     "The only known solvers are HLL, HLLE, HLLC, & HLLD"),
+    ("mhd_choice = constrained_transport; riemann_solver = hllc",
+     "can't support mhd"),
+    ("mhd_choice = no_bfield; riemann_solver = hlld", "requires magnetic fields"),
+    ("mhd_choice = no_bfield; riemann_solver = hlle", "untested"),
+    ("mhd_choice = constrained_transport; riemann_solver = hll", "HLL"),
+    ("mhd_choice = constrained_transport; theta_limiter = 2.5",
+     "theta_limiter must satisfy 1<=theta_limiter<=2"),
+    ("mhd_choice = constrained_transport; reconstruct_method = ppm",
+     "The only allowed solvers are NN, PLM, PLM_ENZO, & PLM_ATHENA"),
+    ("mhd_choice = constrained_transport; time_scheme = rk3",
+     "must be \"vl\" or \"euler\""),
+    ("mhd_choice = constrained_transport; half_dt_reconstruct_method = nn",
+     "have been removed"),
+    ("mhd_choice = magnetic", "Unrecognized choice"),
+    ("mhd_choice = constrained_transport; time_scheme = euler",
+     "num_partial_timesteps!=2"),
+    ("mhd_choice = constrained_transport; device_fused = false; precision = mixed",
+     "mixed precision is only provided by the fused device path"),
+])
+def test_host_method_errors_exit_like_the_reference(params, message):
+    r = _run_host(params)
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert "ERROR in" in r.stderr and message in r.stderr, r.stderr
+    assert "constructed" not in r.stdout
