@@ -43,9 +43,11 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="active cells per axis")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the run at the other precision")
     ap.add_argument("--cpu-n", type=int, default=128,
                     help="edge of the Orszag-Tang block timed on the CPU")
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (HBM counter scale)")
     return ap.parse_args()
@@ -141,6 +143,46 @@ def cpu_baseline(n, nsteps):
                       "steps, %.1f s" % (n, nsteps, el)}
 
 
+METRIC = "zone-updates/s/GPU (batched FP32-mixed vs plain FP64 VL+CT) + HBM GB/s"
+
+
+def timed_run(precision, n, steps, warmup):
+    """Warm-up, then `steps` timed steps with per-kernel HIP-event timing."""
+    prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
+    method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
+                           theta=2.0, courant=0.4, gamma=5.0 / 3.0,
+                           precision=prec)
+    grid = make_state(n, method)
+    for _ in range(warmup):
+        step(grid)
+    ev.synchronize()
+    ev.profiling(True)
+    ev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(grid)
+    ev.synchronize()
+    el = time.perf_counter() - t0
+    prof = ev.profile_report()
+    ev.profiling(False)
+    del grid, method
+    zones = n ** 3
+    m = n + 6
+    # dominant kernel: the flux kernel of the full (PLM) stage
+    dom = "k_flux_plm"
+    dom_ms = prof.get(dom, (float("nan"), 0))[0]
+    cells = (m - 4) ** 3  # flux box of the PLM stage: [2, m-2)^3
+    dom_bytes = FLUX_BYTES_PER_CELL[prec] * cells
+    gpu_step_ms = sum(v[0] * v[1] for v in prof.values()) / steps
+    return {"precision": precision, "value": zones * steps / el,
+            "ms_per_step": el * 1e3 / steps, "prof": prof, "dom": dom,
+            "dom_ms": dom_ms, "dom_bytes": dom_bytes,
+            "achieved": dom_bytes / (dom_ms * 1e-3) / 1e9,
+            "gpu_step_ms": gpu_step_ms,
+            "step_gbs": STEP_BYTES_PER_ZONE_FP64 * zones / (gpu_step_ms * 1e-3) / 1e9,
+            "workload": "orszag_tang_%d3_%s_vlct_plm_hlld" % (n, precision)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -153,76 +195,60 @@ def main():
     if a.copy_calibration:
         print(json.dumps(copy_bandwidth()))
         return
-    prec = ev.ENZOE_PREC_DOUBLE if a.precision == "fp64" else ev.ENZOE_PREC_MIXED
-    method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
-                           theta=2.0, courant=0.4, gamma=5.0 / 3.0,
-                           precision=prec)
-    grid = make_state(a.n, method)
-    for _ in range(a.warmup):
-        step(grid)
-    ev.synchronize()
-
-    ev.profiling(True)
-    ev.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step(grid)
-    ev.synchronize()
-    el = time.perf_counter() - t0
-    prof = ev.profile_report()
-    ev.profiling(False)
-
+    r = timed_run(a.precision, a.n, a.steps, a.warmup)
+    other = None
+    if not a.no_compare:
+        other = timed_run("mixed" if a.precision == "fp64" else "fp64", a.n,
+                          a.steps, a.warmup)
     copy = copy_bandwidth()
-    zones = a.n ** 3
-    value = zones * a.steps / el
-    m = a.n + 6
-    # dominant kernel: the flux kernel of the full (PLM) stage
-    dom = "k_flux_plm"
-    dom_ms = prof.get(dom, (float("nan"), 0))[0]
-    cells = (m - 4) ** 3  # flux box of the PLM stage: [2, m-2)^3
-    dom_bytes = FLUX_BYTES_PER_CELL[prec] * cells
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    gpu_step_ms = sum(v[0] * v[1] for v in prof.values()) / a.steps
-    step_gbs = STEP_BYTES_PER_ZONE_FP64 * zones / (gpu_step_ms * 1e-3) / 1e9
-
-    workload = "orszag_tang_%d3_%s_vlct_plm_hlld" % (a.n, a.precision)
     out = {
-        "metric": "zone-updates/s/GPU (VL+CT MHD step)",
-        "value": value,
+        "metric": METRIC,
+        "value": r["value"],
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": el * 1e3 / a.steps,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == 0 else "f32-riemann/f64-update",
+        "dtype": "f64" if a.precision == "fp64" else "f32-riemann/f64-update",
         "data": "synthetic (3D Orszag-Tang vortex, seeded 1e-3 v_z noise)",
-        "config": {"workload": workload,
+        "config": {"workload": r["workload"],
                    "block": [a.n, a.n, a.n], "ghost": 3, "riemann": "hlld",
                    "reconstruct": "plm", "theta": 2.0, "courant": 0.4,
                    "precision": a.precision},
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
-            "achieved": achieved,
+            "kernel": r["dom"],
+            "achieved": r["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, workload),
-            "bytes_per_launch": dom_bytes,
-            "avg_launch_ms": dom_ms,
+            "frac": r["achieved"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(r["dom"], r["workload"]),
+            "bytes_per_launch": r["dom_bytes"],
+            "avg_launch_ms": r["dom_ms"],
         },
         "step_roofline": {
             "bytes_per_zone": STEP_BYTES_PER_ZONE_FP64,
-            "gpu_ms_per_step": gpu_step_ms,
-            "achieved_GBs": step_gbs,
-            "frac": step_gbs / HBM_PEAK_GBS,
+            "gpu_ms_per_step": r["gpu_step_ms"],
+            "achieved_GBs": r["step_gbs"],
+            "frac": r["step_gbs"] / HBM_PEAK_GBS,
         },
-        "kernels_ms": {k: round(v[0], 4) for k, v in sorted(prof.items())},
+        "kernels_ms": {k: round(v[0], 4) for k, v in sorted(r["prof"].items())},
         "copy_kernel_GBs": copy["GBs"],
     }
+    if other is not None:
+        fp64, mixed = (r, other) if a.precision == "fp64" else (other, r)
+        out["precision_comparison"] = {
+            "fp64_value": fp64["value"], "mixed_value": mixed["value"],
+            "mixed_over_fp64": mixed["value"] / fp64["value"],
+            "mixed_ms_per_step": mixed["ms_per_step"],
+            "mixed_kernels_ms": {k: round(v[0], 4)
+                                 for k, v in sorted(mixed["prof"].items())},
+            "mixed_dominant_GBs": mixed["achieved"],
+            "mixed_traffic": measured_traffic(mixed["dom"], mixed["workload"]),
+        }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)
     print(json.dumps(out))
