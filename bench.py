@@ -33,6 +33,11 @@ STEP_BYTES_PER_ZONE_FP64 = 440.0
 # Algorithmic bytes per cell of one flux launch: it must read the 11-word
 # state of the cell and write 7 flux components on each of its 3 low faces.
 FLUX_BYTES_PER_CELL = {0: (11 + 21) * 8.0, 1: 11 * 8.0 + 21 * 4.0}
+# hydro: read 5 words, write 5 fluxes on 3 faces; step: 25 words per zone
+FLUX_BYTES_PER_CELL_HD = {0: (5 + 15) * 8.0, 1: 5 * 8.0 + 15 * 4.0}
+STEP_BYTES_PER_ZONE_HD = 200.0
+WORKLOADS = ["orszag_tang", "fast_wave", "batched_amr", "turbulence",
+             "kelvin_helmholtz"]
 
 
 def parse():
@@ -48,6 +53,12 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=128,
                     help="edge of the Orszag-Tang block timed on the CPU")
     ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--workload", choices=WORKLOADS, default="orszag_tang",
+                    help="orszag_tang (default): one 256^3 block; fast_wave: "
+                         "32^3 fast eigenmode for one period; batched_amr: "
+                         "512/1536/3000 blocks of 32^3 on 3 levels with 2:1 "
+                         "sub-cycling; turbulence: 512 blocks of 64^3, Mach 5; "
+                         "kelvin_helmholtz: 512 blocks of 128^3, hydro")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (HBM counter scale)")
     return ap.parse_args()
@@ -183,6 +194,183 @@ def timed_run(precision, n, steps, warmup):
             "workload": "orszag_tang_%d3_%s_vlct_plm_hlld" % (n, precision)}
 
 
+def _roofline(prof, steps, cells_per_step, prec, mhd, workload):
+    """Dominant-kernel roofline: algorithmic flux-kernel bytes of a step
+    over the flux kernel's HIP-event time in that step."""
+    dom = "k_flux_plm"
+    avg_ms, count = prof.get(dom, (float("nan"), 0))
+    per_step_ms = avg_ms * count / steps
+    table = FLUX_BYTES_PER_CELL if mhd else FLUX_BYTES_PER_CELL_HD
+    nbytes = table[prec] * cells_per_step
+    achieved = nbytes / (per_step_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(dom, workload),
+            "bytes_per_step": nbytes, "kernel_ms_per_step": per_step_ms}
+
+
+def _profiled(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    ev.synchronize()
+    ev.profiling(True)
+    ev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    ev.synchronize()
+    el = time.perf_counter() - t0
+    prof = ev.profile_report()
+    ev.profiling(False)
+    return el, prof
+
+
+def run_fast_wave(a):
+    """Config 1: 32^3 fast eigenmode along (1,1,1), evolved one period."""
+    n = 32
+    blk, cw, cf, period = problems.fast_wave_111(n)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
+        method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
+                               theta=2.0, courant=0.4, precision=prec)
+        grid = ev.DeviceGrid(method, (1, 1, 1), (n, n, n), [cw])
+        grid.upload_from([blk])
+        grid.refresh()
+        ev.synchronize()
+        t0 = time.perf_counter()
+        ncycles, _ = grid.evolve(0.0, period)
+        ev.synchronize()
+        el = time.perf_counter() - t0
+        fin = grid.blocks[0].download(["density", "velocity_x", "total_energy"])
+        s = (slice(3, -3),) * 3
+        l1 = {k: float(np.abs(fin[k][s] - blk[k][s]).mean()) for k in fin}
+        out[precision] = {"value": n ** 3 * ncycles / el, "cycles": ncycles,
+                          "seconds": el, "l1_vs_initial": l1}
+    return {
+        "metric": METRIC, "value": out["fp64"]["value"], "unit": "zone-updates/s",
+        "n_gpus": 1, "steps": out["fp64"]["cycles"], "warmup": 0,
+        "ms_per_step": out["fp64"]["seconds"] * 1e3 / out["fp64"]["cycles"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (fast magnetosonic eigenmode, amplitude 1e-6)",
+        "config": {"workload": "fast_wave_32_one_period", "block": [n, n, n],
+                   "riemann": "hlld", "reconstruct": "plm", "theta": 2.0,
+                   "courant": 0.4, "period": period, "fast_speed": cf},
+        "precision_comparison": {"fp64": out["fp64"], "mixed": out["mixed"]},
+    }
+
+
+def run_batched_amr(a):
+    """Config 3: 512 / 1536 / 3000 independent 32^3 blocks on three levels,
+    2:1 sub-cycling (launches of 5048, 3000, 4536, 3000 blocks per coarse
+    step), one batched launch sequence per substep with per-block dt."""
+    import workloads
+    res = {}
+    for precision in (a.precision, "mixed" if a.precision == "fp64" else "fp64"):
+        prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
+        method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
+                               theta=2.0, courant=0.4, precision=prec)
+        lv = workloads.SubcycledLevels(method)
+        el, prof = _profiled(lv.coarse_step, a.steps, a.warmup)
+        zones = lv.zones_per_coarse_step()
+        cells = sum(34 ** 3 * c * 2 ** l for l, c in enumerate(lv.counts))
+        res[precision] = {"value": zones * a.steps / el,
+                          "ms_per_step": el * 1e3 / a.steps,
+                          "kernels_ms": {k: round(v[0], 4) for k, v in sorted(prof.items())},
+                          "roofline": _roofline(prof, a.steps, cells, prec, True,
+                                                "batched_amr_%s" % precision),
+                          "launch_blocks": lv.launch_sizes()}
+        del lv, method
+        if a.no_compare:
+            break
+    r = res[a.precision]
+    out = {
+        "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32-riemann/f64-update",
+        "data": "synthetic (seeded random smooth MHD blocks: lognormal rho, Mach 2, beta 1)",
+        "config": {"workload": "batched_amr_32_512_1536_3000_%s" % a.precision,
+                   "blocks_per_level": [512, 1536, 3000], "block": [32, 32, 32],
+                   "subcycling": "2:1", "launch_blocks": r["launch_blocks"]},
+        "roofline": r["roofline"], "kernels_ms": r["kernels_ms"],
+    }
+    if len(res) == 2:
+        f, m = res["fp64"], res["mixed"]
+        out["precision_comparison"] = {"fp64_value": f["value"], "mixed_value": m["value"],
+                                       "mixed_over_fp64": m["value"] / f["value"],
+                                       "mixed_kernels_ms": m["kernels_ms"]}
+    return out
+
+
+def run_uniform(a):
+    """Configs 4 and 5: periodic grids of 512 blocks (turbulence 64^3 MHD,
+    Kelvin-Helmholtz 128^3 hydro); the primary precision is timed, the other
+    one run beside it, and the relative L1 between them reported."""
+    import workloads
+    if a.workload == "turbulence":
+        nb, n, mhd, gamma, rs = (8, 8, 8), 64, True, 5.0 / 3.0, "hlld"
+        spec = problems.turbulence_spectrum()
+        norm = problems.turbulence_vrms_norm(spec, mach=5.0)
+        builder = lambda ib: problems.turbulence_block(spec, norm, ib, nb=nb, n=n)
+        data = "synthetic (k^-2 solenoidal velocity k=1..8, Mach 5, beta 0.1)"
+    else:
+        nb, n, mhd, gamma, rs = (8, 8, 8), 128, False, 1.4, "hllc"
+        builder = lambda ib: problems.kelvin_helmholtz_block(ib, nb=nb, n=n)
+        data = "synthetic (Kelvin-Helmholtz slab, density ratio 2, 1e-2 v_y noise)"
+    precs = [a.precision] + ([] if a.no_compare else
+                             ["mixed" if a.precision == "fp64" else "fp64"])
+    grids, res = {}, {}
+    first = None
+    for precision in precs:
+        prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
+        method = ev.VlctMethod(mhd=mhd, riemann=rs, reconstruct="plm", theta=2.0,
+                               courant=0.4, gamma=gamma, precision=prec)
+        if first is None:
+            g = workloads.uniform_grid(method, nb, n, builder)
+            first = g
+        else:
+            g = ev.DeviceGrid(method, nb, (n, n, n), first.blocks[0].desc.cell_width[:],
+                              bc=("periodic",) * 3)
+            workloads.copy_grid_state(first, g)
+        grids[precision] = g
+    for precision in precs:
+        g = grids[precision]
+        prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
+        el, prof = _profiled(lambda: step(g), a.steps, a.warmup)
+        nblk = nb[0] * nb[1] * nb[2]
+        cells = (n + 2) ** 3 * nblk
+        res[precision] = {"value": n ** 3 * nblk * a.steps / el,
+                          "ms_per_step": el * 1e3 / a.steps,
+                          "kernels_ms": {k: round(v[0], 4) for k, v in sorted(prof.items())},
+                          "roofline": _roofline(prof, a.steps, cells, prec, mhd,
+                                                "%s_%s" % (a.workload, precision))}
+    r = res[a.precision]
+    out = {
+        "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32-riemann/f64-update",
+        "data": data,
+        "config": {"workload": "%s_%dx%d3_%s" % (a.workload, nb[0] * nb[1] * nb[2], n,
+                                                 a.precision),
+                   "blocks": list(nb), "block": [n, n, n], "riemann": rs,
+                   "reconstruct": "plm", "theta": 2.0, "courant": 0.4, "gamma": gamma},
+        "roofline": r["roofline"], "kernels_ms": r["kernels_ms"],
+    }
+    if len(res) == 2:
+        f, m = res["fp64"], res["mixed"]
+        names = (["density", "velocity_x", "velocity_y", "velocity_z", "total_energy"]
+                 + (["bfield_x", "bfield_y", "bfield_z"] if mhd else []))
+        l1 = workloads.relative_l1(grids["mixed"], grids["fp64"], names)
+        out["precision_comparison"] = {
+            "fp64_value": f["value"], "mixed_value": m["value"],
+            "mixed_over_fp64": m["value"] / f["value"],
+            "mixed_kernels_ms": m["kernels_ms"], "fp64_kernels_ms": f["kernels_ms"],
+            "relative_l1_mixed_vs_fp64": l1, "max_relative_l1": max(l1.values())}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -194,6 +382,13 @@ def main():
         sys.exit(1)
     if a.copy_calibration:
         print(json.dumps(copy_bandwidth()))
+        return
+    if a.workload != "orszag_tang":
+        run = {"fast_wave": run_fast_wave, "batched_amr": run_batched_amr,
+               "turbulence": run_uniform, "kelvin_helmholtz": run_uniform}[a.workload]
+        out = run(a)
+        out["copy_kernel_GBs"] = copy_bandwidth()["GBs"]
+        print(json.dumps(out))
         return
     r = timed_run(a.precision, a.n, a.steps, a.warmup)
     other = None

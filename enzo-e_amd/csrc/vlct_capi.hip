@@ -139,6 +139,8 @@ int enzoe_vlct_create(const enzoe_vlct_params *p, enzoe_vlct_t *out) {
   {
     const char *e = std::getenv("ENZOE_XCD_REMAP");
     if (e && *e) m->xcd_remap = std::atoi(e) & 15;
+    const char *k = std::getenv("ENZOE_CT_KERNELS");
+    if (k && *k) m->ct_mode = (k[0] == '0') ? 0 : 1;
   }
   // no device call here: device memory is taken on first use, so a
   // configuration can be validated on a host without a GPU
@@ -152,36 +154,66 @@ int enzoe_vlct_destroy(enzoe_vlct_t m) {
   if (m->d_desc) (void)hipFree(m->d_desc);
   if (m->d_dt) (void)hipFree(m->d_dt);
   if (m->d_ptrtab) (void)hipFree(m->d_ptrtab);
+  if (m->d_dts) (void)hipFree(m->d_dts);
   delete m;
   return ENZOE_OK;
 }
 
 double enzoe_vlct_courant(enzoe_vlct_t m) { return m ? m->courant : -1.0; }
 
-int enzoe_vlct_compute(enzoe_vlct_t m, const enzoe_block *blocks, int n,
-                       double dt, enzoe_stream_t stream) {
-  if (!m || !blocks || n <= 0) return ENZOE_ERR_ARG;
-  hipStream_t s = (hipStream_t)stream;
+namespace {
+
+// blocks grouped into runs of equal shape, each run one compute_group
+int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
+                 double dt, const double *d_dts, hipStream_t s) {
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
-  // group consecutive blocks of equal shape
   int b0 = 0;
   while (b0 < n) {
     int b1 = b0 + 1;
     while (b1 < n && blocks[b1].mx == blocks[b0].mx &&
            blocks[b1].my == blocks[b0].my && blocks[b1].mz == blocks[b0].mz)
       b1++;
+    const double *dts = d_dts ? d_dts + b0 : nullptr;
     int rc;
     if (m->precision == ENZOE_PREC_DOUBLE) {
-      rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, s)
-                  : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, s);
+      rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s)
+                  : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, dts, s);
     } else {
-      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, s);
+      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, dts, s);
     }
     if (rc) return rc;
     b0 = b1;
   }
   return ENZOE_OK;
+}
+
+}  // namespace
+
+int enzoe_vlct_compute(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                       double dt, enzoe_stream_t stream) {
+  if (!m || !blocks || n <= 0) return ENZOE_ERR_ARG;
+  return compute_runs(m, blocks, n, dt, nullptr, (hipStream_t)stream);
+}
+
+int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                          const double *dt, int dt_on_device,
+                          enzoe_stream_t stream) {
+  if (!m || !blocks || n <= 0 || !dt) return ENZOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const double *d_dts = dt;
+  if (!dt_on_device) {
+    for (int i = 0; i < n; i++)
+      if (!(dt[i] >= 0.0)) return ENZOE_ERR_ARG;
+    if (int rc = ensure_bytes(&m->d_dts, &m->dts_bytes, n * sizeof(double))) return rc;
+    // the copy completes before the host array may change
+    if (hipMemcpyAsync(m->d_dts, dt, n * sizeof(double), hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return ENZOE_ERR_HIP;
+    d_dts = (const double *)m->d_dts;
+  }
+  return compute_runs(m, blocks, n, 0.0, d_dts, s);
 }
 
 int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,

@@ -49,6 +49,8 @@ struct StageParams {
   double density_floor;
   double pressure_floor;
   double theta;
+  const double *dts;   // per-block dt of the step (device) or null: sp.dt
+  double dt_factor;    // 0.5 for the partial step, 1 for the final one
   int xcd_remap;       // bit mask over k_flux(1) k_edge(2) k_faces(4)
                        // k_cells(8): XCD-contiguous order, see wg_coords
 };
@@ -342,7 +344,7 @@ k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int lo = cs + 1;
   if (!box_coords(t, lo, mx - cs - lo, my - cs - lo, mz - cs - lo, k, j, i))
     return;
-  const double dt = sp.dt;
+  const double dt = sp.dts ? sp.dts[by] * sp.dt_factor : sp.dt;
   const int p[3] = {i, j, k};
   const int m[3] = {mx, my, mz};
   const long str[3] = {1, (long)mx, (long)mx * my};
@@ -406,7 +408,7 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   if (k < ul || j < ul || i < ul || k >= mz - ul || j >= my - ul ||
       i >= mx - ul)
     return;
-  const double dt = sp.dt;
+  const double dt = sp.dts ? sp.dts[by] * sp.dt_factor : sp.dt;
   const long str[3] = {1, (long)mx, (long)mx * my};
   const int fq[5] = {S_RHO, S_VX, S_VY, S_VZ, S_E};
   double dU[5];

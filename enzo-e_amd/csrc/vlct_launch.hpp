@@ -10,9 +10,11 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <cmath>
 #include <vector>
 
 #include "../../include/enzoe_vlct.h"
+#include "vlct_ct.hpp"
 #include "vlct_kernels.hpp"
 
 namespace enzoe {
@@ -73,6 +75,9 @@ struct enzoe_vlct_method {
   // kernels launched in XCD-contiguous workgroup order (StageParams mask);
   // measured on the 256^3 Orszag-Tang block it pays for k_edge only
   int xcd_remap = 2;
+  // 1: edge+face kernel and z-marching cell kernel (vlct_ct.hpp);
+  // 0: the three plane kernels k_edge, k_faces, k_cells
+  int ct_mode = 1;
   double theta = 1.5, courant = 0.3, gamma = 5.0 / 3.0;
   double dfloor = 1e-200, pfloor = 1e-200;
 
@@ -86,6 +91,8 @@ struct enzoe_vlct_method {
   unsigned long long *d_dt = nullptr;
   void *d_ptrtab = nullptr;
   size_t ptrtab_bytes = 0;
+  void *d_dts = nullptr;  // per-block dt uploaded by enzoe_vlct_compute_dt
+  size_t dts_bytes = 0;
 };
 
 namespace enzoe {
@@ -226,6 +233,47 @@ void dispatch_flux(const enzoe_vlct_method *m, int recon,
   }
 }
 
+
+// Tile shape of k_ctface for a face box of nx * ny columns and nz planes:
+// the tx * ty (tx * ty <= 256, tx, ty <= 64) shape that wastes the fewest
+// thread slots, chunks of about 16 planes.
+inline CtTile ct_tile(int x0, int nx, int ny, int z0, int nz) {
+  CtTile best{};
+  double best_eff = -1.0;
+  const int cand[5] = {8, 16, 32, 64, nx};
+  for (int i = 0; i < 5; i++) {
+    int tx = cand[i];
+    if (tx < 4 || tx > 64) continue;
+    int ty = std::min(64, std::min(256 / tx, std::max(1, ny)));
+    if ((tx + 1) * (ty + 1) > kCtLds) continue;
+    int ntx = (nx + tx - 1) / tx, nty = (ny + ty - 1) / ty;
+    double eff = (double)nx * ny / ((double)ntx * nty * 256.0);
+    if (eff > best_eff + 1e-9 || (std::fabs(eff - best_eff) <= 1e-9 && tx > best.tx)) {
+      best_eff = eff;
+      best.tx = tx; best.ty = ty; best.ntx = ntx; best.nty = nty;
+    }
+  }
+  int nzc = std::max(1, (nz + 15) / 16);
+  best.zc = (nz + nzc - 1) / nzc;
+  best.nzc = (nz + best.zc - 1) / best.zc;
+  best.x0 = x0; best.y0 = x0; best.z0 = z0;
+  best.nx = nx; best.ny = ny; best.nz = nz;
+  return best;
+}
+
+inline ColShape col_shape(int lo, int mx, int my, int mz) {
+  ColShape s;
+  s.lo = lo;
+  s.ncol_x = mx - 2 * lo;
+  s.ncol_y = my - 2 * lo;
+  s.ncol = s.ncol_x * s.ncol_y;
+  const int nz = mz - 2 * lo;
+  int nzc = std::max(1, (nz + 15) / 16);
+  s.zc = (nz + nzc - 1) / nzc;
+  s.nzc = (nz + s.zc - 1) / s.zc;
+  return s;
+}
+
 inline long cube(int mx, int my, int mz, int lo, int hi_trim) {
   long a = mx - lo - hi_trim, b = my - lo - hi_trim, c = mz - lo - hi_trim;
   if (a <= 0 || b <= 0 || c <= 0) return 0;
@@ -233,23 +281,31 @@ inline long cube(int mx, int my, int mz, int lo, int hi_trim) {
 }
 
 // One VL+CT step (EnzoMethodMHDVlct::compute) on blocks of equal shape.
+// dts: per-block dt in device memory (n values) or null (all blocks use dt).
 template <typename FluxT, bool MHD>
 int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
-                  double dt, hipStream_t s) {
+                  double dt, const double *dts, hipStream_t s) {
   const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
   const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd);
-  // chunk size: as many blocks as fit a scratch budget of ~4 GiB (at least 1)
-  const size_t budget = (size_t)4 << 30;
+  // chunk size: as many blocks as fit the scratch budget (at least 1); the
+  // chunks run one after the other on the stream and share the pool
+  size_t budget = (size_t)4 << 30;
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      budget = std::max(budget, std::min((size_t)16 << 30, free_b / 3));
+  }
   int chunk = (int)std::max<size_t>(1, std::min<size_t>(n, budget / per));
   if (int rc = ensure_bytes(&m->scratch, &m->scratch_bytes, per * chunk)) return rc;
 
-  std::vector<BlockDesc<FluxT>> desc;
+  // descriptor table of every chunk, uploaded once (and only when changed)
+  std::vector<BlockDesc<FluxT>> desc, part;
   for (int c0 = 0; c0 < n; c0 += chunk) {
-    const int c1 = std::min(n, c0 + chunk);
-    const int nb = c1 - c0;
-    fill_desc<FluxT>(m, blocks, c0, c1, desc);
+    fill_desc<FluxT>(m, blocks, c0, std::min(n, c0 + chunk), part);
+    desc.insert(desc.end(), part.begin(), part.end());
+  }
+  {
     const size_t dbytes = desc.size() * sizeof(BlockDesc<FluxT>);
-    // upload the descriptor table only when it changed
     const char *raw = (const char *)desc.data();
     bool same = (m->last_desc.size() == dbytes) &&
                 std::memcmp(m->last_desc.data(), raw, dbytes) == 0;
@@ -262,7 +318,11 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       if (hipStreamSynchronize(s) != hipSuccess) return ENZOE_ERR_HIP;
       m->last_desc.assign(raw, raw + dbytes);
     }
-    const BlockDesc<FluxT> *d = (const BlockDesc<FluxT> *)m->d_desc;
+  }
+  for (int c0 = 0; c0 < n; c0 += chunk) {
+    const int c1 = std::min(n, c0 + chunk);
+    const int nb = c1 - c0;
+    const BlockDesc<FluxT> *d = (const BlockDesc<FluxT> *)m->d_desc + c0;
 
     int stale = 0;
     for (int stage = 0; stage < m->nstages; stage++) {
@@ -275,6 +335,8 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       sp.cur_stale = stale + immediate_staling(recon);
       // EnzoMethodMHDVlct.cpp:462: dt/2 for the partial step
       sp.dt = (!is_final) ? dt / 2. : dt;
+      sp.dts = dts ? dts + c0 : nullptr;
+      sp.dt_factor = (!is_final) ? 0.5 : 1.0;
       sp.gamma = m->gamma;
       sp.density_floor = m->dfloor;
       sp.pressure_floor = m->pfloor;
@@ -285,6 +347,26 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm");
         dispatch_flux<FluxT, MHD>(m, recon, d, nb, cube(mx, my, mz, cs, cs), sp, s);
       }
+      if (m->ct_mode == 1) {
+        if (MHD) {
+          KernelTimer kt(s, "k_ctface");
+          // face box [cs+1, m-cs) in x, y; planes [cs, mz-cs-1)
+          const CtTile T = ct_tile(cs + 1, mx - 2 * cs - 1, my - 2 * cs - 1, cs,
+                                   mz - 2 * cs - 1);
+          if (T.nx > 0 && T.ny > 0 && T.nz > 0)
+            hipLaunchKernelGGL((k_ctface<FluxT>), dim3(T.ntx * T.nty * T.nzc, nb),
+                               dim3(256), 0, s, d, sp, T);
+        }
+        {
+          KernelTimer kt(s, "k_cells");
+          const int lo = is_final ? cs : cs + 1;
+          const ColShape S = col_shape(lo, mx, my, mz);
+          if (S.ncol > 0 && S.nzc > 0)
+            hipLaunchKernelGGL((k_cells_z<FluxT, MHD>),
+                               dim3(((S.ncol + 255) / 256) * S.nzc, nb), dim3(256), 0,
+                               s, d, sp, S);
+        }
+      } else {
       if (MHD) {
         {
           KernelTimer kt(s, "k_edge");
@@ -305,6 +387,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                            dim3(nblk8(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
                            s, d, sp);
       }
+      }
       stale += total_staling(recon);
     }
   }
@@ -314,6 +397,6 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
 
 // FP32 flux path, compiled in vlct_mixed.hip
 int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
-                  double dt, hipStream_t s);
+                  double dt, const double *dts, hipStream_t s);
 
 }  // namespace enzoe

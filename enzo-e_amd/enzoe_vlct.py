@@ -120,6 +120,8 @@ def lib():
                                           vp, _dp]
         L.enzoe_vlct_compute.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                          ctypes.c_double, vp]
+        L.enzoe_vlct_compute_dt.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                            _dp, ctypes.c_int, vp]
         L.enzoe_grid_refresh.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), vp]
@@ -274,6 +276,15 @@ class VlctMethod:
     def compute(self, blocks, dt, stream=None):
         check(lib().enzoe_vlct_compute(self.handle, self._blocks(blocks), len(blocks),
                                        dt, stream), "enzoe_vlct_compute")
+
+    def compute_dt(self, blocks, dts, stream=None, block_array=None):
+        """One step with one dt per block (sub-cycled levels advanced
+        together).  `block_array` may be a prepared ctypes array of the
+        blocks' descriptors (it is rebuilt otherwise)."""
+        arr = block_array if block_array is not None else self._blocks(blocks)
+        d = (ctypes.c_double * len(dts))(*dts)
+        check(lib().enzoe_vlct_compute_dt(self.handle, arr, len(dts), d, 0, stream),
+              "enzoe_vlct_compute_dt")
 
     def refresh(self, blocks, nblocks, bc, stream=None):
         bcs = (ctypes.c_int * 3)(*[BC[b] if isinstance(b, str) else b for b in bc])

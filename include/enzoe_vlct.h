@@ -106,6 +106,16 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
 int enzoe_vlct_compute(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                        double dt, enzoe_stream_t stream);
 
+/* As enzoe_vlct_compute with one dt per block (blocks of different
+ * refinement levels advanced together under sub-cycling, each by its own
+ * level's dt; the reference advances every block by the global dt,
+ * control_stopping.cpp:94-141).  dt[n] is host memory, or device memory when
+ * dt_on_device is nonzero (e.g. written by a previous kernel, so a whole
+ * step can run without a host round trip).                                  */
+int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                          const double *dt, int dt_on_device,
+                          enzoe_stream_t stream);
+
 /* ---- ghost refresh for a uniform grid of blocks ------------------------- */
 
 #define ENZOE_BC_PERIODIC 0
