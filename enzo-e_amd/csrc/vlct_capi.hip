@@ -139,8 +139,6 @@ int enzoe_vlct_create(const enzoe_vlct_params *p, enzoe_vlct_t *out) {
   {
     const char *e = std::getenv("ENZOE_XCD_REMAP");
     if (e && *e) m->xcd_remap = std::atoi(e) & 15;
-    const char *k = std::getenv("ENZOE_CT_KERNELS");
-    if (k && *k) m->ct_mode = (k[0] == '0') ? 0 : 1;
   }
   // no device call here: device memory is taken on first use, so a
   // configuration can be validated on a host without a GPU
@@ -155,6 +153,7 @@ int enzoe_vlct_destroy(enzoe_vlct_t m) {
   if (m->d_dt) (void)hipFree(m->d_dt);
   if (m->d_ptrtab) (void)hipFree(m->d_ptrtab);
   if (m->d_dts) (void)hipFree(m->d_dts);
+  if (m->d_rtab) (void)hipFree(m->d_rtab);
   delete m;
   return ENZOE_OK;
 }
@@ -216,10 +215,14 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   return compute_runs(m, blocks, n, 0.0, d_dts, s);
 }
 
-int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
-                        enzoe_stream_t stream, double *dt_out) {
-  if (!m || !blocks || n <= 0 || !dt_out) return ENZOE_ERR_ARG;
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+namespace {
+
+// Launches the CFL reduction of the blocks into m->d_dt (bit pattern of
+// the unscaled min); the descriptor table is uploaded only when the block
+// list changed, so the call makes no host round trip otherwise.
+int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_t s) {
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
   if (!m->d_dt &&
@@ -227,7 +230,6 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     m->d_dt = nullptr;
     return ENZOE_ERR_ALLOC;
   }
-  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
   // descriptors for the dt kernel: only U, pressure, shape and widths used
   std::vector<BlockDesc<double>> desc(n);
   for (int i = 0; i < n; i++) {
@@ -244,26 +246,43 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
   }
   const size_t dbytes = desc.size() * sizeof(BlockDesc<double>);
-  if (int rc = ensure_bytes(&m->d_ptrtab, &m->ptrtab_bytes, dbytes)) return rc;
-  if (hipMemcpyAsync(m->d_ptrtab, desc.data(), dbytes, hipMemcpyHostToDevice,
-                     s) != hipSuccess)
-    return ENZOE_ERR_HIP;
+  const char *raw = (const char *)desc.data();
+  if (m->last_dt_desc.size() != dbytes ||
+      std::memcmp(m->last_dt_desc.data(), raw, dbytes) != 0) {
+    if (int rc = ensure_bytes(&m->d_ptrtab, &m->ptrtab_bytes, dbytes)) return rc;
+    if (hipMemcpyAsync(m->d_ptrtab, raw, dbytes, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return ENZOE_ERR_HIP;
+    m->last_dt_desc.assign(raw, raw + dbytes);
+  }
+  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
   long maxcells = 0;
   for (int i = 0; i < n; i++)
     maxcells = std::max<long>(maxcells, (long)blocks[i].mx * blocks[i].my * blocks[i].mz);
-  {
-    KernelTimer kt(s, "k_dt");
-    // enough workgroups to fill the device; each thread strides over cells
-    const unsigned wg = (unsigned)std::max<long>(1, std::min<long>(nblk(maxcells, 256),
-                                                                   std::max<long>(1, 2048 / n)));
-    dim3 grid(wg, n);
-    if (m->mhd)
-      hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s,
-                         (const BlockDesc<double> *)m->d_ptrtab, m->gamma, m->d_dt);
-    else
-      hipLaunchKernelGGL((k_dt<double, false>), grid, dim3(256), 0, s,
-                         (const BlockDesc<double> *)m->d_ptrtab, m->gamma, m->d_dt);
-  }
+  KernelTimer kt(s, "k_dt");
+  // enough workgroups to fill the device; each thread strides over cells
+  const unsigned wg = (unsigned)std::max<long>(1, std::min<long>(nblk(maxcells, 256),
+                                                                 std::max<long>(1, 2048 / n)));
+  dim3 grid(wg, n);
+  if (m->mhd)
+    hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s,
+                       (const BlockDesc<double> *)m->d_ptrtab, m->gamma, m->d_dt);
+  else
+    hipLaunchKernelGGL((k_dt<double, false>), grid, dim3(256), 0, s,
+                       (const BlockDesc<double> *)m->d_ptrtab, m->gamma, m->d_dt);
+  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                        enzoe_stream_t stream, double *dt_out) {
+  if (!m || !blocks || n <= 0 || !dt_out) return ENZOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (int rc = launch_dt(m, blocks, n, s)) return rc;
   unsigned long long bits = 0;
   if (hipMemcpyAsync(&bits, m->d_dt, sizeof(bits), hipMemcpyDeviceToHost, s) !=
       hipSuccess)
@@ -273,6 +292,19 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   std::memcpy(&dtb, &bits, sizeof(dtb));
   *dt_out = dtb * m->courant;
   return ENZOE_OK;
+}
+
+int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                               double *d_time, double t_stop, double *d_dts,
+                               int ndts, enzoe_stream_t stream) {
+  if (!m || !blocks || n <= 0 || !d_time || !d_dts || ndts <= 0)
+    return ENZOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (int rc = launch_dt(m, blocks, n, s)) return rc;
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s,
+                     (const unsigned long long *)m->d_dt, m->courant, d_time, t_stop,
+                     d_dts, ndts);
+  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
 }
 
 int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
@@ -309,16 +341,13 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
   }
   const size_t bytes = tab.size() * sizeof(double *);
   // the pointer table is uploaded only when the block list changes
-  static thread_local void *d_tab = nullptr;
-  static thread_local size_t d_tab_bytes = 0;
-  static thread_local std::vector<double *> last_tab;
-  if (last_tab != tab) {
-    if (int rc = ensure_bytes(&d_tab, &d_tab_bytes, bytes)) return rc;
-    if (hipMemcpyAsync(d_tab, tab.data(), bytes, hipMemcpyHostToDevice, s) !=
+  if (m->last_rtab != tab) {
+    if (int rc = ensure_bytes(&m->d_rtab, &m->rtab_bytes, bytes)) return rc;
+    if (hipMemcpyAsync(m->d_rtab, tab.data(), bytes, hipMemcpyHostToDevice, s) !=
         hipSuccess)
       return ENZOE_ERR_HIP;
     if (hipStreamSynchronize(s) != hipSuccess) return ENZOE_ERR_HIP;
-    last_tab = tab;
+    m->last_rtab = tab;
   }
   KernelTimer kt(s, "k_refresh");
   // largest ghost-element count over the field kinds (a face field has one
@@ -333,7 +362,7 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
     nghost = std::max(nghost, tot - in);
   }
   hipLaunchKernelGGL(k_refresh, dim3(nblk(nghost, 256), nb, nf), dim3(256), 0, s,
-                     (double *const *)d_tab, G);
+                     (double *const *)m->d_rtab, G);
   if (hipGetLastError() != hipSuccess) return ENZOE_ERR_HIP;
   return ENZOE_OK;
 }
@@ -366,6 +395,153 @@ int enzoe_profile_report(char *names, int name_len, double *ms, int *count,
     i++;
   }
   return i;
+}
+
+}  // extern "C"
+
+// The device buffers a method's launches read, moved in and out of the
+// method so that a graph owns a private set (a later eager call may
+// re-upload or regrow the method's own ones while the graph is alive).
+struct MethodBuffers {
+  void *scratch = nullptr, *d_desc = nullptr, *d_dt = nullptr,
+       *d_ptrtab = nullptr, *d_rtab = nullptr;
+  size_t scratch_bytes = 0, desc_bytes = 0, ptrtab_bytes = 0, rtab_bytes = 0;
+  std::vector<char> last_desc, last_dt_desc;
+  std::vector<double *> last_rtab;
+
+  void swap_with(enzoe_vlct_method *m) {
+    std::swap(scratch, m->scratch);
+    std::swap(scratch_bytes, m->scratch_bytes);
+    std::swap(d_desc, m->d_desc);
+    std::swap(desc_bytes, m->desc_bytes);
+    std::swap(last_desc, m->last_desc);
+    void *dt = (void *)m->d_dt;
+    std::swap(d_dt, dt);
+    m->d_dt = (unsigned long long *)dt;
+    std::swap(d_ptrtab, m->d_ptrtab);
+    std::swap(ptrtab_bytes, m->ptrtab_bytes);
+    std::swap(last_dt_desc, m->last_dt_desc);
+    std::swap(d_rtab, m->d_rtab);
+    std::swap(rtab_bytes, m->rtab_bytes);
+    std::swap(last_rtab, m->last_rtab);
+  }
+  void release() {
+    for (void *p : {scratch, d_desc, d_dt, d_ptrtab, d_rtab})
+      if (p) (void)hipFree(p);
+    scratch = d_desc = d_dt = d_ptrtab = d_rtab = nullptr;
+  }
+};
+
+struct enzoe_vlct_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double *d_state = nullptr;  // [0] time, [1 .. n] dt of every block
+  int n = 0;
+  MethodBuffers own;
+};
+
+extern "C" {
+
+int enzoe_vlct_graph_create(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
+                            int nby, int nbz, const int bc[3], double t0,
+                            double t_stop, int ncycles,
+                            enzoe_vlct_graph_t *out) {
+  if (!m || !blocks || !bc || !out || nbx <= 0 || nby <= 0 || nbz <= 0 ||
+      ncycles <= 0)
+    return ENZOE_ERR_ARG;
+  *out = nullptr;
+  const int n = nbx * nby * nbz;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return ENZOE_ERR_HIP;
+  enzoe_vlct_graph_s *g = new enzoe_vlct_graph_s;
+  g->n = n;
+  int rc = ENZOE_OK;
+  bool swapped = false;
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(s);
+    if (swapped) g->own.swap_with(m);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->d_state) (void)hipFree(g->d_state);
+    g->own.release();
+    delete g;
+    (void)hipStreamDestroy(s);
+    return code;
+  };
+  if (hipMalloc((void **)&g->d_state, (n + 1) * sizeof(double)) != hipSuccess) {
+    g->d_state = nullptr;
+    return fail(ENZOE_ERR_ALLOC);
+  }
+  if (hipMemcpy(g->d_state, &t0, sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(ENZOE_ERR_HIP);
+  // the graph's private buffers: take every buffer and upload every table
+  // now (the refresh is idempotent, the reduction writes only the pressure
+  // field), so the captured work allocates and uploads nothing
+  if (hipDeviceSynchronize() != hipSuccess) return fail(ENZOE_ERR_HIP);
+  g->own.swap_with(m);  // the method now holds empty buffers
+  swapped = true;
+  if ((rc = enzoe_grid_refresh(m, blocks, nbx, nby, nbz, bc, s))) return fail(rc);
+  if ((rc = launch_dt(m, blocks, n, s))) return fail(rc);
+  m->dry = true;
+  rc = compute_runs(m, blocks, n, 0.0, g->d_state + 1, s);
+  m->dry = false;
+  if (rc) return fail(rc);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(ENZOE_ERR_HIP);
+  const bool prof = g_profiling;
+  g_profiling = false;  // no timing events inside the graph
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    g_profiling = prof;
+    return fail(ENZOE_ERR_HIP);
+  }
+  for (int c = 0; c < ncycles && rc == ENZOE_OK; c++) {
+    rc = enzoe_vlct_timestep_device(m, blocks, n, g->d_state, t_stop, g->d_state + 1,
+                                    n, s);
+    if (!rc) rc = enzoe_grid_refresh(m, blocks, nbx, nby, nbz, bc, s);
+    if (!rc) rc = compute_runs(m, blocks, n, 0.0, g->d_state + 1, s);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  g_profiling = prof;
+  g->graph = graph;
+  if (rc) return fail(rc);
+  if (ec != hipSuccess || !graph) return fail(ENZOE_ERR_HIP);
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+    g->exec = nullptr;
+    return fail(ENZOE_ERR_HIP);
+  }
+  g->own.swap_with(m);  // the graph keeps its buffers, the method its own
+  swapped = false;
+  (void)hipStreamDestroy(s);
+  *out = g;
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_graph_launch(enzoe_vlct_graph_t g, enzoe_stream_t stream) {
+  if (!g || !g->exec) return ENZOE_ERR_ARG;
+  return hipGraphLaunch(g->exec, (hipStream_t)stream) == hipSuccess ? ENZOE_OK
+                                                                     : ENZOE_ERR_HIP;
+}
+
+int enzoe_vlct_graph_time(enzoe_vlct_graph_t g, enzoe_stream_t stream, double *t) {
+  if (!g || !t) return ENZOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(t, g->d_state, sizeof(double), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ENZOE_ERR_HIP;
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_graph_destroy(enzoe_vlct_graph_t g) {
+  if (!g) return ENZOE_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->d_state) (void)hipFree(g->d_state);
+  g->own.release();
+  delete g;
+  return ENZOE_OK;
 }
 
 }  // extern "C"

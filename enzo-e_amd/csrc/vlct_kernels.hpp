@@ -79,6 +79,12 @@ __device__ __forceinline__ void wg_coords(int xcd_remap, int &bx, int &by) {
   bx = (int)(lg - (unsigned)by * gx);
 }
 
+// A zero-length step (a device-driven cycle past its stop time) changes
+// nothing, so the stage kernels return at once.
+__device__ __forceinline__ bool zero_step(const StageParams &sp, int by) {
+  return sp.dts != nullptr && sp.dts[by] == 0.0;
+}
+
 // Decodes a linear thread index into (k,j,i) inside the box [lo, lo+n).
 __device__ __forceinline__ bool box_coords(long t, int lo, int nx, int ny,
                                            int nz, int &k, int &j, int &i) {
@@ -227,6 +233,7 @@ __global__ void __launch_bounds__(256)
 k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
   wg_coords(sp.xcd_remap & 1, bx, by);
+  if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
@@ -278,6 +285,7 @@ __global__ void __launch_bounds__(256)
 k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
   wg_coords(sp.xcd_remap & 2, bx, by);
+  if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
@@ -336,6 +344,7 @@ __global__ void __launch_bounds__(256)
 k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
   wg_coords(sp.xcd_remap & 4, bx, by);
+  if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
@@ -381,6 +390,7 @@ __global__ void __launch_bounds__(256)
 k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
   wg_coords(sp.xcd_remap & 8, bx, by);
+  if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
@@ -514,6 +524,22 @@ k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
 
 static __global__ void k_dt_init(unsigned long long *result) {
   *result = 0x7FEFFFFFFFFFFFFFull;  // numeric_limits<double>::max()
+}
+
+// dt of the cycle on the device: courant * min, clipped so the cycle ends
+// at t_stop (control_stopping.cpp:83-86, as the host driver clips), copied
+// into dts[0..n) for the compute launches; the time advances by dt.
+static __global__ void k_dt_finalize(const unsigned long long *bits,
+                                     double courant, double *time,
+                                     double t_stop, double *dts, int n) {
+  const double t = *time;
+  double dt = __longlong_as_double((long long)*bits) * courant;
+  if (t < t_stop && t_stop < t + dt) dt = t_stop - t;
+  const double rem = t_stop - t;
+  dt = dt < rem ? dt : rem;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dts[i] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) *time = t + dt;
 }
 
 // -------------------------------------------------------------- refresh

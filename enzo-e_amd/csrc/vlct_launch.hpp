@@ -14,7 +14,6 @@
 #include <vector>
 
 #include "../../include/enzoe_vlct.h"
-#include "vlct_ct.hpp"
 #include "vlct_kernels.hpp"
 
 namespace enzoe {
@@ -75,9 +74,6 @@ struct enzoe_vlct_method {
   // kernels launched in XCD-contiguous workgroup order (StageParams mask);
   // measured on the 256^3 Orszag-Tang block it pays for k_edge only
   int xcd_remap = 2;
-  // 1: edge+face kernel and z-marching cell kernel (vlct_ct.hpp);
-  // 0: the three plane kernels k_edge, k_faces, k_cells
-  int ct_mode = 1;
   double theta = 1.5, courant = 0.3, gamma = 5.0 / 3.0;
   double dfloor = 1e-200, pfloor = 1e-200;
 
@@ -91,6 +87,12 @@ struct enzoe_vlct_method {
   unsigned long long *d_dt = nullptr;
   void *d_ptrtab = nullptr;
   size_t ptrtab_bytes = 0;
+  std::vector<char> last_dt_desc;  // host copy of the dt descriptor table
+  void *d_rtab = nullptr;          // refresh pointer table
+  size_t rtab_bytes = 0;
+  std::vector<double *> last_rtab;
+  size_t scratch_budget = 0;       // bytes of the chunk scratch pool
+  bool dry = false;                // prepare buffers and tables, launch nothing
   void *d_dts = nullptr;  // per-block dt uploaded by enzoe_vlct_compute_dt
   size_t dts_bytes = 0;
 };
@@ -234,46 +236,6 @@ void dispatch_flux(const enzoe_vlct_method *m, int recon,
 }
 
 
-// Tile shape of k_ctface for a face box of nx * ny columns and nz planes:
-// the tx * ty (tx * ty <= 256, tx, ty <= 64) shape that wastes the fewest
-// thread slots, chunks of about 16 planes.
-inline CtTile ct_tile(int x0, int nx, int ny, int z0, int nz) {
-  CtTile best{};
-  double best_eff = -1.0;
-  const int cand[5] = {8, 16, 32, 64, nx};
-  for (int i = 0; i < 5; i++) {
-    int tx = cand[i];
-    if (tx < 4 || tx > 64) continue;
-    int ty = std::min(64, std::min(256 / tx, std::max(1, ny)));
-    if ((tx + 1) * (ty + 1) > kCtLds) continue;
-    int ntx = (nx + tx - 1) / tx, nty = (ny + ty - 1) / ty;
-    double eff = (double)nx * ny / ((double)ntx * nty * 256.0);
-    if (eff > best_eff + 1e-9 || (std::fabs(eff - best_eff) <= 1e-9 && tx > best.tx)) {
-      best_eff = eff;
-      best.tx = tx; best.ty = ty; best.ntx = ntx; best.nty = nty;
-    }
-  }
-  int nzc = std::max(1, (nz + 15) / 16);
-  best.zc = (nz + nzc - 1) / nzc;
-  best.nzc = (nz + best.zc - 1) / best.zc;
-  best.x0 = x0; best.y0 = x0; best.z0 = z0;
-  best.nx = nx; best.ny = ny; best.nz = nz;
-  return best;
-}
-
-inline ColShape col_shape(int lo, int mx, int my, int mz) {
-  ColShape s;
-  s.lo = lo;
-  s.ncol_x = mx - 2 * lo;
-  s.ncol_y = my - 2 * lo;
-  s.ncol = s.ncol_x * s.ncol_y;
-  const int nz = mz - 2 * lo;
-  int nzc = std::max(1, (nz + 15) / 16);
-  s.zc = (nz + nzc - 1) / nzc;
-  s.nzc = (nz + s.zc - 1) / s.zc;
-  return s;
-}
-
 inline long cube(int mx, int my, int mz, int lo, int hi_trim) {
   long a = mx - lo - hi_trim, b = my - lo - hi_trim, c = mz - lo - hi_trim;
   if (a <= 0 || b <= 0 || c <= 0) return 0;
@@ -289,12 +251,14 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
   const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd);
   // chunk size: as many blocks as fit the scratch budget (at least 1); the
   // chunks run one after the other on the stream and share the pool
-  size_t budget = (size_t)4 << 30;
-  {
+  if (m->scratch_budget == 0) {
+    size_t budget = (size_t)4 << 30;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
       budget = std::max(budget, std::min((size_t)16 << 30, free_b / 3));
+    m->scratch_budget = budget;
   }
+  const size_t budget = m->scratch_budget;
   int chunk = (int)std::max<size_t>(1, std::min<size_t>(n, budget / per));
   if (int rc = ensure_bytes(&m->scratch, &m->scratch_bytes, per * chunk)) return rc;
 
@@ -319,6 +283,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       m->last_desc.assign(raw, raw + dbytes);
     }
   }
+  if (m->dry) return 0;
   for (int c0 = 0; c0 < n; c0 += chunk) {
     const int c1 = std::min(n, c0 + chunk);
     const int nb = c1 - c0;
@@ -347,26 +312,6 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm");
         dispatch_flux<FluxT, MHD>(m, recon, d, nb, cube(mx, my, mz, cs, cs), sp, s);
       }
-      if (m->ct_mode == 1) {
-        if (MHD) {
-          KernelTimer kt(s, "k_ctface");
-          // face box [cs+1, m-cs) in x, y; planes [cs, mz-cs-1)
-          const CtTile T = ct_tile(cs + 1, mx - 2 * cs - 1, my - 2 * cs - 1, cs,
-                                   mz - 2 * cs - 1);
-          if (T.nx > 0 && T.ny > 0 && T.nz > 0)
-            hipLaunchKernelGGL((k_ctface<FluxT>), dim3(T.ntx * T.nty * T.nzc, nb),
-                               dim3(256), 0, s, d, sp, T);
-        }
-        {
-          KernelTimer kt(s, "k_cells");
-          const int lo = is_final ? cs : cs + 1;
-          const ColShape S = col_shape(lo, mx, my, mz);
-          if (S.ncol > 0 && S.nzc > 0)
-            hipLaunchKernelGGL((k_cells_z<FluxT, MHD>),
-                               dim3(((S.ncol + 255) / 256) * S.nzc, nb), dim3(256), 0,
-                               s, d, sp, S);
-        }
-      } else {
       if (MHD) {
         {
           KernelTimer kt(s, "k_edge");
@@ -386,7 +331,6 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
         hipLaunchKernelGGL((k_cells<FluxT, MHD>),
                            dim3(nblk8(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
                            s, d, sp);
-      }
       }
       stale += total_staling(recon);
     }

@@ -122,6 +122,17 @@ def lib():
                                          ctypes.c_double, vp]
         L.enzoe_vlct_compute_dt.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                             _dp, ctypes.c_int, vp]
+        L.enzoe_vlct_timestep_device.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                                 _dp, ctypes.c_double, _dp, ctypes.c_int,
+                                                 vp]
+        L.enzoe_vlct_graph_create.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int), ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int,
+                                              ctypes.POINTER(vp)]
+        L.enzoe_vlct_graph_launch.argtypes = [vp, vp]
+        L.enzoe_vlct_graph_time.argtypes = [vp, vp, _dp]
+        L.enzoe_vlct_graph_destroy.argtypes = [vp]
         L.enzoe_grid_refresh.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), vp]
@@ -351,6 +362,23 @@ class DeviceGrid:
         self.method.compute(self.blocks, dt)
         return dt
 
+    def graph(self, t0, t_stop, ncycles):
+        """A captured graph of `ncycles` cycles of this grid (device dt)."""
+        return CycleGraph(self, t0, t_stop, ncycles)
+
+    def evolve_graph(self, t0, t_stop, ncycles=16, max_replays=1 << 20):
+        """evolve() with device-driven cycles: replays a graph of `ncycles`
+        cycles until the device time reaches t_stop.  Returns the number of
+        replays and the final time."""
+        g = self.graph(t0, t_stop, ncycles)
+        t, k = t0, 0
+        while t < t_stop and k < max_replays:
+            g.launch()
+            t = g.time()
+            k += 1
+        g.destroy()
+        return k, t
+
     def evolve(self, t0, t_stop, max_cycles=1 << 30):
         t, n, log = t0, 0, []
         while t < t_stop and n < max_cycles:
@@ -360,6 +388,40 @@ class DeviceGrid:
             n += 1
         synchronize()
         return n, log
+
+
+class CycleGraph:
+    """enzoe_vlct_graph_*: cycles of a DeviceGrid captured in a HIP graph."""
+
+    def __init__(self, grid, t0, t_stop, ncycles):
+        self.grid = grid
+        self.arr = (Block * len(grid.blocks))(*[b.desc for b in grid.blocks])
+        bcs = (ctypes.c_int * 3)(*[BC[b] if isinstance(b, str) else b for b in grid.bc])
+        self.handle = ctypes.c_void_p()
+        check(lib().enzoe_vlct_graph_create(grid.method.handle, self.arr, grid.nblocks[0],
+                                            grid.nblocks[1], grid.nblocks[2], bcs, t0,
+                                            t_stop, ncycles, ctypes.byref(self.handle)),
+              "enzoe_vlct_graph_create")
+
+    def launch(self, stream=None):
+        check(lib().enzoe_vlct_graph_launch(self.handle, stream), "enzoe_vlct_graph_launch")
+
+    def time(self, stream=None):
+        t = ctypes.c_double()
+        check(lib().enzoe_vlct_graph_time(self.handle, stream, ctypes.byref(t)),
+              "enzoe_vlct_graph_time")
+        return t.value
+
+    def destroy(self):
+        if self.handle is not None and self.handle.value:
+            lib().enzoe_vlct_graph_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------ single-operation views

@@ -170,13 +170,14 @@ def fast_wave_111(n=32, amplitude=1e-6, gamma=5.0 / 3.0, g=3):
     vel = [(r[1] * nrm[a] + r[2] * t1[a] + r[3] * t2[a]) * s for a in range(3)]
     p = p0 + r[6] * s
     B0 = b_par * nrm + b_t1 * t1 + b_t2 * t2
-    # dB = s (bt1 t1 + bt2 t2) = curl A with A = -(amp/k) sin(phase) (db x n)
+    # dB = s (bt1 t1 + bt2 t2) = curl A with A = (amp/k) sin(phase) (db x n):
+    # curl(f(phase) c) = f' k (n x c) and n x (db x n) = db for db . n = 0
     db_dir = r[4] * t1 + r[5] * t2
     cdir = np.cross(db_dir, nrm)
 
     def edge(coords):
         Z, Y, X = np.meshgrid(*coords, indexing="ij")
-        return -(amplitude / kw) * np.sin(2.0 * np.pi * (X + Y + Z))
+        return (amplitude / kw) * np.sin(2.0 * np.pi * (X + Y + Z))
 
     # all three on the (m+1)^3 edge lattice of _edge_curl
     Ax = cdir[0] * edge((xf, xf, xce))

@@ -105,14 +105,29 @@ def copy_grid_state(src, dst):
 
 
 def relative_l1(grid_a, grid_b, names, ghost=3):
-    """Per field: sum over active cells |a - b| / sum |b|."""
+    """Two L1 measures of a against b over the active cells:
+    per field, sum |a - b| / sum |b|; and per field normalised by the
+    magnitude of its vector (sum |a - b| / sum |b_vec| for the velocity and
+    B components, as the field itself for scalars) -- the first is not
+    meaningful for a component that is zero in b (e.g. v_z of a 2D slab)."""
     num = dict.fromkeys(names, 0.0)
     den = dict.fromkeys(names, 0.0)
+    vec = {"velocity": 0.0, "bfield": 0.0}
     for ba, bb in zip(grid_a.blocks, grid_b.blocks):
         da, db = ba.download(names), bb.download(names)
+        s = (slice(ghost, -ghost),) * 3
         for nm in names:
-            s = (slice(ghost, -ghost),) * 3
             x, y = da[nm][s], db[nm][s]
             num[nm] += float(np.abs(x - y).sum())
             den[nm] += float(np.abs(y).sum())
-    return {nm: (num[nm] / den[nm] if den[nm] > 0 else num[nm]) for nm in names}
+        for g in vec:
+            comps = [g + "_" + c for c in "xyz"]
+            if all(c in db for c in comps):
+                vec[g] += float(np.sqrt(sum(db[c][s] ** 2 for c in comps)).sum())
+    per_field = {nm: (num[nm] / den[nm] if den[nm] > 0 else num[nm]) for nm in names}
+    by_magnitude = {}
+    for nm in names:
+        g = nm.rsplit("_", 1)[0]
+        d = vec[g] if g in vec and vec[g] > 0 else den[nm]
+        by_magnitude[nm] = num[nm] / d if d > 0 else num[nm]
+    return per_field, by_magnitude

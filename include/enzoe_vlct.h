@@ -116,6 +116,31 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                           const double *dt, int dt_on_device,
                           enzoe_stream_t stream);
 
+/* EnzoMethodMHDVlct::timestep + the clip of control_stopping.cpp:83-86
+ * without a host round trip: reads the time from *d_time (device), writes
+ * dt = min(courant * min_cfl, t_stop - t) into d_dts[0..ndts) (device) and
+ * advances *d_time by dt.  For device-driven cycles (graph capture).        */
+int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                               double *d_time, double t_stop, double *d_dts,
+                               int ndts, enzoe_stream_t stream);
+
+/* A HIP graph of `ncycles` Cello cycles of a uniform periodic/outflow grid
+ * of blocks (device dt + refresh + compute), captured once and replayed
+ * with no host work per cycle.  The time lives on the device, starts at t0
+ * and stops at t_stop: cycles past t_stop are zero-length and change
+ * nothing.  The block list and its buffers must stay valid; the method's
+ * other calls must not run concurrently with a replay.                      */
+typedef struct enzoe_vlct_graph_s *enzoe_vlct_graph_t;
+int enzoe_vlct_graph_create(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
+                            int nby, int nbz, const int bc[3], double t0,
+                            double t_stop, int ncycles,
+                            enzoe_vlct_graph_t *out);
+int enzoe_vlct_graph_launch(enzoe_vlct_graph_t g, enzoe_stream_t stream);
+/* Synchronises `stream` and reads the device time.                          */
+int enzoe_vlct_graph_time(enzoe_vlct_graph_t g, enzoe_stream_t stream,
+                          double *t);
+int enzoe_vlct_graph_destroy(enzoe_vlct_graph_t g);
+
 /* ---- ghost refresh for a uniform grid of blocks ------------------------- */
 
 #define ENZOE_BC_PERIODIC 0

@@ -150,3 +150,39 @@ def test_fast_wave_111_convergence():
     print("L1(density) fp64:", errs[(0, 16)], errs[(0, 32)], "order", order64)
     print("L1(density) mixed:", errs[(1, 16)], errs[(1, 32)], "order", order32)
     assert order64 > 1.8
+
+
+def test_graph_replays_match_eager_cycles():
+    """A captured graph of 2 cycles, replayed 3 times with eager work and
+    synchronisations between the replays, equals 6 eager cycles after every
+    replay; past t_stop the cycles are zero-length and change nothing."""
+    ev = _ev()
+    from _states import random_periodic_grid
+    grid = random_periodic_grid(mhd=True, seed=12, n=(11, 10, 9), vamp=0.4)
+    method = method_for(grid.cfg)
+    eager = device_copy(grid, method)
+    dev = device_copy(grid, method)
+    t_stop = 10.0
+    g = dev.graph(0.0, t_stop, 2)
+    t = 0.0
+    for rep in range(3):
+        g.launch()
+        for _ in range(2):
+            t += eager.cycle(t, t_stop)
+        ev.synchronize()
+        assert g.time() == t
+        a, b = dev.blocks[0].download(), eager.blocks[0].download()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg="replay %d %s" % (rep, k))
+    g.destroy()
+    # a short stop: the graph clips the last cycle and then idles
+    dev2 = device_copy(grid, method)
+    eager2 = device_copy(grid, method)
+    t_stop = 0.6 * t
+    n_rep, t_end = dev2.evolve_graph(0.0, t_stop, ncycles=4)
+    n_cyc, _ = eager2.evolve(0.0, t_stop)
+    ev.synchronize()
+    assert t_end == t_stop
+    a, b = dev2.blocks[0].download(), eager2.blocks[0].download()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])

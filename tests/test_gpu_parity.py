@@ -215,30 +215,3 @@ def test_mixed_precision_close_to_fp64():
         rel = np.abs(x - y).sum() / np.abs(x).sum()
         print(f, "L1 rel", rel)
         assert rel < 1e-5
-
-
-@pytest.mark.parametrize("mhd", [True, False])
-def test_plane_kernels_and_marching_kernels_agree(mhd, monkeypatch):
-    """The three plane kernels (k_edge, k_faces, k_cells) and the two
-    z-marching kernels (k_ctface, k_cells_z) give the same state; both
-    equal the oracle."""
-    grid = random_periodic_grid(mhd=mhd, seed=21, n=(19, 17, 23), vamp=0.5)
-    out = {}
-    for mode in ("0", "1"):
-        monkeypatch.setenv("ENZOE_CT_KERNELS", mode)
-        method = method_for(grid.cfg)
-        dg = device_copy(grid, method)
-        t = 0.0
-        for _ in range(2):
-            t += dg.cycle(t, 10.0)
-        _ev().synchronize()
-        out[mode] = dg.blocks[0].download()
-    for k in out["0"]:
-        np.testing.assert_array_equal(out["0"][k], out["1"][k])
-    t = 0.0
-    for _ in range(2):
-        t += grid.cycle(t, 10.0)
-    ref = grid.block_arrays(0)
-    for k, a in out["1"].items():
-        if k in ref:
-            np.testing.assert_allclose(a, ref[k], rtol=REL, atol=0)
