@@ -107,6 +107,12 @@ __device__ __forceinline__ P gld(const P *p, int i) {
   return ((const __attribute__((address_space(1))) P *)p)[i];
 }
 
+// Stores through the global address space (see gld).
+template <typename P>
+__device__ __forceinline__ void gst(P *p, long i, P v) {
+  ((__attribute__((address_space(1))) P *)p)[i] = v;
+}
+
 // Cell primitives w = (rho, vx, vy, vz, p, bx, by, bz) from the integration
 // quantities (EnzoPhysicsFluidProps.cpp:64-138, EnzoComputePressure.cpp:
 // 177-189), in FP64.
@@ -201,13 +207,13 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
     } else {
       hlle_mhd(l8, r8, gamma, fo);
     }
-    b.F[D][S_RHO][c] = (FluxT)fo[L_D];
-    b.F[D][S_VX + D][c] = (FluxT)fo[L_VI];
-    b.F[D][S_VX + DJ][c] = (FluxT)fo[L_VJ];
-    b.F[D][S_VX + DK][c] = (FluxT)fo[L_VK];
-    b.F[D][S_E][c] = (FluxT)fo[L_E];
-    b.F[D][S_BX + DJ][c] = (FluxT)fo[L_BJ];
-    b.F[D][S_BX + DK][c] = (FluxT)fo[L_BK];
+    gst(b.F[D][S_RHO], c, (FluxT)fo[L_D]);
+    gst(b.F[D][S_VX + D], c, (FluxT)fo[L_VI]);
+    gst(b.F[D][S_VX + DJ], c, (FluxT)fo[L_VJ]);
+    gst(b.F[D][S_VX + DK], c, (FluxT)fo[L_VK]);
+    gst(b.F[D][S_E], c, (FluxT)fo[L_E]);
+    gst(b.F[D][S_BX + DJ], c, (FluxT)fo[L_BJ]);
+    gst(b.F[D][S_BX + DK], c, (FluxT)fo[L_BK]);
   } else {
     T l5[5], r5[5];
     l5[H_D] = wl[S_RHO]; r5[H_D] = wr[S_RHO];
@@ -216,11 +222,11 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
     l5[H_VK] = wl[S_VX + DK]; r5[H_VK] = wr[S_VX + DK];
     l5[H_E] = wl[S_E]; r5[H_E] = wr[S_E];
     hllc(l5, r5, gamma, fo);
-    b.F[D][S_RHO][c] = (FluxT)fo[H_D];
-    b.F[D][S_VX + D][c] = (FluxT)fo[H_VI];
-    b.F[D][S_VX + DJ][c] = (FluxT)fo[H_VJ];
-    b.F[D][S_VX + DK][c] = (FluxT)fo[H_VK];
-    b.F[D][S_E][c] = (FluxT)fo[H_E];
+    gst(b.F[D][S_RHO], c, (FluxT)fo[H_D]);
+    gst(b.F[D][S_VX + D], c, (FluxT)fo[H_VI]);
+    gst(b.F[D][S_VX + DJ], c, (FluxT)fo[H_VJ]);
+    gst(b.F[D][S_VX + DK], c, (FluxT)fo[H_VK]);
+    gst(b.F[D][S_E], c, (FluxT)fo[H_E]);
   }
 }
 
@@ -228,8 +234,13 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
 // the three faces between p and p + e_d that lie in the flux region of the
 // stage (EnzoRiemannImpl.hpp:317-338 with stale_depth = cur_stale: face
 // index range [cs, m_d - 1 - cs) along d).
+// Minimum waves per SIMD the flux kernel is compiled for (caps its VGPRs).
+#ifndef ENZOE_FLUX_MIN_WAVES
+#define ENZOE_FLUX_MIN_WAVES 3
+#endif
+
 template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, ENZOE_FLUX_MIN_WAVES)
 k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
   wg_coords(sp.xcd_remap & 1, bx, by);
