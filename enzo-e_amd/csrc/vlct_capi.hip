@@ -307,6 +307,90 @@ int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
 }
 
+int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                        double dt, enzoe_stream_t stream, int *mask_out) {
+  if (!m || !blocks || n <= 0 || !(dt > 0.0)) return ENZOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < n; i++)
+    if (int rc = check_block(m, blocks[i])) return rc;
+  const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
+  // sample: the leading blocks of the first shape, at most ~2 GiB of fields
+  const size_t cells = (size_t)mx * my * mz;
+  const size_t faces = (size_t)(mx + 1) * my * mz + (size_t)mx * (my + 1) * mz +
+                       (size_t)mx * my * (mz + 1);
+  const size_t per = (6 * cells + (m->mhd ? 3 * cells + faces : 0)) * sizeof(double);
+  int ns = 0;
+  while (ns < n && blocks[ns].mx == mx && blocks[ns].my == my && blocks[ns].mz == mz &&
+         (ns == 0 || (size_t)(ns + 1) * per <= ((size_t)2 << 30)))
+    ns++;
+  char *pool = nullptr;
+  if (hipMalloc((void **)&pool, per * ns) != hipSuccess) return ENZOE_ERR_ALLOC;
+  std::vector<enzoe_block> copy(blocks, blocks + ns);
+  int rc = ENZOE_OK;
+  {
+    char *p = pool;
+    auto take = [&](double *&dst, const double *src, size_t count) {
+      double *d = (double *)p;
+      p += count * sizeof(double);
+      if (src && rc == ENZOE_OK &&
+          hipMemcpyAsync(d, src, count * sizeof(double), hipMemcpyDeviceToDevice, s) !=
+              hipSuccess)
+        rc = ENZOE_ERR_HIP;
+      dst = d;
+    };
+    for (int i = 0; i < ns; i++) {
+      enzoe_block &b = copy[i];
+      take(b.density, blocks[i].density, cells);
+      for (int a = 0; a < 3; a++) take(b.velocity[a], blocks[i].velocity[a], cells);
+      take(b.total_energy, blocks[i].total_energy, cells);
+      take(b.pressure, blocks[i].pressure, cells);
+      if (m->mhd) {
+        for (int a = 0; a < 3; a++) take(b.bfield[a], blocks[i].bfield[a], cells);
+        for (int a = 0; a < 3; a++)
+          take(b.bfieldi[a], blocks[i].bfieldi[a],
+               (size_t)(mx + (a == 0)) * (my + (a == 1)) * (mz + (a == 2)));
+      }
+    }
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == ENZOE_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = ENZOE_ERR_HIP;
+  const bool prof = g_profiling;
+  g_profiling = false;
+  int best = m->xcd_remap;
+  float best_ms = 1e30f;
+  // every combination of XCD-contiguous order over the four stage kernels
+  for (int mask = 0; mask < 16 && rc == ENZOE_OK; mask++) {
+    m->force_mask = mask;
+    rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);  // warm-up
+    if (rc) break;
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < 3 && rc == ENZOE_OK; r++)
+      rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);
+    (void)hipEventRecord(e1, s);
+    if (rc || hipEventSynchronize(e1) != hipSuccess) {
+      rc = rc ? rc : ENZOE_ERR_HIP;
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = mask;
+    }
+  }
+  m->force_mask = -1;
+  g_profiling = prof;
+  (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(pool);
+  if (rc) return rc;
+  m->tuned_mask[shape_key(mx, my, mz)] = best;
+  if (mask_out) *mask_out = best;
+  return ENZOE_OK;
+}
+
 int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
                        int nby, int nbz, const int bc[3],
                        enzoe_stream_t stream) {

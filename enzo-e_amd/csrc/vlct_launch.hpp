@@ -93,6 +93,10 @@ struct enzoe_vlct_method {
   std::vector<double *> last_rtab;
   size_t scratch_budget = 0;       // bytes of the chunk scratch pool
   bool dry = false;                // prepare buffers and tables, launch nothing
+  // launch-order mask chosen by enzoe_vlct_autotune per block shape, and a
+  // mask forced while the autotuner times a candidate (-1: none)
+  std::map<long long, int> tuned_mask;
+  int force_mask = -1;
   void *d_dts = nullptr;  // per-block dt uploaded by enzoe_vlct_compute_dt
   size_t dts_bytes = 0;
 };
@@ -236,6 +240,17 @@ void dispatch_flux(const enzoe_vlct_method *m, int recon,
 }
 
 
+inline long long shape_key(int mx, int my, int mz) {
+  return ((long long)mx << 40) | ((long long)my << 20) | (long long)mz;
+}
+
+// XCD-order mask of the stage kernels for blocks of this shape
+inline int launch_mask(const enzoe_vlct_method *m, int mx, int my, int mz) {
+  if (m->force_mask >= 0) return m->force_mask;
+  auto it = m->tuned_mask.find(shape_key(mx, my, mz));
+  return it != m->tuned_mask.end() ? it->second : m->xcd_remap;
+}
+
 inline long cube(int mx, int my, int mz, int lo, int hi_trim) {
   long a = mx - lo - hi_trim, b = my - lo - hi_trim, c = mz - lo - hi_trim;
   if (a <= 0 || b <= 0 || c <= 0) return 0;
@@ -306,7 +321,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       sp.density_floor = m->dfloor;
       sp.pressure_floor = m->pfloor;
       sp.theta = m->theta;
-      sp.xcd_remap = m->xcd_remap;
+      sp.xcd_remap = launch_mask(m, mx, my, mz);
       const int cs = sp.cur_stale;
       {
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm");

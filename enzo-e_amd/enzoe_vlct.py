@@ -122,6 +122,8 @@ def lib():
                                          ctypes.c_double, vp]
         L.enzoe_vlct_compute_dt.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                             _dp, ctypes.c_int, vp]
+        L.enzoe_vlct_autotune.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                          ctypes.c_double, vp, ctypes.POINTER(ctypes.c_int)]
         L.enzoe_vlct_timestep_device.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                                  _dp, ctypes.c_double, _dp, ctypes.c_int,
                                                  vp]
@@ -287,6 +289,14 @@ class VlctMethod:
     def compute(self, blocks, dt, stream=None):
         check(lib().enzoe_vlct_compute(self.handle, self._blocks(blocks), len(blocks),
                                        dt, stream), "enzoe_vlct_compute")
+
+    def autotune(self, blocks, dt, stream=None):
+        """Picks the launch order of the stage kernels for this block shape
+        by timing the candidates on a copy of the blocks; returns the mask."""
+        mask = ctypes.c_int(-1)
+        check(lib().enzoe_vlct_autotune(self.handle, self._blocks(blocks), len(blocks), dt,
+                                        stream, ctypes.byref(mask)), "enzoe_vlct_autotune")
+        return mask.value
 
     def compute_dt(self, blocks, dts, stream=None, block_array=None):
         """One step with one dt per block (sub-cycled levels advanced

@@ -120,6 +120,16 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * without a host round trip: reads the time from *d_time (device), writes
  * dt = min(courant * min_cfl, t_stop - t) into d_dts[0..ndts) (device) and
  * advances *d_time by dt.  For device-driven cycles (graph capture).        */
+/* Launch-shape autotuner (no counterpart in the reference): times every
+ * combination of XCD-contiguous workgroup order over the four stage kernels
+ * on a scratch copy of the leading blocks (at most ~2 GiB; the caller's
+ * fields are not touched), advancing them by dt, and keeps the fastest for
+ * this block shape; later compute calls on blocks of that shape use it.
+ * *mask_out (optional) gets the chosen mask: bit 0 k_flux, 1 k_edge,
+ * 2 k_faces, 3 k_cells.                                                     */
+int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                        double dt, enzoe_stream_t stream, int *mask_out);
+
 int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                                double *d_time, double t_stop, double *d_dts,
                                int ndts, enzoe_stream_t stream);

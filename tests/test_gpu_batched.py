@@ -186,3 +186,27 @@ def test_graph_replays_match_eager_cycles():
     a, b = dev2.blocks[0].download(), eager2.blocks[0].download()
     for k in a:
         np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_autotuned_launch_order_keeps_results():
+    """The autotuner only reorders workgroups: after it has chosen a mask
+    for the shape (on a copy, leaving the blocks alone), cycles still equal
+    the oracle's."""
+    ev = _ev()
+    from _states import random_periodic_grid
+    grid = random_periodic_grid(mhd=True, seed=31, n=(16, 14, 12), vamp=0.5)
+    method = method_for(grid.cfg)
+    dg = device_copy(grid, method)
+    before = dg.blocks[0].download()
+    mask = method.autotune(dg.blocks, 1e-3)
+    assert 0 <= mask < 16
+    after = dg.blocks[0].download()
+    for k in before:
+        np.testing.assert_array_equal(before[k], after[k])
+    t = 0.0
+    for _ in range(2):
+        dt_o = grid.cycle(t, 10.0)
+        assert dg.cycle(t, 10.0) == dt_o
+        t += dt_o
+    ev.synchronize()
+    compare_grids(dg, grid)
