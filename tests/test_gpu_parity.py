@@ -215,3 +215,61 @@ def test_mixed_precision_close_to_fp64():
         rel = np.abs(x - y).sum() / np.abs(x).sum()
         print(f, "L1 rel", rel)
         assert rel < 1e-5
+
+
+def test_orszag_tang_256_full_size_matches_oracle():
+    """The benchmark workload at its full size (one 256^3 block): two
+    cycles on the GPU equal the oracle's cell for cell, mass is conserved to
+    roundoff and the discrete div B of the face field stays at roundoff."""
+    import problems
+    ev = _ev()
+    n = 256
+    cfg = po.make_config(riemann="hlld", recon=("nn", "plm"), theta=2.0, courant=0.4)
+    blk, cw = problems.orszag_tang(n)
+    grid = po.Grid(cfg, (1, 1, 1), (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    view = grid.block_arrays(0)
+    for k, v in blk.items():
+        view[k][...] = v
+    grid.refresh()
+    dg = device_copy(grid, method_for(cfg))
+    s = (slice(3, -3),) * 3
+    mass0 = float(view["density"][s].sum())
+    t = 0.0
+    for _ in range(2):
+        dt_o = grid.cycle(t, 10.0, nthreads=16)
+        assert dg.cycle(t, 10.0) == dt_o
+        t += dt_o
+    ev.synchronize()
+    compare_grids(dg, grid)
+    got = dg.blocks[0].download()
+    assert abs(float(got["density"][s].sum()) - mass0) <= 1e-12 * mass0
+    bx, by, bz = got["bfieldi_x"], got["bfieldi_y"], got["bfieldi_z"]
+    div = ((bx[s[0], s[1], 4:-3] - bx[s[0], s[1], 3:-4]) +
+           (by[s[0], 4:-3, s[2]] - by[s[0], 3:-4, s[2]]) +
+           (bz[4:-3, s[1], s[2]] - bz[3:-4, s[1], s[2]])) / cw[0]
+    bmax = float(np.abs(bx).max())
+    assert float(np.abs(div).max()) <= 1e-13 * bmax / cw[0]
+
+
+def test_mixed_block_shapes_in_one_call():
+    """A compute call over blocks of different shapes (runs of equal shape
+    each get their own launches) equals the oracle block by block; an empty
+    list is rejected without touching the device."""
+    ev = _ev()
+    grids = [random_periodic_grid(mhd=True, seed=40 + i, n=n, vamp=0.4)
+             for i, n in enumerate([(9, 8, 7), (9, 8, 7), (12, 7, 9)])]
+    method = method_for(grids[0].cfg)
+    dgs = [device_copy(g, method) for g in grids]
+    blocks = [d.blocks[0] for d in dgs]
+    for d in dgs:
+        d.refresh()
+    dt = min(method.timestep([b]) for b in blocks)
+    method.compute(blocks, dt)
+    for g in grids:
+        g.refresh()
+        assert po.lib().o_vlct_compute(ctypes.byref(g.cfg), ctypes.byref(g.block(0)), dt) == 0
+    ev.synchronize()
+    for d, g in zip(dgs, grids):
+        compare_grids(d, g)
+    arr = (ev.Block * 1)()
+    assert ev.lib().enzoe_vlct_compute(method.handle, arr, 0, dt, None) == 2
