@@ -35,6 +35,7 @@ struct BlockDesc {
   double *bft[3];      // half-step face B
   FluxT *F[3][S_N];    // fluxes along x, y, z; slot = field
   double *E[3];        // edge EMFs along x, y, z
+  double *Ec[3];       // cell-centred E = -v x B of the stage's state
   int mx, my, mz;
   double cw[3];
 };
@@ -269,6 +270,14 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int c = (k * my + j) * mx + i;
   double wc[S_N];
   load_prim<MHD>(U, c, fc.gm1, wc);
+  if (MHD) {
+    // centre E_i = -v_j B_k + v_k B_j of this cell for the edge kernel
+    // (EnzoBfieldMethodCT::compute_center_efield, EnzoBfieldMethodCT.cpp:
+    // 267-292); the box of k_flux covers every cell the edges read
+    gst(b.Ec[0], c, (-wc[S_VY] * wc[S_BZ] + wc[S_VZ] * wc[S_BY]));
+    gst(b.Ec[1], c, (-wc[S_VZ] * wc[S_BX] + wc[S_VX] * wc[S_BZ]));
+    gst(b.Ec[2], c, (-wc[S_VX] * wc[S_BY] + wc[S_VY] * wc[S_BX]));
+  }
   if (i < mx - 1 - cs)
     flux_face<T, FluxT, RECON, RIEMANN, MHD, 0>(b, U, bfr[0], k, j, i, c, wc, fc);
   if (j < my - 1 - cs)
@@ -289,8 +298,8 @@ __device__ __forceinline__ double upwind_weight(double f) {
 // along each axis i at (cell p_i, faces p_j + 1/2, p_k + 1/2) when the edge
 // lies in [cs+1, m_i-cs-1) x [cs, m_j-cs-1) x [cs, m_k-cs-1)
 // (EnzoBfieldMethodCT::compute_edge_efield / compute_edge_<true>,
-// EnzoBfieldMethodCT.cpp:384-557; centre E from compute_center_efield,
-// :267-292, on the current-stage state).
+// EnzoBfieldMethodCT.cpp:384-557); the centre E of the current-stage state
+// (compute_center_efield, :267-292) was written by k_flux.
 template <typename FluxT>
 __global__ void __launch_bounds__(256)
 k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
@@ -304,7 +313,6 @@ k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const long t = (long)bx * blockDim.x + threadIdx.x;
   if (!box_coords(t, cs, mx - 2 * cs, my - 2 * cs, mz - 2 * cs, k, j, i))
     return;
-  double *const *U = (sp.stage == 0) ? b.U : b.Ut;
   const int p[3] = {i, j, k};
   const int m[3] = {mx, my, mz};
   const long c = idx3(k, j, i, my, mx);
@@ -315,12 +323,10 @@ k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     if (p[ax] < cs + 1 || p[ax] >= m[ax] - cs - 1) continue;
     if (p[aj] >= m[aj] - cs - 1 || p[ak] >= m[ak] - cs - 1) continue;
     const long sj = str[aj], sk = str[ak];
-    // centre E_i = -v_j B_k + v_k B_j at the four cells around the edge
-    const double *vj = U[S_VX + aj], *vk = U[S_VX + ak];
-    const double *bj = U[S_BX + aj], *bk = U[S_BX + ak];
-    auto ecen = [&](long cc) { return (-vj[cc] * bk[cc] + vk[cc] * bj[cc]); };
-    const double Ec = ecen(c), Ec_jp1 = ecen(c + sj), Ec_kp1 = ecen(c + sk),
-                 Ec_jkp1 = ecen(c + sj + sk);
+    // centre E_i at the four cells around the edge (written by k_flux)
+    const double *Eci = b.Ec[ax];
+    const double Ec = Eci[c], Ec_jp1 = Eci[c + sj], Ec_kp1 = Eci[c + sk],
+                 Ec_jkp1 = Eci[c + sj + sk];
     // face E: j-faces carry -F_j(B_k), k-faces +F_k(B_j)
     const FluxT *Fj = b.F[aj][S_BX + ak];
     const FluxT *Fk = b.F[ak][S_BX + aj];

@@ -124,7 +124,7 @@ inline size_t scratch_per_block(int mx, int my, int mz, int mhd) {
   if (mhd) {
     bytes += ((size_t)mz * my * (mx + 1) + (size_t)mz * (my + 1) * mx +
               (size_t)(mz + 1) * my * mx) * sizeof(double);  // bft
-    bytes += 3 * ncc * sizeof(double);                 // E
+    bytes += 6 * ncc * sizeof(double);                 // E, Ec
   }
   bytes += 3 * (mhd ? 7 : 5) * ncc * sizeof(FluxT);    // F
   return bytes;
@@ -195,6 +195,10 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
       d.bft[2] = (double *)base; base += (size_t)(b.mz + 1) * b.my * b.mx * sizeof(double);
       for (int a = 0; a < 3; a++) {
         d.E[a] = (double *)base;
+        base += ncc * sizeof(double);
+      }
+      for (int a = 0; a < 3; a++) {
+        d.Ec[a] = (double *)base;
         base += ncc * sizeof(double);
       }
     }
