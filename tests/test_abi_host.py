@@ -167,3 +167,44 @@ def test_host_method_errors_exit_like_the_reference(params, message):
     assert r.returncode == 1, (r.stdout, r.stderr)
     assert "ERROR in" in r.stderr and message in r.stderr, r.stderr
     assert "constructed" not in r.stdout
+
+
+def test_fused_entry_points_reject_bad_arguments_without_a_device():
+    """Argument checks of the fused entry points run before any device call,
+    so they can be exercised here."""
+    L = ev.lib()
+    rc, h = _create()
+    assert rc == 0
+    empty = (ev.Block * 1)()
+    assert L.enzoe_vlct_compute(h, empty, 0, 0.1, None) == ERR_ARG
+    assert L.enzoe_vlct_compute(None, empty, 1, 0.1, None) == ERR_ARG
+    dts = (ctypes.c_double * 1)(-1.0)
+    assert L.enzoe_vlct_compute_dt(h, empty, 1, dts, 0, None) == ERR_ARG
+    dtp = ctypes.c_double()
+    assert L.enzoe_vlct_timestep(h, empty, 0, None, ctypes.byref(dtp)) == ERR_ARG
+    mask = ctypes.c_int()
+    assert L.enzoe_vlct_autotune(h, empty, 1, 0.0, None, ctypes.byref(mask)) == ERR_ARG
+    g = ctypes.c_void_p()
+    bcs = (ctypes.c_int * 3)(0, 0, 0)
+    assert L.enzoe_vlct_graph_create(h, empty, 0, 1, 1, bcs, 0.0, 1.0, 2,
+                                     ctypes.byref(g)) == ERR_ARG
+    assert L.enzoe_vlct_graph_create(h, empty, 1, 1, 1, bcs, 0.0, 1.0, 0,
+                                     ctypes.byref(g)) == ERR_ARG
+    assert L.enzoe_vlct_graph_launch(None, None) == ERR_ARG
+    # a block too shallow for the staling of NN + PLM (ghost 2 < 3) or
+    # without its face fields is rejected before anything is launched
+    fake = 1 << 40
+    b = ev.Block()
+    b.mx = b.my = b.mz = 12
+    b.ghost = 2
+    b.density = fake
+    for a in range(3):
+        b.velocity[a] = b.bfield[a] = b.bfieldi[a] = fake
+    b.total_energy = fake
+    arr = (ev.Block * 1)(b)
+    assert L.enzoe_vlct_compute(h, arr, 1, 0.1, None) == ERR_ARG
+    b.ghost = 3
+    b.bfieldi[1] = None
+    arr = (ev.Block * 1)(b)
+    assert L.enzoe_vlct_compute(h, arr, 1, 0.1, None) == ERR_ARG
+    L.enzoe_vlct_destroy(h)
