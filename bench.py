@@ -127,9 +127,8 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def cpu_baseline(n, nsteps):
-    """The oracle (plain-C restatement of the reference loops, OpenMP over
-    the z loops) on an n^3 Orszag-Tang block: zone-updates/s."""
+def _oracle_rate(n, nsteps, threads):
+    """Zone-updates/s of the oracle on an n^3 Orszag-Tang block."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     cfg = po.make_config(riemann="hlld", recon=("nn", "plm"), theta=2.0,
@@ -140,18 +139,29 @@ def cpu_baseline(n, nsteps):
     for k, v in blk.items():
         view[k][...] = v
     grid.refresh()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     grid.cycle(0.0, 1e9, nthreads=threads)  # warm the scratch
     t0 = time.perf_counter()
     t = 0.0
     for _ in range(nsteps):
         t += grid.cycle(t, 1e9, nthreads=threads)
     el = time.perf_counter() - t0
-    return {"value": n ** 3 * nsteps / el, "unit": "zone-updates/s",
+    return n ** 3 * nsteps / el, el
+
+
+def cpu_baseline(n, nsteps):
+    """The oracle (plain-C restatement of the reference loops, OpenMP over
+    the z loops) on the host cores, and on one core on a smaller sample."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rate, el = _oracle_rate(n, nsteps, threads)
+    n1, s1 = 64, 10
+    rate1, el1 = _oracle_rate(n1, s1, 1)
+    return {"value": rate, "unit": "zone-updates/s",
             "cores": threads, "kind": "port",
             "sample": "oracle (C restatement of the reference loops, OpenMP "
                       "over z) on the same Orszag-Tang problem at %d^3, %d "
-                      "steps, %.1f s" % (n, nsteps, el)}
+                      "steps, %.1f s" % (n, nsteps, el),
+            "one_core": {"value": rate1, "cores": 1,
+                         "sample": "same at %d^3, %d steps, %.1f s" % (n1, s1, el1)}}
 
 
 METRIC = "zone-updates/s/GPU (batched FP32-mixed vs plain FP64 VL+CT) + HBM GB/s"
