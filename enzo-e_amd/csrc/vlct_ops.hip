@@ -136,6 +136,7 @@ int launched() {
 constexpr unsigned kMhdSlots = 0xFFu;
 constexpr unsigned kHydroSlots = 0x1Fu;
 constexpr unsigned kUpdateSlots = 0x1Fu;  // density, velocity, total_energy
+constexpr unsigned kPassiveSlots = 0xFF00u;
 
 // ------------------------------------------------------------------ kernels
 
@@ -186,7 +187,7 @@ __global__ void k_op_reconstruct(KMap w, KMap wl, KMap wr, unsigned slots,
 // order of the kernels (EnzoRiemannLUT, KernelConfig).
 template <int RIEMANN, bool MHD>
 __global__ void k_op_riemann(KMap pl, KMap pr, KMap fx8, Box box, int dim,
-                             double gamma) {
+                             double gamma, unsigned pmask) {
   int p[3];
   if (!box_pos(box, p)) return;
   const long a = off(pl.s, p), b = off(pr.s, p), f = off(fx8.s, p);
@@ -205,6 +206,16 @@ __global__ void k_op_riemann(KMap pl, KMap pr, KMap fx8, Box box, int dim,
     for (int q = 0; q < 5; q++) { l[q] = pl.p[src[q]][a]; r[q] = pr.p[src[q]][b]; }
     hllc(l, r, gamma, o);
     for (int q = 0; q < 5; q++) fx8.p[src[q]][f] = o[q];
+  }
+  // passive scalars: upwind mass fraction times the density flux
+  // (calc_passive_scalar_flux_, EnzoRiemannUtils.hpp:224-232, branch-free
+  // form kept for identical rounding)
+  const double dens_flux = fx8.p[S_RHO][f];
+  for (int q = ENZOE_SLOT_PASSIVE0; q < ENZOE_MAP_SLOTS; q++) {
+    if (!(pmask >> q & 1u)) continue;
+    const double wl = pl.p[q][a], wr = pr.p[q][b];
+    const double upwind = (double)(dens_flux > 0) * wl + (double)(dens_flux <= 0) * wr;
+    fx8.p[q][f] = upwind * dens_flux;
   }
 }
 
@@ -324,10 +335,15 @@ __global__ void k_op_accumulate(KMap F, KMap dU, unsigned slots, Box box,
 // (EnzoPhysicsFluidProps.cpp:162-290, dual energy off) at stale s+1, which
 // uses the cell-centred B already written into the output map.
 __global__ void k_op_update(KMap U0, KMap dU, KMap Uo, Box box, int mhd,
-                            double gamma, double dfloor, double pfloor) {
+                            double gamma, double dfloor, double pfloor,
+                            unsigned pmask) {
   int p[3];
   if (!box_pos(box, p)) return;
   const long a = off(U0.s, p), d = off(dU.s, p), o = off(Uo.s, p);
+  // passive scalars in conserved form (update_passive_scalars_,
+  // EnzoIntegrationQuanUpdate.cpp:242-268)
+  for (int q = ENZOE_SLOT_PASSIVE0; q < ENZOE_MAP_SLOTS; q++)
+    if (pmask >> q & 1u) Uo.p[q][o] = U0.p[q][a] + dU.p[q][d];
   double old_rho = U0.p[S_RHO][a];
   double new_rho = dmax(U0.p[S_RHO][a] + dU.p[S_RHO][d], dfloor);
   double inv_new_rho = 1. / new_rho;
@@ -364,13 +380,17 @@ __device__ __forceinline__ double pressure_at(const KMap &U, long c, int mhd,
 // EnzoPhysicsFluidProps::primitive_from_integration
 // (EnzoPhysicsFluidProps.cpp:64-138) over [s, n-s): copies density, velocity
 // and B, pressure into the pressure slot.
-__global__ void k_op_primitive(KMap U, KMap W, Box box, int mhd, double gm1) {
+__global__ void k_op_primitive(KMap U, KMap W, Box box, int mhd, double gm1,
+                               unsigned pmask) {
   int p[3];
   if (!box_pos(box, p)) return;
   const long c = off(U.s, p), w = off(W.s, p);
   const int nq = mhd ? 8 : 5;
   for (int q = 0; q < nq; q++)
     if (q != S_E) W.p[q][w] = U.p[q][c];
+  // passive densities -> mass fractions (EnzoPhysicsFluidProps.cpp:121-134)
+  for (int q = ENZOE_SLOT_PASSIVE0; q < ENZOE_MAP_SLOTS; q++)
+    if (pmask >> q & 1u) W.p[q][w] = U.p[q][c] / U.p[S_RHO][c];
   W.p[S_E][w] = pressure_at(U, c, mhd, gm1);
 }
 
@@ -537,6 +557,9 @@ int enzoe_riemann_solve(const char *solver, int mhd, double gamma,
       !to_kmap(primr, need, r) || !to_kmap(flux, need, f))
     return ENZOE_ERR_ARG;
   if (!same_shape(l.n, r.n) || !same_shape(l.n, f.n)) return ENZOE_ERR_ARG;
+  const unsigned pmask = present(l) & kPassiveSlots;
+  if ((present(r) & kPassiveSlots) != pmask || (present(f) & pmask) != pmask)
+    return ENZOE_ERR_ARG;
   int lo[3], hi[3];
   stale_box(f.n, stale_depth, lo, hi);
   long count;
@@ -545,13 +568,13 @@ int enzoe_riemann_solve(const char *solver, int mhd, double gamma,
   hipStream_t st = (hipStream_t)stream;
   if (kind == RS_HLLD)
     hipLaunchKernelGGL((k_op_riemann<RS_HLLD, true>), dim3(nblk(count)),
-                       dim3(256), 0, st, l, r, f, box, dim, gamma);
+                       dim3(256), 0, st, l, r, f, box, dim, gamma, pmask);
   else if (kind == RS_HLLE)
     hipLaunchKernelGGL((k_op_riemann<RS_HLLE, true>), dim3(nblk(count)),
-                       dim3(256), 0, st, l, r, f, box, dim, gamma);
+                       dim3(256), 0, st, l, r, f, box, dim, gamma, pmask);
   else
     hipLaunchKernelGGL((k_op_riemann<RS_HLLC, false>), dim3(nblk(count)),
-                       dim3(256), 0, st, l, r, f, box, dim, gamma);
+                       dim3(256), 0, st, l, r, f, box, dim, gamma, pmask);
   return launched();
 }
 
@@ -756,6 +779,9 @@ int enzoe_update_quantities(const enzoe_fmap *initial,
       !to_kmap(dUcons, kUpdateSlots, d) || !to_kmap(out, need_out, o) ||
       !same_shape(a.n, d.n) || !same_shape(a.n, o.n))
     return ENZOE_ERR_ARG;
+  const unsigned pmask = present(d) & kPassiveSlots;
+  if ((present(a) & pmask) != pmask || (present(o) & pmask) != pmask)
+    return ENZOE_ERR_ARG;
   int lo[3], hi[3];
   stale_box(a.n, stale_depth + 1, lo, hi);
   long count;
@@ -763,7 +789,7 @@ int enzoe_update_quantities(const enzoe_fmap *initial,
   if (count == 0) return ENZOE_OK;
   hipLaunchKernelGGL(k_op_update, dim3(nblk(count)), dim3(256), 0,
                      (hipStream_t)stream, a, d, o, box, mhd, gamma,
-                     density_floor, pressure_floor);
+                     density_floor, pressure_floor, pmask);
   return launched();
 }
 
@@ -776,13 +802,14 @@ int enzoe_primitive_from_integration(const enzoe_fmap *integration,
   if (stale_depth < 0 || !to_kmap(integration, need, u) ||
       !to_kmap(primitive, need, w) || !same_shape(u.n, w.n))
     return ENZOE_ERR_ARG;
+  const unsigned pmask = present(u) & present(w) & kPassiveSlots;
   int lo[3], hi[3];
   stale_box(u.n, stale_depth, lo, hi);
   long count;
   Box box = make_box(lo, hi, count);
   if (count == 0) return ENZOE_OK;
   hipLaunchKernelGGL(k_op_primitive, dim3(nblk(count)), dim3(256), 0,
-                     (hipStream_t)stream, u, w, box, mhd, gamma - 1.0);
+                     (hipStream_t)stream, u, w, box, mhd, gamma - 1.0, pmask);
   return launched();
 }
 

@@ -449,7 +449,7 @@ class FArray(ctypes.Structure):
 
 class FMap(ctypes.Structure):
     """enzoe_fmap: same-shape device views by slot."""
-    _fields_ = [("data", ctypes.c_void_p * 8), ("shape", ctypes.c_int * 3),
+    _fields_ = [("data", ctypes.c_void_p * 16), ("shape", ctypes.c_int * 3),
                 ("stride", ctypes.c_longlong * 3)]
 
 
@@ -537,6 +537,10 @@ def host_lib():
         H.enzoe_host_method_compute.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                                 ctypes.c_double, vp]
         H.enzoe_host_method_compute.restype = ctypes.c_int
+        H.enzoe_host_method_compute_passive.argtypes = [
+            vp, ctypes.POINTER(Block), ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+            ctypes.POINTER(vp), ctypes.c_double, vp]
+        H.enzoe_host_method_compute_passive.restype = ctypes.c_int
         H.enzoe_host_method_timestep.argtypes = [vp, ctypes.POINTER(Block), vp]
         H.enzoe_host_method_timestep.restype = ctypes.c_double
         _host = H
@@ -573,6 +577,15 @@ class HostMethod:
         if done != len(blocks):
             raise EnzoeError("compute_done called %d times for %d blocks"
                              % (done, len(blocks)))
+
+    def compute_passive(self, block, passives, dt, stream=None):
+        """compute() of one block with passive scalars {name: device ptr}."""
+        names = (ctypes.c_char_p * len(passives))(*[k.encode() for k in passives])
+        ptrs = (ctypes.c_void_p * len(passives))(*passives.values())
+        done = host_lib().enzoe_host_method_compute_passive(
+            self.handle, ctypes.byref(block.desc), len(passives), names, ptrs, dt, stream)
+        if done != 1:
+            raise EnzoeError("compute_done called %d times" % done)
 
     def timestep(self, block, stream=None):
         return host_lib().enzoe_host_method_timestep(self.handle, ctypes.byref(block.desc),

@@ -70,10 +70,6 @@ int key_slot(const std::string &key) {
   return it == slots.end() ? -1 : it->second;
 }
 
-void require_no_passives(const str_vec_t &passive_list, const char *where) {
-  ENZOE_ASSERT(where, "passively advected scalars are not provided by the "
-                      "device library", passive_list.empty());
-}
 
 // map restricted to `keys` (in that order)
 EnzoEFltArrayMap select_keys(const EnzoEFltArrayMap &map, const str_vec_t &keys) {
@@ -81,6 +77,12 @@ EnzoEFltArrayMap select_keys(const EnzoEFltArrayMap &map, const str_vec_t &keys)
   arrays.reserve(keys.size());
   for (const std::string &k : keys) arrays.push_back(map.at(k));
   return EnzoEFltArrayMap(map.name(), keys, arrays);
+}
+
+str_vec_t concat(const str_vec_t &a, const str_vec_t &b) {
+  str_vec_t out(a);
+  out.insert(out.end(), b.begin(), b.end());
+  return out;
 }
 
 const char *const kBNames[3] = {"bfield_x", "bfield_y", "bfield_z"};
@@ -225,11 +227,21 @@ EnzoEFltArrayMap EnzoEFltArrayMap::subarray_map(const CSlice &z, const CSlice &y
   return EnzoEFltArrayMap(name.empty() ? name_ : name, keys_, arrays);
 }
 
-enzoe_fmap EnzoEFltArrayMap::slot_map() const {
+enzoe_fmap EnzoEFltArrayMap::slot_map(const str_vec_t &passive_list) const {
   enzoe_fmap m;
   std::memset(&m, 0, sizeof(m));
   for (std::size_t q = 0; q < keys_.size(); q++) {
     int slot = key_slot(keys_[q]);
+    if (slot < 0) {
+      auto it = std::find(passive_list.begin(), passive_list.end(), keys_[q]);
+      if (it != passive_list.end()) {
+        const int idx = (int)(it - passive_list.begin());
+        ENZOE_ASSERT("EnzoEFltArrayMap::slot_map",
+                     "at most 8 passively advected scalars are supported",
+                     idx < ENZOE_MAP_SLOTS - ENZOE_SLOT_PASSIVE0);
+        slot = ENZOE_SLOT_PASSIVE0 + idx;
+      }
+    }
     if (slot < 0)
       ENZOE_ERROR("EnzoEFltArrayMap::slot_map",
                   "key \"" + keys_[q] + "\" has no counterpart in the device library");
@@ -266,9 +278,8 @@ EnzoPhysicsFluidProps::EnzoPhysicsFluidProps(double gamma, double density_floor,
 void EnzoPhysicsFluidProps::primitive_from_integration(
     const EnzoEFltArrayMap &integration_map, EnzoEFltArrayMap &primitive_map,
     int stale_depth, const str_vec_t &passive_list, bool) const {
-  require_no_passives(passive_list, "EnzoPhysicsFluidProps::primitive_from_integration");
-  const enzoe_fmap u = integration_map.slot_map();
-  const enzoe_fmap w = primitive_map.slot_map();
+  const enzoe_fmap u = integration_map.slot_map(passive_list);
+  const enzoe_fmap w = primitive_map.slot_map(passive_list);
   const int mhd = u.data[ENZOE_SLOT_BFIELD_X] != nullptr;
   check_rc(enzoe_primitive_from_integration(&u, &w, mhd, gamma_, stale_depth, g_stream),
            "EnzoPhysicsFluidProps::primitive_from_integration");
@@ -414,6 +425,14 @@ void Block::bind_(const enzoe_block &b) {
   put("bfieldi_z", b.bfieldi[2], 1, 0, 0);
 }
 
+void Block::add_passive_scalar(const std::string &name, double *data) {
+  const EFlt3DArray &rho = field_view("density");
+  ENZOE_ASSERT("Block::add_passive_scalar", "\"" + name + "\" is already a field",
+               fields_.count(name) == 0);
+  fields_[name] = EFlt3DArray(data, rho.shape(0), rho.shape(1), rho.shape(2));
+  passive_.push_back(name);
+}
+
 bool Block::has_field(const std::string &name) const {
   return fields_.count(name) != 0;
 }
@@ -474,10 +493,10 @@ void EnzoReconstructor::reconstruct_interface(const EnzoEFltArrayMap &prim_map,
                                               EnzoEFltArrayMap &primr_map,
                                               const int dim, const int stale_depth,
                                               const str_vec_t &passive_list) {
-  require_no_passives(passive_list, "EnzoReconstructor::reconstruct_interface");
-  const enzoe_fmap w = select_keys(prim_map, active_key_names_).slot_map();
-  const enzoe_fmap l = select_keys(priml_map, active_key_names_).slot_map();
-  const enzoe_fmap r = select_keys(primr_map, active_key_names_).slot_map();
+  const str_vec_t keys = concat(active_key_names_, passive_list);
+  const enzoe_fmap w = select_keys(prim_map, keys).slot_map(passive_list);
+  const enzoe_fmap l = select_keys(priml_map, keys).slot_map(passive_list);
+  const enzoe_fmap r = select_keys(primr_map, keys).slot_map(passive_list);
   const EnzoPhysicsFluidProps *fp = enzo::fluid_props();
   check_rc(enzoe_reconstruct_interface(device_name_(), theta_(), fp->density_floor(),
                                        fp->pressure_floor(), &w, &l, &r, dim,
@@ -540,10 +559,11 @@ void EnzoRiemann::solve(const EnzoEFltArrayMap &prim_map_l,
                         EnzoEFltArrayMap &flux_map, const int dim,
                         const int stale_depth, const str_vec_t &passive_list,
                         const EFlt3DArray *const) const {
-  require_no_passives(passive_list, "EnzoRiemann::solve");
-  const enzoe_fmap l = select_keys(prim_map_l, primitive_keys_).slot_map();
-  const enzoe_fmap r = select_keys(prim_map_r, primitive_keys_).slot_map();
-  const enzoe_fmap f = select_keys(flux_map, integration_keys_).slot_map();
+  const str_vec_t pk = concat(primitive_keys_, passive_list);
+  const str_vec_t ik = concat(integration_keys_, passive_list);
+  const enzoe_fmap l = select_keys(prim_map_l, pk).slot_map(passive_list);
+  const enzoe_fmap r = select_keys(prim_map_r, pk).slot_map(passive_list);
+  const enzoe_fmap f = select_keys(flux_map, ik).slot_map(passive_list);
   check_rc(enzoe_riemann_solve(solver_.c_str(), mhd_ ? 1 : 0,
                                enzo::fluid_props()->gamma(), &l, &r, &f, dim,
                                stale_depth, g_stream),
@@ -789,8 +809,8 @@ void EnzoIntegrationQuanUpdate::clear_dUcons_map(EnzoEFltArrayMap &dUcons_map,
                                                  enzo_float value,
                                                  const str_vec_t &passive_list) const noexcept {
   // EnzoIntegrationQuanUpdate.cpp:83-101
-  require_no_passives(passive_list, "EnzoIntegrationQuanUpdate::clear_dUcons_map");
-  const enzoe_fmap m = select_keys(dUcons_map, integration_keys_).slot_map();
+  const enzoe_fmap m =
+      select_keys(dUcons_map, concat(integration_keys_, passive_list)).slot_map(passive_list);
   check_rc(enzoe_fill_map(&m, value, g_stream), "EnzoIntegrationQuanUpdate::clear_dUcons_map");
 }
 
@@ -799,9 +819,9 @@ void EnzoIntegrationQuanUpdate::accumulate_flux_component(
     EnzoEFltArrayMap &dUcons_map, int stale_depth,
     const str_vec_t &passive_list) const noexcept {
   // EnzoIntegrationQuanUpdate.cpp:105-147
-  require_no_passives(passive_list, "EnzoIntegrationQuanUpdate::accumulate_flux_component");
-  const enzoe_fmap f = select_keys(flux_map, integration_keys_).slot_map();
-  const enzoe_fmap d = select_keys(dUcons_map, integration_keys_).slot_map();
+  const str_vec_t keys = concat(integration_keys_, passive_list);
+  const enzoe_fmap f = select_keys(flux_map, keys).slot_map(passive_list);
+  const enzoe_fmap d = select_keys(dUcons_map, keys).slot_map(passive_list);
   check_rc(enzoe_accumulate_flux_component(dim, dt, cell_width, &f, &d, stale_depth, g_stream),
            "EnzoIntegrationQuanUpdate::accumulate_flux_component");
 }
@@ -812,13 +832,13 @@ void EnzoIntegrationQuanUpdate::update_quantities(EnzoEFltArrayMap &initial_inte
                                                   const int stale_depth,
                                                   const str_vec_t &passive_list) const {
   // EnzoIntegrationQuanUpdate.cpp:183-238 + apply_floor_to_energy_and_sync
-  require_no_passives(passive_list, "EnzoIntegrationQuanUpdate::update_quantities");
-  const enzoe_fmap a = select_keys(initial_integration_map, integration_keys_).slot_map();
-  const enzoe_fmap d = select_keys(dUcons_map, integration_keys_).slot_map();
+  const str_vec_t keys = concat(integration_keys_, passive_list);
+  const enzoe_fmap a = select_keys(initial_integration_map, keys).slot_map(passive_list);
+  const enzoe_fmap d = select_keys(dUcons_map, keys).slot_map(passive_list);
   const bool mhd = out_integration_map.contains("bfield_x");
-  str_vec_t out_keys = integration_keys_;
+  str_vec_t out_keys = keys;
   if (mhd) out_keys.insert(out_keys.end(), {"bfield_x", "bfield_y", "bfield_z"});
-  const enzoe_fmap o = select_keys(out_integration_map, out_keys).slot_map();
+  const enzoe_fmap o = select_keys(out_integration_map, out_keys).slot_map(passive_list);
   const EnzoPhysicsFluidProps *fp = enzo::fluid_props();
   check_rc(enzoe_update_quantities(&a, &d, &o, mhd ? 1 : 0, fp->gamma(), fp->density_floor(),
                                    fp->pressure_floor(), stale_depth, g_stream),
@@ -972,18 +992,20 @@ double EnzoMHDIntegratorStageCommands::timestep(EnzoEFltArrayMap integration_map
 EnzoVlctScratchSpace::EnzoVlctScratchSpace(const std::array<int, 3> &shape,
                                            const str_vec_t &integration_keys,
                                            const str_vec_t &primitive_keys,
-                                           const str_vec_t &dUcons_keys)
-    : temp_integration_map("temp_integration", integration_keys, shape),
-      primitive_map("primitive", primitive_keys, shape),
-      priml_map("priml", primitive_keys, shape),
-      primr_map("primr", primitive_keys, shape),
-      xflux_map("xflux", integration_keys,
+                                           const str_vec_t &dUcons_keys,
+                                           const str_vec_t &passives)
+    : temp_integration_map("temp_integration", concat(integration_keys, passives), shape),
+      primitive_map("primitive", concat(primitive_keys, passives), shape),
+      priml_map("priml", concat(primitive_keys, passives), shape),
+      primr_map("primr", concat(primitive_keys, passives), shape),
+      xflux_map("xflux", concat(integration_keys, passives),
                 std::array<int, 3>{{shape[0], shape[1], shape[2] - 1}}),
-      yflux_map("yflux", integration_keys,
+      yflux_map("yflux", concat(integration_keys, passives),
                 std::array<int, 3>{{shape[0], shape[1] - 1, shape[2]}}),
-      zflux_map("zflux", integration_keys,
+      zflux_map("zflux", concat(integration_keys, passives),
                 std::array<int, 3>{{shape[0] - 1, shape[1], shape[2]}}),
-      dUcons_map("dUcons", dUcons_keys, shape) {}
+      dUcons_map("dUcons", concat(dUcons_keys, passives), shape),
+      passive_list(passives) {}
 
 // ----------------------------------------------------------------- method
 
@@ -1076,29 +1098,33 @@ EnzoMethodMHDVlct::~EnzoMethodMHDVlct() {
   delete bfield_method_;
 }
 
-EnzoEFltArrayMap EnzoMethodMHDVlct::get_integration_map_(Block *block) const noexcept {
-  // EnzoMethodMHDVlct.cpp:205-221
+EnzoEFltArrayMap EnzoMethodMHDVlct::get_integration_map_(
+    Block *block, const str_vec_t &passive_list) const noexcept {
+  // EnzoMethodMHDVlct.cpp:201-216 (passive scalars as densities)
+  const str_vec_t field_list = concat(integration_field_list_, passive_list);
   std::vector<EFlt3DArray> arrays;
-  arrays.reserve(integration_field_list_.size());
-  for (const std::string &name : integration_field_list_)
-    arrays.push_back(block->field_view(name));
-  return EnzoEFltArrayMap("integration", integration_field_list_, arrays);
+  arrays.reserve(field_list.size());
+  for (const std::string &name : field_list) arrays.push_back(block->field_view(name));
+  return EnzoEFltArrayMap("integration", field_list, arrays);
 }
 
 EnzoVlctScratchSpace *EnzoMethodMHDVlct::get_scratch_ptr_(
-    const std::array<int, 3> &shape) noexcept {
-  // EnzoMethodMHDVlct.cpp:236-246 (re-made if a block of another shape comes)
+    const std::array<int, 3> &shape, const str_vec_t &passive_list) noexcept {
+  // EnzoMethodMHDVlct.cpp:236-246 (re-made if a block of another shape or
+  // with other passive scalars comes)
   if (scratch_space_ != nullptr &&
       (scratch_space_->primitive_map.array_shape(0) != shape[0] ||
        scratch_space_->primitive_map.array_shape(1) != shape[1] ||
-       scratch_space_->primitive_map.array_shape(2) != shape[2])) {
+       scratch_space_->primitive_map.array_shape(2) != shape[2] ||
+       scratch_space_->passive_list != passive_list)) {
     delete scratch_space_;
     scratch_space_ = nullptr;
   }
   if (scratch_space_ == nullptr)
     scratch_space_ = new EnzoVlctScratchSpace(shape, integration_field_list_,
                                               primitive_field_list_,
-                                              integrator_->dUcons_map_keys());
+                                              integrator_->dUcons_map_keys(),
+                                              passive_list);
   return scratch_space_;
 }
 
@@ -1112,6 +1138,9 @@ void EnzoMethodMHDVlct::compute(Block *block) throw() {
                  "ghost depth must be at least " + std::to_string(min_gdepth_req) + ".",
                  block->ghost_depth() >= min_gdepth_req);
     if (fused_) {
+      ENZOE_ASSERT("EnzoMethodMHDVlct::compute",
+                   "passively advected scalars need device_fused = false",
+                   block->passive_scalars().empty());
       const enzoe_block b = block->desc();
       check_rc(enzoe_vlct_compute(device_, &b, 1, block->dt(), g_stream),
                "EnzoMethodMHDVlct::compute");
@@ -1133,6 +1162,9 @@ void EnzoMethodMHDVlct::compute_blocks(const std::vector<Block *> &blocks) throw
   for (Block *b : blocks) {
     ENZOE_ASSERT("EnzoMethodMHDVlct::compute_blocks",
                  "blocks of one batched launch share dt", b->dt() == blocks[0]->dt());
+    ENZOE_ASSERT("EnzoMethodMHDVlct::compute_blocks",
+                 "passively advected scalars need device_fused = false",
+                 b->passive_scalars().empty());
     desc.push_back(b->desc());
   }
   check_rc(enzoe_vlct_compute(device_, desc.data(), (int)desc.size(), blocks[0]->dt(),
@@ -1143,12 +1175,12 @@ void EnzoMethodMHDVlct::compute_blocks(const std::vector<Block *> &blocks) throw
 
 void EnzoMethodMHDVlct::compute_staged_(Block *block) noexcept {
   // EnzoMethodMHDVlct.cpp:371-497
-  const str_vec_t passive_list;
-  EnzoEFltArrayMap external_integration_map = get_integration_map_(block);
+  const str_vec_t passive_list = block->passive_scalars();
+  EnzoEFltArrayMap external_integration_map = get_integration_map_(block, passive_list);
   const std::array<int, 3> shape = {external_integration_map.array_shape(0),
                                     external_integration_map.array_shape(1),
                                     external_integration_map.array_shape(2)};
-  EnzoVlctScratchSpace *const scratch = get_scratch_ptr_(shape);
+  EnzoVlctScratchSpace *const scratch = get_scratch_ptr_(shape, passive_list);
   EnzoEFltArrayMap temp_integration_map = scratch->temp_integration_map;
   EnzoEFltArrayMap primitive_map = scratch->primitive_map;
   EnzoEFltArrayMap priml_map = scratch->priml_map;
@@ -1195,7 +1227,7 @@ double EnzoMethodMHDVlct::timestep(Block *block) throw() {
              "EnzoMethodMHDVlct::timestep");
     return dt;  // the library applies the same courant factor
   }
-  EnzoEFltArrayMap integration_map = get_integration_map_(block);
+  EnzoEFltArrayMap integration_map = get_integration_map_(block, str_vec_t());
   EFlt3DArray pressure = block->field_view("pressure");
   enzo::fluid_props()->pressure_from_integration(integration_map, pressure, 0);
   const double *w = block->CellWidth();

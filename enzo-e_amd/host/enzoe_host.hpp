@@ -130,9 +130,10 @@ class EnzoEFltArrayMap {
   EnzoEFltArrayMap subarray_map(const CSlice &z, const CSlice &y,
                                 const CSlice &x,
                                 const std::string &name = "") const;
-  // the device descriptor: views by slot (ENZOE_SLOT_*); ERROR for a key
-  // without a slot or views of different shapes or strides
-  enzoe_fmap slot_map() const;
+  // the device descriptor: views by slot (ENZOE_SLOT_*; the keys of
+  // passive_list take slots ENZOE_SLOT_PASSIVE0 + their index); ERROR for a
+  // key without a slot or views of different shapes or strides
+  enzoe_fmap slot_map(const str_vec_t &passive_list = str_vec_t()) const;
 
  private:
   std::string name_;
@@ -205,6 +206,10 @@ class Block {
         const std::array<double, 3> &cell_width, bool mhd);
   // fields owned by the caller
   explicit Block(const enzoe_block &fields);
+  // a passively advected scalar (a density, cell-centred, the block's
+  // shape), owned by the caller: the "color" field group of the reference
+  void add_passive_scalar(const std::string &name, double *data);
+  const str_vec_t &passive_scalars() const { return passive_; }
 
   EFlt3DArray field_view(const std::string &name) const;
   bool has_field(const std::string &name) const;
@@ -222,6 +227,7 @@ class Block {
   void bind_(const enzoe_block &b);
   std::shared_ptr<void> storage_;
   std::map<std::string, EFlt3DArray> fields_;
+  str_vec_t passive_;
   std::array<double, 3> cell_width_{{0., 0., 0.}};
   int ghost_ = 0;
   double dt_ = 0.;
@@ -525,9 +531,11 @@ struct EnzoVlctScratchSpace {
   EnzoVlctScratchSpace(const std::array<int, 3> &field_shape,
                        const str_vec_t &integration_keys,
                        const str_vec_t &primitive_keys,
-                       const str_vec_t &dUcons_keys);
+                       const str_vec_t &dUcons_keys,
+                       const str_vec_t &passive_list);
   EnzoEFltArrayMap temp_integration_map, primitive_map, priml_map, primr_map,
       xflux_map, yflux_map, zflux_map, dUcons_map;
+  str_vec_t passive_list;
 };
 
 // ----------------------------------------------------------------- method
@@ -555,8 +563,10 @@ class EnzoMethodMHDVlct : public Method {
   bool device_fused() const { return fused_; }
 
  private:
-  EnzoEFltArrayMap get_integration_map_(Block *block) const noexcept;
-  EnzoVlctScratchSpace *get_scratch_ptr_(const std::array<int, 3> &field_shape) noexcept;
+  EnzoEFltArrayMap get_integration_map_(Block *block,
+                                        const str_vec_t &passive_list) const noexcept;
+  EnzoVlctScratchSpace *get_scratch_ptr_(const std::array<int, 3> &field_shape,
+                                         const str_vec_t &passive_list) noexcept;
   void compute_staged_(Block *block) noexcept;
 
   std::string time_scheme_;

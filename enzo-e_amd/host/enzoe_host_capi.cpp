@@ -56,6 +56,20 @@ int enzoe_host_method_compute(enzoe_host_method_t m, const enzoe_block *blocks,
   return done;
 }
 
+int enzoe_host_method_compute_passive(enzoe_host_method_t m,
+                                      const enzoe_block *block, int npassive,
+                                      const char *const *names,
+                                      double *const *passive, double dt,
+                                      enzoe_stream_t stream) {
+  if (!m || !block || npassive < 0 || (npassive > 0 && (!names || !passive))) return 0;
+  enzoe_host_set_stream(stream);
+  Block b(*block);
+  for (int i = 0; i < npassive; i++) b.add_passive_scalar(names[i], passive[i]);
+  b.set_dt(dt);
+  m->method->compute(&b);
+  return b.compute_done_count();
+}
+
 double enzoe_host_method_timestep(enzoe_host_method_t m, const enzoe_block *block,
                                   enzoe_stream_t stream) {
   if (!m || !block) return -1.0;

@@ -174,7 +174,7 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
  * passed without copies.  A map (EnzoEFltArrayMap = ViewMap<double>,
  * enzo_typedefs.hpp:36; view_ViewMap.hpp:43-395) is a set of same-shape,
  * same-stride views addressed by slot instead of by key:                    */
-#define ENZOE_MAP_SLOTS 8
+#define ENZOE_MAP_SLOTS 16
 #define ENZOE_SLOT_DENSITY 0     /* "density"                               */
 #define ENZOE_SLOT_VELOCITY_X 1  /* "velocity_x"                            */
 #define ENZOE_SLOT_VELOCITY_Y 2  /* "velocity_y"                            */
@@ -184,6 +184,10 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
 #define ENZOE_SLOT_BFIELD_X 5    /* "bfield_x"                              */
 #define ENZOE_SLOT_BFIELD_Y 6    /* "bfield_y"                              */
 #define ENZOE_SLOT_BFIELD_Z 7    /* "bfield_z"                              */
+#define ENZOE_SLOT_PASSIVE0 8    /* passively advected scalars, slots 8..15:
+                                    conserved form (densities) in integration,
+                                    dU and flux maps, specific form (mass
+                                    fractions) in primitive maps            */
 
 typedef struct enzoe_farray {
   double *data;           /* element (0,0,0) of the view                      */
@@ -222,7 +226,10 @@ int enzoe_reconstructor_staling(const char *name, int *total_rate,
 /* EnzoRiemann::construct_riemann + EnzoRiemannImpl::solve
  * (riemann/EnzoRiemann.cpp:26-68, riemann/EnzoRiemannImpl.hpp:265-338):
  * "hlld", "hlle" (MHD) or "hllc" (hydro) over [stale, n - stale) of the
- * flux view.  Other names or combinations: ENZOE_ERR_CONFIG.                */
+ * flux view, then the upwinded flux of every passive slot present in both
+ * states (enzo_riemann_utils::solve_passive_advection,
+ * riemann/EnzoRiemannUtils.hpp:224-314).  Other names or combinations:
+ * ENZOE_ERR_CONFIG.                                                         */
 int enzoe_riemann_solve(const char *solver, int mhd, double gamma,
                         const enzoe_fmap *priml, const enzoe_fmap *primr,
                         const enzoe_fmap *flux, int dim, int stale_depth,
@@ -291,7 +298,9 @@ int enzoe_accumulate_flux_component(int dim, double dt, double cell_width,
  * EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync
  * (EnzoPhysicsFluidProps.cpp:162-290) at stale_depth + 1.  initial/dUcons
  * need slots 0-4; out's bfield slots (MHD) must already hold the updated
- * cell-centred B (the reference runs CT first for this reason).             */
+ * cell-centred B (the reference runs CT first for this reason).  Passive
+ * slots of dUcons (update_passive_scalars_, :242-268) need the same slots in
+ * initial and out.                                                          */
 int enzoe_update_quantities(const enzoe_fmap *initial,
                             const enzoe_fmap *dUcons, const enzoe_fmap *out,
                             int mhd, double gamma, double density_floor,
@@ -300,7 +309,8 @@ int enzoe_update_quantities(const enzoe_fmap *initial,
 
 /* EnzoPhysicsFluidProps::primitive_from_integration
  * (EnzoPhysicsFluidProps.cpp:64-138) with the pressure of
- * EnzoComputePressure::compute_pressure (EnzoComputePressure.cpp:177-189).  */
+ * EnzoComputePressure::compute_pressure (EnzoComputePressure.cpp:177-189);
+ * passive slots present in both maps become mass fractions (density / rho).*/
 int enzoe_primitive_from_integration(const enzoe_fmap *integration,
                                      const enzoe_fmap *primitive, int mhd,
                                      double gamma, int stale_depth,

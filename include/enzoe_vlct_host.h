@@ -40,6 +40,16 @@ double enzoe_host_method_courant(enzoe_host_method_t m);
 int enzoe_host_method_compute(enzoe_host_method_t m, const enzoe_block *blocks,
                               int n, double dt, enzoe_stream_t stream);
 
+/* EnzoMethodMHDVlct::compute(Block*) on one block carrying `npassive`
+ * passively advected scalars (densities, cell-centred, the block's shape;
+ * the reference's "color" field group): names[i] and device arrays
+ * passive[i].  Needs device_fused = false.  Returns compute_done() calls.  */
+int enzoe_host_method_compute_passive(enzoe_host_method_t m,
+                                      const enzoe_block *block, int npassive,
+                                      const char *const *names,
+                                      double *const *passive, double dt,
+                                      enzoe_stream_t stream);
+
 /* EnzoMethodMHDVlct::timestep(Block*) (control_stopping.cpp:67-69); writes
  * the block's pressure field.                                               */
 double enzoe_host_method_timestep(enzoe_host_method_t m,
