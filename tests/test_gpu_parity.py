@@ -267,6 +267,7 @@ def test_mixed_block_shapes_in_one_call():
     method.compute(blocks, dt)
     for g in grids:
         g.refresh()
+        po.lib().o_vlct_timestep(ctypes.byref(g.cfg), ctypes.byref(g.block(0)))  # pressure
         assert po.lib().o_vlct_compute(ctypes.byref(g.cfg), ctypes.byref(g.block(0)), dt) == 0
     ev.synchronize()
     for d, g in zip(dgs, grids):
