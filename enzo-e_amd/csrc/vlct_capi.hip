@@ -154,6 +154,7 @@ int enzoe_vlct_destroy(enzoe_vlct_t m) {
   if (m->d_ptrtab) (void)hipFree(m->d_ptrtab);
   if (m->d_dts) (void)hipFree(m->d_dts);
   if (m->d_rtab) (void)hipFree(m->d_rtab);
+  if (m->d_fdtab) (void)hipFree(m->d_fdtab);
   delete m;
   return ENZOE_OK;
 }
@@ -164,7 +165,8 @@ namespace {
 
 // blocks grouped into runs of equal shape, each run one compute_group
 int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
-                 double dt, const double *d_dts, hipStream_t s) {
+                 double dt, const double *d_dts, hipStream_t s,
+                 const StoreArgs *st = nullptr) {
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
   int b0 = 0;
@@ -174,12 +176,18 @@ int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
            blocks[b1].my == blocks[b0].my && blocks[b1].mz == blocks[b0].mz)
       b1++;
     const double *dts = d_dts ? d_dts + b0 : nullptr;
+    StoreArgs run_st;
+    if (st) {
+      run_st = *st;
+      run_st.d_fdtab = st->d_fdtab + b0;
+    }
+    const StoreArgs *rs = st ? &run_st : nullptr;
     int rc;
     if (m->precision == ENZOE_PREC_DOUBLE) {
-      rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s)
-                  : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, dts, s);
+      rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s, rs)
+                  : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, dts, s, rs);
     } else {
-      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, dts, s);
+      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, dts, s, rs);
     }
     if (rc) return rc;
     b0 = b1;
@@ -213,6 +221,43 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     d_dts = (const double *)m->d_dts;
   }
   return compute_runs(m, blocks, n, 0.0, d_dts, s);
+}
+
+int enzoe_vlct_compute_store_fluxes(enzoe_vlct_t m, const enzoe_block *blocks,
+                                    int n, double dt, int nf,
+                                    const int *fields,
+                                    double *const *flux_data,
+                                    enzoe_stream_t stream) {
+  if (!m || !blocks || n <= 0 || !fields || !flux_data) return ENZOE_ERR_ARG;
+  // EnzoMethodMHDVlct.cpp:137-141: flux corrections are hydro-only
+  if (m->mhd) return ENZOE_ERR_CONFIG;
+  if (nf < 1 || nf > 8) return ENZOE_ERR_ARG;
+  StoreArgs st;
+  st.sl.nf = nf;
+  for (int i = 0; i < 8; i++) st.sl.slot[i] = 0;
+  for (int i = 0; i < nf; i++) {
+    if (fields[i] < ENZOE_SLOT_DENSITY || fields[i] > ENZOE_SLOT_ENERGY)
+      return ENZOE_ERR_ARG;
+    st.sl.slot[i] = fields[i];
+  }
+  for (int i = 0; i < n; i++) {
+    const enzoe_block &b = blocks[i];
+    const int ext[3] = {b.mx - 2 * b.ghost, b.my - 2 * b.ghost, b.mz - 2 * b.ghost};
+    // FaceFluxes needs even block dimensions (data_FaceFluxes.cpp:38-45)
+    for (int a = 0; a < 3; a++)
+      if (ext[a] < 2 || (ext[a] & 1)) return ENZOE_ERR_ARG;
+    if (!flux_data[i]) return ENZOE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (int rc = ensure_bytes(&m->d_fdtab, &m->fdtab_bytes, n * sizeof(double *)))
+    return rc;
+  // the copy completes before the host array may change
+  if (hipMemcpyAsync(m->d_fdtab, flux_data, n * sizeof(double *),
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ENZOE_ERR_HIP;
+  st.d_fdtab = (double *const *)m->d_fdtab;
+  return compute_runs(m, blocks, n, dt, nullptr, s, &st);
 }
 
 }  // extern "C"

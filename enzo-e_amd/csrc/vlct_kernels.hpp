@@ -37,6 +37,7 @@ struct BlockDesc {
   double *E[3];        // edge EMFs along x, y, z
   double *Ec[3];       // cell-centred E = -v x B of the stage's state
   int mx, my, mz;
+  int g;               // ghost depth
   double cw[3];
 };
 
@@ -674,6 +675,98 @@ k_refresh(double *const *__restrict__ table, RefreshGrid G) {
   double *const *tab = table + (size_t)f * nb;
   tab[bidx][idx3(k, j, i, ext[1], ext[0])] =
       tab[sbid][idx3(si[2], si[1], si[0], ext[1], ext[0])];
+}
+
+// ------------------------------------------------------- flux corrections
+
+// One block's FluxData as a single array (src/Cello/data_FluxData.cpp:
+// 57-87): for each field i_f, axis, face the block's face fluxes, then the
+// neighbour's, m[axis] values each.  The face array of axis 0 is (nz, ny)
+// with y fastest, of axis 1 (nz, nx), of axis 2 (ny, nx)
+// (FaceFluxes::flux_array, src/Cello/data_FaceFluxes.cpp:114-126).
+struct FluxLayout {
+  int n[3];        // active cells along x, y, z
+  long m[3];       // face sizes
+  long per_field;  // values per field (4 faces' worth per axis)
+
+  __host__ __device__ void init(int nx, int ny, int nz) {
+    n[0] = nx; n[1] = ny; n[2] = nz;
+    m[0] = (long)ny * nz;
+    m[1] = (long)nx * nz;
+    m[2] = (long)nx * ny;
+    per_field = 4 * (m[0] + m[1] + m[2]);
+  }
+  __host__ __device__ long offset(int i_f, int axis, int face, int nbr) const {
+    long o = (long)i_f * per_field;
+    for (int a = 0; a < axis; a++) o += 4 * m[a];
+    return o + (long)face * 2 * m[axis] + (nbr ? m[axis] : 0);
+  }
+  // entry t -> (field, axis, face, neighbour part, index in the face array)
+  __host__ __device__ void decode(long t, int &i_f, int &axis, int &face,
+                                  int &nbr, long &idx) const {
+    i_f = (int)(t / per_field);
+    long r = t - (long)i_f * per_field;
+    axis = 0;
+    while (axis < 2 && r >= 4 * m[axis]) {
+      r -= 4 * m[axis];
+      axis++;
+    }
+    face = (int)(r / (2 * m[axis]));
+    r -= (long)face * 2 * m[axis];
+    nbr = r >= m[axis];
+    idx = r - (nbr ? m[axis] : 0);
+  }
+  // active-region cell (x, y, z), from 0, of entry idx of a face array of
+  // axis: its two in-face coordinates; the normal one is left at 0
+  __host__ __device__ void face_cell(int axis, long idx, int p[3]) const {
+    const int a0 = (axis == 0) ? 1 : 0;          // fastest in-face axis
+    const int a1 = (axis == 2) ? 1 : 2;          // slowest in-face axis
+    p[axis] = 0;
+    p[a0] = (int)(idx % n[a0]);
+    p[a1] = (int)(idx / n[a0]);
+  }
+};
+
+// Fields saved in the FluxData, in the order of the "conserved" group.
+struct StoreSlots {
+  int nf;
+  int slot[8];
+};
+
+// After the final stage: dt/dx * F of that stage's fluxes through the six
+// faces of the active region (EnzoMethodMHDVlct::save_fluxes_for_
+// corrections_, EnzoMethodMHDVlct.cpp:250-330: the faces between cells g-1
+// and g, and m-g-1 and m-g) into the block part of the FluxData, and zeros
+// into the neighbour part (allocate_FC_flux_buffer_, :333-355, allocates the
+// FluxData zeroed every cycle).  One thread per FluxData entry.
+template <typename FluxT>
+__global__ void __launch_bounds__(256)
+k_store_fluxes(const BlockDesc<FluxT> *__restrict__ blocks,
+               double *const *__restrict__ fd, StoreSlots sl, double dt,
+               const double *__restrict__ dts) {
+  const int by = blockIdx.y;
+  const BlockDesc<FluxT> &b = blocks[by];
+  const int g = b.g;
+  FluxLayout L;
+  L.init(b.mx - 2 * g, b.my - 2 * g, b.mz - 2 * g);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.per_field * sl.nf) return;
+  int i_f, axis, face, nbr;
+  long idx;
+  L.decode(t, i_f, axis, face, nbr, idx);
+  double v = 0.0;
+  if (!nbr) {
+    int p[3];
+    L.face_cell(axis, idx, p);
+    const int m[3] = {b.mx, b.my, b.mz};
+    for (int a = 0; a < 3; a++) p[a] += g;
+    p[axis] = face ? m[axis] - g - 1 : g - 1;
+    const int c = (p[2] * b.my + p[1]) * b.mx + p[0];
+    // dt_dxi = dt/cell_width (EnzoMethodMHDVlct.cpp:265)
+    const double dt_dxi = (dts ? dts[by] : dt) / b.cw[axis];
+    v = dt_dxi * (double)gld(b.F[axis][sl.slot[i_f]], c);
+  }
+  gst(fd[by], t, v);
 }
 
 }  // namespace enzoe

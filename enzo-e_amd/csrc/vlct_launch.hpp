@@ -99,6 +99,8 @@ struct enzoe_vlct_method {
   int force_mask = -1;
   void *d_dts = nullptr;  // per-block dt uploaded by enzoe_vlct_compute_dt
   size_t dts_bytes = 0;
+  void *d_fdtab = nullptr;  // FluxData pointers of enzoe_vlct_compute_store_fluxes
+  size_t fdtab_bytes = 0;
 };
 
 namespace enzoe {
@@ -180,6 +182,7 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
       }
     d.pressure = b.pressure;
     d.mx = b.mx; d.my = b.my; d.mz = b.mz;
+    d.g = b.ghost;
     for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
     const size_t ncc = (size_t)b.mx * b.my * b.mz;
     const int nq = m->mhd ? 8 : 5;
@@ -261,11 +264,20 @@ inline long cube(int mx, int my, int mz, int lo, int hi_trim) {
   return a * b * c;
 }
 
+// Saving the final stage's face fluxes for flux corrections: the fields and
+// the device table of one FluxData pointer per block of the call.
+struct StoreArgs {
+  StoreSlots sl;
+  double *const *d_fdtab;
+};
+
 // One VL+CT step (EnzoMethodMHDVlct::compute) on blocks of equal shape.
 // dts: per-block dt in device memory (n values) or null (all blocks use dt).
+// st: store the final stage's face fluxes (hydro only) or null.
 template <typename FluxT, bool MHD>
 int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
-                  double dt, const double *dts, hipStream_t s) {
+                  double dt, const double *dts, hipStream_t s,
+                  const StoreArgs *st = nullptr) {
   const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
   const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd);
   // chunk size: as many blocks as fit the scratch budget (at least 1); the
@@ -353,6 +365,20 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       }
       stale += total_staling(recon);
     }
+    if (st) {
+      // the chunk's flux scratch still holds the final stage's fluxes
+      long work = 0;
+      for (int bi = c0; bi < c1; bi++) {
+        const int g = blocks[bi].ghost;
+        FluxLayout L;
+        L.init(mx - 2 * g, my - 2 * g, mz - 2 * g);
+        work = std::max(work, L.per_field * st->sl.nf);
+      }
+      KernelTimer kt(s, "k_store_fluxes");
+      hipLaunchKernelGGL((k_store_fluxes<FluxT>), dim3(nblk8(work, 256), nb),
+                         dim3(256), 0, s, d, st->d_fdtab + c0, st->sl, dt,
+                         dts ? dts + c0 : nullptr);
+    }
   }
   if (hipGetLastError() != hipSuccess) return ENZOE_ERR_HIP;
   return 0;
@@ -360,6 +386,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
 
 // FP32 flux path, compiled in vlct_mixed.hip
 int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
-                  double dt, const double *dts, hipStream_t s);
+                  double dt, const double *dts, hipStream_t s,
+                  const StoreArgs *st = nullptr);
 
 }  // namespace enzoe

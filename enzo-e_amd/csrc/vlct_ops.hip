@@ -872,3 +872,258 @@ int enzoe_integrator_timestep(const enzoe_fmap *integration,
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- flux corrections
+
+namespace {
+
+// save_fluxes_for_corrections_ for one dim (EnzoMethodMHDVlct.cpp:250-330):
+// one thread per (field, face, face-array entry) of the block part.
+__global__ void k_op_save_fluxes(KMap F, StoreSlots sl, int dim, double dt_dxi,
+                                 int g, FluxLayout L, double *fd) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = L.m[dim];
+  if (t >= 2 * m * sl.nf) return;
+  const int i_f = (int)(t / (2 * m));
+  const long r = t - (long)i_f * 2 * m;
+  const int face = (int)(r / m);
+  const long idx = r - face * m;
+  int p[3];
+  L.face_cell(dim, idx, p);
+  for (int a = 0; a < 3; a++) p[a] += g;
+  // the flux view is the cell shape minus one along dim
+  p[dim] = face ? F.n[dim] - g : g - 1;
+  const double v = dt_dxi * F.p[sl.slot[i_f]][off(F.s, p)];
+  gst(fd, L.offset(i_f, dim, face, 0) + idx, v);
+}
+
+// Block::refresh_load_flux_face_ (control_refresh.cpp:1458-1500): the fine
+// block's fluxes on (axis, face) coarsened as FaceFluxes::coarsen does for a
+// rank-3 cell-centred face (data_FaceFluxes.cpp:128-250: every fine value
+// added four times with weight 0.25 * 0.125, fine faces in loop order) into
+// quadrant `child` of the coarse face, then summed into the coarse block's
+// neighbour fluxes on (axis, 1 - face) (FluxData::sum_neighbor_fluxes,
+// data_FluxData.hpp:136-145).  One thread per (field, coarse entry).
+struct Child {
+  int c[3];
+};
+
+__global__ void k_op_send_to_coarse(const double *__restrict__ fine,
+                                    double *__restrict__ coarse, int nf,
+                                    FluxLayout L, int axis, int face,
+                                    Child ch) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = L.m[axis];
+  if (t >= m * nf) return;
+  const int i_f = (int)(t / m);
+  const long idx = t - (long)i_f * m;
+  const int a0 = (axis == 0) ? 1 : 0, a1 = (axis == 2) ? 1 : 2;
+  const int nu = L.n[a0], nv = L.n[a1];
+  const int u = (int)(idx % nu), v = (int)(idx / nu);
+  const int hu = nu / 2, hv = nv / 2;
+  const int cu = ch.c[a0], cv = ch.c[a1];
+  double acc = 0.0;
+  if (u >= cu * hu && u < cu * hu + hu && v >= cv * hv && v < cv * hv + hv) {
+    const double *src = fine + L.offset(i_f, axis, face, 0);
+    const int u2 = 2 * (u - cu * hu), v2 = 2 * (v - cv * hv);
+    const double w = 0.25 * 0.125;
+    for (int dv = 0; dv < 2; dv++)
+      for (int du = 0; du < 2; du++) {
+        const double f = src[(long)(v2 + dv) * nu + (u2 + du)];
+        acc += w * f;
+        acc += w * f;
+        acc += w * f;
+        acc += w * f;
+      }
+  }
+  double *dst = coarse + L.offset(i_f, axis, 1 - face, 1);
+  dst[idx] = dst[idx] + acc;
+}
+
+// MethodFluxCorrect::flux_correct_ (problem_MethodFluxCorrect.cpp:425-560)
+// for one leaf block, one thread per cell of the shell of the active
+// region: density first (its start value kept), then every other field,
+// through rho * q for fields that must be made conservative; a cell on
+// several corrected faces takes them in axis, then face, order
+// (flux_correct_helper_, :275-370), and is divided by the corrected density
+// once (divide_flux_corrected_cells_by_density_, :372-423).
+struct CorrectArgs {
+  double *field[16];
+  int conservative[16];
+  int nf, i_density;
+  int correct[6];
+};
+
+__global__ void k_op_flux_correct(CorrectArgs a, const double *__restrict__ fd,
+                                  FluxLayout L, int g, int mx, int my) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+  // shell: the two x faces, then the y faces without the x boundary, then
+  // the z faces without the x and y boundaries
+  const long nA = 2L * ny * nz, nB = 2L * (nx - 2) * nz,
+             nC = 2L * (nx - 2) * (ny - 2);
+  int p[3];
+  if (t < nA) {
+    const long r = t % ((long)ny * nz);
+    p[0] = (t < (long)ny * nz) ? 0 : nx - 1;
+    p[1] = (int)(r % ny);
+    p[2] = (int)(r / ny);
+  } else if (t < nA + nB) {
+    const long u = t - nA, half = (long)(nx - 2) * nz;
+    const long r = u % half;
+    p[1] = (u < half) ? 0 : ny - 1;
+    p[0] = 1 + (int)(r % (nx - 2));
+    p[2] = (int)(r / (nx - 2));
+  } else if (t < nA + nB + nC) {
+    const long u = t - nA - nB, half = (long)(nx - 2) * (ny - 2);
+    const long r = u % half;
+    p[2] = (u < half) ? 0 : nz - 1;
+    p[0] = 1 + (int)(r % (nx - 2));
+    p[1] = 1 + (int)(r / (nx - 2));
+  } else {
+    return;
+  }
+  bool on[6];
+  long ib[3];
+  bool any = false;
+  for (int ax = 0; ax < 3; ax++) {
+    const int a0 = (ax == 0) ? 1 : 0, a1 = (ax == 2) ? 1 : 2;
+    ib[ax] = p[a0] + (long)L.n[a0] * p[a1];
+    for (int f = 0; f < 2; f++) {
+      on[2 * ax + f] = a.correct[2 * ax + f] && p[ax] == (f ? L.n[ax] - 1 : 0);
+      any = any || on[2 * ax + f];
+    }
+  }
+  if (!any) return;
+  const long c = (p[0] + g) + (long)mx * ((p[1] + g) + (long)my * (p[2] + g));
+  double rho_old = 0.0, rho_new = 0.0;
+  if (a.i_density >= 0) {
+    rho_old = a.field[a.i_density][c];
+    double r = rho_old;
+    for (int k = 0; k < 6; k++) {
+      if (!on[k]) continue;
+      const int ax = k >> 1, f = k & 1;
+      const double *bf = fd + L.offset(a.i_density, ax, f, 0);
+      const double *nb = fd + L.offset(a.i_density, ax, f, 1);
+      r += (double)(2 * f - 1) * (bf[ib[ax]] - nb[ib[ax]]);
+    }
+    rho_new = r;
+    a.field[a.i_density][c] = r;
+  }
+  for (int i_f = 0; i_f < a.nf; i_f++) {
+    if (i_f == a.i_density) continue;
+    double q = a.conservative[i_f] ? rho_old * a.field[i_f][c] : a.field[i_f][c];
+    for (int k = 0; k < 6; k++) {
+      if (!on[k]) continue;
+      const int ax = k >> 1, f = k & 1;
+      const double *bf = fd + L.offset(i_f, ax, f, 0);
+      const double *nb = fd + L.offset(i_f, ax, f, 1);
+      q += (double)(2 * f - 1) * (bf[ib[ax]] - nb[ib[ax]]);
+    }
+    a.field[i_f][c] = a.conservative[i_f] ? q / rho_new : q;
+  }
+}
+
+bool even_extents(int nx, int ny, int nz) {
+  return nx >= 2 && ny >= 2 && nz >= 2 && !(nx & 1) && !(ny & 1) && !(nz & 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+long long enzoe_flux_data_size(int nf, int nx, int ny, int nz) {
+  if (nf < 0 || nx < 1 || ny < 1 || nz < 1) return -1;
+  FluxLayout L;
+  L.init(nx, ny, nz);
+  return (long long)(L.per_field * nf);
+}
+
+int enzoe_save_fluxes_for_corrections(const enzoe_fmap *flux, int dim,
+                                      double cell_width, double dt, int ghost,
+                                      int nf, const int *fields,
+                                      double *flux_data,
+                                      enzoe_stream_t stream) {
+  KMap f;
+  if (!valid_dim(dim) || ghost < 1 || nf < 1 || nf > 8 || !fields || !flux_data)
+    return ENZOE_ERR_ARG;
+  StoreSlots sl;
+  sl.nf = nf;
+  unsigned need = 0;
+  for (int i = 0; i < 8; i++) sl.slot[i] = 0;
+  for (int i = 0; i < nf; i++) {
+    if (fields[i] < 0 || fields[i] >= ENZOE_MAP_SLOTS) return ENZOE_ERR_ARG;
+    sl.slot[i] = fields[i];
+    need |= 1u << fields[i];
+  }
+  if (!to_kmap(flux, need, f)) return ENZOE_ERR_ARG;
+  int cell[3];
+  for (int a = 0; a < 3; a++) cell[a] = f.n[a] + (a == dim ? 1 : 0);
+  FluxLayout L;
+  L.init(cell[0] - 2 * ghost, cell[1] - 2 * ghost, cell[2] - 2 * ghost);
+  if (L.n[0] < 1 || L.n[1] < 1 || L.n[2] < 1) return ENZOE_ERR_ARG;
+  // dt_dxi = dt/cell_width (EnzoMethodMHDVlct.cpp:265)
+  const double dt_dxi = dt / cell_width;
+  const long work = 2 * L.m[dim] * nf;
+  hipLaunchKernelGGL(k_op_save_fluxes, dim3(nblk(work)), dim3(256), 0,
+                     (hipStream_t)stream, f, sl, dim, dt_dxi, ghost, L,
+                     flux_data);
+  return launched();
+}
+
+int enzoe_flux_data_send_to_coarse(const double *fine_flux_data,
+                                   double *coarse_flux_data, int nf, int nx,
+                                   int ny, int nz, int axis, int face,
+                                   const int child[3], enzoe_stream_t stream) {
+  if (!fine_flux_data || !coarse_flux_data || nf < 1 || !valid_dim(axis) ||
+      (face != 0 && face != 1) || !child || !even_extents(nx, ny, nz))
+    return ENZOE_ERR_ARG;
+  Child ch;
+  for (int a = 0; a < 3; a++) {
+    if (child[a] != 0 && child[a] != 1) return ENZOE_ERR_ARG;
+    ch.c[a] = child[a];
+  }
+  FluxLayout L;
+  L.init(nx, ny, nz);
+  const long work = L.m[axis] * nf;
+  hipLaunchKernelGGL(k_op_send_to_coarse, dim3(nblk(work)), dim3(256), 0,
+                     (hipStream_t)stream, fine_flux_data, coarse_flux_data, nf,
+                     L, axis, face, ch);
+  return launched();
+}
+
+int enzoe_flux_correct(double *const *fields, int nf, int i_density,
+                       const int *conservative, const double *flux_data,
+                       int nx, int ny, int nz, int ghost, const int correct[6],
+                       enzoe_stream_t stream) {
+  if (!fields || nf < 0 || nf > 16 || !conservative || !flux_data || !correct ||
+      ghost < 0 || i_density >= nf || !even_extents(nx, ny, nz))
+    return ENZOE_ERR_ARG;
+  if (nf == 0) return ENZOE_OK;
+  CorrectArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nf = nf;
+  a.i_density = i_density < 0 ? -1 : i_density;
+  bool any = false;
+  for (int k = 0; k < 6; k++) {
+    a.correct[k] = correct[k] ? 1 : 0;
+    any = any || a.correct[k];
+  }
+  for (int i = 0; i < nf; i++) {
+    if (!fields[i]) return ENZOE_ERR_ARG;
+    a.field[i] = fields[i];
+    a.conservative[i] = (conservative[i] && i != a.i_density) ? 1 : 0;
+    // problem_MethodFluxCorrect.cpp:520-524: needs the density field
+    if (a.conservative[i] && a.i_density < 0) return ENZOE_ERR_ARG;
+  }
+  if (!any) return ENZOE_OK;
+  FluxLayout L;
+  L.init(nx, ny, nz);
+  const long shell = 2L * ny * nz + 2L * (nx - 2) * nz + 2L * (nx - 2) * (ny - 2);
+  hipLaunchKernelGGL(k_op_flux_correct, dim3(nblk(shell)), dim3(256), 0,
+                     (hipStream_t)stream, a, flux_data, L, ghost,
+                     nx + 2 * ghost, ny + 2 * ghost);
+  return launched();
+}
+
+}  // extern "C"

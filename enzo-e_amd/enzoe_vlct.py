@@ -506,8 +506,91 @@ def ops():
         L.enzoe_primitive_from_integration.argtypes = [PM, PM, i, d, i, vp]
         L.enzoe_pressure_from_integration.argtypes = [PM, PA, i, d, i, vp]
         L.enzoe_integrator_timestep.argtypes = [PM, PA, i, d, d, d, d, vp, _dp]
+        PI = ctypes.POINTER(i)
+        L.enzoe_flux_data_size.argtypes = [i, i, i, i]
+        L.enzoe_flux_data_size.restype = ctypes.c_longlong
+        L.enzoe_vlct_compute_store_fluxes.argtypes = [vp, ctypes.POINTER(Block), i, d, i,
+                                                      PI, ctypes.POINTER(vp), vp]
+        L.enzoe_save_fluxes_for_corrections.argtypes = [PM, i, d, d, i, i, PI, vp, vp]
+        L.enzoe_flux_data_send_to_coarse.argtypes = [vp, vp, i, i, i, i, i, i, PI, vp]
+        L.enzoe_flux_correct.argtypes = [ctypes.POINTER(vp), i, i, PI, vp, i, i, i, i,
+                                         PI, vp]
         _ops_bound = True
     return L
+
+
+# ----------------------------------------------------- flux corrections
+
+# The "conserved" group of the reference's flux-correction decks
+# (input/FluxCorrect/inclined_contact.incl) and its "make_field_conservative"
+# members, as slots.
+CONSERVED_FIELDS = ("density", "total_energy", "velocity_x", "velocity_y", "velocity_z")
+SLOT = {"density": 0, "velocity_x": 1, "velocity_y": 2, "velocity_z": 3,
+        "total_energy": 4}
+MAKE_CONSERVATIVE = ("velocity_x", "velocity_y", "velocity_z", "total_energy")
+
+
+def _ints(v):
+    return (ctypes.c_int * len(v))(*v)
+
+
+class FluxData:
+    """One block's FluxData (single device array, data_FluxData.cpp:14-110)
+    for the fields `names` and nx*ny*nz active cells."""
+
+    def __init__(self, n, names=CONSERVED_FIELDS):
+        self.n = tuple(n)
+        self.names = tuple(names)
+        self.size = ops().enzoe_flux_data_size(len(names), *self.n)
+        if self.size < 0:
+            raise EnzoeError("invalid FluxData size")
+        self.buf = DeviceBuffer(self.size * 8)
+        hip_check(hip().hipMemset(self.buf.ptr, 0, self.size * 8), "hipMemset")
+
+    @property
+    def ptr(self):
+        return self.buf.ptr.value
+
+    def download(self):
+        return from_device(self.ptr, (self.size,))
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.size == self.size
+        to_device(self.ptr, a)
+
+    def slots(self):
+        return _ints([SLOT[k] for k in self.names])
+
+
+def compute_store_fluxes(method, blocks, dt, flux_data, stream=None):
+    """enzoe_vlct_compute_store_fluxes over `blocks` with one FluxData each."""
+    L = ops()
+    names = flux_data[0].names
+    ptrs = (ctypes.c_void_p * len(blocks))(*[f.ptr for f in flux_data])
+    check(L.enzoe_vlct_compute_store_fluxes(method.handle, method._blocks(blocks),
+                                            len(blocks), dt, len(names),
+                                            flux_data[0].slots(), ptrs, stream),
+          "enzoe_vlct_compute_store_fluxes")
+
+
+def flux_send_to_coarse(fine, coarse, axis, face, child, stream=None):
+    """Fine block's FluxData on (axis, face) into the coarse neighbour's."""
+    check(ops().enzoe_flux_data_send_to_coarse(fine.ptr, coarse.ptr, len(fine.names),
+                                               *fine.n, axis, face, _ints(child), stream),
+          "enzoe_flux_data_send_to_coarse")
+
+
+def flux_correct(block, flux_data, correct, stream=None,
+                 conservative=MAKE_CONSERVATIVE):
+    """MethodFluxCorrect::flux_correct_ on a DeviceBlock; correct[2*axis+face]."""
+    names = flux_data.names
+    fields = (ctypes.c_void_p * len(names))(*[block.ptr(k) for k in names])
+    i_den = names.index("density") if "density" in names else -1
+    cons = _ints([1 if k in conservative else 0 for k in names])
+    check(ops().enzoe_flux_correct(fields, len(names), i_den, cons, flux_data.ptr,
+                                   *flux_data.n, block.ghost, _ints(correct), stream),
+          "enzoe_flux_correct")
 
 
 # ------------------------------------------------------ C++ drop-in layer
@@ -543,8 +626,46 @@ def host_lib():
         H.enzoe_host_method_compute_passive.restype = ctypes.c_int
         H.enzoe_host_method_timestep.argtypes = [vp, ctypes.POINTER(Block), vp]
         H.enzoe_host_method_timestep.restype = ctypes.c_double
+        PI = ctypes.POINTER(ctypes.c_int)
+        H.enzoe_host_amr_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                            ctypes.POINTER(Block), PI, ctypes.c_int]
+        H.enzoe_host_amr_create.restype = vp
+        H.enzoe_host_amr_set_face.argtypes = [vp] + [ctypes.c_int] * 5 + [PI]
+        H.enzoe_host_amr_cycle.argtypes = [vp, ctypes.c_double, vp, _dp, ctypes.c_int]
+        H.enzoe_host_amr_destroy.argtypes = [vp]
         _host = H
     return _host
+
+
+class HostAmr:
+    """The C++ Method list ["mhd_vlct", "flux_correct"] over a static set
+    of hydro DeviceBlocks of given refinement levels."""
+
+    def __init__(self, vlct_parameters, flux_correct_parameters, blocks, levels):
+        self.blocks = blocks  # keeps the device memory alive
+        arr = (Block * len(blocks))(*[b.desc for b in blocks])
+        self.handle = host_lib().enzoe_host_amr_create(
+            vlct_parameters.encode(), flux_correct_parameters.encode(), arr,
+            _ints(levels), len(blocks))
+        if not self.handle:
+            raise EnzoeError("enzoe_host_amr_create failed")
+
+    def set_face(self, block, axis, face, neighbor_level, coarse_neighbor=-1,
+                 child=(0, 0, 0)):
+        check(host_lib().enzoe_host_amr_set_face(self.handle, block, axis, face,
+                                                 neighbor_level, coarse_neighbor,
+                                                 _ints(child)), "enzoe_host_amr_set_face")
+
+    def cycle(self, dt, stream=None):
+        """One cycle; returns (conserved digits per field, min_digits met)."""
+        d = (ctypes.c_double * 8)()
+        rc = host_lib().enzoe_host_amr_cycle(self.handle, dt, stream, d, 8)
+        return list(d)[:len(CONSERVED_FIELDS)], rc == 0
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _host is not None:
+            _host.enzoe_host_amr_destroy(self.handle)
+            self.handle = None
 
 
 class HostMethod:

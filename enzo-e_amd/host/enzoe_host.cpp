@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -445,6 +446,36 @@ EFlt3DArray Block::field_view(const std::string &name) const {
   return it->second;
 }
 
+std::array<int, 3> Block::active_size() const {
+  const EFlt3DArray &rho = field_view("density");
+  return {rho.shape(2) - 2 * ghost_, rho.shape(1) - 2 * ghost_, rho.shape(0) - 2 * ghost_};
+}
+
+int Block::face_level(int axis, int face) const {
+  const int l = face_level_[2 * axis + face];
+  return l < 0 ? level_ : l;
+}
+
+void Block::set_face_level(int axis, int face, int level) {
+  ENZOE_ASSERT("Block::set_face_level", "axis in [0,3) and face in [0,2)",
+               axis >= 0 && axis < 3 && (face == 0 || face == 1));
+  face_level_[2 * axis + face] = level;
+}
+
+void refresh_flux_face_to_coarse(Block *fine, Block *coarse, int axis, int face,
+                                 const std::array<int, 3> &child) {
+  FluxData *ff = fine->flux_data(), *fc = coarse->flux_data();
+  ENZOE_ASSERT("Block::refresh_load_flux_face_", "both blocks hold FluxData",
+               ff->allocated() && fc->allocated());
+  ENZOE_ASSERT("Block::refresh_load_flux_face_", "FluxData of the same fields and size",
+               ff->field_list() == fc->field_list() && ff->size() == fc->size());
+  const std::array<int, 3> n = ff->size();
+  check_rc(enzoe_flux_data_send_to_coarse(ff->flux_array(), fc->flux_array(),
+                                          ff->num_fields(), n[0], n[1], n[2], axis, face,
+                                          child.data(), g_stream),
+           "Block::refresh_load_flux_face_");
+}
+
 enzoe_block Block::desc() const {
   enzoe_block b;
   std::memset(&b, 0, sizeof(b));
@@ -467,6 +498,81 @@ enzoe_block Block::desc() const {
     b.bfieldi[a] = ptr(kBiNames[a]);
   }
   return b;
+}
+
+// ---------------------------------------------- field groups, FluxData
+
+Grouping::Grouping() {
+  for (const char *f : {"density", "total_energy", "velocity_x", "velocity_y",
+                        "velocity_z"})
+    add(f, "conserved");
+  for (const char *f : {"velocity_x", "velocity_y", "velocity_z", "total_energy"})
+    add(f, "make_field_conservative");
+}
+
+void Grouping::add(const std::string &field, const std::string &group) {
+  str_vec_t &g = groups_[group];
+  if (std::find(g.begin(), g.end(), field) == g.end()) g.push_back(field);
+}
+
+void Grouping::clear(const std::string &group) { groups_.erase(group); }
+
+bool Grouping::is_in(const std::string &field, const std::string &group) const {
+  auto it = groups_.find(group);
+  return it != groups_.end() &&
+         std::find(it->second.begin(), it->second.end(), field) != it->second.end();
+}
+
+str_vec_t Grouping::group_list(const std::string &group) const {
+  auto it = groups_.find(group);
+  return it == groups_.end() ? str_vec_t() : it->second;
+}
+
+namespace cello {
+Grouping *field_groups() {
+  static Grouping groups;
+  return &groups;
+}
+}  // namespace cello
+
+void FluxData::allocate(int nx, int ny, int nz, const str_vec_t &field_list) {
+  const long long n = enzoe_flux_data_size((int)field_list.size(), nx, ny, nz);
+  ENZOE_ASSERT("FluxData::allocate", "invalid block size", n >= 0);
+  // FaceFluxes::FaceFluxes (data_FaceFluxes.cpp:38-45)
+  ENZOE_ASSERT("FluxData::allocate", "Block dimensions must be even",
+               !(nx & 1) && !(ny & 1) && !(nz & 1));
+  const std::array<int, 3> sz = {nx, ny, nz};
+  allocated_ = true;
+  if (storage_ && size_ == n && n_ == sz && fields_ == field_list) {
+    // the buffer of an earlier cycle again, zeroed (the reference allocates
+    // it anew every cycle)
+    if (hipMemsetAsync(storage_.get(), 0, (std::size_t)n * sizeof(double),
+                       (hipStream_t)g_stream) != hipSuccess)
+      ENZOE_ERROR("FluxData::allocate", "hipMemsetAsync failed");
+    return;
+  }
+  storage_ = device_alloc((std::size_t)n * sizeof(double), "FluxData::allocate");
+  fields_ = field_list;
+  n_ = sz;
+  size_ = n;
+}
+
+void FluxData::deallocate() { allocated_ = false; }
+
+long long FluxData::offset_(int axis, int face, int i_f, int nbr) const {
+  const long long m[3] = {(long long)n_[1] * n_[2], (long long)n_[0] * n_[2],
+                          (long long)n_[0] * n_[1]};
+  long long o = (long long)i_f * 4 * (m[0] + m[1] + m[2]);
+  for (int a = 0; a < axis; a++) o += 4 * m[a];
+  return o + (long long)face * 2 * m[axis] + (nbr ? m[axis] : 0);
+}
+
+double *FluxData::block_fluxes(int axis, int face, int i_f) const {
+  return flux_array() + offset_(axis, face, i_f, 0);
+}
+
+double *FluxData::neighbor_fluxes(int axis, int face, int i_f) const {
+  return flux_array() + offset_(axis, face, i_f, 1);
 }
 
 // --------------------------------------------------------- reconstruction
@@ -1048,12 +1154,12 @@ EnzoMethodMHDVlct::EnzoMethodMHDVlct(ParameterGroup p, bool store_fluxes_for_cor
   primitive_field_list_ = integrator_->primitive_quantity_keys();
   bfield_method_ = integrator_->construct_bfield_method(nstages);
   if (bfield_method_ != nullptr) bfield_method_->check_required_fields();
+  // EnzoMethodMHDVlct.cpp:136-141
+  store_fluxes_for_corrections_ = store_fluxes_for_corrections;
   if (store_fluxes_for_corrections) {
     ENZOE_ASSERT("EnzoMethodMHDVlct::EnzoMethodMHDVlct",
                  "Flux corrections are currently only supported in hydro-mode",
                  integrator_->is_pure_hydro());
-    ENZOE_ERROR("EnzoMethodMHDVlct::EnzoMethodMHDVlct",
-                "saving fluxes for flux corrections is not provided by the device library");
   }
 
   // device extensions
@@ -1137,18 +1243,48 @@ void EnzoMethodMHDVlct::compute(Block *block) throw() {
     ENZOE_ASSERT("EnzoMethodMHDVlct::compute",
                  "ghost depth must be at least " + std::to_string(min_gdepth_req) + ".",
                  block->ghost_depth() >= min_gdepth_req);
+    if (store_fluxes_for_corrections_) allocate_FC_flux_buffer_(block);
     if (fused_) {
       ENZOE_ASSERT("EnzoMethodMHDVlct::compute",
                    "passively advected scalars need device_fused = false",
                    block->passive_scalars().empty());
       const enzoe_block b = block->desc();
-      check_rc(enzoe_vlct_compute(device_, &b, 1, block->dt(), g_stream),
-               "EnzoMethodMHDVlct::compute");
+      if (store_fluxes_for_corrections_) {
+        const std::vector<int> slots = flux_slots_(*block->flux_data());
+        double *fd = block->flux_data()->flux_array();
+        check_rc(enzoe_vlct_compute_store_fluxes(device_, &b, 1, block->dt(),
+                                                 (int)slots.size(), slots.data(), &fd,
+                                                 g_stream),
+                 "EnzoMethodMHDVlct::compute");
+      } else {
+        check_rc(enzoe_vlct_compute(device_, &b, 1, block->dt(), g_stream),
+                 "EnzoMethodMHDVlct::compute");
+      }
     } else {
       compute_staged_(block);
     }
   }
   block->compute_done();
+}
+
+void EnzoMethodMHDVlct::allocate_FC_flux_buffer_(Block *block) const {
+  // EnzoMethodMHDVlct.cpp:333-355: the "conserved" group, every cycle
+  const std::array<int, 3> n = block->active_size();
+  block->flux_data()->allocate(n[0], n[1], n[2],
+                               cello::field_groups()->group_list("conserved"));
+}
+
+std::vector<int> EnzoMethodMHDVlct::flux_slots_(const FluxData &fd) const {
+  std::vector<int> slots;
+  for (const std::string &name : fd.field_list()) {
+    const int slot = key_slot(name);
+    ENZOE_ASSERT("EnzoMethodMHDVlct::save_fluxes_for_corrections_",
+                 "the fused step saves fluxes of density, velocity and total_energy only ("
+                 "\"" + name + "\" is in the \"conserved\" group)",
+                 slot >= ENZOE_SLOT_DENSITY && slot <= ENZOE_SLOT_ENERGY);
+    slots.push_back(slot);
+  }
+  return slots;
 }
 
 void EnzoMethodMHDVlct::compute_blocks(const std::vector<Block *> &blocks) throw() {
@@ -1158,6 +1294,7 @@ void EnzoMethodMHDVlct::compute_blocks(const std::vector<Block *> &blocks) throw
     return;
   }
   std::vector<enzoe_block> desc;
+  std::vector<double *> fds;
   desc.reserve(blocks.size());
   for (Block *b : blocks) {
     ENZOE_ASSERT("EnzoMethodMHDVlct::compute_blocks",
@@ -1166,10 +1303,22 @@ void EnzoMethodMHDVlct::compute_blocks(const std::vector<Block *> &blocks) throw
                  "passively advected scalars need device_fused = false",
                  b->passive_scalars().empty());
     desc.push_back(b->desc());
+    if (store_fluxes_for_corrections_) {
+      allocate_FC_flux_buffer_(b);
+      fds.push_back(b->flux_data()->flux_array());
+    }
   }
-  check_rc(enzoe_vlct_compute(device_, desc.data(), (int)desc.size(), blocks[0]->dt(),
-                              g_stream),
-           "EnzoMethodMHDVlct::compute_blocks");
+  if (store_fluxes_for_corrections_) {
+    const std::vector<int> slots = flux_slots_(*blocks[0]->flux_data());
+    check_rc(enzoe_vlct_compute_store_fluxes(device_, desc.data(), (int)desc.size(),
+                                             blocks[0]->dt(), (int)slots.size(),
+                                             slots.data(), fds.data(), g_stream),
+             "EnzoMethodMHDVlct::compute_blocks");
+  } else {
+    check_rc(enzoe_vlct_compute(device_, desc.data(), (int)desc.size(), blocks[0]->dt(),
+                                g_stream),
+             "EnzoMethodMHDVlct::compute_blocks");
+  }
   for (Block *b : blocks) b->compute_done();
 }
 
@@ -1214,6 +1363,28 @@ void EnzoMethodMHDVlct::compute_staged_(Block *block) noexcept {
                                       bfield_method_, stage_index, cur_dt, stale_depth,
                                       cell_widths_xyz);
     stale_depth += integrator_->staling_from_stage((int)stage_index);
+    if (is_final_stage && store_fluxes_for_corrections_) {
+      // EnzoMethodMHDVlct.cpp:480-489
+      FluxData *fd = block->flux_data();
+      std::vector<int> slots;
+      for (const std::string &name : fd->field_list()) {
+        int slot = key_slot(name);
+        auto it = std::find(passive_list.begin(), passive_list.end(), name);
+        if (slot < 0 && it != passive_list.end())
+          slot = ENZOE_SLOT_PASSIVE0 + (int)(it - passive_list.begin());
+        ENZOE_ASSERT("EnzoMethodMHDVlct::save_fluxes_for_corrections_",
+                     "\"" + name + "\" of the \"conserved\" group has no flux", slot >= 0);
+        slots.push_back(slot);
+      }
+      for (int i = 0; i < 3; i++) {
+        const enzoe_fmap f = flux_maps_xyz[i].slot_map(passive_list);
+        check_rc(enzoe_save_fluxes_for_corrections(&f, i, cell_widths_xyz[i], cur_dt,
+                                                   block->ghost_depth(), (int)slots.size(),
+                                                   slots.data(), fd->flux_array(),
+                                                   g_stream),
+                 "EnzoMethodMHDVlct::save_fluxes_for_corrections_");
+      }
+    }
     if (bfield_method_ != nullptr) bfield_method_->increment_partial_timestep();
   }
 }
@@ -1233,4 +1404,169 @@ double EnzoMethodMHDVlct::timestep(Block *block) throw() {
   const double *w = block->CellWidth();
   double dtBaryons = integrator_->timestep(integration_map, pressure, w[0], w[1], w[2]);
   return dtBaryons * courant_;
+}
+
+// ------------------------------------------------------- flux correction
+
+namespace {
+
+// parse_min_digit_map_ (problem_MethodFluxCorrect.cpp:15-52): a float is the
+// density's; a list "[field, digits, ...]" pairs fields with digits
+std::map<std::string, double> parse_min_digit_map(const ParameterGroup &p) {
+  const std::string *v = p.param("min_digits");
+  std::map<std::string, double> out;
+  if (v == nullptr) return out;
+  std::string t = *v;
+  if (t.find('[') == std::string::npos) {
+    out["density"] = p.value_float("min_digits", 0.0);
+    return out;
+  }
+  for (char &c : t)
+    if (c == '[' || c == ']' || c == ',' || c == '"') c = ' ';
+  std::istringstream is(t);
+  std::vector<std::string> items;
+  std::string w;
+  while (is >> w) items.push_back(w);
+  ENZOE_ASSERT("parse_min_digit_map_",
+               "The list assigned to " + p.full_name("min_digits") +
+                   " must have a non-negative, even length",
+               items.size() % 2 == 0);
+  for (std::size_t i = 0; i < items.size(); i += 2) {
+    ENZOE_ASSERT("parse_min_digit_map_",
+                 "The \"" + items[i] + "\" field is listed more than once in " +
+                     p.full_name("min_digits"),
+                 out.find(items[i]) == out.end());
+    out[items[i]] = std::atof(items[i + 1].c_str());
+  }
+  return out;
+}
+
+bool parse_logical(const ParameterGroup &p, const char *name, bool dflt) {
+  const std::string v = lowered(p.value_string(name, dflt ? "true" : "false"));
+  ENZOE_ASSERT("MethodFluxCorrect", p.full_name(name) + " must be true or false",
+               v == "true" || v == "false");
+  return v == "true";
+}
+
+}  // namespace
+
+MethodFluxCorrect::MethodFluxCorrect(ParameterGroup p)
+    : MethodFluxCorrect(p.value_string("group", "conserved"),
+                        parse_logical(p, "enable", true), parse_min_digit_map(p)) {}
+
+MethodFluxCorrect::MethodFluxCorrect(std::string group, bool enable,
+                                     std::map<std::string, double> min_digits_map)
+    : group_(std::move(group)), enable_(enable), min_digits_map_(std::move(min_digits_map)) {
+  // problem_MethodFluxCorrect.cpp:67-112
+  Grouping *groups = cello::field_groups();
+  fields_ = groups->group_list(group_);
+  ENZOE_ASSERT("MethodFluxCorrect::MethodFluxCorrect",
+               "Must add field(s) to the field group \"" + group_ + "\"", !fields_.empty());
+  for (const auto &kv : min_digits_map_) {
+    ENZOE_ASSERT("MethodFluxCorrect::MethodFluxCorrect",
+                 "Can't check conserved digits for the \"" + kv.first +
+                     "\" field since it's not in the field group \"" + group_ + "\".",
+                 groups->is_in(kv.first, group_));
+  }
+  field_sum_.assign(fields_.size(), 0.0L);
+  field_sum_0_.assign(fields_.size(), 0.0L);
+}
+
+void MethodFluxCorrect::flux_correct_(Block *block) {
+  FluxData *fd = block->flux_data();
+  const int nf = fd->num_fields();
+  if (!(enable_ && block->is_leaf()) || nf == 0) return;
+  ENZOE_ASSERT("MethodFluxCorrect::flux_correct_", "the block holds no FluxData",
+               fd->allocated());
+  // faces whose neighbour is finer (:439-451)
+  int correct[6];
+  for (int axis = 0; axis < 3; axis++)
+    for (int face = 0; face < 2; face++)
+      correct[2 * axis + face] = block->face_level(axis, face) > block->level() ? 1 : 0;
+  Grouping *groups = cello::field_groups();
+  std::vector<double *> fields;
+  std::vector<int> conservative;
+  int i_density = -1;
+  for (int i_f = 0; i_f < nf; i_f++) {
+    const std::string &name = fd->field_name(i_f);
+    if (name == "density") i_density = i_f;
+    fields.push_back(block->field_view(name).data());
+    conservative.push_back(groups->is_in(name, "make_field_conservative") ? 1 : 0);
+  }
+  for (int i_f = 0; i_f < nf; i_f++)
+    ENZOE_ASSERT("MethodFluxCorrect::flux_correct_",
+                 "The \"density\" field must exist to perform flux corrections on \"" +
+                     fd->field_name(i_f) + "\".",
+                 !(conservative[i_f] && i_f != i_density && i_density < 0));
+  const std::array<int, 3> n = fd->size();
+  check_rc(enzoe_flux_correct(fields.data(), nf, i_density, conservative.data(),
+                              fd->flux_array(), n[0], n[1], n[2], block->ghost_depth(),
+                              correct, g_stream),
+           "MethodFluxCorrect::flux_correct_");
+}
+
+void MethodFluxCorrect::local_sums_(Block *block, std::vector<long double> &reduce) const {
+  // compute_continue_refresh (:134-208): active-region sums, times density
+  // for "make_field_conservative" fields, divided by 2^(level * rank)
+  const EFlt3DArray &rho_v = block->field_view("density");
+  const int mz = rho_v.shape(0), my = rho_v.shape(1), mx = rho_v.shape(2);
+  const int g = block->ghost_depth();
+  const std::size_t cells = (std::size_t)mx * my * mz;
+  std::vector<double> rho(cells), values(cells);
+  if (hipStreamSynchronize((hipStream_t)g_stream) != hipSuccess ||
+      hipMemcpy(rho.data(), rho_v.data(), cells * sizeof(double), hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    ENZOE_ERROR("MethodFluxCorrect::compute_continue_refresh", "hipMemcpy failed");
+  Grouping *groups = cello::field_groups();
+  const long double w = (long double)(1 << (block->level() * 3));
+  for (std::size_t i_f = 0; i_f < fields_.size(); i_f++) {
+    const EFlt3DArray a = block->field_view(fields_[i_f]);
+    if (hipMemcpy(values.data(), a.data(), cells * sizeof(double), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      ENZOE_ERROR("MethodFluxCorrect::compute_continue_refresh", "hipMemcpy failed");
+    const bool scale = groups->is_in(fields_[i_f], "make_field_conservative");
+    long double sum = 0.0L;
+    for (int iz = g; iz < mz - g; iz++)
+      for (int iy = g; iy < my - g; iy++)
+        for (int ix = g; ix < mx - g; ix++) {
+          const std::size_t i = ix + (std::size_t)mx * (iy + (std::size_t)my * iz);
+          if (scale)
+            sum += values[i] * rho[i];
+          else
+            sum += values[i];
+        }
+    reduce[i_f] += sum / w;
+  }
+}
+
+void MethodFluxCorrect::compute_blocks(const std::vector<Block *> &blocks) throw() {
+  std::vector<long double> reduce(fields_.size(), 0.0L);
+  for (Block *b : blocks) {
+    flux_correct_(b);
+    if (b->is_leaf()) local_sums_(b, reduce);
+  }
+  // compute_continue_sum_fields (:212-262)
+  field_sum_ = reduce;
+  if (cycle_ == 0) field_sum_0_ = field_sum_;
+  min_digits_ok_ = true;
+  for (std::size_t i_f = 0; i_f < fields_.size(); i_f++) {
+    const long double a = field_sum_0_[i_f], b = field_sum_[i_f];
+    const long double err = (a != 0.0L) ? std::fabs((a - b) / a) : std::fabs(a - b);
+    const double digits = -std::log10((double)err);
+    digits_[fields_[i_f]] = digits;
+    auto it = min_digits_map_.find(fields_[i_f]);
+    if (it != min_digits_map_.end() && !(digits >= it->second)) min_digits_ok_ = false;
+  }
+  cycle_++;
+  for (Block *b : blocks) {
+    b->flux_data()->deallocate();
+    b->compute_done();
+  }
+}
+
+void MethodFluxCorrect::compute(Block *block) throw() { compute_blocks({block}); }
+
+double MethodFluxCorrect::conserved_digits(const std::string &field) const {
+  auto it = digits_.find(field);
+  return it == digits_.end() ? 0.0 : it->second;
 }

@@ -195,8 +195,62 @@ class ParameterGroup {
   std::map<std::string, std::string> values_;
 };
 
+// Cello's field groups (cello::field_groups(), src/Cello/data_Grouping.hpp):
+// group name -> ordered field list.  It starts with the groups of the
+// reference's flux-correction decks (input/FluxCorrect/
+// inclined_contact.incl): "conserved" = density, total_energy, velocity_x,
+// velocity_y, velocity_z; "make_field_conservative" = velocity_x,
+// velocity_y, velocity_z, total_energy.
+class Grouping {
+ public:
+  Grouping();
+  void add(const std::string &field, const std::string &group);
+  void clear(const std::string &group);
+  bool is_in(const std::string &field, const std::string &group) const;
+  str_vec_t group_list(const std::string &group) const;
+  int size(const std::string &group) const { return (int)group_list(group).size(); }
+
+ private:
+  std::map<std::string, str_vec_t> groups_;
+};
+
+namespace cello {
+Grouping *field_groups();
+}
+
+// Cello FluxData (src/Cello/data_FluxData.hpp) with single-array storage
+// (data_FluxData.cpp:14-110) in device memory: for each listed field, axis
+// and face, the block's face fluxes and its neighbours' (the layout of
+// include/enzoe_vlct.h).  allocate() zeroes it, as the reference's
+// std::vector does.
+class FluxData {
+ public:
+  void allocate(int nx, int ny, int nz, const str_vec_t &field_list);
+  void deallocate();
+  bool allocated() const { return allocated_; }
+  int num_fields() const { return (int)fields_.size(); }
+  const std::string &field_name(int i_f) const { return fields_[i_f]; }
+  const str_vec_t &field_list() const { return fields_; }
+  // FaceFluxes::flux_array of block_fluxes / neighbor_fluxes (device)
+  double *block_fluxes(int axis, int face, int i_f) const;
+  double *neighbor_fluxes(int axis, int face, int i_f) const;
+  double *flux_array() const { return static_cast<double *>(storage_.get()); }
+  long long num_values() const { return size_; }
+  const std::array<int, 3> &size() const { return n_; }
+
+ private:
+  long long offset_(int axis, int face, int i_f, int nbr) const;
+  std::shared_ptr<void> storage_;  // kept by deallocate() for the next cycle
+  bool allocated_ = false;
+  str_vec_t fields_;
+  std::array<int, 3> n_{{0, 0, 0}};
+  long long size_ = 0;
+};
+
 // The slice of a Cello Block the method touches: its permanent fields
-// (device), cell widths, dt, and the compute_done continuation.
+// (device), cell widths, dt, the compute_done continuation, its refinement
+// level and its neighbours' (Block::level, Block::face_level) and its
+// FluxData.
 class Block {
  public:
   // device-resident fields, one allocation holding every permanent field
@@ -222,6 +276,14 @@ class Block {
   int compute_done_count() const { return compute_done_count_; }
   // the fields as the library's block descriptor
   enzoe_block desc() const;
+  // active cells along x, y, z (Field::size)
+  std::array<int, 3> active_size() const;
+  int level() const { return level_; }
+  void set_level(int level) { level_ = level; }
+  // level of the neighbour across (axis, face); the block's own by default
+  int face_level(int axis, int face) const;
+  void set_face_level(int axis, int face, int level);
+  FluxData *flux_data() { return &flux_data_; }
 
  private:
   void bind_(const enzoe_block &b);
@@ -232,7 +294,18 @@ class Block {
   int ghost_ = 0;
   double dt_ = 0.;
   int compute_done_count_ = 0;
+  int level_ = 0;
+  std::array<int, 6> face_level_{{-1, -1, -1, -1, -1, -1}};
+  FluxData flux_data_;
 };
+
+// Block::refresh_load_flux_face_ toward a coarser neighbour
+// (src/Cello/control_refresh.cpp:1458-1500) and the message's arrival
+// (data_DataMsg.cpp:383-396): the fine block's fluxes on (axis, face),
+// coarsened into quadrant `child` (its child index in its parent), summed
+// into the coarse block's neighbour fluxes on (axis, 1 - face).
+void refresh_flux_face_to_coarse(Block *fine, Block *coarse, int axis, int face,
+                                 const std::array<int, 3> &child);
 
 // Cello Method: compute(Block*) + timestep(Block*) + courant
 // (src/Cello/problem_Method.hpp:14-127).
@@ -548,6 +621,9 @@ class EnzoMethodMHDVlct : public Method {
   // vl, 1.0 for euler).  Device extensions: "device_fused" (true: one fused
   // library launch sequence per compute; false: the stage-by-stage plugin
   // composition below) and "precision" ("double" or "mixed", fused only).
+  // store_fluxes_for_corrections (hydro only, as the reference asserts):
+  // each compute allocates the block's FluxData for the "conserved" group
+  // and fills it with the final stage's face fluxes.
   EnzoMethodMHDVlct(ParameterGroup p, bool store_fluxes_for_corrections);
   ~EnzoMethodMHDVlct() override;
 
@@ -568,7 +644,11 @@ class EnzoMethodMHDVlct : public Method {
   EnzoVlctScratchSpace *get_scratch_ptr_(const std::array<int, 3> &field_shape,
                                          const str_vec_t &passive_list) noexcept;
   void compute_staged_(Block *block) noexcept;
+  // EnzoMethodMHDVlct.cpp:333-355
+  void allocate_FC_flux_buffer_(Block *block) const;
+  std::vector<int> flux_slots_(const FluxData &fd) const;
 
+  bool store_fluxes_for_corrections_ = false;
   std::string time_scheme_;
   EnzoMHDIntegratorStageArgPack *integrator_arg_pack_ = nullptr;
   EnzoMHDIntegratorStageCommands *integrator_ = nullptr;
@@ -578,4 +658,53 @@ class EnzoMethodMHDVlct : public Method {
   bool fused_ = true;
   int precision_ = ENZOE_PREC_DOUBLE;
   enzoe_vlct_t device_ = nullptr;
+};
+
+// ------------------------------------------------------- flux correction
+
+// Cello MethodFluxCorrect (src/Cello/problem_MethodFluxCorrect.cpp): on
+// every leaf block, replaces the fluxes through faces whose neighbour is
+// finer by the neighbour's (coarsened) fluxes, then sums the conserved
+// fields over the blocks and reports how many digits of the first cycle's
+// sums are kept.
+class MethodFluxCorrect : public Method {
+ public:
+  // Parameters: group ("conserved"), enable (true), min_digits (a float
+  // for density, or a list "[field, digits, ...]"), :57-63, 15-52.
+  explicit MethodFluxCorrect(ParameterGroup p);
+  MethodFluxCorrect(std::string group, bool enable,
+                    std::map<std::string, double> min_digits_map);
+
+  // compute() of one block and the sum over it alone
+  void compute(Block *block) throw() override;
+  // every leaf block of a cycle: corrections, the sum reduction over the
+  // blocks (compute_continue_refresh / compute_continue_sum_fields,
+  // :134-262), FluxData deallocated, compute_done on each block
+  void compute_blocks(const std::vector<Block *> &blocks) throw();
+  std::string name() throw() override { return "flux_correct"; }
+  double timestep(Block *) throw() override { return 1.7976931348623157e308; }
+
+  // the last reduction: sums of the group's fields (times density for
+  // "make_field_conservative" ones, scaled by 2^-(3 level)), the first
+  // cycle's, and -log10 of their relative difference
+  const std::vector<long double> &field_sums() const { return field_sum_; }
+  double conserved_digits(const std::string &field) const;
+  // false if a field of min_digits kept fewer digits (the unit test of the
+  // reference, :248-253)
+  bool min_digits_ok() const { return min_digits_ok_; }
+
+  // :425-560 (public for plugin-level tests)
+  void flux_correct_(Block *block);
+
+ private:
+  void local_sums_(Block *block, std::vector<long double> &reduce) const;
+
+  std::string group_;
+  bool enable_ = true;
+  std::map<std::string, double> min_digits_map_;
+  str_vec_t fields_;
+  std::vector<long double> field_sum_, field_sum_0_;
+  std::map<std::string, double> digits_;
+  bool min_digits_ok_ = true;
+  int cycle_ = 0;
 };

@@ -1,6 +1,7 @@
 // enzoe_host_capi.cpp -- C entry points over the C++ drop-in classes.
 #include "../../include/enzoe_vlct_host.h"
 
+#include <array>
 #include <memory>
 #include <vector>
 
@@ -9,6 +10,19 @@
 struct enzoe_host_method_s {
   std::unique_ptr<EnzoMethodMHDVlct> method;
   std::string name;
+};
+
+// a static mesh of hydro blocks under the Method list
+// ["mhd_vlct", "flux_correct"]
+struct enzoe_host_amr_s {
+  std::unique_ptr<EnzoMethodMHDVlct> vlct;
+  std::unique_ptr<MethodFluxCorrect> flux_correct;
+  std::vector<std::unique_ptr<Block>> blocks;
+  struct Link {
+    int block, axis, face, coarse;
+    std::array<int, 3> child;
+  };
+  std::vector<Link> links;  // faces whose neighbour is coarser
 };
 
 extern "C" {
@@ -77,5 +91,63 @@ double enzoe_host_method_timestep(enzoe_host_method_t m, const enzoe_block *bloc
   Block b(*block);
   return m->method->timestep(&b);
 }
+
+enzoe_host_amr_t enzoe_host_amr_create(const char *vlct_parameters,
+                                       const char *flux_correct_parameters,
+                                       const enzoe_block *blocks, const int *level,
+                                       int n) {
+  if (!blocks || !level || n <= 0) return nullptr;
+  enzoe_host_amr_s *a = new enzoe_host_amr_s;
+  // EnzoProblem.cpp:565-566, 655: flux_correct in the list makes the VL+CT
+  // method store its fluxes
+  a->vlct.reset(new EnzoMethodMHDVlct(ParameterGroup(vlct_parameters ? vlct_parameters : ""),
+                                      true));
+  a->flux_correct.reset(new MethodFluxCorrect(
+      ParameterGroup(flux_correct_parameters ? flux_correct_parameters : "",
+                     "Method:flux_correct")));
+  for (int i = 0; i < n; i++) {
+    a->blocks.emplace_back(new Block(blocks[i]));
+    a->blocks.back()->set_level(level[i]);
+  }
+  return a;
+}
+
+int enzoe_host_amr_set_face(enzoe_host_amr_t a, int block, int axis, int face,
+                            int neighbor_level, int coarse_neighbor, const int child[3]) {
+  if (!a || block < 0 || block >= (int)a->blocks.size() || axis < 0 || axis > 2 ||
+      (face != 0 && face != 1))
+    return ENZOE_ERR_ARG;
+  a->blocks[block]->set_face_level(axis, face, neighbor_level);
+  if (coarse_neighbor >= 0) {
+    if (coarse_neighbor >= (int)a->blocks.size() || !child) return ENZOE_ERR_ARG;
+    a->links.push_back({block, axis, face, coarse_neighbor, {child[0], child[1], child[2]}});
+  }
+  return ENZOE_OK;
+}
+
+int enzoe_host_amr_cycle(enzoe_host_amr_t a, double dt, enzoe_stream_t stream,
+                         double *digits, int ndigits) {
+  if (!a) return ENZOE_ERR_ARG;
+  enzoe_host_set_stream(stream);
+  std::vector<Block *> list;
+  for (auto &b : a->blocks) {
+    b->set_dt(dt);
+    list.push_back(b.get());
+  }
+  // Method 1: mhd_vlct on every block (one batched launch)
+  a->vlct->compute_blocks(list);
+  // refresh of the fluxes toward coarser neighbours
+  for (const auto &l : a->links)
+    refresh_flux_face_to_coarse(a->blocks[l.block].get(), a->blocks[l.coarse].get(), l.axis,
+                                l.face, l.child);
+  // Method 2: flux_correct over every block and its sum reduction
+  a->flux_correct->compute_blocks(list);
+  const str_vec_t fields = cello::field_groups()->group_list("conserved");
+  for (int i = 0; i < ndigits && i < (int)fields.size(); i++)
+    digits[i] = a->flux_correct->conserved_digits(fields[i]);
+  return a->flux_correct->min_digits_ok() ? ENZOE_OK : 1;
+}
+
+void enzoe_host_amr_destroy(enzoe_host_amr_t a) { delete a; }
 
 }  // extern "C"

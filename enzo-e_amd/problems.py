@@ -330,3 +330,41 @@ def kelvin_helmholtz_block(ib, nb=(8, 8, 8), n=128, g=3, gamma=1.4, seed=7,
     blk = _empty_block((n, n, n), g, mhd=False)
     _fill_state(blk, rho, [vx, vy, vz], p, None, gamma)
     return blk, (dx, dx, dx)
+
+
+# ------------------------------- two refinement levels (flux corrections)
+
+def smooth_hydro_block(lower, dx, n=16, g=4, gamma=5.0 / 3.0):
+    """Hydro block whose cell centres (ghosts included) sample a smooth
+    state: rho = 1 + 0.3 sin(2 pi x) cos(2 pi y) + 0.1 sin(2 pi z),
+    v = (0.5 + 0.2 sin 2 pi (y + z), 0.1 cos 2 pi x, 0.1 sin 2 pi y),
+    p = 1 + 0.1 cos 2 pi (x + y + z).  Each block is a self-contained
+    Method input (its ghost zones hold the same analytic state)."""
+    m = n + 2 * g
+    c = [lower[a] + (np.arange(m) - g + 0.5) * dx for a in range(3)]
+    z, y, x = np.meshgrid(c[2], c[1], c[0], indexing="ij")
+    tp = 2.0 * np.pi
+    rho = 1.0 + 0.3 * np.sin(tp * x) * np.cos(tp * y) + 0.1 * np.sin(tp * z)
+    vx = 0.5 + 0.2 * np.sin(tp * (y + z))
+    vy = 0.1 * np.cos(tp * x)
+    vz = 0.1 * np.sin(tp * y)
+    p = 1.0 + 0.1 * np.cos(tp * (x + y + z))
+    blk = _empty_block((n, n, n), g, mhd=False)
+    _fill_state(blk, rho, [vx, vy, vz], p, None, gamma)
+    return blk
+
+
+def two_level_composite(n=16, g=4, gamma=5.0 / 3.0):
+    """A coarse block on [0,1]^3 (cell width 1/n) and, across its high-x
+    face, the four fine blocks (cell width 1/(2n)) of x in [1, 1.5] that
+    cover it: children (0, cy, cz) of the coarse-level block [1,2] x [0,1]^2.
+    Returns [(kind, child, lower, dx, arrays)], the coarse block first."""
+    dxc = 1.0 / n
+    out = [("coarse", None, (0.0, 0.0, 0.0), dxc,
+            smooth_hydro_block((0.0, 0.0, 0.0), dxc, n, g, gamma))]
+    for cz in range(2):
+        for cy in range(2):
+            lower = (1.0, 0.5 * cy, 0.5 * cz)
+            out.append(("fine", (0, cy, cz), lower, 0.5 * dxc,
+                        smooth_hydro_block(lower, 0.5 * dxc, n, g, gamma)))
+    return out

@@ -333,6 +333,74 @@ int enzoe_pressure_from_integration(const enzoe_fmap *integration,
 
 /* ---- diagnostics ---------------------------------------------------------- */
 
+
+/* ---- flux corrections at refinement boundaries (hydro) ------------------
+ *
+ * One block's FluxData is a single device array of doubles
+ * (src/Cello/data_FluxData.cpp:14-110, allocate with single_array): for each
+ * field i_f of the list, axis 0..2, face 0..1 (low, high), the block's face
+ * fluxes followed by the neighbour's, each n_j*n_k values; the face array of
+ * axis 0 is (nz, ny) with y fastest, of axis 1 (nz, nx), of axis 2 (ny, nx)
+ * (FaceFluxes::flux_array, data_FaceFluxes.cpp:114-126).  nx, ny, nz are
+ * the active cells of the block (even, as FaceFluxes requires,
+ * data_FaceFluxes.cpp:38-45).  Field lists hold ENZOE_SLOT_* numbers in the
+ * order of the "conserved" group, e.g. density, total_energy, velocity_x,
+ * velocity_y, velocity_z (input/FluxCorrect/inclined_contact.incl).         */
+
+/* Number of doubles of one block's FluxData (-1 for invalid sizes).         */
+long long enzoe_flux_data_size(int nf, int nx, int ny, int nz);
+
+/* EnzoMethodMHDVlct::compute with store_fluxes_for_corrections
+ * (EnzoMethodMHDVlct.cpp:137-141, 333-363, 480-489): the step of
+ * enzoe_vlct_compute, then, from the final stage's fluxes, dt/dx * F through
+ * the six faces of each block's active region into the block parts of
+ * flux_data[b] (a device pointer per block), with the neighbour parts set
+ * to zero (the reference allocates the FluxData zeroed every cycle).
+ * Hydro only: ENZOE_ERR_CONFIG for a constrained_transport method, as the
+ * reference asserts.  fields: nf slots among density, velocity, energy.     */
+int enzoe_vlct_compute_store_fluxes(enzoe_vlct_t m, const enzoe_block *blocks,
+                                    int n, double dt, int nf,
+                                    const int *fields,
+                                    double *const *flux_data,
+                                    enzoe_stream_t stream);
+
+/* EnzoMethodMHDVlct::save_fluxes_for_corrections_ (EnzoMethodMHDVlct.cpp:
+ * 250-330) for one dim, from a flux map of that dim (the cell shape minus
+ * one along dim; entry f is the face between cells f and f+1): writes
+ * dt/cell_width * F at the faces between cells ghost-1 and ghost, and
+ * m-ghost-1 and m-ghost, into the block parts of flux_data.                 */
+int enzoe_save_fluxes_for_corrections(const enzoe_fmap *flux, int dim,
+                                      double cell_width, double dt, int ghost,
+                                      int nf, const int *fields,
+                                      double *flux_data,
+                                      enzoe_stream_t stream);
+
+/* Block::refresh_load_flux_face_ (src/Cello/control_refresh.cpp:1458-1500)
+ * toward a coarser neighbour, with FaceFluxes::coarsen (rank 3, cell-
+ * centred; data_FaceFluxes.cpp:128-250) and FluxData::sum_neighbor_fluxes
+ * (data_FluxData.hpp:136-145): for each of the nf fields, the fine block's
+ * fluxes on (axis, face) are coarsened into quadrant `child` (the fine
+ * block's child index in its parent) and added to the coarse block's
+ * neighbour fluxes on (axis, 1 - face).  Both blocks have nx*ny*nz active
+ * cells.                                                                    */
+int enzoe_flux_data_send_to_coarse(const double *fine_flux_data,
+                                   double *coarse_flux_data, int nf, int nx,
+                                   int ny, int nz, int axis, int face,
+                                   const int child[3], enzoe_stream_t stream);
+
+/* MethodFluxCorrect::flux_correct_ (src/Cello/problem_MethodFluxCorrect.cpp:
+ * 425-560) on one leaf block: fields[i] is the device (mz, my, mx) array,
+ * ghosts included, of field i of the FluxData list; i_density its density
+ * entry (-1 if absent); conservative[i] marks the "make_field_conservative"
+ * fields, corrected as rho * q and divided by the corrected density;
+ * correct[2*axis + face] marks the faces whose neighbour is finer
+ * (face_level > level).  ENZOE_ERR_ARG for a conservative field without
+ * density (the reference asserts).                                          */
+int enzoe_flux_correct(double *const *fields, int nf, int i_density,
+                       const int *conservative, const double *flux_data,
+                       int nx, int ny, int nz, int ghost, const int correct[6],
+                       enzoe_stream_t stream);
+
 /* Per-kernel average duration (ms) of the launches made while profiling was
  * on (enzoe_set_profiling(1)), measured with HIP events recorded on the
  * launch stream around each kernel; the events are read when the report is

@@ -56,6 +56,34 @@ double enzoe_host_method_timestep(enzoe_host_method_t m,
                                   const enzoe_block *block,
                                   enzoe_stream_t stream);
 
+/* ---- static mesh refinement with flux corrections (hydro) ---------------
+ * The Method list ["mhd_vlct", "flux_correct"] over a fixed set of blocks
+ * (EnzoProblem.cpp:565-566: flux_correct in the list makes EnzoMethodMHDVlct
+ * store its fluxes).  `vlct_parameters` as for enzoe_host_method_create,
+ * `flux_correct_parameters` those of "Method:flux_correct" (group, enable,
+ * min_digits; problem_MethodFluxCorrect.cpp:57-63).  level[i] is block i's
+ * refinement level.                                                        */
+typedef struct enzoe_host_amr_s *enzoe_host_amr_t;
+enzoe_host_amr_t enzoe_host_amr_create(const char *vlct_parameters,
+                                       const char *flux_correct_parameters,
+                                       const enzoe_block *blocks,
+                                       const int *level, int n);
+/* The neighbour of `block` across (axis, face) has level neighbor_level;
+ * when it is coarser, coarse_neighbor is its index and child the block's
+ * child index in its parent (the fluxes go there each cycle).              */
+int enzoe_host_amr_set_face(enzoe_host_amr_t a, int block, int axis, int face,
+                            int neighbor_level, int coarse_neighbor,
+                            const int child[3]);
+/* One cycle with the given dt: EnzoMethodMHDVlct::compute on every block,
+ * the flux refresh toward coarser neighbours (control_refresh.cpp:
+ * 1458-1500), MethodFluxCorrect over every block with its sum reduction
+ * (problem_MethodFluxCorrect.cpp:119-262).  digits[i] receives the
+ * conserved digits of field i of the "conserved" group.  Returns 0, or 1
+ * when a field kept fewer digits than its min_digits.                      */
+int enzoe_host_amr_cycle(enzoe_host_amr_t a, double dt, enzoe_stream_t stream,
+                         double *digits, int ndigits);
+void enzoe_host_amr_destroy(enzoe_host_amr_t a);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,8 +111,26 @@ def lib():
         L.o_compute_center_bfield.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.o_flux_data_size.argtypes = [ctypes.c_int] * 4
+        L.o_flux_data_size.restype = ctypes.c_long
+        L.o_flux_data_offset.argtypes = [ctypes.c_int] * 7
+        L.o_flux_data_offset.restype = ctypes.c_long
+        L.o_vlct_compute_fc.argtypes = [ctypes.POINTER(OConfig), ctypes.POINTER(OBlock),
+                                        ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, dp]
+        L.o_vlct_compute_fc.restype = ctypes.c_int
+        L.o_face_fluxes_coarsen.argtypes = [dp, dp] + [ctypes.c_int] * 7
+        L.o_flux_send_to_coarse.argtypes = [dp, dp] + [ctypes.c_int] * 6 + [ip]
+        L.o_flux_correct.argtypes = [ctypes.POINTER(dp), ctypes.c_int, ctypes.c_int, ip,
+                                     dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ip]
+        L.o_flux_correct.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def ints(v):
+    return (ctypes.c_int * len(v))(*v)
 
 
 def dptr(a):

@@ -1179,8 +1179,28 @@ static void compute_update_stage(const o_config *cfg, o_block *b,
   update_quantities(cfg, U0, sc->dU, Uo, mz, my, mx, stale);
 }
 
-/* EnzoMethodMHDVlct::compute (EnzoMethodMHDVlct.cpp:356-500) */
+/* EnzoMethodMHDVlct::compute (EnzoMethodMHDVlct.cpp:356-500); with
+ * flux_data, store_fluxes_for_corrections_ (:137-141, 360-363, 480-489):
+ * the FluxData is allocated zeroed at the start (allocate_FC_flux_buffer_,
+ * :333-355) and the final stage's fluxes are saved on the block faces. */
+static int vlct_compute_impl(const o_config *cfg, o_block *b, double dt,
+                             int ghost, int nf, const int *fields,
+                             double *flux_data);
+
 int o_vlct_compute(const o_config *cfg, o_block *b, double dt) {
+  return vlct_compute_impl(cfg, b, dt, 0, 0, NULL, NULL);
+}
+
+int o_vlct_compute_fc(const o_config *cfg, o_block *b, double dt, int ghost,
+                      int nf, const int *fields, double *flux_data) {
+  /* EnzoMethodMHDVlct.cpp:137-141: hydro only */
+  if (cfg->mhd) return -2;
+  return vlct_compute_impl(cfg, b, dt, ghost, nf, fields, flux_data);
+}
+
+static int vlct_compute_impl(const o_config *cfg, o_block *b, double dt,
+                             int ghost, int nf, const int *fields,
+                             double *flux_data) {
   if (cfg->mhd && cfg->nstages != 2) return -1; /* EnzoBfieldMethod.cpp:23-27 */
   int req = 0;
   for (int st = 0; st < cfg->nstages; st++) req += total_staling(cfg->recon[st]);
@@ -1198,9 +1218,21 @@ int o_vlct_compute(const o_config *cfg, o_block *b, double dt) {
      * (EnzoBfieldMethodCT.cpp:128-137, 228-240). */
     double **bfi_recon = (stage == 0) ? b->bfi : sc->tbfi;
     double **bfi_out = is_final ? b->bfi : sc->tbfi;
+    if (stage == 0 && flux_data) {
+      const int nx = b->mx - 2 * ghost, ny = b->my - 2 * ghost,
+                nz = b->mz - 2 * ghost;
+      memset(flux_data, 0,
+             (size_t)o_flux_data_size(nf, nx, ny, nz) * sizeof(double));
+    }
     compute_update_stage(cfg, b, sc, b->cc, Uc, Uo, bfi_recon, bfi_out, stage,
                          cur_dt, stale);
     stale += total_staling(cfg->recon[stage]);
+    if (is_final && flux_data) {
+      for (int d = 0; d < 3; d++)
+        o_save_fluxes_for_corrections(sc->flux[d], d, b->cw[d], cur_dt, b->mx,
+                                      b->my, b->mz, ghost, nf, fields,
+                                      flux_data);
+    }
   }
   return 0;
 }

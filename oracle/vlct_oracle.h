@@ -66,6 +66,49 @@ double o_vlct_timestep(const o_config *cfg, o_block *b);
  * and one timestep dt.  Returns 0 on success.                              */
 int o_vlct_compute(const o_config *cfg, o_block *b, double dt);
 
+/* Flux corrections at refinement boundaries (fluxcorr_oracle.c).
+ *
+ * One block's FluxData as a single array (src/Cello/data_FluxData.cpp:
+ * 14-110): for each listed field i_f, axis 0..2, face 0..1, the block's face
+ * fluxes then the neighbour's, each n_j*n_k values; the face array of axis 0
+ * is (nz, ny) with y fastest, axis 1 (nz, nx), axis 2 (ny, nx)
+ * (FaceFluxes::flux_array, data_FaceFluxes.cpp:114-126).  fields[] are cc
+ * slots O_RHO..O_ETOT; nx, ny, nz are active cells.                        */
+long o_flux_data_size(int nf, int nx, int ny, int nz);
+long o_flux_data_offset(int i_f, int axis, int face, int neighbor, int nx,
+                        int ny, int nz);
+/* EnzoMethodMHDVlct::compute with store_fluxes_for_corrections (hydro only;
+ * returns -2 for MHD as EnzoMethodMHDVlct.cpp:137-141 asserts).           */
+int o_vlct_compute_fc(const o_config *cfg, o_block *b, double dt, int ghost,
+                      int nf, const int *fields, double *flux_data);
+/* EnzoMethodMHDVlct::save_fluxes_for_corrections_ (EnzoMethodMHDVlct.cpp:
+ * 250-330) for one dim; flux[q] are dense face-trimmed flux arrays of dim
+ * (shape (mz,my,mx) minus one along dim).                                  */
+void o_save_fluxes_for_corrections(double *const *flux, int dim,
+                                   double cell_width, double dt, int mx,
+                                   int my, int mz, int ghost, int nf,
+                                   const int *fields, double *flux_data);
+/* FaceFluxes::coarsen (data_FaceFluxes.cpp:128-250), rank 3, cell-centred:
+ * the face array `fine` of a block (n x n x n active, normal `axis`) into
+ * `out` (same size), the quadrant of child (cx, cy, cz) filled.           */
+void o_face_fluxes_coarsen(const double *fine, double *out, int axis, int nx,
+                           int ny, int nz, int cx, int cy, int cz);
+/* Block::refresh_load_flux_face_ (control_refresh.cpp:1458-1500) with
+ * FluxData::sum_neighbor_fluxes (data_FluxData.hpp:136-145): the fine
+ * block's fluxes on (axis, face) coarsened into the coarse block's
+ * neighbour fluxes on (axis, 1 - face), for every field.                   */
+void o_flux_send_to_coarse(const double *fine_fd, double *coarse_fd, int nf,
+                           int nx, int ny, int nz, int axis, int face,
+                           const int child[3]);
+/* MethodFluxCorrect::flux_correct_ (problem_MethodFluxCorrect.cpp:425-560)
+ * on one leaf block, rank 3: fields[i_f] are (mz,my,mx) arrays with ghost
+ * depth g; correct[2*axis+face] marks faces whose neighbour is finer;
+ * conservative[i_f] marks fields of "make_field_conservative".  Returns -1
+ * when a conservative field is listed without density.                     */
+int o_flux_correct(double *const *fields, int nf, int i_density,
+                   const int *conservative, const double *flux_data, int nx,
+                   int ny, int nz, int g, const int correct[6]);
+
 /* EnzoComputePressure::compute_pressure (EnzoComputePressure.cpp:82-198)
  * over [stale, m-stale) into p.                                            */
 void o_compute_pressure(const o_config *cfg, const o_block *b, double *p,

@@ -208,3 +208,94 @@ def test_fused_entry_points_reject_bad_arguments_without_a_device():
     arr = (ev.Block * 1)(b)
     assert L.enzoe_vlct_compute(h, arr, 1, 0.1, None) == ERR_ARG
     L.enzoe_vlct_destroy(h)
+
+
+def test_flux_correction_entry_points_reject_bad_arguments_without_a_device():
+    """The flux-correction entry points validate before any device call:
+    the MHD method refuses to store fluxes (EnzoMethodMHDVlct.cpp:137-141),
+    FaceFluxes needs even block dimensions (data_FaceFluxes.cpp:38-45), a
+    conservative field needs density (problem_MethodFluxCorrect.cpp:520)."""
+    L = ev.ops()
+    assert L.enzoe_flux_data_size(5, 16, 16, 16) == 5 * 12 * 256
+    assert L.enzoe_flux_data_size(3, 4, 6, 8) == 3 * 4 * (48 + 32 + 24)
+    assert L.enzoe_flux_data_size(1, 0, 4, 4) == -1
+    fake = 1 << 40
+    b = ev.Block()
+    b.mx = b.my = b.mz = 24
+    b.ghost = 4
+    b.density = b.total_energy = fake
+    for a in range(3):
+        b.velocity[a] = fake
+    arr = (ev.Block * 1)(b)
+    slots = (ctypes.c_int * 5)(0, 4, 1, 2, 3)
+    fd = (ctypes.c_void_p * 1)(fake)
+    rc, h = _create()                      # constrained_transport
+    assert L.enzoe_vlct_compute_store_fluxes(h, arr, 1, 0.1, 5, slots, fd, None) == ERR_CONFIG
+    L.enzoe_vlct_destroy(h)
+    rc, h = _create(mhd_choice=b"no_bfield", riemann=b"hllc")
+    assert rc == 0
+    bad = (ctypes.c_int * 1)(5)            # bfield_x has no hydro flux
+    assert L.enzoe_vlct_compute_store_fluxes(h, arr, 1, 0.1, 1, bad, fd, None) == ERR_ARG
+    b.mx = 25                              # 17 active cells along x
+    arr = (ev.Block * 1)(b)
+    assert L.enzoe_vlct_compute_store_fluxes(h, arr, 1, 0.1, 5, slots, fd, None) == ERR_ARG
+    L.enzoe_vlct_destroy(h)
+    child = (ctypes.c_int * 3)(0, 1, 0)
+    assert L.enzoe_flux_data_send_to_coarse(fake, fake, 5, 16, 15, 16, 0, 0, child,
+                                            None) == ERR_ARG
+    assert L.enzoe_flux_data_send_to_coarse(fake, fake, 5, 16, 16, 16, 0, 2, child,
+                                            None) == ERR_ARG
+    child2 = (ctypes.c_int * 3)(0, 2, 0)
+    assert L.enzoe_flux_data_send_to_coarse(fake, fake, 5, 16, 16, 16, 0, 0, child2,
+                                            None) == ERR_ARG
+    fields = (ctypes.c_void_p * 5)(*([fake] * 5))
+    cons = (ctypes.c_int * 5)(0, 1, 1, 1, 1)
+    corr = (ctypes.c_int * 6)(0, 1, 0, 0, 0, 0)
+    assert L.enzoe_flux_correct(fields, 5, -1, cons, fake, 16, 16, 16, 4, corr,
+                                None) == ERR_ARG
+    assert L.enzoe_flux_correct(fields, 5, 0, cons, fake, 16, 16, 15, 4, corr,
+                                None) == ERR_ARG
+
+
+def _run_host_amr(vlct_params, fc_params):
+    """Constructs the C++ Method list ["mhd_vlct", "flux_correct"] (no
+    device call is made before the constructors' checks)."""
+    code = ("import sys; sys.path.insert(0, %r); import enzoe_vlct as ev\n"
+            "ev.host_lib().enzoe_host_set_fluid_props(5.0 / 3.0, 1e-30, 1e-30)\n"
+            "class B: desc = ev.Block()\n"
+            "a = ev.HostAmr(%r, %r, [B()], [0]); print('constructed')"
+            % (os.path.join(ROOT, "enzo-e_amd"), vlct_params, fc_params))
+    return subprocess.run([sys.executable, "-c", code], capture_output=True,
+                          text=True, timeout=120)
+
+
+@pytest.mark.parametrize("vlct,fc,message", [
+    ("mhd_choice = constrained_transport", "",
+     "Flux corrections are currently only supported in hydro-mode"),
+    ("mhd_choice = no_bfield; riemann_solver = hllc", "min_digits = [density, 1, pressure]",
+     "must have a non-negative, even length"),
+    ("mhd_choice = no_bfield; riemann_solver = hllc", "min_digits = [pressure, 12.0]",
+     "since it's not in the field group"),
+    ("mhd_choice = no_bfield; riemann_solver = hllc", "group = colour",
+     "Must add field(s) to the field group"),
+])
+def test_host_flux_correct_errors_exit_like_the_reference(vlct, fc, message):
+    r = _run_host_amr(vlct, fc)
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert "ERROR in" in r.stderr and message in r.stderr, r.stderr
+
+
+def test_host_method_list_with_flux_correct_constructs():
+    r = _run_host_amr("mhd_choice = no_bfield; riemann_solver = hllc",
+                      "min_digits = [density, 16.5, total_energy, 16.2]")
+    assert r.returncode == 0, r.stderr
+    assert "constructed" in r.stdout
+
+
+def test_host_amr_entry_points_reject_bad_arguments():
+    H = ev.host_lib()
+    child = (ctypes.c_int * 3)(0, 1, 1)
+    assert H.enzoe_host_amr_set_face(None, 0, 0, 1, 1, -1, child) == ERR_ARG
+    d = (ctypes.c_double * 5)()
+    assert H.enzoe_host_amr_cycle(None, 0.1, None, d, 5) == ERR_ARG
+    assert H.enzoe_host_amr_create(b"", b"", None, None, 0) is None
