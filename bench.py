@@ -36,7 +36,7 @@ FLUX_BYTES_PER_CELL = {0: (11 + 21) * 8.0, 1: 11 * 8.0 + 21 * 4.0}
 # hydro: read 5 words, write 5 fluxes on 3 faces; step: 25 words per zone
 FLUX_BYTES_PER_CELL_HD = {0: (5 + 15) * 8.0, 1: 5 * 8.0 + 15 * 4.0}
 STEP_BYTES_PER_ZONE_HD = 200.0
-WORKLOADS = ["orszag_tang", "fast_wave", "batched_amr", "turbulence",
+WORKLOADS = ["orszag_tang", "fast_wave", "batched_amr", "turbulence", "smr_flux_correct",
              "kelvin_helmholtz"]
 
 
@@ -58,7 +58,9 @@ def parse():
                          "32^3 fast eigenmode for one period; batched_amr: "
                          "512/1536/3000 blocks of 32^3 on 3 levels with 2:1 "
                          "sub-cycling; turbulence: 512 blocks of 64^3, Mach 5; "
-                         "kelvin_helmholtz: 512 blocks of 128^3, hydro")
+                         "kelvin_helmholtz: 512 blocks of 128^3, hydro; "
+                         "smr_flux_correct: 64 x (1 coarse + 4 fine) 32^3 hydro "
+                         "blocks with flux corrections")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (HBM counter scale)")
     return ap.parse_args()
@@ -400,6 +402,76 @@ def run_uniform(a):
     return out
 
 
+def run_smr_flux_correct(a):
+    """Static mesh refinement with flux corrections (hydro, the reference's
+    VL+CT + flux_correct pairing): 64 units of one coarse 32^3 block and the
+    four fine 32^3 blocks across its high-x face (320 blocks, ghost depth 4).
+    A step: dt, the VL+CT step storing the final-stage face fluxes (one
+    batched launch sequence), the 256 fine-to-coarse flux sends (one launch)
+    and the 64 coarse-block corrections (one launch).  The same step without
+    the flux corrections is timed beside it."""
+    n, g, units = 32, 4, 64
+    comp = problems.two_level_composite(n, g)
+    res = {}
+    for store in (True, False):
+        prec = ev.ENZOE_PREC_DOUBLE if a.precision == "fp64" else ev.ENZOE_PREC_MIXED
+        method = ev.VlctMethod(mhd=False, riemann="hllc", reconstruct="plm", theta=2.0,
+                               courant=0.4, gamma=5.0 / 3.0, density_floor=1e-30,
+                               pressure_floor=1e-30, precision=prec)
+        blocks, fds = [], []
+        for _ in range(units):
+            for (_, _, _, dx, arr) in comp:
+                m = n + 2 * g
+                db = ev.DeviceBlock(m, m, m, g, (dx, dx, dx), mhd=False)
+                db.upload({k: v for k, v in arr.items() if k in db.shapes})
+                blocks.append(db)
+                if store:
+                    fds.append(ev.FluxData((n, n, n)))
+        links, entries = [], []
+        if store:
+            for u in range(units):
+                for j, (_, child, _, _, _) in enumerate(comp[1:], start=1):
+                    links.append((fds[5 * u + j], fds[5 * u], 0, 0, child))
+                entries.append((blocks[5 * u], fds[5 * u], [0, 1, 0, 0, 0, 0]))
+
+        def one_step():
+            dt = method.timestep(blocks)
+            if store:
+                ev.compute_store_fluxes(method, blocks, dt, fds)
+                ev.flux_send_to_coarse_batch(method, links)
+                ev.flux_correct_blocks(method, entries)
+            else:
+                method.compute(blocks, dt)
+        el, prof = _profiled(one_step, a.steps, a.warmup)
+        zones = n ** 3 * len(blocks)
+        res[store] = {"value": zones * a.steps / el, "ms_per_step": el * 1e3 / a.steps,
+                      "kernels_ms": {k: round(v[0], 4) for k, v in sorted(prof.items())},
+                      "roofline": _roofline(prof, a.steps, (n + 2 * g - 4) ** 3 * len(blocks),
+                                            prec, False,
+                                            "smr_flux_correct_%s" % a.precision)}
+        del blocks, fds, method
+    r, plain = res[True], res[False]
+    fc_ms = sum(r["kernels_ms"].get(k, 0.0) for k in
+                ("k_store_fluxes", "k_send_to_coarse", "k_flux_correct"))
+    return {
+        "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32-riemann/f64-update",
+        "data": "synthetic (smooth hydro state sampled on a coarse and four fine blocks)",
+        "config": {"workload": "smr_flux_correct_64x(1+4)x32_%s" % a.precision,
+                   "blocks": 5 * units, "block": [n, n, n], "ghost": g, "riemann": "hllc",
+                   "reconstruct": "plm", "theta": 2.0, "courant": 0.4,
+                   "flux_links": 4 * units, "corrected_blocks": units},
+        "roofline": r["roofline"], "kernels_ms": r["kernels_ms"],
+        "flux_correction_kernels_ms_per_step": fc_ms,
+        "without_flux_correction": {"value": plain["value"],
+                                    "ms_per_step": plain["ms_per_step"],
+                                    "kernels_ms": plain["kernels_ms"]},
+        "flux_correction_overhead": plain["value"] / r["value"] - 1.0,
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -414,7 +486,8 @@ def main():
         return
     if a.workload != "orszag_tang":
         run = {"fast_wave": run_fast_wave, "batched_amr": run_batched_amr,
-               "turbulence": run_uniform, "kelvin_helmholtz": run_uniform}[a.workload]
+               "turbulence": run_uniform, "kelvin_helmholtz": run_uniform,
+               "smr_flux_correct": run_smr_flux_correct}[a.workload]
         out = run(a)
         out["copy_kernel_GBs"] = copy_bandwidth()["GBs"]
         print(json.dumps(out))

@@ -154,7 +154,8 @@ int enzoe_vlct_destroy(enzoe_vlct_t m) {
   if (m->d_ptrtab) (void)hipFree(m->d_ptrtab);
   if (m->d_dts) (void)hipFree(m->d_dts);
   if (m->d_rtab) (void)hipFree(m->d_rtab);
-  if (m->d_fdtab) (void)hipFree(m->d_fdtab);
+  for (void *p : {m->d_fdtab, m->d_fltab, m->d_fbtab})
+    if (p) (void)hipFree(p);
   delete m;
   return ENZOE_OK;
 }
@@ -249,13 +250,9 @@ int enzoe_vlct_compute_store_fluxes(enzoe_vlct_t m, const enzoe_block *blocks,
     if (!flux_data[i]) return ENZOE_ERR_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
-  if (int rc = ensure_bytes(&m->d_fdtab, &m->fdtab_bytes, n * sizeof(double *)))
+  if (int rc = upload_table(&m->d_fdtab, &m->fdtab_bytes, &m->last_fdtab, flux_data,
+                            n * sizeof(double *), s))
     return rc;
-  // the copy completes before the host array may change
-  if (hipMemcpyAsync(m->d_fdtab, flux_data, n * sizeof(double *),
-                     hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return ENZOE_ERR_HIP;
   st.d_fdtab = (double *const *)m->d_fdtab;
   return compute_runs(m, blocks, n, dt, nullptr, s, &st);
 }

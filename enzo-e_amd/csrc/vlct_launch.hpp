@@ -99,8 +99,12 @@ struct enzoe_vlct_method {
   int force_mask = -1;
   void *d_dts = nullptr;  // per-block dt uploaded by enzoe_vlct_compute_dt
   size_t dts_bytes = 0;
-  void *d_fdtab = nullptr;  // FluxData pointers of enzoe_vlct_compute_store_fluxes
-  size_t fdtab_bytes = 0;
+  // device tables of the flux-correction calls (FluxData pointers of
+  // enzoe_vlct_compute_store_fluxes, links of the batched send, blocks of
+  // the batched correction), re-uploaded only when they change
+  void *d_fdtab = nullptr, *d_fltab = nullptr, *d_fbtab = nullptr;
+  size_t fdtab_bytes = 0, fltab_bytes = 0, fbtab_bytes = 0;
+  std::vector<char> last_fdtab, last_fltab, last_fbtab;
 };
 
 namespace enzoe {
@@ -143,6 +147,22 @@ inline int ensure_bytes(void **ptr, size_t *have, size_t need) {
   *have = 0;
   if (hipMalloc(ptr, need) != hipSuccess) return ENZOE_ERR_ALLOC;
   *have = need;
+  return 0;
+}
+
+// Copies a host table to a device buffer unless the buffer already holds
+// exactly these bytes; the stream is synchronised after a copy, so the
+// host table may change as soon as the call returns.
+inline int upload_table(void **dptr, size_t *have, std::vector<char> *last,
+                        const void *host, size_t bytes, hipStream_t s) {
+  const char *raw = (const char *)host;
+  if (*dptr && last->size() == bytes && std::memcmp(last->data(), raw, bytes) == 0)
+    return 0;
+  if (int rc = ensure_bytes(dptr, have, bytes)) return rc;
+  if (hipMemcpyAsync(*dptr, raw, bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ENZOE_ERR_HIP;
+  last->assign(raw, raw + bytes);
   return 0;
 }
 

@@ -20,6 +20,7 @@
 
 #include "../../include/enzoe_vlct.h"
 #include "vlct_kernels.hpp"
+#include "vlct_launch.hpp"
 
 using namespace enzoe;
 
@@ -908,22 +909,25 @@ struct Child {
   int c[3];
 };
 
-__global__ void k_op_send_to_coarse(const double *__restrict__ fine,
-                                    double *__restrict__ coarse, int nf,
-                                    FluxLayout L, int axis, int face,
-                                    Child ch) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long m = L.m[axis];
-  if (t >= m * nf) return;
-  const int i_f = (int)(t / m);
-  const long idx = t - (long)i_f * m;
+// quadrant_only: leave the entries outside the child's quadrant untouched
+// instead of adding the coarsened array's zeros there (the batched launch,
+// where the four siblings of one coarse face run concurrently; the result
+// differs from the sequential sum only in the sign of a zero).
+__device__ __forceinline__ void send_to_coarse(const double *__restrict__ fine,
+                                               double *__restrict__ coarse,
+                                               const FluxLayout &L, int axis,
+                                               int face, const int child[3],
+                                               int i_f, long idx,
+                                               bool quadrant_only) {
   const int a0 = (axis == 0) ? 1 : 0, a1 = (axis == 2) ? 1 : 2;
   const int nu = L.n[a0], nv = L.n[a1];
   const int u = (int)(idx % nu), v = (int)(idx / nu);
   const int hu = nu / 2, hv = nv / 2;
-  const int cu = ch.c[a0], cv = ch.c[a1];
+  const int cu = child[a0], cv = child[a1];
   double acc = 0.0;
-  if (u >= cu * hu && u < cu * hu + hu && v >= cv * hv && v < cv * hv + hv) {
+  const bool inside =
+      u >= cu * hu && u < cu * hu + hu && v >= cv * hv && v < cv * hv + hv;
+  if (inside) {
     const double *src = fine + L.offset(i_f, axis, face, 0);
     const int u2 = 2 * (u - cu * hu), v2 = 2 * (v - cv * hv);
     const double w = 0.25 * 0.125;
@@ -935,9 +939,35 @@ __global__ void k_op_send_to_coarse(const double *__restrict__ fine,
         acc += w * f;
         acc += w * f;
       }
+  } else if (quadrant_only) {
+    return;
   }
   double *dst = coarse + L.offset(i_f, axis, 1 - face, 1);
   dst[idx] = dst[idx] + acc;
+}
+
+__global__ void k_op_send_to_coarse(const double *__restrict__ fine,
+                                    double *__restrict__ coarse, int nf,
+                                    FluxLayout L, int axis, int face,
+                                    Child ch) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = L.m[axis];
+  if (t >= m * nf) return;
+  const int i_f = (int)(t / m);
+  send_to_coarse(fine, coarse, L, axis, face, ch.c, i_f, t - (long)i_f * m,
+                 false);
+}
+
+// one link per blockIdx.y
+__global__ void k_op_send_to_coarse_batch(const enzoe_flux_link *__restrict__ links,
+                                          int nf, FluxLayout L) {
+  const enzoe_flux_link &k = links[blockIdx.y];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = L.m[k.axis];
+  if (t >= m * nf) return;
+  const int i_f = (int)(t / m);
+  send_to_coarse(k.fine_flux_data, k.coarse_flux_data, L, k.axis, k.face,
+                 k.child, i_f, t - (long)i_f * m, true);
 }
 
 // MethodFluxCorrect::flux_correct_ (problem_MethodFluxCorrect.cpp:425-560)
@@ -954,9 +984,16 @@ struct CorrectArgs {
   int correct[6];
 };
 
-__global__ void k_op_flux_correct(CorrectArgs a, const double *__restrict__ fd,
-                                  FluxLayout L, int g, int mx, int my) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// the fields' shared parameters of a batched correction
+struct CorrectShared {
+  int conservative[16];
+  int nf, i_density;
+};
+
+__device__ __forceinline__ void flux_correct_cell(
+    double *const *field, const int *conservative, int nf, int i_density,
+    const int *correct, const double *__restrict__ fd, const FluxLayout &L,
+    int g, int mx, int my, long t) {
   const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
   // shell: the two x faces, then the y faces without the x boundary, then
   // the z faces without the x and y boundaries
@@ -990,29 +1027,29 @@ __global__ void k_op_flux_correct(CorrectArgs a, const double *__restrict__ fd,
     const int a0 = (ax == 0) ? 1 : 0, a1 = (ax == 2) ? 1 : 2;
     ib[ax] = p[a0] + (long)L.n[a0] * p[a1];
     for (int f = 0; f < 2; f++) {
-      on[2 * ax + f] = a.correct[2 * ax + f] && p[ax] == (f ? L.n[ax] - 1 : 0);
+      on[2 * ax + f] = correct[2 * ax + f] && p[ax] == (f ? L.n[ax] - 1 : 0);
       any = any || on[2 * ax + f];
     }
   }
   if (!any) return;
   const long c = (p[0] + g) + (long)mx * ((p[1] + g) + (long)my * (p[2] + g));
   double rho_old = 0.0, rho_new = 0.0;
-  if (a.i_density >= 0) {
-    rho_old = a.field[a.i_density][c];
+  if (i_density >= 0) {
+    rho_old = field[i_density][c];
     double r = rho_old;
     for (int k = 0; k < 6; k++) {
       if (!on[k]) continue;
       const int ax = k >> 1, f = k & 1;
-      const double *bf = fd + L.offset(a.i_density, ax, f, 0);
-      const double *nb = fd + L.offset(a.i_density, ax, f, 1);
+      const double *bf = fd + L.offset(i_density, ax, f, 0);
+      const double *nb = fd + L.offset(i_density, ax, f, 1);
       r += (double)(2 * f - 1) * (bf[ib[ax]] - nb[ib[ax]]);
     }
     rho_new = r;
-    a.field[a.i_density][c] = r;
+    field[i_density][c] = r;
   }
-  for (int i_f = 0; i_f < a.nf; i_f++) {
-    if (i_f == a.i_density) continue;
-    double q = a.conservative[i_f] ? rho_old * a.field[i_f][c] : a.field[i_f][c];
+  for (int i_f = 0; i_f < nf; i_f++) {
+    if (i_f == i_density) continue;
+    double q = conservative[i_f] ? rho_old * field[i_f][c] : field[i_f][c];
     for (int k = 0; k < 6; k++) {
       if (!on[k]) continue;
       const int ax = k >> 1, f = k & 1;
@@ -1020,8 +1057,25 @@ __global__ void k_op_flux_correct(CorrectArgs a, const double *__restrict__ fd,
       const double *nb = fd + L.offset(i_f, ax, f, 1);
       q += (double)(2 * f - 1) * (bf[ib[ax]] - nb[ib[ax]]);
     }
-    a.field[i_f][c] = a.conservative[i_f] ? q / rho_new : q;
+    field[i_f][c] = conservative[i_f] ? q / rho_new : q;
   }
+}
+
+__global__ void k_op_flux_correct(CorrectArgs a, const double *__restrict__ fd,
+                                  FluxLayout L, int g, int mx, int my) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  flux_correct_cell(a.field, a.conservative, a.nf, a.i_density, a.correct, fd,
+                    L, g, mx, my, t);
+}
+
+// one block per blockIdx.y
+__global__ void k_op_flux_correct_batch(
+    const enzoe_flux_correct_block *__restrict__ blocks, CorrectShared sh,
+    FluxLayout L, int g, int mx, int my) {
+  const enzoe_flux_correct_block &b = blocks[blockIdx.y];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  flux_correct_cell(b.fields, sh.conservative, sh.nf, sh.i_density, b.correct,
+                    b.flux_data, L, g, mx, my, t);
 }
 
 bool even_extents(int nx, int ny, int nz) {
@@ -1089,6 +1143,78 @@ int enzoe_flux_data_send_to_coarse(const double *fine_flux_data,
   hipLaunchKernelGGL(k_op_send_to_coarse, dim3(nblk(work)), dim3(256), 0,
                      (hipStream_t)stream, fine_flux_data, coarse_flux_data, nf,
                      L, axis, face, ch);
+  return launched();
+}
+
+int enzoe_vlct_flux_send_to_coarse_batch(enzoe_vlct_t m,
+                                         const enzoe_flux_link *links,
+                                         int nlinks, int nf, int nx, int ny,
+                                         int nz, enzoe_stream_t stream) {
+  // one link per grid row (at most 65535)
+  if (!m || !links || nlinks < 0 || nlinks > 65535 || nf < 1 ||
+      !even_extents(nx, ny, nz))
+    return ENZOE_ERR_ARG;
+  if (nlinks == 0) return ENZOE_OK;
+  for (int i = 0; i < nlinks; i++) {
+    const enzoe_flux_link &k = links[i];
+    if (!k.fine_flux_data || !k.coarse_flux_data || !valid_dim(k.axis) ||
+        (k.face != 0 && k.face != 1))
+      return ENZOE_ERR_ARG;
+    for (int a = 0; a < 3; a++)
+      if (k.child[a] != 0 && k.child[a] != 1) return ENZOE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (int rc = upload_table(&m->d_fltab, &m->fltab_bytes, &m->last_fltab, links,
+                            (size_t)nlinks * sizeof(enzoe_flux_link), s))
+    return rc;
+  FluxLayout L;
+  L.init(nx, ny, nz);
+  const long work = std::max(L.m[0], std::max(L.m[1], L.m[2])) * nf;
+  {
+    KernelTimer kt(s, "k_send_to_coarse");
+    hipLaunchKernelGGL(k_op_send_to_coarse_batch, dim3(nblk(work), nlinks), dim3(256), 0,
+                       s, (const enzoe_flux_link *)m->d_fltab, nf, L);
+  }
+  return launched();
+}
+
+int enzoe_vlct_flux_correct_blocks(enzoe_vlct_t m,
+                                   const enzoe_flux_correct_block *blocks,
+                                   int n, int nf, int i_density,
+                                   const int *conservative, int nx, int ny,
+                                   int nz, int ghost, enzoe_stream_t stream) {
+  // one block per grid row (at most 65535)
+  if (!m || !blocks || n < 0 || n > 65535 || nf < 0 || nf > 16 || !conservative ||
+      ghost < 0 || i_density >= nf || !even_extents(nx, ny, nz))
+    return ENZOE_ERR_ARG;
+  if (n == 0 || nf == 0) return ENZOE_OK;
+  CorrectShared sh;
+  std::memset(&sh, 0, sizeof(sh));
+  sh.nf = nf;
+  sh.i_density = i_density < 0 ? -1 : i_density;
+  for (int i = 0; i < nf; i++) {
+    sh.conservative[i] = (conservative[i] && i != sh.i_density) ? 1 : 0;
+    // problem_MethodFluxCorrect.cpp:520-524: needs the density field
+    if (sh.conservative[i] && sh.i_density < 0) return ENZOE_ERR_ARG;
+  }
+  for (int b = 0; b < n; b++) {
+    if (!blocks[b].flux_data) return ENZOE_ERR_ARG;
+    for (int i = 0; i < nf; i++)
+      if (!blocks[b].fields[i]) return ENZOE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (int rc = upload_table(&m->d_fbtab, &m->fbtab_bytes, &m->last_fbtab, blocks,
+                            (size_t)n * sizeof(enzoe_flux_correct_block), s))
+    return rc;
+  FluxLayout L;
+  L.init(nx, ny, nz);
+  const long shell = 2L * ny * nz + 2L * (nx - 2) * nz + 2L * (nx - 2) * (ny - 2);
+  {
+    KernelTimer kt(s, "k_flux_correct");
+    hipLaunchKernelGGL(k_op_flux_correct_batch, dim3(nblk(shell), n), dim3(256), 0, s,
+                       (const enzoe_flux_correct_block *)m->d_fbtab, sh, L, ghost,
+                       nx + 2 * ghost, ny + 2 * ghost);
+  }
   return launched();
 }
 

@@ -515,6 +515,10 @@ def ops():
         L.enzoe_flux_data_send_to_coarse.argtypes = [vp, vp, i, i, i, i, i, i, PI, vp]
         L.enzoe_flux_correct.argtypes = [ctypes.POINTER(vp), i, i, PI, vp, i, i, i, i,
                                          PI, vp]
+        L.enzoe_vlct_flux_send_to_coarse_batch.argtypes = [vp, ctypes.POINTER(FluxLink), i,
+                                                           i, i, i, i, vp]
+        L.enzoe_vlct_flux_correct_blocks.argtypes = [vp, ctypes.POINTER(FluxCorrectBlock),
+                                                     i, i, i, PI, i, i, i, i, vp]
         _ops_bound = True
     return L
 
@@ -532,6 +536,47 @@ MAKE_CONSERVATIVE = ("velocity_x", "velocity_y", "velocity_z", "total_energy")
 
 def _ints(v):
     return (ctypes.c_int * len(v))(*v)
+
+
+class FluxLink(ctypes.Structure):
+    _fields_ = [("fine_flux_data", ctypes.c_void_p), ("coarse_flux_data", ctypes.c_void_p),
+                ("axis", ctypes.c_int), ("face", ctypes.c_int),
+                ("child", ctypes.c_int * 3)]
+
+
+class FluxCorrectBlock(ctypes.Structure):
+    _fields_ = [("fields", ctypes.c_void_p * 16), ("flux_data", ctypes.c_void_p),
+                ("correct", ctypes.c_int * 6)]
+
+
+def flux_send_to_coarse_batch(method, links, stream=None):
+    """links: [(fine FluxData, coarse FluxData, axis, face, child)]."""
+    arr = (FluxLink * len(links))()
+    for k, (f, c, axis, face, child) in enumerate(links):
+        arr[k].fine_flux_data, arr[k].coarse_flux_data = f.ptr, c.ptr
+        arr[k].axis, arr[k].face = axis, face
+        arr[k].child[:] = list(child)
+    f0 = links[0][0]
+    check(ops().enzoe_vlct_flux_send_to_coarse_batch(method.handle, arr, len(links),
+                                                     len(f0.names), *f0.n, stream),
+          "enzoe_vlct_flux_send_to_coarse_batch")
+
+
+def flux_correct_blocks(method, entries, stream=None, conservative=MAKE_CONSERVATIVE):
+    """entries: [(DeviceBlock, FluxData, correct[6])], blocks of one size."""
+    names = entries[0][1].names
+    arr = (FluxCorrectBlock * len(entries))()
+    for k, (blk, fd, corr) in enumerate(entries):
+        for i, name in enumerate(names):
+            arr[k].fields[i] = blk.ptr(name)
+        arr[k].flux_data = fd.ptr
+        arr[k].correct[:] = list(corr)
+    i_den = names.index("density") if "density" in names else -1
+    cons = _ints([1 if k in conservative else 0 for k in names])
+    fd0, b0 = entries[0][1], entries[0][0]
+    check(ops().enzoe_vlct_flux_correct_blocks(method.handle, arr, len(entries), len(names),
+                                               i_den, cons, *fd0.n, b0.ghost, stream),
+          "enzoe_vlct_flux_correct_blocks")
 
 
 class FluxData:

@@ -388,6 +388,41 @@ int enzoe_flux_data_send_to_coarse(const double *fine_flux_data,
                                    int ny, int nz, int axis, int face,
                                    const int child[3], enzoe_stream_t stream);
 
+/* A fine block's face toward a coarser neighbour.                          */
+typedef struct {
+  const double *fine_flux_data;  /* the fine block's FluxData              */
+  double *coarse_flux_data;      /* the coarse neighbour's FluxData        */
+  int axis, face;                /* the fine block's face                  */
+  int child[3];                  /* its child index in its parent          */
+} enzoe_flux_link;
+
+/* enzoe_flux_data_send_to_coarse for nlinks links in one launch, blocks of
+ * nx*ny*nz active cells.  The links into one coarse face must be distinct
+ * children; each writes only its own quadrant, concurrently (the one-link
+ * form also adds the zeros of the other quadrants, as the reference does:
+ * the results differ at most in the sign of a zero).  The link table is
+ * copied through the method's device buffer: calls on one handle are
+ * ordered by their stream.                                                 */
+int enzoe_vlct_flux_send_to_coarse_batch(enzoe_vlct_t m,
+                                         const enzoe_flux_link *links,
+                                         int nlinks, int nf, int nx, int ny,
+                                         int nz, enzoe_stream_t stream);
+
+/* One leaf block of a batched flux correction.                              */
+typedef struct {
+  double *fields[16];            /* the nf fields (device, ghosts included) */
+  const double *flux_data;       /* its FluxData                           */
+  int correct[6];                /* [2*axis + face]: the neighbour is finer */
+} enzoe_flux_correct_block;
+
+/* MethodFluxCorrect::flux_correct_ on n blocks of one size in one launch
+ * (arguments as enzoe_flux_correct).                                        */
+int enzoe_vlct_flux_correct_blocks(enzoe_vlct_t m,
+                                   const enzoe_flux_correct_block *blocks,
+                                   int n, int nf, int i_density,
+                                   const int *conservative, int nx, int ny,
+                                   int nz, int ghost, enzoe_stream_t stream);
+
 /* MethodFluxCorrect::flux_correct_ (src/Cello/problem_MethodFluxCorrect.cpp:
  * 425-560) on one leaf block: fields[i] is the device (mz, my, mx) array,
  * ghosts included, of field i of the FluxData list; i_density its density

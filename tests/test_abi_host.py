@@ -255,6 +255,25 @@ def test_flux_correction_entry_points_reject_bad_arguments_without_a_device():
                                 None) == ERR_ARG
     assert L.enzoe_flux_correct(fields, 5, 0, cons, fake, 16, 16, 15, 4, corr,
                                 None) == ERR_ARG
+    rc, h = _create(mhd_choice=b"no_bfield", riemann=b"hllc")
+    links = (ev.FluxLink * 1)()
+    links[0].fine_flux_data = links[0].coarse_flux_data = fake
+    links[0].axis, links[0].face = 0, 0
+    links[0].child[:] = [0, 2, 0]                       # not a child index
+    assert L.enzoe_vlct_flux_send_to_coarse_batch(h, links, 1, 5, 16, 16, 16,
+                                                  None) == ERR_ARG
+    assert L.enzoe_vlct_flux_send_to_coarse_batch(h, links, 70000, 5, 16, 16, 16,
+                                                  None) == ERR_ARG
+    blocks = (ev.FluxCorrectBlock * 1)()
+    blocks[0].flux_data = fake
+    for i in range(5):
+        blocks[0].fields[i] = fake
+    assert L.enzoe_vlct_flux_correct_blocks(h, blocks, 1, 5, -1, cons, 16, 16, 16, 4,
+                                            None) == ERR_ARG
+    blocks[0].fields[3] = None
+    assert L.enzoe_vlct_flux_correct_blocks(h, blocks, 1, 5, 0, cons, 16, 16, 16, 4,
+                                            None) == ERR_ARG
+    L.enzoe_vlct_destroy(h)
 
 
 def _run_host_amr(vlct_params, fc_params):

@@ -73,6 +73,39 @@ def test_store_send_and_correct_match_the_oracle_composite():
         assert abs(err) <= 2e-14 * scale, name
 
 
+def test_batched_send_and_correct_match_the_oracle_composite():
+    """Three copies of the composite in one batched step, one batched send
+    (12 links) and one batched correction (3 coarse blocks)."""
+    ev = _ev()
+    n, g, copies = 16, 4, 3
+    print("batched composites x%d" % copies, flush=True)
+    _, grids_o, fds_o, _, _, dt_o = run_composite_oracle(n, g)
+    cfg = _hydro_cfg()
+    method = method_for(cfg)
+    comp = problems.two_level_composite(n, g)
+    blocks, fds = [], []
+    for _ in range(copies):
+        blocks += [_device_block(arr, n, g, dx) for (_, _, _, dx, arr) in comp]
+        fds += [ev.FluxData((n, n, n)) for _ in comp]
+    ev.compute_store_fluxes(method, blocks, dt_o, fds)
+    links, entries = [], []
+    for c in range(copies):
+        base = 5 * c
+        for j, (_, child, _, _, _) in enumerate(comp[1:], start=1):
+            links.append((fds[base + j], fds[base], 0, 0, child))
+        entries.append((blocks[base], fds[base], [0, 1, 0, 0, 0, 0]))
+    ev.flux_send_to_coarse_batch(method, links)
+    ev.flux_correct_blocks(method, entries)
+    ev.synchronize()
+    for c in range(copies):
+        for j in range(5):
+            got = blocks[5 * c + j].download()
+            ref = grids_o[j].block_arrays(0)
+            for name in NAMES:
+                assert np.array_equal(got[name], ref[name]), (c, j, name)
+            assert np.array_equal(fds[5 * c + j].download(), fds_o[j]), (c, j)
+
+
 def test_send_to_coarse_and_flux_correct_on_random_data():
     ev = _ev()
     rng = np.random.default_rng(5)
