@@ -1078,6 +1078,108 @@ __global__ void k_op_flux_correct_batch(
                     b.flux_data, L, g, mx, my, t);
 }
 
+// ------------------------------------------------ coarse/fine transfers
+
+struct Int3 {
+  int v[3];
+};
+
+// ProlongLinear::apply_ (problem_ProlongLinear.cpp:85-330): source index and
+// weights of fine cell i_f of n_f along one axis; gc = 1 when the coarse
+// region has no ghost cells around the fine region (one-sided weights at
+// its two ends)
+__device__ __forceinline__ void prolong_axis(int i_f, int n_f, int gc, int &ic,
+                                             double &w0, double &w1) {
+  int c = ((i_f + 1) >> 1) - gc;
+  int wa = 1, wb = 3;
+  if (i_f == 0) c += gc;
+  if (i_f == n_f - 1) c -= gc;
+  if (i_f == 0 || i_f == n_f - 1) {
+    wa += 4 * gc;
+    wb -= 4 * gc;
+  }
+  ic = c;
+  w0 = 0.25 * ((i_f & 1) ? wb : wa);
+  w1 = 0.25 * ((i_f & 1) ? wa : wb);
+}
+
+// one thread per fine cell of the region; the sums are formed in the
+// reference's order (8, 4 or 2 terms left to right)
+__global__ void k_op_prolong_linear(double *__restrict__ vf, Int3 mf, Int3 of,
+                                    Int3 nf, const double *__restrict__ vc,
+                                    Int3 mc, Int3 oc, Int3 gc, int rank,
+                                    int accumulate) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nxy = (long)nf.v[0] * nf.v[1];
+  if (t >= nxy * nf.v[2]) return;
+  const int ifz = (int)(t / nxy);
+  const long r = t - (long)ifz * nxy;
+  const int ify = (int)(r / nf.v[0]);
+  const int ifx = (int)(r - (long)ify * nf.v[0]);
+  const long dcx = 1, dcy = mc.v[0], dcz = (long)mc.v[0] * mc.v[1];
+  int icx, icy = 0, icz = 0;
+  double wx0, wx1, wy0 = 0., wy1 = 0., wz0 = 0., wz1 = 0.;
+  prolong_axis(ifx, nf.v[0], gc.v[0], icx, wx0, wx1);
+  if (rank >= 2) prolong_axis(ify, nf.v[1], gc.v[1], icy, wy0, wy1);
+  if (rank >= 3) prolong_axis(ifz, nf.v[2], gc.v[2], icz, wz0, wz1);
+  const long i_c = (oc.v[0] + icx) +
+                   (long)mc.v[0] * ((oc.v[1] + icy) + (long)mc.v[1] * (oc.v[2] + icz));
+  const long i_f = (of.v[0] + ifx) +
+                   (long)mf.v[0] * ((of.v[1] + ify) + (long)mf.v[1] * (of.v[2] + ifz));
+  double v;
+  if (rank == 1) {
+    v = wx0 * vc[i_c] + wx1 * vc[i_c + dcx];
+  } else if (rank == 2) {
+    v = wx0 * wy0 * vc[i_c] + wx1 * wy0 * vc[i_c + dcx] +
+        wx0 * wy1 * vc[i_c + dcy] + wx1 * wy1 * vc[i_c + dcx + dcy];
+  } else {
+    v = wx0 * wy0 * wz0 * vc[i_c] + wx1 * wy0 * wz0 * vc[i_c + dcx] +
+        wx0 * wy1 * wz0 * vc[i_c + dcy] + wx1 * wy1 * wz0 * vc[i_c + dcx + dcy] +
+        wx0 * wy0 * wz1 * vc[i_c + dcz] + wx1 * wy0 * wz1 * vc[i_c + dcx + dcz] +
+        wx0 * wy1 * wz1 * vc[i_c + dcy + dcz] +
+        wx1 * wy1 * wz1 * vc[i_c + dcx + dcy + dcz];
+  }
+  vf[i_f] = accumulate ? vf[i_f] + v : v;
+}
+
+// RestrictLinear::apply_ (problem_RestrictLinear.cpp:60-198): one thread per
+// coarse cell, the mean of its 2^rank children summed in the reference's
+// order
+__global__ void k_op_restrict_linear(double *__restrict__ vc, Int3 ndc, Int3 imc,
+                                     Int3 nc, const double *__restrict__ vf,
+                                     Int3 ndf, Int3 imf, int rank,
+                                     int accumulate) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nxy = (long)nc.v[0] * nc.v[1];
+  if (t >= nxy * nc.v[2]) return;
+  const int izc = (int)(t / nxy);
+  const long r = t - (long)izc * nxy;
+  const int iyc = (int)(r / nc.v[0]);
+  const int ixc = (int)(r - (long)iyc * nc.v[0]);
+  const long dx = 1, dy = ndf.v[0], dz = (long)ndf.v[0] * ndf.v[1];
+  const long i_c = (imc.v[0] + ixc) +
+                   (long)ndc.v[0] * ((imc.v[1] + iyc) + (long)ndc.v[1] * (imc.v[2] + izc));
+  const long i_f = (imf.v[0] + 2 * ixc) +
+                   (long)ndf.v[0] * ((imf.v[1] + 2 * iyc) + (long)ndf.v[1] * (imf.v[2] + 2 * izc));
+  double v;
+  if (rank == 1) {
+    v = 0.5 * (vf[i_f] + vf[i_f + dx]);
+  } else if (rank == 2) {
+    v = 0.25 * (vf[i_f] + vf[i_f + dx] + vf[i_f + dy] + vf[i_f + dx + dy]);
+  } else {
+    v = 0.125 * (vf[i_f] + vf[i_f + dx] + vf[i_f + dy] + vf[i_f + dx + dy] +
+                 vf[i_f + dz] + vf[i_f + dx + dz] + vf[i_f + dy + dz] +
+                 vf[i_f + dx + dy + dz]);
+  }
+  vc[i_c] = accumulate ? vc[i_c] + v : v;
+}
+
+bool region_in(const int m[3], const int o[3], const int n[3]) {
+  for (int a = 0; a < 3; a++)
+    if (m[a] < 1 || o[a] < 0 || n[a] < 1 || o[a] + n[a] > m[a]) return false;
+  return true;
+}
+
 bool even_extents(int nx, int ny, int nz) {
   return nx >= 2 && ny >= 2 && nz >= 2 && !(nx & 1) && !(ny & 1) && !(nz & 1);
 }
@@ -1215,6 +1317,56 @@ int enzoe_vlct_flux_correct_blocks(enzoe_vlct_t m,
                        (const enzoe_flux_correct_block *)m->d_fbtab, sh, L, ghost,
                        nx + 2 * ghost, ny + 2 * ghost);
   }
+  return launched();
+}
+
+int enzoe_prolong_linear(double *values_f, const int mf3[3], const int of3[3],
+                         const int nf3[3], const double *values_c,
+                         const int mc3[3], const int oc3[3], const int nc3[3],
+                         int accumulate, enzoe_stream_t stream) {
+  if (!values_f || !values_c || !mf3 || !of3 || !nf3 || !mc3 || !oc3 || !nc3)
+    return ENZOE_ERR_ARG;
+  if (!region_in(mf3, of3, nf3) || !region_in(mc3, oc3, nc3)) return ENZOE_ERR_ARG;
+  const int rank = (mf3[1] == 1) ? 1 : ((mf3[2] == 1) ? 2 : 3);
+  Int3 mf, of, nf, mc, oc, gc;
+  for (int a = 0; a < 3; a++) {
+    // problem_ProlongLinear.cpp:97-103
+    if (a < rank && !(nf3[a] == 2 * nc3[a] || nf3[a] == 2 * (nc3[a] - 2)))
+      return ENZOE_ERR_ARG;
+    // the two-cell stencil stays inside the coarse region
+    if (a < rank && nc3[a] < 2) return ENZOE_ERR_ARG;
+    mf.v[a] = mf3[a]; of.v[a] = of3[a]; nf.v[a] = nf3[a];
+    mc.v[a] = mc3[a]; oc.v[a] = oc3[a];
+    gc.v[a] = (nf3[a] == 2 * nc3[a]) ? 1 : 0;
+  }
+  const long work = (long)nf3[0] * nf3[1] * nf3[2];
+  hipLaunchKernelGGL(k_op_prolong_linear, dim3(nblk(work)), dim3(256), 0,
+                     (hipStream_t)stream, values_f, mf, of, nf, values_c, mc, oc, gc,
+                     rank, accumulate ? 1 : 0);
+  return launched();
+}
+
+int enzoe_restrict_linear(double *values_c, const int nd3_c[3],
+                          const int im3_c[3], const int n3_c[3],
+                          const double *values_f, const int nd3_f[3],
+                          const int im3_f[3], const int n3_f[3],
+                          int accumulate, enzoe_stream_t stream) {
+  if (!values_c || !values_f || !nd3_c || !im3_c || !n3_c || !nd3_f || !im3_f || !n3_f)
+    return ENZOE_ERR_ARG;
+  const int rank = (nd3_f[1] == 1) ? 1 : ((nd3_f[2] == 1) ? 2 : 3);
+  int fine_n[3];
+  for (int a = 0; a < 3; a++) fine_n[a] = a < rank ? 2 * n3_c[a] : 1;
+  if (!region_in(nd3_c, im3_c, n3_c) || !region_in(nd3_f, im3_f, fine_n))
+    return ENZOE_ERR_ARG;
+  Int3 ndc, imc, nc, ndf, imf;
+  for (int a = 0; a < 3; a++) {
+    ndc.v[a] = nd3_c[a]; imc.v[a] = im3_c[a]; nc.v[a] = n3_c[a];
+    ndf.v[a] = nd3_f[a]; imf.v[a] = im3_f[a];
+  }
+  const long work = (long)n3_c[0] * n3_c[1] * n3_c[2];
+  hipLaunchKernelGGL(k_op_restrict_linear, dim3(nblk(work)), dim3(256), 0,
+                     (hipStream_t)stream, values_c, ndc, imc, nc, values_f, ndf, imf, rank,
+                     accumulate ? 1 : 0);
   return launched();
 }
 

@@ -515,6 +515,8 @@ def ops():
         L.enzoe_flux_data_send_to_coarse.argtypes = [vp, vp, i, i, i, i, i, i, PI, vp]
         L.enzoe_flux_correct.argtypes = [ctypes.POINTER(vp), i, i, PI, vp, i, i, i, i,
                                          PI, vp]
+        L.enzoe_prolong_linear.argtypes = [vp, PI, PI, PI, vp, PI, PI, PI, i, vp]
+        L.enzoe_restrict_linear.argtypes = [vp, PI, PI, PI, vp, PI, PI, PI, i, vp]
         L.enzoe_vlct_flux_send_to_coarse_batch.argtypes = [vp, ctypes.POINTER(FluxLink), i,
                                                            i, i, i, i, vp]
         L.enzoe_vlct_flux_correct_blocks.argtypes = [vp, ctypes.POINTER(FluxCorrectBlock),
@@ -678,6 +680,10 @@ def host_lib():
         H.enzoe_host_amr_set_face.argtypes = [vp] + [ctypes.c_int] * 5 + [PI]
         H.enzoe_host_amr_cycle.argtypes = [vp, ctypes.c_double, vp, _dp, ctypes.c_int]
         H.enzoe_host_amr_destroy.argtypes = [vp]
+        H.enzoe_host_prolong_apply.argtypes = [ctypes.c_char_p, ctypes.c_int, vp, PI, PI, PI,
+                                               vp, PI, PI, PI, ctypes.c_int, vp]
+        H.enzoe_host_restrict_apply.argtypes = [ctypes.c_char_p, vp, PI, PI, PI, vp, PI, PI,
+                                                PI, ctypes.c_int, vp]
         _host = H
     return _host
 
@@ -756,3 +762,21 @@ class HostMethod:
     def timestep(self, block, stream=None):
         return host_lib().enzoe_host_method_timestep(self.handle, ctypes.byref(block.desc),
                                                      stream)
+
+
+# ------------------------------------------------ coarse/fine transfers
+
+def prolong_linear(values_f, mf3, of3, nf3, values_c, mc3, oc3, nc3, accumulate=False,
+                   stream=None):
+    """ProlongLinear::apply on device pointers (dims x, y, z)."""
+    check(ops().enzoe_prolong_linear(values_f, _ints(mf3), _ints(of3), _ints(nf3), values_c,
+                                     _ints(mc3), _ints(oc3), _ints(nc3), int(accumulate),
+                                     stream), "enzoe_prolong_linear")
+
+
+def restrict_linear(values_c, nd3_c, im3_c, n3_c, values_f, nd3_f, im3_f, n3_f,
+                    accumulate=False, stream=None):
+    """RestrictLinear::apply on device pointers (dims x, y, z)."""
+    check(ops().enzoe_restrict_linear(values_c, _ints(nd3_c), _ints(im3_c), _ints(n3_c),
+                                      values_f, _ints(nd3_f), _ints(im3_f), _ints(n3_f),
+                                      int(accumulate), stream), "enzoe_restrict_linear")

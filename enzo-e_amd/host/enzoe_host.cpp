@@ -1406,6 +1406,72 @@ double EnzoMethodMHDVlct::timestep(Block *block) throw() {
   return dtBaryons * courant_;
 }
 
+// ------------------------------------------------ coarse/fine transfers
+
+void ProlongLinear::apply(precision_type precision, void *values_f, int m3_f[3],
+                          int o3_f[3], int n3_f[3], const void *values_c, int m3_c[3],
+                          int o3_c[3], int n3_c[3], bool accumulate) {
+  ENZOE_ASSERT("ProlongLinear::apply()",
+               "Unknown precision " + std::to_string(precision) +
+                   " (double precision only)",
+               precision == precision_double || precision == precision_default);
+  check_rc(enzoe_prolong_linear(static_cast<double *>(values_f), m3_f, o3_f, n3_f,
+                                static_cast<const double *>(values_c), m3_c, o3_c, n3_c,
+                                accumulate ? 1 : 0, g_stream),
+           "ProlongLinear::apply_");
+}
+
+EnzoProlong::EnzoProlong(std::string type, bool positive, bool use_linear)
+    : positive_(positive ? 1 : 0), use_linear_(use_linear) {
+  // EnzoProlong.cpp:19-31
+  if (type == "3A")
+    method_ = 0;
+  else if (type == "2A")
+    method_ = 1;
+  else if (type == "2B")
+    method_ = 2;
+  else
+    ENZOE_ERROR("EnzoProlong::EnzoProlong", "Unrecognized interpolation method " + type);
+}
+
+void EnzoProlong::apply(precision_type precision, void *values_f, int m3_f[3],
+                        int o3_f[3], int n3_f[3], const void *values_c, int m3_c[3],
+                        int o3_c[3], int n3_c[3], bool accumulate) {
+  ProlongLinear prolong_linear;
+  if (!accumulate) {
+    ENZOE_ASSERT("EnzoProlong::apply()",
+                 "Enzo's interpolate() (use_linear = false) is not provided by the "
+                 "device library",
+                 use_linear_);
+    // EnzoProlong.cpp:62-78: the coarse region without its padding
+    const int rank = (m3_f[1] == 1) ? 1 : ((m3_f[2] == 1) ? 2 : 3);
+    int n3[3] = {n3_c[0], n3_c[1], n3_c[2]}, o3[3] = {o3_c[0], o3_c[1], o3_c[2]};
+    for (int i = 0; i < rank; i++) {
+      n3[i] -= 2;
+      o3[i] += 1;
+    }
+    prolong_linear.apply(precision, values_f, m3_f, o3_f, n3_f, values_c, m3_c, o3, n3,
+                         accumulate);
+  } else {
+    prolong_linear.apply(precision, values_f, m3_f, o3_f, n3_f, values_c, m3_c, o3_c, n3_c,
+                         accumulate);
+  }
+}
+
+int RestrictLinear::apply(precision_type precision, void *values_c, int nd3_c[3],
+                          int im3_c[3], int n3_c[3], const void *values_f, int nd3_f[3],
+                          int im3_f[3], int n3_f[3], bool accumulate) {
+  ENZOE_ASSERT("RestrictLinear::apply()",
+               "Unknown precision " + std::to_string(precision) +
+                   " (double precision only)",
+               precision == precision_double || precision == precision_default);
+  check_rc(enzoe_restrict_linear(static_cast<double *>(values_c), nd3_c, im3_c, n3_c,
+                                 static_cast<const double *>(values_f), nd3_f, im3_f, n3_f,
+                                 accumulate ? 1 : 0, g_stream),
+           "RestrictLinear::apply_");
+  return (int)(sizeof(double) * n3_c[0] * n3_c[1] * n3_c[2]);
+}
+
 // ------------------------------------------------------- flux correction
 
 namespace {

@@ -660,6 +660,87 @@ class EnzoMethodMHDVlct : public Method {
   enzoe_vlct_t device_ = nullptr;
 };
 
+// ------------------------------------------------ coarse/fine transfers
+
+// Cello's precision_enum (src/Cello/cello.hpp:152-163)
+enum precision_enum {
+  precision_unknown, precision_default, precision_single, precision_double,
+  precision_extended64, precision_extended80, precision_extended96, precision_quadruple
+};
+typedef int precision_type;
+
+// Cello Prolong (src/Cello/problem_Prolong.hpp): values_f / values_c are
+// device arrays, dims (x, y, z); only double precision (enzo_float) is
+// provided.  Launches on the stream of enzoe_host_set_stream.
+class Prolong {
+ public:
+  virtual ~Prolong() = default;
+  virtual void apply(precision_type precision, void *values_f, int m3_f[3], int o3_f[3],
+                     int n3_f[3], const void *values_c, int m3_c[3], int o3_c[3],
+                     int n3_c[3], bool accumulate = false) = 0;
+  virtual std::string name() const = 0;
+  // cells of padding needed around the coarse region (problem_Prolong.hpp:61)
+  virtual int coarse_padding() const { return 0; }
+};
+
+// problem_ProlongLinear.cpp:25-370
+class ProlongLinear : public Prolong {
+ public:
+  void apply(precision_type precision, void *values_f, int m3_f[3], int o3_f[3],
+             int n3_f[3], const void *values_c, int m3_c[3], int o3_c[3], int n3_c[3],
+             bool accumulate = false) override;
+  std::string name() const override { return "linear"; }
+};
+
+// src/Enzo/mesh/EnzoProlong.cpp: types "3A", "2A", "2B"; with use_linear
+// (and for accumulate) it is ProlongLinear on the coarse region shrunk by
+// one cell per side (:49-97).  Enzo's Fortran interpolate() behind
+// use_linear = false is not provided (ERROR).
+class EnzoProlong : public Prolong {
+ public:
+  EnzoProlong(std::string type, bool positive, bool use_linear);
+  void apply(precision_type precision, void *values_f, int m3_f[3], int o3_f[3],
+             int n3_f[3], const void *values_c, int m3_c[3], int o3_c[3], int n3_c[3],
+             bool accumulate = false) override;
+  std::string name() const override { return "enzo"; }
+  int coarse_padding() const override { return 1; }
+
+ private:
+  int method_ = -1;
+  int positive_ = 0;
+  bool use_linear_ = false;
+};
+
+// Cello Restrict (src/Cello/problem_Restrict.hpp); returns bytes written.
+class Restrict {
+ public:
+  virtual ~Restrict() = default;
+  virtual int apply(precision_type precision, void *values_c, int nd3_c[3], int im3_c[3],
+                    int n3_c[3], const void *values_f, int nd3_f[3], int im3_f[3],
+                    int n3_f[3], bool accumulate = false) = 0;
+  virtual std::string name() const = 0;
+};
+
+// problem_RestrictLinear.cpp:20-198
+class RestrictLinear : public Restrict {
+ public:
+  int apply(precision_type precision, void *values_c, int nd3_c[3], int im3_c[3],
+            int n3_c[3], const void *values_f, int nd3_f[3], int im3_f[3], int n3_f[3],
+            bool accumulate = false) override;
+  std::string name() const override { return "linear"; }
+};
+
+// src/Enzo/mesh/EnzoRestrict.cpp: writes nothing and returns 0 (:36-45)
+class EnzoRestrict : public Restrict {
+ public:
+  explicit EnzoRestrict(std::string restrict_type = "enzo") { (void)restrict_type; }
+  int apply(precision_type, void *, int[3], int[3], int[3], const void *, int[3], int[3],
+            int[3], bool = false) override {
+    return 0;
+  }
+  std::string name() const override { return "enzo"; }
+};
+
 // ------------------------------------------------------- flux correction
 
 // Cello MethodFluxCorrect (src/Cello/problem_MethodFluxCorrect.cpp): on

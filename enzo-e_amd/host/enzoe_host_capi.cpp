@@ -150,4 +150,50 @@ int enzoe_host_amr_cycle(enzoe_host_amr_t a, double dt, enzoe_stream_t stream,
 
 void enzoe_host_amr_destroy(enzoe_host_amr_t a) { delete a; }
 
+int enzoe_host_prolong_apply(const char *name, int use_linear, double *values_f,
+                             const int m3_f[3], const int o3_f[3], const int n3_f[3],
+                             const double *values_c, const int m3_c[3], const int o3_c[3],
+                             const int n3_c[3], int accumulate, enzoe_stream_t stream) {
+  if (!name) return ENZOE_ERR_ARG;
+  enzoe_host_set_stream(stream);
+  std::unique_ptr<Prolong> p;
+  const std::string n(name);
+  if (n == "linear")
+    p.reset(new ProlongLinear());
+  else if (n == "enzo")
+    p.reset(new EnzoProlong("3A", true, use_linear != 0));
+  else
+    return ENZOE_ERR_CONFIG;
+  int mf[3], of[3], nf[3], mc[3], oc[3], nc[3];
+  for (int a = 0; a < 3; a++) {
+    mf[a] = m3_f[a]; of[a] = o3_f[a]; nf[a] = n3_f[a];
+    mc[a] = m3_c[a]; oc[a] = o3_c[a]; nc[a] = n3_c[a];
+  }
+  p->apply(precision_double, values_f, mf, of, nf, values_c, mc, oc, nc, accumulate != 0);
+  return ENZOE_OK;
+}
+
+int enzoe_host_restrict_apply(const char *name, double *values_c, const int nd3_c[3],
+                              const int im3_c[3], const int n3_c[3], const double *values_f,
+                              const int nd3_f[3], const int im3_f[3], const int n3_f[3],
+                              int accumulate, enzoe_stream_t stream) {
+  if (!name) return -1;
+  enzoe_host_set_stream(stream);
+  std::unique_ptr<Restrict> r;
+  const std::string n(name);
+  if (n == "linear")
+    r.reset(new RestrictLinear());
+  else if (n == "enzo")
+    r.reset(new EnzoRestrict());
+  else
+    return -1;
+  int ndc[3], imc[3], nc[3], ndf[3], imf[3], nf[3];
+  for (int a = 0; a < 3; a++) {
+    ndc[a] = nd3_c[a]; imc[a] = im3_c[a]; nc[a] = n3_c[a];
+    ndf[a] = nd3_f[a]; imf[a] = im3_f[a]; nf[a] = n3_f[a];
+  }
+  return r->apply(precision_double, values_c, ndc, imc, nc, values_f, ndf, imf, nf,
+                  accumulate != 0);
+}
+
 }  // extern "C"

@@ -131,3 +131,35 @@ def relative_l1(grid_a, grid_b, names, ghost=3):
         d = vec[g] if g in vec and vec[g] > 0 else den[nm]
         by_magnitude[nm] = num[nm] / d if d > 0 else num[nm]
     return per_field, by_magnitude
+
+
+# ----------------------------------- ghost zones across the refinement face
+
+def composite_interface_ghosts(coarse, fines, n, g, names, stream=None):
+    """Fills the ghost zones on both sides of the refinement face of the
+    two-level composite (problems.two_level_composite: coarse block on
+    [0,1]^3, fine children (0, cy, cz) across its high-x face), from device
+    data: each fine block's low-x ghost slab (g fine cells deep, its whole
+    y-z extent with ghosts) by linear prolongation of the coarse block (with
+    one coarse cell of stencil around it), and the part of the coarse
+    block's high-x ghost slab that each fine block covers by restriction of
+    its active cells.  fines: {(cy, cz): DeviceBlock}.  The restrictions
+    go first: the prolongation stencil of the fine cells next to the face
+    reaches one coarse cell across it, into the restricted slab (the coarse
+    ghost cells beyond the fine blocks' y-z extent come from the coarse
+    block's other neighbours and are read as they are)."""
+    m = n + 2 * g
+    for (cy, cz), fb in fines.items():
+        for name in names:
+            ev.restrict_linear(coarse.ptr(name), (m, m, m),
+                               (g + n, g + cy * n // 2, g + cz * n // 2),
+                               (g, n // 2, n // 2), fb.ptr(name), (m, m, m),
+                               (g, g, g), (2 * g, n, n), stream=stream)
+    for (cy, cz), fb in fines.items():
+        for name in names:
+            # coarse cells under the fine ghost slab, plus one cell of stencil
+            oc = (g + n - g // 2 - 1, g + cy * n // 2 - g // 2 - 1,
+                  g + cz * n // 2 - g // 2 - 1)
+            nc = (g // 2 + 2, m // 2 + 2, m // 2 + 2)
+            ev.prolong_linear(fb.ptr(name), (m, m, m), (0, 0, 0), (g, m, m),
+                              coarse.ptr(name), (m, m, m), oc, nc, stream=stream)

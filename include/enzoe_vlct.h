@@ -436,6 +436,33 @@ int enzoe_flux_correct(double *const *fields, int nf, int i_density,
                        int nx, int ny, int nz, int ghost, const int correct[6],
                        enzoe_stream_t stream);
 
+/* ---- coarse/fine transfers (ghost zones across refinement levels) -------
+ * Arrays are dense, x fastest, dims (x, y, z); rank 1 if dims[1] == 1,
+ * 2 if dims[2] == 1, else 3.  FP64 (enzo_float).                           */
+
+/* ProlongLinear::apply (src/Cello/problem_ProlongLinear.cpp:25-370): region
+ * (of3, nf3) of the fine array values_f (dims mf3) from region (oc3, nc3)
+ * of the coarse array values_c (dims mc3) by linear interpolation (weights
+ * 1/4, 3/4 per axis), with nf3 = 2*(nc3-2) when the coarse region carries a
+ * layer of ghost cells around the fine one, or nf3 = 2*nc3 (one-sided
+ * weights 5/4, -1/4 at the ends); other sizes: ENZOE_ERR_ARG (the reference
+ * asserts).  accumulate adds to values_f.  EnzoProlong with use_linear
+ * (src/Enzo/mesh/EnzoProlong.cpp:49-97) is this call with the coarse region
+ * shrunk by one cell on every side.                                         */
+int enzoe_prolong_linear(double *values_f, const int mf3[3], const int of3[3],
+                         const int nf3[3], const double *values_c,
+                         const int mc3[3], const int oc3[3], const int nc3[3],
+                         int accumulate, enzoe_stream_t stream);
+
+/* RestrictLinear::apply (src/Cello/problem_RestrictLinear.cpp:20-198):
+ * region (im3_c, n3_c) of values_c (dims nd3_c) = mean of the 2^rank fine
+ * children starting at im3_f of values_f (dims nd3_f); accumulate adds.    */
+int enzoe_restrict_linear(double *values_c, const int nd3_c[3],
+                          const int im3_c[3], const int n3_c[3],
+                          const double *values_f, const int nd3_f[3],
+                          const int im3_f[3], const int n3_f[3],
+                          int accumulate, enzoe_stream_t stream);
+
 /* Per-kernel average duration (ms) of the launches made while profiling was
  * on (enzoe_set_profiling(1)), measured with HIP events recorded on the
  * launch stream around each kernel; the events are read when the report is

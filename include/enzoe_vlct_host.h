@@ -84,6 +84,27 @@ int enzoe_host_amr_cycle(enzoe_host_amr_t a, double dt, enzoe_stream_t stream,
                          double *digits, int ndigits);
 void enzoe_host_amr_destroy(enzoe_host_amr_t a);
 
+/* Prolong::apply of the C++ layer: name "linear" (ProlongLinear,
+ * src/Cello/problem_ProlongLinear.cpp:25-370) or "enzo" (EnzoProlong "3A",
+ * src/Enzo/mesh/EnzoProlong.cpp:49-97; only use_linear = 1 is provided: the
+ * coarse region given with one cell of padding per side).  Device arrays,
+ * dims (x, y, z).                                                          */
+int enzoe_host_prolong_apply(const char *name, int use_linear, double *values_f,
+                             const int m3_f[3], const int o3_f[3],
+                             const int n3_f[3], const double *values_c,
+                             const int m3_c[3], const int o3_c[3],
+                             const int n3_c[3], int accumulate,
+                             enzoe_stream_t stream);
+/* Restrict::apply: "linear" (RestrictLinear, problem_RestrictLinear.cpp:
+ * 20-198) or "enzo" (EnzoRestrict, which writes nothing,
+ * src/Enzo/mesh/EnzoRestrict.cpp:36-45).  Returns the bytes written.       */
+int enzoe_host_restrict_apply(const char *name, double *values_c,
+                              const int nd3_c[3], const int im3_c[3],
+                              const int n3_c[3], const double *values_f,
+                              const int nd3_f[3], const int im3_f[3],
+                              const int n3_f[3], int accumulate,
+                              enzoe_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,6 +125,10 @@ def lib():
                                      dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ip]
         L.o_flux_correct.restype = ctypes.c_int
+        L.o_prolong_linear.argtypes = [dp, ip, ip, ip, dp, ip, ip, ip, ctypes.c_int]
+        L.o_prolong_linear.restype = ctypes.c_int
+        L.o_restrict_linear.argtypes = [dp, ip, ip, ip, dp, ip, ip, ip, ctypes.c_int]
+        L.o_restrict_linear.restype = ctypes.c_int
         _lib = L
     return _lib
 

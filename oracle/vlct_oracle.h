@@ -109,6 +109,21 @@ int o_flux_correct(double *const *fields, int nf, int i_density,
                    const int *conservative, const double *flux_data, int nx,
                    int ny, int nz, int g, const int correct[6]);
 
+/* Coarse/fine transfers (mesh_oracle.c).  ProlongLinear::apply_
+ * (src/Cello/problem_ProlongLinear.cpp:85-370): region (of3, nf3) of the
+ * fine array (dims mf3, x fastest) from region (oc3, nc3) of the coarse
+ * array; nf3 = 2*nc3 (coarse ghosts available) or 2*(nc3-2).  Returns -1
+ * for other sizes (the reference asserts).  RestrictLinear::apply_
+ * (problem_RestrictLinear.cpp:60-198): the mean of the 2^rank children.    */
+int o_prolong_linear(double *values_f, const int mf3[3], const int of3[3],
+                     const int nf3[3], const double *values_c,
+                     const int mc3[3], const int oc3[3], const int nc3[3],
+                     int accumulate);
+int o_restrict_linear(double *values_c, const int nd3_c[3], const int im3_c[3],
+                      const int n3_c[3], const double *values_f,
+                      const int nd3_f[3], const int im3_f[3],
+                      const int n3_f[3], int accumulate);
+
 /* EnzoComputePressure::compute_pressure (EnzoComputePressure.cpp:82-198)
  * over [stale, m-stale) into p.                                            */
 void o_compute_pressure(const o_config *cfg, const o_block *b, double *p,
