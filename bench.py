@@ -60,7 +60,8 @@ def parse():
                          "sub-cycling; turbulence: 512 blocks of 64^3, Mach 5; "
                          "kelvin_helmholtz: 512 blocks of 128^3, hydro; "
                          "smr_flux_correct: 64 x (1 coarse + 4 fine) 32^3 hydro "
-                         "blocks with flux corrections")
+                         "blocks with coarse/fine ghost transfers and flux "
+                         "corrections")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (HBM counter scale)")
     return ap.parse_args()
@@ -406,11 +407,15 @@ def run_smr_flux_correct(a):
     """Static mesh refinement with flux corrections (hydro, the reference's
     VL+CT + flux_correct pairing): 64 units of one coarse 32^3 block and the
     four fine 32^3 blocks across its high-x face (320 blocks, ghost depth 4).
-    A step: dt, the VL+CT step storing the final-stage face fluxes (one
-    batched launch sequence), the 256 fine-to-coarse flux sends (one launch)
-    and the 64 coarse-block corrections (one launch).  The same step without
-    the flux corrections is timed beside it."""
+    A step: the ghost zones on both sides of each refinement face (1280
+    restrictions, then 1280 prolongations: one launch each), dt, the VL+CT
+    step storing the final-stage face fluxes (one batched launch sequence),
+    the 256 fine-to-coarse flux sends (one launch) and the 64 coarse-block
+    corrections (one launch).  The plain step (dt and VL+CT only) is timed
+    beside it."""
+    import workloads
     n, g, units = 32, 4, 64
+    names = ["density", "velocity_x", "velocity_y", "velocity_z", "total_energy"]
     comp = problems.two_level_composite(n, g)
     res = {}
     for store in (True, False):
@@ -427,14 +432,26 @@ def run_smr_flux_correct(a):
                 blocks.append(db)
                 if store:
                     fds.append(ev.FluxData((n, n, n)))
-        links, entries = [], []
+        links, entries, restricts, prolongs = [], [], [], []
         if store:
             for u in range(units):
+                fines = {}
                 for j, (_, child, _, _, _) in enumerate(comp[1:], start=1):
                     links.append((fds[5 * u + j], fds[5 * u], 0, 0, child))
+                    fines[(child[1], child[2])] = blocks[5 * u + j]
                 entries.append((blocks[5 * u], fds[5 * u], [0, 1, 0, 0, 0, 0]))
+                r, p_ = workloads.composite_interface_transfers(blocks[5 * u], fines, n, g,
+                                                                names)
+                restricts += r
+                prolongs += p_
+            # tables built once, outside the timed steps
+            restricts, prolongs = ev.transfer_table(restricts), ev.transfer_table(prolongs)
+            links, entries = ev.FluxLinks(links), ev.FluxCorrections(entries)
 
         def one_step():
+            if store:
+                ev.transfer_batch(method, restricts)
+                ev.transfer_batch(method, prolongs)
             dt = method.timestep(blocks)
             if store:
                 ev.compute_store_fluxes(method, blocks, dt, fds)
@@ -452,7 +469,8 @@ def run_smr_flux_correct(a):
         del blocks, fds, method
     r, plain = res[True], res[False]
     fc_ms = sum(r["kernels_ms"].get(k, 0.0) for k in
-                ("k_store_fluxes", "k_send_to_coarse", "k_flux_correct"))
+                ("k_store_fluxes", "k_send_to_coarse", "k_flux_correct",
+                 "k_restrict_batch", "k_prolong_batch"))
     return {
         "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
@@ -462,13 +480,13 @@ def run_smr_flux_correct(a):
         "config": {"workload": "smr_flux_correct_64x(1+4)x32_%s" % a.precision,
                    "blocks": 5 * units, "block": [n, n, n], "ghost": g, "riemann": "hllc",
                    "reconstruct": "plm", "theta": 2.0, "courant": 0.4,
-                   "flux_links": 4 * units, "corrected_blocks": units},
+                   "flux_links": 4 * units, "corrected_blocks": units,
+                   "ghost_transfers": 2 * len(names) * 4 * units},
         "roofline": r["roofline"], "kernels_ms": r["kernels_ms"],
-        "flux_correction_kernels_ms_per_step": fc_ms,
-        "without_flux_correction": {"value": plain["value"],
-                                    "ms_per_step": plain["ms_per_step"],
-                                    "kernels_ms": plain["kernels_ms"]},
-        "flux_correction_overhead": plain["value"] / r["value"] - 1.0,
+        "refinement_kernels_ms_per_step": fc_ms,
+        "plain_step": {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
+                       "kernels_ms": plain["kernels_ms"]},
+        "refinement_overhead": plain["value"] / r["value"] - 1.0,
     }
 
 

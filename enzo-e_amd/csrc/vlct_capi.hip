@@ -154,7 +154,7 @@ int enzoe_vlct_destroy(enzoe_vlct_t m) {
   if (m->d_ptrtab) (void)hipFree(m->d_ptrtab);
   if (m->d_dts) (void)hipFree(m->d_dts);
   if (m->d_rtab) (void)hipFree(m->d_rtab);
-  for (void *p : {m->d_fdtab, m->d_fltab, m->d_fbtab})
+  for (void *p : {m->d_fdtab, m->d_fltab, m->d_fbtab, m->d_trtab[0], m->d_trtab[1]})
     if (p) (void)hipFree(p);
   delete m;
   return ENZOE_OK;

@@ -105,6 +105,11 @@ struct enzoe_vlct_method {
   void *d_fdtab = nullptr, *d_fltab = nullptr, *d_fbtab = nullptr;
   size_t fdtab_bytes = 0, fltab_bytes = 0, fbtab_bytes = 0;
   std::vector<char> last_fdtab, last_fltab, last_fbtab;
+  // transfer tables of enzoe_vlct_transfer_batch: two, so that the
+  // restriction and prolongation batches of a cycle both stay cached
+  void *d_trtab[2] = {nullptr, nullptr};
+  size_t trtab_bytes[2] = {0, 0};
+  std::vector<char> last_trtab[2];
 };
 
 namespace enzoe {

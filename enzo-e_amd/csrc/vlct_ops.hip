@@ -1103,29 +1103,31 @@ __device__ __forceinline__ void prolong_axis(int i_f, int n_f, int gc, int &ic,
   w1 = 0.25 * ((i_f & 1) ? wa : wb);
 }
 
-// one thread per fine cell of the region; the sums are formed in the
-// reference's order (8, 4 or 2 terms left to right)
-__global__ void k_op_prolong_linear(double *__restrict__ vf, Int3 mf, Int3 of,
-                                    Int3 nf, const double *__restrict__ vc,
-                                    Int3 mc, Int3 oc, Int3 gc, int rank,
-                                    int accumulate) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long nxy = (long)nf.v[0] * nf.v[1];
-  if (t >= nxy * nf.v[2]) return;
+// one fine cell t of the region; the sums are formed in the reference's
+// order (8, 4 or 2 terms left to right)
+__device__ __forceinline__ void prolong_cell(double *__restrict__ vf,
+                                             const int *mf, const int *of,
+                                             const int *nf,
+                                             const double *__restrict__ vc,
+                                             const int *mc, const int *oc,
+                                             const int *gc, int rank,
+                                             int accumulate, long t) {
+  const long nxy = (long)nf[0] * nf[1];
+  if (t >= nxy * nf[2]) return;
   const int ifz = (int)(t / nxy);
   const long r = t - (long)ifz * nxy;
-  const int ify = (int)(r / nf.v[0]);
-  const int ifx = (int)(r - (long)ify * nf.v[0]);
-  const long dcx = 1, dcy = mc.v[0], dcz = (long)mc.v[0] * mc.v[1];
+  const int ify = (int)(r / nf[0]);
+  const int ifx = (int)(r - (long)ify * nf[0]);
+  const long dcx = 1, dcy = mc[0], dcz = (long)mc[0] * mc[1];
   int icx, icy = 0, icz = 0;
   double wx0, wx1, wy0 = 0., wy1 = 0., wz0 = 0., wz1 = 0.;
-  prolong_axis(ifx, nf.v[0], gc.v[0], icx, wx0, wx1);
-  if (rank >= 2) prolong_axis(ify, nf.v[1], gc.v[1], icy, wy0, wy1);
-  if (rank >= 3) prolong_axis(ifz, nf.v[2], gc.v[2], icz, wz0, wz1);
-  const long i_c = (oc.v[0] + icx) +
-                   (long)mc.v[0] * ((oc.v[1] + icy) + (long)mc.v[1] * (oc.v[2] + icz));
-  const long i_f = (of.v[0] + ifx) +
-                   (long)mf.v[0] * ((of.v[1] + ify) + (long)mf.v[1] * (of.v[2] + ifz));
+  prolong_axis(ifx, nf[0], gc[0], icx, wx0, wx1);
+  if (rank >= 2) prolong_axis(ify, nf[1], gc[1], icy, wy0, wy1);
+  if (rank >= 3) prolong_axis(ifz, nf[2], gc[2], icz, wz0, wz1);
+  const long i_c = (oc[0] + icx) +
+                   (long)mc[0] * ((oc[1] + icy) + (long)mc[1] * (oc[2] + icz));
+  const long i_f = (of[0] + ifx) +
+                   (long)mf[0] * ((of[1] + ify) + (long)mf[1] * (of[2] + ifz));
   double v;
   if (rank == 1) {
     v = wx0 * vc[i_c] + wx1 * vc[i_c + dcx];
@@ -1142,25 +1144,25 @@ __global__ void k_op_prolong_linear(double *__restrict__ vf, Int3 mf, Int3 of,
   vf[i_f] = accumulate ? vf[i_f] + v : v;
 }
 
-// RestrictLinear::apply_ (problem_RestrictLinear.cpp:60-198): one thread per
-// coarse cell, the mean of its 2^rank children summed in the reference's
-// order
-__global__ void k_op_restrict_linear(double *__restrict__ vc, Int3 ndc, Int3 imc,
-                                     Int3 nc, const double *__restrict__ vf,
-                                     Int3 ndf, Int3 imf, int rank,
-                                     int accumulate) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long nxy = (long)nc.v[0] * nc.v[1];
-  if (t >= nxy * nc.v[2]) return;
+// RestrictLinear::apply_ (problem_RestrictLinear.cpp:60-198): one coarse
+// cell t, the mean of its 2^rank children summed in the reference's order
+__device__ __forceinline__ void restrict_cell(double *__restrict__ vc,
+                                              const int *ndc, const int *imc,
+                                              const int *nc,
+                                              const double *__restrict__ vf,
+                                              const int *ndf, const int *imf,
+                                              int rank, int accumulate, long t) {
+  const long nxy = (long)nc[0] * nc[1];
+  if (t >= nxy * nc[2]) return;
   const int izc = (int)(t / nxy);
   const long r = t - (long)izc * nxy;
-  const int iyc = (int)(r / nc.v[0]);
-  const int ixc = (int)(r - (long)iyc * nc.v[0]);
-  const long dx = 1, dy = ndf.v[0], dz = (long)ndf.v[0] * ndf.v[1];
-  const long i_c = (imc.v[0] + ixc) +
-                   (long)ndc.v[0] * ((imc.v[1] + iyc) + (long)ndc.v[1] * (imc.v[2] + izc));
-  const long i_f = (imf.v[0] + 2 * ixc) +
-                   (long)ndf.v[0] * ((imf.v[1] + 2 * iyc) + (long)ndf.v[1] * (imf.v[2] + 2 * izc));
+  const int iyc = (int)(r / nc[0]);
+  const int ixc = (int)(r - (long)iyc * nc[0]);
+  const long dx = 1, dy = ndf[0], dz = (long)ndf[0] * ndf[1];
+  const long i_c = (imc[0] + ixc) +
+                   (long)ndc[0] * ((imc[1] + iyc) + (long)ndc[1] * (imc[2] + izc));
+  const long i_f = (imf[0] + 2 * ixc) +
+                   (long)ndf[0] * ((imf[1] + 2 * iyc) + (long)ndf[1] * (imf[2] + 2 * izc));
   double v;
   if (rank == 1) {
     v = 0.5 * (vf[i_f] + vf[i_f + dx]);
@@ -1174,10 +1176,65 @@ __global__ void k_op_restrict_linear(double *__restrict__ vc, Int3 ndc, Int3 imc
   vc[i_c] = accumulate ? vc[i_c] + v : v;
 }
 
+__global__ void k_op_prolong_linear(double *__restrict__ vf, Int3 mf, Int3 of,
+                                    Int3 nf, const double *__restrict__ vc,
+                                    Int3 mc, Int3 oc, Int3 gc, int rank,
+                                    int accumulate) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  prolong_cell(vf, mf.v, of.v, nf.v, vc, mc.v, oc.v, gc.v, rank, accumulate, t);
+}
+
+__global__ void k_op_restrict_linear(double *__restrict__ vc, Int3 ndc, Int3 imc,
+                                     Int3 nc, const double *__restrict__ vf,
+                                     Int3 ndf, Int3 imf, int rank,
+                                     int accumulate) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  restrict_cell(vc, ndc.v, imc.v, nc.v, vf, ndf.v, imf.v, rank, accumulate, t);
+}
+
+// one transfer per grid row
+__global__ void k_op_transfer_batch(const enzoe_transfer *__restrict__ items) {
+  const enzoe_transfer &it = items[blockIdx.y];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it.kind == ENZOE_TRANSFER_PROLONG) {
+    const int rank = (it.m_dst[1] == 1) ? 1 : ((it.m_dst[2] == 1) ? 2 : 3);
+    int gc[3];
+    for (int a = 0; a < 3; a++) gc[a] = (it.n_dst[a] == 2 * it.n_src[a]) ? 1 : 0;
+    prolong_cell(it.dst, it.m_dst, it.o_dst, it.n_dst, it.src, it.m_src, it.o_src, gc,
+                 rank, it.accumulate, t);
+  } else {
+    const int rank = (it.m_src[1] == 1) ? 1 : ((it.m_src[2] == 1) ? 2 : 3);
+    restrict_cell(it.dst, it.m_dst, it.o_dst, it.n_dst, it.src, it.m_src, it.o_src, rank,
+                  it.accumulate, t);
+  }
+}
+
 bool region_in(const int m[3], const int o[3], const int n[3]) {
   for (int a = 0; a < 3; a++)
     if (m[a] < 1 || o[a] < 0 || n[a] < 1 || o[a] + n[a] > m[a]) return false;
   return true;
+}
+
+// region bounds, the size relation the reference asserts
+// (problem_ProlongLinear.cpp:97-103) and a two-cell stencil inside the
+// coarse region
+bool prolong_ok(const int mf3[3], const int of3[3], const int nf3[3],
+                const int mc3[3], const int oc3[3], const int nc3[3]) {
+  if (!region_in(mf3, of3, nf3) || !region_in(mc3, oc3, nc3)) return false;
+  const int rank = (mf3[1] == 1) ? 1 : ((mf3[2] == 1) ? 2 : 3);
+  for (int a = 0; a < rank; a++) {
+    if (!(nf3[a] == 2 * nc3[a] || nf3[a] == 2 * (nc3[a] - 2))) return false;
+    if (nc3[a] < 2) return false;
+  }
+  return true;
+}
+
+bool restrict_ok(const int nd3_c[3], const int im3_c[3], const int n3_c[3],
+                 const int nd3_f[3], const int im3_f[3]) {
+  const int rank = (nd3_f[1] == 1) ? 1 : ((nd3_f[2] == 1) ? 2 : 3);
+  int fine_n[3];
+  for (int a = 0; a < 3; a++) fine_n[a] = a < rank ? 2 * n3_c[a] : 1;
+  return region_in(nd3_c, im3_c, n3_c) && region_in(nd3_f, im3_f, fine_n);
 }
 
 bool even_extents(int nx, int ny, int nz) {
@@ -1326,15 +1383,10 @@ int enzoe_prolong_linear(double *values_f, const int mf3[3], const int of3[3],
                          int accumulate, enzoe_stream_t stream) {
   if (!values_f || !values_c || !mf3 || !of3 || !nf3 || !mc3 || !oc3 || !nc3)
     return ENZOE_ERR_ARG;
-  if (!region_in(mf3, of3, nf3) || !region_in(mc3, oc3, nc3)) return ENZOE_ERR_ARG;
+  if (!prolong_ok(mf3, of3, nf3, mc3, oc3, nc3)) return ENZOE_ERR_ARG;
   const int rank = (mf3[1] == 1) ? 1 : ((mf3[2] == 1) ? 2 : 3);
   Int3 mf, of, nf, mc, oc, gc;
   for (int a = 0; a < 3; a++) {
-    // problem_ProlongLinear.cpp:97-103
-    if (a < rank && !(nf3[a] == 2 * nc3[a] || nf3[a] == 2 * (nc3[a] - 2)))
-      return ENZOE_ERR_ARG;
-    // the two-cell stencil stays inside the coarse region
-    if (a < rank && nc3[a] < 2) return ENZOE_ERR_ARG;
     mf.v[a] = mf3[a]; of.v[a] = of3[a]; nf.v[a] = nf3[a];
     mc.v[a] = mc3[a]; oc.v[a] = oc3[a];
     gc.v[a] = (nf3[a] == 2 * nc3[a]) ? 1 : 0;
@@ -1346,6 +1398,42 @@ int enzoe_prolong_linear(double *values_f, const int mf3[3], const int of3[3],
   return launched();
 }
 
+int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
+                              int n, enzoe_stream_t stream) {
+  // one transfer per grid row (at most 65535)
+  if (!m || !items || n < 0 || n > 65535) return ENZOE_ERR_ARG;
+  if (n == 0) return ENZOE_OK;
+  long work = 0;
+  int kinds = 0;
+  for (int i = 0; i < n; i++) {
+    const enzoe_transfer &it = items[i];
+    if (!it.dst || !it.src) return ENZOE_ERR_ARG;
+    if (it.kind == ENZOE_TRANSFER_PROLONG) {
+      if (!prolong_ok(it.m_dst, it.o_dst, it.n_dst, it.m_src, it.o_src, it.n_src))
+        return ENZOE_ERR_ARG;
+    } else if (it.kind == ENZOE_TRANSFER_RESTRICT) {
+      if (!restrict_ok(it.m_dst, it.o_dst, it.n_dst, it.m_src, it.o_src))
+        return ENZOE_ERR_ARG;
+    } else {
+      return ENZOE_ERR_ARG;
+    }
+    kinds |= 1 << it.kind;
+    work = std::max(work, (long)it.n_dst[0] * it.n_dst[1] * it.n_dst[2]);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  // restriction-only and other batches keep separate cached tables
+  const int slot = (kinds == (1 << ENZOE_TRANSFER_RESTRICT)) ? 1 : 0;
+  if (int rc = upload_table(&m->d_trtab[slot], &m->trtab_bytes[slot], &m->last_trtab[slot],
+                            items, (size_t)n * sizeof(enzoe_transfer), s))
+    return rc;
+  {
+    KernelTimer kt(s, slot ? "k_restrict_batch" : "k_prolong_batch");
+    hipLaunchKernelGGL(k_op_transfer_batch, dim3(nblk(work), n), dim3(256), 0, s,
+                       (const enzoe_transfer *)m->d_trtab[slot]);
+  }
+  return launched();
+}
+
 int enzoe_restrict_linear(double *values_c, const int nd3_c[3],
                           const int im3_c[3], const int n3_c[3],
                           const double *values_f, const int nd3_f[3],
@@ -1354,10 +1442,7 @@ int enzoe_restrict_linear(double *values_c, const int nd3_c[3],
   if (!values_c || !values_f || !nd3_c || !im3_c || !n3_c || !nd3_f || !im3_f || !n3_f)
     return ENZOE_ERR_ARG;
   const int rank = (nd3_f[1] == 1) ? 1 : ((nd3_f[2] == 1) ? 2 : 3);
-  int fine_n[3];
-  for (int a = 0; a < 3; a++) fine_n[a] = a < rank ? 2 * n3_c[a] : 1;
-  if (!region_in(nd3_c, im3_c, n3_c) || !region_in(nd3_f, im3_f, fine_n))
-    return ENZOE_ERR_ARG;
+  if (!restrict_ok(nd3_c, im3_c, n3_c, nd3_f, im3_f)) return ENZOE_ERR_ARG;
   Int3 ndc, imc, nc, ndf, imf;
   for (int a = 0; a < 3; a++) {
     ndc.v[a] = nd3_c[a]; imc.v[a] = im3_c[a]; nc.v[a] = n3_c[a];

@@ -516,6 +516,7 @@ def ops():
         L.enzoe_flux_correct.argtypes = [ctypes.POINTER(vp), i, i, PI, vp, i, i, i, i,
                                          PI, vp]
         L.enzoe_prolong_linear.argtypes = [vp, PI, PI, PI, vp, PI, PI, PI, i, vp]
+        L.enzoe_vlct_transfer_batch.argtypes = [vp, ctypes.POINTER(Transfer), i, vp]
         L.enzoe_restrict_linear.argtypes = [vp, PI, PI, PI, vp, PI, PI, PI, i, vp]
         L.enzoe_vlct_flux_send_to_coarse_batch.argtypes = [vp, ctypes.POINTER(FluxLink), i,
                                                            i, i, i, i, vp]
@@ -540,6 +541,36 @@ def _ints(v):
     return (ctypes.c_int * len(v))(*v)
 
 
+TRANSFER_PROLONG, TRANSFER_RESTRICT = 0, 1
+
+
+class Transfer(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("accumulate", ctypes.c_int),
+                ("dst", ctypes.c_void_p), ("src", ctypes.c_void_p),
+                ("m_dst", ctypes.c_int * 3), ("o_dst", ctypes.c_int * 3),
+                ("n_dst", ctypes.c_int * 3), ("m_src", ctypes.c_int * 3),
+                ("o_src", ctypes.c_int * 3), ("n_src", ctypes.c_int * 3)]
+
+
+def transfer_table(items):
+    """ctypes table of [(kind, dst, m_dst, o_dst, n_dst, src, m_src, o_src,
+    n_src, accumulate)] (build once, launch many times)."""
+    arr = (Transfer * len(items))()
+    for k, (kind, dst, md, od, nd, src, ms, os_, ns, acc) in enumerate(items):
+        t = arr[k]
+        t.kind, t.accumulate, t.dst, t.src = kind, int(acc), dst, src
+        t.m_dst[:], t.o_dst[:], t.n_dst[:] = list(md), list(od), list(nd)
+        t.m_src[:], t.o_src[:], t.n_src[:] = list(ms), list(os_), list(ns)
+    return arr
+
+
+def transfer_batch(method, items, stream=None):
+    """The transfers (a list, or a table from transfer_table) in one launch."""
+    arr = items if isinstance(items, ctypes.Array) else transfer_table(items)
+    check(ops().enzoe_vlct_transfer_batch(method.handle, arr, len(arr), stream),
+          "enzoe_vlct_transfer_batch")
+
+
 class FluxLink(ctypes.Structure):
     _fields_ = [("fine_flux_data", ctypes.c_void_p), ("coarse_flux_data", ctypes.c_void_p),
                 ("axis", ctypes.c_int), ("face", ctypes.c_int),
@@ -551,33 +582,49 @@ class FluxCorrectBlock(ctypes.Structure):
                 ("correct", ctypes.c_int * 6)]
 
 
+class FluxLinks:
+    """Table of [(fine FluxData, coarse FluxData, axis, face, child)]."""
+
+    def __init__(self, links):
+        self.arr = (FluxLink * len(links))()
+        for k, (f, c, axis, face, child) in enumerate(links):
+            self.arr[k].fine_flux_data, self.arr[k].coarse_flux_data = f.ptr, c.ptr
+            self.arr[k].axis, self.arr[k].face = axis, face
+            self.arr[k].child[:] = list(child)
+        self.nf, self.n = len(links[0][0].names), links[0][0].n
+
+
 def flux_send_to_coarse_batch(method, links, stream=None):
-    """links: [(fine FluxData, coarse FluxData, axis, face, child)]."""
-    arr = (FluxLink * len(links))()
-    for k, (f, c, axis, face, child) in enumerate(links):
-        arr[k].fine_flux_data, arr[k].coarse_flux_data = f.ptr, c.ptr
-        arr[k].axis, arr[k].face = axis, face
-        arr[k].child[:] = list(child)
-    f0 = links[0][0]
-    check(ops().enzoe_vlct_flux_send_to_coarse_batch(method.handle, arr, len(links),
-                                                     len(f0.names), *f0.n, stream),
+    """links: a list as FluxLinks takes, or a FluxLinks table."""
+    t = links if isinstance(links, FluxLinks) else FluxLinks(links)
+    check(ops().enzoe_vlct_flux_send_to_coarse_batch(method.handle, t.arr, len(t.arr),
+                                                     t.nf, *t.n, stream),
           "enzoe_vlct_flux_send_to_coarse_batch")
 
 
+class FluxCorrections:
+    """Table of [(DeviceBlock, FluxData, correct[6])], blocks of one size."""
+
+    def __init__(self, entries, conservative=MAKE_CONSERVATIVE):
+        names = entries[0][1].names
+        self.arr = (FluxCorrectBlock * len(entries))()
+        for k, (blk, fd, corr) in enumerate(entries):
+            for i, name in enumerate(names):
+                self.arr[k].fields[i] = blk.ptr(name)
+            self.arr[k].flux_data = fd.ptr
+            self.arr[k].correct[:] = list(corr)
+        self.nf = len(names)
+        self.i_den = names.index("density") if "density" in names else -1
+        self.cons = _ints([1 if k in conservative else 0 for k in names])
+        self.n, self.ghost = entries[0][1].n, entries[0][0].ghost
+
+
 def flux_correct_blocks(method, entries, stream=None, conservative=MAKE_CONSERVATIVE):
-    """entries: [(DeviceBlock, FluxData, correct[6])], blocks of one size."""
-    names = entries[0][1].names
-    arr = (FluxCorrectBlock * len(entries))()
-    for k, (blk, fd, corr) in enumerate(entries):
-        for i, name in enumerate(names):
-            arr[k].fields[i] = blk.ptr(name)
-        arr[k].flux_data = fd.ptr
-        arr[k].correct[:] = list(corr)
-    i_den = names.index("density") if "density" in names else -1
-    cons = _ints([1 if k in conservative else 0 for k in names])
-    fd0, b0 = entries[0][1], entries[0][0]
-    check(ops().enzoe_vlct_flux_correct_blocks(method.handle, arr, len(entries), len(names),
-                                               i_den, cons, *fd0.n, b0.ghost, stream),
+    """entries: a list as FluxCorrections takes, or a FluxCorrections table."""
+    t = entries if isinstance(entries, FluxCorrections) else \
+        FluxCorrections(entries, conservative)
+    check(ops().enzoe_vlct_flux_correct_blocks(method.handle, t.arr, len(t.arr), t.nf,
+                                               t.i_den, t.cons, *t.n, t.ghost, stream),
           "enzoe_vlct_flux_correct_blocks")
 
 

@@ -135,6 +135,26 @@ def relative_l1(grid_a, grid_b, names, ghost=3):
 
 # ----------------------------------- ghost zones across the refinement face
 
+def composite_interface_transfers(coarse, fines, n, g, names):
+    """The transfers of composite_interface_ghosts as two lists (the
+    restrictions, then the prolongations that read them) for
+    enzoe_vlct_transfer_batch."""
+    m = n + 2 * g
+    R, P = ev.TRANSFER_RESTRICT, ev.TRANSFER_PROLONG
+    restricts, prolongs = [], []
+    for (cy, cz), fb in fines.items():
+        for name in names:
+            restricts.append((R, coarse.ptr(name), (m, m, m),
+                              (g + n, g + cy * n // 2, g + cz * n // 2), (g, n // 2, n // 2),
+                              fb.ptr(name), (m, m, m), (g, g, g), (2 * g, n, n), 0))
+            prolongs.append((P, fb.ptr(name), (m, m, m), (0, 0, 0), (g, m, m),
+                             coarse.ptr(name), (m, m, m),
+                             (g + n - g // 2 - 1, g + cy * n // 2 - g // 2 - 1,
+                              g + cz * n // 2 - g // 2 - 1),
+                             (g // 2 + 2, m // 2 + 2, m // 2 + 2), 0))
+    return restricts, prolongs
+
+
 def composite_interface_ghosts(coarse, fines, n, g, names, stream=None):
     """Fills the ghost zones on both sides of the refinement face of the
     two-level composite (problems.two_level_composite: coarse block on

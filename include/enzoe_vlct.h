@@ -463,6 +463,31 @@ int enzoe_restrict_linear(double *values_c, const int nd3_c[3],
                           const int im3_f[3], const int n3_f[3],
                           int accumulate, enzoe_stream_t stream);
 
+/* One transfer of a batch: kind ENZOE_TRANSFER_PROLONG (arguments of
+ * enzoe_prolong_linear: dst = values_f with m_dst/o_dst/n_dst = mf3/of3/nf3,
+ * src = values_c with m_src/o_src/n_src = mc3/oc3/nc3) or
+ * ENZOE_TRANSFER_RESTRICT (enzoe_restrict_linear: dst = values_c with
+ * nd3_c/im3_c/n3_c, src = values_f with nd3_f/im3_f/n3_f).                 */
+#define ENZOE_TRANSFER_PROLONG 0
+#define ENZOE_TRANSFER_RESTRICT 1
+typedef struct {
+  int kind, accumulate;
+  double *dst;
+  const double *src;
+  int m_dst[3], o_dst[3], n_dst[3];
+  int m_src[3], o_src[3], n_src[3];
+} enzoe_transfer;
+
+/* n transfers in one launch (grid row = transfer), each validated as the
+ * one-transfer entry points validate theirs.  The transfers of one call run
+ * concurrently: their destination regions must not overlap each other, and
+ * no transfer may read what another writes (order restrictions before the
+ * prolongations that read them by making two calls).  The table is copied
+ * through the method's device buffer (calls on one handle are ordered by
+ * their stream).                                                            */
+int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
+                              int n, enzoe_stream_t stream);
+
 /* Per-kernel average duration (ms) of the launches made while profiling was
  * on (enzoe_set_profiling(1)), measured with HIP events recorded on the
  * launch stream around each kernel; the events are read when the report is

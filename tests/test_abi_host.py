@@ -318,3 +318,29 @@ def test_host_amr_entry_points_reject_bad_arguments():
     d = (ctypes.c_double * 5)()
     assert H.enzoe_host_amr_cycle(None, 0.1, None, d, 5) == ERR_ARG
     assert H.enzoe_host_amr_create(b"", b"", None, None, 0) is None
+
+
+def test_transfer_entry_points_reject_bad_arguments_without_a_device():
+    L = ev.ops()
+    fake = 1 << 40
+    I = lambda *v: (ctypes.c_int * 3)(*v)  # noqa: E731
+    # nf = 2*nc or 2*(nc-2) only (problem_ProlongLinear.cpp:97-103)
+    assert L.enzoe_prolong_linear(fake, I(12, 12, 12), I(0, 0, 0), I(6, 8, 8), fake,
+                                  I(8, 8, 8), I(1, 1, 1), I(4, 4, 4), 0, None) == ERR_ARG
+    # regions outside their arrays
+    assert L.enzoe_prolong_linear(fake, I(8, 8, 8), I(1, 0, 0), I(8, 8, 8), fake,
+                                  I(8, 8, 8), I(0, 0, 0), I(4, 4, 4), 0, None) == ERR_ARG
+    assert L.enzoe_restrict_linear(fake, I(8, 8, 8), I(0, 0, 0), I(4, 4, 4), fake,
+                                   I(8, 8, 8), I(1, 0, 0), I(8, 8, 8), 0, None) == ERR_ARG
+    rc, h = _create(mhd_choice=b"no_bfield", riemann=b"hllc")
+    items = (ev.Transfer * 1)()
+    items[0].kind = 2
+    items[0].dst = items[0].src = fake
+    assert L.enzoe_vlct_transfer_batch(h, items, 1, None) == ERR_ARG
+    items[0].kind = ev.TRANSFER_RESTRICT
+    items[0].m_dst[:] = [8, 8, 8]
+    items[0].n_dst[:] = [4, 4, 4]
+    items[0].m_src[:] = [8, 8, 8]
+    items[0].o_src[:] = [1, 0, 0]              # 2*4 fine cells from x = 1: outside
+    assert L.enzoe_vlct_transfer_batch(h, items, 1, None) == ERR_ARG
+    L.enzoe_vlct_destroy(h)

@@ -114,7 +114,8 @@ def test_host_enzo_prolong_and_restrict():
                                        I3([8, 8, 8]), 0, None) == 8 * 64
 
 
-def test_composite_interface_ghosts_reproduce_a_linear_field():
+@pytest.mark.parametrize("batched", [False, True])
+def test_composite_interface_ghosts_reproduce_a_linear_field(batched):
     ev = _ev()
     import workloads
     n, g = 16, 4
@@ -140,7 +141,14 @@ def test_composite_interface_ghosts_reproduce_a_linear_field():
         blocks[(kind, child)] = (db, f)
     coarse = blocks[("coarse", None)][0]
     fines = {(c[1], c[2]): blocks[("fine", c)][0] for (k, c) in blocks if k == "fine"}
-    workloads.composite_interface_ghosts(coarse, fines, n, g, ["density"])
+    if batched:
+        method = ev.VlctMethod(mhd=False, riemann="hllc")
+        restricts, prolongs = workloads.composite_interface_transfers(
+            coarse, fines, n, g, ["density"])
+        ev.transfer_batch(method, restricts)
+        ev.transfer_batch(method, prolongs)
+    else:
+        workloads.composite_interface_ghosts(coarse, fines, n, g, ["density"])
     ev.synchronize()
     for (kind, child), (db, f) in blocks.items():
         got = db.download(["density"])["density"]
