@@ -727,6 +727,8 @@ def host_lib():
         H.enzoe_host_amr_set_face.argtypes = [vp] + [ctypes.c_int] * 5 + [PI]
         H.enzoe_host_amr_cycle.argtypes = [vp, ctypes.c_double, vp, _dp, ctypes.c_int]
         H.enzoe_host_amr_destroy.argtypes = [vp]
+        H.enzoe_host_initial_inclined_wave.argtypes = [ctypes.c_char_p, ctypes.POINTER(Block),
+                                                       _dp]
         H.enzoe_host_prolong_apply.argtypes = [ctypes.c_char_p, ctypes.c_int, vp, PI, PI, PI,
                                                vp, PI, PI, PI, ctypes.c_int, vp]
         H.enzoe_host_restrict_apply.argtypes = [ctypes.c_char_p, vp, PI, PI, PI, vp, PI, PI,
@@ -827,3 +829,12 @@ def restrict_linear(values_c, nd3_c, im3_c, n3_c, values_f, nd3_f, im3_f, n3_f,
     check(ops().enzoe_restrict_linear(values_c, _ints(nd3_c), _ints(im3_c), _ints(n3_c),
                                       values_f, _ints(nd3_f), _ints(im3_f), _ints(n3_f),
                                       int(accumulate), stream), "enzoe_restrict_linear")
+
+
+def initial_inclined_wave(parameters, block, lower):
+    """EnzoInitialInclinedWave of the C++ layer on a DeviceBlock whose
+    active region starts at `lower`."""
+    lo = (ctypes.c_double * 3)(*lower)
+    check(host_lib().enzoe_host_initial_inclined_wave(parameters.encode(),
+                                                      ctypes.byref(block.desc), lo),
+          "enzoe_host_initial_inclined_wave")

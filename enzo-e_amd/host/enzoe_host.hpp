@@ -660,6 +660,30 @@ class EnzoMethodMHDVlct : public Method {
   enzoe_vlct_t device_ = nullptr;
 };
 
+// -------------------------------------------------------- initial values
+
+// EnzoInitialInclinedWave (src/Enzo/initial/EnzoInitialInclinedWave.cpp):
+// a linear wave along the direction rotated by (alpha, beta).  Parameters
+// (defaults of :647-666): alpha (0), beta (0), amplitude (1e-6), lambda (1),
+// positive_vel (true), wave_type (required: "fast", "alfven", "slow",
+// "mhd_entropy", "sound", "hd_entropy", "hd_transv_entropy_v1",
+// "hd_transv_entropy_v2").  The values are computed on the host, as the
+// reference's initializers do, and copied into the block's device fields
+// (ghost zones included); "circ_alfven", "jeans" (gravity) and
+// parallel_vel are not provided (ERROR).
+class EnzoInitialInclinedWave {
+ public:
+  explicit EnzoInitialInclinedWave(ParameterGroup p);
+  // enforce_block (:722-772) for a leaf block whose active region starts at
+  // `lower` (the block's lower corner in the domain)
+  void enforce_block(Block *block, const std::array<double, 3> &lower) const;
+
+ private:
+  double alpha_, beta_, amplitude_, lambda_;
+  bool pos_vel_;
+  std::string wave_type_;
+};
+
 // ------------------------------------------------ coarse/fine transfers
 
 // Cello's precision_enum (src/Cello/cello.hpp:152-163)

@@ -150,6 +150,16 @@ int enzoe_host_amr_cycle(enzoe_host_amr_t a, double dt, enzoe_stream_t stream,
 
 void enzoe_host_amr_destroy(enzoe_host_amr_t a) { delete a; }
 
+int enzoe_host_initial_inclined_wave(const char *parameters, const enzoe_block *block,
+                                     const double lower[3]) {
+  if (!block || !lower) return ENZOE_ERR_ARG;
+  EnzoInitialInclinedWave init(
+      ParameterGroup(parameters ? parameters : "", "Initial:inclined_wave"));
+  Block b(*block);
+  init.enforce_block(&b, {lower[0], lower[1], lower[2]});
+  return ENZOE_OK;
+}
+
 int enzoe_host_prolong_apply(const char *name, int use_linear, double *values_f,
                              const int m3_f[3], const int o3_f[3], const int n3_f[3],
                              const double *values_c, const int m3_c[3], const int o3_c[3],

@@ -84,6 +84,18 @@ int enzoe_host_amr_cycle(enzoe_host_amr_t a, double dt, enzoe_stream_t stream,
                          double *digits, int ndigits);
 void enzoe_host_amr_destroy(enzoe_host_amr_t a);
 
+/* EnzoInitialInclinedWave(ParameterGroup) + enforce_block on one block
+ * whose active region starts at `lower` (src/Enzo/initial/
+ * EnzoInitialInclinedWave.cpp:647-772): `parameters` holds alpha, beta,
+ * amplitude, lambda, positive_vel and wave_type ("fast", "alfven", "slow",
+ * "mhd_entropy", "sound", "hd_entropy", "hd_transv_entropy_v1|2"); every
+ * field of the block, ghost zones included, is written (device copies of
+ * host-computed values, as the reference initializes on the CPU).  Reference
+ * errors print and exit(1).                                                 */
+int enzoe_host_initial_inclined_wave(const char *parameters,
+                                     const enzoe_block *block,
+                                     const double lower[3]);
+
 /* Prolong::apply of the C++ layer: name "linear" (ProlongLinear,
  * src/Cello/problem_ProlongLinear.cpp:25-370) or "enzo" (EnzoProlong "3A",
  * src/Enzo/mesh/EnzoProlong.cpp:49-97; only use_linear = 1 is provided: the

@@ -344,3 +344,29 @@ def test_transfer_entry_points_reject_bad_arguments_without_a_device():
     items[0].o_src[:] = [1, 0, 0]              # 2*4 fine cells from x = 1: outside
     assert L.enzoe_vlct_transfer_batch(h, items, 1, None) == ERR_ARG
     L.enzoe_vlct_destroy(h)
+
+
+def _run_host_initial(params):
+    code = ("import sys, ctypes; sys.path.insert(0, %r); import enzoe_vlct as ev\n"
+            "ev.host_lib().enzoe_host_set_fluid_props(5.0 / 3.0, 1e-30, 1e-30)\n"
+            "b = ev.Block(); lo = (ctypes.c_double * 3)(0, 0, 0)\n"
+            "print(ev.host_lib().enzoe_host_initial_inclined_wave(%r, ctypes.byref(b), lo))"
+            % (os.path.join(ROOT, "enzo-e_amd"), params.encode()))
+    return subprocess.run([sys.executable, "-c", code], capture_output=True,
+                          text=True, timeout=120)
+
+
+@pytest.mark.parametrize("params,message", [
+    ("alpha = 0.5", "wave_type must be specified"),
+    ("wave_type = shear", "has invalid value, \"shear\""),
+    ("wave_type = jeans", "is not provided"),
+    ("wave_type = circ_alfven", "is not provided"),
+    ("wave_type = sound; parallel_vel = 0.5", "parallel_vel is not provided"),
+    ("wave_type = fast", "A MHD wave requires fields to store magnetic field values."),
+])
+def test_host_initial_inclined_wave_errors_exit_like_the_reference(params, message):
+    """Parameter checks and the MHD-fields assert come before any device
+    copy (the block here has no fields at all)."""
+    r = _run_host_initial(params)
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert "ERROR in" in r.stderr and message in r.stderr, r.stderr
