@@ -158,8 +158,18 @@ def cpu_baseline(n, nsteps):
     rate, el = _oracle_rate(n, nsteps, threads)
     n1, s1 = 64, 10
     rate1, el1 = _oracle_rate(n1, s1, 1)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": rate, "unit": "zone-updates/s",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "cpu_model": model,
+            "host_cpus": os.cpu_count(),
             "sample": "oracle (C restatement of the reference loops, OpenMP "
                       "over z) on the same Orszag-Tang problem at %d^3, %d "
                       "steps, %.1f s" % (n, nsteps, el),
