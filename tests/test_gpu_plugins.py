@@ -208,3 +208,71 @@ def test_passive_scalars_are_advected_with_the_density():
     r = ev.from_device(ptrs["color_rnd"], rho.shape)
     m0, m1 = float(rnd[s].sum()), float(r[s].sum())
     assert abs(m1 - m0) <= 1e-13 * m0, (m0, m1)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_passive_advection_sound_wave_golden(axis):
+    """The reference's passive-advection sound wave
+    (input/vlct/run_passive_advect_sound_test.py, decks
+    input/vlct/passive_advect_sound_wave/method_vlct_{x,y,z}_passive_sound.in):
+    a 1e-6 sound wave along one axis of a 16x4x4 periodic block with HLLD
+    + CT, and a "red" scalar holding 10% of the density plus a 1e-7
+    perturbation a quarter wave apart, evolved to t = 1 through the staged
+    plugin path; the L1 error norm over density, velocity, total energy,
+    bfield and red is the reference's 6.918011605252798e-08 (rtol 1e-13,
+    atol 7e-14, input/vlct/testing_utils.py:275-292)."""
+    ev = _ev()
+    print("passive sound wave axis", axis, flush=True)
+    g = 3
+    n = [4, 4, 4]
+    n[axis] = 16
+    upper = [0.25, 0.25, 0.25]
+    upper[axis] = 1.0
+    cw = tuple(upper[a] / n[a] for a in range(3))
+    method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm", theta=2.0,
+                           courant=0.4)
+    dg = ev.DeviceGrid(method, (1, 1, 1), tuple(n), [cw], bc=("periodic",) * 3)
+    m = [n[a] + 2 * g for a in range(3)]
+    # the "value" initializer at every cell centre, ghosts included
+    c = [(np.arange(m[a]) - g + 0.5) * cw[a] for a in range(3)]
+    z, y, x = np.meshgrid(c[2], c[1], c[0], indexing="ij")
+    s = (x, y, z)[axis]
+    sn = np.sin(2.0 * np.pi * s)
+    rho = 1.0 + 1.e-6 * sn
+    state = {"density": rho, "total_energy": 1.5 * (0.6 + 1.e-6 * sn) / (1.0 + 1.e-6 * sn)}
+    for a, name in enumerate(("velocity_x", "velocity_y", "velocity_z")):
+        state[name] = -1.e-6 * sn if a == axis else np.zeros_like(rho)
+    for name in ("bfield_x", "bfield_y", "bfield_z"):
+        state[name] = np.zeros_like(rho)
+    state["bfieldi_x"] = np.zeros((m[2], m[1], m[0] + 1))
+    state["bfieldi_y"] = np.zeros((m[2], m[1] + 1, m[0]))
+    state["bfieldi_z"] = np.zeros((m[2] + 1, m[1], m[0]))
+    red = (0.1 + 1.e-7 * np.sin(2.0 * np.pi * (s - 0.25))) * (1.0 + 1.e-6 * sn)
+    dg.upload_from([state])
+    red_buf = _dev(red)
+    hm = ev.HostMethod("mhd_choice = constrained_transport; riemann_solver = hlld; "
+                       "reconstruct_method = plm; theta_limiter = 2.0; courant = 0.4; "
+                       "device_fused = false")
+    names = ["density", "velocity_x", "velocity_y", "velocity_z", "total_energy",
+             "bfield_x", "bfield_y", "bfield_z"]
+    act = tuple(slice(g, g + n[a]) for a in (2, 1, 0))
+    init = [state[k][act].copy() for k in names] + [red[act].copy()]
+    t, t_stop, cycles = 0.0, 1.0, 0
+    while t < t_stop and cycles < 10000:
+        # Cello's cycle: dt from the current state, then refresh, then compute
+        dt = hm.timestep(dg.blocks[0])
+        if t + dt > t_stop:
+            dt = t_stop - t
+        dg.refresh()
+        r = ev.from_device(red_buf.ptr.value, red.shape)
+        r = np.pad(r[act], g, mode="wrap")        # periodic ghost zones of red
+        ev.to_device(red_buf.ptr.value, np.ascontiguousarray(r))
+        hm.compute_passive(dg.blocks[0], {"red": red_buf.ptr.value}, dt)
+        t += dt
+        cycles += 1
+    ev.synchronize()
+    fin = dg.blocks[0].download(names)
+    final = [fin[k][act] for k in names] + [ev.from_device(red_buf.ptr.value, red.shape)[act]]
+    l1 = po.l1_norm(init, final)
+    print("cycles", cycles, "L1", repr(l1))
+    assert np.isclose(l1, 6.918011605252798e-08, rtol=1e-13, atol=7e-14)
