@@ -6,9 +6,10 @@ on the GPU blocks it fills.
   initial state bit for bit, for all eight wave types, on the reference
   decks' domain [0,3]x[0,1.5]^2 split over 2x2x1 blocks (the blocks' lower
   corners enter the positions), both wave directions.
-* Evolved on the GPU from the product's own initial state, the N16 fast
-  wave reproduces the reference's golden L1 error
-  (input/vlct/run_MHD_linear_wave_test.py:178-188) at rtol 1e-13.
+* Evolved on the GPU from the product's own initial state, every MHD and
+  HD linear wave of the reference's tests (N16 and N32) reproduces its
+  golden L1 error at rtol 1e-13 / atol 7e-14, and the left- and
+  right-propagating N32 sound waves agree to 6 decimal places.
 """
 import numpy as np
 import pytest
@@ -68,24 +69,73 @@ def test_inclined_wave_initial_state_matches_the_oracle(wave, mhd, positive):
             assert np.array_equal(a, ref[name]), (wave, positive, i, name)
 
 
-def test_fast_wave_golden_through_the_product_initializer():
+def _product_l1(wave, res, t_stop, mhd, nblocks, positive=True):
+    """The reference's linear-wave L1 error computed entirely through the
+    product: the C++ initializer, the GPU cycles, the L1 over the gathered
+    fields (the oracle grid only lends its gather)."""
     ev = _ev()
     h = ev.host_lib()
-    grid = po.linear_wave_grid("fast", 16)
+    grid = po.linear_wave_grid(wave, res, mhd=mhd, nblocks=nblocks, positive=positive)
     h.enzoe_host_set_fluid_props(grid.cfg.gamma, grid.cfg.density_floor,
                                  grid.cfg.pressure_floor)
-    fields = po.MHD_L1_FIELDS
-    init = [grid.gather(f) for f in fields]
+    fields = po.MHD_L1_FIELDS if mhd else po.HD_L1_FIELDS
     method = method_for(grid.cfg)
-    dg = _product_grid(grid, "fast", True, method)
+    dg = _product_grid(grid, wave, positive, method)
     ev.synchronize()
-    dg.evolve(0.0, 0.5)
-    arrs = dg.blocks[0].download()
-    view = grid.block_arrays(0)
-    for k, v in arrs.items():
-        if k in view:
-            view[k][...] = v
-    final = [grid.gather(f) for f in fields]
-    l1 = po.l1_norm(init, final, res=16)
-    print("L1", repr(l1))
-    assert np.isclose(l1, 1.6388526155394664e-07, rtol=1e-13, atol=7e-14)
+    nb = nblocks[0] * nblocks[1] * nblocks[2]
+
+    def pull():
+        for i in range(nb):
+            arrs = dg.blocks[i].download()
+            view = grid.block_arrays(i)
+            for k, v in arrs.items():
+                if k in view:
+                    view[k][...] = v
+        return [grid.gather(f) for f in fields]
+    init = pull()
+    ncycles, _ = dg.evolve(0.0, t_stop)
+    final = pull()
+    return po.l1_norm(init, final, res=res), ncycles
+
+
+# the reference's golden L1 errors (input/vlct/run_MHD_linear_wave_test.py:
+# 178-188, input/vlct/run_HD_linear_wave_test.py:70-80; the HD decks split
+# the grid over 2x2x2 blocks)
+GOLDEN = [
+    ("fast", 16, 0.5, True, 1.6388526155394664e-07),
+    ("fast", 32, 0.5, True, 3.302538226654406e-08),
+    ("alfven", 16, 1.0, True, 1.927245356389947e-07),
+    ("alfven", 32, 1.0, True, 3.005870212811956e-08),
+    ("slow", 16, 2.0, True, 2.2373810027584788e-07),
+    ("slow", 32, 2.0, True, 4.43702e-08),
+    ("mhd_entropy", 16, 1.0, True, 1.0021263485338544e-07),
+    ("mhd_entropy", 32, 1.0, True, 2.9194839706868883e-08),
+    ("sound", 16, 1.0, False, 1.3704437791196907e-07),
+    ("sound", 32, 1.0, False, 2.3484946798755385e-08),
+    ("hd_entropy", 16, 1.0, False, 8.736217559091042e-08),
+    ("hd_entropy", 32, 1.0, False, 2.2383944940640457e-08),
+    ("hd_transv_entropy_v1", 16, 1.0, False, 7.671004076321944e-08),
+    ("hd_transv_entropy_v1", 32, 1.0, False, 1.6725791096617187e-08),
+    ("hd_transv_entropy_v2", 16, 1.0, False, 8.640730176761958e-08),
+    ("hd_transv_entropy_v2", 32, 1.0, False, 1.6960029746447077e-08),
+]
+
+
+@pytest.mark.parametrize("wave,res,t_stop,mhd,ref", GOLDEN)
+def test_linear_wave_golden_through_the_product(wave, res, t_stop, mhd, ref):
+    print("golden", wave, res, flush=True)
+    l1, ncycles = _product_l1(wave, res, t_stop, mhd, (1, 1, 1) if mhd else (2, 2, 2))
+    print("cycles", ncycles, "L1", repr(l1), "ref", ref)
+    assert ncycles > 10
+    # the reference's tolerance (input/vlct/testing_utils.py:275-292)
+    assert np.isclose(l1, ref, rtol=1e-13, atol=7e-14)
+
+
+def test_sound_waves_left_and_right_have_the_same_l1():
+    """identical_l1_error_linwave(1, 'sound', 32, ..., prec=6)
+    (input/vlct/run_HD_linear_wave_test.py:82-84): the left- and the
+    right-propagating N32 sound waves agree to 6 decimal places."""
+    l1_r, _ = _product_l1("sound", 32, 1.0, False, (2, 2, 2), positive=True)
+    l1_l, _ = _product_l1("sound", 32, 1.0, False, (2, 2, 2), positive=False)
+    print("right", repr(l1_r), "left", repr(l1_l))
+    assert ("%.6e" % l1_r) == ("%.6e" % l1_l)
