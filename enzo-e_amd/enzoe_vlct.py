@@ -729,6 +729,8 @@ def host_lib():
         H.enzoe_host_amr_destroy.argtypes = [vp]
         H.enzoe_host_initial_inclined_wave.argtypes = [ctypes.c_char_p, ctypes.POINTER(Block),
                                                        _dp]
+        H.enzoe_host_initial_shock_tube.argtypes = [ctypes.c_char_p, ctypes.POINTER(Block),
+                                                    _dp, _dp]
         H.enzoe_host_prolong_apply.argtypes = [ctypes.c_char_p, ctypes.c_int, vp, PI, PI, PI,
                                                vp, PI, PI, PI, ctypes.c_int, vp]
         H.enzoe_host_restrict_apply.argtypes = [ctypes.c_char_p, vp, PI, PI, PI, vp, PI, PI,
@@ -838,3 +840,13 @@ def initial_inclined_wave(parameters, block, lower):
     check(host_lib().enzoe_host_initial_inclined_wave(parameters.encode(),
                                                       ctypes.byref(block.desc), lo),
           "enzoe_host_initial_inclined_wave")
+
+
+def initial_shock_tube(parameters, block, lower, upper):
+    """EnzoInitialShockTube of the C++ layer on a DeviceBlock whose active
+    region is [lower, upper)."""
+    lo = (ctypes.c_double * 3)(*lower)
+    hi = (ctypes.c_double * 3)(*upper)
+    check(host_lib().enzoe_host_initial_shock_tube(parameters.encode(),
+                                                   ctypes.byref(block.desc), lo, hi),
+          "enzoe_host_initial_shock_tube")

@@ -684,6 +684,25 @@ class EnzoInitialInclinedWave {
   std::string wave_type_;
 };
 
+// EnzoInitialShockTube (src/Enzo/initial/EnzoInitialShockTube.cpp): the
+// "rj2a" (Ryu & Jones 1995, fig. 2a) and "sod" tubes along aligned_ax
+// ("x", "y", "z"), with axis_velocity, transverse_velocity and
+// flip_initialize.  Host values copied into the block's device fields.
+class EnzoInitialShockTube {
+ public:
+  explicit EnzoInitialShockTube(ParameterGroup p);
+  // enforce_block (:183-292) for a block whose active region is
+  // [lower, upper)
+  void enforce_block(Block *block, const std::array<double, 3> &lower,
+                     const std::array<double, 3> &upper) const;
+
+ private:
+  std::string setup_name_;
+  int aligned_ax_ = 0;
+  double axis_velocity_ = 0.0, trans_velocity_ = 0.0;
+  bool flipped_initialize_ = false;
+};
+
 // ------------------------------------------------ coarse/fine transfers
 
 // Cello's precision_enum (src/Cello/cello.hpp:152-163)

@@ -160,6 +160,16 @@ int enzoe_host_initial_inclined_wave(const char *parameters, const enzoe_block *
   return ENZOE_OK;
 }
 
+int enzoe_host_initial_shock_tube(const char *parameters, const enzoe_block *block,
+                                  const double lower[3], const double upper[3]) {
+  if (!block || !lower || !upper) return ENZOE_ERR_ARG;
+  EnzoInitialShockTube init(
+      ParameterGroup(parameters ? parameters : "", "Initial:shock_tube"));
+  Block b(*block);
+  init.enforce_block(&b, {lower[0], lower[1], lower[2]}, {upper[0], upper[1], upper[2]});
+  return ENZOE_OK;
+}
+
 int enzoe_host_prolong_apply(const char *name, int use_linear, double *values_f,
                              const int m3_f[3], const int o3_f[3], const int n3_f[3],
                              const double *values_c, const int m3_c[3], const int o3_c[3],

@@ -309,3 +309,173 @@ void EnzoInitialInclinedWave::enforce_block(Block *block,
   to_device(block->field_view("velocity_z").data(), vz);
   to_device(block->field_view("total_energy").data(), etot);
 }
+
+// ------------------------------------------------------------ shock tubes
+
+namespace {
+
+// left / right states (EnzoInitialShockTube.cpp:36-86): density, pressure,
+// velocity_0..2 (0 along the tube), bfield_1, bfield_2 (transverse)
+struct TubeState {
+  double density, pressure, v[3], b1, b2;
+};
+
+bool tube_states(const std::string &name, TubeState &l, TubeState &r, double &b0) {
+  if (name == "rj2a") {
+    l = {1.08, 0.95, {1.2, 0.01, 0.5}, 1.0155412503859613, 0.5641895835477563};
+    r = {1., 1.0, {0.0, 0.0, 0.0}, 1.1283791670955126, 0.5641895835477563};
+    b0 = 0.5641895835477563;
+    return true;
+  }
+  if (name == "sod") {
+    l = {1.0, 1.0, {0., 0., 0.}, 0., 0.};
+    r = {0.125, 0.1, {0., 0., 0.}, 0., 0.};
+    b0 = 0.0;
+    return true;
+  }
+  return false;
+}
+
+// setup_maps_ (:161-176): vectors change sign when the tube is flipped
+TubeState flipped(const TubeState &s) {
+  TubeState o = s;
+  for (int a = 0; a < 3; a++) o.v[a] = -1. * s.v[a];
+  o.b1 = -1. * s.b1;
+  o.b2 = -1. * s.b2;
+  return o;
+}
+
+}  // namespace
+
+EnzoInitialShockTube::EnzoInitialShockTube(ParameterGroup p)
+    : setup_name_(p.value_string("setup_name", "")),
+      axis_velocity_(p.value_float("axis_velocity", 0.0)),
+      trans_velocity_(p.value_float("transverse_velocity", 0.0)) {
+  TubeState l, r;
+  double b0 = 0.0;
+  // :107-122
+  if (!tube_states(setup_name_, l, r, b0))
+    ENZOE_ERROR("EnzoInitialShockTube", p.full_name("setup_name") +
+                                            " specifies an invalid name: (must be rj2a, sod), "
+                                            "not " + setup_name_ + ".");
+  const std::string ax = p.value_string("aligned_ax", "x");
+  if (ax == "x")
+    aligned_ax_ = 0;
+  else if (ax == "y")
+    aligned_ax_ = 1;
+  else if (ax == "z")
+    aligned_ax_ = 2;
+  else
+    ENZOE_ERROR("EnzoInitialShockTube", p.full_name("aligned_ax") +
+                                            " must specify \"x\", \"y\", or \"z\", not \"" +
+                                            ax + "\"");
+  const std::string flip = p.value_string("flip_initialize", "false");
+  ENZOE_ASSERT("EnzoInitialShockTube", p.full_name("flip_initialize") + " must be true or false",
+               flip == "true" || flip == "false");
+  flipped_initialize_ = (flip == "true");
+}
+
+void EnzoInitialShockTube::enforce_block(Block *block, const std::array<double, 3> &lower,
+                                         const std::array<double, 3> &upper) const {
+  TubeState l0, r0;
+  double b0 = 0.0;
+  tube_states(setup_name_, l0, r0, b0);
+  const TubeState L = flipped_initialize_ ? flipped(r0) : l0;
+  const TubeState R = flipped_initialize_ ? flipped(l0) : r0;
+  const double gamma = enzo::fluid_props()->gamma();
+  const double flip = flipped_initialize_ ? -1. : 1.;
+  const double aligned_bfield_val = flip * b0;
+  const double axis_velocity = flip * axis_velocity_;
+  const double trans_velocity = flip * trans_velocity_;
+  const bool mhd = block->has_field("bfieldi_x");
+  ENZOE_ASSERT("EnzoInitialShockTube::enforce_block",
+               "the \"" + setup_name_ + "\" tube needs the magnetic fields",
+               mhd || (b0 == 0.0 && L.b1 == 0.0 && L.b2 == 0.0 && R.b1 == 0.0 &&
+                       R.b2 == 0.0));
+
+  const EFlt3DArray rho_v = block->field_view("density");
+  const int m[3] = {rho_v.shape(2), rho_v.shape(1), rho_v.shape(0)};
+  const int g = block->ghost_depth();
+  const int ai = aligned_ax_, aj = (ai + 1) % 3, ak = (ai + 2) % 3;
+  // prep_aligned_slices_ (:296-335): cells [0, shock_ind) left, the rest right
+  const int n_ai = m[ai] - 2 * g;
+  const double di = (upper[ai] - lower[ai]) / n_ai;
+  int shock_ind = (int)std::ceil((0.5 - lower[ai]) / di - 0.5 + (double)g);
+  shock_ind = std::min(std::max(shock_ind, 0), m[ai]);
+
+  const char *vel[3] = {"velocity_x", "velocity_y", "velocity_z"};
+  const char *bfi[3] = {"bfieldi_x", "bfieldi_y", "bfieldi_z"};
+  // host copies of the fields this initializer writes
+  auto download = [&](const char *name) {
+    const EFlt3DArray a = block->field_view(name);
+    std::vector<double> h((std::size_t)a.shape(0) * a.shape(1) * a.shape(2));
+    if (hipMemcpy(h.data(), a.data(), h.size() * sizeof(double), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      ENZOE_ERROR("EnzoInitialShockTube::enforce_block", "hipMemcpy failed");
+    return h;
+  };
+  // initializer_helper_ (:338-350): the slice along the tube, every index
+  // across it (face-centred extents included)
+  auto fill = [&](const char *name, int s0, int s1, double val, std::vector<double> &h) {
+    const EFlt3DArray a = block->field_view(name);
+    const int ext[3] = {a.shape(2), a.shape(1), a.shape(0)};
+    for (int k = 0; k < ext[2]; k++)
+      for (int j = 0; j < ext[1]; j++)
+        for (int i = 0; i < ext[0]; i++) {
+          const int p[3] = {i, j, k};
+          if (p[ai] < s0 || p[ai] >= s1) continue;
+          h[((std::size_t)k * ext[1] + j) * ext[0] + i] = val;
+        }
+  };
+  std::vector<double> rho = download("density"), etot = download("total_energy");
+  std::vector<double> v[3] = {download(vel[0]), download(vel[1]), download(vel[2])};
+  std::vector<double> bf[3];
+  if (mhd)
+    for (int a = 0; a < 3; a++) bf[a] = download(bfi[a]);
+  for (int side = 0; side < 2; side++) {
+    const TubeState &S = side ? R : L;
+    const int s0 = side ? shock_ind : 0, s1 = side ? m[ai] : shock_ind;
+    if (s0 >= s1) continue;
+    const double velocity_0 = S.v[0] + axis_velocity;
+    const double velocity_1 = S.v[1] + trans_velocity;
+    const double velocity_2 = S.v[2];
+    fill("density", s0, s1, S.density, rho);
+    fill(vel[ai], s0, s1, velocity_0, v[ai]);
+    fill(vel[aj], s0, s1, velocity_1, v[aj]);
+    fill(vel[ak], s0, s1, velocity_2, v[ak]);
+    if (mhd) {
+      fill(bfi[aj], s0, s1, S.b1, bf[aj]);
+      fill(bfi[ak], s0, s1, S.b2, bf[ak]);
+    }
+    const double eint = (S.pressure / ((gamma - 1.) * S.density));
+    const double v2 = (velocity_0 * velocity_0 + velocity_1 * velocity_1 +
+                       velocity_2 * velocity_2);
+    const double b2 = (aligned_bfield_val * aligned_bfield_val + S.b1 * S.b1 + S.b2 * S.b2);
+    fill("total_energy", s0, s1, (eint + 0.5 * (v2 + b2 / S.density)), etot);
+  }
+  to_device(block->field_view("density").data(), rho);
+  to_device(block->field_view("total_energy").data(), etot);
+  for (int a = 0; a < 3; a++) to_device(block->field_view(vel[a]).data(), v[a]);
+  if (mhd) {
+    // the aligned face field is uniform (:279-280)
+    std::fill(bf[ai].begin(), bf[ai].end(), aligned_bfield_val);
+    for (int a = 0; a < 3; a++) to_device(block->field_view(bfi[a]).data(), bf[a]);
+    // EnzoInitialBCenter::initialize_bfield_center: cell-centred B = face average
+    const int mx = m[0], my = m[1], mz = m[2];
+    std::vector<double> c[3];
+    for (int a = 0; a < 3; a++) c[a].assign((std::size_t)mx * my * mz, 0.0);
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          const std::size_t q = ((std::size_t)k * my + j) * mx + i;
+          c[0][q] = 0.5 * (bf[0][((std::size_t)k * my + j) * (mx + 1) + i] +
+                           bf[0][((std::size_t)k * my + j) * (mx + 1) + i + 1]);
+          c[1][q] = 0.5 * (bf[1][((std::size_t)k * (my + 1) + j) * mx + i] +
+                           bf[1][((std::size_t)k * (my + 1) + j + 1) * mx + i]);
+          c[2][q] = 0.5 * (bf[2][((std::size_t)k * my + j) * mx + i] +
+                           bf[2][((std::size_t)(k + 1) * my + j) * mx + i]);
+        }
+    const char *bc[3] = {"bfield_x", "bfield_y", "bfield_z"};
+    for (int a = 0; a < 3; a++) to_device(block->field_view(bc[a]).data(), c[a]);
+  }
+}

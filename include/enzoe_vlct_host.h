@@ -96,6 +96,14 @@ int enzoe_host_initial_inclined_wave(const char *parameters,
                                      const enzoe_block *block,
                                      const double lower[3]);
 
+/* EnzoInitialShockTube(ParameterGroup) + enforce_block
+ * (src/Enzo/initial/EnzoInitialShockTube.cpp:97-350) on one block whose
+ * active region is [lower, upper): setup_name ("rj2a", "sod"), aligned_ax
+ * ("x", "y", "z"), axis_velocity, transverse_velocity, flip_initialize.    */
+int enzoe_host_initial_shock_tube(const char *parameters,
+                                  const enzoe_block *block,
+                                  const double lower[3], const double upper[3]);
+
 /* Prolong::apply of the C++ layer: name "linear" (ProlongLinear,
  * src/Cello/problem_ProlongLinear.cpp:25-370) or "enzo" (EnzoProlong "3A",
  * src/Enzo/mesh/EnzoProlong.cpp:49-97; only use_linear = 1 is provided: the

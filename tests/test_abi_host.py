@@ -370,3 +370,24 @@ def test_host_initial_inclined_wave_errors_exit_like_the_reference(params, messa
     r = _run_host_initial(params)
     assert r.returncode == 1, (r.stdout, r.stderr)
     assert "ERROR in" in r.stderr and message in r.stderr, r.stderr
+
+
+def _run_host_tube(params):
+    code = ("import sys, ctypes; sys.path.insert(0, %r); import enzoe_vlct as ev\n"
+            "ev.host_lib().enzoe_host_set_fluid_props(5.0 / 3.0, 1e-30, 1e-30)\n"
+            "b = ev.Block(); lo = (ctypes.c_double * 3)(0, 0, 0)\n"
+            "print(ev.host_lib().enzoe_host_initial_shock_tube(%r, ctypes.byref(b), lo, lo))"
+            % (os.path.join(ROOT, "enzo-e_amd"), params.encode()))
+    return subprocess.run([sys.executable, "-c", code], capture_output=True,
+                          text=True, timeout=120)
+
+
+@pytest.mark.parametrize("params,message", [
+    ("setup_name = brio_wu", "specifies an invalid name: (must be rj2a, sod), not brio_wu."),
+    ("setup_name = rj2a; aligned_ax = w", "must specify \"x\", \"y\", or \"z\", not \"w\""),
+    ("setup_name = rj2a", "the \"rj2a\" tube needs the magnetic fields"),
+])
+def test_host_shock_tube_errors_exit_like_the_reference(params, message):
+    r = _run_host_tube(params)
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert "ERROR in" in r.stderr and message in r.stderr, r.stderr

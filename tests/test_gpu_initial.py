@@ -139,3 +139,80 @@ def test_sound_waves_left_and_right_have_the_same_l1():
     l1_l, _ = _product_l1("sound", 32, 1.0, False, (2, 2, 2), positive=False)
     print("right", repr(l1_r), "left", repr(l1_l))
     assert ("%.6e" % l1_r) == ("%.6e" % l1_l)
+
+
+def _tube_grid(axis, n=256, ntrans=4, params=None, method=None):
+    """The product's shock tube on the grid of po.rj2a_grid (one block)."""
+    ev = _ev()
+    grid = po.rj2a_grid(axis, n=n, ntrans=ntrans)
+    ev.host_lib().enzoe_host_set_fluid_props(grid.cfg.gamma, grid.cfg.density_floor,
+                                             grid.cfg.pressure_floor)
+    method = method or method_for(grid.cfg)
+    bc = ["periodic"] * 3
+    bc[axis] = "outflow"
+    dg = ev.DeviceGrid(method, (1, 1, 1), grid.ncells, [tuple(grid.block(0).cw)], bc=tuple(bc))
+    params = params or ("setup_name = rj2a; aligned_ax = %s" % "xyz"[axis])
+    ev.initial_shock_tube(params, dg.blocks[0], (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    return grid, dg
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_rj2a_initial_state_matches_the_oracle(axis):
+    ev = _ev()
+    grid, dg = _tube_grid(axis, n=64)
+    ev.synchronize()
+    got = dg.blocks[0].download()
+    ref = grid.block_arrays(0)
+    for name, a in got.items():
+        if name != "pressure":
+            assert np.array_equal(a, ref[name]), (axis, name)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_rj2a_golden_through_the_product_initializer(axis):
+    """input/vlct/run_MHD_shock_tube_test.py:82-87 with the reference's table
+    (tests/golden/rj2a_shock_tube_t0.2_res256.csv), every axis."""
+    import os
+    ev = _ev()
+    print("rj2a axis", axis, flush=True)
+    table = po.load_reference_table(os.path.join(os.path.dirname(__file__), "golden",
+                                                 "rj2a_shock_tube_t0.2_res256.csv"))
+    grid, dg = _tube_grid(axis)
+    dg.evolve(0.0, 0.2)
+    arrs = dg.blocks[0].download()
+    view = grid.block_arrays(0)
+    for k, v in arrs.items():
+        if k in view:
+            view[k][...] = v
+    l1 = po.table_l1(grid, table, axis)
+    ref = {0: 0.012523489882320429, 1: 0.012523489882320308, 2: 0.012523489882320315}[axis]
+    print("L1", repr(l1), "ref", ref)
+    assert np.isclose(l1, ref, rtol=1e-13, atol=7e-14)
+
+
+def test_sod_flip_and_velocities_closed_form():
+    """"sod" on a hydro block, flipped, with axis and transverse velocities:
+    the two uniform states of EnzoInitialShockTube.cpp:183-292 split at
+    x = 0.5 (cell centres below 0.5 hold the left state)."""
+    ev = _ev()
+    ev.host_lib().enzoe_host_set_fluid_props(1.4, 1e-30, 1e-30)
+    n, g = 16, 3
+    m = n + 2 * g
+    db = ev.DeviceBlock(m, m, m, g, (1.0 / n,) * 3, mhd=False)
+    ev.initial_shock_tube("setup_name = sod; aligned_ax = y; flip_initialize = true; "
+                          "axis_velocity = 0.25; transverse_velocity = -0.5",
+                          db, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    ev.synchronize()
+    f = db.download()
+    yc = (np.arange(m) - g + 0.5) / n
+    left = (yc < 0.5)[None, :, None] * np.ones((m, m, m), bool)
+    # flipped: the left state is sod's right one, vectors reversed
+    assert np.all(f["density"][left] == 0.125) and np.all(f["density"][~left] == 1.0)
+    # velocity_0 (along y) = flip * (v0 + axis_velocity); velocity_1 (along z)
+    assert np.all(f["velocity_y"] == -0.25)
+    assert np.all(f["velocity_z"] == 0.5)
+    assert np.all(f["velocity_x"] == 0.0)
+    e_left = 0.1 / (0.4 * 0.125) + 0.5 * (0.25 ** 2 + 0.5 ** 2)
+    e_right = 1.0 / (0.4 * 1.0) + 0.5 * (0.25 ** 2 + 0.5 ** 2)
+    np.testing.assert_allclose(f["total_energy"][left], e_left, rtol=1e-15)
+    np.testing.assert_allclose(f["total_energy"][~left], e_right, rtol=1e-15)
