@@ -167,7 +167,8 @@ namespace {
 // blocks grouped into runs of equal shape, each run one compute_group
 int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                  double dt, const double *d_dts, hipStream_t s,
-                 const StoreArgs *st = nullptr) {
+                 const StoreArgs *st = nullptr,
+                 const double *const *accel = nullptr) {
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
   int b0 = 0;
@@ -183,12 +184,14 @@ int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       run_st.d_fdtab = st->d_fdtab + b0;
     }
     const StoreArgs *rs = st ? &run_st : nullptr;
+    const double *const *ra = accel ? accel + 3 * b0 : nullptr;
     int rc;
     if (m->precision == ENZOE_PREC_DOUBLE) {
-      rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s, rs)
-                  : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, dts, s, rs);
+      rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra)
+                  : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, dts, s, rs,
+                                                 ra);
     } else {
-      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, dts, s, rs);
+      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra);
     }
     if (rc) return rc;
     b0 = b1;
@@ -222,6 +225,16 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     d_dts = (const double *)m->d_dts;
   }
   return compute_runs(m, blocks, n, 0.0, d_dts, s);
+}
+
+int enzoe_vlct_compute_gravity(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                               double dt, double *const *accel,
+                               enzoe_stream_t stream) {
+  if (!m || !blocks || n <= 0 || !accel) return ENZOE_ERR_ARG;
+  for (int i = 0; i < 3 * n; i++)
+    if (!accel[i]) return ENZOE_ERR_ARG;
+  return compute_runs(m, blocks, n, dt, nullptr, (hipStream_t)stream, nullptr,
+                      (const double *const *)accel);
 }
 
 int enzoe_vlct_compute_store_fluxes(enzoe_vlct_t m, const enzoe_block *blocks,

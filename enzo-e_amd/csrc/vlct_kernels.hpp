@@ -36,6 +36,7 @@ struct BlockDesc {
   FluxT *F[3][S_N];    // fluxes along x, y, z; slot = field
   double *E[3];        // edge EMFs along x, y, z
   double *Ec[3];       // cell-centred E = -v x B of the stage's state
+  const double *acc[3];  // acceleration_x..z for the gravity source, or null
   int mx, my, mz;
   int g;               // ghost depth
   double cw[3];
@@ -451,6 +452,19 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
       double fr = (double)F[c], fl = (double)F[c - str[d]];
       dU[q] -= dtdx * (fr - fl);
     }
+  }
+  // gravity (EnzoSourceGravity::calculate_source, EnzoSourceGravity.cpp:
+  // 17-69) on the stage with index 1, from the start-of-step state, after
+  // the flux divergence (EnzoMHDIntegratorStageCommands.cpp:180-182,
+  // 265-295)
+  if (sp.stage == 1 && b.acc[0] != nullptr) {
+    const double rho = b.U[S_RHO][c];
+    const double ax = b.acc[0][c], ay = b.acc[1][c], az = b.acc[2][c];
+    dU[1] += dt * rho * ax;
+    dU[2] += dt * rho * ay;
+    dU[3] += dt * rho * az;
+    dU[4] += dt * rho * ((b.U[S_VX][c] * ax) + (b.U[S_VY][c] * ay) +
+                         (b.U[S_VZ][c] * az));
   }
   // the update base is always the start-of-step state
   double old_rho = b.U[S_RHO][c];

@@ -188,9 +188,11 @@ inline int check_block(const enzoe_vlct_method *m, const enzoe_block &b) {
 }
 
 // Fills descriptors for blocks [b0, b1) using the scratch pool.
+// accel: 3 acceleration pointers per block of the call (x, y, z), or null
 template <typename FluxT>
 void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
-               int b1, std::vector<BlockDesc<FluxT>> &out) {
+               int b1, std::vector<BlockDesc<FluxT>> &out,
+               const double *const *accel = nullptr) {
   out.clear();
   char *base = (char *)m->scratch;
   for (int bi = b0; bi < b1; bi++) {
@@ -208,6 +210,8 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
     d.pressure = b.pressure;
     d.mx = b.mx; d.my = b.my; d.mz = b.mz;
     d.g = b.ghost;
+    if (accel)
+      for (int a = 0; a < 3; a++) d.acc[a] = accel[3 * bi + a];
     for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
     const size_t ncc = (size_t)b.mx * b.my * b.mz;
     const int nq = m->mhd ? 8 : 5;
@@ -302,7 +306,8 @@ struct StoreArgs {
 template <typename FluxT, bool MHD>
 int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   double dt, const double *dts, hipStream_t s,
-                  const StoreArgs *st = nullptr) {
+                  const StoreArgs *st = nullptr,
+                  const double *const *accel = nullptr) {
   const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
   const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd);
   // chunk size: as many blocks as fit the scratch budget (at least 1); the
@@ -321,7 +326,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
   // descriptor table of every chunk, uploaded once (and only when changed)
   std::vector<BlockDesc<FluxT>> desc, part;
   for (int c0 = 0; c0 < n; c0 += chunk) {
-    fill_desc<FluxT>(m, blocks, c0, std::min(n, c0 + chunk), part);
+    fill_desc<FluxT>(m, blocks, c0, std::min(n, c0 + chunk), part, accel);
     desc.insert(desc.end(), part.begin(), part.end());
   }
   {
@@ -412,6 +417,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
 // FP32 flux path, compiled in vlct_mixed.hip
 int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   double dt, const double *dts, hipStream_t s,
-                  const StoreArgs *st = nullptr);
+                  const StoreArgs *st = nullptr,
+                  const double *const *accel = nullptr);
 
 }  // namespace enzoe

@@ -1078,6 +1078,23 @@ __global__ void k_op_flux_correct_batch(
                     b.flux_data, L, g, mx, my, t);
 }
 
+// ------------------------------------------------------- gravity source
+
+// EnzoSourceGravity::calculate_source (toolkit/EnzoSourceGravity.cpp:17-69)
+// over the box: momentum += dt rho a, energy density += dt rho (v . a)
+__global__ void k_op_gravity(KMap U, KMap dU, KMap A, Box box, double dt) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long cu = off(U.s, p), cd = off(dU.s, p), ca = off(A.s, p);
+  const double rho = U.p[ENZOE_SLOT_DENSITY][cu];
+  const double ax = A.p[1][ca], ay = A.p[2][ca], az = A.p[3][ca];
+  dU.p[1][cd] += dt * rho * ax;
+  dU.p[2][cd] += dt * rho * ay;
+  dU.p[3][cd] += dt * rho * az;
+  dU.p[ENZOE_SLOT_ENERGY][cd] +=
+      dt * rho * ((U.p[1][cu] * ax) + (U.p[2][cu] * ay) + (U.p[3][cu] * az));
+}
+
 // ------------------------------------------------ coarse/fine transfers
 
 struct Int3 {
@@ -1374,6 +1391,27 @@ int enzoe_vlct_flux_correct_blocks(enzoe_vlct_t m,
                        (const enzoe_flux_correct_block *)m->d_fbtab, sh, L, ghost,
                        nx + 2 * ghost, ny + 2 * ghost);
   }
+  return launched();
+}
+
+int enzoe_source_gravity(double dt, const enzoe_fmap *integration,
+                         const enzoe_fmap *dUcons, const enzoe_fmap *accel,
+                         int stale_depth, enzoe_stream_t stream) {
+  KMap u, d, a;
+  if (stale_depth < 0 || !to_kmap(integration, 0xFu, u) || !to_kmap(dUcons, 0x1Eu, d) ||
+      !to_kmap(accel, 0xEu, a))
+    return ENZOE_ERR_ARG;
+  if (!same_shape(u.n, d.n) || !same_shape(u.n, a.n)) return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  for (int k = 0; k < 3; k++) {
+    lo[k] = stale_depth;
+    hi[k] = u.n[k] - stale_depth;
+  }
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_gravity, dim3(nblk(count)), dim3(256), 0, (hipStream_t)stream,
+                     u, d, a, box, dt);
   return launched();
 }
 

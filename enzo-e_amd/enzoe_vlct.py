@@ -518,6 +518,9 @@ def ops():
         L.enzoe_prolong_linear.argtypes = [vp, PI, PI, PI, vp, PI, PI, PI, i, vp]
         L.enzoe_vlct_transfer_batch.argtypes = [vp, ctypes.POINTER(Transfer), i, vp]
         L.enzoe_restrict_linear.argtypes = [vp, PI, PI, PI, vp, PI, PI, PI, i, vp]
+        L.enzoe_vlct_compute_gravity.argtypes = [vp, ctypes.POINTER(Block), i, d,
+                                                 ctypes.POINTER(vp), vp]
+        L.enzoe_source_gravity.argtypes = [d, PM, PM, PM, i, vp]
         L.enzoe_vlct_flux_send_to_coarse_batch.argtypes = [vp, ctypes.POINTER(FluxLink), i,
                                                            i, i, i, i, vp]
         L.enzoe_vlct_flux_correct_blocks.argtypes = [vp, ctypes.POINTER(FluxCorrectBlock),
@@ -718,6 +721,9 @@ def host_lib():
             vp, ctypes.POINTER(Block), ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
             ctypes.POINTER(vp), ctypes.c_double, vp]
         H.enzoe_host_method_compute_passive.restype = ctypes.c_int
+        H.enzoe_host_method_compute_accel.argtypes = [vp, ctypes.POINTER(Block),
+                                                      ctypes.POINTER(vp), ctypes.c_double, vp]
+        H.enzoe_host_method_compute_accel.restype = ctypes.c_int
         H.enzoe_host_method_timestep.argtypes = [vp, ctypes.POINTER(Block), vp]
         H.enzoe_host_method_timestep.restype = ctypes.c_double
         PI = ctypes.POINTER(ctypes.c_int)
@@ -810,6 +816,14 @@ class HostMethod:
         if done != 1:
             raise EnzoeError("compute_done called %d times" % done)
 
+    def compute_accel(self, block, accel, dt, stream=None):
+        """compute() of one block carrying acceleration_x/y/z (device ptrs)."""
+        ptrs = (ctypes.c_void_p * 3)(*accel)
+        done = host_lib().enzoe_host_method_compute_accel(
+            self.handle, ctypes.byref(block.desc), ptrs, dt, stream)
+        if done != 1:
+            raise EnzoeError("compute_done called %d times" % done)
+
     def timestep(self, block, stream=None):
         return host_lib().enzoe_host_method_timestep(self.handle, ctypes.byref(block.desc),
                                                      stream)
@@ -850,3 +864,11 @@ def initial_shock_tube(parameters, block, lower, upper):
     check(host_lib().enzoe_host_initial_shock_tube(parameters.encode(),
                                                    ctypes.byref(block.desc), lo, hi),
           "enzoe_host_initial_shock_tube")
+
+
+def compute_gravity(method, blocks, dt, accel, stream=None):
+    """enzoe_vlct_compute_gravity: accel[i] = (ax, ay, az) device pointers."""
+    ptrs = (ctypes.c_void_p * (3 * len(blocks)))(*[p for a in accel for p in a])
+    check(ops().enzoe_vlct_compute_gravity(method.handle, method._blocks(blocks),
+                                           len(blocks), dt, ptrs, stream),
+          "enzoe_vlct_compute_gravity")

@@ -426,6 +426,13 @@ void Block::bind_(const enzoe_block &b) {
   put("bfieldi_z", b.bfieldi[2], 1, 0, 0);
 }
 
+void Block::add_field(const std::string &name, double *data) {
+  const EFlt3DArray &rho = field_view("density");
+  ENZOE_ASSERT("Block::add_field", "\"" + name + "\" is already a field",
+               fields_.count(name) == 0);
+  fields_[name] = EFlt3DArray(data, rho.shape(0), rho.shape(1), rho.shape(2));
+}
+
 void Block::add_passive_scalar(const std::string &name, double *data) {
   const EFlt3DArray &rho = field_view("density");
   ENZOE_ASSERT("Block::add_passive_scalar", "\"" + name + "\" is already a field",
@@ -1025,9 +1032,10 @@ void EnzoMHDIntegratorStageCommands::compute_update_stage(
     EnzoEFltArrayMap cur_stage_integration_map, EnzoEFltArrayMap out_integration_map,
     EnzoEFltArrayMap primitive_map, EnzoEFltArrayMap priml_map,
     EnzoEFltArrayMap primr_map, std::array<EnzoEFltArrayMap, 3> flux_maps_xyz,
-    EnzoEFltArrayMap dUcons_map, const str_vec_t &passive_list,
-    EnzoBfieldMethod *bfield_method, unsigned short stage_index, double cur_dt,
-    int stale_depth, const std::array<enzo_float, 3> cell_widths_xyz) const noexcept {
+    EnzoEFltArrayMap dUcons_map, const EnzoEFltArrayMap &accel_map,
+    const str_vec_t &passive_list, EnzoBfieldMethod *bfield_method,
+    unsigned short stage_index, double cur_dt, int stale_depth,
+    const std::array<enzo_float, 3> cell_widths_xyz) const noexcept {
   // EnzoMHDIntegratorStageCommands.cpp:102-203
   check_valid_stage_index_((int)stage_index);
   EnzoReconstructor *reconstructor = reconstructors_[stage_index].get();
@@ -1046,7 +1054,13 @@ void EnzoMHDIntegratorStageCommands::compute_update_stage(
                   stale_depth, passive_list);
   }
   stale_depth += reconstructor->immediate_staling_rate();
-  // source terms (gravity, dual-energy) are not provided: nothing to add
+  // compute_source_terms_ (EnzoMHDIntegratorStageCommands.cpp:265-295):
+  // gravity on the stage with index 1, from the start-of-step state
+  if ((stage_index == 1) && (accel_map.size() != 0)) {
+    EnzoSourceGravity gravity_source;
+    gravity_source.calculate_source(cur_dt, tstep_begin_integration_map, dUcons_map,
+                                    accel_map, stale_depth);
+  }
   if (bfield_method != nullptr)
     bfield_method->update_all_bfield_components(cur_stage_integration_map,
                                                 flux_maps_xyz[0], flux_maps_xyz[1],
@@ -1055,6 +1069,32 @@ void EnzoMHDIntegratorStageCommands::compute_update_stage(
   integration_quan_updater_->update_quantities(tstep_begin_integration_map, dUcons_map,
                                                out_integration_map, stale_depth,
                                                passive_list);
+}
+
+void EnzoSourceGravity::calculate_source(const double dt, const EnzoEFltArrayMap &prim_map,
+                                         EnzoEFltArrayMap &dUcons_map,
+                                         const EnzoEFltArrayMap &accel_map,
+                                         const int stale_depth) const noexcept {
+  const enzoe_fmap u = select_keys(prim_map, {"density", "velocity_x", "velocity_y",
+                                              "velocity_z"}).slot_map();
+  const enzoe_fmap d = select_keys(dUcons_map, {"velocity_x", "velocity_y", "velocity_z",
+                                                "total_energy"}).slot_map();
+  // acceleration_x..z in the velocity slots
+  enzoe_fmap a;
+  std::memset(&a, 0, sizeof(a));
+  const char *names[3] = {"acceleration_x", "acceleration_y", "acceleration_z"};
+  for (int i = 0; i < 3; i++) {
+    const EFlt3DArray arr = accel_map.at(names[i]);
+    a.data[ENZOE_SLOT_VELOCITY_X + i] = arr.data();
+    for (int k = 0; k < 3; k++) {
+      if (i == 0) {
+        a.shape[k] = arr.shape(k);
+        a.stride[k] = arr.stride(k);
+      }
+    }
+  }
+  check_rc(enzoe_source_gravity(dt, &u, &d, &a, stale_depth, g_stream),
+           "EnzoSourceGravity::calculate_source");
 }
 
 void EnzoMHDIntegratorStageCommands::compute_flux_(
@@ -1214,6 +1254,15 @@ EnzoEFltArrayMap EnzoMethodMHDVlct::get_integration_map_(
   return EnzoEFltArrayMap("integration", field_list, arrays);
 }
 
+EnzoEFltArrayMap EnzoMethodMHDVlct::get_accel_map_(Block *block) noexcept {
+  if (!block->has_field("acceleration_x"))
+    return EnzoEFltArrayMap("accel", str_vec_t(), std::vector<EFlt3DArray>());
+  const str_vec_t keys = {"acceleration_x", "acceleration_y", "acceleration_z"};
+  std::vector<EFlt3DArray> arrays;
+  for (const std::string &k : keys) arrays.push_back(block->field_view(k));
+  return EnzoEFltArrayMap("accel", keys, arrays);
+}
+
 EnzoVlctScratchSpace *EnzoMethodMHDVlct::get_scratch_ptr_(
     const std::array<int, 3> &shape, const str_vec_t &passive_list) noexcept {
   // EnzoMethodMHDVlct.cpp:236-246 (re-made if a block of another shape or
@@ -1255,6 +1304,12 @@ void EnzoMethodMHDVlct::compute(Block *block) throw() {
         check_rc(enzoe_vlct_compute_store_fluxes(device_, &b, 1, block->dt(),
                                                  (int)slots.size(), slots.data(), &fd,
                                                  g_stream),
+                 "EnzoMethodMHDVlct::compute");
+      } else if (block->has_field("acceleration_x")) {
+        double *acc[3] = {block->field_view("acceleration_x").data(),
+                          block->field_view("acceleration_y").data(),
+                          block->field_view("acceleration_z").data()};
+        check_rc(enzoe_vlct_compute_gravity(device_, &b, 1, block->dt(), acc, g_stream),
                  "EnzoMethodMHDVlct::compute");
       } else {
         check_rc(enzoe_vlct_compute(device_, &b, 1, block->dt(), g_stream),
@@ -1337,6 +1392,7 @@ void EnzoMethodMHDVlct::compute_staged_(Block *block) noexcept {
   std::array<EnzoEFltArrayMap, 3> flux_maps_xyz = {scratch->xflux_map, scratch->yflux_map,
                                                    scratch->zflux_map};
   EnzoEFltArrayMap dUcons_map = scratch->dUcons_map;
+  const EnzoEFltArrayMap accel_map = get_accel_map_(block);
   if (bfield_method_ != nullptr) bfield_method_->register_target_block(block);
   const std::array<enzo_float, 3> cell_widths_xyz = {
       block->CellWidth()[0], block->CellWidth()[1], block->CellWidth()[2]};
@@ -1359,7 +1415,8 @@ void EnzoMethodMHDVlct::compute_staged_(Block *block) noexcept {
         (is_final_stage) ? external_integration_map : temp_integration_map;
     integrator_->compute_update_stage(external_integration_map, cur_stage_integration_map,
                                       out_integration_map, primitive_map, priml_map,
-                                      primr_map, flux_maps_xyz, dUcons_map, passive_list,
+                                      primr_map, flux_maps_xyz, dUcons_map, accel_map,
+                                      passive_list,
                                       bfield_method_, stage_index, cur_dt, stale_depth,
                                       cell_widths_xyz);
     stale_depth += integrator_->staling_from_stage((int)stage_index);

@@ -263,6 +263,9 @@ class Block {
   // a passively advected scalar (a density, cell-centred, the block's
   // shape), owned by the caller: the "color" field group of the reference
   void add_passive_scalar(const std::string &name, double *data);
+  // another cell-centred field of the block's shape owned by the caller
+  // (e.g. acceleration_x/y/z, which make EnzoMethodMHDVlct add gravity)
+  void add_field(const std::string &name, double *data);
   const str_vec_t &passive_scalars() const { return passive_; }
 
   EFlt3DArray field_view(const std::string &name) const;
@@ -548,6 +551,16 @@ struct EnzoMHDIntegratorStageArgPack {
   std::string mhd_choice;
 };
 
+// EnzoSourceGravity (toolkit/EnzoSourceGravity.hpp, .cpp:17-69)
+class EnzoSourceGravity {
+ public:
+  // dUcons momentum += dt rho a and energy += dt rho (v . a) over
+  // [stale_depth, m - stale_depth); accel_map holds acceleration_x/y/z
+  void calculate_source(const double dt, const EnzoEFltArrayMap &prim_map,
+                        EnzoEFltArrayMap &dUcons_map, const EnzoEFltArrayMap &accel_map,
+                        const int stale_depth) const noexcept;
+};
+
 class EnzoMHDIntegratorStageCommands {
  public:
   enum class bfield_choice { no_bfield, unsafe_const_uniform, constrained_transport };
@@ -566,6 +579,7 @@ class EnzoMHDIntegratorStageCommands {
                             EnzoEFltArrayMap primr_map,
                             std::array<EnzoEFltArrayMap, 3> flux_maps_xyz,
                             EnzoEFltArrayMap dUcons_map,
+                            const EnzoEFltArrayMap &accel_map,
                             const str_vec_t &passive_list,
                             EnzoBfieldMethod *bfield_method,
                             unsigned short stage_index, double cur_dt,
@@ -641,6 +655,8 @@ class EnzoMethodMHDVlct : public Method {
  private:
   EnzoEFltArrayMap get_integration_map_(Block *block,
                                         const str_vec_t &passive_list) const noexcept;
+  // EnzoMethodMHDVlct.cpp:219-231: acceleration_x/y/z or an empty map
+  static EnzoEFltArrayMap get_accel_map_(Block *block) noexcept;
   EnzoVlctScratchSpace *get_scratch_ptr_(const std::array<int, 3> &field_shape,
                                          const str_vec_t &passive_list) noexcept;
   void compute_staged_(Block *block) noexcept;

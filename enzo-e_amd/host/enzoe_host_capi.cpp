@@ -84,6 +84,20 @@ int enzoe_host_method_compute_passive(enzoe_host_method_t m,
   return b.compute_done_count();
 }
 
+int enzoe_host_method_compute_accel(enzoe_host_method_t m, const enzoe_block *block,
+                                    double *const *accel, double dt,
+                                    enzoe_stream_t stream) {
+  if (!m || !block || !accel || !accel[0] || !accel[1] || !accel[2]) return 0;
+  enzoe_host_set_stream(stream);
+  Block b(*block);
+  b.add_field("acceleration_x", accel[0]);
+  b.add_field("acceleration_y", accel[1]);
+  b.add_field("acceleration_z", accel[2]);
+  b.set_dt(dt);
+  m->method->compute(&b);
+  return b.compute_done_count();
+}
+
 double enzoe_host_method_timestep(enzoe_host_method_t m, const enzoe_block *block,
                                   enzoe_stream_t stream) {
   if (!m || !block) return -1.0;

@@ -151,6 +151,17 @@ int enzoe_vlct_graph_time(enzoe_vlct_graph_t g, enzoe_stream_t stream,
                           double *t);
 int enzoe_vlct_graph_destroy(enzoe_vlct_graph_t g);
 
+/* EnzoMethodMHDVlct::compute on blocks with acceleration fields
+ * (EnzoMethodMHDVlct.cpp:219-231, 410-414): the step of enzoe_vlct_compute
+ * plus the gravity source of EnzoSourceGravity::calculate_source
+ * (toolkit/EnzoSourceGravity.cpp:17-69) on the stage with index 1, from the
+ * start-of-step density and velocity (EnzoMHDIntegratorStageCommands.cpp:
+ * 265-295).  accel[3*i + a]: device acceleration_x/y/z of block i, cell
+ * shaped with ghosts.                                                       */
+int enzoe_vlct_compute_gravity(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                               double dt, double *const *accel,
+                               enzoe_stream_t stream);
+
 /* ---- ghost refresh for a uniform grid of blocks ------------------------- */
 
 #define ENZOE_BC_PERIODIC 0
@@ -324,6 +335,14 @@ int enzoe_integrator_timestep(const enzoe_fmap *integration,
                               const enzoe_farray *pressure, int mhd,
                               double gamma, double dx, double dy, double dz,
                               enzoe_stream_t stream, double *dt_out);
+
+/* EnzoSourceGravity::calculate_source (toolkit/EnzoSourceGravity.cpp:17-69)
+ * over [stale, n - stale): dUcons momentum (velocity slots) += dt rho a,
+ * dUcons energy += dt rho (v . a), with rho and v from `integration` and
+ * acceleration_x..z in slots 1..3 of `accel` (the velocity slots).        */
+int enzoe_source_gravity(double dt, const enzoe_fmap *integration,
+                         const enzoe_fmap *dUcons, const enzoe_fmap *accel,
+                         int stale_depth, enzoe_stream_t stream);
 
 /* EnzoPhysicsFluidProps::pressure_from_integration (:142-156).               */
 int enzoe_pressure_from_integration(const enzoe_fmap *integration,

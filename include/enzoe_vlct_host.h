@@ -50,6 +50,15 @@ int enzoe_host_method_compute_passive(enzoe_host_method_t m,
                                       double *const *passive, double dt,
                                       enzoe_stream_t stream);
 
+/* EnzoMethodMHDVlct::compute(Block*) on one block carrying the device fields
+ * acceleration_x/y/z (accel[0..2], cell-centred, the block's shape): the
+ * gravity source is added on the full step (EnzoMethodMHDVlct.cpp:219-231,
+ * 410-414).  Returns compute_done() calls.                                 */
+int enzoe_host_method_compute_accel(enzoe_host_method_t m,
+                                    const enzoe_block *block,
+                                    double *const *accel, double dt,
+                                    enzoe_stream_t stream);
+
 /* EnzoMethodMHDVlct::timestep(Block*) (control_stopping.cpp:67-69); writes
  * the block's pressure field.                                               */
 double enzoe_host_method_timestep(enzoe_host_method_t m,

@@ -119,6 +119,9 @@ def lib():
         L.o_vlct_compute_fc.argtypes = [ctypes.POINTER(OConfig), ctypes.POINTER(OBlock),
                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, dp]
         L.o_vlct_compute_fc.restype = ctypes.c_int
+        L.o_vlct_compute_accel.argtypes = [ctypes.POINTER(OConfig), ctypes.POINTER(OBlock),
+                                           ctypes.c_double, ctypes.POINTER(dp)]
+        L.o_vlct_compute_accel.restype = ctypes.c_int
         L.o_face_fluxes_coarsen.argtypes = [dp, dp] + [ctypes.c_int] * 7
         L.o_flux_send_to_coarse.argtypes = [dp, dp] + [ctypes.c_int] * 6 + [ip]
         L.o_flux_correct.argtypes = [ctypes.POINTER(dp), ctypes.c_int, ctypes.c_int, ip,

@@ -1117,11 +1117,30 @@ static int immediate_staling(int kind) { return (kind == O_NN) ? 0 : 1; }
  * (EnzoMHDIntegratorStageCommands.cpp:102-203) with compute_flux_
  * (:207-261) and, for CT, EnzoBfieldMethodCT::update_all_bfield_components
  * (EnzoBfieldMethodCT.cpp:220-263). */
+/* EnzoSourceGravity::calculate_source (toolkit/EnzoSourceGravity.cpp:
+ * 17-69) with the start-of-step density and velocity, over [s, m-s). */
+static void gravity_source(double dt, double *const *U0, double *const *acc,
+                           double **dU, int mz, int my, int mx, int s) {
+  OMP_FOR
+  for (int k = s; k < mz - s; k++)
+    for (int j = s; j < my - s; j++)
+      for (int i = s; i < mx - s; i++) {
+        long c = IDX(k, j, i, my, mx);
+        double rho = U0[O_RHO][c];
+        double ax = acc[0][c], ay = acc[1][c], az = acc[2][c];
+        dU[1][c] += dt * rho * ax;
+        dU[2][c] += dt * rho * ay;
+        dU[3][c] += dt * rho * az;
+        dU[4][c] += dt * rho * ((U0[O_VX][c] * ax) + (U0[O_VY][c] * ay) +
+                                (U0[O_VZ][c] * az));
+      }
+}
+
 static void compute_update_stage(const o_config *cfg, o_block *b,
                                  o_scratch *sc, double **U0, double **Uc,
                                  double **Uo, double **bfi_recon,
                                  double **bfi_out, int stage, double cur_dt,
-                                 int stale) {
+                                 int stale, double *const *accel) {
   const int mz = b->mz, my = b->my, mx = b->mx;
   const long ncc = (long)mx * my * mz;
   const int kind = cfg->recon[stage];
@@ -1159,6 +1178,11 @@ static void compute_update_stage(const o_config *cfg, o_block *b,
   }
   stale += immediate_staling(kind);
 
+  /* compute_source_terms_ (EnzoMHDIntegratorStageCommands.cpp:265-295):
+   * gravity on the stage with index 1 only, from the start-of-step state */
+  if (stage == 1 && accel)
+    gravity_source(cur_dt, U0, accel, sc->dU, mz, my, mx, stale);
+
   if (cfg->mhd) {
     /* compute_all_edge_efields (EnzoBfieldMethodCT.cpp:561-590) */
     for (int i = 0; i < 3; i++) {
@@ -1185,22 +1209,27 @@ static void compute_update_stage(const o_config *cfg, o_block *b,
  * :333-355) and the final stage's fluxes are saved on the block faces. */
 static int vlct_compute_impl(const o_config *cfg, o_block *b, double dt,
                              int ghost, int nf, const int *fields,
-                             double *flux_data);
+                             double *flux_data, double *const *accel);
 
 int o_vlct_compute(const o_config *cfg, o_block *b, double dt) {
-  return vlct_compute_impl(cfg, b, dt, 0, 0, NULL, NULL);
+  return vlct_compute_impl(cfg, b, dt, 0, 0, NULL, NULL, NULL);
 }
 
 int o_vlct_compute_fc(const o_config *cfg, o_block *b, double dt, int ghost,
                       int nf, const int *fields, double *flux_data) {
   /* EnzoMethodMHDVlct.cpp:137-141: hydro only */
   if (cfg->mhd) return -2;
-  return vlct_compute_impl(cfg, b, dt, ghost, nf, fields, flux_data);
+  return vlct_compute_impl(cfg, b, dt, ghost, nf, fields, flux_data, NULL);
+}
+
+int o_vlct_compute_accel(const o_config *cfg, o_block *b, double dt,
+                         double *const *accel) {
+  return vlct_compute_impl(cfg, b, dt, 0, 0, NULL, NULL, accel);
 }
 
 static int vlct_compute_impl(const o_config *cfg, o_block *b, double dt,
                              int ghost, int nf, const int *fields,
-                             double *flux_data) {
+                             double *flux_data, double *const *accel) {
   if (cfg->mhd && cfg->nstages != 2) return -1; /* EnzoBfieldMethod.cpp:23-27 */
   int req = 0;
   for (int st = 0; st < cfg->nstages; st++) req += total_staling(cfg->recon[st]);
@@ -1225,7 +1254,7 @@ static int vlct_compute_impl(const o_config *cfg, o_block *b, double dt,
              (size_t)o_flux_data_size(nf, nx, ny, nz) * sizeof(double));
     }
     compute_update_stage(cfg, b, sc, b->cc, Uc, Uo, bfi_recon, bfi_out, stage,
-                         cur_dt, stale);
+                         cur_dt, stale, accel);
     stale += total_staling(cfg->recon[stage]);
     if (is_final && flux_data) {
       for (int d = 0; d < 3; d++)

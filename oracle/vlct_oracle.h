@@ -124,6 +124,14 @@ int o_restrict_linear(double *values_c, const int nd3_c[3], const int im3_c[3],
                       const int nd3_f[3], const int im3_f[3],
                       const int n3_f[3], int accumulate);
 
+/* EnzoMethodMHDVlct::compute with acceleration fields (accel[0..2] =
+ * acceleration_x..z, (mz,my,mx)): the gravity source of
+ * EnzoSourceGravity::calculate_source (toolkit/EnzoSourceGravity.cpp:17-69),
+ * added on the stage with index 1 (EnzoMHDIntegratorStageCommands.cpp:
+ * 265-295).                                                                 */
+int o_vlct_compute_accel(const o_config *cfg, o_block *b, double dt,
+                         double *const *accel);
+
 /* EnzoComputePressure::compute_pressure (EnzoComputePressure.cpp:82-198)
  * over [stale, m-stale) into p.                                            */
 void o_compute_pressure(const o_config *cfg, const o_block *b, double *p,

@@ -391,3 +391,16 @@ def test_host_shock_tube_errors_exit_like_the_reference(params, message):
     r = _run_host_tube(params)
     assert r.returncode == 1, (r.stdout, r.stderr)
     assert "ERROR in" in r.stderr and message in r.stderr, r.stderr
+
+
+def test_gravity_entry_points_reject_bad_arguments_without_a_device():
+    L = ev.ops()
+    rc, h = _create()
+    b = (ev.Block * 1)()
+    acc = (ctypes.c_void_p * 3)(1 << 40, None, 1 << 40)
+    assert L.enzoe_vlct_compute_gravity(h, b, 1, 0.1, acc, None) == ERR_ARG
+    assert L.enzoe_vlct_compute_gravity(h, b, 1, 0.1, None, None) == ERR_ARG
+    L.enzoe_vlct_destroy(h)
+    m = ev.FMap()
+    assert L.enzoe_source_gravity(0.1, ctypes.byref(m), ctypes.byref(m), ctypes.byref(m), 0,
+                                  None) == ERR_ARG
