@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "liboracle_vlct.so")
 O_RHO, O_VX, O_VY, O_VZ, O_ETOT, O_BX, O_BY, O_BZ = range(8)
 PRESSURE = 8
 FIELD_NAMES = ["density", "velocity_x", "velocity_y", "velocity_z",
-               "total_energy", "bfield_x", "bfield_y", "bfield_z", "pressure"]
+               "total_energy", "bfield_x", "bfield_y", "bfield_z", "pressure",
+               "internal_energy"]
 RIEMANN = {"hlle": 0, "hllc": 1, "hlld": 2}
 RECON = {"nn": 0, "plm": 1, "plm_enzo": 1, "plm_athena": 2}
 BC = {"periodic": 0, "outflow": 1}
@@ -32,7 +33,9 @@ class OConfig(ctypes.Structure):
                 ("recon", ctypes.c_int * 2),
                 ("nstages", ctypes.c_int),
                 ("mhd", ctypes.c_int),
-                ("courant", ctypes.c_double)]
+                ("courant", ctypes.c_double),
+                ("dual_energy", ctypes.c_int),
+                ("eta", ctypes.c_double)]
 
 
 class OBlock(ctypes.Structure):
@@ -41,7 +44,8 @@ class OBlock(ctypes.Structure):
                 ("cw", ctypes.c_double * 3),
                 ("cc", ctypes.POINTER(ctypes.c_double) * 8),
                 ("bfi", ctypes.POINTER(ctypes.c_double) * 3),
-                ("pressure", ctypes.POINTER(ctypes.c_double))]
+                ("pressure", ctypes.POINTER(ctypes.c_double)),
+                ("eint", ctypes.POINTER(ctypes.c_double))]
 
 
 class OGrid(ctypes.Structure):
@@ -86,6 +90,9 @@ def lib():
             ctypes.c_int]
         L.o_init_shock_tube_rj2a.argtypes = [ctypes.POINTER(OConfig),
                                              ctypes.POINTER(OGrid), ctypes.c_int]
+        L.o_init_shock_tube.argtypes = [ctypes.POINTER(OConfig), ctypes.POINTER(OGrid),
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int]
         L.o_grid_gather.argtypes = [ctypes.POINTER(OConfig), ctypes.POINTER(OGrid),
                                     ctypes.c_int, dp]
         L.o_vlct_timestep.argtypes = [ctypes.POINTER(OConfig), ctypes.POINTER(OBlock)]
@@ -96,6 +103,14 @@ def lib():
         L.o_hlld_face.argtypes = [dp, dp, ctypes.c_double, dp]
         L.o_hlle_mhd_face.argtypes = [dp, dp, ctypes.c_double, dp]
         L.o_hllc_face.argtypes = [dp, dp, ctypes.c_double, dp]
+        L.o_hlld_face_de.argtypes = [dp, dp, ctypes.c_double, dp, dp]
+        L.o_hlle_mhd_face_de.argtypes = [dp, dp, ctypes.c_double, dp, dp]
+        L.o_hllc_face_de.argtypes = [dp, dp, ctypes.c_double, dp, dp]
+        L.o_floor_energy_and_sync.argtypes = [ctypes.POINTER(OConfig), ctypes.POINTER(dp),
+                                              dp] + [ctypes.c_int] * 4
+        L.o_internal_energy_source.argtypes = [ctypes.POINTER(OConfig), ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_double, dp, dp,
+                                               dp] + [ctypes.c_int] * 4
         L.o_limiter_enzo.argtypes = [ctypes.c_double] * 4
         L.o_limiter_enzo.restype = ctypes.c_double
         L.o_limiter_athena.argtypes = [ctypes.c_double] * 3
@@ -147,8 +162,13 @@ def dptr(a):
 
 def make_config(gamma=1.6666666666666667, riemann="hlld", recon=("nn", "plm"),
                 theta=2.0, mhd=True, courant=0.4, density_floor=1.0e-200,
-                pressure_floor=1.0e-200):
+                pressure_floor=1.0e-200, dual_energy_eta=None):
+    """Method and fluid parameters of input/vlct/vl.incl and vlct.incl;
+    dual_energy_eta selects the "modern" dual-energy formalism with that
+    eta (input/vlct/vlct_de.incl)."""
     c = OConfig()
+    c.dual_energy = 0 if dual_energy_eta is None else 1
+    c.eta = 0.0 if dual_energy_eta is None else dual_energy_eta
     c.gamma = gamma
     c.density_floor = density_floor
     c.pressure_floor = pressure_floor
@@ -199,6 +219,7 @@ class Grid:
         for q in range(8):
             out[FIELD_NAMES[q]] = np.ctypeslib.as_array(b.cc[q], shape=(mz, my, mx))
         out["pressure"] = np.ctypeslib.as_array(b.pressure, shape=(mz, my, mx))
+        out["internal_energy"] = np.ctypeslib.as_array(b.eint, shape=(mz, my, mx))
         out["bfieldi_x"] = np.ctypeslib.as_array(b.bfi[0], shape=(mz, my, mx + 1))
         out["bfieldi_y"] = np.ctypeslib.as_array(b.bfi[1], shape=(mz, my + 1, mx))
         out["bfieldi_z"] = np.ctypeslib.as_array(b.bfi[2], shape=(mz + 1, my, mx))
@@ -311,6 +332,123 @@ def table_l1(grid, table, axis, fields=MHD_L1_FIELDS):
         sim = grid.gather(f)
         errs.append(np.sum(np.abs(sim - ref)) / float(sim.size))
     return float(np.sqrt(np.sum(np.square(errs))))
+
+
+def shifted_table_l1(gather, table, axis, lower, cell_width, offset=0.0,
+                     bkg_velocity=(0.0, 0.0, 0.0), reverse=False, per_column=False,
+                     fields=("density", "velocity_x", "velocity_y", "velocity_z")):
+    """The reference's table-mode L1 error norm with --permute, --reverse,
+    --offset_soln and --bkg_velocity (compare_to_1D_reference,
+    tools/l1_error_norm.py:537-627): the table's positions are shifted by
+    `offset`, the simulation is compared on the cells the shifted table
+    covers along `axis` (every transverse cell), the table is optionally
+    reversed with its vector components negated, vectors are permuted onto
+    `axis`, and the background velocity is added (total energy adjusted when
+    tabulated).  `gather(name)` returns a simulation field as a (z, y, x)
+    array of the whole domain; `lower`/`cell_width` describe that axis.
+    Fields missing from the table are skipped (find_common_fields, :274-330).
+    per_column: the norms of every 1D slice along `axis` (the --std input,
+    L1_error_vector with sum_axis, :195-234) instead of the total."""
+    pos_key = [k for k in table if k in ("position", "pos", "x", "y", "z")][0]
+    pos = np.asarray(table[pos_key], dtype=np.float64).copy()
+    cw = np.diff(pos)[0]
+    if offset != 0.0:
+        if offset % abs(cw) != 0.0:
+            raise ValueError("offset is not a multiple of the table's cell width")
+        pos += offset
+    left = pos[0] - cw * 0.5
+    i0 = int((left - lower) / cell_width + 0.5)
+    if not np.isclose(lower + i0 * cell_width, left, atol=1e-13, rtol=0):
+        raise ValueError("the table does not start at a cell boundary")
+    n = pos.size
+    ref = {k: np.asarray(v, dtype=np.float64).copy() for k, v in table.items()
+           if k != pos_key}
+
+    def vectors(d):
+        cand = {}
+        for k in d:
+            t = k.split("_")
+            if len(t) == 2 and t[1] in "xyz" and len(t[1]) == 1:
+                cand.setdefault(t[0], [None, None, None])["xyz".index(t[1])] = k
+        return cand
+
+    if reverse:
+        for k in ref:
+            ref[k] = ref[k][::-1].copy()
+        for names in vectors(ref).values():
+            for k in names:
+                if k is not None:
+                    ref[k] = ref[k] * -1.0
+    check = ["velocity_x", "velocity_y", "velocity_z"]
+    if any(b in ref for b in ("bfield_x", "bfield_y", "bfield_z")):
+        check += ["bfield_x", "bfield_y", "bfield_z"]
+    for k in check:
+        if k not in ref:
+            ref[k] = np.zeros_like(pos)
+    for prefix, names in vectors(ref).items():
+        if None in names:
+            continue
+        tmp = [ref[names[0]], ref[names[1]], ref[names[2]]]
+        ref[names[axis]] = tmp[0]
+        ref[names[(axis + 1) % 3]] = tmp[1]
+        ref[names[(axis + 2) % 3]] = tmp[2]
+    for i, k in enumerate(["velocity_x", "velocity_y", "velocity_z"]):
+        old = ref[k].copy()
+        ref[k] = ref[k] + bkg_velocity[i]
+        if "total_energy" in ref:
+            ref["total_energy"] = ref["total_energy"] + 0.5 * (np.square(ref[k]) -
+                                                               np.square(old))
+    shp = [1, 1, 1]
+    shp[2 - axis] = n
+    ax_np = 2 - axis
+    errs = []
+    for f in fields:
+        if f not in ref:
+            continue
+        sim = np.take(gather(f), np.arange(i0, i0 + n), axis=ax_np)
+        diff = np.abs(sim - ref[f].reshape(shp))
+        if per_column:
+            errs.append(np.sum(diff, axis=ax_np) / float(n))
+        else:
+            errs.append(np.sum(diff) / float(sim.size))
+    errs = np.array(errs)
+    if per_column:
+        return np.sqrt(np.sum(np.square(errs), axis=0))
+    return float(np.sqrt(np.sum(np.square(errs))))
+
+
+# the dual-energy shock tube decks (input/vlct/dual_energy_shock_tube/
+# method_vlct_sod_*_de_M10*.in with vlct_de.incl and
+# initial_sod_shock_tube.incl): 508 cells along the tube over 4 blocks,
+# 4 across, a Mach ~10 background velocity along the tube
+DE_TUBE_ETA = 0.00769
+DE_TUBE_AXIS_VELOCITY = 11.875
+DE_TUBE_OFFSET = 2.96875
+
+
+def sod_de_grid(axis, reverse=False, n=508, ntrans=4, nblocks=4, eta=DE_TUBE_ETA):
+    """Grid + initial state of the dual-energy Sod shock tube deck along
+    `axis` (the _reverse deck: flip_initialize on [-2.96875, 1])."""
+    cfg = make_config(gamma=1.4, dual_energy_eta=eta)
+    nc = [ntrans, ntrans, ntrans]
+    nc[axis] = n // nblocks
+    nb = [1, 1, 1]
+    nb[axis] = nblocks
+    lower = [0.0, 0.0, 0.0]
+    upper = [0.03125, 0.03125, 0.03125]
+    if reverse:
+        lower[axis], upper[axis] = -2.96875, 1.0
+    else:
+        upper[axis] = 3.96875
+    bc = ["periodic"] * 3
+    bc[axis] = "outflow"
+    grid = Grid(cfg, tuple(nb), tuple(nc), tuple(lower), tuple(upper), bc=tuple(bc))
+    rc = lib().o_init_shock_tube(ctypes.byref(cfg), ctypes.byref(grid.g), 1, axis,
+                                 DE_TUBE_AXIS_VELOCITY, 0.0, 1 if reverse else 0)
+    if rc != 0:
+        raise ValueError("sod init failed")
+    grid.lower, grid.upper = lower, upper
+    return grid
 
 
 def rj2a_grid(axis, n=256, ntrans=4):

@@ -60,6 +60,7 @@ int o_grid_alloc(o_grid *gr) {
         for (int q = 0; q < O_NCC; q++)
           b->cc[q] = (double *)calloc((size_t)ncc, sizeof(double));
         b->pressure = (double *)calloc((size_t)ncc, sizeof(double));
+        b->eint = (double *)calloc((size_t)ncc, sizeof(double));
         b->bfi[0] = (double *)calloc((size_t)b->mz * b->my * (b->mx + 1), sizeof(double));
         b->bfi[1] = (double *)calloc((size_t)b->mz * (b->my + 1) * b->mx, sizeof(double));
         b->bfi[2] = (double *)calloc((size_t)(b->mz + 1) * b->my * b->mx, sizeof(double));
@@ -74,6 +75,7 @@ void o_grid_free(o_grid *gr) {
     o_block *b = &gr->blocks[i];
     for (int q = 0; q < O_NCC; q++) free(b->cc[q]);
     free(b->pressure);
+    free(b->eint);
     for (int d = 0; d < 3; d++) free(b->bfi[d]);
   }
   free(gr->blocks);
@@ -120,7 +122,7 @@ static int map_axis(const o_grid *gr, int axis, int blk, int idx, int centred,
 }
 
 static void refresh_array(const o_grid *gr, int which /* -1 cc, 0..2 face */,
-                          int q) {
+                          int q /* cc slot, O_NCC = internal_energy */) {
   const int nbx = gr->nbx, nby = gr->nby, nbz = gr->nbz;
   const int ce[3] = {which == 0, which == 1, which == 2};
   for (int bz = 0; bz < nbz; bz++)
@@ -128,7 +130,8 @@ static void refresh_array(const o_grid *gr, int which /* -1 cc, 0..2 face */,
       for (int bx = 0; bx < nbx; bx++) {
         o_block *b = &gr->blocks[(bz * nby + by) * nbx + bx];
         const int ax = b->mx + ce[0], ay = b->my + ce[1], az = b->mz + ce[2];
-        double *dst = (which < 0) ? b->cc[q] : b->bfi[which];
+        double *dst = (which < 0) ? (q == O_NCC ? b->eint : b->cc[q])
+                                  : b->bfi[which];
         for (int k = 0; k < az; k++)
           for (int j = 0; j < ay; j++)
             for (int i = 0; i < ax; i++) {
@@ -139,18 +142,22 @@ static void refresh_array(const o_grid *gr, int which /* -1 cc, 0..2 face */,
               if (!ghost) continue;
               const o_block *s =
                   &gr->blocks[(sb[2] * nby + sb[1]) * nbx + sb[0]];
-              const double *src = (which < 0) ? s->cc[q] : s->bfi[which];
+              const double *src = (which < 0)
+                                       ? (q == O_NCC ? s->eint : s->cc[q])
+                                       : s->bfi[which];
               dst[IDX(k, j, i, ay, ax)] = src[IDX(si[2], si[1], si[0], ay, ax)];
             }
       }
 }
 
-/* Refresh of every field (EnzoMethodMHDVlct.cpp:146-151 adds all fields). */
+/* Refresh of every field (EnzoMethodMHDVlct.cpp:146-151 adds all fields;
+ * internal_energy is one of them with dual energy). */
 void o_grid_refresh(const o_config *cfg, o_grid *gr) {
   for (int q = 0; q < O_NCC; q++) {
     if (!cfg->mhd && q >= O_BX) continue;
     refresh_array(gr, -1, q);
   }
+  if (cfg->dual_energy) refresh_array(gr, -1, O_NCC);
   if (cfg->mhd)
     for (int d = 0; d < 3; d++) refresh_array(gr, d, 0);
 }
@@ -480,15 +487,33 @@ int o_init_inclined_wave(const o_config *cfg, o_grid *gr, const char *wave,
 /* RJ2a shock tube (EnzoInitialShockTube.cpp recipe)                        */
 /* ------------------------------------------------------------------------ */
 
-int o_init_shock_tube_rj2a(const o_config *cfg, o_grid *gr, int ax) {
-  if (!cfg->mhd) return -1;
-  /* left / right states of Ryu & Jones (1995) fig 2a
-     (EnzoInitialShockTube.cpp:24-60) */
-  const double L[7] = {1.08, 0.95, 1.2, 0.01, 0.5, 1.0155412503859613,
-                       0.5641895835477563};
-  const double R[7] = {1., 1.0, 0.0, 0.0, 0.0, 1.1283791670955126,
-                       0.5641895835477563};
-  const double b0 = 0.5641895835477563;
+/* left / right states (EnzoInitialShockTube.cpp:36-86): density, pressure,
+ * velocity_0..2 (0 along the tube), bfield_1, bfield_2; and bfield_0 */
+static const double TUBE_L[2][7] = {
+    {1.08, 0.95, 1.2, 0.01, 0.5, 1.0155412503859613, 0.5641895835477563},
+    {1.0, 1.0, 0., 0., 0., 0., 0.}};
+static const double TUBE_R[2][7] = {
+    {1., 1.0, 0.0, 0.0, 0.0, 1.1283791670955126, 0.5641895835477563},
+    {0.125, 0.1, 0., 0., 0., 0., 0.}};
+static const double TUBE_B0[2] = {0.5641895835477563, 0.0};
+
+int o_init_shock_tube(const o_config *cfg, o_grid *gr, int setup, int ax,
+                      double axis_velocity_p, double trans_velocity_p,
+                      int flipped) {
+  if (setup < 0 || setup > 1) return -1;
+  if (setup == 0 && !cfg->mhd) return -1;
+  /* setup_maps_ (EnzoInitialShockTube.cpp:161-176): the flipped tube swaps
+     the states and negates every velocity and bfield entry */
+  double L[7], R[7];
+  for (int q = 0; q < 7; q++) {
+    const double f = (flipped && q >= 2) ? -1. : 1.;
+    L[q] = f * (flipped ? TUBE_R[setup][q] : TUBE_L[setup][q]);
+    R[q] = f * (flipped ? TUBE_L[setup][q] : TUBE_R[setup][q]);
+  }
+  const double flip = flipped ? -1. : 1.;
+  const double b0 = flip * TUBE_B0[setup];
+  const double axis_velocity = flip * axis_velocity_p;
+  const double trans_velocity = flip * trans_velocity_p;
   const double gamma = cfg->gamma;
   const int ai = ax, aj = (ax + 1) % 3, ak = (ax + 2) % 3;
   const int g = gr->g;
@@ -509,8 +534,8 @@ int o_init_shock_tube_rj2a(const o_config *cfg, o_grid *gr, int ax) {
           const double *S = side ? R : L;
           int s0 = side ? shock_ind : 0, s1 = side ? m[ai] : shock_ind;
           if (s0 >= s1) continue;
-          double velocity_0 = S[2] + 0.0, velocity_1 = S[3] + 0.0,
-                 velocity_2 = S[4];
+          double velocity_0 = S[2] + axis_velocity,
+                 velocity_1 = S[3] + trans_velocity, velocity_2 = S[4];
           double eint = (S[1] / ((gamma - 1.) * S[0]));
           double v2 = (velocity_0 * velocity_0 + velocity_1 * velocity_1 +
                        velocity_2 * velocity_2);
@@ -518,6 +543,7 @@ int o_init_shock_tube_rj2a(const o_config *cfg, o_grid *gr, int ax) {
           double etot = (eint + 0.5 * (v2 + b2 / S[0]));
           /* cell-centred fields and transverse face fields */
           for (int f = 0; f < 3; f++) { /* f: 0 = cc, 1 = bfieldi_j, 2 = bfieldi_k */
+            if (f > 0 && !cfg->mhd) continue;
             int ext[3] = {b->mx, b->my, b->mz};
             if (f == 1) ext[aj] += 1;
             if (f == 2) ext[ak] += 1;
@@ -532,6 +558,8 @@ int o_init_shock_tube_rj2a(const o_config *cfg, o_grid *gr, int ax) {
                     b->cc[O_VX + ai][c] = velocity_0;
                     b->cc[O_VX + aj][c] = velocity_1;
                     b->cc[O_VX + ak][c] = velocity_2;
+                    /* (optionally) the specific internal energy (:255-261) */
+                    if (cfg->dual_energy) b->eint[c] = eint;
                     b->cc[O_ETOT][c] = etot;
                   } else if (f == 1) {
                     b->bfi[aj][c] = S[5];
@@ -541,15 +569,19 @@ int o_init_shock_tube_rj2a(const o_config *cfg, o_grid *gr, int ax) {
                 }
           }
         }
-        {
+        if (cfg->mhd) {
           int ext[3] = {b->mx, b->my, b->mz};
           ext[ai] += 1;
           long n = (long)ext[0] * ext[1] * ext[2];
           for (long c = 0; c < n; c++) b->bfi[ai][c] = b0;
+          center_bfield_all(b);
         }
-        center_bfield_all(b);
       }
   return 0;
+}
+
+int o_init_shock_tube_rj2a(const o_config *cfg, o_grid *gr, int ax) {
+  return o_init_shock_tube(cfg, gr, 0, ax, 0.0, 0.0, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -571,7 +603,8 @@ void o_grid_gather(const o_config *cfg, const o_grid *gr, int field,
           p = (double *)malloc(sizeof(double) * (size_t)b->mx * b->my * b->mz);
           o_compute_pressure(cfg, b, p, 0);
         }
-        const double *src = (field == 8) ? p : b->cc[field];
+        const double *src =
+            (field == 8) ? p : ((field == 9) ? b->eint : b->cc[field]);
         for (int k = 0; k < gr->nz; k++)
           for (int j = 0; j < gr->ny; j++)
             for (int i = 0; i < gr->nx; i++) {

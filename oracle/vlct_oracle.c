@@ -164,9 +164,25 @@ static inline double fast_ms_speed(double d, double p, double bi, double bj,
 
 typedef struct { double d, mx, my, mz, e, by, bz; } cons1d;
 
-/* HLLDKernel::operator() (EnzoRiemannHLLD.hpp:40-448) */
-void o_hlld_face(const double wli[8], const double wri[8], double gamma,
-                 double flxo[8]) {
+/* enzo_riemann_utils::passive_eint_flux (EnzoRiemannUtils.hpp:224-249):
+ * the specific internal energy eint = p / ((gamma - 1) rho) of the side the
+ * density flux comes from (branch-free selection as the reference writes
+ * it), times the density flux. */
+static double passive_eint_flux(double density_l, double pressure_l,
+                                double density_r, double pressure_r,
+                                double gamma, double density_flux) {
+  double eint_l = pressure_l / ((gamma - 1.0) * density_l);
+  double eint_r = pressure_r / ((gamma - 1.0) * density_r);
+  double upwind = (double)(density_flux > 0) * eint_l +
+                  (double)(density_flux <= 0) * eint_r;
+  return upwind * density_flux;
+}
+
+/* HLLDKernel::operator() (EnzoRiemannHLLD.hpp:40-448); with de, the dual
+ * energy outputs of :404-447: the internal-energy flux and vi_bar from the
+ * outer and contact speeds. */
+static void hlld_impl(const double wli[8], const double wri[8], double gamma,
+                      double flxo[8], double *de) {
   const double SMALL_NUMBER = 1.0e-8;
   const double igm1 = 1.0 / (gamma - 1.0);
   double spd[5];
@@ -418,6 +434,34 @@ void o_hlld_face(const double wli[8], const double wri[8], double gamma,
   flxo[L_VJ] = flxi[L_VJ];
   flxo[L_VK] = flxi[L_VK];
   flxo[L_E] = flxi[L_E];
+  if (de) {
+    de[0] = passive_eint_flux(wli[L_D], pressure_l, wri[L_D], pressure_r,
+                              gamma, flxo[L_D]);
+    const double S_M = spd[2];
+    const double S_l = spd[0];
+    const double S_r = spd[4];
+    const double l_coef = (S_l - wli[L_VI]) / (S_l - S_M);
+    const double r_coef = (S_r - wri[L_VI]) / (S_r - S_M);
+    if (S_l > 0) {
+      de[1] = wli[L_VI];
+    } else if (S_r < 0) {
+      de[1] = wri[L_VI];
+    } else if (S_M >= 0) {
+      de[1] = S_M * l_coef;
+    } else {
+      de[1] = S_M * r_coef;
+    }
+  }
+}
+
+void o_hlld_face(const double wli[8], const double wri[8], double gamma,
+                 double flxo[8]) {
+  hlld_impl(wli, wri, gamma, flxo, NULL);
+}
+
+void o_hlld_face_de(const double wli[8], const double wri[8], double gamma,
+                    double flxo[8], double de[2]) {
+  hlld_impl(wli, wri, gamma, flxo, de);
 }
 
 /* enzo_riemann_utils::compute_conserved (EnzoRiemannUtils.hpp:48-82) for
@@ -505,9 +549,10 @@ static void einfeldt_mhd(const double *wl, const double *wr, const double *Ul,
 }
 
 /* HLLKernel<EinfeldtWavespeed<MHDLUT>>::operator() (EnzoRiemannHLL.hpp:
- * 238-345) -- the "hlle" MHD solver. */
-void o_hlle_mhd_face(const double wl[8], const double wr[8], double gamma,
-                     double flux[8]) {
+ * 238-345) -- the "hlle" MHD solver; with de, the internal-energy flux and
+ * vi_bar interpolated between the outer speeds (:322-344). */
+static void hlle_mhd_impl(const double wl[8], const double wr[8],
+                          double gamma, double flux[8], double *de) {
   double cl[8], cr[8], fl[8], fr[8];
   mhd_cons_and_flux(wl, gamma, cl, fl);
   mhd_cons_and_flux(wr, gamma, cr, fr);
@@ -520,6 +565,21 @@ void o_hlle_mhd_face(const double wl[8], const double wr[8], double gamma,
     flux[f] = ((bp * fl[f] - bm * fr[f] + (cr[f] - cl[f]) * bp * bm) *
                inv_speed_diff);
   }
+  if (de) {
+    de[0] = passive_eint_flux(wl[L_D], wl[L_E], wr[L_D], wr[L_E], gamma,
+                              flux[L_D]);
+    de[1] = (bp * wl[L_VI] - bm * wr[L_VI]) * inv_speed_diff;
+  }
+}
+
+void o_hlle_mhd_face(const double wl[8], const double wr[8], double gamma,
+                     double flux[8]) {
+  hlle_mhd_impl(wl, wr, gamma, flux, NULL);
+}
+
+void o_hlle_mhd_face_de(const double wl[8], const double wr[8], double gamma,
+                        double flux[8], double de[2]) {
+  hlle_mhd_impl(wl, wr, gamma, flux, de);
 }
 
 /* compute_conserved for the hydro LUT (EnzoRiemannUtils.hpp:48-82) */
@@ -536,8 +596,8 @@ static void hydro_cons(const double *w, double gamma, double *cons) {
 
 /* HLLCKernel::operator() (EnzoRiemannHLLC.hpp:34-172) with
  * EinfeldtWavespeed<HydroLUT> and roe_cs_ (EnzoRiemannHLL.hpp:44-130). */
-void o_hllc_face(const double wl[5], const double wr[5], double gamma,
-                 double flux[5]) {
+static void hllc_impl(const double wl[5], const double wr[5], double gamma,
+                      double flux[5], double *de) {
   const double pressure_l = wl[H_E];
   const double pressure_r = wr[H_E];
   double cons_l[5], cons_r[5];
@@ -613,15 +673,32 @@ void o_hllc_face(const double wl[5], const double wr[5], double gamma,
   flux[H_E] = sl * efl + sr * efr;
   flux[H_VI] += (sm * cp);
   flux[H_E] += (sm * cp * cw);
+  if (de) {
+    /* EnzoRiemannHLLC.hpp:159-170 */
+    de[0] = passive_eint_flux(wl[H_D], pressure_l, wr[H_D], pressure_r, gamma,
+                              flux[H_D]);
+    de[1] = (sl * (wl[H_VI] - bm) + sr * (wr[H_VI] - bp));
+  }
+}
+
+void o_hllc_face(const double wl[5], const double wr[5], double gamma,
+                 double flux[5]) {
+  hllc_impl(wl, wr, gamma, flux, NULL);
+}
+
+void o_hllc_face_de(const double wl[5], const double wr[5], double gamma,
+                    double flux[5], double de[2]) {
+  hllc_impl(wl, wr, gamma, flux, de);
 }
 
 /* EnzoRiemannImpl<K>::solve / solve_ (EnzoRiemannImpl.hpp:265-338):
  * faces in [s, F-s) of the face array along every axis.  pl/pr/flux are
  * indexed by cell-centred field slot (O_RHO.. with O_ETOT holding pressure
  * for pl/pr). */
-void o_riemann_solve(const o_config *cfg, double *const *pl,
-                     double *const *pr, double **flux, int mz, int my,
-                     int mx, int dim, int s) {
+static void riemann_solve_impl(const o_config *cfg, double *const *pl,
+                               double *const *pr, double **flux,
+                               double *eflux, double *vbar, int mz, int my,
+                               int mx, int dim, int s) {
   int fz, fy, fx;
   face_shape(mz, my, mx, dim, &fz, &fy, &fx);
   const int vi = O_VX + dim, vj = O_VX + (dim + 1) % 3,
@@ -643,10 +720,16 @@ void o_riemann_solve(const o_config *cfg, double *const *pl,
           wl[L_BJ] = pl[bj][f];   wr[L_BJ] = pr[bj][f];
           wl[L_BK] = pl[bk][f];   wr[L_BK] = pr[bk][f];
           wl[L_E] = pl[O_ETOT][f]; wr[L_E] = pr[O_ETOT][f];
+          double de[2];
+          double *dep = eflux ? de : NULL;
           if (cfg->riemann == O_HLLD) {
-            o_hlld_face(wl, wr, cfg->gamma, fo);
+            hlld_impl(wl, wr, cfg->gamma, fo, dep);
           } else {
-            o_hlle_mhd_face(wl, wr, cfg->gamma, fo);
+            hlle_mhd_impl(wl, wr, cfg->gamma, fo, dep);
+          }
+          if (eflux) {
+            eflux[f] = de[0];
+            vbar[f] = de[1];
           }
           flux[O_RHO][f] = fo[L_D];
           flux[vi][f] = fo[L_VI];
@@ -663,7 +746,12 @@ void o_riemann_solve(const o_config *cfg, double *const *pl,
           wl[H_VJ] = pl[vj][f];   wr[H_VJ] = pr[vj][f];
           wl[H_VK] = pl[vk][f];   wr[H_VK] = pr[vk][f];
           wl[H_E] = pl[O_ETOT][f]; wr[H_E] = pr[O_ETOT][f];
-          o_hllc_face(wl, wr, cfg->gamma, fo);
+          double de[2];
+          hllc_impl(wl, wr, cfg->gamma, fo, eflux ? de : NULL);
+          if (eflux) {
+            eflux[f] = de[0];
+            vbar[f] = de[1];
+          }
           flux[O_RHO][f] = fo[H_D];
           flux[vi][f] = fo[H_VI];
           flux[vj][f] = fo[H_VJ];
@@ -673,21 +761,41 @@ void o_riemann_solve(const o_config *cfg, double *const *pl,
       }
 }
 
+void o_riemann_solve(const o_config *cfg, double *const *pl,
+                     double *const *pr, double **flux, int mz, int my,
+                     int mx, int dim, int s) {
+  riemann_solve_impl(cfg, pl, pr, flux, NULL, NULL, mz, my, mx, dim, s);
+}
+
+/* The solve with the dual-energy outputs (EnzoRiemannImpl.hpp:270-301):
+ * the internal-energy flux and vi_bar of every face, face-trimmed arrays. */
+void o_riemann_solve_de(const o_config *cfg, double *const *pl,
+                        double *const *pr, double **flux, double *eflux,
+                        double *vbar, int mz, int my, int mx, int dim,
+                        int s) {
+  riemann_solve_impl(cfg, pl, pr, flux, eflux, vbar, mz, my, mx, dim, s);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Pressure, primitives, dt                                                 */
 /* ------------------------------------------------------------------------ */
 
-/* EnzoComputePressure::compute_pressure, rank-3 non-dual-energy branch
- * (EnzoComputePressure.cpp:177-189), region from enzo_utils::exec_loop
- * (utils.hpp:122-148). */
-static void pressure_region(const o_config *cfg, double *const *U, double *p,
-                            int mz, int my, int mx, int s) {
+/* EnzoComputePressure::compute_pressure, rank 3 (EnzoComputePressure.cpp:
+ * 127-189): gm1 * rho * eint with dual energy, otherwise from the total
+ * energy; region from enzo_utils::exec_loop (utils.hpp:122-148). */
+static void pressure_region(const o_config *cfg, double *const *U,
+                            const double *eint, double *p, int mz, int my,
+                            int mx, int s) {
   const double gm1 = cfg->gamma - 1.0;
   OMP_FOR
   for (int k = s; k < mz - s; k++)
     for (int j = s; j < my - s; j++)
       for (int i = s; i < mx - s; i++) {
         long c = IDX(k, j, i, my, mx);
+        if (cfg->dual_energy) {
+          p[c] = gm1 * U[O_RHO][c] * eint[c];
+          continue;
+        }
         double vx = U[O_VX][c], vy = U[O_VY][c], vz = U[O_VZ][c];
         double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
         double me_den = 0.;
@@ -701,15 +809,15 @@ static void pressure_region(const o_config *cfg, double *const *U, double *p,
 
 void o_compute_pressure(const o_config *cfg, const o_block *b, double *p,
                         int s) {
-  pressure_region(cfg, b->cc, p, b->mz, b->my, b->mx, s);
+  pressure_region(cfg, b->cc, b->eint, p, b->mz, b->my, b->mx, s);
 }
 
 /* EnzoPhysicsFluidProps::primitive_from_integration
  * (EnzoPhysicsFluidProps.cpp:64-138): copy density, velocity and bfield over
  * [s, m-s), then pressure into the O_ETOT slot. */
 static void primitive_from_integration(const o_config *cfg, double *const *U,
-                                       double **W, int mz, int my, int mx,
-                                       int s) {
+                                       const double *eint, double **W,
+                                       int mz, int my, int mx, int s) {
   for (int q = 0; q < O_NCC; q++) {
     if (q == O_ETOT) continue;
     if (!cfg->mhd && q >= O_BX) continue;
@@ -721,16 +829,67 @@ static void primitive_from_integration(const o_config *cfg, double *const *U,
           W[q][c] = U[q][c];
         }
   }
-  pressure_region(cfg, U, W[O_ETOT], mz, my, mx, s);
+  pressure_region(cfg, U, eint, W[O_ETOT], mz, my, mx, s);
+}
+
+/* EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync
+ * (EnzoPhysicsFluidProps.cpp:158-290).  Without dual energy the total
+ * energy gets the floor eint_floor + non-thermal energy; with the "modern"
+ * formulation the internal energy is replaced by etot - non-thermal where
+ * the sound speed that implies exceeds eta times the kinetic and magnetic
+ * scales (and eint_1 > eint / 2 unless eta = 0), floored, and the total
+ * energy rebuilt from it.  ggm1 is a float in the reference (:236). */
+void o_floor_energy_and_sync(const o_config *cfg, double *const *U,
+                             double *eint, int mz, int my, int mx, int s) {
+  const int idual = cfg->dual_energy;
+  const double eta = idual ? cfg->eta : 0.0;
+  const float ggm1 = (float)(cfg->gamma * (cfg->gamma - 1.));
+  const double pressure_floor = cfg->pressure_floor;
+  const double inv_gm1 = 1. / (cfg->gamma - 1.);
+  const double half_factor = (eta != 0.) ? 0.5 : 0.;
+  OMP_FOR
+  for (int k = s; k < mz - s; k++)
+    for (int j = s; j < my - s; j++)
+      for (int i = s; i < mx - s; i++) {
+        long c = IDX(k, j, i, my, mx);
+        double inv_rho = 1. / U[O_RHO][c];
+        double eint_floor = pressure_floor * inv_gm1 * inv_rho;
+        double vx = U[O_VX][c], vy = U[O_VY][c], vz = U[O_VZ][c];
+        double v2 = (vx * vx + vy * vy + vz * vz);
+        double non_thermal_e = 0.5 * v2;
+        double b2 = 0;
+        if (cfg->mhd) {
+          double bx = U[O_BX][c], by = U[O_BY][c], bz = U[O_BZ][c];
+          b2 = (bx * bx + by * by + bz * bz);
+          non_thermal_e += (0.5 * b2 * inv_rho);
+        }
+        if (idual) {
+          double eint_1 = U[O_ETOT][c] - non_thermal_e;
+          double cur_eint = eint[c];
+          double cs2_1 = fmax(0., ggm1 * eint_1);
+          if ((cs2_1 > fmax(eta * v2, eta * b2 * inv_rho)) &&
+              (eint_1 > half_factor * cur_eint)) {
+            cur_eint = eint_1;
+          }
+          cur_eint = apply_floor(cur_eint, eint_floor);
+          eint[c] = cur_eint;
+          U[O_ETOT][c] = cur_eint + non_thermal_e;
+        } else {
+          double etot_floor = eint_floor + non_thermal_e;
+          U[O_ETOT][c] = apply_floor(U[O_ETOT][c], etot_floor);
+        }
+      }
 }
 
 /* EnzoMethodMHDVlct::timestep (EnzoMethodMHDVlct.cpp:551-588) ->
  * EnzoMHDIntegratorStageCommands::timestep
  * (EnzoMHDIntegratorStageCommands.cpp:299-366): min over every cell,
- * ghosts included. */
+ * ghosts included.  With dual energy the block's energies are synchronised
+ * first, in place (:563-568). */
 double o_vlct_timestep(const o_config *cfg, o_block *b) {
   const int mz = b->mz, my = b->my, mx = b->mx;
-  pressure_region(cfg, b->cc, b->pressure, mz, my, mx, 0);
+  if (cfg->dual_energy) o_floor_energy_and_sync(cfg, b->cc, b->eint, mz, my, mx, 0);
+  pressure_region(cfg, b->cc, b->eint, b->pressure, mz, my, mx, 0);
   const double dx = b->cw[0], dy = b->cw[1], dz = b->cw[2];
   const double gamma = cfg->gamma;
   double dt = 1.7976931348623157e308; /* numeric_limits<double>::max() */
@@ -768,35 +927,63 @@ static const int dU_fields[5] = {O_RHO, O_VX, O_VY, O_VZ, O_ETOT};
 
 /* EnzoIntegrationQuanUpdate::accumulate_flux_component
  * (EnzoIntegrationQuanUpdate.cpp:105-147): cells c in [s+1, m_d-s-1) along
- * dim, [s, m-s) transverse: dU(c) -= dtdx * (F(c) - F(c-1)). */
-static void accumulate_flux(int dim, double dt, double cw, double **flux,
-                            double **dU, int mz, int my, int mx, int s) {
+ * dim, [s, m-s) transverse: dU(c) -= dtdx * (F(c) - F(c-1)).  With dual
+ * energy the "internal_energy" key (eflux into dUe) follows. */
+static void accumulate_one(int dim, double dtdx_i, const double *F, double *D,
+                           int mz, int my, int mx, int s) {
   int fz, fy, fx;
   face_shape(mz, my, mx, dim, &fz, &fy, &fx);
   const int di = (dim == 0), dj = (dim == 1), dk = (dim == 2);
+  OMP_FOR
+  for (int k = s + dk; k < mz - s - dk; k++)
+    for (int j = s + dj; j < my - s - dj; j++)
+      for (int i = s + di; i < mx - s - di; i++) {
+        long c = IDX(k, j, i, my, mx);
+        double fr = F[IDX(k, j, i, fy, fx)];
+        double fl = F[IDX(k - dk, j - dj, i - di, fy, fx)];
+        D[c] -= dtdx_i * (fr - fl);
+      }
+}
+
+static void accumulate_flux(int dim, double dt, double cw, double **flux,
+                            double **dU, const double *eflux, double *dUe,
+                            int mz, int my, int mx, int s) {
   double dtdx_i = dt / cw;
-  for (int q = 0; q < 5; q++) {
-    const double *F = flux[dU_fields[q]];
-    double *D = dU[q];
-    OMP_FOR
-    for (int k = s + dk; k < mz - s - dk; k++)
-      for (int j = s + dj; j < my - s - dj; j++)
-        for (int i = s + di; i < mx - s - di; i++) {
-          long c = IDX(k, j, i, my, mx);
-          double fr = F[IDX(k, j, i, fy, fx)];
-          double fl = F[IDX(k - dk, j - dj, i - di, fy, fx)];
-          D[c] -= dtdx_i * (fr - fl);
-        }
-  }
+  for (int q = 0; q < 5; q++)
+    accumulate_one(dim, dtdx_i, flux[dU_fields[q]], dU[q], mz, my, mx, s);
+  if (eflux) accumulate_one(dim, dtdx_i, eflux, dUe, mz, my, mx, s);
+}
+
+void o_internal_energy_source(const o_config *cfg, int dim, double dt,
+                              double cw, const double *p, double *deint,
+                              const double *vbar, int mz, int my, int mx,
+                              int s) {
+  const double dtdx = dt / cw;
+  const double p_floor = cfg->pressure_floor;
+  int fz, fy, fx;
+  face_shape(mz, my, mx, dim, &fz, &fy, &fx);
+  const int di = (dim == 0), dj = (dim == 1), dk = (dim == 2);
+  OMP_FOR
+  for (int k = s + dk; k < mz - s - dk; k++)
+    for (int j = s + dj; j < my - s - dj; j++)
+      for (int i = s + di; i < mx - s - di; i++) {
+        long c = IDX(k, j, i, my, mx);
+        double vr = vbar[IDX(k, j, i, fy, fx)];
+        double vl = vbar[IDX(k - dk, j - dj, i - di, fy, fx)];
+        double pc = apply_floor(p[c], p_floor);
+        deint[c] -= dtdx * pc * (vr - vl);
+      }
 }
 
 /* EnzoIntegrationQuanUpdate::update_quantities
- * (EnzoIntegrationQuanUpdate.cpp:183-238) over [s+1, m-s-1), followed by
- * EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync
- * (EnzoPhysicsFluidProps.cpp:162-290, dual energy off) at stale s+1. */
-static void update_quantities(const o_config *cfg, double *const *U0,
-                              double *const *dU, double **Uo, int mz, int my,
-                              int mx, int s) {
+ * (EnzoIntegrationQuanUpdate.cpp:183-238) over [s+1, m-s-1) -- the specific
+ * quantities include internal_energy with dual energy -- followed by
+ * EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync at stale s+1. */
+static void update_quantities_de(const o_config *cfg, double *const *U0,
+                                 const double *eint0, double *const *dU,
+                                 const double *dUe, double **Uo,
+                                 double *eint_o, int mz, int my, int mx,
+                                 int s) {
   const double density_floor = cfg->density_floor;
   OMP_FOR
   for (int k = s + 1; k < mz - s - 1; k++)
@@ -812,27 +999,18 @@ static void update_quantities(const o_config *cfg, double *const *U0,
           int fq = dU_fields[q];
           Uo[fq][c] = (U0[fq][c] * old_rho + dU[q][c]) * inv_new_rho;
         }
+        if (cfg->dual_energy)
+          eint_o[c] = (eint0[c] * old_rho + dUe[c]) * inv_new_rho;
       }
-  const double pressure_floor = cfg->pressure_floor;
-  const double inv_gm1 = 1. / (cfg->gamma - 1.);
-  OMP_FOR
-  for (int k = s + 1; k < mz - s - 1; k++)
-    for (int j = s + 1; j < my - s - 1; j++)
-      for (int i = s + 1; i < mx - s - 1; i++) {
-        long c = IDX(k, j, i, my, mx);
-        double inv_rho = 1. / Uo[O_RHO][c];
-        double eint_floor = pressure_floor * inv_gm1 * inv_rho;
-        double vx = Uo[O_VX][c], vy = Uo[O_VY][c], vz = Uo[O_VZ][c];
-        double v2 = (vx * vx + vy * vy + vz * vz);
-        double non_thermal_e = 0.5 * v2;
-        if (cfg->mhd) {
-          double bx = Uo[O_BX][c], by = Uo[O_BY][c], bz = Uo[O_BZ][c];
-          double b2 = (bx * bx + by * by + bz * bz);
-          non_thermal_e += (0.5 * b2 * inv_rho);
-        }
-        double etot_floor = eint_floor + non_thermal_e;
-        Uo[O_ETOT][c] = apply_floor(Uo[O_ETOT][c], etot_floor);
-      }
+  o_floor_energy_and_sync(cfg, Uo, eint_o, mz, my, mx, s + 1);
+}
+
+static void update_quantities(const o_config *cfg, double *const *U0,
+                              double *const *dU, double **Uo, int mz, int my,
+                              int mx, int s) {
+  o_config c = *cfg;
+  c.dual_energy = 0;
+  update_quantities_de(&c, U0, NULL, dU, NULL, Uo, NULL, mz, my, mx, s);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1018,7 +1196,19 @@ void o_compute_edge_efield(int dim, const double *Ec, double *edge,
 }
 void o_accumulate_flux_component(int dim, double dt, double cw, double **flux,
                                  double **dU, int mz, int my, int mx, int s) {
-  accumulate_flux(dim, dt, cw, flux, dU, mz, my, mx, s);
+  accumulate_flux(dim, dt, cw, flux, dU, NULL, NULL, mz, my, mx, s);
+}
+void o_accumulate_flux_component_de(int dim, double dt, double cw,
+                                    double **flux, double **dU,
+                                    const double *eflux, double *dUe, int mz,
+                                    int my, int mx, int s) {
+  accumulate_flux(dim, dt, cw, flux, dU, eflux, dUe, mz, my, mx, s);
+}
+void o_update_quantities_de(const o_config *cfg, double *const *U0,
+                            const double *eint0, double *const *dU,
+                            const double *dUe, double **Uo, double *eint_o,
+                            int mz, int my, int mx, int s) {
+  update_quantities_de(cfg, U0, eint0, dU, dUe, Uo, eint_o, mz, my, mx, s);
 }
 void o_update_quantities(const o_config *cfg, double *const *U0,
                          double *const *dU, double **Uo, int mz, int my,
@@ -1027,7 +1217,12 @@ void o_update_quantities(const o_config *cfg, double *const *U0,
 }
 void o_primitive_from_integration(const o_config *cfg, double *const *U,
                                   double **W, int mz, int my, int mx, int s) {
-  primitive_from_integration(cfg, U, W, mz, my, mx, s);
+  primitive_from_integration(cfg, U, NULL, W, mz, my, mx, s);
+}
+void o_primitive_from_integration_de(const o_config *cfg, double *const *U,
+                                     const double *eint, double **W, int mz,
+                                     int my, int mx, int s) {
+  primitive_from_integration(cfg, U, eint, W, mz, my, mx, s);
 }
 
 /* EnzoBfieldMethodCT::compute_center_bfield (EnzoBfieldMethodCT.cpp:702-728)
@@ -1061,6 +1256,11 @@ typedef struct {
   double *edge[3];
   double *ecenter;
   double *tbfi[3];
+  /* dual energy: half-step internal energy, its flux and vi_bar per dim,
+     its accumulated change */
+  double *temp_eint;
+  double *eflux[3], *vbar[3];
+  double *dUe;
 } o_scratch;
 
 static void scratch_free(o_scratch *sc) {
@@ -1074,6 +1274,11 @@ static void scratch_free(o_scratch *sc) {
     free(sc->weight[d]); free(sc->edge[d]); free(sc->tbfi[d]);
   }
   free(sc->ecenter);
+  free(sc->temp_eint);
+  for (int d = 0; d < 3; d++) {
+    free(sc->eflux[d]); free(sc->vbar[d]);
+  }
+  free(sc->dUe);
   free(sc);
 }
 
@@ -1101,6 +1306,12 @@ static o_scratch *scratch_get(int mz, int my, int mx, int mhd) {
     sc->edge[d] = zalloc(ncc);
   }
   sc->ecenter = zalloc(ncc);
+  sc->temp_eint = zalloc(ncc);
+  for (int d = 0; d < 3; d++) {
+    sc->eflux[d] = zalloc(ncc);
+    sc->vbar[d] = zalloc(ncc);
+  }
+  sc->dUe = zalloc(ncc);
   sc->tbfi[0] = zalloc((long)mz * my * (mx + 1));
   sc->tbfi[1] = zalloc((long)mz * (my + 1) * mx);
   sc->tbfi[2] = zalloc((long)(mz + 1) * my * mx);
@@ -1138,7 +1349,8 @@ static void gravity_source(double dt, double *const *U0, double *const *acc,
 
 static void compute_update_stage(const o_config *cfg, o_block *b,
                                  o_scratch *sc, double **U0, double **Uc,
-                                 double **Uo, double **bfi_recon,
+                                 double **Uo, double *eint0, double *eint_c,
+                                 double *eint_o, double **bfi_recon,
                                  double **bfi_out, int stage, double cur_dt,
                                  int stale, double *const *accel) {
   const int mz = b->mz, my = b->my, mx = b->mx;
@@ -1147,8 +1359,10 @@ static void compute_update_stage(const o_config *cfg, o_block *b,
 
   /* clear_dUcons_map (EnzoIntegrationQuanUpdate.cpp:83-101) */
   for (int q = 0; q < 5; q++) memset(sc->dU[q], 0, ncc * sizeof(double));
+  const int de = cfg->dual_energy;
+  if (de) memset(sc->dUe, 0, ncc * sizeof(double));
 
-  primitive_from_integration(cfg, Uc, sc->prim, mz, my, mx, stale);
+  primitive_from_integration(cfg, Uc, eint_c, sc->prim, mz, my, mx, stale);
 
   for (int dim = 0; dim < 3; dim++) {
     /* reconstruct_interface: active keys are the primitive keys; floors on
@@ -1167,10 +1381,17 @@ static void compute_update_stage(const o_config *cfg, o_block *b,
                                    sc->pr[O_BX + dim], mz, my, mx, dim,
                                    cur_stale);
     }
-    o_riemann_solve(cfg, sc->pl, sc->pr, sc->flux[dim], mz, my, mx, dim,
+    riemann_solve_impl(cfg, sc->pl, sc->pr, sc->flux[dim],
+                       de ? sc->eflux[dim] : NULL, de ? sc->vbar[dim] : NULL,
+                       mz, my, mx, dim, cur_stale);
+    accumulate_flux(dim, cur_dt, b->cw[dim], sc->flux[dim], sc->dU,
+                    de ? sc->eflux[dim] : NULL, sc->dUe, mz, my, mx,
                     cur_stale);
-    accumulate_flux(dim, cur_dt, b->cw[dim], sc->flux[dim], sc->dU, mz, my,
-                    mx, cur_stale);
+    /* the internal-energy source of this dim
+       (EnzoMHDIntegratorStageCommands.cpp:248-255) */
+    if (de)
+      o_internal_energy_source(cfg, dim, cur_dt, b->cw[dim], sc->prim[O_ETOT],
+                               sc->dUe, sc->vbar[dim], mz, my, mx, cur_stale);
     if (cfg->mhd) {
       identify_upwind(sc->flux[dim][O_RHO], sc->weight[dim], mz, my, mx, dim,
                       cur_stale);
@@ -1200,7 +1421,8 @@ static void compute_update_stage(const o_config *cfg, o_block *b,
     }
   }
 
-  update_quantities(cfg, U0, sc->dU, Uo, mz, my, mx, stale);
+  update_quantities_de(cfg, U0, eint0, sc->dU, sc->dUe, Uo, eint_o, mz, my,
+                       mx, stale);
 }
 
 /* EnzoMethodMHDVlct::compute (EnzoMethodMHDVlct.cpp:356-500); with
@@ -1253,8 +1475,10 @@ static int vlct_compute_impl(const o_config *cfg, o_block *b, double dt,
       memset(flux_data, 0,
              (size_t)o_flux_data_size(nf, nx, ny, nz) * sizeof(double));
     }
-    compute_update_stage(cfg, b, sc, b->cc, Uc, Uo, bfi_recon, bfi_out, stage,
-                         cur_dt, stale, accel);
+    double *eint_c = (stage == 0) ? b->eint : sc->temp_eint;
+    double *eint_o = is_final ? b->eint : sc->temp_eint;
+    compute_update_stage(cfg, b, sc, b->cc, Uc, Uo, b->eint, eint_c, eint_o,
+                         bfi_recon, bfi_out, stage, cur_dt, stale, accel);
     stale += total_staling(cfg->recon[stage]);
     if (is_final && flux_data) {
       for (int d = 0; d < 3; d++)

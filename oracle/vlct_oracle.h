@@ -44,6 +44,9 @@ typedef struct {
   int nstages;      /* 2 = "vl", 1 = "euler"                            */
   int mhd;          /* 1 = constrained_transport, 0 = no_bfield         */
   double courant;
+  int dual_energy;  /* 1 = "modern" dual-energy formalism
+                       (EnzoDualEnergyConfig.hpp:40-47), 0 = disabled     */
+  double eta;       /* its eta (Physics:fluid_props:dual_energy:eta)      */
 } o_config;
 
 /* One Cello block: ghost-extended fields owned by the caller. */
@@ -54,6 +57,8 @@ typedef struct {
   double *bfi[3];          /* bfieldi_x (mz,my,mx+1), _y (mz,my+1,mx),
                               _z (mz+1,my,mx)                          */
   double *pressure;        /* permanent "pressure" field (mz,my,mx)   */
+  double *eint;            /* specific "internal_energy" (mz,my,mx);
+                              read and written with dual energy only     */
 } o_block;
 
 /* ---- whole-method entry points ---------------------------------------- */
@@ -148,6 +153,32 @@ void o_hlle_mhd_face(const double wl[8], const double wr[8], double gamma,
 void o_hllc_face(const double wl[5], const double wr[5], double gamma,
                  double flux[5]);
 
+/* Dual-energy outputs of the single-face solvers: the internal-energy flux
+ * of the passively advected specific internal energy
+ * (enzo_riemann_utils::passive_eint_flux, EnzoRiemannUtils.hpp:224-249)
+ * and the interface velocity vi_bar (EnzoRiemannHLLD.hpp:404-450,
+ * EnzoRiemannHLL.hpp:322-344, EnzoRiemannHLLC.hpp:159-170):
+ * de[0] = internal-energy flux, de[1] = vi_bar.                            */
+void o_hlld_face_de(const double wl[8], const double wr[8], double gamma,
+                    double flux[8], double de[2]);
+void o_hlle_mhd_face_de(const double wl[8], const double wr[8], double gamma,
+                        double flux[8], double de[2]);
+void o_hllc_face_de(const double wl[5], const double wr[5], double gamma,
+                    double flux[5], double de[2]);
+/* EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync
+ * (EnzoPhysicsFluidProps.cpp:158-290) over [s, m-s) of one block's
+ * integration quantities (U: O_* slots, eint used with dual energy).       */
+void o_floor_energy_and_sync(const o_config *cfg, double *const *U,
+                             double *eint, int mz, int my, int mx, int s);
+/* EnzoSourceInternalEnergy::calculate_source
+ * (toolkit/EnzoSourceInternalEnergy.cpp:16-92): cells [s+1, m_d-s-1) along
+ * dim, [s, m-s) across: deint(c) -= dt/dx * max(p(c), p_floor) *
+ * (vbar(c) - vbar(c - e_dim)); vbar is face-trimmed along dim.             */
+void o_internal_energy_source(const o_config *cfg, int dim, double dt,
+                              double cw, const double *p, double *deint,
+                              const double *vbar, int mz, int my, int mx,
+                              int s);
+
 /* slope limiters (EnzoReconstructorPLM.hpp:287-346) */
 double o_limiter_enzo(double vm1, double v, double vp1, double theta);
 double o_limiter_athena(double vm1, double v, double vp1);
@@ -185,6 +216,23 @@ void o_update_quantities(const o_config *cfg, double *const *U0,
                          int mx, int s);
 void o_primitive_from_integration(const o_config *cfg, double *const *U,
                                   double **W, int mz, int my, int mx, int s);
+/* the same with the dual-energy formalism: the internal-energy flux and
+ * vi_bar of every face, the "internal_energy" key of the flux divergence
+ * (eflux into dUe), its update as a specific quantity, pressure from eint */
+void o_riemann_solve_de(const o_config *cfg, double *const *pl,
+                        double *const *pr, double **flux, double *eflux,
+                        double *vbar, int mz, int my, int mx, int dim, int s);
+void o_accumulate_flux_component_de(int dim, double dt, double cw,
+                                    double **flux, double **dU,
+                                    const double *eflux, double *dUe, int mz,
+                                    int my, int mx, int s);
+void o_update_quantities_de(const o_config *cfg, double *const *U0,
+                            const double *eint0, double *const *dU,
+                            const double *dUe, double **Uo, double *eint_o,
+                            int mz, int my, int mx, int s);
+void o_primitive_from_integration_de(const o_config *cfg, double *const *U,
+                                     const double *eint, double **W, int mz,
+                                     int my, int mx, int s);
 
 
 /* ---- multi-block harness ------------------------------------------------ */
@@ -219,9 +267,16 @@ int o_init_inclined_wave(const o_config *cfg, o_grid *gr, const char *wave,
                          double alpha, double beta, double amplitude,
                          double lambda, int positive_vel);
 int o_init_shock_tube_rj2a(const o_config *cfg, o_grid *gr, int aligned_ax);
+/* setup 0 = "rj2a", 1 = "sod" with axis_velocity, transverse_velocity and
+ * flip_initialize (EnzoInitialShockTube.cpp:92-292); the specific internal
+ * energy is written when the blocks carry it (cfg->dual_energy).          */
+int o_init_shock_tube(const o_config *cfg, o_grid *gr, int setup,
+                      int aligned_ax, double axis_velocity,
+                      double transverse_velocity, int flip);
 
 /* Copy the active region of one field of the grid into a dense global array
- * (nz_tot, ny_tot, nx_tot).  field: 0..7 cc fields, 8 = derived pressure.  */
+ * (nz_tot, ny_tot, nx_tot).  field: 0..7 cc fields, 8 = derived pressure,
+ * 9 = internal_energy.                                                     */
 void o_grid_gather(const o_config *cfg, const o_grid *gr, int field,
                    double *out);
 
