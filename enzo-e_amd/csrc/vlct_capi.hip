@@ -134,6 +134,20 @@ int enzoe_vlct_create(const enzoe_vlct_params *p, enzoe_vlct_t *out) {
     return ENZOE_ERR_CONFIG;
   }
   m->precision = p->precision;
+  // the dual-energy formalism: "modern" or disabled
+  // (EnzoMHDIntegratorStageCommands.cpp:31-34, EnzoConfig.cpp:980-1008)
+  {
+    const std::string de = lower(p->dual_energy, "disabled");
+    if (de == "modern") {
+      m->dual = 1;
+      m->eta = (p->dual_energy_eta < 0) ? 0.001 : p->dual_energy_eta;
+      // the FP32 flux path carries no internal-energy terms
+      if (m->precision != ENZOE_PREC_DOUBLE) { delete m; return ENZOE_ERR_CONFIG; }
+    } else if (de != "disabled") {
+      delete m;
+      return ENZOE_ERR_CONFIG;
+    }
+  }
   // CT requires exactly 2 partial timesteps (EnzoBfieldMethod.cpp:23-27)
   if (m->mhd && m->nstages != 2) { delete m; return ENZOE_ERR_CONFIG; }
   {
@@ -186,7 +200,12 @@ int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
     const StoreArgs *rs = st ? &run_st : nullptr;
     const double *const *ra = accel ? accel + 3 * b0 : nullptr;
     int rc;
-    if (m->precision == ENZOE_PREC_DOUBLE) {
+    if (m->precision == ENZOE_PREC_DOUBLE && m->dual) {
+      rc = m->mhd ? compute_group<double, true, true>(m, blocks + b0, b1 - b0, dt, dts, s,
+                                                      rs, ra)
+                  : compute_group<double, false, true>(m, blocks + b0, b1 - b0, dt, dts, s,
+                                                       rs, ra);
+    } else if (m->precision == ENZOE_PREC_DOUBLE) {
       rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra)
                   : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, dts, s, rs,
                                                  ra);
@@ -277,7 +296,10 @@ namespace {
 // Launches the CFL reduction of the blocks into m->d_dt (bit pattern of
 // the unscaled min); the descriptor table is uploaded only when the block
 // list changed, so the call makes no host round trip otherwise.
-int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_t s) {
+// d_time / t_stop: the device time of a device-driven cycle and its stop
+// time (a cycle past it leaves the dual-energy fields alone), or null.
+int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_t s,
+              const double *d_time = nullptr, double t_stop = 0.0) {
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
   if (!m->d_dt &&
@@ -297,6 +319,7 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
     if (m->mhd)
       for (int a = 0; a < 3; a++) d.U[S_BX + a] = b.bfield[a];
     d.pressure = b.pressure;
+    if (m->dual) d.ei = b.internal_energy;
     d.mx = b.mx; d.my = b.my; d.mz = b.mz;
     for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
   }
@@ -311,6 +334,9 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
       return ENZOE_ERR_HIP;
     m->last_dt_desc.assign(raw, raw + dbytes);
   }
+  // with dual energy the reduction also writes the energies: a dry call
+  // (graph set-up) uploads the table only
+  if (m->dry) return ENZOE_OK;
   hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
   long maxcells = 0;
   for (int i = 0; i < n; i++)
@@ -320,12 +346,20 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
   const unsigned wg = (unsigned)std::max<long>(1, std::min<long>(nblk(maxcells, 256),
                                                                  std::max<long>(1, 2048 / n)));
   dim3 grid(wg, n);
-  if (m->mhd)
-    hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s,
-                       (const BlockDesc<double> *)m->d_ptrtab, m->gamma, m->d_dt);
+  const BlockDesc<double> *dd = (const BlockDesc<double> *)m->d_ptrtab;
+  const DualParams dp = {m->pfloor, m->eta, sync_ggm1(m->gamma), d_time, t_stop};
+  if (m->mhd && m->dual)
+    hipLaunchKernelGGL((k_dt<double, true, true>), grid, dim3(256), 0, s, dd, m->gamma,
+                       m->d_dt, dp);
+  else if (m->mhd)
+    hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s, dd, m->gamma, m->d_dt,
+                       dp);
+  else if (m->dual)
+    hipLaunchKernelGGL((k_dt<double, false, true>), grid, dim3(256), 0, s, dd, m->gamma,
+                       m->d_dt, dp);
   else
-    hipLaunchKernelGGL((k_dt<double, false>), grid, dim3(256), 0, s,
-                       (const BlockDesc<double> *)m->d_ptrtab, m->gamma, m->d_dt);
+    hipLaunchKernelGGL((k_dt<double, false>), grid, dim3(256), 0, s, dd, m->gamma, m->d_dt,
+                       dp);
   return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
 }
 
@@ -355,7 +389,7 @@ int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   if (!m || !blocks || n <= 0 || !d_time || !d_dts || ndts <= 0)
     return ENZOE_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  if (int rc = launch_dt(m, blocks, n, s)) return rc;
+  if (int rc = launch_dt(m, blocks, n, s, d_time, t_stop)) return rc;
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s,
                      (const unsigned long long *)m->d_dt, m->courant, d_time, t_stop,
                      d_dts, ndts);
@@ -373,7 +407,8 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   const size_t cells = (size_t)mx * my * mz;
   const size_t faces = (size_t)(mx + 1) * my * mz + (size_t)mx * (my + 1) * mz +
                        (size_t)mx * my * (mz + 1);
-  const size_t per = (6 * cells + (m->mhd ? 3 * cells + faces : 0)) * sizeof(double);
+  const size_t per =
+      ((6 + (m->dual ? 1 : 0)) * cells + (m->mhd ? 3 * cells + faces : 0)) * sizeof(double);
   int ns = 0;
   while (ns < n && blocks[ns].mx == mx && blocks[ns].my == my && blocks[ns].mz == mz &&
          (ns == 0 || (size_t)(ns + 1) * per <= ((size_t)2 << 30)))
@@ -399,6 +434,7 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
       for (int a = 0; a < 3; a++) take(b.velocity[a], blocks[i].velocity[a], cells);
       take(b.total_energy, blocks[i].total_energy, cells);
       take(b.pressure, blocks[i].pressure, cells);
+      if (m->dual) take(b.internal_energy, blocks[i].internal_energy, cells);
       if (m->mhd) {
         for (int a = 0; a < 3; a++) take(b.bfield[a], blocks[i].bfield[a], cells);
         for (int a = 0; a < 3; a++)
@@ -464,18 +500,25 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
     if (blocks[i].mx != blocks[0].mx || blocks[i].my != blocks[0].my ||
         blocks[i].mz != blocks[0].mz || blocks[i].ghost != G.g)
       return ENZOE_ERR_ARG;
+    if (m->dual && !blocks[i].internal_energy) return ENZOE_ERR_ARG;
   }
-  // one pointer table per field: 8 cc + 3 face
-  const int nf = m->mhd ? 11 : 5;
+  // one pointer table per field: the cell-centred fields, then for MHD the
+  // 3 face fields, then internal_energy with dual energy
+  const int ncc = m->mhd ? 8 : 5;
+  G.face0 = m->mhd ? ncc : (1 << 20);
+  const int nf = ncc + (m->mhd ? 3 : 0) + (m->dual ? 1 : 0);
   std::vector<double *> tab((size_t)nf * nb);
   for (int i = 0; i < nb; i++) {
     const enzoe_block &b = blocks[i];
-    double *f[11] = {b.density, b.velocity[0], b.velocity[1], b.velocity[2],
+    double *f[12] = {b.density, b.velocity[0], b.velocity[1], b.velocity[2],
                      b.total_energy, b.bfield[0], b.bfield[1], b.bfield[2],
-                     b.bfieldi[0], b.bfieldi[1], b.bfieldi[2]};
-    for (int q = 0; q < nf; q++) {
-      int slot = m->mhd ? q : q;  // hydro: first 5
+                     b.bfieldi[0], b.bfieldi[1], b.bfieldi[2], b.internal_energy};
+    int q = 0;
+    for (int slot = 0; slot < 12; slot++) {
+      if (!m->mhd && slot >= 5 && slot < 11) continue;
+      if (!m->dual && slot == 11) continue;
       tab[(size_t)q * nb + i] = f[slot];
+      q++;
     }
   }
   const size_t bytes = tab.size() * sizeof(double *);
@@ -488,6 +531,7 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
     if (hipStreamSynchronize(s) != hipSuccess) return ENZOE_ERR_HIP;
     m->last_rtab = tab;
   }
+  if (m->dry) return ENZOE_OK;  // graph set-up: the table only
   KernelTimer kt(s, "k_refresh");
   // largest ghost-element count over the field kinds (a face field has one
   // more interior layer and one more layer of extent along its axis)
@@ -615,15 +659,16 @@ int enzoe_vlct_graph_create(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
   if (hipMemcpy(g->d_state, &t0, sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return fail(ENZOE_ERR_HIP);
   // the graph's private buffers: take every buffer and upload every table
-  // now (the refresh is idempotent, the reduction writes only the pressure
-  // field), so the captured work allocates and uploads nothing
+  // now, so the captured work allocates and uploads nothing
   if (hipDeviceSynchronize() != hipSuccess) return fail(ENZOE_ERR_HIP);
   g->own.swap_with(m);  // the method now holds empty buffers
   swapped = true;
-  if ((rc = enzoe_grid_refresh(m, blocks, nbx, nby, nbz, bc, s))) return fail(rc);
-  if ((rc = launch_dt(m, blocks, n, s))) return fail(rc);
+  // (dry: nothing is launched, so the first replayed cycle sees the fields
+  // exactly as an eager cycle would, stale ghost zones included)
   m->dry = true;
-  rc = compute_runs(m, blocks, n, 0.0, g->d_state + 1, s);
+  rc = enzoe_grid_refresh(m, blocks, nbx, nby, nbz, bc, s);
+  if (!rc) rc = launch_dt(m, blocks, n, s);
+  if (!rc) rc = compute_runs(m, blocks, n, 0.0, g->d_state + 1, s);
   m->dry = false;
   if (rc) return fail(rc);
   if (hipStreamSynchronize(s) != hipSuccess) return fail(ENZOE_ERR_HIP);

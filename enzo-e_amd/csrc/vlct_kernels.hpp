@@ -37,6 +37,13 @@ struct BlockDesc {
   double *E[3];        // edge EMFs along x, y, z
   double *Ec[3];       // cell-centred E = -v x B of the stage's state
   const double *acc[3];  // acceleration_x..z for the gravity source, or null
+  // dual-energy formalism (null otherwise): specific internal_energy (start
+  // of step / final output), its half-step copy, and per dim the
+  // internal-energy flux and the interface velocity vi_bar of every face
+  double *ei;
+  double *eit;
+  double *Fe[3];
+  double *Vb[3];
   int mx, my, mz;
   int g;               // ghost depth
   double cw[3];
@@ -56,7 +63,46 @@ struct StageParams {
   double dt_factor;    // 0.5 for the partial step, 1 for the final one
   int xcd_remap;       // bit mask over k_flux(1) k_edge(2) k_faces(4)
                        // k_cells(8): XCD-contiguous order, see wg_coords
+  double eta;          // dual-energy eta ("modern" formulation)
+  double ggm1;         // gamma (gamma - 1) rounded to float, as the
+                       // reference's synchronisation declares it
 };
+
+// Energy floor and, with the dual-energy formalism, the synchronisation of
+// internal and total energy of one cell
+// (EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync,
+// EnzoPhysicsFluidProps.cpp:158-290).  rho, v, b2 are the cell's (updated)
+// density, velocity and |B|^2 (0 for hydro); etot and eint are updated in
+// place.  The internal energy is taken from etot - non-thermal energy where
+// the sound speed that implies beats eta times the kinetic and magnetic
+// scales (and it exceeds half the tracked value unless eta = 0).
+template <bool MHD, bool DE>
+__device__ __forceinline__ void floor_and_sync(double rho, const double v[3],
+                                               double b2, double gamma,
+                                               double pfloor, double eta,
+                                               double ggm1, double &etot,
+                                               double &eint) {
+  const double inv_rho = 1. / rho;
+  const double eint_floor = pfloor * (1. / (gamma - 1.)) * inv_rho;
+  const double v2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  double non_thermal_e = 0.5 * v2;
+  if (MHD) non_thermal_e += (0.5 * b2 * inv_rho);
+  if (DE) {
+    const double half_factor = (eta != 0.) ? 0.5 : 0.;
+    const double eint_1 = etot - non_thermal_e;
+    double cur_eint = eint;
+    const double cs2_1 = fmax(0., ggm1 * eint_1);
+    if ((cs2_1 > fmax(eta * v2, eta * (MHD ? b2 : 0.) * inv_rho)) &&
+        (eint_1 > half_factor * cur_eint))
+      cur_eint = eint_1;
+    cur_eint = dmax(cur_eint, eint_floor);
+    eint = cur_eint;
+    etot = cur_eint + non_thermal_e;
+  } else {
+    const double etot_floor = eint_floor + non_thermal_e;
+    etot = dmax(etot, etot_floor);
+  }
+}
 
 __device__ __forceinline__ long idx3(int k, int j, int i, int ny, int nx) {
   return ((long)k * ny + j) * (long)nx + i;
@@ -118,10 +164,11 @@ __device__ __forceinline__ void gst(P *p, long i, P v) {
 
 // Cell primitives w = (rho, vx, vy, vz, p, bx, by, bz) from the integration
 // quantities (EnzoPhysicsFluidProps.cpp:64-138, EnzoComputePressure.cpp:
-// 177-189), in FP64.
-template <bool MHD>
+// 127-189), in FP64; with dual energy the pressure is gm1 rho eint.
+template <bool MHD, bool DE = false>
 __device__ __forceinline__ void load_prim(double *const U[S_N], int c,
-                                          double gm1, double w[S_N]) {
+                                          double gm1, double w[S_N],
+                                          const double *ei = nullptr) {
   double rho = gld(U[S_RHO], c);
   double vx = gld(U[S_VX], c), vy = gld(U[S_VY], c), vz = gld(U[S_VZ], c);
   double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
@@ -134,7 +181,10 @@ __device__ __forceinline__ void load_prim(double *const U[S_N], int c,
   } else {
     w[S_BX] = 0.; w[S_BY] = 0.; w[S_BZ] = 0.;
   }
-  w[S_E] = gm1 * (rho * (gld(U[S_E], c) - ke) - me);
+  if (DE)
+    w[S_E] = gm1 * rho * gld(ei, c);
+  else
+    w[S_E] = gm1 * (rho * (gld(U[S_E], c) - ke) - me);
 }
 
 // ---------------------------------------------------------------- k_flux
@@ -147,25 +197,27 @@ struct FluxConsts {
 // time so the i/j/k permutation costs nothing): reconstruction of both
 // sides, the face-centred normal B (EnzoBfieldMethodCT.cpp:122-166) and the
 // Riemann solve, written at index c of the flux arrays of dim D.
-template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD, int D>
+template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD, int D,
+          bool DE = false>
 __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
                                           double *const *U,
                                           const double *bfi, int k, int j,
                                           int i, int c, const double wc[S_N],
-                                          const FluxConsts &fc) {
+                                          const FluxConsts &fc,
+                                          const double *ei = nullptr) {
   constexpr int DJ = (D + 1) % 3, DK = (D + 2) % 3;
   const int s = (D == 0) ? 1 : ((D == 1) ? b.mx : b.mx * b.my);
   T wl[S_N], wr[S_N];
   if (RECON == RC_NN) {
     double a[S_N];
-    load_prim<MHD>(U, c + s, fc.gm1, a);
+    load_prim<MHD, DE>(U, c + s, fc.gm1, a, ei);
 #pragma unroll
     for (int q = 0; q < S_N; q++) { wl[q] = (T)wc[q]; wr[q] = (T)a[q]; }
   } else {
     double w0[S_N], w2[S_N], w3[S_N];  // cells c-1, c+1, c+2 along D
-    load_prim<MHD>(U, c - s, fc.gm1, w0);
-    load_prim<MHD>(U, c + s, fc.gm1, w2);
-    load_prim<MHD>(U, c + 2 * s, fc.gm1, w3);
+    load_prim<MHD, DE>(U, c - s, fc.gm1, w0, ei);
+    load_prim<MHD, DE>(U, c + s, fc.gm1, w2, ei);
+    load_prim<MHD, DE>(U, c + 2 * s, fc.gm1, w3, ei);
     const T theta = (T)fc.theta;
 #pragma unroll
     for (int q = 0; q < S_N; q++) {
@@ -192,6 +244,8 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
   }
   const T gamma = (T)fc.gamma;
   T fo[8];
+  T vbar = T(0);
+  T *vb = DE ? &vbar : nullptr;
   if (MHD) {
     const int bnx = b.mx + (D == 0), bny = b.my + (D == 1);
     const T bface =
@@ -206,9 +260,9 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
     l8[L_VK] = wl[S_VX + DK]; r8[L_VK] = wr[S_VX + DK];
     l8[L_E] = wl[S_E]; r8[L_E] = wr[S_E];
     if (RIEMANN == RS_HLLD) {
-      hlld(l8, r8, gamma, fo);
+      hlld(l8, r8, gamma, fo, vb);
     } else {
-      hlle_mhd(l8, r8, gamma, fo);
+      hlle_mhd(l8, r8, gamma, fo, vb);
     }
     gst(b.F[D][S_RHO], c, (FluxT)fo[L_D]);
     gst(b.F[D][S_VX + D], c, (FluxT)fo[L_VI]);
@@ -224,12 +278,20 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
     l5[H_VJ] = wl[S_VX + DJ]; r5[H_VJ] = wr[S_VX + DJ];
     l5[H_VK] = wl[S_VX + DK]; r5[H_VK] = wr[S_VX + DK];
     l5[H_E] = wl[S_E]; r5[H_E] = wr[S_E];
-    hllc(l5, r5, gamma, fo);
+    hllc(l5, r5, gamma, fo, vb);
     gst(b.F[D][S_RHO], c, (FluxT)fo[H_D]);
     gst(b.F[D][S_VX + D], c, (FluxT)fo[H_VI]);
     gst(b.F[D][S_VX + DJ], c, (FluxT)fo[H_VJ]);
     gst(b.F[D][S_VX + DK], c, (FluxT)fo[H_VK]);
     gst(b.F[D][S_E], c, (FluxT)fo[H_E]);
+  }
+  if (DE) {
+    // the dual-energy outputs of the solve (EnzoRiemannHLLD.hpp:404-447,
+    // EnzoRiemannHLL.hpp:322-344, EnzoRiemannHLLC.hpp:159-170), from the
+    // stored density flux (slot 0 of either LUT)
+    gst(b.Fe[D], c, (double)passive_eint_flux(wl[S_RHO], wl[S_E], wr[S_RHO], wr[S_E],
+                                              gamma, fo[0]));
+    gst(b.Vb[D], c, (double)vbar);
   }
 }
 
@@ -242,7 +304,8 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
 #define ENZOE_FLUX_MIN_WAVES 3
 #endif
 
-template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD>
+template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD,
+          bool DE = false>
 __global__ void __launch_bounds__(256, ENZOE_FLUX_MIN_WAVES)
 k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
@@ -270,8 +333,9 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   fc.dfloor = sp.density_floor;
   fc.pfloor = sp.pressure_floor;
   const int c = (k * my + j) * mx + i;
+  const double *ei = DE ? ((sp.stage == 0) ? b.ei : b.eit) : nullptr;
   double wc[S_N];
-  load_prim<MHD>(U, c, fc.gm1, wc);
+  load_prim<MHD, DE>(U, c, fc.gm1, wc, ei);
   if (MHD) {
     // centre E_i = -v_j B_k + v_k B_j of this cell for the edge kernel
     // (EnzoBfieldMethodCT::compute_center_efield, EnzoBfieldMethodCT.cpp:
@@ -281,11 +345,11 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     gst(b.Ec[2], c, (-wc[S_VX] * wc[S_BY] + wc[S_VY] * wc[S_BX]));
   }
   if (i < mx - 1 - cs)
-    flux_face<T, FluxT, RECON, RIEMANN, MHD, 0>(b, U, bfr[0], k, j, i, c, wc, fc);
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 0, DE>(b, U, bfr[0], k, j, i, c, wc, fc, ei);
   if (j < my - 1 - cs)
-    flux_face<T, FluxT, RECON, RIEMANN, MHD, 1>(b, U, bfr[1], k, j, i, c, wc, fc);
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 1, DE>(b, U, bfr[1], k, j, i, c, wc, fc, ei);
   if (k < mz - 1 - cs)
-    flux_face<T, FluxT, RECON, RIEMANN, MHD, 2>(b, U, bfr[2], k, j, i, c, wc, fc);
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 2, DE>(b, U, bfr[2], k, j, i, c, wc, fc, ei);
 }
 
 // ---------------------------------------------------------------- k_edge
@@ -404,7 +468,14 @@ k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
 // EnzoIntegrationQuanUpdate.cpp:105-147), the conserved update with the
 // density floor (update_quantities, :183-238) and the energy floor
 // (EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync, :162-290).
-template <typename FluxT, bool MHD>
+// With the dual-energy formalism the internal energy is updated as a
+// specific quantity: per dim its flux divergence, then the source
+// -dt/dx p (vi_bar_r - vi_bar_l) with the stage's cell pressure
+// (EnzoSourceInternalEnergy::calculate_source, EnzoSourceInternalEnergy.cpp:
+// 16-92, called after each dim's divergence,
+// EnzoMHDIntegratorStageCommands.cpp:248-255), and the energies are
+// synchronised instead of floored.
+template <typename FluxT, bool MHD, bool DE = false>
 __global__ void __launch_bounds__(256)
 k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
@@ -443,6 +514,15 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   double dU[5];
 #pragma unroll
   for (int q = 0; q < 5; q++) dU[q] = 0.;
+  double dUe = 0.;
+  double pc = 0.;
+  if (DE) {
+    // the stage's primitive pressure, floored (EnzoSourceInternalEnergy.cpp:
+    // 75-88)
+    double *const *Us = (sp.stage == 0) ? b.U : b.Ut;
+    const double *es = (sp.stage == 0) ? b.ei : b.eit;
+    pc = dmax((sp.gamma - 1.0) * Us[S_RHO][c] * es[c], sp.pressure_floor);
+  }
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const double dtdx = dt / b.cw[d];
@@ -451,6 +531,10 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
       const FluxT *F = b.F[d][fq[q]];
       double fr = (double)F[c], fl = (double)F[c - str[d]];
       dU[q] -= dtdx * (fr - fl);
+    }
+    if (DE) {
+      dUe -= dtdx * (b.Fe[d][c] - b.Fe[d][c - str[d]]);
+      dUe -= dtdx * pc * (b.Vb[d][c] - b.Vb[d][c - str[d]]);
     }
   }
   // gravity (EnzoSourceGravity::calculate_source, EnzoSourceGravity.cpp:
@@ -476,22 +560,18 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   for (int q = 1; q < 4; q++)
     v[q - 1] = (b.U[fq[q]][c] * old_rho + dU[q]) * inv_new_rho;
   double etot = (b.U[S_E][c] * old_rho + dU[4]) * inv_new_rho;
-  // energy floor
-  double inv_rho = 1. / new_rho;
-  double eint_floor = sp.pressure_floor * (1. / (sp.gamma - 1.)) * inv_rho;
-  double v2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  double non_thermal_e = 0.5 * v2;
-  if (MHD) {
-    double b2 = (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
-    non_thermal_e += (0.5 * b2 * inv_rho);
-  }
-  double etot_floor = eint_floor + non_thermal_e;
-  etot = dmax(etot, etot_floor);
+  double eint = 0.;
+  if (DE) eint = (b.ei[c] * old_rho + dUe) * inv_new_rho;
+  // energy floor (and synchronisation)
+  const double b2 = MHD ? (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]) : 0.;
+  floor_and_sync<MHD, DE>(new_rho, v, b2, sp.gamma, sp.pressure_floor, sp.eta, sp.ggm1,
+                          etot, eint);
   Uo[S_RHO][c] = new_rho;
   Uo[S_VX][c] = v[0];
   Uo[S_VY][c] = v[1];
   Uo[S_VZ][c] = v[2];
   Uo[S_E][c] = etot;
+  if (DE) (sp.is_final ? b.ei : b.eit)[c] = eint;
 }
 
 // ----------------------------------------------------------------- k_dt
@@ -503,13 +583,23 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
 // it).  EnzoMHDIntegratorStageCommands::timestep
 // (EnzoMHDIntegratorStageCommands.cpp:299-366).  Grid-stride loop; wave64
 // butterfly, then LDS across the workgroup, then one atomic per workgroup.
-template <typename FluxT, bool MHD>
+// Dual-energy parameters of the CFL kernel's synchronisation.  time (device,
+// may be null) and t_stop: a device-driven cycle that starts at or past the
+// stop time is zero-length and leaves the energies alone.
+struct DualParams {
+  double pressure_floor, eta, ggm1;
+  const double *time;
+  double t_stop;
+};
+
+template <typename FluxT, bool MHD, bool DE = false>
 __global__ void __launch_bounds__(256)
 k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
-     unsigned long long *result) {
+     unsigned long long *result, DualParams dp) {
   const BlockDesc<FluxT> &b = blocks[blockIdx.y];
   const long n = (long)b.mx * b.my * b.mz;
   double local = 1.7976931348623157e308;
+  const bool live = !DE || dp.time == nullptr || *dp.time < dp.t_stop;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (long)gridDim.x * blockDim.x) {
     double d = b.U[S_RHO][c];
@@ -521,7 +611,24 @@ k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
       bx = b.U[S_BX][c]; by = b.U[S_BY][c]; bz = b.U[S_BZ][c];
       me = 0.5 * (bx * bx + by * by + bz * bz);
     }
-    double p = (gamma - 1.0) * (d * (b.U[S_E][c] - ke) - me);
+    double p;
+    if (DE) {
+      // the block's energies are synchronised first, in place
+      // (EnzoMethodMHDVlct.cpp:563-568), then p = gm1 rho eint
+      // (EnzoComputePressure.cpp:127-134)
+      const double v[3] = {vx, vy, vz};
+      const double b2 = MHD ? (bx * bx + by * by + bz * bz) : 0.;
+      double etot = b.U[S_E][c], eint = b.ei[c];
+      if (live) {
+        floor_and_sync<MHD, true>(d, v, b2, gamma, dp.pressure_floor, dp.eta, dp.ggm1,
+                                  etot, eint);
+        b.U[S_E][c] = etot;
+        b.ei[c] = eint;
+      }
+      p = (gamma - 1.0) * d * eint;
+    } else {
+      p = (gamma - 1.0) * (d * (b.U[S_E][c] - ke) - me);
+    }
     if (b.pressure) b.pressure[c] = p;
     double sig;
     if (MHD) {
@@ -584,6 +691,8 @@ struct RefreshGrid {
   int n[3];      // active cells per block
   int g;         // ghost depth
   int bc[3];     // 0 periodic, 1 outflow
+  int face0;     // table index of bfieldi_x (then _y, _z); every other
+                 // field is cell-centred
 };
 
 __device__ __forceinline__ bool refresh_axis(const RefreshGrid &G, int a,
@@ -660,15 +769,15 @@ __device__ __forceinline__ bool ghost_coords(long t, const int ext[3],
   return true;
 }
 
-// table[f * nb + b] = device pointer of field f of block b; fields 0..7 are
-// cell-centred, 8..10 the face-centred bfieldi_x..z.  grid: x = ghost
-// elements, y = block, z = field.
+// table[f * nb + b] = device pointer of field f of block b; fields
+// face0..face0+2 are the face-centred bfieldi_x..z, the others cell-centred.
+// grid: x = ghost elements, y = block, z = field.
 static __global__ void __launch_bounds__(256)
 k_refresh(double *const *__restrict__ table, RefreshGrid G) {
   const int nb = G.nb[0] * G.nb[1] * G.nb[2];
   const int bidx = blockIdx.y;
   const int f = blockIdx.z;
-  const int fc = (f >= 8) ? f - 8 : -1;
+  const int fc = (f >= G.face0 && f < G.face0 + 3) ? f - G.face0 : -1;
   const int bx = bidx % G.nb[0];
   const int by = (bidx / G.nb[0]) % G.nb[1];
   const int bz = bidx / (G.nb[0] * G.nb[1]);

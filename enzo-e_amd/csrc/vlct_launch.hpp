@@ -76,6 +76,9 @@ struct enzoe_vlct_method {
   int xcd_remap = 2;
   double theta = 1.5, courant = 0.3, gamma = 5.0 / 3.0;
   double dfloor = 1e-200, pfloor = 1e-200;
+  // dual-energy formalism ("modern" formulation) and its eta
+  int dual = 0;
+  double eta = 0.001;
 
   // scratch pool (grown on demand)
   void *scratch = nullptr;
@@ -127,7 +130,7 @@ inline int parse_recon(const std::string &n, int *out) {
 
 // bytes of scratch per block
 template <typename FluxT>
-inline size_t scratch_per_block(int mx, int my, int mz, int mhd) {
+inline size_t scratch_per_block(int mx, int my, int mz, int mhd, int dual = 0) {
   size_t ncc = (size_t)mx * my * mz;
   size_t nq = mhd ? 8 : 5;
   size_t bytes = 0;
@@ -138,6 +141,7 @@ inline size_t scratch_per_block(int mx, int my, int mz, int mhd) {
     bytes += 6 * ncc * sizeof(double);                 // E, Ec
   }
   bytes += 3 * (mhd ? 7 : 5) * ncc * sizeof(FluxT);    // F
+  if (dual) bytes += 7 * ncc * sizeof(double);         // eit, Fe, Vb
   return bytes;
 }
 
@@ -184,6 +188,8 @@ inline int check_block(const enzoe_vlct_method *m, const enzoe_block &b) {
     for (int d = 0; d < 3; d++)
       if (!b.bfield[d] || !b.bfieldi[d]) return ENZOE_ERR_ARG;
   }
+  // the dual-energy formalism integrates the internal_energy field
+  if (m->dual && !b.internal_energy) return ENZOE_ERR_ARG;
   return 0;
 }
 
@@ -242,37 +248,54 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
         base += ncc * sizeof(FluxT);
       }
     }
+    if (m->dual) {
+      d.ei = b.internal_energy;
+      d.eit = (double *)base;
+      base += ncc * sizeof(double);
+      for (int a = 0; a < 3; a++) {
+        d.Fe[a] = (double *)base;
+        base += ncc * sizeof(double);
+        d.Vb[a] = (double *)base;
+        base += ncc * sizeof(double);
+      }
+    }
     out.push_back(d);
   }
 }
 
-template <typename FluxT, int RECON, int RIEMANN, bool MHD>
+template <typename FluxT, int RECON, int RIEMANN, bool MHD, bool DE>
 void launch_flux(const BlockDesc<FluxT> *d, int nb, long nthreads,
                  StageParams sp, hipStream_t s) {
   dim3 grid(nblk8(nthreads, 256), nb);
-  hipLaunchKernelGGL((k_flux<FluxT, FluxT, RECON, RIEMANN, MHD>), grid,
+  hipLaunchKernelGGL((k_flux<FluxT, FluxT, RECON, RIEMANN, MHD, DE>), grid,
                      dim3(256), 0, s, d, sp);
 }
 
-template <typename FluxT, bool MHD>
+template <typename FluxT, bool MHD, bool DE>
 void dispatch_flux(const enzoe_vlct_method *m, int recon,
                    const BlockDesc<FluxT> *d, int nb, long nthreads,
                    StageParams sp, hipStream_t s) {
   if (MHD) {
     if (m->riemann == RS_HLLD) {
-      if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLD, MHD>(d, nb, nthreads, sp, s);
-      else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLD, MHD>(d, nb, nthreads, sp, s);
-      else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLD, MHD>(d, nb, nthreads, sp, s);
+      if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLD, MHD, DE>(d, nb, nthreads, sp, s);
+      else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLD, MHD, DE>(d, nb, nthreads, sp, s);
+      else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLD, MHD, DE>(d, nb, nthreads, sp, s);
     } else {
-      if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLE, MHD>(d, nb, nthreads, sp, s);
-      else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLE, MHD>(d, nb, nthreads, sp, s);
-      else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD>(d, nb, nthreads, sp, s);
+      if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLE, MHD, DE>(d, nb, nthreads, sp, s);
+      else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLE, MHD, DE>(d, nb, nthreads, sp, s);
+      else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD, DE>(d, nb, nthreads, sp, s);
     }
   } else {
-    if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLC, MHD>(d, nb, nthreads, sp, s);
-    else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLC, MHD>(d, nb, nthreads, sp, s);
-    else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLC, MHD>(d, nb, nthreads, sp, s);
+    if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLC, MHD, DE>(d, nb, nthreads, sp, s);
+    else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLC, MHD, DE>(d, nb, nthreads, sp, s);
+    else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLC, MHD, DE>(d, nb, nthreads, sp, s);
   }
+}
+
+// gamma (gamma - 1) as the reference's synchronisation holds it: a float
+// (EnzoPhysicsFluidProps.cpp:236)
+inline double sync_ggm1(double gamma) {
+  return (double)(float)(gamma * (gamma - 1.));
 }
 
 
@@ -303,13 +326,15 @@ struct StoreArgs {
 // One VL+CT step (EnzoMethodMHDVlct::compute) on blocks of equal shape.
 // dts: per-block dt in device memory (n values) or null (all blocks use dt).
 // st: store the final stage's face fluxes (hydro only) or null.
-template <typename FluxT, bool MHD>
+// DE: the dual-energy formalism (FP64 path only).
+template <typename FluxT, bool MHD, bool DE = false>
 int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   double dt, const double *dts, hipStream_t s,
                   const StoreArgs *st = nullptr,
                   const double *const *accel = nullptr) {
   const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
-  const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd);
+  if (DE != (m->dual != 0)) return ENZOE_ERR_CONFIG;
+  const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd, m->dual);
   // chunk size: as many blocks as fit the scratch budget (at least 1); the
   // chunks run one after the other on the stream and share the pool
   if (m->scratch_budget == 0) {
@@ -368,10 +393,12 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       sp.pressure_floor = m->pfloor;
       sp.theta = m->theta;
       sp.xcd_remap = launch_mask(m, mx, my, mz);
+      sp.eta = m->eta;
+      sp.ggm1 = sync_ggm1(m->gamma);
       const int cs = sp.cur_stale;
       {
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm");
-        dispatch_flux<FluxT, MHD>(m, recon, d, nb, cube(mx, my, mz, cs, cs), sp, s);
+        dispatch_flux<FluxT, MHD, DE>(m, recon, d, nb, cube(mx, my, mz, cs, cs), sp, s);
       }
       if (MHD) {
         {
@@ -389,7 +416,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       {
         KernelTimer kt(s, "k_cells");
         const int lo = is_final ? cs : cs + 1;
-        hipLaunchKernelGGL((k_cells<FluxT, MHD>),
+        hipLaunchKernelGGL((k_cells<FluxT, MHD, DE>),
                            dim3(nblk8(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
                            s, d, sp);
       }

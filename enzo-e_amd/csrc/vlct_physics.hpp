@@ -87,12 +87,28 @@ __device__ __forceinline__ T fast_ms_speed(T d, T p, T bi, T bj, T bk,
 
 template <typename T> struct Cons1D { T d, mx, my, mz, e, by, bz; };
 
+// enzo_riemann_utils::passive_eint_flux (EnzoRiemannUtils.hpp:224-249):
+// the specific internal energy p / ((gamma-1) rho) of the side the density
+// flux comes from, selected branch-free as the reference writes it, times
+// the density flux (dual-energy formalism)
+template <typename T>
+__device__ __forceinline__ T passive_eint_flux(T density_l, T pressure_l,
+                                               T density_r, T pressure_r,
+                                               T gamma, T density_flux) {
+  const T eint_l = qdiv(pressure_l, (gamma - T(1)) * density_l);
+  const T eint_r = qdiv(pressure_r, (gamma - T(1)) * density_r);
+  const T upwind = T(density_flux > T(0)) * eint_l + T(density_flux <= T(0)) * eint_r;
+  return upwind * density_flux;
+}
+
 // HLLD (Miyoshi & Kusano 2005) following HLLDKernel::operator()
 // (src/Enzo/hydro-mhd/riemann/EnzoRiemannHLLD.hpp:40-404).  In/out in the
-// MHD LUT order; flux[L_BI] is 0.
+// MHD LUT order; flux[L_BI] is 0.  vbar (dual energy, may be null): the
+// interface velocity of :418-447 -- the left/right normal velocity outside
+// the fan, else the contact speed scaled by the outer state's coefficient.
 template <typename T>
 __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
-                                     T flx[8]) {
+                                     T flx[8], T *vbar = nullptr) {
   const T SMALL_NUMBER = T(1.0e-8);
   const T igm1 = qdiv(T(1), gamma - T(1));
   T spd[5];
@@ -302,6 +318,20 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     flx[L_BK] = fr.bz + spd[4] * (urst.bz - ur.bz);
   }
   flx[L_BI] = T(0);
+  if (vbar) {
+    const T S_M = spd[2], S_l = spd[0], S_r = spd[4];
+    const T l_coef = qdiv(S_l - wli[L_VI], S_l - S_M);
+    const T r_coef = qdiv(S_r - wri[L_VI], S_r - S_M);
+    if (S_l > T(0)) {
+      *vbar = wli[L_VI];
+    } else if (S_r < T(0)) {
+      *vbar = wri[L_VI];
+    } else if (S_M >= T(0)) {
+      *vbar = S_M * l_coef;
+    } else {
+      *vbar = S_M * r_coef;
+    }
+  }
 }
 
 // conserved state + physical flux, MHD LUT (EnzoRiemannUtils.hpp:48-149)
@@ -335,10 +365,12 @@ __device__ __forceinline__ void mhd_cons_flux(const T w[8], T gamma, T cons[8],
 }
 
 // HLLE for MHD: HLLKernel<EinfeldtWavespeed<MHDLUT>>
-// (EnzoRiemannHLL.hpp:44-172, 238-320)
+// (EnzoRiemannHLL.hpp:44-172, 238-320); vbar (dual energy, may be null):
+// the normal velocity interpolated between the outer speeds (:330-340)
 template <typename T>
 __device__ __forceinline__ void hlle_mhd(const T wl[8], const T wr[8],
-                                         T gamma, T flux[8]) {
+                                         T gamma, T flux[8],
+                                         T *vbar = nullptr) {
   T cl[8], cr[8], fl[8], fr[8];
   mhd_cons_flux(wl, gamma, cl, fl);
   mhd_cons_flux(wr, gamma, cr, fr);
@@ -393,13 +425,15 @@ __device__ __forceinline__ void hlle_mhd(const T wl[8], const T wr[8],
     flux[f] = ((bp * fl[f] - bm * fr[f] + (cr[f] - cl[f]) * bp * bm) *
                inv_speed_diff);
   }
+  if (vbar) *vbar = (bp * wl[L_VI] - bm * wr[L_VI]) * inv_speed_diff;
 }
 
 // HLLC (HLLCKernel::operator(), EnzoRiemannHLLC.hpp:34-172) with
 // EinfeldtWavespeed<HydroLUT> (EnzoRiemannHLL.hpp:44-130).  Hydro LUT.
+// vbar (dual energy, may be null): the weighted outer velocities of :166-170
 template <typename T>
 __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T gamma,
-                                     T flux[5]) {
+                                     T flux[5], T *vbar = nullptr) {
   const T pressure_l = wl[H_E];
   const T pressure_r = wr[H_E];
   T cons_l[5], cons_r[5];
@@ -476,6 +510,7 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T gamma,
   flux[H_VJ] = sl * vfl + sr * vfr;
   flux[H_VK] = sl * wfl + sr * wfr;
   flux[H_E] = sl * efl + sr * efr + (sm * cp * cw);
+  if (vbar) *vbar = (sl * (wl[H_VI] - bm) + sr * (wr[H_VI] - bp));
 }
 
 }  // namespace enzoe

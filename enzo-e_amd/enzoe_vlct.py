@@ -40,7 +40,9 @@ class Params(ctypes.Structure):
                 ("gamma", ctypes.c_double),
                 ("density_floor", ctypes.c_double),
                 ("pressure_floor", ctypes.c_double),
-                ("precision", ctypes.c_int)]
+                ("precision", ctypes.c_int),
+                ("dual_energy", ctypes.c_char_p),
+                ("dual_energy_eta", ctypes.c_double)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -55,7 +57,8 @@ class Block(ctypes.Structure):
                 ("total_energy", ctypes.c_void_p),
                 ("bfield", ctypes.c_void_p * 3),
                 ("bfieldi", ctypes.c_void_p * 3),
-                ("pressure", ctypes.c_void_p)]
+                ("pressure", ctypes.c_void_p),
+                ("internal_energy", ctypes.c_void_p)]
 
 
 _lib = None
@@ -184,12 +187,14 @@ def from_device(src_ptr, shape):
     return a
 
 
-def field_shapes(mx, my, mz, mhd):
+def field_shapes(mx, my, mz, mhd, dual_energy=False):
     shapes = {}
     names = CC_FIELDS if mhd else CC_FIELDS[:5]
     for f in names:
         shapes[f] = (mz, my, mx)
     shapes["pressure"] = (mz, my, mx)
+    if dual_energy:
+        shapes["internal_energy"] = (mz, my, mx)
     if mhd:
         shapes["bfieldi_x"] = (mz, my, mx + 1)
         shapes["bfieldi_y"] = (mz, my + 1, mx)
@@ -200,10 +205,11 @@ def field_shapes(mx, my, mz, mhd):
 class DeviceBlock:
     """One block's permanent fields in one device allocation."""
 
-    def __init__(self, mx, my, mz, ghost, cell_width, mhd=True):
+    def __init__(self, mx, my, mz, ghost, cell_width, mhd=True, dual_energy=False):
         self.mx, self.my, self.mz, self.ghost = mx, my, mz, ghost
         self.mhd = mhd
-        self.shapes = field_shapes(mx, my, mz, mhd)
+        self.dual_energy = dual_energy
+        self.shapes = field_shapes(mx, my, mz, mhd, dual_energy)
         self.offsets = {}
         off = 0
         for name, shp in self.shapes.items():
@@ -225,6 +231,8 @@ class DeviceBlock:
                 b.bfieldi[a] = base + self.offsets["bfieldi_" + c]
         b.total_energy = base + self.offsets["total_energy"]
         b.pressure = base + self.offsets["pressure"]
+        if dual_energy:
+            b.internal_energy = base + self.offsets["internal_energy"]
         self.desc = b
 
     def ptr(self, name):
@@ -249,7 +257,9 @@ class VlctMethod:
     def __init__(self, mhd=True, riemann="hlld", reconstruct="plm",
                  time_scheme="vl", theta=1.5, courant=None, gamma=5.0 / 3.0,
                  density_floor=1e-200, pressure_floor=1e-200,
-                 precision=ENZOE_PREC_DOUBLE):
+                 precision=ENZOE_PREC_DOUBLE, dual_energy_eta=None):
+        """dual_energy_eta: None disables the dual-energy formalism; a
+        number selects the "modern" formulation with that eta."""
         p = Params()
         p.mhd_choice = b"constrained_transport" if mhd else b"no_bfield"
         p.riemann_solver = riemann.encode() if riemann else None
@@ -261,7 +271,10 @@ class VlctMethod:
         p.density_floor = density_floor
         p.pressure_floor = pressure_floor
         p.precision = precision
+        p.dual_energy = b"disabled" if dual_energy_eta is None else b"modern"
+        p.dual_energy_eta = -1.0 if dual_energy_eta is None else dual_energy_eta
         self.mhd = mhd
+        self.dual_energy = dual_energy_eta is not None
         self.handle = ctypes.c_void_p()
         check(lib().enzoe_vlct_create(ctypes.byref(p), ctypes.byref(self.handle)),
               "enzoe_vlct_create")
@@ -349,7 +362,9 @@ class DeviceGrid:
         nb = nblocks[0] * nblocks[1] * nblocks[2]
         self.blocks = [DeviceBlock(m[0], m[1], m[2], ghost, cell_width[i] if
                                    isinstance(cell_width[0], (list, tuple)) else cell_width,
-                                   mhd=method.mhd) for i in range(nb)]
+                                   mhd=method.mhd,
+                                   dual_energy=getattr(method, "dual_energy", False))
+                       for i in range(nb)]
 
     def upload_from(self, arrays_per_block):
         for blk, arrs in zip(self.blocks, arrays_per_block):
@@ -707,6 +722,7 @@ def host_lib():
         H = ctypes.CDLL(HOST_LIB_PATH)
         vp = ctypes.c_void_p
         H.enzoe_host_set_fluid_props.argtypes = [ctypes.c_double] * 3
+        H.enzoe_host_set_dual_energy.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int]
         H.enzoe_host_method_create.argtypes = [ctypes.c_char_p]
         H.enzoe_host_method_create.restype = vp
         H.enzoe_host_method_destroy.argtypes = [vp]
@@ -776,14 +792,25 @@ class HostAmr:
             self.handle = None
 
 
+def set_dual_energy(kind, eta=()):
+    """Physics:fluid_props:dual_energy of the C++ layer's fluid properties
+    (set after enzoe_host_set_fluid_props)."""
+    e = (ctypes.c_double * max(1, len(eta)))(*eta)
+    host_lib().enzoe_host_set_dual_energy(kind.encode(), e, len(eta))
+
+
 class HostMethod:
     """EnzoMethodMHDVlct of the C++ drop-in layer, built from the text of a
     "Method:mhd_vlct" parameter group."""
 
     def __init__(self, parameters, gamma=5.0 / 3.0, density_floor=1e-200,
-                 pressure_floor=1e-200):
+                 pressure_floor=1e-200, dual_energy=None):
+        """dual_energy: None, or (type, [eta...]) of the fluid properties'
+        "dual_energy" group."""
         H = host_lib()
         H.enzoe_host_set_fluid_props(gamma, density_floor, pressure_floor)
+        if dual_energy is not None:
+            set_dual_energy(*dual_energy)
         self.handle = H.enzoe_host_method_create(parameters.encode())
 
     def __del__(self):

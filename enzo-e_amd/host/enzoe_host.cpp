@@ -276,6 +276,39 @@ EnzoPhysicsFluidProps::EnzoPhysicsFluidProps(double gamma, double density_floor,
                density_floor > 0 && pressure_floor > 0);
 }
 
+void EnzoPhysicsFluidProps::set_dual_energy(const std::string &type,
+                                            const std::vector<double> &eta) {
+  // parse_de_config_ (EnzoConfig.cpp:952-1010)
+  const bool eta_exists = !eta.empty();
+  if (type == "disabled") {
+    ENZOE_ASSERT("parse_de_config_",
+                 "when dual energy is disabled, \"Physics:fluid_props:dual_energy:eta\" "
+                 "can't be specified", !eta_exists);
+    dual_type_ = type;
+  } else if (type == "modern") {
+    ENZOE_ASSERT("parse_de_config_",
+                 "\"Physics:fluid_props:dual_energy:eta\" must provide 1 value for the "
+                 "\"modern\" dual energy formalism", eta.size() == 1 || !eta_exists);
+    eta_[0] = eta_exists ? eta[0] : 0.001;
+    // EnzoDualEnergyConfig::build_modern_formulation (:42-48)
+    ENZOE_ASSERT("EnzoDualEnergyConfig::build_modern_formulation",
+                 "eta must be non-negative", eta_[0] >= 0);
+    dual_type_ = type;
+  } else if (type == "bryan95") {
+    ENZOE_ASSERT("parse_de_config_",
+                 "\"Physics:fluid_props:dual_energy:eta\" must provide 2 values for the "
+                 "\"bryan95\" dual energy formalism", eta.size() == 2 || !eta_exists);
+    eta_[0] = eta_exists ? eta[0] : 0.001;
+    eta_[1] = eta_exists ? eta[1] : 0.1;
+    ENZOE_ASSERT("EnzoDualEnergyConfig::build_bryan95_formulation",
+                 "eta1 and eta2 must be non-negative", eta_[0] >= 0 && eta_[1] >= 0);
+    dual_type_ = type;
+  } else {
+    ENZOE_ERROR("parse_de_config_",
+                "\"Physics:fluid_props:dual_energy:type\" is invalid: \"" + type + "\"");
+  }
+}
+
 void EnzoPhysicsFluidProps::primitive_from_integration(
     const EnzoEFltArrayMap &integration_map, EnzoEFltArrayMap &primitive_map,
     int stale_depth, const str_vec_t &passive_list, bool) const {
@@ -424,6 +457,7 @@ void Block::bind_(const enzoe_block &b) {
   put("bfieldi_x", b.bfieldi[0], 0, 0, 1);
   put("bfieldi_y", b.bfieldi[1], 0, 1, 0);
   put("bfieldi_z", b.bfieldi[2], 1, 0, 0);
+  put("internal_energy", b.internal_energy, 0, 0, 0);
 }
 
 void Block::add_field(const std::string &name, double *data) {
@@ -504,6 +538,7 @@ enzoe_block Block::desc() const {
     b.bfield[a] = ptr(kBNames[a]);
     b.bfieldi[a] = ptr(kBiNames[a]);
   }
+  b.internal_energy = ptr("internal_energy");
   return b;
 }
 
@@ -622,9 +657,6 @@ void EnzoReconstructor::reconstruct_interface(const EnzoEFltArrayMap &prim_map,
 EnzoRiemann *EnzoRiemann::construct_riemann(const FactoryArgs &factory_args) noexcept {
   // EnzoRiemann.cpp:26-68
   const std::string formatted = lowered(factory_args.solver);
-  ENZOE_ASSERT("EnzoRiemann::construct_riemann",
-               "the dual-energy formalism is not provided by the device library",
-               !factory_args.internal_energy);
   if (formatted == "hll") {
     ENZOE_ASSERT("EnzoRiemann::construct_riemann",
                  "An \"HLL\" Riemann solver without magnetic fields isn't "
@@ -637,22 +669,22 @@ EnzoRiemann *EnzoRiemann::construct_riemann(const FactoryArgs &factory_args) noe
     if (!factory_args.mhd)
       ENZOE_ERROR("EnzoRiemann::construct_riemann",
                   "The \"HLLE\" Riemann solver without magnetic fields is untested");
-    return new EnzoRiemann("hlle", true);
+    return new EnzoRiemann("hlle", true, factory_args.internal_energy);
   } else if (formatted == "hllc") {
     ENZOE_ASSERT("EnzoRiemann::construct_riemann",
                  "The \"HLLC\" Riemann Solver can't support mhd", !factory_args.mhd);
-    return new EnzoRiemann("hllc", false);
+    return new EnzoRiemann("hllc", false, factory_args.internal_energy);
   } else if (formatted == "hlld") {
     ENZOE_ASSERT("EnzoRiemann::construct_riemann",
                  "The \"HLLD\" Riemann Solver requires magnetic fields",
                  factory_args.mhd);
-    return new EnzoRiemann("hlld", true);
+    return new EnzoRiemann("hlld", true, factory_args.internal_energy);
   }
   ENZOE_ERROR("EnzoRiemann::construct_riemann",
               "The only known solvers are HLL, HLLE, HLLC, & HLLD");
 }
 
-EnzoRiemann::EnzoRiemann(std::string solver, bool mhd)
+EnzoRiemann::EnzoRiemann(std::string solver, bool mhd, bool internal_energy)
     : solver_(std::move(solver)), mhd_(mhd) {
   // key order of the LUTs (EnzoRiemannLUT.hpp; EnzoRiemannHLLD.hpp,
   // EnzoRiemannHLLC.hpp): density, [bfield], velocity, total_energy/pressure
@@ -665,6 +697,7 @@ EnzoRiemann::EnzoRiemann(std::string solver, bool mhd)
   }
   primitive_keys_ = integration_keys_;
   primitive_keys_.back() = "pressure";
+  if (internal_energy) integration_keys_.push_back("internal_energy");
 }
 
 void EnzoRiemann::solve(const EnzoEFltArrayMap &prim_map_l,
@@ -964,6 +997,10 @@ EnzoMHDIntegratorStageCommands::EnzoMHDIntegratorStageCommands(
     const EnzoMHDIntegratorStageArgPack &args)
     : mhd_choice_(parse_bfield_choice_(args.mhd_choice)) {
   // EnzoMHDIntegratorStageCommands.cpp:18-69
+  const EnzoPhysicsFluidProps *fp = enzo::fluid_props();
+  ENZOE_ASSERT("EnzoMethodMHDVlct::EnzoMethodMHDVlct",
+               "selected formulation of dual energy formalism is incompatible",
+               !fp->dual_energy_enabled() || fp->dual_energy_modern());
   riemann_solver_ = EnzoRiemann::construct_riemann(
       {args.rsolver, mhd_choice_ != bfield_choice::no_bfield,
        enzo::fluid_props()->dual_energy_enabled()});
@@ -1233,6 +1270,20 @@ EnzoMethodMHDVlct::EnzoMethodMHDVlct(ParameterGroup p, bool store_fluxes_for_cor
   prm.density_floor = fp->density_floor();
   prm.pressure_floor = fp->pressure_floor();
   prm.precision = precision_;
+  if (fp->dual_energy_enabled()) {
+    // the internal-energy terms are part of the fused device step only
+    ENZOE_ASSERT("EnzoMethodMHDVlct",
+                 "the dual-energy formalism needs " + p.full_name("device_fused") + " = true",
+                 fused_);
+    ENZOE_ASSERT("EnzoMethodMHDVlct",
+                 "the dual-energy formalism needs " + p.full_name("precision") +
+                     " = \"double\"", precision_ == ENZOE_PREC_DOUBLE);
+    prm.dual_energy = "modern";
+    prm.dual_energy_eta = fp->dual_energy_eta();
+  } else {
+    prm.dual_energy = "disabled";
+    prm.dual_energy_eta = -1.0;
+  }
   check_rc(enzoe_vlct_create(&prm, &device_), "EnzoMethodMHDVlct::EnzoMethodMHDVlct");
 }
 

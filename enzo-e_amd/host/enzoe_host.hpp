@@ -153,7 +153,12 @@ class EnzoPhysicsFluidProps {
   double gamma() const { return gamma_; }
   double density_floor() const { return density_floor_; }
   double pressure_floor() const { return pressure_floor_; }
-  bool dual_energy_enabled() const { return false; }
+  // Physics:fluid_props:dual_energy (EnzoConfig.cpp:952-1040): "disabled",
+  // "modern" (one eta, default 0.001) or "bryan95" (eta_1, eta_2)
+  void set_dual_energy(const std::string &type, const std::vector<double> &eta);
+  bool dual_energy_enabled() const { return dual_type_ != "disabled"; }
+  bool dual_energy_modern() const { return dual_type_ == "modern"; }
+  double dual_energy_eta() const { return eta_[0]; }
 
   // EnzoPhysicsFluidProps::primitive_from_integration (:64-138)
   void primitive_from_integration(const EnzoEFltArrayMap &integration_map,
@@ -167,6 +172,8 @@ class EnzoPhysicsFluidProps {
 
  private:
   double gamma_, density_floor_, pressure_floor_;
+  std::string dual_type_ = "disabled";
+  double eta_[2] = {0.0, 0.0};
 };
 
 namespace enzo {
@@ -405,7 +412,9 @@ class EnzoRiemann {
   // EnzoRiemann::construct_riemann (EnzoRiemann.cpp:26-68)
   static EnzoRiemann *construct_riemann(const FactoryArgs &factory_args) noexcept;
 
-  EnzoRiemann(std::string solver, bool mhd);
+  // internal_energy: the dual-energy formalism adds "internal_energy" to
+  // the integration quantities (EnzoRiemannImpl.hpp:240-262)
+  EnzoRiemann(std::string solver, bool mhd, bool internal_energy = false);
   virtual ~EnzoRiemann() = default;
 
   // EnzoRiemann.hpp:125-129 (interface_velocity: dual energy only)

@@ -32,6 +32,14 @@ void enzoe_host_set_fluid_props(double gamma, double density_floor,
   enzo::set_fluid_props(EnzoPhysicsFluidProps(gamma, density_floor, pressure_floor));
 }
 
+void enzoe_host_set_dual_energy(const char *type, const double *eta, int n_eta) {
+  ENZOE_ASSERT("enzoe_host_set_dual_energy", "n_eta must be 0, 1 or 2",
+               n_eta >= 0 && n_eta <= 2 && (n_eta == 0 || eta != nullptr));
+  std::vector<double> e;
+  for (int i = 0; i < n_eta; i++) e.push_back(eta[i]);
+  enzo::fluid_props()->set_dual_energy(type ? type : "disabled", e);
+}
+
 enzoe_host_method_t enzoe_host_method_create(const char *parameters) {
   ParameterGroup p(parameters ? parameters : "");
   enzoe_host_method_s *m = new enzoe_host_method_s;

@@ -225,6 +225,7 @@ void EnzoInitialInclinedWave::enforce_block(Block *block,
   };
 
   const std::size_t ncc = (std::size_t)mx * my * mz;
+  std::vector<double> cx, cy, cz;  // cell-centred B (MHD)
   if (mhd) {
     std::vector<double> bx((std::size_t)mz * my * (mx + 1), 0.0),
         by((std::size_t)mz * (my + 1) * mx, 0.0), bz((std::size_t)(mz + 1) * my * mx, 0.0);
@@ -271,7 +272,9 @@ void EnzoInitialInclinedWave::enforce_block(Block *block,
                  (Ax[at(k, j + 1, i, my + 1, mx)] - Ax[at(k, j, i, my + 1, mx)]) / dy);
     }
     // cell-centred B = face average (EnzoBfieldMethodCT::compute_center_bfield)
-    std::vector<double> cx(ncc), cy(ncc), cz(ncc);
+    cx.assign(ncc, 0.0);
+    cy.assign(ncc, 0.0);
+    cz.assign(ncc, 0.0);
     for (int k = 0; k < mz; k++)
       for (int j = 0; j < my; j++)
         for (int i = 0; i < mx; i++) {
@@ -308,6 +311,18 @@ void EnzoInitialInclinedWave::enforce_block(Block *block,
   to_device(block->field_view("velocity_y").data(), vy);
   to_device(block->field_view("velocity_z").data(), vz);
   to_device(block->field_view("total_energy").data(), etot);
+  // setup_eint_ (:495-538): with an internal_energy field (dual energy),
+  // eint = etot - kinetic - magnetic / density
+  if (block->has_field("internal_energy")) {
+    std::vector<double> eint(ncc);
+    for (std::size_t c = 0; c < ncc; c++) {
+      const double kinetic = 0.5 * (vx[c] * vx[c] + vy[c] * vy[c] + vz[c] * vz[c]);
+      const double magnetic =
+          mhd ? 0.5 * (cx[c] * cx[c] + cy[c] * cy[c] + cz[c] * cz[c]) / rho[c] : 0.;
+      eint[c] = etot[c] - kinetic - magnetic;
+    }
+    to_device(block->field_view("internal_energy").data(), eint);
+  }
 }
 
 // ------------------------------------------------------------ shock tubes
@@ -429,6 +444,9 @@ void EnzoInitialShockTube::enforce_block(Block *block, const std::array<double, 
   };
   std::vector<double> rho = download("density"), etot = download("total_energy");
   std::vector<double> v[3] = {download(vel[0]), download(vel[1]), download(vel[2])};
+  const bool has_eint = block->has_field("internal_energy");
+  std::vector<double> eint_h;
+  if (has_eint) eint_h = download("internal_energy");
   std::vector<double> bf[3];
   if (mhd)
     for (int a = 0; a < 3; a++) bf[a] = download(bfi[a]);
@@ -448,6 +466,8 @@ void EnzoInitialShockTube::enforce_block(Block *block, const std::array<double, 
       fill(bfi[ak], s0, s1, S.b2, bf[ak]);
     }
     const double eint = (S.pressure / ((gamma - 1.) * S.density));
+    // (optionally) the specific internal energy (:255-261)
+    if (has_eint) fill("internal_energy", s0, s1, eint, eint_h);
     const double v2 = (velocity_0 * velocity_0 + velocity_1 * velocity_1 +
                        velocity_2 * velocity_2);
     const double b2 = (aligned_bfield_val * aligned_bfield_val + S.b1 * S.b1 + S.b2 * S.b2);
@@ -455,6 +475,7 @@ void EnzoInitialShockTube::enforce_block(Block *block, const std::array<double, 
   }
   to_device(block->field_view("density").data(), rho);
   to_device(block->field_view("total_energy").data(), etot);
+  if (has_eint) to_device(block->field_view("internal_energy").data(), eint_h);
   for (int a = 0; a < 3; a++) to_device(block->field_view(vel[a]).data(), v[a]);
   if (mhd) {
     // the aligned face field is uniform (:279-280)

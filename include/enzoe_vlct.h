@@ -57,6 +57,13 @@ typedef struct {
   double density_floor;           /* > 0                                    */
   double pressure_floor;          /* > 0                                    */
   int precision;                  /* ENZOE_PREC_DOUBLE | ENZOE_PREC_MIXED   */
+  /* Physics:fluid_props:dual_energy (EnzoConfig.cpp:952-1040,
+   * EnzoDualEnergyConfig.hpp:17-80): "disabled" (NULL) or "modern"; the
+   * VL+CT method rejects "bryan95" (EnzoMHDIntegratorStageCommands.cpp:
+   * 31-34).  With "modern" every block must carry internal_energy, and the
+   * precision must be ENZOE_PREC_DOUBLE.                                   */
+  const char *dual_energy;
+  double dual_energy_eta;         /* "modern" eta; negative = 0.001         */
 } enzoe_vlct_params;
 
 /* Device view of one block's permanent fields (a Cello Block's FieldData,
@@ -73,6 +80,8 @@ typedef struct {
   double *bfieldi[3];      /* face-centred B: (mz,my,mx+1), (mz,my+1,mx),
                               (mz+1,my,mx)                                   */
   double *pressure;        /* permanent "pressure" field (may be NULL)      */
+  double *internal_energy; /* specific internal energy: required with the
+                              dual-energy formalism, ignored otherwise     */
 } enzoe_block;
 
 typedef struct enzoe_vlct_method *enzoe_vlct_t;
@@ -116,10 +125,6 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                           const double *dt, int dt_on_device,
                           enzoe_stream_t stream);
 
-/* EnzoMethodMHDVlct::timestep + the clip of control_stopping.cpp:83-86
- * without a host round trip: reads the time from *d_time (device), writes
- * dt = min(courant * min_cfl, t_stop - t) into d_dts[0..ndts) (device) and
- * advances *d_time by dt.  For device-driven cycles (graph capture).        */
 /* Launch-shape autotuner (no counterpart in the reference): times every
  * combination of XCD-contiguous workgroup order over the four stage kernels
  * on a scratch copy of the leading blocks (at most ~2 GiB; the caller's
@@ -130,6 +135,12 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
 int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                         double dt, enzoe_stream_t stream, int *mask_out);
 
+/* EnzoMethodMHDVlct::timestep + the clip of control_stopping.cpp:83-86
+ * without a host round trip: reads the time from *d_time (device), writes
+ * dt = min(courant * min_cfl, t_stop - t) into d_dts[0..ndts) (device) and
+ * advances *d_time by dt.  For device-driven cycles (graph capture); a
+ * cycle that starts at t_stop is zero-length and, with dual energy, leaves
+ * the energies unsynchronised.                                              */
 int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                                double *d_time, double t_stop, double *d_dts,
                                int ndts, enzoe_stream_t stream);

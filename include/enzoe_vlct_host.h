@@ -22,6 +22,14 @@ typedef struct enzoe_host_method_s *enzoe_host_method_t;
 void enzoe_host_set_fluid_props(double gamma, double density_floor,
                                 double pressure_floor);
 
+/* Physics:fluid_props:dual_energy of the fluid properties set last
+ * (EnzoConfig.cpp:952-1010): type "disabled" | "modern" | "bryan95" and the
+ * n_eta values of its "eta" list (0 = not given: 0.001, and 0.1 for the
+ * second bryan95 value).  EnzoMethodMHDVlct accepts "disabled" and "modern"
+ * (EnzoMHDIntegratorStageCommands.cpp:31-34); with "modern" its blocks carry
+ * internal_energy and it runs the fused device step.                       */
+void enzoe_host_set_dual_energy(const char *type, const double *eta, int n_eta);
+
 /* new EnzoMethodMHDVlct(ParameterGroup(parameters), false); `parameters`
  * holds "name = value" items separated by ';' or newlines with the names of
  * the reference's "Method:mhd_vlct" group (mhd_choice, riemann_solver,

@@ -473,6 +473,21 @@ int o_init_inclined_wave(const o_config *cfg, o_grid *gr, const char *wave,
               double etot_dens = lin_scalar(&w, w.e_back, w.e_ev, x, y, z);
               b->cc[O_ETOT][c] = etot_dens / rho;
             }
+        /* setup_eint_ (:495-538): with dual energy the blocks carry
+           eint = etot - kinetic - magnetic / density */
+        if (cfg->dual_energy) {
+          long ncc = (long)mx * my * mz;
+          for (long c = 0; c < ncc; c++) {
+            double vx = b->cc[O_VX][c], vy = b->cc[O_VY][c], vz = b->cc[O_VZ][c];
+            double kinetic = 0.5 * (vx * vx + vy * vy + vz * vz);
+            double magnetic = 0.;
+            if (cfg->mhd) {
+              double bx = b->cc[O_BX][c], by = b->cc[O_BY][c], bz = b->cc[O_BZ][c];
+              magnetic = 0.5 * (bx * bx + by * by + bz * bz) / b->cc[O_RHO][c];
+            }
+            b->eint[c] = b->cc[O_ETOT][c] - kinetic - magnetic;
+          }
+        }
 #undef PX
 #undef PY
 #undef PZ

@@ -47,6 +47,8 @@ def _create(**kw):
     p.density_floor = kw.get("dfloor", 1e-200)
     p.pressure_floor = kw.get("pfloor", 1e-200)
     p.precision = kw.get("precision", 0)
+    p.dual_energy = kw.get("dual_energy", None)
+    p.dual_energy_eta = kw.get("eta", -1.0)
     h = ctypes.c_void_p()
     rc = ev.lib().enzoe_vlct_create(ctypes.byref(p), ctypes.byref(h))
     return rc, h
@@ -60,6 +62,10 @@ def _create(**kw):
     dict(recon=b"ppm"), dict(time_scheme=b"rk3"), dict(theta=2.5), dict(theta=0.5),
     dict(gamma=1.0), dict(dfloor=0.0), dict(precision=7),
     dict(time_scheme=b"euler"),  # CT needs two partial timesteps
+    # the VL+CT integrator takes the "modern" dual energy only
+    # (EnzoMHDIntegratorStageCommands.cpp:31-34), in FP64
+    dict(dual_energy=b"bryan95"), dict(dual_energy=b"on"),
+    dict(dual_energy=b"modern", precision=1),
 ])
 def test_abi_rejects_configurations_the_reference_rejects(kw):
     rc, h = _create(**kw)
@@ -120,10 +126,10 @@ def test_single_operations_validate_views_without_launching():
                                      2, None) == ERR_ARG
 
 
-def _run_host(params):
+def _run_host(params, dual_energy=None):
     code = ("import sys; sys.path.insert(0, %r); import enzoe_vlct as ev; "
-            "m = ev.HostMethod(%r); print('constructed', m.name, m.courant)"
-            % (os.path.join(ROOT, "enzo-e_amd"), params))
+            "m = ev.HostMethod(%r, dual_energy=%r); print('constructed', m.name, m.courant)"
+            % (os.path.join(ROOT, "enzo-e_amd"), params, dual_energy))
     return subprocess.run([sys.executable, "-c", code], capture_output=True,
                           text=True, timeout=120)
 
@@ -404,3 +410,51 @@ def test_gravity_entry_points_reject_bad_arguments_without_a_device():
     m = ev.FMap()
     assert L.enzoe_source_gravity(0.1, ctypes.byref(m), ctypes.byref(m), ctypes.byref(m), 0,
                                   None) == ERR_ARG
+
+
+def test_dual_energy_configuration():
+    """The "modern" formulation is accepted (eta default 0.001,
+    EnzoConfig.cpp:988-995); a dual-energy method rejects blocks without
+    internal_energy before anything is launched."""
+    L = ev.lib()
+    rc, h = _create(dual_energy=b"modern")
+    assert rc == 0
+    rc2, h2 = _create(dual_energy=b"MODERN", eta=0.0)
+    assert rc2 == 0
+    fake = 1 << 40
+    b = ev.Block()
+    b.mx = b.my = b.mz = 12
+    b.ghost = 3
+    b.density = b.total_energy = fake
+    for a in range(3):
+        b.velocity[a] = b.bfield[a] = b.bfieldi[a] = fake
+    arr = (ev.Block * 1)(b)
+    assert L.enzoe_vlct_compute(h, arr, 1, 0.1, None) == ERR_ARG
+    bcs = (ctypes.c_int * 3)(0, 0, 0)
+    assert L.enzoe_grid_refresh(h, arr, 1, 1, 1, bcs, None) == ERR_ARG
+    L.enzoe_vlct_destroy(h)
+    L.enzoe_vlct_destroy(h2)
+
+
+@pytest.mark.parametrize("params,dual,message", [
+    ("mhd_choice = constrained_transport", ("bryan95", []),
+     "selected formulation of dual energy formalism is incompatible"),
+    ("mhd_choice = constrained_transport; device_fused = false", ("modern", [0.01]),
+     "the dual-energy formalism needs"),
+    ("mhd_choice = constrained_transport; precision = mixed", ("modern", []),
+     "the dual-energy formalism needs"),
+    ("mhd_choice = constrained_transport", ("disabled", [0.01]), "can't be specified"),
+    ("mhd_choice = constrained_transport", ("modern", [0.1, 0.2]), "must provide 1 value"),
+    ("mhd_choice = constrained_transport", ("modern", [-1.0]), "eta must be non-negative"),
+    ("mhd_choice = constrained_transport", ("dual", []), "is invalid"),
+])
+def test_host_dual_energy_errors_exit_like_the_reference(params, dual, message):
+    r = _run_host(params, dual)
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert "ERROR in" in r.stderr and message in r.stderr, r.stderr
+
+
+def test_host_dual_energy_modern_constructs():
+    r = _run_host("mhd_choice = constrained_transport", ("modern", [0.00769]))
+    assert r.returncode == 0, r.stderr
+    assert "constructed mhd_vlct 0.3" in r.stdout

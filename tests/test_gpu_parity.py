@@ -35,7 +35,8 @@ def method_for(cfg_o, precision=0, courant=None):
                          courant=cfg_o.courant if courant is None else courant,
                          gamma=cfg_o.gamma, density_floor=cfg_o.density_floor,
                          pressure_floor=cfg_o.pressure_floor,
-                         precision=precision)
+                         precision=precision,
+                         dual_energy_eta=cfg_o.eta if cfg_o.dual_energy else None)
 
 
 def device_copy(grid_o, method):
