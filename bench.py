@@ -69,6 +69,14 @@ def parse():
 
 def make_state(n, method):
     blk, cw = problems.orszag_tang(n)
+    if method.dual_energy:
+        # the specific internal energy of the state (setup_eint_,
+        # EnzoInitialInclinedWave.cpp:495-538); ghost zones come from the refresh
+        rho = blk["density"]
+        v2 = sum(np.square(blk[k]) for k in ("velocity_x", "velocity_y", "velocity_z"))
+        b2 = sum(np.square(blk[k]) for k in ("bfield_x", "bfield_y", "bfield_z"))
+        blk["internal_energy"] = blk["total_energy"] - 0.5 * v2 - np.divide(
+            0.5 * b2, rho, out=np.zeros_like(rho), where=rho > 0)
     grid = ev.DeviceGrid(method, (1, 1, 1), (n, n, n), [cw])
     grid.upload_from([blk])
     grid.refresh()
@@ -180,12 +188,14 @@ def cpu_baseline(n, nsteps):
 METRIC = "zone-updates/s/GPU (batched FP32-mixed vs plain FP64 VL+CT) + HBM GB/s"
 
 
-def timed_run(precision, n, steps, warmup):
-    """Warm-up, then `steps` timed steps with per-kernel HIP-event timing."""
+def timed_run(precision, n, steps, warmup, dual_energy=False):
+    """Warm-up, then `steps` timed steps with per-kernel HIP-event timing.
+    dual_energy: the "modern" dual-energy formalism (eta 0.001) on."""
     prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
     method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
                            theta=2.0, courant=0.4, gamma=5.0 / 3.0,
-                           precision=prec)
+                           precision=prec,
+                           dual_energy_eta=0.001 if dual_energy else None)
     grid = make_state(n, method)
     for _ in range(warmup):
         step(grid)
@@ -217,7 +227,8 @@ def timed_run(precision, n, steps, warmup):
             "achieved": dom_bytes / (dom_ms * 1e-3) / 1e9,
             "gpu_step_ms": gpu_step_ms,
             "step_gbs": STEP_BYTES_PER_ZONE_FP64 * zones / (gpu_step_ms * 1e-3) / 1e9,
-            "workload": "orszag_tang_%d3_%s_vlct_plm_hlld" % (n, precision),
+            "workload": "orszag_tang_%d3_%s%s_vlct_plm_hlld" % (
+                n, precision, "_dual_energy" if dual_energy else ""),
             "launch_mask": mask}
 
 
@@ -521,10 +532,11 @@ def main():
         print(json.dumps(out))
         return
     r = timed_run(a.precision, a.n, a.steps, a.warmup)
-    other = None
+    other = dual = None
     if not a.no_compare:
         other = timed_run("mixed" if a.precision == "fp64" else "fp64", a.n,
                           a.steps, a.warmup)
+        dual = timed_run("fp64", a.n, a.steps, a.warmup, dual_energy=True)
     copy = copy_bandwidth()
     out = {
         "metric": METRIC,
@@ -573,6 +585,14 @@ def main():
                                  for k, v in sorted(mixed["prof"].items())},
             "mixed_dominant_GBs": mixed["achieved"],
             "mixed_traffic": measured_traffic(mixed["dom"], mixed["workload"]),
+        }
+    if dual is not None:
+        fp64 = r if a.precision == "fp64" else other
+        out["dual_energy_comparison"] = {
+            "workload": dual["workload"], "value": dual["value"],
+            "over_fp64": dual["value"] / fp64["value"],
+            "ms_per_step": dual["ms_per_step"],
+            "kernels_ms": {k: round(v[0], 4) for k, v in sorted(dual["prof"].items())},
         }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)

@@ -246,6 +246,11 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
   T fo[8];
   T vbar = T(0);
   T *vb = DE ? &vbar : nullptr;
+  T eint_l = T(0), eint_r = T(0);
+  if (DE) {
+    eint_l = specific_eint(wl[S_RHO], wl[S_E], gamma);
+    eint_r = specific_eint(wr[S_RHO], wr[S_E], gamma);
+  }
   if (MHD) {
     const int bnx = b.mx + (D == 0), bny = b.my + (D == 1);
     const T bface =
@@ -287,10 +292,10 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
   }
   if (DE) {
     // the dual-energy outputs of the solve (EnzoRiemannHLLD.hpp:404-447,
-    // EnzoRiemannHLL.hpp:322-344, EnzoRiemannHLLC.hpp:159-170), from the
-    // stored density flux (slot 0 of either LUT)
-    gst(b.Fe[D], c, (double)passive_eint_flux(wl[S_RHO], wl[S_E], wr[S_RHO], wr[S_E],
-                                              gamma, fo[0]));
+    // EnzoRiemannHLL.hpp:322-344, EnzoRiemannHLLC.hpp:159-170): the
+    // internal-energy flux from the stored density flux (slot 0 of either
+    // LUT), and vi_bar
+    gst(b.Fe[D], c, (double)upwind_eint_flux(eint_l, eint_r, fo[0]));
     gst(b.Vb[D], c, (double)vbar);
   }
 }

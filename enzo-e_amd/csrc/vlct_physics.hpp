@@ -87,16 +87,19 @@ __device__ __forceinline__ T fast_ms_speed(T d, T p, T bi, T bj, T bk,
 
 template <typename T> struct Cons1D { T d, mx, my, mz, e, by, bz; };
 
-// enzo_riemann_utils::passive_eint_flux (EnzoRiemannUtils.hpp:224-249):
-// the specific internal energy p / ((gamma-1) rho) of the side the density
-// flux comes from, selected branch-free as the reference writes it, times
-// the density flux (dual-energy formalism)
+// enzo_riemann_utils::passive_eint_flux (EnzoRiemannUtils.hpp:224-249),
+// dual-energy formalism: the specific internal energy p / ((gamma-1) rho)
+// of the side the density flux comes from, selected branch-free as the
+// reference writes it, times the density flux.  Split in two: the specific
+// energies (EnzoEOSIdeal::specific_eint, EnzoEOSIdeal.hpp:79-82) depend on
+// the face states only, so the flux kernel evaluates them before the solve
+// and keeps two values live through it instead of four.
 template <typename T>
-__device__ __forceinline__ T passive_eint_flux(T density_l, T pressure_l,
-                                               T density_r, T pressure_r,
-                                               T gamma, T density_flux) {
-  const T eint_l = qdiv(pressure_l, (gamma - T(1)) * density_l);
-  const T eint_r = qdiv(pressure_r, (gamma - T(1)) * density_r);
+__device__ __forceinline__ T specific_eint(T density, T pressure, T gamma) {
+  return qdiv(pressure, (gamma - T(1)) * density);
+}
+template <typename T>
+__device__ __forceinline__ T upwind_eint_flux(T eint_l, T eint_r, T density_flux) {
   const T upwind = T(density_flux > T(0)) * eint_l + T(density_flux <= T(0)) * eint_r;
   return upwind * density_flux;
 }
@@ -180,6 +183,22 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
 
   T sdml = spd[0] - spd[2];
   T sdmr = spd[4] - spd[2];
+  if (vbar) {
+    // as soon as the outer and contact speeds are known (fewer live
+    // registers through the rest of the solve): l_coef = (S_l - vx_L) /
+    // (S_l - S_M) = sdl / sdml, r_coef = sdr / sdmr
+    const T l_coef = qdiv(sdl, sdml);
+    const T r_coef = qdiv(sdr, sdmr);
+    if (spd[0] > T(0)) {
+      *vbar = wli[L_VI];
+    } else if (spd[4] < T(0)) {
+      *vbar = wri[L_VI];
+    } else if (spd[2] >= T(0)) {
+      *vbar = spd[2] * l_coef;
+    } else {
+      *vbar = spd[2] * r_coef;
+    }
+  }
   T sdml_inv = qdiv(T(1), sdml);
   T sdmr_inv = qdiv(T(1), sdmr);
   ulst.d = ul.d * sdl * sdml_inv;
@@ -318,20 +337,6 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     flx[L_BK] = fr.bz + spd[4] * (urst.bz - ur.bz);
   }
   flx[L_BI] = T(0);
-  if (vbar) {
-    const T S_M = spd[2], S_l = spd[0], S_r = spd[4];
-    const T l_coef = qdiv(S_l - wli[L_VI], S_l - S_M);
-    const T r_coef = qdiv(S_r - wri[L_VI], S_r - S_M);
-    if (S_l > T(0)) {
-      *vbar = wli[L_VI];
-    } else if (S_r < T(0)) {
-      *vbar = wri[L_VI];
-    } else if (S_M >= T(0)) {
-      *vbar = S_M * l_coef;
-    } else {
-      *vbar = S_M * r_coef;
-    }
-  }
 }
 
 // conserved state + physical flux, MHD LUT (EnzoRiemannUtils.hpp:48-149)
@@ -492,6 +497,7 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T gamma,
     sr = qdiv(-cw, bp - cw);
     sm = qdiv(bp, bp - cw);
   }
+  if (vbar) *vbar = (sl * (wl[H_VI] - bm) + sr * (wr[H_VI] - bp));
   cp = dmax(cp, T(0));
   T momentumi_l = cons_l[H_VI];
   T momentumi_r = cons_r[H_VI];
@@ -510,7 +516,6 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T gamma,
   flux[H_VJ] = sl * vfl + sr * vfr;
   flux[H_VK] = sl * wfl + sr * wfr;
   flux[H_E] = sl * efl + sr * efr + (sm * cp * cw);
-  if (vbar) *vbar = (sl * (wl[H_VI] - bm) + sr * (wr[H_VI] - bp));
 }
 
 }  // namespace enzoe
