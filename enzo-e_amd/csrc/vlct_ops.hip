@@ -138,6 +138,7 @@ constexpr unsigned kMhdSlots = 0xFFu;
 constexpr unsigned kHydroSlots = 0x1Fu;
 constexpr unsigned kUpdateSlots = 0x1Fu;  // density, velocity, total_energy
 constexpr unsigned kPassiveSlots = 0xFF00u;
+constexpr unsigned kEintSlot = 1u << ENZOE_SLOT_INTERNAL_ENERGY;
 
 // ------------------------------------------------------------------ kernels
 
@@ -186,27 +187,45 @@ __global__ void k_op_reconstruct(KMap w, KMap wl, KMap wr, unsigned slots,
 // EnzoRiemannImpl<K>::solve_ (EnzoRiemannImpl.hpp:317-338): one face per
 // thread over [s, n-s) of the flux view; states permuted into the i/j/k
 // order of the kernels (EnzoRiemannLUT, KernelConfig).
-template <int RIEMANN, bool MHD>
+// DE: the dual-energy outputs -- the internal-energy flux into the flux
+// map's internal_energy slot and vi_bar into `vel` (EnzoRiemannImpl.hpp:
+// 270-301 with calculate_internal_energy_flux_).
+template <int RIEMANN, bool MHD, bool DE = false>
 __global__ void k_op_riemann(KMap pl, KMap pr, KMap fx8, Box box, int dim,
-                             double gamma, unsigned pmask) {
+                             double gamma, unsigned pmask, KArr vel = KArr()) {
   int p[3];
   if (!box_pos(box, p)) return;
   const long a = off(pl.s, p), b = off(pr.s, p), f = off(fx8.s, p);
   const int dj = (dim + 1) % 3, dk = (dim + 2) % 3;
+  double vbar = 0.;
+  double *vb = DE ? &vbar : nullptr;
+  double eint_l = 0., eint_r = 0.;
   if (MHD) {
     const int src[8] = {S_RHO, S_BX + dim, S_BX + dj, S_BX + dk,
                         S_VX + dim, S_VX + dj, S_VX + dk, S_E};
     double l[8], r[8], o[8];
     for (int q = 0; q < 8; q++) { l[q] = pl.p[src[q]][a]; r[q] = pr.p[src[q]][b]; }
-    if (RIEMANN == RS_HLLD) hlld(l, r, gamma, o);
-    else hlle_mhd(l, r, gamma, o);
+    if (DE) {
+      eint_l = specific_eint(l[L_D], l[L_E], gamma);
+      eint_r = specific_eint(r[L_D], r[L_E], gamma);
+    }
+    if (RIEMANN == RS_HLLD) hlld(l, r, gamma, o, vb);
+    else hlle_mhd(l, r, gamma, o, vb);
     for (int q = 0; q < 8; q++) fx8.p[src[q]][f] = o[q];
   } else {
     const int src[5] = {S_RHO, S_VX + dim, S_VX + dj, S_VX + dk, S_E};
     double l[5], r[5], o[5];
     for (int q = 0; q < 5; q++) { l[q] = pl.p[src[q]][a]; r[q] = pr.p[src[q]][b]; }
-    hllc(l, r, gamma, o);
+    if (DE) {
+      eint_l = specific_eint(l[H_D], l[H_E], gamma);
+      eint_r = specific_eint(r[H_D], r[H_E], gamma);
+    }
+    hllc(l, r, gamma, o, vb);
     for (int q = 0; q < 5; q++) fx8.p[src[q]][f] = o[q];
+  }
+  if (DE) {
+    fx8.p[ENZOE_SLOT_INTERNAL_ENERGY][f] = upwind_eint_flux(eint_l, eint_r, fx8.p[S_RHO][f]);
+    vel.p[off(vel.s, p)] = vbar;
   }
   // passive scalars: upwind mass fraction times the density flux
   // (calc_passive_scalar_flux_, EnzoRiemannUtils.hpp:224-232, branch-free
@@ -362,6 +381,93 @@ __global__ void k_op_update(KMap U0, KMap dU, KMap Uo, Box box, int mhd,
   }
   double etot_floor = eint_floor + non_thermal_e;
   Uo.p[S_E][o] = dmax(v[3], etot_floor);
+}
+
+// EnzoSourceInternalEnergy::calculate_source
+// (EnzoSourceInternalEnergy.cpp:16-92): cells with p_dim in [s+1, n-s-1),
+// [s, n-s) across: deint(c) -= dtdx max(p(c), p_floor) (vbar(c+1/2) -
+// vbar(c-1/2)); vbar is face-trimmed along dim (entry p is face p+1/2).
+__global__ void k_op_source_eint(KMap W, KMap dU, KArr V, Box box, int dim,
+                                 double dtdx, double pfloor) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long vr = off(V.s, p), vl = vr - V.s[dim];
+  const double pc = dmax(W.p[S_E][off(W.s, p)], pfloor);
+  dU.p[ENZOE_SLOT_INTERNAL_ENERGY][off(dU.s, p)] -= dtdx * pc * (V.p[vr] - V.p[vl]);
+}
+
+// update_quantities with the internal energy as a specific quantity and the
+// dual-energy synchronisation (EnzoIntegrationQuanUpdate.cpp:183-238,
+// EnzoPhysicsFluidProps.cpp:158-290 at stale s+1).
+template <bool MHD>
+__global__ void k_op_update_de(KMap U0, KMap dU, KMap Uo, Box box, double gamma,
+                               double dfloor, double pfloor, double eta, double ggm1,
+                               unsigned pmask) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long a = off(U0.s, p), d = off(dU.s, p), o = off(Uo.s, p);
+  for (int q = ENZOE_SLOT_PASSIVE0; q < ENZOE_MAP_SLOTS; q++)
+    if (pmask >> q & 1u) Uo.p[q][o] = U0.p[q][a] + dU.p[q][d];
+  const int E = ENZOE_SLOT_INTERNAL_ENERGY;
+  double old_rho = U0.p[S_RHO][a];
+  double new_rho = dmax(U0.p[S_RHO][a] + dU.p[S_RHO][d], dfloor);
+  double inv_new_rho = 1. / new_rho;
+  double v[3];
+  for (int q = 1; q < 4; q++)
+    v[q - 1] = (U0.p[q][a] * old_rho + dU.p[q][d]) * inv_new_rho;
+  double etot = (U0.p[S_E][a] * old_rho + dU.p[S_E][d]) * inv_new_rho;
+  double eint = (U0.p[E][a] * old_rho + dU.p[E][d]) * inv_new_rho;
+  double b2 = 0.;
+  if (MHD) {
+    double bx = Uo.p[S_BX][o], by = Uo.p[S_BY][o], bz = Uo.p[S_BZ][o];
+    b2 = (bx * bx + by * by + bz * bz);
+  }
+  floor_and_sync<MHD, true>(new_rho, v, b2, gamma, pfloor, eta, ggm1, etot, eint);
+  Uo.p[S_RHO][o] = new_rho;
+  for (int q = 1; q < 4; q++) Uo.p[q][o] = v[q - 1];
+  Uo.p[S_E][o] = etot;
+  Uo.p[E][o] = eint;
+}
+
+// apply_floor_to_energy_and_sync in place over a box of an integration map
+template <bool MHD, bool DE>
+__global__ void k_op_sync(KMap U, Box box, double gamma, double pfloor, double eta,
+                          double ggm1) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long c = off(U.s, p);
+  const double v[3] = {U.p[S_VX][c], U.p[S_VY][c], U.p[S_VZ][c]};
+  double b2 = 0.;
+  if (MHD) {
+    double bx = U.p[S_BX][c], by = U.p[S_BY][c], bz = U.p[S_BZ][c];
+    b2 = (bx * bx + by * by + bz * bz);
+  }
+  double etot = U.p[S_E][c];
+  double eint = DE ? U.p[ENZOE_SLOT_INTERNAL_ENERGY][c] : 0.;
+  floor_and_sync<MHD, DE>(U.p[S_RHO][c], v, b2, gamma, pfloor, eta, ggm1, etot, eint);
+  U.p[S_E][c] = etot;
+  if (DE) U.p[ENZOE_SLOT_INTERNAL_ENERGY][c] = eint;
+}
+
+// p = (gamma-1) rho eint (EnzoComputePressure.cpp:127-134), dual energy
+__global__ void k_op_primitive_de(KMap U, KMap W, Box box, int mhd, double gm1,
+                                  unsigned pmask) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long c = off(U.s, p), w = off(W.s, p);
+  const int nq = mhd ? 8 : 5;
+  for (int q = 0; q < nq; q++)
+    if (q != S_E) W.p[q][w] = U.p[q][c];
+  for (int q = ENZOE_SLOT_PASSIVE0; q < ENZOE_MAP_SLOTS; q++)
+    if (pmask >> q & 1u) W.p[q][w] = U.p[q][c] / U.p[S_RHO][c];
+  W.p[S_E][w] = gm1 * U.p[S_RHO][c] * U.p[ENZOE_SLOT_INTERNAL_ENERGY][c];
+}
+
+__global__ void k_op_pressure_de(KMap U, KArr P, Box box, double gm1) {
+  int p[3];
+  if (!box_pos(box, p)) return;
+  const long c = off(U.s, p);
+  P.p[off(P.s, p)] = gm1 * U.p[S_RHO][c] * U.p[ENZOE_SLOT_INTERNAL_ENERGY][c];
 }
 
 // p = (gamma-1) (rho (e_tot - v^2/2) - B^2/2)
@@ -783,6 +889,8 @@ int enzoe_update_quantities(const enzoe_fmap *initial,
   const unsigned pmask = present(d) & kPassiveSlots;
   if ((present(a) & pmask) != pmask || (present(o) & pmask) != pmask)
     return ENZOE_ERR_ARG;
+  // an internal-energy change belongs to the dual-energy update
+  if (present(d) & kEintSlot) return ENZOE_ERR_ARG;
   int lo[3], hi[3];
   stale_box(a.n, stale_depth + 1, lo, hi);
   long count;
@@ -831,6 +939,182 @@ int enzoe_pressure_from_integration(const enzoe_fmap *integration,
   if (count == 0) return ENZOE_OK;
   hipLaunchKernelGGL(k_op_pressure, dim3(nblk(count)), dim3(256), 0,
                      (hipStream_t)stream, u, p, box, mhd, gamma - 1.0);
+  return launched();
+}
+
+int enzoe_riemann_solve_dual_energy(const char *solver, int mhd, double gamma,
+                                    const enzoe_fmap *priml,
+                                    const enzoe_fmap *primr,
+                                    const enzoe_fmap *flux,
+                                    const enzoe_farray *interface_velocity,
+                                    int dim, int stale_depth,
+                                    enzoe_stream_t stream) {
+  std::string s = lower(solver);
+  int kind;
+  if (s == "hlld" && mhd) kind = RS_HLLD;
+  else if (s == "hlle" && mhd) kind = RS_HLLE;
+  else if (s == "hllc" && !mhd) kind = RS_HLLC;
+  else return ENZOE_ERR_CONFIG;
+  const unsigned need = mhd ? kMhdSlots : kHydroSlots;
+  KMap l, r, f;
+  KArr v;
+  // EnzoRiemannUtils.hpp:381-384: the interface velocity must be given
+  if (!valid_dim(dim) || stale_depth < 0 || !to_kmap(priml, need, l) ||
+      !to_kmap(primr, need, r) || !to_kmap(flux, need | kEintSlot, f) ||
+      !to_karr(interface_velocity, v))
+    return ENZOE_ERR_ARG;
+  if (!same_shape(l.n, r.n) || !same_shape(l.n, f.n) || !same_shape(l.n, v.n))
+    return ENZOE_ERR_ARG;
+  const unsigned pmask = present(l) & kPassiveSlots;
+  if ((present(r) & kPassiveSlots) != pmask || (present(f) & pmask) != pmask)
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(f.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (kind == RS_HLLD)
+    hipLaunchKernelGGL((k_op_riemann<RS_HLLD, true, true>), dim3(nblk(count)),
+                       dim3(256), 0, st, l, r, f, box, dim, gamma, pmask, v);
+  else if (kind == RS_HLLE)
+    hipLaunchKernelGGL((k_op_riemann<RS_HLLE, true, true>), dim3(nblk(count)),
+                       dim3(256), 0, st, l, r, f, box, dim, gamma, pmask, v);
+  else
+    hipLaunchKernelGGL((k_op_riemann<RS_HLLC, false, true>), dim3(nblk(count)),
+                       dim3(256), 0, st, l, r, f, box, dim, gamma, pmask, v);
+  return launched();
+}
+
+int enzoe_source_internal_energy(int dim, double dt, double cell_width,
+                                 const enzoe_fmap *primitive,
+                                 const enzoe_fmap *dUcons,
+                                 const enzoe_farray *interface_velocity,
+                                 double pressure_floor, int stale_depth,
+                                 enzoe_stream_t stream) {
+  KMap w, d;
+  KArr v;
+  if (!valid_dim(dim) || stale_depth < 0 ||
+      !to_kmap(primitive, 1u << ENZOE_SLOT_ENERGY, w) || !to_kmap(dUcons, kEintSlot, d) ||
+      !to_karr(interface_velocity, v) || !same_shape(w.n, d.n) ||
+      !shifted(v.n, d.n, dim, -1))
+    return ENZOE_ERR_ARG;
+  // dtdx = dt/cell_width (EnzoSourceInternalEnergy.cpp:34)
+  const double dtdx = dt / cell_width;
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = stale_depth + (a == dim ? 1 : 0);
+    hi[a] = d.n[a] - stale_depth - (a == dim ? 1 : 0);
+  }
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_source_eint, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, w, d, v, box, dim, dtdx, pressure_floor);
+  return launched();
+}
+
+int enzoe_update_quantities_dual_energy(const enzoe_fmap *initial,
+                                        const enzoe_fmap *dUcons,
+                                        const enzoe_fmap *out, int mhd,
+                                        double gamma, double density_floor,
+                                        double pressure_floor, double eta,
+                                        int stale_depth, enzoe_stream_t stream) {
+  KMap a, d, o;
+  const unsigned need = kUpdateSlots | kEintSlot;
+  const unsigned need_out = (mhd ? kMhdSlots : kUpdateSlots) | kEintSlot;
+  if (stale_depth < 0 || !(eta >= 0) || !to_kmap(initial, need, a) ||
+      !to_kmap(dUcons, need, d) || !to_kmap(out, need_out, o) ||
+      !same_shape(a.n, d.n) || !same_shape(a.n, o.n))
+    return ENZOE_ERR_ARG;
+  const unsigned pmask = present(d) & kPassiveSlots;
+  if ((present(a) & pmask) != pmask || (present(o) & pmask) != pmask)
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(a.n, stale_depth + 1, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  const double ggm1 = sync_ggm1(gamma);
+  if (mhd)
+    hipLaunchKernelGGL(k_op_update_de<true>, dim3(nblk(count)), dim3(256), 0,
+                       (hipStream_t)stream, a, d, o, box, gamma, density_floor,
+                       pressure_floor, eta, ggm1, pmask);
+  else
+    hipLaunchKernelGGL(k_op_update_de<false>, dim3(nblk(count)), dim3(256), 0,
+                       (hipStream_t)stream, a, d, o, box, gamma, density_floor,
+                       pressure_floor, eta, ggm1, pmask);
+  return launched();
+}
+
+int enzoe_primitive_from_integration_dual_energy(const enzoe_fmap *integration,
+                                                 const enzoe_fmap *primitive,
+                                                 int mhd, double gamma,
+                                                 int stale_depth,
+                                                 enzoe_stream_t stream) {
+  KMap u, w;
+  const unsigned need = mhd ? kMhdSlots : kHydroSlots;
+  if (stale_depth < 0 || !to_kmap(integration, need | kEintSlot, u) ||
+      !to_kmap(primitive, need, w) || !same_shape(u.n, w.n))
+    return ENZOE_ERR_ARG;
+  const unsigned pmask = present(u) & present(w) & kPassiveSlots;
+  int lo[3], hi[3];
+  stale_box(u.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_primitive_de, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, u, w, box, mhd, gamma - 1.0, pmask);
+  return launched();
+}
+
+int enzoe_pressure_from_integration_dual_energy(const enzoe_fmap *integration,
+                                                const enzoe_farray *pressure,
+                                                double gamma, int stale_depth,
+                                                enzoe_stream_t stream) {
+  KMap u;
+  KArr p;
+  if (stale_depth < 0 || !to_kmap(integration, 1u | kEintSlot, u) ||
+      !to_karr(pressure, p) || !same_shape(u.n, p.n))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(u.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  hipLaunchKernelGGL(k_op_pressure_de, dim3(nblk(count)), dim3(256), 0,
+                     (hipStream_t)stream, u, p, box, gamma - 1.0);
+  return launched();
+}
+
+int enzoe_apply_floor_to_energy_and_sync(const enzoe_fmap *integration,
+                                         int mhd, double gamma,
+                                         double pressure_floor, int dual_energy,
+                                         double eta, int stale_depth,
+                                         enzoe_stream_t stream) {
+  KMap u;
+  const unsigned need = (mhd ? kMhdSlots : kUpdateSlots) | (dual_energy ? kEintSlot : 0u);
+  if (stale_depth < 0 || (dual_energy && !(eta >= 0)) || !to_kmap(integration, need, u))
+    return ENZOE_ERR_ARG;
+  int lo[3], hi[3];
+  stale_box(u.n, stale_depth, lo, hi);
+  long count;
+  Box box = make_box(lo, hi, count);
+  if (count == 0) return ENZOE_OK;
+  const double ggm1 = sync_ggm1(gamma);
+  hipStream_t st = (hipStream_t)stream;
+  if (mhd && dual_energy)
+    hipLaunchKernelGGL((k_op_sync<true, true>), dim3(nblk(count)), dim3(256), 0, st, u,
+                       box, gamma, pressure_floor, eta, ggm1);
+  else if (mhd)
+    hipLaunchKernelGGL((k_op_sync<true, false>), dim3(nblk(count)), dim3(256), 0, st, u,
+                       box, gamma, pressure_floor, eta, ggm1);
+  else if (dual_energy)
+    hipLaunchKernelGGL((k_op_sync<false, true>), dim3(nblk(count)), dim3(256), 0, st, u,
+                       box, gamma, pressure_floor, eta, ggm1);
+  else
+    hipLaunchKernelGGL((k_op_sync<false, false>), dim3(nblk(count)), dim3(256), 0, st, u,
+                       box, gamma, pressure_floor, eta, ggm1);
   return launched();
 }
 

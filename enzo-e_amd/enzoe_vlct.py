@@ -453,7 +453,8 @@ class CycleGraph:
 
 SLOTS = {"density": 0, "velocity_x": 1, "velocity_y": 2, "velocity_z": 3,
          "total_energy": 4, "pressure": 4, "bfield_x": 5, "bfield_y": 6,
-         "bfield_z": 7}
+         "bfield_z": 7, "internal_energy": 16}
+MAP_SLOTS = 17
 
 
 class FArray(ctypes.Structure):
@@ -464,7 +465,7 @@ class FArray(ctypes.Structure):
 
 class FMap(ctypes.Structure):
     """enzoe_fmap: same-shape device views by slot."""
-    _fields_ = [("data", ctypes.c_void_p * 16), ("shape", ctypes.c_int * 3),
+    _fields_ = [("data", ctypes.c_void_p * MAP_SLOTS), ("shape", ctypes.c_int * 3),
                 ("stride", ctypes.c_longlong * 3)]
 
 
@@ -520,6 +521,13 @@ def ops():
         L.enzoe_update_quantities.argtypes = [PM, PM, PM, i, d, d, d, i, vp]
         L.enzoe_primitive_from_integration.argtypes = [PM, PM, i, d, i, vp]
         L.enzoe_pressure_from_integration.argtypes = [PM, PA, i, d, i, vp]
+        L.enzoe_riemann_solve_dual_energy.argtypes = [ctypes.c_char_p, i, d, PM, PM, PM,
+                                                      PA, i, i, vp]
+        L.enzoe_source_internal_energy.argtypes = [i, d, d, PM, PM, PA, d, i, vp]
+        L.enzoe_update_quantities_dual_energy.argtypes = [PM, PM, PM, i, d, d, d, d, i, vp]
+        L.enzoe_primitive_from_integration_dual_energy.argtypes = [PM, PM, i, d, i, vp]
+        L.enzoe_pressure_from_integration_dual_energy.argtypes = [PM, PA, d, i, vp]
+        L.enzoe_apply_floor_to_energy_and_sync.argtypes = [PM, i, d, d, i, d, i, vp]
         L.enzoe_integrator_timestep.argtypes = [PM, PA, i, d, d, d, d, vp, _dp]
         PI = ctypes.POINTER(i)
         L.enzoe_flux_data_size.argtypes = [i, i, i, i]

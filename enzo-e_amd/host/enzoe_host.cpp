@@ -66,7 +66,7 @@ int key_slot(const std::string &key) {
       {"velocity_y", ENZOE_SLOT_VELOCITY_Y}, {"velocity_z", ENZOE_SLOT_VELOCITY_Z},
       {"total_energy", ENZOE_SLOT_ENERGY},   {"pressure", ENZOE_SLOT_ENERGY},
       {"bfield_x", ENZOE_SLOT_BFIELD_X},     {"bfield_y", ENZOE_SLOT_BFIELD_Y},
-      {"bfield_z", ENZOE_SLOT_BFIELD_Z}};
+      {"bfield_z", ENZOE_SLOT_BFIELD_Z},     {"internal_energy", ENZOE_SLOT_INTERNAL_ENERGY}};
   auto it = slots.find(key);
   return it == slots.end() ? -1 : it->second;
 }
@@ -239,7 +239,7 @@ enzoe_fmap EnzoEFltArrayMap::slot_map(const str_vec_t &passive_list) const {
         const int idx = (int)(it - passive_list.begin());
         ENZOE_ASSERT("EnzoEFltArrayMap::slot_map",
                      "at most 8 passively advected scalars are supported",
-                     idx < ENZOE_MAP_SLOTS - ENZOE_SLOT_PASSIVE0);
+                     idx < ENZOE_PASSIVE_SLOTS);
         slot = ENZOE_SLOT_PASSIVE0 + idx;
       }
     }
@@ -315,8 +315,15 @@ void EnzoPhysicsFluidProps::primitive_from_integration(
   const enzoe_fmap u = integration_map.slot_map(passive_list);
   const enzoe_fmap w = primitive_map.slot_map(passive_list);
   const int mhd = u.data[ENZOE_SLOT_BFIELD_X] != nullptr;
-  check_rc(enzoe_primitive_from_integration(&u, &w, mhd, gamma_, stale_depth, g_stream),
-           "EnzoPhysicsFluidProps::primitive_from_integration");
+  // pressure from the internal energy with dual energy
+  // (EnzoComputePressure.cpp:127-134)
+  if (dual_energy_enabled())
+    check_rc(enzoe_primitive_from_integration_dual_energy(&u, &w, mhd, gamma_, stale_depth,
+                                                          g_stream),
+             "EnzoPhysicsFluidProps::primitive_from_integration");
+  else
+    check_rc(enzoe_primitive_from_integration(&u, &w, mhd, gamma_, stale_depth, g_stream),
+             "EnzoPhysicsFluidProps::primitive_from_integration");
 }
 
 void EnzoPhysicsFluidProps::pressure_from_integration(
@@ -325,8 +332,32 @@ void EnzoPhysicsFluidProps::pressure_from_integration(
   const enzoe_fmap u = integration_map.slot_map();
   const enzoe_farray p = pressure.desc();
   const int mhd = u.data[ENZOE_SLOT_BFIELD_X] != nullptr;
-  check_rc(enzoe_pressure_from_integration(&u, &p, mhd, gamma_, stale_depth, g_stream),
-           "EnzoPhysicsFluidProps::pressure_from_integration");
+  if (dual_energy_enabled())
+    check_rc(enzoe_pressure_from_integration_dual_energy(&u, &p, gamma_, stale_depth,
+                                                         g_stream),
+             "EnzoPhysicsFluidProps::pressure_from_integration");
+  else
+    check_rc(enzoe_pressure_from_integration(&u, &p, mhd, gamma_, stale_depth, g_stream),
+             "EnzoPhysicsFluidProps::pressure_from_integration");
+}
+
+void EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync(EnzoEFltArrayMap &integration_map,
+                                                           const int stale_depth) const {
+  // EnzoPhysicsFluidProps.cpp:158-290
+  const bool idual = dual_energy_enabled();
+  ENZOE_ASSERT("EnzoEOSIdeal::apply_floor_to_energy_and_sync",
+               "The current implementation only works when the dual energy formalism is "
+               "disabled or uses the \"modern formulation\"",
+               !idual || dual_energy_modern());
+  str_vec_t keys = {"density", "velocity_x", "velocity_y", "velocity_z", "total_energy"};
+  const bool mag = integration_map.contains("bfield_x");
+  if (mag) keys.insert(keys.end(), {"bfield_x", "bfield_y", "bfield_z"});
+  if (idual) keys.push_back("internal_energy");
+  const enzoe_fmap u = select_keys(integration_map, keys).slot_map();
+  check_rc(enzoe_apply_floor_to_energy_and_sync(&u, mag ? 1 : 0, gamma_, pressure_floor_,
+                                                idual ? 1 : 0, idual ? eta_[0] : 0.0,
+                                                stale_depth, g_stream),
+           "EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync");
 }
 
 namespace enzo {
@@ -704,12 +735,25 @@ void EnzoRiemann::solve(const EnzoEFltArrayMap &prim_map_l,
                         const EnzoEFltArrayMap &prim_map_r,
                         EnzoEFltArrayMap &flux_map, const int dim,
                         const int stale_depth, const str_vec_t &passive_list,
-                        const EFlt3DArray *const) const {
+                        const EFlt3DArray *const interface_velocity) const {
   const str_vec_t pk = concat(primitive_keys_, passive_list);
   const str_vec_t ik = concat(integration_keys_, passive_list);
   const enzoe_fmap l = select_keys(prim_map_l, pk).slot_map(passive_list);
   const enzoe_fmap r = select_keys(prim_map_r, pk).slot_map(passive_list);
   const enzoe_fmap f = select_keys(flux_map, ik).slot_map(passive_list);
+  if (f.data[ENZOE_SLOT_INTERNAL_ENERGY] != nullptr) {
+    // calculate_internal_energy_flux_ (EnzoRiemannImpl.hpp:278-301,
+    // prep_dual_energy_arrays_ EnzoRiemannUtils.hpp:374-395)
+    ENZOE_ASSERT("enzo_riemann_utils::prep_dual_energy_arrays_",
+                 "interface_velocity is expected to be non-NULL when computing the "
+                 "internal energy flux", interface_velocity != nullptr);
+    const enzoe_farray v = interface_velocity->desc();
+    check_rc(enzoe_riemann_solve_dual_energy(solver_.c_str(), mhd_ ? 1 : 0,
+                                             enzo::fluid_props()->gamma(), &l, &r, &f, &v,
+                                             dim, stale_depth, g_stream),
+             "EnzoRiemann::solve");
+    return;
+  }
   check_rc(enzoe_riemann_solve(solver_.c_str(), mhd_ ? 1 : 0,
                                enzo::fluid_props()->gamma(), &l, &r, &f, dim,
                                stale_depth, g_stream),
@@ -942,7 +986,8 @@ EnzoIntegrationQuanUpdate::EnzoIntegrationQuanUpdate(
     for (const std::string &k : integration_quantity_keys)
       if (k.rfind("bfield_", 0) == 0) integration_keys_.push_back(k);
   for (const std::string &k : integration_quantity_keys)
-    if (k.rfind("velocity_", 0) == 0 || k == "total_energy") integration_keys_.push_back(k);
+    if (k.rfind("velocity_", 0) == 0 || k == "total_energy" || k == "internal_energy")
+      integration_keys_.push_back(k);
   ENZOE_ASSERT("EnzoIntegrationQuanUpdate",
                "\"density\" must be a registered integration quantity.",
                !integration_keys_.empty() && integration_keys_[0] == "density");
@@ -986,9 +1031,19 @@ void EnzoIntegrationQuanUpdate::update_quantities(EnzoEFltArrayMap &initial_inte
   if (mhd) out_keys.insert(out_keys.end(), {"bfield_x", "bfield_y", "bfield_z"});
   const enzoe_fmap o = select_keys(out_integration_map, out_keys).slot_map(passive_list);
   const EnzoPhysicsFluidProps *fp = enzo::fluid_props();
-  check_rc(enzoe_update_quantities(&a, &d, &o, mhd ? 1 : 0, fp->gamma(), fp->density_floor(),
-                                   fp->pressure_floor(), stale_depth, g_stream),
-           "EnzoIntegrationQuanUpdate::update_quantities");
+  if (fp->dual_energy_enabled())
+    // internal_energy is among the specific keys; the synchronisation
+    // replaces the energy floor
+    check_rc(enzoe_update_quantities_dual_energy(&a, &d, &o, mhd ? 1 : 0, fp->gamma(),
+                                                 fp->density_floor(), fp->pressure_floor(),
+                                                 fp->dual_energy_eta(), stale_depth,
+                                                 g_stream),
+             "EnzoIntegrationQuanUpdate::update_quantities");
+  else
+    check_rc(enzoe_update_quantities(&a, &d, &o, mhd ? 1 : 0, fp->gamma(),
+                                     fp->density_floor(), fp->pressure_floor(), stale_depth,
+                                     g_stream),
+             "EnzoIntegrationQuanUpdate::update_quantities");
 }
 
 // ------------------------------------------------------- stage commands
@@ -1070,9 +1125,9 @@ void EnzoMHDIntegratorStageCommands::compute_update_stage(
     EnzoEFltArrayMap primitive_map, EnzoEFltArrayMap priml_map,
     EnzoEFltArrayMap primr_map, std::array<EnzoEFltArrayMap, 3> flux_maps_xyz,
     EnzoEFltArrayMap dUcons_map, const EnzoEFltArrayMap &accel_map,
-    const str_vec_t &passive_list, EnzoBfieldMethod *bfield_method,
-    unsigned short stage_index, double cur_dt, int stale_depth,
-    const std::array<enzo_float, 3> cell_widths_xyz) const noexcept {
+    EFlt3DArray interface_vel_arr, const str_vec_t &passive_list,
+    EnzoBfieldMethod *bfield_method, unsigned short stage_index, double cur_dt,
+    int stale_depth, const std::array<enzo_float, 3> cell_widths_xyz) const noexcept {
   // EnzoMHDIntegratorStageCommands.cpp:102-203
   check_valid_stage_index_((int)stage_index);
   EnzoReconstructor *reconstructor = reconstructors_[stage_index].get();
@@ -1086,9 +1141,17 @@ void EnzoMHDIntegratorStageCommands::compute_update_stage(
     CSlice z_slc = (dim == 2) ? CSlice(0, -1) : CSlice(0, nullptr);
     EnzoEFltArrayMap pl_map = priml_map.subarray_map(z_slc, y_slc, x_slc);
     EnzoEFltArrayMap pr_map = primr_map.subarray_map(z_slc, y_slc, x_slc);
+    // dual energy: the interface velocities of this dim, trimmed like the
+    // reconstructed states (:156-169); no scratch otherwise
+    EFlt3DArray sliced_interface_vel_arr;
+    const EFlt3DArray *interface_vel_arr_ptr = nullptr;
+    if (enzo::fluid_props()->dual_energy_enabled()) {
+      sliced_interface_vel_arr = interface_vel_arr.subarray(z_slc, y_slc, x_slc);
+      interface_vel_arr_ptr = &sliced_interface_vel_arr;
+    }
     compute_flux_(dim, cur_dt, cell_widths_xyz[dim], primitive_map, pl_map, pr_map,
-                  flux_maps_xyz[dim], dUcons_map, *reconstructor, bfield_method,
-                  stale_depth, passive_list);
+                  flux_maps_xyz[dim], dUcons_map, interface_vel_arr_ptr, *reconstructor,
+                  bfield_method, stale_depth, passive_list);
   }
   stale_depth += reconstructor->immediate_staling_rate();
   // compute_source_terms_ (EnzoMHDIntegratorStageCommands.cpp:265-295):
@@ -1106,6 +1169,26 @@ void EnzoMHDIntegratorStageCommands::compute_update_stage(
   integration_quan_updater_->update_quantities(tstep_begin_integration_map, dUcons_map,
                                                out_integration_map, stale_depth,
                                                passive_list);
+}
+
+void EnzoSourceInternalEnergy::calculate_source(const int dim, const double dt,
+                                                const enzo_float cell_width,
+                                                const EnzoEFltArrayMap &prim_map,
+                                                EnzoEFltArrayMap &dUcons_map,
+                                                const EFlt3DArray &interface_velocity,
+                                                const int stale_depth) const throw() {
+  // EnzoSourceInternalEnergy.cpp:16-92
+  const EnzoPhysicsFluidProps *fp = enzo::fluid_props();
+  ENZOE_ASSERT("EnzoSourceInternalEnergy::calculate_source",
+               "This function should only be called if the \"modern formulation\" of the "
+               "dual energy formalism is in use.",
+               fp->dual_energy_modern());
+  const enzoe_fmap w = select_keys(prim_map, {"pressure"}).slot_map();
+  const enzoe_fmap d = select_keys(dUcons_map, {"internal_energy"}).slot_map();
+  const enzoe_farray v = interface_velocity.desc();
+  check_rc(enzoe_source_internal_energy(dim, dt, cell_width, &w, &d, &v,
+                                        fp->pressure_floor(), stale_depth, g_stream),
+           "EnzoSourceInternalEnergy::calculate_source");
 }
 
 void EnzoSourceGravity::calculate_source(const double dt, const EnzoEFltArrayMap &prim_map,
@@ -1138,9 +1221,9 @@ void EnzoMHDIntegratorStageCommands::compute_flux_(
     const int dim, const double cur_dt, const enzo_float cell_width,
     EnzoEFltArrayMap &primitive_map, EnzoEFltArrayMap &priml_map,
     EnzoEFltArrayMap &primr_map, EnzoEFltArrayMap &flux_map,
-    EnzoEFltArrayMap &dUcons_map, EnzoReconstructor &reconstructor,
-    EnzoBfieldMethod *bfield_method, const int stale_depth,
-    const str_vec_t &passive_list) const noexcept {
+    EnzoEFltArrayMap &dUcons_map, const EFlt3DArray *const interface_velocity_arr_ptr,
+    EnzoReconstructor &reconstructor, EnzoBfieldMethod *bfield_method,
+    const int stale_depth, const str_vec_t &passive_list) const noexcept {
   // EnzoMHDIntegratorStageCommands.cpp:207-261
   reconstructor.reconstruct_interface(primitive_map, priml_map, primr_map, dim,
                                       stale_depth, passive_list);
@@ -1148,10 +1231,16 @@ void EnzoMHDIntegratorStageCommands::compute_flux_(
   if (bfield_method != nullptr)
     bfield_method->correct_reconstructed_bfield(priml_map, primr_map, dim, cur_stale_depth);
   riemann_solver_->solve(priml_map, primr_map, flux_map, dim, cur_stale_depth,
-                         passive_list, nullptr);
+                         passive_list, interface_velocity_arr_ptr);
   integration_quan_updater_->accumulate_flux_component(dim, cur_dt, cell_width, flux_map,
                                                        dUcons_map, cur_stale_depth,
                                                        passive_list);
+  // the internal-energy source term of this dim (:248-255)
+  if (enzo::fluid_props()->dual_energy_enabled()) {
+    EnzoSourceInternalEnergy eint_src;
+    eint_src.calculate_source(dim, cur_dt, cell_width, primitive_map, dUcons_map,
+                              *interface_velocity_arr_ptr, cur_stale_depth);
+  }
   if (bfield_method != nullptr)
     bfield_method->identify_upwind(flux_map, dim, cur_stale_depth);
 }
@@ -1176,7 +1265,7 @@ EnzoVlctScratchSpace::EnzoVlctScratchSpace(const std::array<int, 3> &shape,
                                            const str_vec_t &integration_keys,
                                            const str_vec_t &primitive_keys,
                                            const str_vec_t &dUcons_keys,
-                                           const str_vec_t &passives)
+                                           const str_vec_t &passives, bool dual_energy)
     : temp_integration_map("temp_integration", concat(integration_keys, passives), shape),
       primitive_map("primitive", concat(primitive_keys, passives), shape),
       priml_map("priml", concat(primitive_keys, passives), shape),
@@ -1188,7 +1277,9 @@ EnzoVlctScratchSpace::EnzoVlctScratchSpace(const std::array<int, 3> &shape,
       zflux_map("zflux", concat(integration_keys, passives),
                 std::array<int, 3>{{shape[0] - 1, shape[1], shape[2]}}),
       dUcons_map("dUcons", concat(dUcons_keys, passives), shape),
-      passive_list(passives) {}
+      passive_list(passives) {
+  if (dual_energy) interface_vel_arr = EFlt3DArray(shape[0], shape[1], shape[2]);
+}
 
 // ----------------------------------------------------------------- method
 
@@ -1271,10 +1362,7 @@ EnzoMethodMHDVlct::EnzoMethodMHDVlct(ParameterGroup p, bool store_fluxes_for_cor
   prm.pressure_floor = fp->pressure_floor();
   prm.precision = precision_;
   if (fp->dual_energy_enabled()) {
-    // the internal-energy terms are part of the fused device step only
-    ENZOE_ASSERT("EnzoMethodMHDVlct",
-                 "the dual-energy formalism needs " + p.full_name("device_fused") + " = true",
-                 fused_);
+    // the FP32 flux path has no internal-energy terms
     ENZOE_ASSERT("EnzoMethodMHDVlct",
                  "the dual-energy formalism needs " + p.full_name("precision") +
                      " = \"double\"", precision_ == ENZOE_PREC_DOUBLE);
@@ -1330,7 +1418,8 @@ EnzoVlctScratchSpace *EnzoMethodMHDVlct::get_scratch_ptr_(
     scratch_space_ = new EnzoVlctScratchSpace(shape, integration_field_list_,
                                               primitive_field_list_,
                                               integrator_->dUcons_map_keys(),
-                                              passive_list);
+                                              passive_list,
+                                              enzo::fluid_props()->dual_energy_enabled());
   return scratch_space_;
 }
 
@@ -1467,7 +1556,7 @@ void EnzoMethodMHDVlct::compute_staged_(Block *block) noexcept {
     integrator_->compute_update_stage(external_integration_map, cur_stage_integration_map,
                                       out_integration_map, primitive_map, priml_map,
                                       primr_map, flux_maps_xyz, dUcons_map, accel_map,
-                                      passive_list,
+                                      scratch->interface_vel_arr, passive_list,
                                       bfield_method_, stage_index, cur_dt, stale_depth,
                                       cell_widths_xyz);
     stale_depth += integrator_->staling_from_stage((int)stage_index);
@@ -1507,8 +1596,12 @@ double EnzoMethodMHDVlct::timestep(Block *block) throw() {
     return dt;  // the library applies the same courant factor
   }
   EnzoEFltArrayMap integration_map = get_integration_map_(block, str_vec_t());
+  EnzoPhysicsFluidProps *fluid_props = enzo::fluid_props();
+  // synchronise eint and etot (EnzoMethodMHDVlct.cpp:563-568)
+  if (fluid_props->dual_energy_enabled())
+    fluid_props->apply_floor_to_energy_and_sync(integration_map, 0);
   EFlt3DArray pressure = block->field_view("pressure");
-  enzo::fluid_props()->pressure_from_integration(integration_map, pressure, 0);
+  fluid_props->pressure_from_integration(integration_map, pressure, 0);
   const double *w = block->CellWidth();
   double dtBaryons = integrator_->timestep(integration_map, pressure, w[0], w[1], w[2]);
   return dtBaryons * courant_;

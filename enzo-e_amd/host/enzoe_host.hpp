@@ -165,6 +165,9 @@ class EnzoPhysicsFluidProps {
                                   EnzoEFltArrayMap &primitive_map,
                                   int stale_depth, const str_vec_t &passive_list,
                                   bool ignore_grackle = false) const;
+  // EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync (:158-290), in place
+  void apply_floor_to_energy_and_sync(EnzoEFltArrayMap &integration_map,
+                                      const int stale_depth) const;
   // EnzoPhysicsFluidProps::pressure_from_integration (:142-156)
   void pressure_from_integration(const EnzoEFltArrayMap &integration_map,
                                  const EFlt3DArray &pressure, int stale_depth,
@@ -560,6 +563,18 @@ struct EnzoMHDIntegratorStageArgPack {
   std::string mhd_choice;
 };
 
+// EnzoSourceInternalEnergy (toolkit/EnzoSourceInternalEnergy.hpp,
+// .cpp:16-92): the internal-energy source term -p div(v) of the dual-energy
+// formalism along one dim, from the interface velocities of the Riemann
+// solve; accumulated into dUcons_map's "internal_energy"
+class EnzoSourceInternalEnergy {
+ public:
+  void calculate_source(const int dim, const double dt, const enzo_float cell_width,
+                        const EnzoEFltArrayMap &prim_map, EnzoEFltArrayMap &dUcons_map,
+                        const EFlt3DArray &interface_velocity,
+                        const int stale_depth) const throw();
+};
+
 // EnzoSourceGravity (toolkit/EnzoSourceGravity.hpp, .cpp:17-69)
 class EnzoSourceGravity {
  public:
@@ -589,6 +604,7 @@ class EnzoMHDIntegratorStageCommands {
                             std::array<EnzoEFltArrayMap, 3> flux_maps_xyz,
                             EnzoEFltArrayMap dUcons_map,
                             const EnzoEFltArrayMap &accel_map,
+                            EFlt3DArray interface_vel_arr,
                             const str_vec_t &passive_list,
                             EnzoBfieldMethod *bfield_method,
                             unsigned short stage_index, double cur_dt,
@@ -612,6 +628,7 @@ class EnzoMHDIntegratorStageCommands {
                      const enzo_float cell_width, EnzoEFltArrayMap &primitive_map,
                      EnzoEFltArrayMap &priml_map, EnzoEFltArrayMap &primr_map,
                      EnzoEFltArrayMap &flux_map, EnzoEFltArrayMap &dUcons_map,
+                     const EFlt3DArray *const interface_velocity_arr_ptr,
                      EnzoReconstructor &reconstructor,
                      EnzoBfieldMethod *bfield_method, const int stale_depth,
                      const str_vec_t &passive_list) const noexcept;
@@ -628,9 +645,12 @@ struct EnzoVlctScratchSpace {
                        const str_vec_t &integration_keys,
                        const str_vec_t &primitive_keys,
                        const str_vec_t &dUcons_keys,
-                       const str_vec_t &passive_list);
+                       const str_vec_t &passive_list, bool dual_energy = false);
   EnzoEFltArrayMap temp_integration_map, primitive_map, priml_map, primr_map,
       xflux_map, yflux_map, zflux_map, dUcons_map;
+  // the interface velocities of the dual-energy formalism (cell-shaped,
+  // sliced per dim like the reconstructed states), null otherwise
+  EFlt3DArray interface_vel_arr;
   str_vec_t passive_list;
 };
 

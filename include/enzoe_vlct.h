@@ -196,7 +196,7 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
  * passed without copies.  A map (EnzoEFltArrayMap = ViewMap<double>,
  * enzo_typedefs.hpp:36; view_ViewMap.hpp:43-395) is a set of same-shape,
  * same-stride views addressed by slot instead of by key:                    */
-#define ENZOE_MAP_SLOTS 16
+#define ENZOE_MAP_SLOTS 17
 #define ENZOE_SLOT_DENSITY 0     /* "density"                               */
 #define ENZOE_SLOT_VELOCITY_X 1  /* "velocity_x"                            */
 #define ENZOE_SLOT_VELOCITY_Y 2  /* "velocity_y"                            */
@@ -210,6 +210,10 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
                                     conserved form (densities) in integration,
                                     dU and flux maps, specific form (mass
                                     fractions) in primitive maps            */
+#define ENZOE_PASSIVE_SLOTS 8    /* how many passive slots there are        */
+#define ENZOE_SLOT_INTERNAL_ENERGY 16 /* "internal_energy" (dual energy):
+                                    specific in integration maps, conserved
+                                    change in dUcons, flux in flux maps     */
 
 typedef struct enzoe_farray {
   double *data;           /* element (0,0,0) of the view                      */
@@ -337,6 +341,69 @@ int enzoe_primitive_from_integration(const enzoe_fmap *integration,
                                      const enzoe_fmap *primitive, int mhd,
                                      double gamma, int stale_depth,
                                      enzoe_stream_t stream);
+
+/* ---- the dual-energy formalism ("modern" formulation) on views ----------
+ * EnzoRiemannImpl::solve with calculate_internal_energy_flux_
+ * (riemann/EnzoRiemannImpl.hpp:240-301): enzoe_riemann_solve plus the
+ * internal-energy flux into the flux map's internal_energy slot
+ * (passive_eint_flux, EnzoRiemannUtils.hpp:224-249) and vi_bar into
+ * interface_velocity (the flux view's shape), as EnzoRiemannHLLD.hpp:404-447,
+ * EnzoRiemannHLL.hpp:322-344, EnzoRiemannHLLC.hpp:159-170 compute them.    */
+int enzoe_riemann_solve_dual_energy(const char *solver, int mhd, double gamma,
+                                    const enzoe_fmap *priml,
+                                    const enzoe_fmap *primr,
+                                    const enzoe_fmap *flux,
+                                    const enzoe_farray *interface_velocity,
+                                    int dim, int stale_depth,
+                                    enzoe_stream_t stream);
+
+/* EnzoSourceInternalEnergy::calculate_source
+ * (toolkit/EnzoSourceInternalEnergy.cpp:16-92): cells with index along dim
+ * in [stale+1, n-stale-1) and [stale, n-stale) across:
+ * dUcons[internal_energy](c) -= dt/dx * max(p(c), pressure_floor) *
+ * (vi_bar(c+1/2) - vi_bar(c-1/2)); p is the primitive map's pressure slot,
+ * interface_velocity the face-trimmed view along dim.                       */
+int enzoe_source_internal_energy(int dim, double dt, double cell_width,
+                                 const enzoe_fmap *primitive,
+                                 const enzoe_fmap *dUcons,
+                                 const enzoe_farray *interface_velocity,
+                                 double pressure_floor, int stale_depth,
+                                 enzoe_stream_t stream);
+
+/* enzoe_update_quantities with the internal energy as one more specific
+ * quantity (EnzoIntegrationQuanUpdate.cpp:183-238; slot internal_energy of
+ * initial, dUcons and out required) and the synchronisation of
+ * apply_floor_to_energy_and_sync (EnzoPhysicsFluidProps.cpp:158-290) with
+ * the given eta in place of the energy floor.                               */
+int enzoe_update_quantities_dual_energy(const enzoe_fmap *initial,
+                                        const enzoe_fmap *dUcons,
+                                        const enzoe_fmap *out, int mhd,
+                                        double gamma, double density_floor,
+                                        double pressure_floor, double eta,
+                                        int stale_depth, enzoe_stream_t stream);
+
+/* primitive_from_integration / pressure_from_integration with dual energy:
+ * p = (gamma-1) rho eint (EnzoComputePressure.cpp:127-134).                 */
+int enzoe_primitive_from_integration_dual_energy(const enzoe_fmap *integration,
+                                                 const enzoe_fmap *primitive,
+                                                 int mhd, double gamma,
+                                                 int stale_depth,
+                                                 enzoe_stream_t stream);
+int enzoe_pressure_from_integration_dual_energy(const enzoe_fmap *integration,
+                                                const enzoe_farray *pressure,
+                                                double gamma, int stale_depth,
+                                                enzoe_stream_t stream);
+
+/* EnzoPhysicsFluidProps::apply_floor_to_energy_and_sync
+ * (EnzoPhysicsFluidProps.cpp:158-290) over [stale, n-stale) of an
+ * integration map, in place: the total-energy floor (dual_energy = 0) or the
+ * synchronisation of internal and total energy with eta (dual_energy = 1,
+ * internal_energy slot required).                                           */
+int enzoe_apply_floor_to_energy_and_sync(const enzoe_fmap *integration,
+                                         int mhd, double gamma,
+                                         double pressure_floor, int dual_energy,
+                                         double eta, int stale_depth,
+                                         enzoe_stream_t stream);
 
 /* EnzoMHDIntegratorStageCommands::timestep
  * (EnzoMHDIntegratorStageCommands.cpp:299-366): min over every cell of the

@@ -439,8 +439,6 @@ def test_dual_energy_configuration():
 @pytest.mark.parametrize("params,dual,message", [
     ("mhd_choice = constrained_transport", ("bryan95", []),
      "selected formulation of dual energy formalism is incompatible"),
-    ("mhd_choice = constrained_transport; device_fused = false", ("modern", [0.01]),
-     "the dual-energy formalism needs"),
     ("mhd_choice = constrained_transport; precision = mixed", ("modern", []),
      "the dual-energy formalism needs"),
     ("mhd_choice = constrained_transport", ("disabled", [0.01]), "can't be specified"),
@@ -458,3 +456,48 @@ def test_host_dual_energy_modern_constructs():
     r = _run_host("mhd_choice = constrained_transport", ("modern", [0.00769]))
     assert r.returncode == 0, r.stderr
     assert "constructed mhd_vlct 0.3" in r.stdout
+    # the stage-by-stage plugin path carries the formalism too
+    r = _run_host("mhd_choice = no_bfield; riemann_solver = hllc; device_fused = false",
+                  ("modern", []))
+    assert r.returncode == 0, r.stderr
+
+
+def test_dual_energy_single_operations_validate_without_a_device():
+    """The dual-energy entry points check their views before launching: the
+    flux map needs the internal_energy slot and the interface velocity view
+    (EnzoRiemannUtils.hpp:381-384), the source needs the pressure and the
+    internal-energy change and a face-trimmed velocity view, the update and
+    the synchronisation need the internal-energy slot and eta >= 0, and the
+    plain update refuses an internal-energy change."""
+    L = ev.ops()
+    fake = 1 << 40
+    keys = {k: fake for k in ("density", "velocity_x", "velocity_y", "velocity_z",
+                              "total_energy", "bfield_x", "bfield_y", "bfield_z")}
+    face = ev.fmap(keys, (8, 8, 7))
+    with_e = ev.fmap(dict(keys, internal_energy=fake), (8, 8, 7))
+    vel = ev.farray(fake, (8, 8, 7))
+    assert L.enzoe_riemann_solve_dual_energy(b"hlld", 1, 5 / 3, face, face, face, vel, 0, 0,
+                                             None) == ERR_ARG
+    assert L.enzoe_riemann_solve_dual_energy(b"hlld", 1, 5 / 3, face, face, with_e, None, 0,
+                                             0, None) == ERR_ARG
+    assert L.enzoe_riemann_solve_dual_energy(b"hllc", 1, 5 / 3, face, face, with_e, vel, 0,
+                                             0, None) == ERR_CONFIG
+    cell = ev.fmap(dict(keys, internal_energy=fake), (8, 8, 8))
+    prim = ev.fmap({"pressure": fake}, (8, 8, 8))
+    assert L.enzoe_source_internal_energy(0, 0.1, 0.1, prim, cell, ev.farray(fake, (8, 8, 8)),
+                                          1e-30, 0, None) == ERR_ARG
+    assert L.enzoe_source_internal_energy(0, 0.1, 0.1, prim, ev.fmap(keys, (8, 8, 8)),
+                                          vel, 1e-30, 0, None) == ERR_ARG
+    plain = ev.fmap(keys, (8, 8, 8))
+    assert L.enzoe_update_quantities_dual_energy(plain, plain, plain, 1, 5 / 3, 1e-30,
+                                                 1e-30, 0.01, 0, None) == ERR_ARG
+    assert L.enzoe_update_quantities_dual_energy(cell, cell, cell, 1, 5 / 3, 1e-30, 1e-30,
+                                                 -1.0, 0, None) == ERR_ARG
+    assert L.enzoe_update_quantities(cell, cell, cell, 1, 5 / 3, 1e-30, 1e-30, 0,
+                                     None) == ERR_ARG
+    assert L.enzoe_apply_floor_to_energy_and_sync(plain, 1, 5 / 3, 1e-30, 1, 0.01, 0,
+                                                  None) == ERR_ARG
+    assert L.enzoe_pressure_from_integration_dual_energy(plain, ev.farray(fake, (8, 8, 8)),
+                                                         5 / 3, 0, None) == ERR_ARG
+    assert L.enzoe_primitive_from_integration_dual_energy(plain, plain, 1, 5 / 3, 0,
+                                                          None) == ERR_ARG

@@ -241,3 +241,173 @@ def test_dual_energy_tube_golden_through_the_product(axis, reverse):
     # input/vlct/testing_utils.py:275-292 (abs_tol) and the exact --std check
     assert np.isclose(l1, DE_L1, rtol=1e-13, atol=7e-14)
     assert np.std(cols) == 0.0
+
+
+# ------------------------------------------------------------- plugin path
+# EnzoMethodMHDVlct with device_fused = false: the reference's composition
+# stage by stage, the dual-energy calls included (EnzoRiemann::solve with
+# interface velocities, EnzoSourceInternalEnergy, the dual-energy update and
+# the synchronisation before the CFL dt), each a single-operation entry.
+
+def _host_params(case):
+    return ("mhd_choice = %s; riemann_solver = %s; reconstruct_method = %s; "
+            "theta_limiter = %r; courant = 0.4; device_fused = false"
+            % ("constrained_transport" if case["mhd"] else "no_bfield",
+               case["riemann"], case["recon"][1], case.get("theta", 2.0)))
+
+
+PLUGIN_CASES = [CASES[0], CASES[1], CASES[3], CASES[4]]
+
+
+@pytest.mark.parametrize("case", PLUGIN_CASES, ids=lambda c: "%s-%s-%s" % (
+    "mhd" if c["mhd"] else "hd", c["riemann"], c["recon"][1]))
+def test_dual_energy_staged_plugin_path_equals_the_oracle(case):
+    from test_gpu_plugins import host_cycle
+    print("staged case", case, flush=True)
+    ev = _ev()
+    grid = _de_grid(case)
+    dg = device_copy(grid, method_for(grid.cfg))
+    hm = ev.HostMethod(_host_params(case), gamma=grid.cfg.gamma,
+                       dual_energy=("modern", [grid.cfg.eta]))
+    t = 0.0
+    for _ in range(3):
+        dt_o = grid.cycle(t, 10.0)
+        dt_g = host_cycle(hm, dg, t, 10.0)
+        assert dt_g == dt_o
+        t += dt_o
+    ev.synchronize()
+    _assert_equal_grids(dg, grid)
+
+
+def _dev(arr):
+    ev = _ev()
+    buf = ev.DeviceBuffer(arr.nbytes)
+    ev.to_device(buf.ptr.value, np.ascontiguousarray(arr))
+    return buf
+
+
+@pytest.mark.parametrize("solver,mhd", [("hlld", 1), ("hlle", 1), ("hllc", 0)])
+def test_single_operation_riemann_dual_energy(solver, mhd):
+    """enzoe_riemann_solve_dual_energy against the oracle's solve with the
+    internal-energy flux and vi_bar, on random face states (both signs of
+    the normal velocity, sub- and supersonic)."""
+    ev = _ev()
+    rng = np.random.default_rng(21)
+    mz, my, mx, dim, s = 7, 8, 9, 2, 1
+    fshape = (mz - 1, my, mx)
+    nq = 8 if mhd else 5
+    keys = ["density", "velocity_x", "velocity_y", "velocity_z", "pressure",
+            "bfield_x", "bfield_y", "bfield_z"][:nq]
+    fkeys = ["density", "velocity_x", "velocity_y", "velocity_z", "total_energy",
+             "bfield_x", "bfield_y", "bfield_z"][:nq]
+
+    def states():
+        w = np.zeros((8,) + fshape)
+        w[0] = rng.uniform(0.3, 2.0, fshape)
+        w[1:4] = rng.normal(0, 2.0, (3,) + fshape)
+        w[4] = rng.uniform(0.1, 1.5, fshape)
+        if mhd:
+            w[5:8] = rng.normal(0, 0.7, (3,) + fshape)
+        return w
+    wl, wr = states(), states()
+    fn = int(np.prod(fshape))
+    dl, dr = _dev(wl), _dev(wr)
+    dflux, dvel = _dev(np.zeros((9,) + fshape)), _dev(np.zeros(fshape))
+    pl = ev.fmap({k: dl.ptr.value + 8 * fn * q for q, k in enumerate(keys)}, fshape)
+    pr = ev.fmap({k: dr.ptr.value + 8 * fn * q for q, k in enumerate(keys)}, fshape)
+    fmapd = {k: dflux.ptr.value + 8 * fn * q for q, k in enumerate(fkeys)}
+    fmapd["internal_energy"] = dflux.ptr.value + 8 * fn * 8
+    flux = ev.fmap(fmapd, fshape)
+    vel = ev.farray(dvel.ptr.value, fshape)
+    L = ev.ops()
+    assert L.enzoe_riemann_solve_dual_energy(solver.encode(), mhd, 5 / 3, pl, pr, flux,
+                                             vel, dim, s, None) == 0
+    ev.synchronize()
+    got = ev.from_device(dflux.ptr.value, (9,) + fshape)
+    got_v = ev.from_device(dvel.ptr.value, fshape)
+    cfg = po.make_config(riemann=solver, mhd=bool(mhd))
+    P = ctypes.POINTER(ctypes.c_double) * 8
+    ol = [np.ascontiguousarray(wl[q]) for q in range(8)]
+    orr = [np.ascontiguousarray(wr[q]) for q in range(8)]
+    of = [np.zeros(fshape) for _ in range(8)]
+    oe, ov = np.zeros(fshape), np.zeros(fshape)
+    po.lib().o_riemann_solve_de(ctypes.byref(cfg), P(*[po.dptr(a) for a in ol]),
+                                P(*[po.dptr(a) for a in orr]), P(*[po.dptr(a) for a in of]),
+                                po.dptr(oe), po.dptr(ov), mz, my, mx, dim, s)
+    sl = np.s_[s:fshape[0] - s, s:fshape[1] - s, s:fshape[2] - s]
+    for q in range(nq):
+        np.testing.assert_array_equal(got[q][sl], of[q][sl])
+    np.testing.assert_array_equal(got[8][sl], oe[sl])
+    np.testing.assert_array_equal(got_v[sl], ov[sl])
+    assert np.any(got_v[sl] != 0.0) and np.any(got[8][sl] < 0) and np.any(got[8][sl] > 0)
+
+
+@pytest.mark.parametrize("mhd,dual", [(1, 1), (0, 1), (1, 0), (0, 0)])
+def test_single_operation_floor_and_sync(mhd, dual):
+    """enzoe_apply_floor_to_energy_and_sync against the oracle on random
+    cells: some with the internal energy replaced, some kept, some floored."""
+    ev = _ev()
+    rng = np.random.default_rng(33 + 2 * mhd + dual)
+    shape = (6, 7, 8)
+    U = np.zeros((9,) + shape)
+    U[0] = rng.uniform(0.5, 2.0, shape)
+    U[1:4] = rng.normal(0, 1.0, (3,) + shape)
+    if mhd:
+        U[5:8] = rng.normal(0, 0.5, (3,) + shape)
+    ke = 0.5 * np.sum(U[1:4] ** 2, axis=0)
+    me = 0.5 * np.sum(U[5:8] ** 2, axis=0) / U[0]
+    U[8] = rng.uniform(-0.01, 1.0, shape)              # eint, some below the floor
+    U[4] = ke + me + U[8] * rng.uniform(0.3, 1.7, shape)
+    cfg = po.make_config(gamma=1.4, mhd=bool(mhd), pressure_floor=0.05,
+                         dual_energy_eta=0.05 if dual else None)
+    n = int(np.prod(shape))
+    d = _dev(U)
+    keys = ["density", "velocity_x", "velocity_y", "velocity_z", "total_energy",
+            "bfield_x", "bfield_y", "bfield_z", "internal_energy"]
+    use = [q for q in range(9) if (mhd or q not in (5, 6, 7)) and (dual or q != 8)]
+    m = ev.fmap({keys[q]: d.ptr.value + 8 * n * q for q in use}, shape)
+    assert ev.ops().enzoe_apply_floor_to_energy_and_sync(m, mhd, 1.4, 0.05, dual,
+                                                         cfg.eta, 1, None) == 0
+    ev.synchronize()
+    got = ev.from_device(d.ptr.value, U.shape)
+    ref = [U[q].copy() for q in range(8)]
+    e = U[8].copy()
+    P = ctypes.POINTER(ctypes.c_double) * 8
+    po.lib().o_floor_energy_and_sync(ctypes.byref(cfg), P(*[po.dptr(a) for a in ref]),
+                                     po.dptr(e), *shape, 1)
+    np.testing.assert_array_equal(got[4], ref[4])
+    if dual:
+        np.testing.assert_array_equal(got[8], e)
+        inner = np.s_[1:-1, 1:-1, 1:-1]
+        changed = got[8][inner] != U[8][inner]
+        assert np.any(changed) and np.any(~changed)
+
+
+def test_dual_energy_tube_golden_through_the_staged_plugin_path():
+    """The x deck of the dual-energy shock tube through the C++ layer's
+    stage-by-stage composition (device_fused = false)."""
+    from test_gpu_plugins import host_cycle
+    ev = _ev()
+    grid, dg = _tube(0)
+    hm = ev.HostMethod("mhd_choice = constrained_transport; riemann_solver = hlld; "
+                       "reconstruct_method = plm; theta_limiter = 2.0; courant = 0.4; "
+                       "device_fused = false", gamma=1.4,
+                       dual_energy=("modern", [po.DE_TUBE_ETA]))
+    t, n = 0.0, 0
+    while t < 0.25:
+        t += host_cycle(hm, dg, t, 0.25)
+        n += 1
+    ev.synchronize()
+    assert n > 1000
+    for i in range(len(dg.blocks)):
+        arrs = dg.blocks[i].download()
+        view = grid.block_arrays(i)
+        for k, v in arrs.items():
+            if k in view:
+                view[k][...] = v
+    table = po.load_reference_table(GOLDEN)
+    cw = 3.96875 / 508
+    l1 = po.shifted_table_l1(grid.gather, table, 0, 0.0, cw, offset=po.DE_TUBE_OFFSET,
+                             bkg_velocity=(po.DE_TUBE_AXIS_VELOCITY, 0.0, 0.0))
+    print("staged cycles", n, "L1", repr(l1))
+    assert np.isclose(l1, DE_L1, rtol=1e-13, atol=7e-14)
