@@ -62,6 +62,9 @@ def parse():
                          "smr_flux_correct: 64 x (1 coarse + 4 fine) 32^3 hydro "
                          "blocks with coarse/fine ghost transfers and flux "
                          "corrections")
+    ap.add_argument("--dual-energy", action="store_true",
+                    help="orszag_tang with the dual-energy formalism on (eta 0.001) "
+                         "as the measured step (for profiling that path)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (HBM counter scale)")
     return ap.parse_args()
@@ -531,7 +534,10 @@ def main():
         out["copy_kernel_GBs"] = copy_bandwidth()["GBs"]
         print(json.dumps(out))
         return
-    r = timed_run(a.precision, a.n, a.steps, a.warmup)
+    if a.dual_energy and a.precision != "fp64":
+        print("the dual-energy formalism runs in FP64 only", file=sys.stderr)
+        sys.exit(2)
+    r = timed_run(a.precision, a.n, a.steps, a.warmup, dual_energy=a.dual_energy)
     other = dual = None
     if not a.no_compare:
         other = timed_run("mixed" if a.precision == "fp64" else "fp64", a.n,
@@ -554,7 +560,8 @@ def main():
         "config": {"workload": r["workload"],
                    "block": [a.n, a.n, a.n], "ghost": 3, "riemann": "hlld",
                    "reconstruct": "plm", "theta": 2.0, "courant": 0.4,
-                   "precision": a.precision, "launch_mask": r["launch_mask"]},
+                   "precision": a.precision, "launch_mask": r["launch_mask"],
+                   "dual_energy": a.dual_energy},
         "roofline": {
             "bound": "hbm",
             "kernel": r["dom"],

@@ -13,7 +13,8 @@
  *     linear waves (input/vlct/run_MHD_linear_wave_test.py:178-188,
  *     input/vlct/run_HD_linear_wave_test.py:70-80) and the RJ2a shock tube
  *     (input/vlct/run_MHD_shock_tube_test.py:82-87) at rtol 1e-13 /
- *     atol 7e-14 (input/vlct/testing_utils.py:275-292);
+ *     atol 7e-14 (input/vlct/testing_utils.py:275-292); the dual-energy
+ *     Sod shock tube (input/vlct/run_dual_energy_shock_tube_test.py:52-90);
  *   - per function: closed forms, invariants and brute-force cases.
  *
  * Layout: every array is dense, row-major (z, y, x), x fastest, exactly as a
