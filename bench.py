@@ -37,7 +37,7 @@ FLUX_BYTES_PER_CELL = {0: (11 + 21) * 8.0, 1: 11 * 8.0 + 21 * 4.0}
 FLUX_BYTES_PER_CELL_HD = {0: (5 + 15) * 8.0, 1: 5 * 8.0 + 15 * 4.0}
 STEP_BYTES_PER_ZONE_HD = 200.0
 WORKLOADS = ["orszag_tang", "fast_wave", "batched_amr", "turbulence", "smr_flux_correct",
-             "kelvin_helmholtz"]
+             "kelvin_helmholtz", "output"]
 
 
 def parse():
@@ -61,7 +61,8 @@ def parse():
                          "kelvin_helmholtz: 512 blocks of 128^3, hydro; "
                          "smr_flux_correct: 64 x (1 coarse + 4 fine) 32^3 hydro "
                          "blocks with coarse/fine ghost transfers and flux "
-                         "corrections")
+                         "corrections; output: HDF5 block output and restart "
+                         "input of the 256^3 Orszag-Tang state")
     ap.add_argument("--dual-energy", action="store_true",
                     help="orszag_tang with the dual-energy formalism on (eta 0.001) "
                          "as the measured step (for profiling that path)")
@@ -514,6 +515,61 @@ def run_smr_flux_correct(a):
     }
 
 
+def run_output(a):
+    """HDF5 block output of the device-resident 256^3 Orszag-Tang MHD block
+    (12 fields with ghost zones, 1.73 GB) and its restart input, against
+    the same file written from host copies of the fields."""
+    import shutil
+    import tempfile
+    import enzoe_io as eio
+    method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm", theta=2.0,
+                           courant=0.4)
+    grid = make_state(a.n, method)
+    step(grid)
+    ev.synchronize()
+    blk = grid.blocks[0]
+    nbytes = sum(int(np.prod(s)) * 8 for s in blk.shapes.values())
+    host = eio.HostBlock(blk.download(), blk.ghost)
+    reps = max(1, min(a.steps, 3))
+    tmp = tempfile.mkdtemp(prefix="enzoe_output_")
+    cwd = os.getcwd()
+    try:
+        os.chdir(tmp)
+        times = {"device": [], "host": [], "read": []}
+        back = ev.DeviceBlock(blk.mx, blk.my, blk.mz, blk.ghost, (1.0 / a.n,) * 3, mhd=True)
+        for r in range(reps):
+            for kind, b in (("device", blk), ("host", host)):
+                t0 = time.perf_counter()
+                path = eio.OutputData(None, name=("%s-%%d.h5" % kind, "count")).write(
+                    [b], (0, 0, 0), (1, 1, 1), (1, 1, 1), 1, 0.0, 0.0, count=r)
+                times[kind].append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            eio.read_block("./device-%d.h5" % r, "B__", back)
+            times["read"].append(time.perf_counter() - t0)
+            for k in ("device-%d" % r, "host-%d" % r):
+                for ext in (".h5", ".block_list", ".file_list"):
+                    os.remove(k + ext)
+        got = back.download()
+        same = all(np.array_equal(got[k], host.arrays[k]) for k in got)
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(tmp, ignore_errors=True)
+    best = {k: min(v) for k, v in times.items()}
+    return {
+        "metric": "HDF5 block output rate", "value": nbytes / best["device"] / 1e9,
+        "unit": "GB/s", "n_gpus": 1, "steps": reps, "warmup": 0,
+        "ms_per_step": best["device"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Orszag-Tang vortex after one cycle)",
+        "config": {"workload": "output_orszag_tang_%d3_12_fields" % a.n, "bytes": nbytes,
+                   "file_system": "temporary directory of the host"},
+        "host_write_GBs": nbytes / best["host"] / 1e9,
+        "restart_read_GBs": nbytes / best["read"] / 1e9,
+        "device_write_over_host_write": best["host"] / best["device"],
+        "read_back_identical": bool(same),
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -529,7 +585,7 @@ def main():
     if a.workload != "orszag_tang":
         run = {"fast_wave": run_fast_wave, "batched_amr": run_batched_amr,
                "turbulence": run_uniform, "kelvin_helmholtz": run_uniform,
-               "smr_flux_correct": run_smr_flux_correct}[a.workload]
+               "smr_flux_correct": run_smr_flux_correct, "output": run_output}[a.workload]
         out = run(a)
         out["copy_kernel_GBs"] = copy_bandwidth()["GBs"]
         print(json.dumps(out))

@@ -205,6 +205,8 @@ def field_shapes(mx, my, mz, mhd, dual_energy=False):
 class DeviceBlock:
     """One block's permanent fields in one device allocation."""
 
+    on_device = True
+
     def __init__(self, mx, my, mz, ghost, cell_width, mhd=True, dual_energy=False):
         self.mx, self.my, self.mz, self.ghost = mx, my, mz, ghost
         self.mhd = mhd

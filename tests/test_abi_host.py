@@ -1,4 +1,4 @@
-"""CPU checks of the two shared libraries: every function declared in
+"""CPU checks of the shared libraries: every function declared in
 include/*.h is exported, configurations are validated as the reference
 validates them (error codes at the C ABI; message + exit(1) in the C++
 drop-in layer, as the reference's ERROR/ASSERT), and the single-operation
@@ -26,7 +26,8 @@ def declared(header):
 
 
 @pytest.mark.parametrize("header,path", [
-    ("enzoe_vlct.h", ev.LIB_PATH), ("enzoe_vlct_host.h", ev.HOST_LIB_PATH)])
+    ("enzoe_vlct.h", ev.LIB_PATH), ("enzoe_vlct_host.h", ev.HOST_LIB_PATH),
+    ("enzoe_output.h", os.path.join(ROOT, "enzo-e_amd", "libenzoe_io.so"))])
 def test_library_exports_every_declared_symbol(header, path):
     names = declared(header)
     assert len(names) > 5

@@ -1,0 +1,145 @@
+"""HDF5 block output and restart input of device-resident blocks.
+
+* The file written from device memory (through the pinned double buffer)
+  holds exactly what the blocks hold: every dataset equals the downloaded
+  field bit for bit, read back through libhdf5 directly, and h5diff finds
+  no difference between it and the file written from host copies.
+* Checkpoint and restart as the reference's test does it
+  (tools/ckpt_restart_test.py:200-300): a run that writes its state at a
+  cycle, is restarted from that file into fresh blocks and continued, writes
+  block groups identical (h5diff) to those of the uninterrupted run.
+* A 128^3 MHD block (12 fields of about 19 MB) goes through the pipeline and
+  back into another device block unchanged.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import h5read
+import pyoracle as po
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not h5read.available(), reason="libhdf5 not present")]
+
+H5DIFF = "/opt/conda/bin/h5diff"
+WAVE = ("alpha = %r; beta = %r; amplitude = 1e-4; lambda = 1.0; positive_vel = true; "
+        "wave_type = fast" % (po.WAVE_ALPHA, po.WAVE_BETA))
+
+
+def _ev():
+    import enzoe_vlct as ev
+    return ev
+
+
+def _io():
+    import enzoe_io
+    return enzoe_io
+
+
+def _wave_grid(nb=(2, 1, 1), n=16, upper=(2.0, 1.0, 1.0)):
+    """A grid of n^3 MHD blocks over [0, upper] holding the product's
+    inclined fast wave."""
+    ev = _ev()
+    ev.host_lib().enzoe_host_set_fluid_props(5.0 / 3.0, 1e-200, 1e-200)
+    method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm", theta=2.0,
+                           courant=0.4)
+    cw = tuple(upper[a] / (nb[a] * n) for a in range(3))
+    dg = ev.DeviceGrid(method, nb, (n, n, n), cw, bc=("periodic",) * 3)
+    for bz in range(nb[2]):
+        for by in range(nb[1]):
+            for bx in range(nb[0]):
+                i = (bz * nb[1] + by) * nb[0] + bx
+                lower = tuple((bx, by, bz)[a] * upper[a] / nb[a] for a in range(3))
+                ev.initial_inclined_wave(WAVE, dg.blocks[i], lower)
+    ev.synchronize()
+    return dg, upper
+
+
+def _cycles(dg, t, k):
+    for _ in range(k):
+        t = t + dg.cycle(t, 1e30)
+    _ev().synchronize()
+    return t
+
+
+def test_device_output_holds_the_blocks_bit_for_bit(tmp_path, monkeypatch):
+    print("device output", flush=True)
+    monkeypatch.chdir(tmp_path)
+    io = _io()
+    dg, upper = _wave_grid()
+    t = _cycles(dg, 0.0, 3)
+    out = io.OutputData(None, name=("dev-%02d.h5", "cycle"), dir=("Dev_%02d", "cycle"))
+    path = out.write(dg.blocks, (0, 0, 0), upper, dg.nblocks, cycle=3, time=t, dt=0.0)
+    host = [io.HostBlock(b.download(), b.ghost) for b in dg.blocks]
+    hpath = io.OutputData(None, name=("host.h5",)).write(host, (0, 0, 0), upper,
+                                                         dg.nblocks, cycle=3, time=t, dt=0.0)
+    names = ["B0__", "B1__"]
+    with h5read.H5File(path) as h:
+        assert sorted(h.groups("/")) == names
+        for i, b in enumerate(names):
+            for f, arr in host[i].arrays.items():
+                d = h.dataset("/%s/field_%s" % (b, f))
+                assert d.shape == arr.shape and np.array_equal(d, arr), (b, f)
+    if os.path.exists(H5DIFF):
+        for b in names:
+            r = subprocess.run([H5DIFF, "-c", path, hpath, "/" + b], capture_output=True)
+            assert r.returncode == 0 and r.stdout == b"", (b, r.stdout)
+
+
+def test_restart_reproduces_the_uninterrupted_run(tmp_path, monkeypatch):
+    print("restart", flush=True)
+    monkeypatch.chdir(tmp_path)
+    io = _io()
+    # the run that checkpoints at cycle 4 and goes on to cycle 9
+    a, upper = _wave_grid()
+    t = _cycles(a, 0.0, 4)
+    check = io.OutputData(None, name=("check-%02d.h5", "cycle"), dir=("Check_%02d", "cycle"))
+    cpath = check.write(a.blocks, (0, 0, 0), upper, a.nblocks, cycle=4, time=t, dt=0.0)
+    t_a = _cycles(a, t, 5)
+    final = io.OutputData(None, name=("data-%02d.h5", "cycle"), dir=("%d_%02d", "proc",
+                                                                       "cycle"))
+    pa = final.write(a.blocks, (0, 0, 0), upper, a.nblocks, cycle=9, time=t_a, dt=0.0, proc=0)
+
+    # the restarted run: fresh blocks read back from the checkpoint
+    b, _ = _wave_grid()
+    for blk in b.blocks:  # overwrite the fresh initial state
+        blk.upload({k: np.full(s, np.nan) for k, s in blk.shapes.items()})
+    cycle, t0, _ = io.read_state(cpath)
+    assert cycle == 4 and t0 == t
+    lst = io.read_block_list("Check_04/Check_04.block_list")
+    for i, blk in enumerate(b.blocks):
+        name = io.block_name(0, (i, 0, 0), b.nblocks)
+        assert io.read_block(os.path.join("Check_04", lst[name]), name, blk)[0] == 4
+    t_b = _cycles(b, t0, 5)
+    assert t_b == t_a
+    pb = final.write(b.blocks, (0, 0, 0), upper, b.nblocks, cycle=9, time=t_b, dt=0.0, proc=1)
+    assert pa == "0_09/data-09.h5" and pb == "1_09/data-09.h5"
+    for i in range(2):
+        da, db = a.blocks[i].download(), b.blocks[i].download()
+        for f in da:
+            assert np.array_equal(da[f], db[f]), (i, f)
+    if os.path.exists(H5DIFF):
+        for name in ("B0__", "B1__"):
+            r = subprocess.run([H5DIFF, "-c", pa, pb, "/" + name], capture_output=True)
+            assert r.returncode == 0 and r.stdout == b"", (name, r.stdout)
+
+
+def test_large_block_round_trip_through_the_pipeline(tmp_path, monkeypatch):
+    print("128^3 round trip", flush=True)
+    monkeypatch.chdir(tmp_path)
+    ev, io = _ev(), _io()
+    n, g = 128, 3
+    m = n + 2 * g
+    src = ev.DeviceBlock(m, m, m, g, (1.0 / n,) * 3, mhd=True)
+    rng = np.random.default_rng(11)
+    src.upload({k: rng.standard_normal(s) for k, s in src.shapes.items()})
+    path = io.OutputData(None, name=("big.h5",)).write([src], (0, 0, 0), (1, 1, 1),
+                                                       (1, 1, 1), 0, 0.0, 0.0)
+    dst = ev.DeviceBlock(m, m, m, g, (1.0 / n,) * 3, mhd=True)
+    io.read_block(path, "B__", dst)
+    a, b = src.download(), dst.download()
+    assert len(a) == 12
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
