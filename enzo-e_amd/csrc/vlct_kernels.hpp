@@ -46,6 +46,7 @@ struct BlockDesc {
   double *Vb[3];
   int mx, my, mz;
   int g;               // ghost depth
+  int tile_rows;       // rows of the tiled cell order (0: plain order)
   double cw[3];
 };
 
@@ -62,7 +63,8 @@ struct StageParams {
   const double *dts;   // per-block dt of the step (device) or null: sp.dt
   double dt_factor;    // 0.5 for the partial step, 1 for the final one
   int xcd_remap;       // bit mask over k_flux(1) k_edge(2) k_faces(4)
-                       // k_cells(8): XCD-contiguous order, see wg_coords
+                       // k_cells(8): XCD-contiguous order, see wg_coords,
+                       // with the tiled cell order of cell_coords
   double eta;          // dual-energy eta ("modern" formulation)
   double ggm1;         // gamma (gamma - 1) rounded to float, as the
                        // reference's synchronisation declares it
@@ -134,17 +136,65 @@ __device__ __forceinline__ bool zero_step(const StageParams &sp, int by) {
   return sp.dts != nullptr && sp.dts[by] == 0.0;
 }
 
-// Decodes a linear thread index into (k,j,i) inside the box [lo, lo+n).
+// Decodes a linear thread index into (k,j,i) inside the box [lo, lo+n),
+// x fastest (32-bit arithmetic when the box allows it).
 __device__ __forceinline__ bool box_coords(long t, int lo, int nx, int ny,
                                            int nz, int &k, int &j, int &i) {
   long nxy = (long)nx * ny;
   if (t >= nxy * nz) return false;
+  if (nxy * nz < (1L << 31)) {
+    const unsigned tt = (unsigned)t, uxy = (unsigned)nxy, ux = (unsigned)nx;
+    const unsigned kk = tt / uxy;
+    const unsigned r = tt - kk * uxy;
+    const unsigned jj = r / ux;
+    k = (int)kk + lo;
+    j = (int)jj + lo;
+    i = (int)(r - jj * ux) + lo;
+    return true;
+  }
   k = (int)(t / nxy);
   long r = t - (long)k * nxy;
   j = (int)(r / nx);
   i = (int)(r - (long)j * nx);
   k += lo; j += lo; i += lo;
   return true;
+}
+
+// The tiled cell order: the box is cut into slabs of `rows` rows (all of
+// x), walked slab by slab and, inside a slab, plane by plane, so that
+// consecutive workgroups -- one XCD's share under the XCD-contiguous order --
+// sweep a narrow slab through z and the planes a z stencil reuses are still
+// in that XCD's L2 (in the plain order a plane of a large block is larger
+// than the L2, and every z neighbour comes from HBM again).  The last slab
+// may be shorter.  32-bit indices: blocks hold fewer than 2^31 cells
+// (check_block).
+__device__ __forceinline__ bool box_coords_tiled(long t, int lo, int nx, int ny,
+                                                 int nz, int rows, int &k,
+                                                 int &j, int &i) {
+  const unsigned ux = (unsigned)nx;
+  const unsigned tt = (unsigned)t;
+  if (t >= (long)nx * ny * nz) return false;
+  const unsigned full = ux * (unsigned)rows * (unsigned)nz;  // one full slab
+  const unsigned slab = tt / full;
+  const unsigned y0 = slab * (unsigned)rows;
+  const unsigned r = min((unsigned)rows, (unsigned)ny - y0);
+  unsigned u = tt - slab * full;
+  const unsigned tp = ux * r;  // cells of one plane of this slab
+  const unsigned kk = u / tp;
+  u -= kk * tp;
+  const unsigned jj = u / ux;
+  k = (int)kk + lo;
+  j = (int)(jj + y0) + lo;
+  i = (int)(u - jj * ux) + lo;
+  return true;
+}
+
+// The cell of linear index t in the box [lo, lo+n): tiled order when
+// rows > 0 and the box has more rows than one slab, else x fastest.
+__device__ __forceinline__ bool cell_coords(long t, int lo, int nx, int ny, int nz,
+                                            int rows, int &k, int &j, int &i) {
+  if (rows > 0 && rows < ny) return box_coords_tiled(t, lo, nx, ny, nz, rows, k, j, i);
+  return box_coords(t, lo, nx, ny, nz, k, j, i);
 }
 
 // Loads through the global address space (the pointers come from a
@@ -319,16 +369,11 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
-  const int nx = mx - 2 * cs, ny = my - 2 * cs;
-  const int t = bx * blockDim.x + threadIdx.x;
-  const int nxy = nx * ny;
-  if (t >= nxy * (mz - 2 * cs)) return;
-  int k = t / nxy;
-  int r = t - k * nxy;
-  int j = r / nx;
-  int i = r - j * nx + cs;
-  k += cs;
-  j += cs;
+  int k, j, i;
+  const long t = (long)bx * blockDim.x + threadIdx.x;
+  if (!cell_coords(t, cs, mx - 2 * cs, my - 2 * cs, mz - 2 * cs,
+                   (sp.xcd_remap & 1) ? b.tile_rows : 0, k, j, i))
+    return;
   double *const *U = (sp.stage == 0) ? b.U : b.Ut;
   const double *const *bfr = (sp.stage == 0) ? b.bfi : b.bft;
   FluxConsts fc;
@@ -382,7 +427,8 @@ k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int cs = sp.cur_stale;
   int k, j, i;
   const long t = (long)bx * blockDim.x + threadIdx.x;
-  if (!box_coords(t, cs, mx - 2 * cs, my - 2 * cs, mz - 2 * cs, k, j, i))
+  if (!cell_coords(t, cs, mx - 2 * cs, my - 2 * cs, mz - 2 * cs,
+                   (sp.xcd_remap & 2) ? b.tile_rows : 0, k, j, i))
     return;
   const int p[3] = {i, j, k};
   const int m[3] = {mx, my, mz};
@@ -439,7 +485,8 @@ k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int k, j, i;
   const long t = (long)bx * blockDim.x + threadIdx.x;
   const int lo = cs + 1;
-  if (!box_coords(t, lo, mx - cs - lo, my - cs - lo, mz - cs - lo, k, j, i))
+  if (!cell_coords(t, lo, mx - cs - lo, my - cs - lo, mz - cs - lo,
+                   (sp.xcd_remap & 4) ? b.tile_rows : 0, k, j, i))
     return;
   const double dt = sp.dts ? sp.dts[by] * sp.dt_factor : sp.dt;
   const int p[3] = {i, j, k};
@@ -492,7 +539,8 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int lo = sp.is_final ? cs : cs + 1;
   int k, j, i;
   const long t = (long)bx * blockDim.x + threadIdx.x;
-  if (!box_coords(t, lo, mx - 2 * lo, my - 2 * lo, mz - 2 * lo, k, j, i))
+  if (!cell_coords(t, lo, mx - 2 * lo, my - 2 * lo, mz - 2 * lo,
+                   (sp.xcd_remap & 8) ? b.tile_rows : 0, k, j, i))
     return;
   const long c = idx3(k, j, i, my, mx);
   double *const *Uo = sp.is_final ? b.U : b.Ut;

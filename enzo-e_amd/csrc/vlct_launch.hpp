@@ -177,6 +177,8 @@ inline int upload_table(void **dptr, size_t *have, std::vector<char> *last,
 
 inline int check_block(const enzoe_vlct_method *m, const enzoe_block &b) {
   if (b.mx < 7 || b.my < 7 || b.mz < 7) return ENZOE_ERR_ARG;
+  // cell indices of the tiled order are 32-bit
+  if ((long long)(b.mx + 1) * (b.my + 1) * (b.mz + 1) >= (1LL << 31)) return ENZOE_ERR_ARG;
   // EnzoMethodMHDVlct.cpp:124-133: ghost depth >= sum of staling rates
   int req = 0;
   for (int s = 0; s < m->nstages; s++) req += total_staling(m->recon[s]);
@@ -191,6 +193,18 @@ inline int check_block(const enzoe_vlct_method *m, const enzoe_block &b) {
   // the dual-energy formalism integrates the internal_energy field
   if (m->dual && !b.internal_energy) return ENZOE_ERR_ARG;
   return 0;
+}
+
+// Rows of the tiled cell order for a block with planes of mx x my cells:
+// the largest of 64, 32, 16, 8 rows whose slab planes hold the flux
+// kernel's z stencil (about 5 planes of 11 words per cell) in 2 MB, half an
+// XCD's L2; 0 (plain order) when a whole plane of that already fits in 4 MB.
+inline int tile_rows_for(int mx, int my) {
+  const double row = (double)mx * 11 * 8 * 5;
+  if (row * my <= double(4 << 20)) return 0;
+  int rows = 64;
+  while (rows > 8 && row * rows > double(2 << 20)) rows /= 2;
+  return rows;
 }
 
 // Fills descriptors for blocks [b0, b1) using the scratch pool.
@@ -216,6 +230,7 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
     d.pressure = b.pressure;
     d.mx = b.mx; d.my = b.my; d.mz = b.mz;
     d.g = b.ghost;
+    d.tile_rows = tile_rows_for(b.mx, b.my);
     if (accel)
       for (int a = 0; a < 3; a++) d.acc[a] = accel[3 * bi + a];
     for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];

@@ -116,6 +116,9 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
   const T igm1 = qdiv(T(1), gamma - T(1));
   T spd[5];
   Cons1D<T> ul, ur, ulst, uldst, urdst, urst, fl, fr;
+  // the outer states' physical fluxes are formed where the fan region is
+  // selected: held from the start they would stay live through the whole
+  // solve
 
   const T pressure_l = wli[L_E];
   const T pressure_r = wri[L_E];
@@ -157,24 +160,6 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
 
   T ptl = pressure_l + pbl;
   T ptr = pressure_r + pbr;
-
-  fl.d = ul.mx;
-  fl.mx = ul.mx * wli[L_VI] + ptl - bxsq;
-  fl.my = ul.my * wli[L_VI] - bxi * ul.by;
-  fl.mz = ul.mz * wli[L_VI] - bxi * ul.bz;
-  fl.e = wli[L_VI] * (ul.e + ptl - bxsq) -
-         bxi * (wli[L_VJ] * ul.by + wli[L_VK] * ul.bz);
-  fl.by = ul.by * wli[L_VI] - bxi * wli[L_VJ];
-  fl.bz = ul.bz * wli[L_VI] - bxi * wli[L_VK];
-
-  fr.d = ur.mx;
-  fr.mx = ur.mx * wri[L_VI] + ptr - bxsq;
-  fr.my = ur.my * wri[L_VI] - bxi * ur.by;
-  fr.mz = ur.mz * wri[L_VI] - bxi * ur.bz;
-  fr.e = wri[L_VI] * (ur.e + ptr - bxsq) -
-         bxi * (wri[L_VJ] * ur.by + wri[L_VK] * ur.bz);
-  fr.by = ur.by * wri[L_VI] - bxi * wri[L_VJ];
-  fr.bz = ur.bz * wri[L_VI] - bxi * wri[L_VK];
 
   T sdl = spd[0] - wli[L_VI];
   T sdr = spd[4] - wri[L_VI];
@@ -295,15 +280,38 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     urdst.e = urst.e + sqrtdr * bxsig * (vbstr - tmp);
   }
 
+  auto flux_l = [&]() {
+    fl.d = ul.mx;
+    fl.mx = ul.mx * wli[L_VI] + ptl - bxsq;
+    fl.my = ul.my * wli[L_VI] - bxi * ul.by;
+    fl.mz = ul.mz * wli[L_VI] - bxi * ul.bz;
+    fl.e = wli[L_VI] * (ul.e + ptl - bxsq) -
+           bxi * (wli[L_VJ] * ul.by + wli[L_VK] * ul.bz);
+    fl.by = ul.by * wli[L_VI] - bxi * wli[L_VJ];
+    fl.bz = ul.bz * wli[L_VI] - bxi * wli[L_VK];
+  };
+  auto flux_r = [&]() {
+    fr.d = ur.mx;
+    fr.mx = ur.mx * wri[L_VI] + ptr - bxsq;
+    fr.my = ur.my * wri[L_VI] - bxi * ur.by;
+    fr.mz = ur.mz * wri[L_VI] - bxi * ur.bz;
+    fr.e = wri[L_VI] * (ur.e + ptr - bxsq) -
+           bxi * (wri[L_VJ] * ur.by + wri[L_VK] * ur.bz);
+    fr.by = ur.by * wri[L_VI] - bxi * wri[L_VJ];
+    fr.bz = ur.bz * wri[L_VI] - bxi * wri[L_VK];
+  };
   // Select the region of the fan containing the interface; the state
   // differences of each wave are formed lazily (only the ones used).
   if (spd[0] >= T(0)) {
+    flux_l();
     flx[L_D] = fl.d; flx[L_VI] = fl.mx; flx[L_VJ] = fl.my; flx[L_VK] = fl.mz;
     flx[L_E] = fl.e; flx[L_BJ] = fl.by; flx[L_BK] = fl.bz;
   } else if (spd[4] <= T(0)) {
+    flux_r();
     flx[L_D] = fr.d; flx[L_VI] = fr.mx; flx[L_VJ] = fr.my; flx[L_VK] = fr.mz;
     flx[L_E] = fr.e; flx[L_BJ] = fr.by; flx[L_BK] = fr.bz;
   } else if (spd[1] >= T(0)) {
+    flux_l();
     flx[L_D] = fl.d + spd[0] * (ulst.d - ul.d);
     flx[L_VI] = fl.mx + spd[0] * (ulst.mx - ul.mx);
     flx[L_VJ] = fl.my + spd[0] * (ulst.my - ul.my);
@@ -312,6 +320,7 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     flx[L_BJ] = fl.by + spd[0] * (ulst.by - ul.by);
     flx[L_BK] = fl.bz + spd[0] * (ulst.bz - ul.bz);
   } else if (spd[2] >= T(0)) {
+    flux_l();
     flx[L_D] = fl.d + spd[0] * (ulst.d - ul.d) + spd[1] * (uldst.d - ulst.d);
     flx[L_VI] = fl.mx + spd[0] * (ulst.mx - ul.mx) + spd[1] * (uldst.mx - ulst.mx);
     flx[L_VJ] = fl.my + spd[0] * (ulst.my - ul.my) + spd[1] * (uldst.my - ulst.my);
@@ -320,6 +329,7 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     flx[L_BJ] = fl.by + spd[0] * (ulst.by - ul.by) + spd[1] * (uldst.by - ulst.by);
     flx[L_BK] = fl.bz + spd[0] * (ulst.bz - ul.bz) + spd[1] * (uldst.bz - ulst.bz);
   } else if (spd[3] > T(0)) {
+    flux_r();
     flx[L_D] = fr.d + spd[4] * (urst.d - ur.d) + spd[3] * (urdst.d - urst.d);
     flx[L_VI] = fr.mx + spd[4] * (urst.mx - ur.mx) + spd[3] * (urdst.mx - urst.mx);
     flx[L_VJ] = fr.my + spd[4] * (urst.my - ur.my) + spd[3] * (urdst.my - urst.my);
@@ -328,6 +338,7 @@ __device__ __forceinline__ void hlld(const T wli[8], const T wri[8], T gamma,
     flx[L_BJ] = fr.by + spd[4] * (urst.by - ur.by) + spd[3] * (urdst.by - urst.by);
     flx[L_BK] = fr.bz + spd[4] * (urst.bz - ur.bz) + spd[3] * (urdst.bz - urst.bz);
   } else {
+    flux_r();
     flx[L_D] = fr.d + spd[4] * (urst.d - ur.d);
     flx[L_VI] = fr.mx + spd[4] * (urst.mx - ur.mx);
     flx[L_VJ] = fr.my + spd[4] * (urst.my - ur.my);
