@@ -354,14 +354,19 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
 // the three faces between p and p + e_d that lie in the flux region of the
 // stage (EnzoRiemannImpl.hpp:317-338 with stale_depth = cur_stale: face
 // index range [cs, m_d - 1 - cs) along d).
-// Minimum waves per SIMD the flux kernel is compiled for (caps its VGPRs).
+// Minimum waves per SIMD the flux kernel is compiled for (caps its VGPRs);
+// the nearest-neighbour variant has its own.
 #ifndef ENZOE_FLUX_MIN_WAVES
 #define ENZOE_FLUX_MIN_WAVES 3
+#endif
+#ifndef ENZOE_FLUX_NN_WAVES
+#define ENZOE_FLUX_NN_WAVES 3
 #endif
 
 template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD,
           bool DE = false>
-__global__ void __launch_bounds__(256, ENZOE_FLUX_MIN_WAVES)
+__global__ void __launch_bounds__(256, RECON == RC_NN ? ENZOE_FLUX_NN_WAVES
+                                                      : ENZOE_FLUX_MIN_WAVES)
 k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
   wg_coords(sp.xcd_remap & 1, bx, by);

@@ -275,3 +275,26 @@ def test_mixed_block_shapes_in_one_call():
         compare_grids(d, g)
     arr = (ev.Block * 1)()
     assert ev.lib().enzoe_vlct_compute(method.handle, arr, 0, dt, None) == 2
+
+
+@pytest.mark.parametrize("mhd", [True, False], ids=["mhd", "hd"])
+@pytest.mark.parametrize("mask", [15, 0], ids=["tiled", "plain"])
+def test_tiled_cell_order_matches_oracle(mhd, mask, monkeypatch):
+    """Blocks with 134 x 134 planes (more than an L2 holds) are walked in
+    slabs of 32 rows marched through z when a kernel's launch-order bit is
+    set (the last slab of each box is shorter): every stage kernel in the
+    tiled order (mask 15), and all in the plain order (mask 0), give the
+    oracle's cycles bit for bit."""
+    print("tiled order", mhd, mask, flush=True)
+    monkeypatch.setenv("ENZOE_XCD_REMAP", str(mask))
+    grid = random_periodic_grid(mhd=mhd, seed=5, n=(128, 128, 6), ext=(1.0, 1.0, 0.05),
+                                vamp=0.6, bamp=0.08)
+    method = method_for(grid.cfg)
+    dg = device_copy(grid, method)
+    t = 0.0
+    for _ in range(2):
+        dt_o = grid.cycle(t, 10.0)
+        assert dg.cycle(t, 10.0) == dt_o
+        t += dt_o
+    _ev().synchronize()
+    compare_grids(dg, grid, rel=0.0)
