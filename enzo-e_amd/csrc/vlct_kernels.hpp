@@ -399,12 +399,15 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     gst(b.Ec[1], c, (-wc[S_VZ] * wc[S_BX] + wc[S_VX] * wc[S_BZ]));
     gst(b.Ec[2], c, (-wc[S_VX] * wc[S_BY] + wc[S_VY] * wc[S_BX]));
   }
+  // the z face first: in the tiled order the plane above is the one not yet
+  // in L2, so its loads go out with this cell's and the two latencies
+  // overlap (each face writes its own arrays; the order changes no value)
+  if (k < mz - 1 - cs)
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 2, DE>(b, U, bfr[2], k, j, i, c, wc, fc, ei);
   if (i < mx - 1 - cs)
     flux_face<T, FluxT, RECON, RIEMANN, MHD, 0, DE>(b, U, bfr[0], k, j, i, c, wc, fc, ei);
   if (j < my - 1 - cs)
     flux_face<T, FluxT, RECON, RIEMANN, MHD, 1, DE>(b, U, bfr[1], k, j, i, c, wc, fc, ei);
-  if (k < mz - 1 - cs)
-    flux_face<T, FluxT, RECON, RIEMANN, MHD, 2, DE>(b, U, bfr[2], k, j, i, c, wc, fc, ei);
 }
 
 // ---------------------------------------------------------------- k_edge
