@@ -10,6 +10,7 @@
   block groups identical (h5diff) to those of the uninterrupted run.
 * A 128^3 MHD block (12 fields of about 19 MB) goes through the pipeline and
   back into another device block unchanged.
+* The golden L1 of a reference deck computed from its output files.
 """
 import os
 import subprocess
@@ -143,3 +144,50 @@ def test_large_block_round_trip_through_the_pipeline(tmp_path, monkeypatch):
     assert len(a) == 12
     for f in a:
         assert np.array_equal(a[f], b[f]), f
+
+
+def test_linear_wave_golden_from_the_output_files(tmp_path, monkeypatch):
+    """The reference's workflow for its linear-wave decks: outputs at t = 0
+    and at the end, and the L1 error computed from the two files
+    (input/vlct/run_HD_linear_wave_test.py:70-80 with the reference's L1,
+    tools/l1_error_norm.py).  The N16 sound wave on 2x2x2 blocks, evolved
+    on the GPU from the product's initializer, read back from the files
+    through libhdf5 only, reproduces the golden L1 at the reference's
+    tolerance (input/vlct/testing_utils.py:275-292)."""
+    print("golden from files", flush=True)
+    from test_gpu_initial import _product_grid
+    from test_gpu_parity import method_for
+    monkeypatch.chdir(tmp_path)
+    ev, io = _ev(), _io()
+    grid = po.linear_wave_grid("sound", 16, mhd=False, nblocks=(2, 2, 2), positive=True)
+    ev.host_lib().enzoe_host_set_fluid_props(grid.cfg.gamma, grid.cfg.density_floor,
+                                             grid.cfg.pressure_floor)
+    dg = _product_grid(grid, "sound", True, method_for(grid.cfg))
+    ev.synchronize()
+    out = io.OutputData(["density", "velocity_x", "velocity_y", "velocity_z",
+                         "total_energy", "pressure"], name=("data-%04d.h5", "cycle"),
+                        dir=("Dir_%04d", "cycle"))
+    domain = ((0.0, 0.0, 0.0), (3.0, 1.5, 1.5))
+    p0 = out.write(dg.blocks, *domain, grid.nblocks, cycle=0, time=0.0, dt=0.0)
+    ncycles, _ = dg.evolve(0.0, 1.0)
+    p1 = out.write(dg.blocks, *domain, grid.nblocks, cycle=ncycles, time=1.0, dt=0.0)
+
+    def from_file(path):
+        lst = io.read_block_list(os.path.join(os.path.dirname(path),
+                                              os.path.basename(os.path.dirname(path)) +
+                                              ".block_list"))
+        assert len(lst) == 8
+        with h5read.H5File(path) as h:
+            for i in range(8):
+                gi = (i % 2, (i // 2) % 2, i // 4)
+                name = io.block_name(0, gi, (2, 2, 2))
+                assert lst[name] == os.path.basename(path)
+                view = grid.block_arrays(i)
+                for f in out.field_list:
+                    view[f][...] = h.dataset("/%s/field_%s" % (name, f))
+        return [grid.gather(f) for f in po.HD_L1_FIELDS]
+
+    l1 = po.l1_norm(from_file(p0), from_file(p1), res=16)
+    print("cycles", ncycles, "L1", repr(l1))
+    assert ncycles > 10
+    assert np.isclose(l1, 1.3704437791196907e-07, rtol=1e-13, atol=7e-14)
