@@ -182,7 +182,8 @@ namespace {
 int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                  double dt, const double *d_dts, hipStream_t s,
                  const StoreArgs *st = nullptr,
-                 const double *const *accel = nullptr) {
+                 const double *const *accel = nullptr,
+                 unsigned long long *cfl = nullptr) {
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
   int b0 = 0;
@@ -206,11 +207,12 @@ int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   : compute_group<double, false, true>(m, blocks + b0, b1 - b0, dt, dts, s,
                                                        rs, ra);
     } else if (m->precision == ENZOE_PREC_DOUBLE) {
-      rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra)
+      rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra,
+                                                cfl)
                   : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, dts, s, rs,
-                                                 ra);
+                                                 ra, cfl);
     } else {
-      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra);
+      rc = compute_mixed(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra, cfl);
     }
     if (rc) return rc;
     b0 = b1;
@@ -298,8 +300,10 @@ namespace {
 // list changed, so the call makes no host round trip otherwise.
 // d_time / t_stop: the device time of a device-driven cycle and its stop
 // time (a cycle past it leaves the dual-energy fields alone), or null.
+// skip_lo > 0: only the cells outside [skip_lo, m - skip_lo) (the rest came
+// from the final k_cells) and no reset of the minimum.
 int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_t s,
-              const double *d_time = nullptr, double t_stop = 0.0) {
+              const double *d_time = nullptr, double t_stop = 0.0, int skip_lo = 0) {
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
   if (!m->d_dt &&
@@ -337,7 +341,7 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
   // with dual energy the reduction also writes the energies: a dry call
   // (graph set-up) uploads the table only
   if (m->dry) return ENZOE_OK;
-  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
+  if (skip_lo == 0) hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
   long maxcells = 0;
   for (int i = 0; i < n; i++)
     maxcells = std::max<long>(maxcells, (long)blocks[i].mx * blocks[i].my * blocks[i].mz);
@@ -350,16 +354,16 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
   const DualParams dp = {m->pfloor, m->eta, sync_ggm1(m->gamma), d_time, t_stop};
   if (m->mhd && m->dual)
     hipLaunchKernelGGL((k_dt<double, true, true>), grid, dim3(256), 0, s, dd, m->gamma,
-                       m->d_dt, dp);
+                       m->d_dt, dp, skip_lo);
   else if (m->mhd)
     hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s, dd, m->gamma, m->d_dt,
-                       dp);
+                       dp, skip_lo);
   else if (m->dual)
     hipLaunchKernelGGL((k_dt<double, false, true>), grid, dim3(256), 0, s, dd, m->gamma,
-                       m->d_dt, dp);
+                       m->d_dt, dp, skip_lo);
   else
     hipLaunchKernelGGL((k_dt<double, false>), grid, dim3(256), 0, s, dd, m->gamma, m->d_dt,
-                       dp);
+                       dp, skip_lo);
   return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
 }
 
@@ -380,6 +384,42 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   double dtb;
   std::memcpy(&dtb, &bits, sizeof(dtb));
   *dt_out = dtb * m->courant;
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_compute_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                                double dt, enzoe_stream_t stream, double *dt_next) {
+  if (!m || !blocks || n <= 0 || !dt_next) return ENZOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (m->dual) {
+    // the timestep synchronises the energies of every cell first: run apart
+    if (int rc = compute_runs(m, blocks, n, dt, nullptr, s)) return rc;
+    return enzoe_vlct_timestep(m, blocks, n, stream, dt_next);
+  }
+  for (int i = 0; i < n; i++)
+    if (int rc = check_block(m, blocks[i])) return rc;
+  if (!m->d_dt &&
+      hipMalloc((void **)&m->d_dt, sizeof(unsigned long long)) != hipSuccess) {
+    m->d_dt = nullptr;
+    return ENZOE_ERR_ALLOC;
+  }
+  // the final k_cells covers [lo, m - lo) with lo the final stage's stale depth
+  int lo = 0;
+  for (int st = 0; st + 1 < m->nstages; st++) lo += total_staling(m->recon[st]);
+  lo += immediate_staling(m->recon[m->nstages - 1]);
+  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
+  if (int rc = compute_runs(m, blocks, n, dt, nullptr, s, nullptr, nullptr,
+                            (unsigned long long *)m->d_dt))
+    return rc;
+  if (int rc = launch_dt(m, blocks, n, s, nullptr, 0.0, lo)) return rc;
+  unsigned long long bits = 0;
+  if (hipMemcpyAsync(&bits, m->d_dt, sizeof(bits), hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
+    return ENZOE_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return ENZOE_ERR_HIP;
+  double dtb;
+  std::memcpy(&dtb, &bits, sizeof(dtb));
+  *dt_next = dtb * m->courant;
   return ENZOE_OK;
 }
 

@@ -68,6 +68,8 @@ struct StageParams {
   double eta;          // dual-energy eta ("modern" formulation)
   double ggm1;         // gamma (gamma - 1) rounded to float, as the
                        // reference's synchronisation declares it
+  unsigned long long *cfl;  // final k_cells: also the CFL minimum of its
+                            // cells and their pressure (null: not)
 };
 
 // Energy floor and, with the dual-energy formalism, the synchronisation of
@@ -519,6 +521,73 @@ k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   }
 }
 
+// ------------------------------------------------------- CFL of one cell
+
+// EnzoMHDIntegratorStageCommands::timestep (EnzoMHDIntegratorStageCommands.cpp:
+// 299-366) for one cell without the dual-energy formalism: its pressure
+// (EnzoComputePressure.cpp:177-189) and min_i dx_i / (|v_i| + c_f) with the
+// fast magnetosonic speed (EnzoEOSIdeal.hpp:111-140) or the sound speed.
+// Shared by k_dt and the final k_cells so both give the same bits; never
+// contracted into FMAs, also where the including unit allows it (the mixed
+// build), so that both round alike.
+template <bool MHD>
+__device__ __forceinline__ double cell_pressure(double d, double vx, double vy, double vz,
+                                                double e, double bx, double by, double bz,
+                                                double gamma) {
+#pragma clang fp contract(off)
+  const double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+  const double me = MHD ? 0.5 * (bx * bx + by * by + bz * bz) : 0.;
+  return (gamma - 1.0) * (d * (e - ke) - me);
+}
+template <bool MHD>
+__device__ __forceinline__ double cfl_speed(double d, double vx, double vy, double vz,
+                                            double bx, double by, double bz, double p,
+                                            double gamma, const double cw[3]) {
+#pragma clang fp contract(off)
+  double sig;
+  if (MHD) {
+    double B2 = ((bx * bx) + ((by * by) + (bz * bz)));
+    double cs2 = gamma * p / d;
+    double va2 = B2 / d;
+    sig = sqrt(va2 + cs2);
+  } else {
+    sig = sqrt(gamma * p / d);
+  }
+  return dmin3(cw[0] / (fabs(vx) + sig), cw[1] / (fabs(vy) + sig),
+               cw[2] / (fabs(vz) + sig));
+}
+template <bool MHD>
+__device__ __forceinline__ double cfl_cell(double d, double vx, double vy, double vz,
+                                           double e, double bx, double by, double bz,
+                                           double gamma, const double cw[3], double &p) {
+  p = cell_pressure<MHD>(d, vx, vy, vz, e, bx, by, bz, gamma);
+  return cfl_speed<MHD>(d, vx, vy, vz, bx, by, bz, p, gamma, cw);
+}
+
+// Min of `local` over the workgroup into *result, as the bit pattern of a
+// non-negative double (an integer atomicMin is then an exact double min; a
+// NaN never wins, as std::min in the reference ignores it): wave64
+// butterfly, LDS across the waves, one atomic per workgroup.  Every thread
+// of the workgroup must call it.
+__device__ __forceinline__ void block_min_to(unsigned long long *result, double local) {
+  unsigned long long bits = __double_as_longlong(local);
+  if (!(local >= 0.0)) bits = 0x7FF8000000000000ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(bits, off, 64);
+    bits = (o < bits) ? o : bits;
+  }
+  __shared__ unsigned long long wmin[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wmin[wave] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = (wmin[w] < m) ? wmin[w] : m;
+    atomicMin(result, m);
+  }
+}
+
 // --------------------------------------------------------------- k_cells
 
 // One thread per cell of [lo, m - lo) with lo = cur_stale for the final
@@ -547,92 +616,114 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int lo = sp.is_final ? cs : cs + 1;
   int k, j, i;
   const long t = (long)bx * blockDim.x + threadIdx.x;
-  if (!cell_coords(t, lo, mx - 2 * lo, my - 2 * lo, mz - 2 * lo,
-                   (sp.xcd_remap & 8) ? b.tile_rows : 0, k, j, i))
-    return;
-  const long c = idx3(k, j, i, my, mx);
-  double *const *Uo = sp.is_final ? b.U : b.Ut;
-  double bc[3] = {0., 0., 0.};
-  if (MHD) {
-    double *const *bf = sp.is_final ? b.bfi : b.bft;
-    bc[0] = 0.5 * (bf[0][idx3(k, j, i, my, mx + 1)] +
-                   bf[0][idx3(k, j, i + 1, my, mx + 1)]);
-    bc[1] = 0.5 * (bf[1][idx3(k, j, i, my + 1, mx)] +
-                   bf[1][idx3(k, j + 1, i, my + 1, mx)]);
-    bc[2] = 0.5 * (bf[2][idx3(k, j, i, my, mx)] +
-                   bf[2][idx3(k + 1, j, i, my, mx)]);
-    Uo[S_BX][c] = bc[0];
-    Uo[S_BY][c] = bc[1];
-    Uo[S_BZ][c] = bc[2];
-  }
-  const int ul = cs + 1;
-  if (k < ul || j < ul || i < ul || k >= mz - ul || j >= my - ul ||
-      i >= mx - ul)
-    return;
-  const double dt = sp.dts ? sp.dts[by] * sp.dt_factor : sp.dt;
-  const long str[3] = {1, (long)mx, (long)mx * my};
-  const int fq[5] = {S_RHO, S_VX, S_VY, S_VZ, S_E};
-  double dU[5];
-#pragma unroll
-  for (int q = 0; q < 5; q++) dU[q] = 0.;
-  double dUe = 0.;
-  double pc = 0.;
-  if (DE) {
-    // the stage's primitive pressure, floored (EnzoSourceInternalEnergy.cpp:
-    // 75-88)
-    double *const *Us = (sp.stage == 0) ? b.U : b.Ut;
-    const double *es = (sp.stage == 0) ? b.ei : b.eit;
-    pc = dmax((sp.gamma - 1.0) * Us[S_RHO][c] * es[c], sp.pressure_floor);
-  }
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const double dtdx = dt / b.cw[d];
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-      const FluxT *F = b.F[d][fq[q]];
-      double fr = (double)F[c], fl = (double)F[c - str[d]];
-      dU[q] -= dtdx * (fr - fl);
+  const bool in_box = cell_coords(t, lo, mx - 2 * lo, my - 2 * lo, mz - 2 * lo,
+                                  (sp.xcd_remap & 8) ? b.tile_rows : 0, k, j, i);
+  // with the CFL minimum every thread takes part in the reduction
+  const bool cfl = !DE && sp.cfl != nullptr;
+  if (!in_box && !cfl) return;
+  double local = 1.7976931348623157e308;
+  auto body = [&]() {
+    const long c = idx3(k, j, i, my, mx);
+    double *const *Uo = sp.is_final ? b.U : b.Ut;
+    double bc[3] = {0., 0., 0.};
+    if (MHD) {
+      double *const *bf = sp.is_final ? b.bfi : b.bft;
+      bc[0] = 0.5 * (bf[0][idx3(k, j, i, my, mx + 1)] +
+                     bf[0][idx3(k, j, i + 1, my, mx + 1)]);
+      bc[1] = 0.5 * (bf[1][idx3(k, j, i, my + 1, mx)] +
+                     bf[1][idx3(k, j + 1, i, my + 1, mx)]);
+      bc[2] = 0.5 * (bf[2][idx3(k, j, i, my, mx)] +
+                     bf[2][idx3(k + 1, j, i, my, mx)]);
+      Uo[S_BX][c] = bc[0];
+      Uo[S_BY][c] = bc[1];
+      Uo[S_BZ][c] = bc[2];
     }
+    const int ul = cs + 1;
+    if (k < ul || j < ul || i < ul || k >= mz - ul || j >= my - ul ||
+        i >= mx - ul) {
+      if (cfl) {
+        // outside the update, inside the cell-centred B rebuild: the
+        // start-of-step state with the new centre B
+        double p;
+        local = cfl_cell<MHD>(Uo[S_RHO][c], Uo[S_VX][c], Uo[S_VY][c], Uo[S_VZ][c],
+                              Uo[S_E][c], bc[0], bc[1], bc[2], sp.gamma, b.cw, p);
+        if (b.pressure) b.pressure[c] = p;
+      }
+      return;
+    }
+    const double dt = sp.dts ? sp.dts[by] * sp.dt_factor : sp.dt;
+    const long str[3] = {1, (long)mx, (long)mx * my};
+    const int fq[5] = {S_RHO, S_VX, S_VY, S_VZ, S_E};
+    double dU[5];
+  #pragma unroll
+    for (int q = 0; q < 5; q++) dU[q] = 0.;
+    double dUe = 0.;
+    double pc = 0.;
     if (DE) {
-      dUe -= dtdx * (b.Fe[d][c] - b.Fe[d][c - str[d]]);
-      dUe -= dtdx * pc * (b.Vb[d][c] - b.Vb[d][c - str[d]]);
+      // the stage's primitive pressure, floored (EnzoSourceInternalEnergy.cpp:
+      // 75-88)
+      double *const *Us = (sp.stage == 0) ? b.U : b.Ut;
+      const double *es = (sp.stage == 0) ? b.ei : b.eit;
+      pc = dmax((sp.gamma - 1.0) * Us[S_RHO][c] * es[c], sp.pressure_floor);
     }
-  }
-  // gravity (EnzoSourceGravity::calculate_source, EnzoSourceGravity.cpp:
-  // 17-69) on the stage with index 1, from the start-of-step state, after
-  // the flux divergence (EnzoMHDIntegratorStageCommands.cpp:180-182,
-  // 265-295)
-  if (sp.stage == 1 && b.acc[0] != nullptr) {
-    const double rho = b.U[S_RHO][c];
-    const double ax = b.acc[0][c], ay = b.acc[1][c], az = b.acc[2][c];
-    dU[1] += dt * rho * ax;
-    dU[2] += dt * rho * ay;
-    dU[3] += dt * rho * az;
-    dU[4] += dt * rho * ((b.U[S_VX][c] * ax) + (b.U[S_VY][c] * ay) +
-                         (b.U[S_VZ][c] * az));
-  }
-  // the update base is always the start-of-step state
-  double old_rho = b.U[S_RHO][c];
-  double new_rho = b.U[S_RHO][c] + dU[0];
-  new_rho = dmax(new_rho, sp.density_floor);
-  double inv_new_rho = 1. / new_rho;
-  double v[3];
-#pragma unroll
-  for (int q = 1; q < 4; q++)
-    v[q - 1] = (b.U[fq[q]][c] * old_rho + dU[q]) * inv_new_rho;
-  double etot = (b.U[S_E][c] * old_rho + dU[4]) * inv_new_rho;
-  double eint = 0.;
-  if (DE) eint = (b.ei[c] * old_rho + dUe) * inv_new_rho;
-  // energy floor (and synchronisation)
-  const double b2 = MHD ? (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]) : 0.;
-  floor_and_sync<MHD, DE>(new_rho, v, b2, sp.gamma, sp.pressure_floor, sp.eta, sp.ggm1,
-                          etot, eint);
-  Uo[S_RHO][c] = new_rho;
-  Uo[S_VX][c] = v[0];
-  Uo[S_VY][c] = v[1];
-  Uo[S_VZ][c] = v[2];
-  Uo[S_E][c] = etot;
-  if (DE) (sp.is_final ? b.ei : b.eit)[c] = eint;
+  #pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double dtdx = dt / b.cw[d];
+  #pragma unroll
+      for (int q = 0; q < 5; q++) {
+        const FluxT *F = b.F[d][fq[q]];
+        double fr = (double)F[c], fl = (double)F[c - str[d]];
+        dU[q] -= dtdx * (fr - fl);
+      }
+      if (DE) {
+        dUe -= dtdx * (b.Fe[d][c] - b.Fe[d][c - str[d]]);
+        dUe -= dtdx * pc * (b.Vb[d][c] - b.Vb[d][c - str[d]]);
+      }
+    }
+    // gravity (EnzoSourceGravity::calculate_source, EnzoSourceGravity.cpp:
+    // 17-69) on the stage with index 1, from the start-of-step state, after
+    // the flux divergence (EnzoMHDIntegratorStageCommands.cpp:180-182,
+    // 265-295)
+    if (sp.stage == 1 && b.acc[0] != nullptr) {
+      const double rho = b.U[S_RHO][c];
+      const double ax = b.acc[0][c], ay = b.acc[1][c], az = b.acc[2][c];
+      dU[1] += dt * rho * ax;
+      dU[2] += dt * rho * ay;
+      dU[3] += dt * rho * az;
+      dU[4] += dt * rho * ((b.U[S_VX][c] * ax) + (b.U[S_VY][c] * ay) +
+                           (b.U[S_VZ][c] * az));
+    }
+    // the update base is always the start-of-step state
+    double old_rho = b.U[S_RHO][c];
+    double new_rho = b.U[S_RHO][c] + dU[0];
+    new_rho = dmax(new_rho, sp.density_floor);
+    double inv_new_rho = 1. / new_rho;
+    double v[3];
+  #pragma unroll
+    for (int q = 1; q < 4; q++)
+      v[q - 1] = (b.U[fq[q]][c] * old_rho + dU[q]) * inv_new_rho;
+    double etot = (b.U[S_E][c] * old_rho + dU[4]) * inv_new_rho;
+    double eint = 0.;
+    if (DE) eint = (b.ei[c] * old_rho + dUe) * inv_new_rho;
+    // energy floor (and synchronisation)
+    const double b2 = MHD ? (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]) : 0.;
+    floor_and_sync<MHD, DE>(new_rho, v, b2, sp.gamma, sp.pressure_floor, sp.eta, sp.ggm1,
+                            etot, eint);
+    Uo[S_RHO][c] = new_rho;
+    Uo[S_VX][c] = v[0];
+    Uo[S_VY][c] = v[1];
+    Uo[S_VZ][c] = v[2];
+    Uo[S_E][c] = etot;
+    if (DE) (sp.is_final ? b.ei : b.eit)[c] = eint;
+    if (cfl) {
+      double p;
+      local = cfl_cell<MHD>(new_rho, v[0], v[1], v[2], etot, bc[0], bc[1], bc[2], sp.gamma,
+                            b.cw, p);
+      if (b.pressure) b.pressure[c] = p;
+    }
+  };
+  if (in_box) body();
+  if (cfl) block_min_to(sp.cfl, local);
 }
 
 // ----------------------------------------------------------------- k_dt
@@ -656,21 +747,28 @@ struct DualParams {
 template <typename FluxT, bool MHD, bool DE = false>
 __global__ void __launch_bounds__(256)
 k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
-     unsigned long long *result, DualParams dp) {
+     unsigned long long *result, DualParams dp, int skip_lo) {
   const BlockDesc<FluxT> &b = blocks[blockIdx.y];
   const long n = (long)b.mx * b.my * b.mz;
+  const long nxy = (long)b.mx * b.my;
   double local = 1.7976931348623157e308;
   const bool live = !DE || dp.time == nullptr || *dp.time < dp.t_stop;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (long)gridDim.x * blockDim.x) {
+    if (skip_lo > 0) {
+      // the cells of [skip_lo, m - skip_lo) were taken by the final k_cells
+      const int k = (int)(c / nxy);
+      const long r = c - (long)k * nxy;
+      const int j = (int)(r / b.mx), i = (int)(r - (long)j * b.mx);
+      if (i >= skip_lo && i < b.mx - skip_lo && j >= skip_lo && j < b.my - skip_lo &&
+          k >= skip_lo && k < b.mz - skip_lo)
+        continue;
+    }
     double d = b.U[S_RHO][c];
     double vx = b.U[S_VX][c], vy = b.U[S_VY][c], vz = b.U[S_VZ][c];
-    double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
-    double me = 0.;
     double bx = 0., by = 0., bz = 0.;
     if (MHD) {
       bx = b.U[S_BX][c]; by = b.U[S_BY][c]; bz = b.U[S_BZ][c];
-      me = 0.5 * (bx * bx + by * by + bz * bz);
     }
     double p;
     if (DE) {
@@ -688,38 +786,12 @@ k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
       }
       p = (gamma - 1.0) * d * eint;
     } else {
-      p = (gamma - 1.0) * (d * (b.U[S_E][c] - ke) - me);
+      p = cell_pressure<MHD>(d, vx, vy, vz, b.U[S_E][c], bx, by, bz, gamma);
     }
     if (b.pressure) b.pressure[c] = p;
-    double sig;
-    if (MHD) {
-      double B2 = ((bx * bx) + ((by * by) + (bz * bz)));
-      double cs2 = gamma * p / d;
-      double va2 = B2 / d;
-      sig = sqrt(va2 + cs2);
-    } else {
-      sig = sqrt(gamma * p / d);
-    }
-    double l = dmin3(b.cw[0] / (fabs(vx) + sig), b.cw[1] / (fabs(vy) + sig),
-                     b.cw[2] / (fabs(vz) + sig));
-    local = dmin(local, l);
+    local = dmin(local, cfl_speed<MHD>(d, vx, vy, vz, bx, by, bz, p, gamma, b.cw));
   }
-  unsigned long long bits = __double_as_longlong(local);
-  if (!(local >= 0.0)) bits = 0x7FF8000000000000ull;  // NaN never wins
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long o = __shfl_xor(bits, off, 64);
-    bits = (o < bits) ? o : bits;
-  }
-  __shared__ unsigned long long wmin[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) wmin[wave] = bits;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long m = wmin[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = (wmin[w] < m) ? wmin[w] : m;
-    atomicMin(result, m);
-  }
+  block_min_to(result, local);
 }
 
 static __global__ void k_dt_init(unsigned long long *result) {

@@ -342,11 +342,14 @@ struct StoreArgs {
 // dts: per-block dt in device memory (n values) or null (all blocks use dt).
 // st: store the final stage's face fluxes (hydro only) or null.
 // DE: the dual-energy formalism (FP64 path only).
+// cfl (non-null): the final stage's k_cells also takes the CFL minimum of
+// its cells into *cfl and writes their pressure.
 template <typename FluxT, bool MHD, bool DE = false>
 int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   double dt, const double *dts, hipStream_t s,
                   const StoreArgs *st = nullptr,
-                  const double *const *accel = nullptr) {
+                  const double *const *accel = nullptr,
+                  unsigned long long *cfl = nullptr) {
   const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
   if (DE != (m->dual != 0)) return ENZOE_ERR_CONFIG;
   const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd, m->dual);
@@ -410,6 +413,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       sp.xcd_remap = launch_mask(m, mx, my, mz);
       sp.eta = m->eta;
       sp.ggm1 = sync_ggm1(m->gamma);
+      sp.cfl = is_final ? cfl : nullptr;
       const int cs = sp.cur_stale;
       {
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm");
@@ -460,6 +464,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
 int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   double dt, const double *dts, hipStream_t s,
                   const StoreArgs *st = nullptr,
-                  const double *const *accel = nullptr);
+                  const double *const *accel = nullptr,
+                  unsigned long long *cfl = nullptr);
 
 }  // namespace enzoe

@@ -11,9 +11,10 @@ namespace enzoe {
 
 int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   double dt, const double *dts, hipStream_t s,
-                  const StoreArgs *st, const double *const *accel) {
-  return m->mhd ? compute_group<float, true>(m, blocks, n, dt, dts, s, st, accel)
-                : compute_group<float, false>(m, blocks, n, dt, dts, s, st, accel);
+                  const StoreArgs *st, const double *const *accel,
+                  unsigned long long *cfl) {
+  return m->mhd ? compute_group<float, true>(m, blocks, n, dt, dts, s, st, accel, cfl)
+                : compute_group<float, false>(m, blocks, n, dt, dts, s, st, accel, cfl);
 }
 
 }  // namespace enzoe

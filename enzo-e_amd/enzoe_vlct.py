@@ -123,6 +123,8 @@ def lib():
                                           vp, _dp]
         L.enzoe_vlct_compute.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                          ctypes.c_double, vp]
+        L.enzoe_vlct_compute_timestep.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                                  ctypes.c_double, vp, _dp]
         L.enzoe_vlct_compute_dt.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                             _dp, ctypes.c_int, vp]
         L.enzoe_vlct_autotune.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
@@ -304,6 +306,15 @@ class VlctMethod:
     def compute(self, blocks, dt, stream=None):
         check(lib().enzoe_vlct_compute(self.handle, self._blocks(blocks), len(blocks),
                                        dt, stream), "enzoe_vlct_compute")
+
+    def compute_timestep(self, blocks, dt, stream=None):
+        """compute(blocks, dt), then the CFL dt of the result, fused; returns
+        the next dt (equal to timestep() after compute())."""
+        nxt = ctypes.c_double(0)
+        check(lib().enzoe_vlct_compute_timestep(self.handle, self._blocks(blocks), len(blocks),
+                                                dt, stream, ctypes.byref(nxt)),
+              "enzoe_vlct_compute_timestep")
+        return nxt.value
 
     def autotune(self, blocks, dt, stream=None):
         """Picks the launch order of the stage kernels for this block shape

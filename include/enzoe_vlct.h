@@ -108,6 +108,21 @@ double enzoe_vlct_courant(enzoe_vlct_t m);
 int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                         enzoe_stream_t stream, double *dt_out);
 
+/* EnzoMethodMHDVlct::compute (EnzoMethodMHDVlct.cpp:356-500) followed by
+ * the next cycle's EnzoMethodMHDVlct::timestep (:551-588) on the blocks it
+ * leaves -- Cello's compute, then the CFL reduction of the next cycle
+ * (control_compute.cpp:102, control_stopping.cpp:67-69) with nothing in
+ * between -- in fewer passes over memory: the final stage's cell update also
+ * writes the pressure field and takes the CFL minimum of the cells it
+ * updates, and the CFL kernel covers only the ghost layers outside them.
+ * *dt_next and every field equal what enzoe_vlct_compute and then
+ * enzoe_vlct_timestep give, bit for bit.  With the dual-energy formalism
+ * (its timestep first synchronises the energies of every cell) the two
+ * calls run as they are.  Synchronises the stream.                          */
+int enzoe_vlct_compute_timestep(enzoe_vlct_t m, const enzoe_block *blocks,
+                                int n, double dt, enzoe_stream_t stream,
+                                double *dt_next);
+
 /* EnzoMethodMHDVlct::compute(Block*) (EnzoMethodMHDVlct.cpp:356-500) on n
  * blocks with one timestep dt: both VL stages (dt/2 with nearest-neighbour
  * reconstruction, then dt with the configured reconstructor), constrained

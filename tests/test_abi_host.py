@@ -189,6 +189,8 @@ def test_fused_entry_points_reject_bad_arguments_without_a_device():
     assert L.enzoe_vlct_compute_dt(h, empty, 1, dts, 0, None) == ERR_ARG
     dtp = ctypes.c_double()
     assert L.enzoe_vlct_timestep(h, empty, 0, None, ctypes.byref(dtp)) == ERR_ARG
+    assert L.enzoe_vlct_compute_timestep(h, empty, 0, 0.1, None, ctypes.byref(dtp)) == ERR_ARG
+    assert L.enzoe_vlct_compute_timestep(h, empty, 1, 0.1, None, None) == ERR_ARG
     mask = ctypes.c_int()
     assert L.enzoe_vlct_autotune(h, empty, 1, 0.0, None, ctypes.byref(mask)) == ERR_ARG
     g = ctypes.c_void_p()
@@ -210,10 +212,17 @@ def test_fused_entry_points_reject_bad_arguments_without_a_device():
     b.total_energy = fake
     arr = (ev.Block * 1)(b)
     assert L.enzoe_vlct_compute(h, arr, 1, 0.1, None) == ERR_ARG
+    assert L.enzoe_vlct_compute_timestep(h, arr, 1, 0.1, None, ctypes.byref(dtp)) == ERR_ARG
+    # cell indices are 32-bit: a block of 2^31 cells or more is refused
     b.ghost = 3
+    b.mx = b.my = b.mz = 1300
+    arr = (ev.Block * 1)(b)
+    assert L.enzoe_vlct_compute(h, arr, 1, 0.1, None) == ERR_ARG
+    b.mx = b.my = b.mz = 12
     b.bfieldi[1] = None
     arr = (ev.Block * 1)(b)
     assert L.enzoe_vlct_compute(h, arr, 1, 0.1, None) == ERR_ARG
+    assert L.enzoe_vlct_compute_timestep(h, arr, 1, 0.1, None, ctypes.byref(dtp)) == ERR_ARG
     L.enzoe_vlct_destroy(h)
 
 

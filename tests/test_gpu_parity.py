@@ -298,3 +298,70 @@ def test_tiled_cell_order_matches_oracle(mhd, mask, monkeypatch):
         t += dt_o
     _ev().synchronize()
     compare_grids(dg, grid, rel=0.0)
+
+
+FUSED_CASES = [
+    dict(mhd=True, riemann="hlld", recon=("nn", "plm"), precision=0, n=(14, 12, 10), mask=None),
+    dict(mhd=True, riemann="hlle", recon=("nn", "nn"), precision=0, n=(14, 12, 10), mask=None),
+    dict(mhd=False, riemann="hllc", recon=("nn", "plm"), precision=0, n=(14, 12, 10), mask=None),
+    dict(mhd=True, riemann="hlld", recon=("nn", "plm"), precision=1, n=(14, 12, 10), mask=None),
+    dict(mhd=True, riemann="hlld", recon=("nn", "plm"), precision=0, n=(128, 128, 6), mask=15),
+]
+
+
+@pytest.mark.parametrize("case", FUSED_CASES, ids=lambda c: "%s-%s-%s-%s-%s" % (
+    "mhd" if c["mhd"] else "hd", c["riemann"], c["recon"][1],
+    "mixed" if c["precision"] else "fp64", "x".join(map(str, c["n"]))))
+def test_compute_timestep_equals_compute_then_timestep(case, monkeypatch):
+    """enzoe_vlct_compute_timestep (the final cell update also takes the CFL
+    minimum and writes the pressure, the CFL kernel only the outer ghost
+    layers) gives the next dt and every field, pressure included, bit for
+    bit as compute() followed by timestep(), over several cycles (NN + NN
+    moves the final update box; one case runs every stage kernel in the
+    tiled order)."""
+    print("fused", case, flush=True)
+    ev = _ev()
+    if case["mask"] is not None:
+        monkeypatch.setenv("ENZOE_XCD_REMAP", str(case["mask"]))
+    grid = random_periodic_grid(mhd=case["mhd"], seed=7, n=case["n"], riemann=case["riemann"],
+                                recon=case["recon"], vamp=0.6, bamp=0.08,
+                                ext=(1.2, 1.0, 0.8) if case["n"][0] < 100 else (1.0, 1.0, 0.05))
+    a = device_copy(grid, method_for(grid.cfg, precision=case["precision"]))
+    b = device_copy(grid, method_for(grid.cfg, precision=case["precision"]))
+    dt_a = a.method.timestep(a.blocks)
+    dt_b = b.method.timestep(b.blocks)
+    assert dt_a == dt_b
+    for _ in range(3):
+        a.refresh()
+        a.method.compute(a.blocks, dt_a)
+        dt_a = a.method.timestep(a.blocks)
+        b.refresh()
+        dt_b = b.method.compute_timestep(b.blocks, dt_b)
+        assert dt_b == dt_a
+    ev.synchronize()
+    ga, gb = a.blocks[0].download(), b.blocks[0].download()
+    for f in ga:
+        assert np.array_equal(ga[f], gb[f]), f
+
+
+def test_compute_timestep_on_blocks_of_two_shapes():
+    """The fused CFL over a call with blocks of two shapes (two runs of the
+    stage kernels, one minimum) equals compute() then timestep()."""
+    ev = _ev()
+    g1 = random_periodic_grid(mhd=True, seed=21, n=(12, 10, 8))
+    g2 = random_periodic_grid(mhd=True, seed=22, n=(10, 12, 14), vamp=0.9)
+    m = method_for(g1.cfg)
+    da, db = device_copy(g1, m), device_copy(g2, m)
+    ea, eb = device_copy(g1, m), device_copy(g2, m)
+    blocks_d = da.blocks + db.blocks
+    blocks_e = ea.blocks + eb.blocks
+    dt = m.timestep(blocks_d)
+    m.compute(blocks_d, dt)
+    ref = m.timestep(blocks_d)
+    got = m.compute_timestep(blocks_e, dt)
+    ev.synchronize()
+    assert got == ref
+    for x, y in zip(blocks_d, blocks_e):
+        gx, gy = x.download(), y.download()
+        for f in gx:
+            assert np.array_equal(gx[f], gy[f]), f
