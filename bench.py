@@ -69,6 +69,10 @@ def parse():
     ap.add_argument("--dual-energy", action="store_true",
                     help="orszag_tang with the dual-energy formalism on (eta 0.001) "
                          "as the measured step (for profiling that path)")
+    ap.add_argument("--launch-mask", type=int, default=-1,
+                    help="orszag_tang: fixed launch order of the stage kernels (bit mask "
+                         "k_flux 1, k_edge 2, k_faces 4, k_cells 8 in the tiled "
+                         "XCD-contiguous order) instead of the autotuner's choice")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (HBM counter scale)")
     return ap.parse_args()
@@ -202,10 +206,13 @@ def cpu_baseline(n, nsteps):
 METRIC = "zone-updates/s/GPU (batched FP32-mixed vs plain FP64 VL+CT) + HBM GB/s"
 
 
-def timed_run(precision, n, steps, warmup, dual_energy=False):
+def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1):
     """Warm-up, then `steps` timed steps with per-kernel HIP-event timing.
-    dual_energy: the "modern" dual-energy formalism (eta 0.001) on."""
+    dual_energy: the "modern" dual-energy formalism (eta 0.001) on.
+    launch_mask >= 0: that launch order instead of the autotuner's."""
     prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
+    if launch_mask >= 0:
+        os.environ["ENZOE_XCD_REMAP"] = str(launch_mask)
     method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
                            theta=2.0, courant=0.4, gamma=5.0 / 3.0,
                            precision=prec,
@@ -217,7 +224,10 @@ def timed_run(precision, n, steps, warmup, dual_energy=False):
     ev.synchronize()
     # launch order of the stage kernels for this block shape (timed on a
     # copy of the block, outside the timed region)
-    mask = method.autotune(grid.blocks, dt)
+    if launch_mask >= 0:
+        mask = launch_mask
+    else:
+        mask = method.autotune(grid.blocks, dt)
     ev.profiling(True)
     ev.synchronize()
     t0 = time.perf_counter()
@@ -604,7 +614,8 @@ def main():
     if a.dual_energy and a.precision != "fp64":
         print("the dual-energy formalism runs in FP64 only", file=sys.stderr)
         sys.exit(2)
-    r = timed_run(a.precision, a.n, a.steps, a.warmup, dual_energy=a.dual_energy)
+    r = timed_run(a.precision, a.n, a.steps, a.warmup, dual_energy=a.dual_energy,
+                  launch_mask=a.launch_mask)
     other = dual = None
     if not a.no_compare:
         other = timed_run("mixed" if a.precision == "fp64" else "fp64", a.n,
