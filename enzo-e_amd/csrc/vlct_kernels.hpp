@@ -584,7 +584,12 @@ __device__ __forceinline__ void block_min_to(unsigned long long *result, double 
   if (threadIdx.x == 0) {
     unsigned long long m = wmin[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = (wmin[w] < m) ? wmin[w] : m;
-    atomicMin(result, m);
+    // one address for every workgroup of a launch (hundreds of thousands
+    // in a batched cell update): most find a smaller minimum there already
+    // and skip the atomic.  The value only decreases, so a stale read is
+    // larger and never skips a needed update.
+    if (m < __hip_atomic_load(result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(result, m);
   }
 }
 
@@ -749,20 +754,48 @@ __global__ void __launch_bounds__(256)
 k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
      unsigned long long *result, DualParams dp, int skip_lo) {
   const BlockDesc<FluxT> &b = blocks[blockIdx.y];
-  const long n = (long)b.mx * b.my * b.mz;
-  const long nxy = (long)b.mx * b.my;
+  const int mx = b.mx, my = b.my, mz = b.mz, lo = skip_lo;
+  // skip_lo > 0: the cells of [lo, m - lo) were taken by the final k_cells;
+  // the rest (the outer ghost layers) are enumerated as the 2 lo outer
+  // planes, then in the other planes the 2 lo outer rows, then in the other
+  // rows the 2 lo cells at either end
+  const long plane = (long)mx * my;
+  const long n1 = lo ? 2L * lo * plane : plane * mz;
+  const long n2 = lo ? (long)(mz - 2 * lo) * 2 * lo * mx : 0;
+  const long n3 = lo ? (long)(mz - 2 * lo) * (my - 2 * lo) * 2 * lo : 0;
+  const long n = n1 + n2 + n3;
   double local = 1.7976931348623157e308;
   const bool live = !DE || dp.time == nullptr || *dp.time < dp.t_stop;
-  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-       c += (long)gridDim.x * blockDim.x) {
-    if (skip_lo > 0) {
-      // the cells of [skip_lo, m - skip_lo) were taken by the final k_cells
-      const int k = (int)(c / nxy);
-      const long r = c - (long)k * nxy;
-      const int j = (int)(r / b.mx), i = (int)(r - (long)j * b.mx);
-      if (i >= skip_lo && i < b.mx - skip_lo && j >= skip_lo && j < b.my - skip_lo &&
-          k >= skip_lo && k < b.mz - skip_lo)
-        continue;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    long c = t;
+    if (lo > 0) {
+      int k, j, i;
+      if (t < n1) {
+        const int p = (int)(t / plane);
+        const long r = t - (long)p * plane;
+        k = p < lo ? p : mz - 2 * lo + p;
+        j = (int)(r / mx);
+        i = (int)(r - (long)j * mx);
+      } else if (t < n1 + n2) {
+        const long u = t - n1, per = 2L * lo * mx;
+        const int kk = (int)(u / per);
+        const long r = u - (long)kk * per;
+        const int q = (int)(r / mx);
+        i = (int)(r - (long)q * mx);
+        k = lo + kk;
+        j = q < lo ? q : my - 2 * lo + q;
+      } else {
+        const long u = t - n1 - n2, per = (long)(my - 2 * lo) * 2 * lo;
+        const int kk = (int)(u / per);
+        const long r = u - (long)kk * per;
+        const int jj = (int)(r / (2 * lo));
+        const int q = (int)(r - (long)jj * 2 * lo);
+        k = lo + kk;
+        j = lo + jj;
+        i = q < lo ? q : mx - 2 * lo + q;
+      }
+      c = ((long)k * my + j) * mx + i;
     }
     double d = b.U[S_RHO][c];
     double vx = b.U[S_VX][c], vy = b.U[S_VY][c], vz = b.U[S_VZ][c];
