@@ -203,9 +203,9 @@ int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
     int rc;
     if (m->precision == ENZOE_PREC_DOUBLE && m->dual) {
       rc = m->mhd ? compute_group<double, true, true>(m, blocks + b0, b1 - b0, dt, dts, s,
-                                                      rs, ra)
+                                                      rs, ra, cfl)
                   : compute_group<double, false, true>(m, blocks + b0, b1 - b0, dt, dts, s,
-                                                       rs, ra);
+                                                       rs, ra, cfl);
     } else if (m->precision == ENZOE_PREC_DOUBLE) {
       rc = m->mhd ? compute_group<double, true>(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra,
                                                 cfl)
@@ -391,11 +391,6 @@ int enzoe_vlct_compute_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n
                                 double dt, enzoe_stream_t stream, double *dt_next) {
   if (!m || !blocks || n <= 0 || !dt_next) return ENZOE_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  if (m->dual) {
-    // the timestep synchronises the energies of every cell first: run apart
-    if (int rc = compute_runs(m, blocks, n, dt, nullptr, s)) return rc;
-    return enzoe_vlct_timestep(m, blocks, n, stream, dt_next);
-  }
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
   if (!m->d_dt &&

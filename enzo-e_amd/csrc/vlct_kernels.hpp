@@ -624,7 +624,7 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const bool in_box = cell_coords(t, lo, mx - 2 * lo, my - 2 * lo, mz - 2 * lo,
                                   (sp.xcd_remap & 8) ? b.tile_rows : 0, k, j, i);
   // with the CFL minimum every thread takes part in the reduction
-  const bool cfl = !DE && sp.cfl != nullptr;
+  const bool cfl = sp.cfl != nullptr;
   if (!in_box && !cfl) return;
   double local = 1.7976931348623157e308;
   auto body = [&]() {
@@ -649,9 +649,24 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
       if (cfl) {
         // outside the update, inside the cell-centred B rebuild: the
         // start-of-step state with the new centre B
+        const double d = Uo[S_RHO][c];
+        const double v[3] = {Uo[S_VX][c], Uo[S_VY][c], Uo[S_VZ][c]};
         double p;
-        local = cfl_cell<MHD>(Uo[S_RHO][c], Uo[S_VX][c], Uo[S_VY][c], Uo[S_VZ][c],
-                              Uo[S_E][c], bc[0], bc[1], bc[2], sp.gamma, b.cw, p);
+        if (DE) {
+          // the timestep's in-place synchronisation first
+          // (EnzoMethodMHDVlct.cpp:563-568), as k_dt does it
+          const double b2 = MHD ? (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]) : 0.;
+          double etot = Uo[S_E][c], eint = b.ei[c];
+          floor_and_sync<MHD, true>(d, v, b2, sp.gamma, sp.pressure_floor, sp.eta, sp.ggm1,
+                                    etot, eint);
+          Uo[S_E][c] = etot;
+          b.ei[c] = eint;
+          p = (sp.gamma - 1.0) * d * eint;
+          local = cfl_speed<MHD>(d, v[0], v[1], v[2], bc[0], bc[1], bc[2], p, sp.gamma, b.cw);
+        } else {
+          local = cfl_cell<MHD>(d, v[0], v[1], v[2], Uo[S_E][c], bc[0], bc[1], bc[2],
+                                sp.gamma, b.cw, p);
+        }
         if (b.pressure) b.pressure[c] = p;
       }
       return;
@@ -718,14 +733,24 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     Uo[S_VX][c] = v[0];
     Uo[S_VY][c] = v[1];
     Uo[S_VZ][c] = v[2];
-    Uo[S_E][c] = etot;
-    if (DE) (sp.is_final ? b.ei : b.eit)[c] = eint;
     if (cfl) {
       double p;
-      local = cfl_cell<MHD>(new_rho, v[0], v[1], v[2], etot, bc[0], bc[1], bc[2], sp.gamma,
-                            b.cw, p);
+      if (DE) {
+        // the timestep synchronises the energies of the cell once more, in
+        // place, before its pressure (EnzoMethodMHDVlct.cpp:563-568)
+        floor_and_sync<MHD, DE>(new_rho, v, b2, sp.gamma, sp.pressure_floor, sp.eta,
+                                sp.ggm1, etot, eint);
+        p = (sp.gamma - 1.0) * new_rho * eint;
+        local = cfl_speed<MHD>(new_rho, v[0], v[1], v[2], bc[0], bc[1], bc[2], p, sp.gamma,
+                               b.cw);
+      } else {
+        local = cfl_cell<MHD>(new_rho, v[0], v[1], v[2], etot, bc[0], bc[1], bc[2],
+                              sp.gamma, b.cw, p);
+      }
       if (b.pressure) b.pressure[c] = p;
     }
+    Uo[S_E][c] = etot;
+    if (DE) (sp.is_final ? b.ei : b.eit)[c] = eint;
   };
   if (in_box) body();
   if (cfl) block_min_to(sp.cfl, local);

@@ -116,9 +116,9 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * writes the pressure field and takes the CFL minimum of the cells it
  * updates, and the CFL kernel covers only the ghost layers outside them.
  * *dt_next and every field equal what enzoe_vlct_compute and then
- * enzoe_vlct_timestep give, bit for bit.  With the dual-energy formalism
- * (its timestep first synchronises the energies of every cell) the two
- * calls run as they are.  Synchronises the stream.                          */
+ * enzoe_vlct_timestep give, bit for bit; with the dual-energy formalism the
+ * updated cells' energies are synchronised a second time there, as the
+ * timestep does it in place.  Synchronises the stream.                      */
 int enzoe_vlct_compute_timestep(enzoe_vlct_t m, const enzoe_block *blocks,
                                 int n, double dt, enzoe_stream_t stream,
                                 double *dt_next);

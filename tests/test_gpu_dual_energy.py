@@ -411,3 +411,32 @@ def test_dual_energy_tube_golden_through_the_staged_plugin_path():
                              bkg_velocity=(po.DE_TUBE_AXIS_VELOCITY, 0.0, 0.0))
     print("staged cycles", n, "L1", repr(l1))
     assert np.isclose(l1, DE_L1, rtol=1e-13, atol=7e-14)
+
+
+@pytest.mark.parametrize("case", CASES[:2], ids=lambda c: "%s-%s-%s" % (
+    "mhd" if c["mhd"] else "hd", c["riemann"], c["recon"][1]))
+def test_dual_energy_compute_timestep_equals_compute_then_timestep(case):
+    """With the dual-energy formalism the fused compute + CFL applies the
+    timestep's in-place synchronisation to the cells of the final update
+    (and the CFL pass to the ghost shell): next dt, energies, pressure and
+    every other field equal compute() followed by timestep(), bit for bit."""
+    print("fused de", case, flush=True)
+    ev = _ev()
+    grid = _de_grid(case, seed=13)
+    a = device_copy(grid, method_for(grid.cfg))
+    b = device_copy(grid, method_for(grid.cfg))
+    dt_a = a.method.timestep(a.blocks)
+    dt_b = b.method.timestep(b.blocks)
+    assert dt_a == dt_b
+    for _ in range(3):
+        a.refresh()
+        a.method.compute(a.blocks, dt_a)
+        dt_a = a.method.timestep(a.blocks)
+        b.refresh()
+        dt_b = b.method.compute_timestep(b.blocks, dt_b)
+        assert dt_b == dt_a
+    ev.synchronize()
+    ga, gb = a.blocks[0].download(), b.blocks[0].download()
+    assert "internal_energy" in ga
+    for f in ga:
+        assert np.array_equal(ga[f], gb[f]), f
