@@ -300,10 +300,8 @@ namespace {
 // list changed, so the call makes no host round trip otherwise.
 // d_time / t_stop: the device time of a device-driven cycle and its stop
 // time (a cycle past it leaves the dual-energy fields alone), or null.
-// skip_lo > 0: only the cells outside [skip_lo, m - skip_lo) (the rest came
-// from the final k_cells) and no reset of the minimum.
 int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_t s,
-              const double *d_time = nullptr, double t_stop = 0.0, int skip_lo = 0) {
+              const double *d_time = nullptr, double t_stop = 0.0) {
   for (int i = 0; i < n; i++)
     if (int rc = check_block(m, blocks[i])) return rc;
   if (!m->d_dt &&
@@ -341,7 +339,7 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
   // with dual energy the reduction also writes the energies: a dry call
   // (graph set-up) uploads the table only
   if (m->dry) return ENZOE_OK;
-  if (skip_lo == 0) hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
+  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
   long maxcells = 0;
   for (int i = 0; i < n; i++)
     maxcells = std::max<long>(maxcells, (long)blocks[i].mx * blocks[i].my * blocks[i].mz);
@@ -354,16 +352,16 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
   const DualParams dp = {m->pfloor, m->eta, sync_ggm1(m->gamma), d_time, t_stop};
   if (m->mhd && m->dual)
     hipLaunchKernelGGL((k_dt<double, true, true>), grid, dim3(256), 0, s, dd, m->gamma,
-                       m->d_dt, dp, skip_lo);
+                       m->d_dt, dp);
   else if (m->mhd)
     hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s, dd, m->gamma, m->d_dt,
-                       dp, skip_lo);
+                       dp);
   else if (m->dual)
     hipLaunchKernelGGL((k_dt<double, false, true>), grid, dim3(256), 0, s, dd, m->gamma,
-                       m->d_dt, dp, skip_lo);
+                       m->d_dt, dp);
   else
     hipLaunchKernelGGL((k_dt<double, false>), grid, dim3(256), 0, s, dd, m->gamma, m->d_dt,
-                       dp, skip_lo);
+                       dp);
   return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
 }
 
@@ -398,15 +396,12 @@ int enzoe_vlct_compute_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n
     m->d_dt = nullptr;
     return ENZOE_ERR_ALLOC;
   }
-  // the final k_cells covers [lo, m - lo) with lo the final stage's stale depth
-  int lo = 0;
-  for (int st = 0; st + 1 < m->nstages; st++) lo += total_staling(m->recon[st]);
-  lo += immediate_staling(m->recon[m->nstages - 1]);
+  // the final k_cells covers every cell: the update, and the CFL term of
+  // each cell of the block into m->d_dt
   hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
   if (int rc = compute_runs(m, blocks, n, dt, nullptr, s, nullptr, nullptr,
                             (unsigned long long *)m->d_dt))
     return rc;
-  if (int rc = launch_dt(m, blocks, n, s, nullptr, 0.0, lo)) return rc;
   unsigned long long bits = 0;
   if (hipMemcpyAsync(&bits, m->d_dt, sizeof(bits), hipMemcpyDeviceToHost, s) !=
       hipSuccess)

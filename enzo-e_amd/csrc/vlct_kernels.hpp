@@ -621,11 +621,16 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int lo = sp.is_final ? cs : cs + 1;
   int k, j, i;
   const long t = (long)bx * blockDim.x + threadIdx.x;
-  const bool in_box = cell_coords(t, lo, mx - 2 * lo, my - 2 * lo, mz - 2 * lo,
-                                  (sp.xcd_remap & 8) ? b.tile_rows : 0, k, j, i);
-  // with the CFL minimum every thread takes part in the reduction
+  // with the CFL minimum the launch covers the whole block (the cells
+  // outside [lo, m - lo) only take their CFL term) and every thread takes
+  // part in the reduction
   const bool cfl = sp.cfl != nullptr;
-  if (!in_box && !cfl) return;
+  const int blo = cfl ? 0 : lo;
+  const bool in_launch = cell_coords(t, blo, mx - 2 * blo, my - 2 * blo, mz - 2 * blo,
+                                     (sp.xcd_remap & 8) ? b.tile_rows : 0, k, j, i);
+  if (!in_launch && !cfl) return;
+  const bool in_box = in_launch && k >= lo && j >= lo && i >= lo && k < mz - lo &&
+                      j < my - lo && i < mx - lo;
   double local = 1.7976931348623157e308;
   auto body = [&]() {
     const long c = idx3(k, j, i, my, mx);
@@ -752,7 +757,34 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     Uo[S_E][c] = etot;
     if (DE) (sp.is_final ? b.ei : b.eit)[c] = eint;
   };
-  if (in_box) body();
+  if (in_box) {
+    body();
+  } else if (in_launch && cfl) {
+    // the outer ghost layers: the timestep's work on the state as it is
+    // (k_dt's per-cell steps)
+    const long c = idx3(k, j, i, my, mx);
+    const double d = b.U[S_RHO][c];
+    const double v[3] = {b.U[S_VX][c], b.U[S_VY][c], b.U[S_VZ][c]};
+    double bxc = 0., byc = 0., bzc = 0.;
+    if (MHD) {
+      bxc = b.U[S_BX][c]; byc = b.U[S_BY][c]; bzc = b.U[S_BZ][c];
+    }
+    double p;
+    if (DE) {
+      const double b2 = MHD ? (bxc * bxc + byc * byc + bzc * bzc) : 0.;
+      double etot = b.U[S_E][c], eint = b.ei[c];
+      floor_and_sync<MHD, true>(d, v, b2, sp.gamma, sp.pressure_floor, sp.eta, sp.ggm1,
+                                etot, eint);
+      b.U[S_E][c] = etot;
+      b.ei[c] = eint;
+      p = (sp.gamma - 1.0) * d * eint;
+      local = cfl_speed<MHD>(d, v[0], v[1], v[2], bxc, byc, bzc, p, sp.gamma, b.cw);
+    } else {
+      local = cfl_cell<MHD>(d, v[0], v[1], v[2], b.U[S_E][c], bxc, byc, bzc, sp.gamma,
+                            b.cw, p);
+    }
+    if (b.pressure) b.pressure[c] = p;
+  }
   if (cfl) block_min_to(sp.cfl, local);
 }
 
@@ -777,51 +809,13 @@ struct DualParams {
 template <typename FluxT, bool MHD, bool DE = false>
 __global__ void __launch_bounds__(256)
 k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
-     unsigned long long *result, DualParams dp, int skip_lo) {
+     unsigned long long *result, DualParams dp) {
   const BlockDesc<FluxT> &b = blocks[blockIdx.y];
-  const int mx = b.mx, my = b.my, mz = b.mz, lo = skip_lo;
-  // skip_lo > 0: the cells of [lo, m - lo) were taken by the final k_cells;
-  // the rest (the outer ghost layers) are enumerated as the 2 lo outer
-  // planes, then in the other planes the 2 lo outer rows, then in the other
-  // rows the 2 lo cells at either end
-  const long plane = (long)mx * my;
-  const long n1 = lo ? 2L * lo * plane : plane * mz;
-  const long n2 = lo ? (long)(mz - 2 * lo) * 2 * lo * mx : 0;
-  const long n3 = lo ? (long)(mz - 2 * lo) * (my - 2 * lo) * 2 * lo : 0;
-  const long n = n1 + n2 + n3;
+  const long n = (long)b.mx * b.my * b.mz;
   double local = 1.7976931348623157e308;
   const bool live = !DE || dp.time == nullptr || *dp.time < dp.t_stop;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x) {
-    long c = t;
-    if (lo > 0) {
-      int k, j, i;
-      if (t < n1) {
-        const int p = (int)(t / plane);
-        const long r = t - (long)p * plane;
-        k = p < lo ? p : mz - 2 * lo + p;
-        j = (int)(r / mx);
-        i = (int)(r - (long)j * mx);
-      } else if (t < n1 + n2) {
-        const long u = t - n1, per = 2L * lo * mx;
-        const int kk = (int)(u / per);
-        const long r = u - (long)kk * per;
-        const int q = (int)(r / mx);
-        i = (int)(r - (long)q * mx);
-        k = lo + kk;
-        j = q < lo ? q : my - 2 * lo + q;
-      } else {
-        const long u = t - n1 - n2, per = (long)(my - 2 * lo) * 2 * lo;
-        const int kk = (int)(u / per);
-        const long r = u - (long)kk * per;
-        const int jj = (int)(r / (2 * lo));
-        const int q = (int)(r - (long)jj * 2 * lo);
-        k = lo + kk;
-        j = lo + jj;
-        i = q < lo ? q : mx - 2 * lo + q;
-      }
-      c = ((long)k * my + j) * mx + i;
-    }
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (long)gridDim.x * blockDim.x) {
     double d = b.U[S_RHO][c];
     double vx = b.U[S_VX][c], vy = b.U[S_VY][c], vz = b.U[S_VZ][c];
     double bx = 0., by = 0., bz = 0.;

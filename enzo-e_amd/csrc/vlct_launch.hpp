@@ -434,7 +434,8 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       }
       {
         KernelTimer kt(s, "k_cells");
-        const int lo = is_final ? cs : cs + 1;
+        // with the fused CFL the final update's launch covers the block
+        const int lo = sp.cfl ? 0 : (is_final ? cs : cs + 1);
         hipLaunchKernelGGL((k_cells<FluxT, MHD, DE>),
                            dim3(nblk8(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
                            s, d, sp);

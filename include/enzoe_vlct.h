@@ -112,9 +112,9 @@ int enzoe_vlct_timestep(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * the next cycle's EnzoMethodMHDVlct::timestep (:551-588) on the blocks it
  * leaves -- Cello's compute, then the CFL reduction of the next cycle
  * (control_compute.cpp:102, control_stopping.cpp:67-69) with nothing in
- * between -- in fewer passes over memory: the final stage's cell update also
- * writes the pressure field and takes the CFL minimum of the cells it
- * updates, and the CFL kernel covers only the ghost layers outside them.
+ * between -- in fewer passes over memory: the final stage's cell update,
+ * launched over the whole block, also writes the pressure field and takes
+ * the CFL minimum of every cell (no separate CFL kernel runs).
  * *dt_next and every field equal what enzoe_vlct_compute and then
  * enzoe_vlct_timestep give, bit for bit; with the dual-energy formalism the
  * updated cells' energies are synchronised a second time there, as the
