@@ -418,7 +418,7 @@ def test_dual_energy_tube_golden_through_the_staged_plugin_path():
 def test_dual_energy_compute_timestep_equals_compute_then_timestep(case):
     """With the dual-energy formalism the fused compute + CFL applies the
     timestep's in-place synchronisation to the cells of the final update
-    (and the CFL pass to the ghost shell): next dt, energies, pressure and
+    (and to the ghost layers): next dt, energies, pressure and
     every other field equal compute() followed by timestep(), bit for bit."""
     print("fused de", case, flush=True)
     ev = _ev()

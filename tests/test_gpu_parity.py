@@ -313,9 +313,9 @@ FUSED_CASES = [
     "mhd" if c["mhd"] else "hd", c["riemann"], c["recon"][1],
     "mixed" if c["precision"] else "fp64", "x".join(map(str, c["n"]))))
 def test_compute_timestep_equals_compute_then_timestep(case, monkeypatch):
-    """enzoe_vlct_compute_timestep (the final cell update also takes the CFL
-    minimum and writes the pressure, the CFL kernel only the outer ghost
-    layers) gives the next dt and every field, pressure included, bit for
+    """enzoe_vlct_compute_timestep (the final cell update, launched over the
+    whole block, also takes the CFL minimum and writes the pressure of every
+    cell) gives the next dt and every field, pressure included, bit for
     bit as compute() followed by timestep(), over several cycles (NN + NN
     moves the final update box; one case runs every stage kernel in the
     tiled order)."""
