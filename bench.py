@@ -156,6 +156,29 @@ def measured_traffic(kernel, workload):
     return None
 
 
+# wave64 FP64 (or non-packed FP32) VALU instructions MI355X issues per
+# second: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per instruction
+VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
+
+
+def measured_valu(kernel, workload, launch_ms):
+    """The flux kernel's issue rate against the VALU peak: wave-instructions
+    per launch from the committed rocprofv3 summary (SQ_INSTS_VALU) over the
+    launch time measured here, or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "valu_k_flux.txt")
+    if not os.path.exists(path) or not launch_ms or launch_ms != launch_ms:
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 4 and p[0] == "valu" and p[1] == kernel and p[2] == workload:
+                rate = float(p[3]) / (launch_ms * 1e-3)
+                return {"wave_instructions_per_launch": float(p[3]), "achieved": rate,
+                        "peak": VALU_PEAK_WAVE_INSTR, "unit": "wave-instructions/s",
+                        "frac": rate / VALU_PEAK_WAVE_INSTR}
+    return None
+
+
 def _oracle_rate(n, nsteps, threads):
     """Zone-updates/s of the oracle on an n^3 Orszag-Tang block."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -654,6 +677,8 @@ def main():
             "traffic": measured_traffic(r["dom"], r["workload"]),
             "bytes_per_launch": r["dom_bytes"],
             "avg_launch_ms": r["dom_ms"],
+            # the kernel's real bound: FP64 instruction issue
+            "valu_issue": measured_valu(r["dom"], r["workload"], r["dom_ms"]),
         },
         "step_roofline": {
             "bytes_per_zone": STEP_BYTES_PER_ZONE_FP64,
