@@ -444,7 +444,8 @@ k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int m[3] = {mx, my, mz};
   const long c = idx3(k, j, i, my, mx);
   const long str[3] = {1, (long)mx, (long)mx * my};
-#pragma unroll 1
+// the three axes unrolled: their loads go out together
+#pragma unroll
   for (int ax = 0; ax < 3; ax++) {
     const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
     if (p[ax] < cs + 1 || p[ax] >= m[ax] - cs - 1) continue;
@@ -502,7 +503,8 @@ k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int p[3] = {i, j, k};
   const int m[3] = {mx, my, mz};
   const long str[3] = {1, (long)mx, (long)mx * my};
-#pragma unroll 1
+// the three axes unrolled: their loads go out together
+#pragma unroll
   for (int ax = 0; ax < 3; ax++) {
     const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
     if (p[aj] >= m[aj] - cs - 1 || p[ak] >= m[ak] - cs - 1) continue;
