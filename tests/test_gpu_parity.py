@@ -365,3 +365,34 @@ def test_compute_timestep_on_blocks_of_two_shapes():
         gx, gy = x.download(), y.download()
         for f in gx:
             assert np.array_equal(gx[f], gy[f]), f
+
+
+def test_two_handles_on_two_host_threads():
+    """One method handle per host thread (the reference's one Method per
+    PE): two threads stepping their own grids at the same time give, bit for
+    bit, what the same cycles give one after the other."""
+    import threading
+    ev = _ev()
+    grids = [random_periodic_grid(mhd=True, seed=s, n=(16, 14, 12)) for s in (31, 32)]
+
+    def run(grid, out, i):
+        dg = device_copy(grid, method_for(grid.cfg))
+        t = 0.0
+        for _ in range(3):
+            t += dg.cycle(t, 10.0)
+        ev.synchronize()
+        out[i] = dg.blocks[0].download()
+
+    seq = [None, None]
+    for i, g in enumerate(grids):
+        run(g, seq, i)
+    par = [None, None]
+    threads = [threading.Thread(target=run, args=(g, par, i)) for i, g in enumerate(grids)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    for i in range(2):
+        assert par[i] is not None
+        for f in seq[i]:
+            assert np.array_equal(seq[i][f], par[i][f]), (i, f)
