@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -61,6 +62,129 @@ std::string lower(const char *s, const char *dflt) {
   std::transform(r.begin(), r.end(), r.begin(),
                  [](unsigned char c) { return (char)std::tolower(c); });
   return r;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+// ------------------------------------------------------------ diagnostics
+// Accumulators of enzoe_vlct_diagnostics.  Minima are kept as order-
+// preserving integer images of doubles (so an integer atomicMin is a double
+// min for either sign); maxima of non-negative values as their bit patterns.
+struct DiagAcc {
+  double sum[5];                 // rho, rho v_x, rho v_y, rho v_z, rho e_tot
+  unsigned long long min_rho, min_p, max_divb, max_bface, nonfinite;
+};
+
+__host__ __device__ inline unsigned long long double_to_ordered(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double ordered_to_double(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  return __builtin_bit_cast(double, b);
+}
+
+__global__ void k_diag_init(DiagAcc *a) {
+  for (int i = 0; i < 5; i++) a->sum[i] = 0.0;
+  a->min_rho = a->min_p = double_to_ordered(1.7976931348623157e308);
+  a->max_divb = a->max_bface = 0ull;
+  a->nonfinite = 0ull;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline unsigned long long wave_umin(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ inline unsigned long long wave_umax(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// One thread per cell of a block (grid.y = block), ghosts included: counts
+// non-finite values of every field at the cell (and of the face fields'
+// last layer at the block's high ends); over the active cells the totals of
+// the conserved quantities, the minima of density and pressure, the largest
+// |div B| (times the smallest cell width) and the largest |face B|.  Every
+// thread reaches the wave reductions (out-of-block lanes contribute
+// neutral values); one atomic per wave and quantity.
+template <bool MHD>
+__global__ void __launch_bounds__(256)
+k_diagnostics(const BlockDesc<double> *__restrict__ blocks, double gamma, DiagAcc *acc) {
+  const BlockDesc<double> &b = blocks[blockIdx.y];
+  const long ncell = (long)b.mx * b.my * b.mz;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double s[5] = {0., 0., 0., 0., 0.};
+  double rmin = 1.7976931348623157e308, pmin = rmin, divb = 0., bmax = 0.;
+  unsigned long long bad = 0;
+  if (c < ncell) {
+    const int i = (int)(c % b.mx), j = (int)((c / b.mx) % b.my), k = (int)(c / ((long)b.mx * b.my));
+    const int nf = MHD ? S_N : 5;
+    for (int q = 0; q < nf; q++) bad += !isfinite(b.U[q][c]);
+    if (MHD) {
+      const long fx = idx3(k, j, i, b.my, b.mx + 1), fy = idx3(k, j, i, b.my + 1, b.mx),
+                 fz = idx3(k, j, i, b.my, b.mx);
+      bad += !isfinite(b.bfi[0][fx]) + !isfinite(b.bfi[1][fy]) + !isfinite(b.bfi[2][fz]);
+      if (i == b.mx - 1) bad += !isfinite(b.bfi[0][fx + 1]);
+      if (j == b.my - 1) bad += !isfinite(b.bfi[1][fy + b.mx]);
+      if (k == b.mz - 1) bad += !isfinite(b.bfi[2][fz + (long)b.mx * b.my]);
+    }
+    const int g = b.g;
+    if (i >= g && j >= g && k >= g && i < b.mx - g && j < b.my - g && k < b.mz - g) {
+      const double d = b.U[S_RHO][c];
+      const double vx = b.U[S_VX][c], vy = b.U[S_VY][c], vz = b.U[S_VZ][c];
+      const double e = b.U[S_E][c];
+      s[0] = d; s[1] = d * vx; s[2] = d * vy; s[3] = d * vz; s[4] = d * e;
+      rmin = d;
+      double bx = 0., by = 0., bz = 0.;
+      if (MHD) { bx = b.U[S_BX][c]; by = b.U[S_BY][c]; bz = b.U[S_BZ][c]; }
+      pmin = cell_pressure<MHD>(d, vx, vy, vz, e, bx, by, bz, gamma);
+      if (MHD) {
+        const long fx = idx3(k, j, i, b.my, b.mx + 1), fy = idx3(k, j, i, b.my + 1, b.mx),
+                   fz = idx3(k, j, i, b.my, b.mx);
+        const double dv = (b.bfi[0][fx + 1] - b.bfi[0][fx]) / b.cw[0] +
+                          (b.bfi[1][fy + b.mx] - b.bfi[1][fy]) / b.cw[1] +
+                          (b.bfi[2][fz + (long)b.mx * b.my] - b.bfi[2][fz]) / b.cw[2];
+        divb = fabs(dv) * fmin(b.cw[0], fmin(b.cw[1], b.cw[2]));
+        bmax = fmax(fabs(b.bfi[0][fx]), fmax(fabs(b.bfi[1][fy]), fabs(b.bfi[2][fz])));
+      }
+    }
+  }
+  const bool lead = (threadIdx.x & 63) == 0;
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const double t = wave_sum(s[q]);
+    if (lead && t != 0.0) atomicAdd(&acc->sum[q], t);
+  }
+  const unsigned long long r = wave_umin(double_to_ordered(rmin));
+  const unsigned long long p = wave_umin(double_to_ordered(pmin));
+  // NaN never wins a max: a non-finite cell is counted instead
+  const unsigned long long dvb = wave_umax(isfinite(divb) ? __double_as_longlong(divb) : 0ull);
+  const unsigned long long bm = wave_umax(isfinite(bmax) ? __double_as_longlong(bmax) : 0ull);
+  unsigned long long nb = bad;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) nb += __shfl_xor(nb, off, 64);
+  if (lead) {
+    atomicMin(&acc->min_rho, r);
+    atomicMin(&acc->min_p, p);
+    atomicMax(&acc->max_divb, dvb);
+    atomicMax(&acc->max_bface, bm);
+    if (nb) atomicAdd(&acc->nonfinite, nb);
+  }
 }
 
 __global__ void k_calibration_copy(double *__restrict__ dst,
@@ -168,7 +292,8 @@ int enzoe_vlct_destroy(enzoe_vlct_t m) {
   if (m->d_ptrtab) (void)hipFree(m->d_ptrtab);
   if (m->d_dts) (void)hipFree(m->d_dts);
   if (m->d_rtab) (void)hipFree(m->d_rtab);
-  for (void *p : {m->d_fdtab, m->d_fltab, m->d_fbtab, m->d_trtab[0], m->d_trtab[1]})
+  for (void *p : {m->d_fdtab, m->d_fltab, m->d_fbtab, m->d_trtab[0], m->d_trtab[1],
+                  (void *)m->d_dbg})
     if (p) (void)hipFree(p);
   delete m;
   return ENZOE_OK;
@@ -577,6 +702,117 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
   hipLaunchKernelGGL(k_refresh, dim3(nblk(nghost, 256), nb, nf), dim3(256), 0, s,
                      (double *const *)m->d_rtab, G);
   if (hipGetLastError() != hipSuccess) return ENZOE_ERR_HIP;
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_set_debug_checks(enzoe_vlct_t m, int on) {
+  if (!m) return ENZOE_ERR_ARG;
+  if (!on) {
+    // work already queued may still count: the counters live until destroy
+    if (m->d_dbg) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(m->d_dbg);
+    }
+    m->d_dbg = nullptr;
+    return ENZOE_OK;
+  }
+  if (m->d_dbg) return ENZOE_OK;
+  unsigned long long *p = nullptr;
+  if (hipMalloc((void **)&p, DBG_N * sizeof(unsigned long long)) != hipSuccess)
+    return ENZOE_ERR_ALLOC;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, nullptr, p, DBG_N);
+  if (hipDeviceSynchronize() != hipSuccess) {
+    (void)hipFree(p);
+    return ENZOE_ERR_HIP;
+  }
+  m->d_dbg = p;
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_debug_counts(enzoe_vlct_t m, enzoe_stream_t stream,
+                            unsigned long long counts[ENZOE_DEBUG_COUNTS]) {
+  if (!m || !counts) return ENZOE_ERR_ARG;
+  if (!m->d_dbg) return ENZOE_ERR_STATE;
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(counts, m->d_dbg, DBG_N * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ENZOE_ERR_HIP;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, m->d_dbg, DBG_N);
+  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+}
+
+int enzoe_vlct_diagnostics(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                           enzoe_stream_t stream, enzoe_diagnostics *out) {
+  if (!m || !blocks || n <= 0 || !out) return ENZOE_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    const enzoe_block &b = blocks[i];
+    if (b.mx < 1 || b.my < 1 || b.mz < 1 || b.ghost < 0 || 2 * b.ghost >= b.mx ||
+        2 * b.ghost >= b.my || 2 * b.ghost >= b.mz || !b.density || !b.velocity[0] ||
+        !b.velocity[1] || !b.velocity[2] || !b.total_energy)
+      return ENZOE_ERR_ARG;
+    if (m->mhd)
+      for (int d = 0; d < 3; d++)
+        if (!b.bfield[d] || !b.bfieldi[d]) return ENZOE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<BlockDesc<double>> desc(n);
+  long maxcells = 0;
+  for (int i = 0; i < n; i++) {
+    BlockDesc<double> &d = desc[i];
+    std::memset(&d, 0, sizeof(d));
+    const enzoe_block &b = blocks[i];
+    d.U[S_RHO] = b.density;
+    for (int a = 0; a < 3; a++) d.U[S_VX + a] = b.velocity[a];
+    d.U[S_E] = b.total_energy;
+    if (m->mhd)
+      for (int a = 0; a < 3; a++) {
+        d.U[S_BX + a] = b.bfield[a];
+        d.bfi[a] = b.bfieldi[a];
+      }
+    d.mx = b.mx; d.my = b.my; d.mz = b.mz;
+    d.g = b.ghost;
+    for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
+    maxcells = std::max<long>(maxcells, (long)b.mx * b.my * b.mz);
+  }
+  // a diagnostic: its own buffers, taken and returned on every call
+  void *d_desc = nullptr, *d_acc = nullptr;
+  const size_t dbytes = desc.size() * sizeof(BlockDesc<double>);
+  if (hipMalloc(&d_desc, dbytes) != hipSuccess) return ENZOE_ERR_ALLOC;
+  if (hipMalloc(&d_acc, sizeof(DiagAcc)) != hipSuccess) {
+    (void)hipFree(d_desc);
+    return ENZOE_ERR_ALLOC;
+  }
+  int rc = ENZOE_OK;
+  DiagAcc acc;
+  if (hipMemcpyAsync(d_desc, desc.data(), dbytes, hipMemcpyHostToDevice, s) != hipSuccess)
+    rc = ENZOE_ERR_HIP;
+  if (rc == ENZOE_OK) {
+    hipLaunchKernelGGL(k_diag_init, dim3(1), dim3(1), 0, s, (DiagAcc *)d_acc);
+    dim3 grid(nblk(maxcells, 256), n);
+    if (m->mhd)
+      hipLaunchKernelGGL((k_diagnostics<true>), grid, dim3(256), 0, s,
+                         (const BlockDesc<double> *)d_desc, m->gamma, (DiagAcc *)d_acc);
+    else
+      hipLaunchKernelGGL((k_diagnostics<false>), grid, dim3(256), 0, s,
+                         (const BlockDesc<double> *)d_desc, m->gamma, (DiagAcc *)d_acc);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&acc, d_acc, sizeof(acc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      rc = ENZOE_ERR_HIP;
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(d_desc);
+  (void)hipFree(d_acc);
+  if (rc) return rc;
+  out->mass = acc.sum[0];
+  for (int a = 0; a < 3; a++) out->momentum[a] = acc.sum[1 + a];
+  out->energy = acc.sum[4];
+  out->min_density = ordered_to_double(acc.min_rho);
+  out->min_pressure = ordered_to_double(acc.min_p);
+  out->max_abs_divb = __builtin_bit_cast(double, acc.max_divb);
+  out->max_abs_bface = __builtin_bit_cast(double, acc.max_bface);
+  out->nonfinite = (long long)acc.nonfinite;
   return ENZOE_OK;
 }
 

@@ -70,7 +70,32 @@ struct StageParams {
                        // reference's synchronisation declares it
   unsigned long long *cfl;  // final k_cells: also the CFL minimum of its
                             // cells and their pressure (null: not)
+  unsigned long long *dbg;  // debug counters (DBG_*), or null: checks off
 };
+
+// Debug checks, off unless a method enables them: counts of the events the
+// reference's debug builds abort on -- a non-finite value
+// (CHECK_FINITE_ELEMENTS, src/Cello/view_CelloView.hpp:207-217) and a floor
+// that has to be applied (RAISE_FLOOR_ERROR in enzo_utils::apply_floor,
+// src/Enzo/utils/utils.hpp:102-114; its callers: the PLM reconstruction,
+// EnzoReconstructorPLM.hpp:219-220, the density update,
+// EnzoIntegrationQuanUpdate.cpp:219, the energy floor,
+// EnzoPhysicsFluidProps.cpp:278-285).
+enum : int {
+  DBG_NONFINITE = 0,       // a non-finite updated cell value
+  DBG_DENSITY_FLOOR = 1,   // updated density below the density floor
+  DBG_ENERGY_FLOOR = 2,    // energy below its floor after the update
+  DBG_RECON_FLOOR = 3,     // a reconstructed density or pressure floored
+  DBG_N = 4
+};
+
+// Adds the number of lanes of the wave with `cond` to dbg[slot]: one atomic
+// per wave, from its lowest lane with the event.
+__device__ __forceinline__ void debug_count(unsigned long long *dbg, int slot, bool cond) {
+  const unsigned long long mask = __ballot(cond);
+  if (mask != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)mask) - 1)
+    atomicAdd(&dbg[slot], (unsigned long long)__popcll(mask));
+}
 
 // Energy floor and, with the dual-energy formalism, the synchronisation of
 // internal and total energy of one cell
@@ -80,12 +105,14 @@ struct StageParams {
 // place.  The internal energy is taken from etot - non-thermal energy where
 // the sound speed that implies beats eta times the kinetic and magnetic
 // scales (and it exceeds half the tracked value unless eta = 0).
+// floored (may be null): set when the floor had to be applied.
 template <bool MHD, bool DE>
 __device__ __forceinline__ void floor_and_sync(double rho, const double v[3],
                                                double b2, double gamma,
                                                double pfloor, double eta,
                                                double ggm1, double &etot,
-                                               double &eint) {
+                                               double &eint,
+                                               bool *floored = nullptr) {
   const double inv_rho = 1. / rho;
   const double eint_floor = pfloor * (1. / (gamma - 1.)) * inv_rho;
   const double v2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -99,11 +126,13 @@ __device__ __forceinline__ void floor_and_sync(double rho, const double v[3],
     if ((cs2_1 > fmax(eta * v2, eta * (MHD ? b2 : 0.) * inv_rho)) &&
         (eint_1 > half_factor * cur_eint))
       cur_eint = eint_1;
+    if (floored) *floored = cur_eint < eint_floor;
     cur_eint = dmax(cur_eint, eint_floor);
     eint = cur_eint;
     etot = cur_eint + non_thermal_e;
   } else {
     const double etot_floor = eint_floor + non_thermal_e;
+    if (floored) *floored = etot < etot_floor;
     etot = dmax(etot, etot_floor);
   }
 }
@@ -243,6 +272,7 @@ __device__ __forceinline__ void load_prim(double *const U[S_N], int c,
 
 struct FluxConsts {
   double gm1, gamma, theta, dfloor, pfloor;
+  unsigned long long *dbg;  // debug counters or null
 };
 
 // Flux through the face between cell c and c + e_D (D fixed at compile
@@ -283,10 +313,14 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
       T lv = a1 + dvl * T(0.5);
       T rv = a2 - dvr * T(0.5);
       if (q == S_RHO) {
+        if (fc.dbg)
+          debug_count(fc.dbg, DBG_RECON_FLOOR, lv < (T)fc.dfloor || rv < (T)fc.dfloor);
         lv = dmax(lv, (T)fc.dfloor);
         rv = dmax(rv, (T)fc.dfloor);
       }
       if (q == S_E) {
+        if (fc.dbg)
+          debug_count(fc.dbg, DBG_RECON_FLOOR, lv < (T)fc.pfloor || rv < (T)fc.pfloor);
         lv = dmax(lv, (T)fc.pfloor);
         rv = dmax(rv, (T)fc.pfloor);
       }
@@ -389,6 +423,7 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   fc.theta = sp.theta;
   fc.dfloor = sp.density_floor;
   fc.pfloor = sp.pressure_floor;
+  fc.dbg = sp.dbg;
   const int c = (k * my + j) * mx + i;
   const double *ei = DE ? ((sp.stage == 0) ? b.ei : b.eit) : nullptr;
   double wc[S_N];
@@ -723,6 +758,7 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     // the update base is always the start-of-step state
     double old_rho = b.U[S_RHO][c];
     double new_rho = b.U[S_RHO][c] + dU[0];
+    if (sp.dbg) debug_count(sp.dbg, DBG_DENSITY_FLOOR, new_rho < sp.density_floor);
     new_rho = dmax(new_rho, sp.density_floor);
     double inv_new_rho = 1. / new_rho;
     double v[3];
@@ -734,8 +770,16 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     if (DE) eint = (b.ei[c] * old_rho + dUe) * inv_new_rho;
     // energy floor (and synchronisation)
     const double b2 = MHD ? (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]) : 0.;
+    bool floored = false;
     floor_and_sync<MHD, DE>(new_rho, v, b2, sp.gamma, sp.pressure_floor, sp.eta, sp.ggm1,
-                            etot, eint);
+                            etot, eint, sp.dbg ? &floored : nullptr);
+    if (sp.dbg) {
+      debug_count(sp.dbg, DBG_ENERGY_FLOOR, floored);
+      bool finite = isfinite(new_rho) && isfinite(v[0]) && isfinite(v[1]) &&
+                    isfinite(v[2]) && isfinite(etot) && isfinite(b2);
+      if (DE) finite = finite && isfinite(eint);
+      debug_count(sp.dbg, DBG_NONFINITE, !finite);
+    }
     Uo[S_RHO][c] = new_rho;
     Uo[S_VX][c] = v[0];
     Uo[S_VY][c] = v[1];
@@ -854,7 +898,10 @@ static __global__ void k_dt_init(unsigned long long *result) {
 
 // dt of the cycle on the device: courant * min, clipped so the cycle ends
 // at t_stop (control_stopping.cpp:83-86, as the host driver clips), copied
-// into dts[0..n) for the compute launches; the time advances by dt.
+// into dts[0..n) for the compute launches; the time advances by dt.  A
+// cycle that starts at or past t_stop is zero-length: the clipped cycle's
+// t + (t_stop - t) can round above t_stop, and the difference of the next
+// one would then be a (tiny) negative step.
 static __global__ void k_dt_finalize(const unsigned long long *bits,
                                      double courant, double *time,
                                      double t_stop, double *dts, int n) {
@@ -862,7 +909,7 @@ static __global__ void k_dt_finalize(const unsigned long long *bits,
   double dt = __longlong_as_double((long long)*bits) * courant;
   if (t < t_stop && t_stop < t + dt) dt = t_stop - t;
   const double rem = t_stop - t;
-  dt = dt < rem ? dt : rem;
+  dt = (rem > 0.0) ? (dt < rem ? dt : rem) : 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) dts[i] = dt;
   __syncthreads();
   if (threadIdx.x == 0) *time = t + dt;

@@ -41,15 +41,17 @@ struct KernelTimer {
   hipStream_t s;
   const char *name;
   hipEvent_t a = nullptr, b = nullptr;
-  KernelTimer(hipStream_t s_, const char *n) : s(s_), name(n) {
-    if (g_profiling) {
+  bool on;
+  KernelTimer(hipStream_t s_, const char *n, bool allowed = true)
+      : s(s_), name(n), on(allowed && g_profiling) {
+    if (on) {
       a = pool_event();
       b = pool_event();
       (void)hipEventRecord(a, s);
     }
   }
   ~KernelTimer() {
-    if (g_profiling) {
+    if (on) {
       (void)hipEventRecord(b, s);
       g_pending.push_back({name, a, b});
       if (g_pending.size() > 4096) resolve_pending();
@@ -108,6 +110,9 @@ struct enzoe_vlct_method {
   void *d_fdtab = nullptr, *d_fltab = nullptr, *d_fbtab = nullptr;
   size_t fdtab_bytes = 0, fltab_bytes = 0, fbtab_bytes = 0;
   std::vector<char> last_fdtab, last_fltab, last_fbtab;
+  // debug counters (DBG_* in vlct_kernels.hpp): device memory while the
+  // checks are on (enzoe_vlct_set_debug_checks), null otherwise
+  unsigned long long *d_dbg = nullptr;
   // transfer tables of enzoe_vlct_transfer_batch: two, so that the
   // restriction and prolongation batches of a cycle both stay cached
   void *d_trtab[2] = {nullptr, nullptr};
@@ -207,14 +212,14 @@ inline int tile_rows_for(int mx, int my) {
   return rows;
 }
 
-// Fills descriptors for blocks [b0, b1) using the scratch pool.
+
+// Fills descriptors for blocks [b0, b1) into out[0 .. b1-b0), their scratch
+// laid out from `base`.
 // accel: 3 acceleration pointers per block of the call (x, y, z), or null
 template <typename FluxT>
-void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
-               int b1, std::vector<BlockDesc<FluxT>> &out,
-               const double *const *accel = nullptr) {
-  out.clear();
-  char *base = (char *)m->scratch;
+void fill_desc_at(const enzoe_vlct_method *m, char *base, const enzoe_block *blocks,
+                  int b0, int b1, BlockDesc<FluxT> *out,
+                  const double *const *accel = nullptr) {
   for (int bi = b0; bi < b1; bi++) {
     const enzoe_block &b = blocks[bi];
     BlockDesc<FluxT> d;
@@ -274,8 +279,17 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
         base += ncc * sizeof(double);
       }
     }
-    out.push_back(d);
+    out[bi - b0] = d;
   }
+}
+
+// fill_desc_at over the method's scratch pool, into a vector
+template <typename FluxT>
+void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
+               int b1, std::vector<BlockDesc<FluxT>> &out,
+               const double *const *accel = nullptr) {
+  out.resize(b1 - b0);
+  fill_desc_at<FluxT>(m, (char *)m->scratch, blocks, b0, b1, out.data(), accel);
 }
 
 template <typename FluxT, int RECON, int RIEMANN, bool MHD, bool DE>
@@ -338,12 +352,95 @@ struct StoreArgs {
   double *const *d_fdtab;
 };
 
+// The launch sequence of compute_group over chunks of `chunk` blocks whose
+// descriptors are table[0 .. n) in device memory.  timers: bracket the
+// kernels with profiling events (when profiling is on).
+template <typename FluxT, bool MHD, bool DE>
+int launch_chunks(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int chunk,
+                  const BlockDesc<FluxT> *table, double dt, const double *dts,
+                  hipStream_t s, const StoreArgs *st, unsigned long long *cfl,
+                  bool timers) {
+  const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
+  for (int c0 = 0; c0 < n; c0 += chunk) {
+    const int c1 = std::min(n, c0 + chunk);
+    const int nb = c1 - c0;
+    const BlockDesc<FluxT> *d = table + c0;
+
+    int stale = 0;
+    for (int stage = 0; stage < m->nstages; stage++) {
+      const bool is_final = (stage + 1) == m->nstages;
+      const int recon = m->recon[stage];
+      StageParams sp;
+      sp.stage = stage;
+      sp.is_final = is_final ? 1 : 0;
+      sp.stale = stale;
+      sp.cur_stale = stale + immediate_staling(recon);
+      // EnzoMethodMHDVlct.cpp:462: dt/2 for the partial step
+      sp.dt = (!is_final) ? dt / 2. : dt;
+      sp.dts = dts ? dts + c0 : nullptr;
+      sp.dt_factor = (!is_final) ? 0.5 : 1.0;
+      sp.gamma = m->gamma;
+      sp.density_floor = m->dfloor;
+      sp.pressure_floor = m->pfloor;
+      sp.theta = m->theta;
+      sp.xcd_remap = launch_mask(m, mx, my, mz);
+      sp.eta = m->eta;
+      sp.ggm1 = sync_ggm1(m->gamma);
+      sp.cfl = is_final ? cfl : nullptr;
+      sp.dbg = m->d_dbg;
+      const int cs = sp.cur_stale;
+      {
+        KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm", timers);
+        dispatch_flux<FluxT, MHD, DE>(m, recon, d, nb, cube(mx, my, mz, cs, cs), sp, s);
+      }
+      if (MHD) {
+        {
+          KernelTimer kt(s, "k_edge", timers);
+          hipLaunchKernelGGL((k_edge<FluxT>), dim3(nblk8(cube(mx, my, mz, cs, cs), 256), nb),
+                             dim3(256), 0, s, d, sp);
+        }
+        {
+          KernelTimer kt(s, "k_faces", timers);
+          hipLaunchKernelGGL((k_faces<FluxT>),
+                             dim3(nblk8(cube(mx, my, mz, cs + 1, cs), 256), nb), dim3(256),
+                             0, s, d, sp);
+        }
+      }
+      {
+        KernelTimer kt(s, "k_cells", timers);
+        // with the fused CFL the final update's launch covers the block
+        const int lo = sp.cfl ? 0 : (is_final ? cs : cs + 1);
+        hipLaunchKernelGGL((k_cells<FluxT, MHD, DE>),
+                           dim3(nblk8(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
+                           s, d, sp);
+      }
+      stale += total_staling(recon);
+    }
+    if (st) {
+      // the chunk's flux scratch still holds the final stage's fluxes
+      long work = 0;
+      for (int bi = c0; bi < c1; bi++) {
+        const int g = blocks[bi].ghost;
+        FluxLayout L;
+        L.init(mx - 2 * g, my - 2 * g, mz - 2 * g);
+        work = std::max(work, L.per_field * st->sl.nf);
+      }
+      KernelTimer kt(s, "k_store_fluxes", timers);
+      hipLaunchKernelGGL((k_store_fluxes<FluxT>), dim3(nblk8(work, 256), nb),
+                         dim3(256), 0, s, d, st->d_fdtab + c0, st->sl, dt,
+                         dts ? dts + c0 : nullptr);
+    }
+  }
+  if (hipGetLastError() != hipSuccess) return ENZOE_ERR_HIP;
+  return 0;
+}
+
 // One VL+CT step (EnzoMethodMHDVlct::compute) on blocks of equal shape.
 // dts: per-block dt in device memory (n values) or null (all blocks use dt).
 // st: store the final stage's face fluxes (hydro only) or null.
 // DE: the dual-energy formalism (FP64 path only).
 // cfl (non-null): the final stage's k_cells also takes the CFL minimum of
-// its cells into *cfl and writes their pressure.
+// its cells and writes their pressure.
 template <typename FluxT, bool MHD, bool DE = false>
 int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   double dt, const double *dts, hipStream_t s,
@@ -388,77 +485,9 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
     }
   }
   if (m->dry) return 0;
-  for (int c0 = 0; c0 < n; c0 += chunk) {
-    const int c1 = std::min(n, c0 + chunk);
-    const int nb = c1 - c0;
-    const BlockDesc<FluxT> *d = (const BlockDesc<FluxT> *)m->d_desc + c0;
-
-    int stale = 0;
-    for (int stage = 0; stage < m->nstages; stage++) {
-      const bool is_final = (stage + 1) == m->nstages;
-      const int recon = m->recon[stage];
-      StageParams sp;
-      sp.stage = stage;
-      sp.is_final = is_final ? 1 : 0;
-      sp.stale = stale;
-      sp.cur_stale = stale + immediate_staling(recon);
-      // EnzoMethodMHDVlct.cpp:462: dt/2 for the partial step
-      sp.dt = (!is_final) ? dt / 2. : dt;
-      sp.dts = dts ? dts + c0 : nullptr;
-      sp.dt_factor = (!is_final) ? 0.5 : 1.0;
-      sp.gamma = m->gamma;
-      sp.density_floor = m->dfloor;
-      sp.pressure_floor = m->pfloor;
-      sp.theta = m->theta;
-      sp.xcd_remap = launch_mask(m, mx, my, mz);
-      sp.eta = m->eta;
-      sp.ggm1 = sync_ggm1(m->gamma);
-      sp.cfl = is_final ? cfl : nullptr;
-      const int cs = sp.cur_stale;
-      {
-        KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm");
-        dispatch_flux<FluxT, MHD, DE>(m, recon, d, nb, cube(mx, my, mz, cs, cs), sp, s);
-      }
-      if (MHD) {
-        {
-          KernelTimer kt(s, "k_edge");
-          hipLaunchKernelGGL((k_edge<FluxT>), dim3(nblk8(cube(mx, my, mz, cs, cs), 256), nb),
-                             dim3(256), 0, s, d, sp);
-        }
-        {
-          KernelTimer kt(s, "k_faces");
-          hipLaunchKernelGGL((k_faces<FluxT>),
-                             dim3(nblk8(cube(mx, my, mz, cs + 1, cs), 256), nb), dim3(256),
-                             0, s, d, sp);
-        }
-      }
-      {
-        KernelTimer kt(s, "k_cells");
-        // with the fused CFL the final update's launch covers the block
-        const int lo = sp.cfl ? 0 : (is_final ? cs : cs + 1);
-        hipLaunchKernelGGL((k_cells<FluxT, MHD, DE>),
-                           dim3(nblk8(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
-                           s, d, sp);
-      }
-      stale += total_staling(recon);
-    }
-    if (st) {
-      // the chunk's flux scratch still holds the final stage's fluxes
-      long work = 0;
-      for (int bi = c0; bi < c1; bi++) {
-        const int g = blocks[bi].ghost;
-        FluxLayout L;
-        L.init(mx - 2 * g, my - 2 * g, mz - 2 * g);
-        work = std::max(work, L.per_field * st->sl.nf);
-      }
-      KernelTimer kt(s, "k_store_fluxes");
-      hipLaunchKernelGGL((k_store_fluxes<FluxT>), dim3(nblk8(work, 256), nb),
-                         dim3(256), 0, s, d, st->d_fdtab + c0, st->sl, dt,
-                         dts ? dts + c0 : nullptr);
-    }
-  }
-  if (hipGetLastError() != hipSuccess) return ENZOE_ERR_HIP;
-  return 0;
+  return launch_chunks<FluxT, MHD, DE>(m, blocks, n, chunk,
+                                       (const BlockDesc<FluxT> *)m->d_desc, dt, dts, s, st,
+                                       cfl, true);
 }
 
 // FP32 flux path, compiled in vlct_mixed.hip

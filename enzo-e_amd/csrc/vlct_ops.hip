@@ -17,6 +17,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/enzoe_vlct.h"
 #include "vlct_kernels.hpp"
@@ -563,19 +564,47 @@ k_op_dt(KMap U, KArr P, double gamma, double dx, double dy, double dz,
   }
 }
 
-// one 8-byte reduction target per device, taken on first use
+// The 8-byte reduction target of enzoe_integrator_timestep: one per host
+// thread and device, so that callers on different threads (each with its
+// own method and stream, or all on the null stream) never initialise, reduce
+// into or read back each other's minimum.  A thread's targets go back to a
+// pool when the thread ends and are reused by later threads (device memory
+// is never freed: a thread may end after the runtime has shut down).
+struct DtTargetPool {
+  std::mutex mtx;
+  std::map<int, std::vector<unsigned long long *>> free_list;
+};
+DtTargetPool &dt_pool() {
+  static DtTargetPool *pool = new DtTargetPool;  // outlives every thread
+  return *pool;
+}
+struct ThreadDtTargets {
+  std::map<int, unsigned long long *> own;
+  ~ThreadDtTargets() {
+    DtTargetPool &p = dt_pool();
+    std::lock_guard<std::mutex> lock(p.mtx);
+    for (auto &kv : own) p.free_list[kv.first].push_back(kv.second);
+  }
+};
 unsigned long long *dt_target() {
-  static std::mutex mtx;
-  static std::map<int, unsigned long long *> targets;
+  thread_local ThreadDtTargets mine;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  std::lock_guard<std::mutex> lock(mtx);
-  auto it = targets.find(dev);
-  if (it != targets.end()) return it->second;
+  auto it = mine.own.find(dev);
+  if (it != mine.own.end()) return it->second;
   unsigned long long *p = nullptr;
-  if (hipMalloc((void **)&p, sizeof(unsigned long long)) != hipSuccess)
+  {
+    DtTargetPool &pool = dt_pool();
+    std::lock_guard<std::mutex> lock(pool.mtx);
+    std::vector<unsigned long long *> &fl = pool.free_list[dev];
+    if (!fl.empty()) {
+      p = fl.back();
+      fl.pop_back();
+    }
+  }
+  if (!p && hipMalloc((void **)&p, sizeof(unsigned long long)) != hipSuccess)
     return nullptr;
-  targets[dev] = p;
+  mine.own[dev] = p;
   return p;
 }
 

@@ -61,6 +61,15 @@ class Block(ctypes.Structure):
                 ("internal_energy", ctypes.c_void_p)]
 
 
+class Diagnostics(ctypes.Structure):
+    _fields_ = [("mass", ctypes.c_double), ("momentum", ctypes.c_double * 3),
+                ("energy", ctypes.c_double), ("min_density", ctypes.c_double),
+                ("min_pressure", ctypes.c_double), ("max_abs_divb", ctypes.c_double),
+                ("max_abs_bface", ctypes.c_double), ("nonfinite", ctypes.c_longlong)]
+
+
+DEBUG_COUNTS = ("nonfinite", "density_floor", "energy_floor", "reconstruction_floor")
+
 _lib = None
 _hip = None
 
@@ -143,6 +152,10 @@ def lib():
         L.enzoe_grid_refresh.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), vp]
+        L.enzoe_vlct_set_debug_checks.argtypes = [vp, ctypes.c_int]
+        L.enzoe_vlct_debug_counts.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.enzoe_vlct_diagnostics.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int, vp,
+                                             ctypes.POINTER(Diagnostics)]
         L.enzoe_set_profiling.argtypes = [ctypes.c_int]
         L.enzoe_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp,
                                            ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -315,6 +328,31 @@ class VlctMethod:
                                                 dt, stream, ctypes.byref(nxt)),
               "enzoe_vlct_compute_timestep")
         return nxt.value
+
+    def set_debug_checks(self, on=True):
+        """Counts non-finite updates and applied floors on the device (the
+        reference's CHECK_FINITE_ELEMENTS / RAISE_FLOOR_ERROR builds)."""
+        check(lib().enzoe_vlct_set_debug_checks(self.handle, 1 if on else 0),
+              "enzoe_vlct_set_debug_checks")
+
+    def debug_counts(self, stream=None):
+        """The counters since the last call, as a dict; clears them."""
+        c = (ctypes.c_ulonglong * 4)()
+        check(lib().enzoe_vlct_debug_counts(self.handle, stream, c),
+              "enzoe_vlct_debug_counts")
+        return dict(zip(DEBUG_COUNTS, [int(x) for x in c]))
+
+    def diagnostics(self, blocks, stream=None):
+        """Device summary of the blocks: conserved totals over the active
+        cells, minimum density and pressure, max |div B| dx, max |face B|,
+        and the number of non-finite values (ghosts included)."""
+        d = Diagnostics()
+        check(lib().enzoe_vlct_diagnostics(self.handle, self._blocks(blocks), len(blocks),
+                                           stream, ctypes.byref(d)), "enzoe_vlct_diagnostics")
+        return {"mass": d.mass, "momentum": tuple(d.momentum), "energy": d.energy,
+                "min_density": d.min_density, "min_pressure": d.min_pressure,
+                "max_abs_divb": d.max_abs_divb, "max_abs_bface": d.max_abs_bface,
+                "nonfinite": int(d.nonfinite)}
 
     def autotune(self, blocks, dt, stream=None):
         """Picks the launch order of the stage kernels for this block shape

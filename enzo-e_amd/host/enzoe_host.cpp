@@ -1335,6 +1335,10 @@ EnzoMethodMHDVlct::EnzoMethodMHDVlct(ParameterGroup p, bool store_fluxes_for_cor
   ENZOE_ASSERT("EnzoMethodMHDVlct", p.full_name("device_fused") + " must be true or false",
                fused == "true" || fused == "false");
   fused_ = (fused == "true");
+  const std::string dbg = lowered(p.value_string("debug_checks", "false"));
+  ENZOE_ASSERT("EnzoMethodMHDVlct", p.full_name("debug_checks") + " must be true or false",
+               dbg == "true" || dbg == "false");
+  debug_checks_ = (dbg == "true");
   const std::string prec = lowered(p.value_string("precision", "double"));
   if (prec == "double") {
     precision_ = ENZOE_PREC_DOUBLE;
@@ -1423,7 +1427,31 @@ EnzoVlctScratchSpace *EnzoMethodMHDVlct::get_scratch_ptr_(
   return scratch_space_;
 }
 
+// The debug counters of the fused step since the last check: a non-finite
+// value ends the run as CelloView's CHECK_IF_FINITE (view_CelloView.hpp:
+// 209-215), an applied floor as enzo_utils::apply_floor under
+// RAISE_FLOOR_ERROR (utils.hpp:105-108).
+void EnzoMethodMHDVlct::debug_check_(const char *where) {
+  unsigned long long c[ENZOE_DEBUG_COUNTS];
+  check_rc(enzoe_vlct_debug_counts(device_, g_stream, c), where);
+  if (c[ENZOE_DEBUG_NONFINITE])
+    ENZOE_ERROR("CelloView", "Non-Finite Value (" + std::to_string(c[ENZOE_DEBUG_NONFINITE]) +
+                                 " updated cell values)");
+  const unsigned long long floors = c[ENZOE_DEBUG_DENSITY_FLOOR] + c[ENZOE_DEBUG_ENERGY_FLOOR] +
+                                    c[ENZOE_DEBUG_RECONSTRUCTION_FLOOR];
+  if (floors)
+    ENZOE_ERROR("enzo_utils::apply_floor",
+                "DEBUG-MODE ERROR: applying a floor. (density " +
+                    std::to_string(c[ENZOE_DEBUG_DENSITY_FLOOR]) + ", energy " +
+                    std::to_string(c[ENZOE_DEBUG_ENERGY_FLOOR]) + ", reconstruction " +
+                    std::to_string(c[ENZOE_DEBUG_RECONSTRUCTION_FLOOR]) + ")");
+}
+
 void EnzoMethodMHDVlct::compute(Block *block) throw() {
+  if (debug_checks_ && fused_ && !debug_on_) {
+    check_rc(enzoe_vlct_set_debug_checks(device_, 1), "EnzoMethodMHDVlct::compute");
+    debug_on_ = true;
+  }
   if (block->is_leaf()) {
     // EnzoMethodMHDVlct.cpp:124-133: ghost depth covers the staling
     int min_gdepth_req = 0;
@@ -1455,6 +1483,7 @@ void EnzoMethodMHDVlct::compute(Block *block) throw() {
         check_rc(enzoe_vlct_compute(device_, &b, 1, block->dt(), g_stream),
                  "EnzoMethodMHDVlct::compute");
       }
+      if (debug_on_) debug_check_("EnzoMethodMHDVlct::compute");
     } else {
       compute_staged_(block);
     }
@@ -1488,6 +1517,10 @@ void EnzoMethodMHDVlct::compute_blocks(const std::vector<Block *> &blocks) throw
     for (Block *b : blocks) compute(b);
     return;
   }
+  if (debug_checks_ && !debug_on_) {
+    check_rc(enzoe_vlct_set_debug_checks(device_, 1), "EnzoMethodMHDVlct::compute_blocks");
+    debug_on_ = true;
+  }
   std::vector<enzoe_block> desc;
   std::vector<double *> fds;
   desc.reserve(blocks.size());
@@ -1514,6 +1547,7 @@ void EnzoMethodMHDVlct::compute_blocks(const std::vector<Block *> &blocks) throw
                                 g_stream),
              "EnzoMethodMHDVlct::compute_blocks");
   }
+  if (debug_on_) debug_check_("EnzoMethodMHDVlct::compute_blocks");
   for (Block *b : blocks) b->compute_done();
 }
 

@@ -703,6 +703,13 @@ class EnzoMethodMHDVlct : public Method {
   bool fused_ = true;
   int precision_ = ENZOE_PREC_DOUBLE;
   enzoe_vlct_t device_ = nullptr;
+  // debug_checks = true: the fused step counts non-finite updates and
+  // applied floors on the device; a nonzero count ends the run as the
+  // reference's debug builds do (switched on at the first compute: the
+  // constructor makes no device call)
+  bool debug_checks_ = false;
+  bool debug_on_ = false;
+  void debug_check_(const char *where);
 };
 
 // -------------------------------------------------------- initial values

@@ -299,6 +299,36 @@ def turbulence_block(spec, vrms_norm, ib, nb=(8, 8, 8), n=64, g=3,
     return blk, (dx, dx, dx)
 
 
+def turbulence_domain_velocity(spec, vrms_norm, N):
+    """The velocity of `spec` (scaled by vrms_norm) at the N^3 cell centres
+    of the unit cube, evaluated once for the whole domain (three
+    matrix products over the dense mode cube instead of one per block).
+    Returns an array (3, N, N, N) indexed (component, z, y, x)."""
+    K, C = spec
+    x = (np.arange(N) + 0.5) / N
+    return np.stack([vrms_norm * _eval_modes(K, C[:, a], x, x, x) for a in range(3)])
+
+
+def turbulence_block_from_domain(vel, ib, n, g=3, gamma=5.0 / 3.0, beta=0.1):
+    """Block ib of the decaying-turbulence workload cut from a whole-domain
+    velocity (turbulence_domain_velocity), ghost zones included (periodic
+    wrap): the state of turbulence_block up to the summation order of the
+    modes."""
+    N = vel.shape[1]
+    m = n + 2 * g
+    idx = [(ib[a] * n + np.arange(m) - g) % N for a in range(3)]
+    v = [vel[c][np.ix_(idx[2], idx[1], idx[0])] for c in range(3)]
+    rho = np.ones((m, m, m))
+    p0 = 1.0 / gamma
+    p = np.full((m, m, m), p0)
+    bx0 = np.sqrt(2.0 * p0 / beta)
+    bfi = (np.full((m, m, m + 1), bx0), np.zeros((m, m + 1, m)),
+           np.zeros((m + 1, m, m)))
+    blk = _empty_block((n, n, n), g)
+    _fill_state(blk, rho, v, p, bfi, gamma)
+    return blk, (1.0 / N,) * 3
+
+
 def turbulence_vrms_norm(spec, mach=5.0, samples=32):
     """Factor that brings the rms speed of `spec` to `mach` (c_s = 1),
     measured on a samples^3 grid of the unit cube."""

@@ -600,6 +600,45 @@ typedef struct {
 int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
                               int n, enzoe_stream_t stream);
 
+/* ---- debug checks and diagnostics ----------------------------------------
+ * The reference's debug builds abort where a field value is not finite
+ * (CHECK_FINITE_ELEMENTS, src/Cello/view_CelloView.hpp:207-217) or where a
+ * floor has to be applied (RAISE_FLOOR_ERROR in enzo_utils::apply_floor,
+ * src/Enzo/utils/utils.hpp:102-114).  With debug checks on, the method's
+ * kernels count those events instead (one device atomic per wave and event;
+ * off, they cost one uniform branch): non-finite updated cell values, the
+ * density floor of the update (EnzoIntegrationQuanUpdate.cpp:219), the
+ * energy floor (EnzoPhysicsFluidProps.cpp:278-285) and the reconstruction's
+ * density/pressure floors (EnzoReconstructorPLM.hpp:219-220).  The C++ layer
+ * turns a nonzero count into the reference's ERROR.                         */
+#define ENZOE_DEBUG_NONFINITE 0
+#define ENZOE_DEBUG_DENSITY_FLOOR 1
+#define ENZOE_DEBUG_ENERGY_FLOOR 2
+#define ENZOE_DEBUG_RECONSTRUCTION_FLOOR 3
+#define ENZOE_DEBUG_COUNTS 4
+int enzoe_vlct_set_debug_checks(enzoe_vlct_t m, int on);
+/* Copies the counters accumulated since the last call (or since the checks
+ * were switched on) into counts and clears them; synchronises the stream.
+ * ENZOE_ERR_STATE while the checks are off.                                 */
+int enzoe_vlct_debug_counts(enzoe_vlct_t m, enzoe_stream_t stream,
+                            unsigned long long counts[ENZOE_DEBUG_COUNTS]);
+
+/* Device-side summary of n blocks (no counterpart in the reference, whose
+ * diagnostics read fields on the host): over the active cells the sums of
+ * density, momentum and total energy density (rho e_tot), the smallest
+ * density and pressure, the largest |div B| of the face field times the
+ * smallest cell width and the largest |face B|; over every cell, ghosts
+ * included, the number of non-finite field values.  Sums are accumulated
+ * with atomics (their last bits depend on the order).  Synchronises.       */
+typedef struct {
+  double mass, momentum[3], energy;
+  double min_density, min_pressure;
+  double max_abs_divb, max_abs_bface;
+  long long nonfinite;
+} enzoe_diagnostics;
+int enzoe_vlct_diagnostics(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                           enzoe_stream_t stream, enzoe_diagnostics *out);
+
 /* Per-kernel average duration (ms) of the launches made while profiling was
  * on (enzoe_set_profiling(1)), measured with HIP events recorded on the
  * launch stream around each kernel; the events are read when the report is

@@ -511,3 +511,19 @@ def test_dual_energy_single_operations_validate_without_a_device():
                                                          5 / 3, 0, None) == ERR_ARG
     assert L.enzoe_primitive_from_integration_dual_energy(plain, plain, 1, 5 / 3, 0,
                                                           None) == ERR_ARG
+
+
+ERR_STATE = 5
+
+
+def test_debug_counts_need_the_checks_on():
+    L = ev.lib()
+    L.enzoe_vlct_debug_counts.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.POINTER(ctypes.c_ulonglong)]
+    rc, h = _create()
+    assert rc == 0
+    try:
+        c = (ctypes.c_ulonglong * 4)()
+        assert L.enzoe_vlct_debug_counts(h, None, c) == ERR_STATE
+    finally:
+        L.enzoe_vlct_destroy(h)

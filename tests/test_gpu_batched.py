@@ -188,6 +188,37 @@ def test_graph_replays_match_eager_cycles():
         np.testing.assert_array_equal(a[k], b[k])
 
 
+def test_graph_clipped_cycle_that_rounds_past_t_stop():
+    """A stop time whose clipped cycle ends, in floating point, just above
+    t_stop: t0 + (t_stop - t0) > t_stop for this pair (found by search;
+    t0 < t_stop / 2, so the difference is not exact).  The first cycle is
+    clipped (the CFL dt of the block, ~0.01, exceeds t_stop - t0), the
+    device time then exceeds t_stop, and the graph's remaining cycles must be
+    zero-length (dt 0, not the negative t_stop - t) so that the fields equal
+    the single clipped eager cycle bit for bit."""
+    ev = _ev()
+    from _states import random_periodic_grid
+    t0, t_stop = 0.0006548562668191901, 0.0049961035292957505
+    assert t0 + (t_stop - t0) > t_stop
+    grid = random_periodic_grid(mhd=True, seed=1, n=(12, 10, 8))
+    method = method_for(grid.cfg)
+    eager = device_copy(grid, method)
+    dev = device_copy(grid, method)
+    dt_cfl = method.timestep(eager.blocks)
+    assert dt_cfl > t_stop - t0
+    n_cyc, log = eager.evolve(t0, t_stop)
+    assert n_cyc == 1 and log[0] == t_stop - t0
+    g = dev.graph(t0, t_stop, 3)
+    for rep in range(2):
+        g.launch()
+        ev.synchronize()
+        assert g.time() == t0 + (t_stop - t0)
+        a, b = dev.blocks[0].download(), eager.blocks[0].download()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg="replay %d %s" % (rep, k))
+    g.destroy()
+
+
 def test_autotuned_launch_order_keeps_results():
     """The autotuner only reorders workgroups: after it has chosen a mask
     for the shape (on a copy, leaving the blocks alone), cycles still equal

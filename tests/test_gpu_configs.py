@@ -1,0 +1,437 @@
+"""The five benchmark workloads on the GPU, each against the oracle at a size
+the oracle runs in seconds and, at full size, through properties that do
+not depend on size.
+
+FP64 bar: <= 1e-12 relative per cell against the oracle, dt equal every
+cycle.  Mixed-precision bar (FP32 reconstruction and Riemann fluxes, FP64
+update, CT and dt): relative L1 error against the FP64 oracle <= 1e-5 per
+field, defined as sum|a - b| / sum|b| over the active cells for the scalar
+fields and, for a vector component (velocity, B), with the denominator the
+sum of the vector's magnitude |b_vec| (a component can be identically zero
+in FP64, e.g. a field that starts at zero, where its own sum is no scale).
+Full-size properties: mass, momentum and total energy conserved to
+roundoff over periodic grids (no floor applied: the device debug counters
+say so), the discrete div B of the face field at roundoff, positive density
+and pressure, no non-finite value.
+"""
+import ctypes
+import gc
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from test_gpu_parity import compare_grids, device_copy, method_for
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+MIXED_L1 = 1e-5
+CONS_REL = 1e-12      # relative drift of conserved totals over a run
+DIVB_REL = 1e-13      # max |div B| dx / max |B_face|
+
+CC = ["density", "velocity_x", "velocity_y", "velocity_z", "total_energy",
+      "bfield_x", "bfield_y", "bfield_z"]
+
+
+def _ev():
+    import enzoe_vlct as ev
+    return ev
+
+
+def _oracle_grid(cfg, nblocks, n, arrays_per_block, cw, bc=("periodic",) * 3):
+    """A periodic oracle grid of blocks with cell widths cw holding the given
+    block arrays (ghost zones refreshed)."""
+    ext = tuple(nblocks[a] * n[a] * cw[a] for a in range(3))
+    g = po.Grid(cfg, nblocks, n, (0.0, 0.0, 0.0), ext, bc=bc)
+    for i in range(nblocks[0] * nblocks[1] * nblocks[2]):
+        b = g.block(i)
+        for a in range(3):
+            b.cw[a] = cw[a]  # the workload's widths, bit for bit
+    for i, arrs in enumerate(arrays_per_block):
+        view = g.block_arrays(i)
+        for k, v in arrs.items():
+            if k in view:
+                view[k][...] = v
+    g.refresh()
+    return g
+
+
+def relative_l1(got_blocks, ref_blocks, names, g=3):
+    """Per-field relative L1 as the module docstring defines it."""
+    num = dict.fromkeys(names, 0.0)
+    den = dict.fromkeys(names, 0.0)
+    mag = {"velocity": 0.0, "bfield": 0.0}
+    s = (slice(g, -g),) * 3
+    for a, b in zip(got_blocks, ref_blocks):
+        for nm in names:
+            num[nm] += float(np.abs(a[nm][s] - b[nm][s]).sum())
+            den[nm] += float(np.abs(b[nm][s]).sum())
+        for vec in mag:
+            comps = [vec + "_" + c for c in "xyz"]
+            if all(c in b for c in comps):
+                mag[vec] += float(np.sqrt(sum(b[c][s] ** 2 for c in comps)).sum())
+    out = {}
+    for nm in names:
+        vec = nm.rsplit("_", 1)[0]
+        d = mag[vec] if vec in mag and mag[vec] > 0 else den[nm]
+        out[nm] = num[nm] / d
+    return out
+
+
+def _props(method, blocks, first=None):
+    """Diagnostics of the blocks; with `first`, also the drift of the
+    conserved totals since then, relative to the first's magnitudes."""
+    d = method.diagnostics(blocks)
+    if first is not None:
+        mass0 = abs(first["mass"])
+        e0 = abs(first["energy"])
+        assert abs(d["mass"] - first["mass"]) <= CONS_REL * mass0, (d["mass"], first["mass"])
+        assert abs(d["energy"] - first["energy"]) <= CONS_REL * e0, (d["energy"], first["energy"])
+        for a in range(3):
+            # momentum: relative to the mass (a velocity scale of one)
+            assert abs(d["momentum"][a] - first["momentum"][a]) <= CONS_REL * mass0, a
+    assert d["nonfinite"] == 0, d
+    assert d["min_density"] > 0.0, d
+    assert d["min_pressure"] > 0.0, d
+    if d["max_abs_bface"] > 0.0:
+        assert d["max_abs_divb"] <= DIVB_REL * d["max_abs_bface"], d
+    return d
+
+
+# ---------------------------------------------------------------- config 1
+
+@pytest.mark.parametrize("recon,theta", [("plm", 2.0), ("plm_athena", 1.5)],
+                         ids=["plm-theta2", "plm_athena"])
+def test_config1_fast_wave_32_one_period(recon, theta):
+    """One 32^3 block, the (1,1,1) fast eigenmode (amplitude 1e-6, B = (1,
+    sqrt 2, 0.5) in the wave frame), PLM + HLLD, CFL 0.4, evolved for one
+    full wave period; both readings of "van Leer" (the reference's "plm"
+    with theta = 2 and the harmonic-mean "plm_athena").  Every cycle's dt
+    and the final state equal the oracle's."""
+    import problems
+    ev = _ev()
+    n = 32
+    print("config 1:", recon, theta, flush=True)
+    cfg = po.make_config(riemann="hlld", recon=("nn", recon), theta=theta, courant=0.4)
+    blk, cw, cf, period = problems.fast_wave_111(n)
+    grid = po.Grid(cfg, (1, 1, 1), (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    view = grid.block_arrays(0)
+    for k, v in blk.items():
+        view[k][...] = v
+    dg = device_copy(grid, method_for(cfg))
+    t, cycles = 0.0, 0
+    while t < period:
+        dt_o = grid.cycle(t, period, nthreads=THREADS)
+        assert dg.cycle(t, period) == dt_o, cycles
+        t += dt_o
+        cycles += 1
+    ev.synchronize()
+    assert t >= period and cycles > 40
+    compare_grids(dg, grid)
+    s = (slice(3, -3),) * 3
+    l1 = float(np.abs(view["density"][s] - blk["density"][s]).mean())
+    print("cycles", cycles, "L1(density) after one period", l1)
+
+
+# ---------------------------------------------------------------- config 2
+
+def _orszag_tang_oracle(n):
+    import problems
+    cfg = po.make_config(riemann="hlld", recon=("nn", "plm"), theta=2.0, courant=0.4)
+    blk, cw = problems.orszag_tang(n)
+    grid = po.Grid(cfg, (1, 1, 1), (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    view = grid.block_arrays(0)
+    for k, v in blk.items():
+        view[k][...] = v
+    grid.refresh()
+    return cfg, grid
+
+
+def test_config2_orszag_tang_20_steps_as_benchmarked():
+    """The Orszag-Tang vortex (PLM theta = 2, HLLD, CFL 0.4) for the
+    benchmark's 20 steps, driven as the benchmark drives it (refresh, then
+    compute with the CFL dt of the next cycle fused into the final update):
+    every step's dt and the final state equal the oracle's cycles, at 128^3
+    (one eighth of the benchmarked cells)."""
+    ev = _ev()
+    n = 128
+    cfg, grid = _orszag_tang_oracle(n)
+    dg = device_copy(grid, method_for(cfg))
+    m = dg.method
+    dt = m.timestep(dg.blocks)
+    t = 0.0
+    for step in range(20):
+        dt_o = grid.cycle(t, 1e9, nthreads=THREADS)
+        assert dt == dt_o, step
+        dg.refresh()
+        dt = m.compute_timestep(dg.blocks, dt)
+        t += dt_o
+    ev.synchronize()
+    assert dt == po.lib().o_vlct_timestep(ctypes.byref(cfg), ctypes.byref(grid.block(0)))
+    compare_grids(dg, grid)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_config2_orszag_tang_256_full_size_properties(precision):
+    """The benchmark workload at full size (one 256^3 block) for 20 steps:
+    conserved totals, div B at roundoff, positive density and pressure, no
+    non-finite value and no floor applied (debug counters)."""
+    import problems
+    ev = _ev()
+    n = 256
+    blk, cw = problems.orszag_tang(n)
+    m = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm", theta=2.0, courant=0.4,
+                      precision=precision)
+    m.set_debug_checks(True)
+    dg = ev.DeviceGrid(m, (1, 1, 1), (n, n, n), [cw])
+    dg.upload_from([blk])
+    dg.refresh()
+    first = _props(m, dg.blocks)
+    dt = m.timestep(dg.blocks)
+    for _ in range(20):
+        dg.refresh()
+        dt = m.compute_timestep(dg.blocks, dt)
+    ev.synchronize()
+    counts = m.debug_counts()
+    print("debug counts", counts)
+    assert counts == dict.fromkeys(counts, 0), counts
+    last = _props(m, dg.blocks, first)
+    print("diagnostics", last)
+    del dg, m
+    gc.collect()
+
+
+# ---------------------------------------------------------------- config 3
+
+def test_config3_batched_amr_32_full_size_per_block_oracle():
+    """512 / 1536 / 3000 blocks of 32^3 on three levels with 2:1 sub-cycling
+    (launches of 5048, 3000, 4536, 3000 blocks), one coarse step.  The
+    per-level CFL dt equals the oracle's over every distinct block state;
+    16 blocks per level (each of a different seeded state, from the first,
+    middle and last part of the launch) equal the oracle's step of that
+    block with its level's dt; blocks holding the same state at other
+    positions of the batch equal each other bit for bit."""
+    import problems
+    import workloads
+    ev = _ev()
+    n, counts, distinct, seed = 32, (512, 1536, 3000), 64, 2024
+    cfg = po.make_config(riemann="hlld", recon=("nn", "plm"), theta=2.0, courant=0.4)
+    method = method_for(cfg)
+    lv = workloads.SubcycledLevels(method, counts=counts, n=n, distinct=distinct, seed=seed)
+    assert lv.launch_sizes() == [5048, 3000, 4536, 3000]
+    states, cfl = [], []
+    for lev, cnt in enumerate(counts):
+        width = 1.0 / (n * 2 ** lev)
+        st = [problems.random_mhd_block(seed + 1000 * lev + s, n=n, width=width)[0]
+              for s in range(distinct)]
+        states.append(st)
+        blocks = [_oracle_grid(cfg, (1, 1, 1), (n, n, n), [a], (width,) * 3) for a in st]
+        cfl.append(min(po.lib().o_vlct_timestep(ctypes.byref(cfg), ctypes.byref(b.block(0)))
+                       for b in blocks))
+        del blocks
+    dt0_o = min(2 ** lev * d for lev, d in enumerate(cfl))
+    dt0 = lv.coarse_step()
+    ev.synchronize()
+    assert dt0 == dt0_o
+    worst = 0.0
+    for lev, cnt in enumerate(counts):
+        width = 1.0 / (n * 2 ** lev)
+        picks = sorted(set(list(range(0, 6)) + list(range(cnt // 2, cnt // 2 + 5)) +
+                           list(range(cnt - 5, cnt))))
+        for i in picks:
+            s = i % distinct
+            o = _oracle_grid(cfg, (1, 1, 1), (n, n, n), [states[lev][s]], (width,) * 3)
+            for _ in range(2 ** lev):
+                assert po.lib().o_vlct_compute(ctypes.byref(cfg), ctypes.byref(o.block(0)),
+                                               dt0_o / 2 ** lev) == 0
+            got = lv.levels[lev][i].download()
+            ref = o.block_arrays(0)
+            for name, a in got.items():
+                if name == "pressure":
+                    continue
+                b = ref[name]
+                denom = np.maximum(np.abs(b), 1e-12 * max(1.0, float(np.abs(b).max())))
+                worst = max(worst, float(np.max(np.abs(a - b) / denom)))
+            twin = i + distinct if i + distinct < cnt else i - distinct
+            other = lv.levels[lev][twin].download()
+            for name in got:
+                if name != "pressure":
+                    assert np.array_equal(got[name], other[name]), (lev, i, twin, name)
+    print("max rel diff over the sampled blocks", worst)
+    assert worst <= 1e-12, worst
+    del lv, method
+    gc.collect()
+
+
+# ---------------------------------------------------------------- config 4
+
+def _turbulence_builder(nb, n, mach=5.0):
+    """builder(ib) -> (block arrays, cell widths) of the Mach-5 workload."""
+    import problems
+    spec = problems.turbulence_spectrum()
+    norm = problems.turbulence_vrms_norm(spec, mach=mach)
+    vel = problems.turbulence_domain_velocity(spec, norm, nb[0] * n)
+    return lambda ib: problems.turbulence_block_from_domain(vel, ib, n)
+
+
+def _turbulence_arrays(nb, n):
+    build = _turbulence_builder(nb, n)
+    order = [(bx, by, bz) for bz in range(nb[2]) for by in range(nb[1]) for bx in range(nb[0])]
+    out = [build(ib) for ib in order]
+    return [a for a, _ in out], out[0][1]
+
+
+def test_config4_turbulence_fp64_and_mixed_vs_oracle():
+    """Decaying Mach-5 turbulence (k^-2 solenoidal velocity on k = 1..8,
+    rho = 1, beta = 0.1 along x) on 2x2x2 periodic blocks of 32^3 for the
+    config's 50 steps: FP64 equals the oracle every cycle; the mixed step
+    (FP32 HLLD at Mach 5, debug counters on: no non-finite value, no floor)
+    stays within the relative L1 bar of the FP64 oracle per field."""
+    ev = _ev()
+    nb, n, steps = (2, 2, 2), 32, 50
+    arrays, cw = _turbulence_arrays(nb, n)
+    cfg = po.make_config(riemann="hlld", recon=("nn", "plm"), theta=2.0, courant=0.4)
+    grid = _oracle_grid(cfg, nb, (n, n, n), arrays, cw)
+    d64 = device_copy(grid, method_for(cfg))
+    m32 = method_for(cfg, precision=1)
+    m32.set_debug_checks(True)
+    d32 = device_copy(grid, m32)
+    t = 0.0
+    for step in range(steps):
+        dt_o = grid.cycle(t, 1e9, nthreads=THREADS)
+        assert d64.cycle(t, 1e9) == dt_o, step
+        # the mixed grid takes the oracle's dt, so the states stay comparable
+        d32.refresh()
+        m32.compute(d32.blocks, dt_o)
+        t += dt_o
+    ev.synchronize()
+    compare_grids(d64, grid)
+    counts = m32.debug_counts()
+    print("mixed debug counts", counts)
+    assert counts["nonfinite"] == 0 and counts["density_floor"] == 0, counts
+    assert counts["energy_floor"] == 0 and counts["reconstruction_floor"] == 0, counts
+    ref = [grid.block_arrays(i) for i in range(8)]
+    l1 = relative_l1(d32.download(), ref, CC)
+    print("mixed relative L1 vs the FP64 oracle after %d steps:" % steps, l1)
+    assert max(l1.values()) <= MIXED_L1, l1
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_config4_turbulence_full_size_properties(precision):
+    """512 blocks of 64^3 (the 512^3 domain) for 50 steps: conserved totals,
+    div B at roundoff, positive density and pressure, no non-finite value,
+    no floor applied."""
+    import workloads
+    ev = _ev()
+    nb, n, steps = (8, 8, 8), 64, 50
+    print("config 4 full size, precision", precision, flush=True)
+    m = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm", theta=2.0, courant=0.4,
+                      precision=precision)
+    m.set_debug_checks(True)
+    grid = workloads.uniform_grid(m, nb, n, _turbulence_builder(nb, n))
+    first = _props(m, grid.blocks)
+    dt = m.timestep(grid.blocks)
+    for _ in range(steps):
+        grid.refresh()
+        dt = m.compute_timestep(grid.blocks, dt)
+    ev.synchronize()
+    counts = m.debug_counts()
+    print("debug counts", counts)
+    assert counts == dict.fromkeys(counts, 0), counts
+    print("diagnostics", _props(m, grid.blocks, first))
+    del grid, m
+    gc.collect()
+
+
+# ---------------------------------------------------------------- config 5
+
+def _kh_arrays(nb, n, distinct_rows=False):
+    import problems
+    order = [(bx, by, bz) for bz in range(nb[2]) for by in range(nb[1]) for bx in range(nb[0])]
+    if not distinct_rows:
+        out = [problems.kelvin_helmholtz_block(ib, nb=nb, n=n) for ib in order]
+        return [a for a, _ in out], out[0][1]
+    # one seeded state per row of blocks along y, repeated along x and z
+    rows = {by: problems.kelvin_helmholtz_block((0, by, 0), nb=nb, n=n)
+            for by in range(nb[1])}
+    return [rows[ib[1]][0] for ib in order], rows[0][1]
+
+
+HD = ["density", "velocity_x", "velocity_y", "velocity_z", "total_energy"]
+
+
+def test_config5_kelvin_helmholtz_128_fp64_and_mixed_vs_oracle():
+    """The hydro Kelvin-Helmholtz slab (shear +-0.5, density ratio 2, p =
+    2.5, gamma = 1.4, 1e-2 random v_y; HLLC, PLM theta = 2) on one periodic
+    block of the config's 128^3 shape for 20 steps: FP64 equals the oracle
+    every cycle; the mixed step (FP32 HLLC) stays within the relative L1
+    bar per field, with no non-finite value and no floor."""
+    ev = _ev()
+    nb, n, steps = (1, 1, 1), 128, 20
+    arrays, cw = _kh_arrays(nb, n)
+    cfg = po.make_config(riemann="hllc", mhd=False, recon=("nn", "plm"), theta=2.0,
+                         courant=0.4, gamma=1.4)
+    grid = _oracle_grid(cfg, nb, (n, n, n), arrays, cw)
+    d64 = device_copy(grid, method_for(cfg))
+    m32 = method_for(cfg, precision=1)
+    m32.set_debug_checks(True)
+    d32 = device_copy(grid, m32)
+    t = 0.0
+    for step in range(steps):
+        dt_o = grid.cycle(t, 1e9, nthreads=THREADS)
+        assert d64.cycle(t, 1e9) == dt_o, step
+        d32.refresh()
+        m32.compute(d32.blocks, dt_o)
+        t += dt_o
+    ev.synchronize()
+    compare_grids(d64, grid)
+    counts = m32.debug_counts()
+    print("mixed debug counts", counts)
+    assert counts == dict.fromkeys(counts, 0), counts
+    l1 = relative_l1(d32.download(), [grid.block_arrays(0)], HD)
+    print("mixed relative L1 vs the FP64 oracle after %d steps:" % steps, l1)
+    assert max(l1.values()) <= MIXED_L1, l1
+
+
+def test_config5_kelvin_helmholtz_full_size_properties():
+    """512 blocks of 128^3 (the 1024^3 domain; the seeded v_y noise repeats
+    along x and z from block to block) for 10 steps in FP64 and in mixed
+    precision side by side: conserved totals, positive density and
+    pressure, no non-finite value, no floor; the mixed state's relative L1
+    against the FP64 one on a sample of blocks."""
+    import workloads
+    ev = _ev()
+    nb, n, steps = (8, 8, 8), 128, 10
+    arrays, cw = _kh_arrays(nb, n, distinct_rows=True)
+    grids = {}
+    for prec in (0, 1):
+        m = ev.VlctMethod(mhd=False, riemann="hllc", reconstruct="plm", theta=2.0,
+                          courant=0.4, gamma=1.4, precision=prec)
+        m.set_debug_checks(True)
+        grids[prec] = workloads.uniform_grid(m, nb, n, lambda ib: (
+            arrays[ib[0] + 8 * (ib[1] + 8 * ib[2])], cw))
+    del arrays
+    g64, g32 = grids[0], grids[1]
+    first = _props(g64.method, g64.blocks)
+    first32 = _props(g32.method, g32.blocks)
+    dt = g64.method.timestep(g64.blocks)
+    for _ in range(steps):
+        g64.refresh()
+        g32.refresh()
+        g32.method.compute(g32.blocks, dt)
+        dt = g64.method.compute_timestep(g64.blocks, dt)
+    ev.synchronize()
+    for g, f in ((g64, first), (g32, first32)):
+        counts = g.method.debug_counts()
+        print("debug counts", counts)
+        assert counts == dict.fromkeys(counts, 0), counts
+        print("diagnostics", _props(g.method, g.blocks, f))
+    sample = [0, 100, 255, 300, 511]
+    l1 = relative_l1([g32.blocks[i].download(HD) for i in sample],
+                     [g64.blocks[i].download(HD) for i in sample], HD)
+    print("mixed vs FP64 relative L1 on blocks", sample, l1)
+    assert max(l1.values()) <= MIXED_L1, l1
+    del grids, g64, g32
+    gc.collect()

@@ -313,3 +313,42 @@ def test_gravity_source_fused_and_staged_match_the_oracle(mhd):
         for k, v in got.items():
             if k != "pressure":
                 assert np.array_equal(v, ref[k]), (mhd, k)
+
+
+def test_staged_path_on_two_host_threads():
+    """The staged plugin path (device_fused = false: every call a
+    single-operation entry point, the CFL reduction by
+    enzoe_integrator_timestep) driven from two host threads at once, each
+    with its own Method and grid, equals the same cycles run one after the
+    other, bit for bit (dt included)."""
+    import threading
+    ev = _ev()
+    case = dict(mhd=True, riemann="hlld", recon=("nn", "plm"), theta=2.0)
+    grids = [random_periodic_grid(mhd=True, seed=s, n=(13, 11, 9), vamp=0.6, bamp=0.08)
+             for s in (41, 42)]
+
+    def run(grid, out, i):
+        dg = device_copy(grid, method_for(grid.cfg))
+        hm = ev.HostMethod(params(case, False), gamma=grid.cfg.gamma)
+        t, dts = 0.0, []
+        for _ in range(3):
+            dt = host_cycle(hm, dg, t, 10.0)
+            dts.append(dt)
+            t += dt
+        ev.synchronize()
+        out[i] = (dts, dg.blocks[0].download())
+
+    seq = [None, None]
+    for i, g in enumerate(grids):
+        run(g, seq, i)
+    par = [None, None]
+    threads = [threading.Thread(target=run, args=(g, par, i)) for i, g in enumerate(grids)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    for i in range(2):
+        assert par[i] is not None
+        assert par[i][0] == seq[i][0], i
+        for f in seq[i][1]:
+            assert np.array_equal(seq[i][1][f], par[i][1][f]), (i, f)
