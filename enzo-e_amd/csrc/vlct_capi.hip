@@ -72,10 +72,15 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 // Accumulators of enzoe_vlct_diagnostics.  Minima are kept as order-
 // preserving integer images of doubles (so an integer atomicMin is a double
 // min for either sign); maxima of non-negative values as their bit patterns.
+// The conserved totals are not summed with atomics (their order, and so
+// their rounding, would change from run to run, by far more than the
+// scheme's own conservation error over a large grid): every workgroup
+// writes compensated partial sums, which the host adds in a fixed order.
 struct DiagAcc {
-  double sum[5];                 // rho, rho v_x, rho v_y, rho v_z, rho e_tot
   unsigned long long min_rho, min_p, max_divb, max_bface, nonfinite;
 };
+constexpr int DIAG_NSUM = 5;   // rho, rho v_x, rho v_y, rho v_z, rho e_tot
+constexpr int DIAG_WG = 64;    // workgroups per block (grid-stride over cells)
 
 __host__ __device__ inline unsigned long long double_to_ordered(double v) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
@@ -86,18 +91,31 @@ __host__ __device__ inline double ordered_to_double(unsigned long long k) {
   return __builtin_bit_cast(double, b);
 }
 
+// s + c += x, Neumaier's compensated summation
+__host__ __device__ inline void comp_add(double &s, double &c, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x))
+    c += (s - t) + x;
+  else
+    c += (x - t) + s;
+  s = t;
+}
+// (s, c) += (s2, c2): Knuth's two-sum of the leading parts, the
+// compensations added to the error term
+__host__ __device__ inline void comp_merge(double &s, double &c, double s2, double c2) {
+  const double t = s + s2;
+  const double bp = t - s;
+  const double err = (s - (t - bp)) + (s2 - bp);
+  c = (c + c2) + err;
+  s = t;
+}
+
 __global__ void k_diag_init(DiagAcc *a) {
-  for (int i = 0; i < 5; i++) a->sum[i] = 0.0;
   a->min_rho = a->min_p = double_to_ordered(1.7976931348623157e308);
   a->max_divb = a->max_bface = 0ull;
   a->nonfinite = 0ull;
 }
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 __device__ inline unsigned long long wave_umin(unsigned long long v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -115,23 +133,27 @@ __device__ inline unsigned long long wave_umax(unsigned long long v) {
   return v;
 }
 
-// One thread per cell of a block (grid.y = block), ghosts included: counts
-// non-finite values of every field at the cell (and of the face fields'
-// last layer at the block's high ends); over the active cells the totals of
-// the conserved quantities, the minima of density and pressure, the largest
-// |div B| (times the smallest cell width) and the largest |face B|.  Every
-// thread reaches the wave reductions (out-of-block lanes contribute
-// neutral values); one atomic per wave and quantity.
+// DIAG_WG workgroups of 256 threads per block (grid.y = block), grid-stride
+// over the block's cells, ghosts included: counts non-finite values of
+// every field at the cell (and of the face fields' last layer at the
+// block's high ends); over the active cells the totals of the conserved
+// quantities (compensated, one partial pair per workgroup and quantity into
+// part[((block * DIAG_WG + wg) * DIAG_NSUM + q) * 2 + {0, 1}]), the minima
+// of density and pressure, the largest |div B| (times the smallest cell
+// width) and the largest |face B|.  Every thread reaches the reductions.
 template <bool MHD>
 __global__ void __launch_bounds__(256)
-k_diagnostics(const BlockDesc<double> *__restrict__ blocks, double gamma, DiagAcc *acc) {
+k_diagnostics(const BlockDesc<double> *__restrict__ blocks, double gamma, DiagAcc *acc,
+              double *__restrict__ part) {
   const BlockDesc<double> &b = blocks[blockIdx.y];
   const long ncell = (long)b.mx * b.my * b.mz;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double s[5] = {0., 0., 0., 0., 0.};
+  double s[DIAG_NSUM], cs[DIAG_NSUM];
+#pragma unroll
+  for (int q = 0; q < DIAG_NSUM; q++) s[q] = cs[q] = 0.;
   double rmin = 1.7976931348623157e308, pmin = rmin, divb = 0., bmax = 0.;
   unsigned long long bad = 0;
-  if (c < ncell) {
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+       c += (long)gridDim.x * blockDim.x) {
     const int i = (int)(c % b.mx), j = (int)((c / b.mx) % b.my), k = (int)(c / ((long)b.mx * b.my));
     const int nf = MHD ? S_N : 5;
     for (int q = 0; q < nf; q++) bad += !isfinite(b.U[q][c]);
@@ -148,42 +170,151 @@ k_diagnostics(const BlockDesc<double> *__restrict__ blocks, double gamma, DiagAc
       const double d = b.U[S_RHO][c];
       const double vx = b.U[S_VX][c], vy = b.U[S_VY][c], vz = b.U[S_VZ][c];
       const double e = b.U[S_E][c];
-      s[0] = d; s[1] = d * vx; s[2] = d * vy; s[3] = d * vz; s[4] = d * e;
-      rmin = d;
+      comp_add(s[0], cs[0], d);
+      comp_add(s[1], cs[1], d * vx);
+      comp_add(s[2], cs[2], d * vy);
+      comp_add(s[3], cs[3], d * vz);
+      comp_add(s[4], cs[4], d * e);
+      rmin = fmin(rmin, d);
       double bx = 0., by = 0., bz = 0.;
       if (MHD) { bx = b.U[S_BX][c]; by = b.U[S_BY][c]; bz = b.U[S_BZ][c]; }
-      pmin = cell_pressure<MHD>(d, vx, vy, vz, e, bx, by, bz, gamma);
+      pmin = fmin(pmin, cell_pressure<MHD>(d, vx, vy, vz, e, bx, by, bz, gamma));
       if (MHD) {
         const long fx = idx3(k, j, i, b.my, b.mx + 1), fy = idx3(k, j, i, b.my + 1, b.mx),
                    fz = idx3(k, j, i, b.my, b.mx);
         const double dv = (b.bfi[0][fx + 1] - b.bfi[0][fx]) / b.cw[0] +
                           (b.bfi[1][fy + b.mx] - b.bfi[1][fy]) / b.cw[1] +
                           (b.bfi[2][fz + (long)b.mx * b.my] - b.bfi[2][fz]) / b.cw[2];
-        divb = fabs(dv) * fmin(b.cw[0], fmin(b.cw[1], b.cw[2]));
-        bmax = fmax(fabs(b.bfi[0][fx]), fmax(fabs(b.bfi[1][fy]), fabs(b.bfi[2][fz])));
+        const double dvb = fabs(dv) * fmin(b.cw[0], fmin(b.cw[1], b.cw[2]));
+        // NaN never wins a max: a non-finite cell is counted instead
+        if (isfinite(dvb)) divb = fmax(divb, dvb);
+        const double bm =
+            fmax(fabs(b.bfi[0][fx]), fmax(fabs(b.bfi[1][fy]), fabs(b.bfi[2][fz])));
+        if (isfinite(bm)) bmax = fmax(bmax, bm);
       }
     }
   }
-  const bool lead = (threadIdx.x & 63) == 0;
+  // compensated sums: wave butterfly, then the waves in order through LDS
 #pragma unroll
-  for (int q = 0; q < 5; q++) {
-    const double t = wave_sum(s[q]);
-    if (lead && t != 0.0) atomicAdd(&acc->sum[q], t);
+  for (int q = 0; q < DIAG_NSUM; q++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double os = __shfl_xor(s[q], off, 64), oc = __shfl_xor(cs[q], off, 64);
+      // both lanes of a pair merge in the same order (lower lane first), so
+      // every lane ends with the same bits
+      if ((threadIdx.x & off) == 0)
+        comp_merge(s[q], cs[q], os, oc);
+      else {
+        double ls = os, lc = oc;
+        comp_merge(ls, lc, s[q], cs[q]);
+        s[q] = ls;
+        cs[q] = lc;
+      }
+    }
   }
+  __shared__ double wsum[4][DIAG_NSUM][2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int q = 0; q < DIAG_NSUM; q++) {
+      wsum[wave][q][0] = s[q];
+      wsum[wave][q][1] = cs[q];
+    }
   const unsigned long long r = wave_umin(double_to_ordered(rmin));
   const unsigned long long p = wave_umin(double_to_ordered(pmin));
-  // NaN never wins a max: a non-finite cell is counted instead
-  const unsigned long long dvb = wave_umax(isfinite(divb) ? __double_as_longlong(divb) : 0ull);
-  const unsigned long long bm = wave_umax(isfinite(bmax) ? __double_as_longlong(bmax) : 0ull);
+  const unsigned long long dvb = wave_umax(__double_as_longlong(divb));
+  const unsigned long long bm = wave_umax(__double_as_longlong(bmax));
   unsigned long long nb = bad;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) nb += __shfl_xor(nb, off, 64);
-  if (lead) {
+  if (lane == 0) {
     atomicMin(&acc->min_rho, r);
     atomicMin(&acc->min_p, p);
     atomicMax(&acc->max_divb, dvb);
     atomicMax(&acc->max_bface, bm);
     if (nb) atomicAdd(&acc->nonfinite, nb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double *out = part + ((long)blockIdx.y * gridDim.x + blockIdx.x) * DIAG_NSUM * 2;
+    for (int q = 0; q < DIAG_NSUM; q++) {
+      double ts = wsum[0][q][0], tc = wsum[0][q][1];
+      for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+        comp_merge(ts, tc, wsum[w][q][0], wsum[w][q][1]);
+      out[2 * q] = ts;
+      out[2 * q + 1] = tc;
+    }
+  }
+}
+
+// Per-field L1 sums of a state against another (enzoe_vlct_state_l1):
+// DIAG_WG workgroups per block pair (grid.y), grid-stride over the active
+// cells, compensated per thread, then per wave and workgroup as the
+// diagnostics sums; part[((pair * DIAG_WG + wg) * L1_NSUM + q) * 2 + {0,1}]
+// with q = diff 0..7, ref 8..15, |v| 16, |B| 17.
+constexpr int L1_NSUM = 18;
+template <bool MHD>
+__global__ void __launch_bounds__(256)
+k_state_l1(const BlockDesc<double> *__restrict__ da, const BlockDesc<double> *__restrict__ db,
+           double *__restrict__ part) {
+  const BlockDesc<double> &a = da[blockIdx.y];
+  const BlockDesc<double> &b = db[blockIdx.y];
+  const int g = b.g;
+  const int nx = b.mx - 2 * g, ny = b.my - 2 * g, nz = b.mz - 2 * g;
+  const long nact = (long)nx * ny * nz;
+  constexpr int NQ = MHD ? S_N : 5;
+  double s[L1_NSUM], cs[L1_NSUM];
+#pragma unroll
+  for (int q = 0; q < L1_NSUM; q++) s[q] = cs[q] = 0.;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nact;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nx) + g, j = (int)((t / nx) % ny) + g,
+              k = (int)(t / ((long)nx * ny)) + g;
+    const long c = idx3(k, j, i, b.my, b.mx);
+    double vb[S_N];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      vb[q] = b.U[q][c];
+      comp_add(s[q], cs[q], fabs(a.U[q][c] - vb[q]));
+      comp_add(s[8 + q], cs[8 + q], fabs(vb[q]));
+    }
+    comp_add(s[16], cs[16], sqrt(vb[S_VX] * vb[S_VX] + vb[S_VY] * vb[S_VY] +
+                                 vb[S_VZ] * vb[S_VZ]));
+    if (MHD)
+      comp_add(s[17], cs[17], sqrt(vb[S_BX] * vb[S_BX] + vb[S_BY] * vb[S_BY] +
+                                   vb[S_BZ] * vb[S_BZ]));
+  }
+#pragma unroll
+  for (int q = 0; q < L1_NSUM; q++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double os = __shfl_xor(s[q], off, 64), oc = __shfl_xor(cs[q], off, 64);
+      if ((threadIdx.x & off) == 0)
+        comp_merge(s[q], cs[q], os, oc);
+      else {
+        double ls = os, lc = oc;
+        comp_merge(ls, lc, s[q], cs[q]);
+        s[q] = ls;
+        cs[q] = lc;
+      }
+    }
+  }
+  __shared__ double wsum[4][L1_NSUM][2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int q = 0; q < L1_NSUM; q++) {
+      wsum[wave][q][0] = s[q];
+      wsum[wave][q][1] = cs[q];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double *out = part + ((long)blockIdx.y * gridDim.x + blockIdx.x) * L1_NSUM * 2;
+    for (int q = 0; q < L1_NSUM; q++) {
+      double ts = wsum[0][q][0], tc = wsum[0][q][1];
+      for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+        comp_merge(ts, tc, wsum[w][q][0], wsum[w][q][1]);
+      out[2 * q] = ts;
+      out[2 * q + 1] = tc;
+    }
   }
 }
 
@@ -637,14 +768,16 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   return ENZOE_OK;
 }
 
-int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
-                       int nby, int nbz, const int bc[3],
-                       enzoe_stream_t stream) {
+// The refresh of a uniform grid; cycle_dt (device, may be null): the dt of
+// a device-driven cycle, the refresh does nothing when it is 0.
+static int grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx, int nby,
+                        int nbz, const int bc[3], hipStream_t s,
+                        const double *cycle_dt) {
   if (!m || !blocks || nbx <= 0 || nby <= 0 || nbz <= 0 || !bc)
     return ENZOE_ERR_ARG;
-  hipStream_t s = (hipStream_t)stream;
   const int nb = nbx * nby * nbz;
   RefreshGrid G;
+  G.dt = cycle_dt;
   G.nb[0] = nbx; G.nb[1] = nby; G.nb[2] = nbz;
   G.g = blocks[0].ghost;
   G.n[0] = blocks[0].mx - 2 * G.g;
@@ -705,6 +838,18 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
   return ENZOE_OK;
 }
 
+int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
+                       int nby, int nbz, const int bc[3],
+                       enzoe_stream_t stream) {
+  return grid_refresh(m, blocks, nbx, nby, nbz, bc, (hipStream_t)stream, nullptr);
+}
+
+int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int on) {
+  if (!m) return ENZOE_ERR_ARG;
+  m->ctu = on ? 1 : 0;
+  return ENZOE_OK;
+}
+
 int enzoe_vlct_set_debug_checks(enzoe_vlct_t m, int on) {
   if (!m) return ENZOE_ERR_ARG;
   if (!on) {
@@ -757,7 +902,6 @@ int enzoe_vlct_diagnostics(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   }
   hipStream_t s = (hipStream_t)stream;
   std::vector<BlockDesc<double>> desc(n);
-  long maxcells = 0;
   for (int i = 0; i < n; i++) {
     BlockDesc<double> &d = desc[i];
     std::memset(&d, 0, sizeof(d));
@@ -773,46 +917,135 @@ int enzoe_vlct_diagnostics(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     d.mx = b.mx; d.my = b.my; d.mz = b.mz;
     d.g = b.ghost;
     for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
-    maxcells = std::max<long>(maxcells, (long)b.mx * b.my * b.mz);
   }
   // a diagnostic: its own buffers, taken and returned on every call
-  void *d_desc = nullptr, *d_acc = nullptr;
+  void *d_desc = nullptr, *d_acc = nullptr, *d_part = nullptr;
   const size_t dbytes = desc.size() * sizeof(BlockDesc<double>);
+  const size_t nparts = (size_t)n * DIAG_WG * DIAG_NSUM * 2;
   if (hipMalloc(&d_desc, dbytes) != hipSuccess) return ENZOE_ERR_ALLOC;
   if (hipMalloc(&d_acc, sizeof(DiagAcc)) != hipSuccess) {
     (void)hipFree(d_desc);
     return ENZOE_ERR_ALLOC;
   }
+  if (hipMalloc(&d_part, nparts * sizeof(double)) != hipSuccess) {
+    (void)hipFree(d_desc);
+    (void)hipFree(d_acc);
+    return ENZOE_ERR_ALLOC;
+  }
   int rc = ENZOE_OK;
   DiagAcc acc;
+  std::vector<double> parts(nparts);
   if (hipMemcpyAsync(d_desc, desc.data(), dbytes, hipMemcpyHostToDevice, s) != hipSuccess)
     rc = ENZOE_ERR_HIP;
   if (rc == ENZOE_OK) {
     hipLaunchKernelGGL(k_diag_init, dim3(1), dim3(1), 0, s, (DiagAcc *)d_acc);
-    dim3 grid(nblk(maxcells, 256), n);
+    dim3 grid(DIAG_WG, n);
     if (m->mhd)
       hipLaunchKernelGGL((k_diagnostics<true>), grid, dim3(256), 0, s,
-                         (const BlockDesc<double> *)d_desc, m->gamma, (DiagAcc *)d_acc);
+                         (const BlockDesc<double> *)d_desc, m->gamma, (DiagAcc *)d_acc,
+                         (double *)d_part);
     else
       hipLaunchKernelGGL((k_diagnostics<false>), grid, dim3(256), 0, s,
-                         (const BlockDesc<double> *)d_desc, m->gamma, (DiagAcc *)d_acc);
+                         (const BlockDesc<double> *)d_desc, m->gamma, (DiagAcc *)d_acc,
+                         (double *)d_part);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(&acc, d_acc, sizeof(acc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(parts.data(), d_part, nparts * sizeof(double), hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       rc = ENZOE_ERR_HIP;
   }
   (void)hipStreamSynchronize(s);
   (void)hipFree(d_desc);
   (void)hipFree(d_acc);
+  (void)hipFree(d_part);
   if (rc) return rc;
-  out->mass = acc.sum[0];
-  for (int a = 0; a < 3; a++) out->momentum[a] = acc.sum[1 + a];
-  out->energy = acc.sum[4];
+  // the partial pairs in a fixed order (block, workgroup)
+  double tot[DIAG_NSUM], comp[DIAG_NSUM];
+  for (int q = 0; q < DIAG_NSUM; q++) tot[q] = comp[q] = 0.;
+  for (size_t w = 0; w < (size_t)n * DIAG_WG; w++)
+    for (int q = 0; q < DIAG_NSUM; q++)
+      comp_merge(tot[q], comp[q], parts[(w * DIAG_NSUM + q) * 2],
+                 parts[(w * DIAG_NSUM + q) * 2 + 1]);
+  out->mass = tot[0] + comp[0];
+  for (int a = 0; a < 3; a++) out->momentum[a] = tot[1 + a] + comp[1 + a];
+  out->energy = tot[4] + comp[4];
   out->min_density = ordered_to_double(acc.min_rho);
   out->min_pressure = ordered_to_double(acc.min_p);
   out->max_abs_divb = __builtin_bit_cast(double, acc.max_divb);
   out->max_abs_bface = __builtin_bit_cast(double, acc.max_bface);
   out->nonfinite = (long long)acc.nonfinite;
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_state_l1(enzoe_vlct_t m, const enzoe_block *a, const enzoe_block *b,
+                        int n, enzoe_stream_t stream, enzoe_state_l1 *out) {
+  if (!m || !a || !b || n <= 0 || !out) return ENZOE_ERR_ARG;
+  std::vector<BlockDesc<double>> desc(2 * (size_t)n);
+  for (int side = 0; side < 2; side++)
+    for (int i = 0; i < n; i++) {
+      const enzoe_block &x = side ? b[i] : a[i];
+      const enzoe_block &y = b[i];
+      if (x.mx != y.mx || x.my != y.my || x.mz != y.mz || x.ghost != y.ghost ||
+          x.mx < 1 || x.my < 1 || x.mz < 1 || x.ghost < 0 || 2 * x.ghost >= x.mx ||
+          2 * x.ghost >= x.my || 2 * x.ghost >= x.mz || !x.density || !x.velocity[0] ||
+          !x.velocity[1] || !x.velocity[2] || !x.total_energy)
+        return ENZOE_ERR_ARG;
+      if (m->mhd && (!x.bfield[0] || !x.bfield[1] || !x.bfield[2])) return ENZOE_ERR_ARG;
+      BlockDesc<double> &d = desc[(size_t)side * n + i];
+      std::memset(&d, 0, sizeof(d));
+      d.U[S_RHO] = x.density;
+      for (int c = 0; c < 3; c++) d.U[S_VX + c] = x.velocity[c];
+      d.U[S_E] = x.total_energy;
+      if (m->mhd)
+        for (int c = 0; c < 3; c++) d.U[S_BX + c] = x.bfield[c];
+      d.mx = x.mx; d.my = x.my; d.mz = x.mz;
+      d.g = x.ghost;
+    }
+  hipStream_t s = (hipStream_t)stream;
+  void *d_desc = nullptr, *d_part = nullptr;
+  const size_t dbytes = desc.size() * sizeof(BlockDesc<double>);
+  const size_t nparts = (size_t)n * DIAG_WG * L1_NSUM * 2;
+  if (hipMalloc(&d_desc, dbytes) != hipSuccess) return ENZOE_ERR_ALLOC;
+  if (hipMalloc(&d_part, nparts * sizeof(double)) != hipSuccess) {
+    (void)hipFree(d_desc);
+    return ENZOE_ERR_ALLOC;
+  }
+  std::vector<double> parts(nparts);
+  int rc = ENZOE_OK;
+  if (hipMemcpyAsync(d_desc, desc.data(), dbytes, hipMemcpyHostToDevice, s) != hipSuccess)
+    rc = ENZOE_ERR_HIP;
+  if (rc == ENZOE_OK) {
+    const BlockDesc<double> *da = (const BlockDesc<double> *)d_desc;
+    dim3 grid(DIAG_WG, n);
+    if (m->mhd)
+      hipLaunchKernelGGL((k_state_l1<true>), grid, dim3(256), 0, s, da, da + n,
+                         (double *)d_part);
+    else
+      hipLaunchKernelGGL((k_state_l1<false>), grid, dim3(256), 0, s, da, da + n,
+                         (double *)d_part);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(parts.data(), d_part, nparts * sizeof(double), hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      rc = ENZOE_ERR_HIP;
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(d_desc);
+  (void)hipFree(d_part);
+  if (rc) return rc;
+  double tot[L1_NSUM], comp[L1_NSUM];
+  for (int q = 0; q < L1_NSUM; q++) tot[q] = comp[q] = 0.;
+  for (size_t w = 0; w < (size_t)n * DIAG_WG; w++)
+    for (int q = 0; q < L1_NSUM; q++)
+      comp_merge(tot[q], comp[q], parts[(w * L1_NSUM + q) * 2],
+                 parts[(w * L1_NSUM + q) * 2 + 1]);
+  for (int q = 0; q < 8; q++) {
+    out->diff[q] = tot[q] + comp[q];
+    out->ref[q] = tot[8 + q] + comp[8 + q];
+  }
+  out->velocity_magnitude = tot[16] + comp[16];
+  out->bfield_magnitude = tot[17] + comp[17];
   return ENZOE_OK;
 }
 
@@ -932,7 +1165,7 @@ int enzoe_vlct_graph_create(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
   // (dry: nothing is launched, so the first replayed cycle sees the fields
   // exactly as an eager cycle would, stale ghost zones included)
   m->dry = true;
-  rc = enzoe_grid_refresh(m, blocks, nbx, nby, nbz, bc, s);
+  rc = grid_refresh(m, blocks, nbx, nby, nbz, bc, s, nullptr);
   if (!rc) rc = launch_dt(m, blocks, n, s);
   if (!rc) rc = compute_runs(m, blocks, n, 0.0, g->d_state + 1, s);
   m->dry = false;
@@ -947,7 +1180,8 @@ int enzoe_vlct_graph_create(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
   for (int c = 0; c < ncycles && rc == ENZOE_OK; c++) {
     rc = enzoe_vlct_timestep_device(m, blocks, n, g->d_state, t_stop, g->d_state + 1,
                                     n, s);
-    if (!rc) rc = enzoe_grid_refresh(m, blocks, nbx, nby, nbz, bc, s);
+    // a cycle past t_stop (dt 0) leaves the ghost zones alone too
+    if (!rc) rc = grid_refresh(m, blocks, nbx, nby, nbz, bc, s, g->d_state + 1);
     if (!rc) rc = compute_runs(m, blocks, n, 0.0, g->d_state + 1, s);
   }
   hipGraph_t graph = nullptr;

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "vlct_physics.hpp"
 
 namespace enzoe {
@@ -32,10 +34,17 @@ struct BlockDesc {
   double *bfi[3];      // bfieldi_x..z (start of step / final output)
   double *pressure;    // permanent pressure field (may be null)
   double *Ut[S_N];     // half-step state (temp integration map)
+  float *Up[S_N];      // mixed path: the half-step state as FP32 primitives
+                       // (rho, v, p, B; replaces Ut there)
   double *bft[3];      // half-step face B
   FluxT *F[3][S_N];    // fluxes along x, y, z; slot = field
   double *E[3];        // edge EMFs along x, y, z
   double *Ec[3];       // cell-centred E = -v x B of the stage's state
+  double *Ech[3];      // mixed path: centre E of the half-step state (the
+                       // first stage's cell update writes it), or null
+  double *bf0[3];      // fused CT update: start-of-step face B of the first
+                       // stage's update region (the final stage reads its
+                       // base values here), or null
   const double *acc[3];  // acceleration_x..z for the gravity source, or null
   // dual-energy formalism (null otherwise): specific internal_energy (start
   // of step / final output), its half-step copy, and per dim the
@@ -246,26 +255,66 @@ __device__ __forceinline__ void gst(P *p, long i, P v) {
 // Cell primitives w = (rho, vx, vy, vz, p, bx, by, bz) from the integration
 // quantities (EnzoPhysicsFluidProps.cpp:64-138, EnzoComputePressure.cpp:
 // 127-189), in FP64; with dual energy the pressure is gm1 rho eint.
+// The primitive pressure gm1 (rho (e - ke) - me) of specific total energy
+// e (EnzoComputePressure.cpp:177-189).  Never contracted, so that the flux
+// kernel's and the mixed path's half-step store round alike.
+template <bool MHD>
+__device__ __forceinline__ double prim_pressure(double rho, double vx, double vy, double vz,
+                                                double e, double bx, double by, double bz,
+                                                double gm1) {
+#pragma clang fp contract(off)
+  const double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+  const double me = MHD ? 0.5 * (bx * bx + by * by + bz * bz) : 0.;
+  return gm1 * (rho * (e - ke) - me);
+}
+
 template <bool MHD, bool DE = false>
 __device__ __forceinline__ void load_prim(double *const U[S_N], int c,
                                           double gm1, double w[S_N],
                                           const double *ei = nullptr) {
   double rho = gld(U[S_RHO], c);
   double vx = gld(U[S_VX], c), vy = gld(U[S_VY], c), vz = gld(U[S_VZ], c);
-  double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
-  double me = 0.;
+  double bx = 0., by = 0., bz = 0.;
   w[S_RHO] = rho; w[S_VX] = vx; w[S_VY] = vy; w[S_VZ] = vz;
   if (MHD) {
-    double bx = gld(U[S_BX], c), by = gld(U[S_BY], c), bz = gld(U[S_BZ], c);
-    me = 0.5 * (bx * bx + by * by + bz * bz);
-    w[S_BX] = bx; w[S_BY] = by; w[S_BZ] = bz;
-  } else {
-    w[S_BX] = 0.; w[S_BY] = 0.; w[S_BZ] = 0.;
+    bx = gld(U[S_BX], c); by = gld(U[S_BY], c); bz = gld(U[S_BZ], c);
   }
+  w[S_BX] = bx; w[S_BY] = by; w[S_BZ] = bz;
   if (DE)
     w[S_E] = gm1 * rho * gld(ei, c);
   else
-    w[S_E] = gm1 * (rho * (gld(U[S_E], c) - ke) - me);
+    w[S_E] = prim_pressure<MHD>(rho, vx, vy, vz, gld(U[S_E], c), bx, by, bz, gm1);
+}
+
+// Primitives of one cell in the flux kernel's arithmetic type T: from the
+// FP64 integration quantities (pressure computed here), or, with PRIMF,
+// loaded from the FP32 primitive half-step state that the previous stage's
+// cell update stored (the same FP64 values rounded to FP32 once).
+template <typename T, bool MHD, bool DE, bool PRIMF>
+struct PrimSource {
+  double *const *U;
+  const double *ei;
+  const float *const *Up;
+  double gm1;
+  __device__ __forceinline__ void operator()(int c, T w[S_N]) const {
+    if constexpr (PRIMF) {
+#pragma unroll
+      for (int q = 0; q < S_N; q++) w[q] = (MHD || q < S_BX) ? (T)gld(Up[q], c) : T(0);
+    } else {
+      double d[S_N];
+      load_prim<MHD, DE>(U, c, gm1, d, ei);
+#pragma unroll
+      for (int q = 0; q < S_N; q++) w[q] = (T)d[q];
+    }
+  }
+};
+
+// Centre E_i = -v_j B_k + v_k B_j (EnzoBfieldMethodCT::compute_center_efield,
+// EnzoBfieldMethodCT.cpp:267-292); never contracted, so that the flux
+// kernel and the mixed path's half-step store give the same bits.
+__device__ __forceinline__ double center_efield(double vj, double vk, double bj, double bk) {
+#pragma clang fp contract(off)
+  return (-vj * bk + vk * bj);
 }
 
 // ---------------------------------------------------------------- k_flux
@@ -280,32 +329,31 @@ struct FluxConsts {
 // sides, the face-centred normal B (EnzoBfieldMethodCT.cpp:122-166) and the
 // Riemann solve, written at index c of the flux arrays of dim D.
 template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD, int D,
-          bool DE = false>
+          bool DE, bool PRIMF>
 __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
-                                          double *const *U,
+                                          const PrimSource<T, MHD, DE, PRIMF> &src,
                                           const double *bfi, int k, int j,
-                                          int i, int c, const double wc[S_N],
-                                          const FluxConsts &fc,
-                                          const double *ei = nullptr) {
+                                          int i, int c, const T wc[S_N],
+                                          const FluxConsts &fc) {
   constexpr int DJ = (D + 1) % 3, DK = (D + 2) % 3;
   const int s = (D == 0) ? 1 : ((D == 1) ? b.mx : b.mx * b.my);
   T wl[S_N], wr[S_N];
   if (RECON == RC_NN) {
-    double a[S_N];
-    load_prim<MHD, DE>(U, c + s, fc.gm1, a, ei);
+    T a[S_N];
+    src(c + s, a);
 #pragma unroll
-    for (int q = 0; q < S_N; q++) { wl[q] = (T)wc[q]; wr[q] = (T)a[q]; }
+    for (int q = 0; q < S_N; q++) { wl[q] = wc[q]; wr[q] = a[q]; }
   } else {
-    double w0[S_N], w2[S_N], w3[S_N];  // cells c-1, c+1, c+2 along D
-    load_prim<MHD, DE>(U, c - s, fc.gm1, w0, ei);
-    load_prim<MHD, DE>(U, c + s, fc.gm1, w2, ei);
-    load_prim<MHD, DE>(U, c + 2 * s, fc.gm1, w3, ei);
+    T w0[S_N], w2[S_N], w3[S_N];  // cells c-1, c+1, c+2 along D
+    src(c - s, w0);
+    src(c + s, w2);
+    src(c + 2 * s, w3);
     const T theta = (T)fc.theta;
 #pragma unroll
     for (int q = 0; q < S_N; q++) {
       if (!MHD && q >= S_BX) continue;
       if (MHD && q == S_BX + D) continue;  // replaced by the face value
-      T a0 = (T)w0[q], a1 = (T)wc[q], a2 = (T)w2[q], a3 = (T)w3[q];
+      T a0 = w0[q], a1 = wc[q], a2 = w2[q], a3 = w3[q];
       T dvl = (RECON == RC_PLM) ? limiter_enzo(a0, a1, a2, theta)
                                 : limiter_athena(a0, a1, a2);
       T dvr = (RECON == RC_PLM) ? limiter_enzo(a1, a2, a3, theta)
@@ -399,8 +447,10 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
 #define ENZOE_FLUX_NN_WAVES 3
 #endif
 
+// PRIMF: the stage reads the FP32 primitive half-step state of the mixed
+// path (b.Up), whose cell update also stored the centre E.
 template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD,
-          bool DE = false>
+          bool DE = false, bool PRIMF = false>
 __global__ void __launch_bounds__(256, RECON == RC_NN ? ENZOE_FLUX_NN_WAVES
                                                       : ENZOE_FLUX_MIN_WAVES)
 k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
@@ -426,25 +476,33 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   fc.dbg = sp.dbg;
   const int c = (k * my + j) * mx + i;
   const double *ei = DE ? ((sp.stage == 0) ? b.ei : b.eit) : nullptr;
-  double wc[S_N];
-  load_prim<MHD, DE>(U, c, fc.gm1, wc, ei);
-  if (MHD) {
-    // centre E_i = -v_j B_k + v_k B_j of this cell for the edge kernel
-    // (EnzoBfieldMethodCT::compute_center_efield, EnzoBfieldMethodCT.cpp:
-    // 267-292); the box of k_flux covers every cell the edges read
-    gst(b.Ec[0], c, (-wc[S_VY] * wc[S_BZ] + wc[S_VZ] * wc[S_BY]));
-    gst(b.Ec[1], c, (-wc[S_VZ] * wc[S_BX] + wc[S_VX] * wc[S_BZ]));
-    gst(b.Ec[2], c, (-wc[S_VX] * wc[S_BY] + wc[S_VY] * wc[S_BX]));
+  const PrimSource<T, MHD, DE, PRIMF> src{U, ei, b.Up, fc.gm1};
+  T wc[S_N];
+  if constexpr (PRIMF) {
+    src(c, wc);
+  } else {
+    double wd[S_N];
+    load_prim<MHD, DE>(U, c, fc.gm1, wd, ei);
+    if (MHD) {
+      // centre E_i = -v_j B_k + v_k B_j of this cell for the edge kernel
+      // (EnzoBfieldMethodCT::compute_center_efield, EnzoBfieldMethodCT.cpp:
+      // 267-292); the box of k_flux covers every cell the edges read
+      gst(b.Ec[0], c, center_efield(wd[S_VY], wd[S_VZ], wd[S_BY], wd[S_BZ]));
+      gst(b.Ec[1], c, center_efield(wd[S_VZ], wd[S_VX], wd[S_BZ], wd[S_BX]));
+      gst(b.Ec[2], c, center_efield(wd[S_VX], wd[S_VY], wd[S_BX], wd[S_BY]));
+    }
+#pragma unroll
+    for (int q = 0; q < S_N; q++) wc[q] = (T)wd[q];
   }
   // the z face first: in the tiled order the plane above is the one not yet
   // in L2, so its loads go out with this cell's and the two latencies
   // overlap (each face writes its own arrays; the order changes no value)
   if (k < mz - 1 - cs)
-    flux_face<T, FluxT, RECON, RIEMANN, MHD, 2, DE>(b, U, bfr[2], k, j, i, c, wc, fc, ei);
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 2, DE, PRIMF>(b, src, bfr[2], k, j, i, c, wc, fc);
   if (i < mx - 1 - cs)
-    flux_face<T, FluxT, RECON, RIEMANN, MHD, 0, DE>(b, U, bfr[0], k, j, i, c, wc, fc, ei);
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 0, DE, PRIMF>(b, src, bfr[0], k, j, i, c, wc, fc);
   if (j < my - 1 - cs)
-    flux_face<T, FluxT, RECON, RIEMANN, MHD, 1, DE>(b, U, bfr[1], k, j, i, c, wc, fc, ei);
+    flux_face<T, FluxT, RECON, RIEMANN, MHD, 1, DE, PRIMF>(b, src, bfr[1], k, j, i, c, wc, fc);
 }
 
 // ---------------------------------------------------------------- k_edge
@@ -455,12 +513,64 @@ __device__ __forceinline__ double upwind_weight(double f) {
   return (f > 0) ? 1.0 : ((f < 0) ? 0.0 : 0.5);
 }
 
+// The centre E of the stage: the mixed path's final stage reads the one its
+// first stage's cell update stored with the half-step state.
+template <typename FluxT>
+__device__ __forceinline__ double *const *center_e(const BlockDesc<FluxT> &b,
+                                                   const StageParams &sp) {
+  return (sp.stage > 0 && b.Ech[0] != nullptr) ? b.Ech : b.Ec;
+}
+
+// The upwinded edge EMF along axis AX at (cell p_AX, faces p_j + 1/2,
+// p_k + 1/2), c the cell-shaped index of p, sj / sk the strides of the
+// other two axes (EnzoBfieldMethodCT::compute_edge_efield /
+// compute_edge_<true>, EnzoBfieldMethodCT.cpp:384-557), from the centre E
+// of the four cells around it (compute_center_efield, :267-292) and the
+// face fluxes of B and density.  FP64, never contracted: the same bits in
+// every kernel and build that evaluates it.
+template <int AX, typename FluxT>
+__device__ __forceinline__ double edge_emf(const BlockDesc<FluxT> &b, double *const *Ecs,
+                                           long c, long sj, long sk) {
+#pragma clang fp contract(off)
+  constexpr int aj = (AX + 1) % 3, ak = (AX + 2) % 3;
+  const double *Eci = Ecs[AX];
+  const double Ec = Eci[c], Ec_jp1 = Eci[c + sj], Ec_kp1 = Eci[c + sk],
+               Ec_jkp1 = Eci[c + sj + sk];
+  // face E: j-faces carry -F_j(B_k), k-faces +F_k(B_j)
+  const FluxT *Fj = b.F[aj][S_BX + ak];
+  const FluxT *Fk = b.F[ak][S_BX + aj];
+  const double Ej = (double)Fj[c], Ej_kp1 = (double)Fj[c + sk];
+  const double Ek = (double)Fk[c], Ek_jp1 = (double)Fk[c + sj];
+  const double Wj = upwind_weight((double)b.F[aj][S_RHO][c]);
+  const double Wj_kp1 = upwind_weight((double)b.F[aj][S_RHO][c + sk]);
+  const double Wk = upwind_weight((double)b.F[ak][S_RHO][c]);
+  const double Wk_jp1 = upwind_weight((double)b.F[ak][S_RHO][c + sj]);
+
+  double dEdj_r = Wk_jp1 * (Ec_jp1 + Ej) + (1 - Wk_jp1) * (Ec_jkp1 + Ej_kp1);
+  double dEdj_l = Wk * (-Ej - Ec) + (1 - Wk) * (-Ej_kp1 - Ec_kp1);
+  double dEdk_r = Wj_kp1 * (Ec_kp1 - Ek) + (1 - Wj_kp1) * (Ec_jkp1 - Ek_jp1);
+  double dEdk_l = Wj * (Ek - Ec) + (1 - Wj) * (Ek_jp1 - Ec_jp1);
+  double Ej_sum = Ej + Ej_kp1;
+  Ej_sum *= -1;
+  double Ek_sum = Ek + Ek_jp1;
+  return 0.25 * (Ej_sum + Ek_sum + (dEdj_l - dEdj_r) + (dEdk_l - dEdk_r));
+}
+
+// Whether the edge EMF along axis ax at cell-shaped index p is one the
+// stage computes: [cs+1, m_ax-cs-1) along ax, [cs, m-cs-1) across.
+__device__ __forceinline__ bool edge_in_region(int ax, const int p[3], const int m[3], int cs) {
+  const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
+  return p[ax] >= cs + 1 && p[ax] < m[ax] - cs - 1 && p[aj] >= cs && p[aj] < m[aj] - cs - 1 &&
+         p[ak] >= cs && p[ak] < m[ak] - cs - 1;
+}
+
 // One thread per position p of the box [cs, m - cs); computes the edge EMF
 // along each axis i at (cell p_i, faces p_j + 1/2, p_k + 1/2) when the edge
 // lies in [cs+1, m_i-cs-1) x [cs, m_j-cs-1) x [cs, m_k-cs-1)
 // (EnzoBfieldMethodCT::compute_edge_efield / compute_edge_<true>,
 // EnzoBfieldMethodCT.cpp:384-557); the centre E of the current-stage state
-// (compute_center_efield, :267-292) was written by k_flux.
+// (compute_center_efield, :267-292) was written by k_flux (or, for the mixed
+// path's final stage, by its first stage's cell update).
 template <typename FluxT>
 __global__ void __launch_bounds__(256)
 k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
@@ -479,40 +589,27 @@ k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const int m[3] = {mx, my, mz};
   const long c = idx3(k, j, i, my, mx);
   const long str[3] = {1, (long)mx, (long)mx * my};
-// the three axes unrolled: their loads go out together
-#pragma unroll
-  for (int ax = 0; ax < 3; ax++) {
-    const int aj = (ax + 1) % 3, ak = (ax + 2) % 3;
-    if (p[ax] < cs + 1 || p[ax] >= m[ax] - cs - 1) continue;
-    if (p[aj] >= m[aj] - cs - 1 || p[ak] >= m[ak] - cs - 1) continue;
-    const long sj = str[aj], sk = str[ak];
-    // centre E_i at the four cells around the edge (written by k_flux)
-    const double *Eci = b.Ec[ax];
-    const double Ec = Eci[c], Ec_jp1 = Eci[c + sj], Ec_kp1 = Eci[c + sk],
-                 Ec_jkp1 = Eci[c + sj + sk];
-    // face E: j-faces carry -F_j(B_k), k-faces +F_k(B_j)
-    const FluxT *Fj = b.F[aj][S_BX + ak];
-    const FluxT *Fk = b.F[ak][S_BX + aj];
-    const double Ej = (double)Fj[c], Ej_kp1 = (double)Fj[c + sk];
-    const double Ek = (double)Fk[c], Ek_jp1 = (double)Fk[c + sj];
-    const double Wj = upwind_weight((double)b.F[aj][S_RHO][c]);
-    const double Wj_kp1 = upwind_weight((double)b.F[aj][S_RHO][c + sk]);
-    const double Wk = upwind_weight((double)b.F[ak][S_RHO][c]);
-    const double Wk_jp1 = upwind_weight((double)b.F[ak][S_RHO][c + sj]);
-
-    double dEdj_r = Wk_jp1 * (Ec_jp1 + Ej) + (1 - Wk_jp1) * (Ec_jkp1 + Ej_kp1);
-    double dEdj_l = Wk * (-Ej - Ec) + (1 - Wk) * (-Ej_kp1 - Ec_kp1);
-    double dEdk_r = Wj_kp1 * (Ec_kp1 - Ek) + (1 - Wj_kp1) * (Ec_jkp1 - Ek_jp1);
-    double dEdk_l = Wj * (Ek - Ec) + (1 - Wj) * (Ek_jp1 - Ec_jp1);
-    double Ej_sum = Ej + Ej_kp1;
-    Ej_sum *= -1;
-    double Ek_sum = Ek + Ek_jp1;
-    b.E[ax][c] =
-        0.25 * (Ej_sum + Ek_sum + (dEdj_l - dEdj_r) + (dEdk_l - dEdk_r));
-  }
+  double *const *Ecs = center_e(b, sp);
+  // the three axes unrolled: their loads go out together
+  if (edge_in_region(0, p, m, cs)) b.E[0][c] = edge_emf<0>(b, Ecs, c, str[1], str[2]);
+  if (edge_in_region(1, p, m, cs)) b.E[1][c] = edge_emf<1>(b, Ecs, c, str[2], str[0]);
+  if (edge_in_region(2, p, m, cs)) b.E[2][c] = edge_emf<2>(b, Ecs, c, str[0], str[1]);
 }
 
 // --------------------------------------------------------------- k_faces
+
+// The updated face B_i from its start-of-step value and the four edge EMFs
+// around it (EnzoBfieldMethodCT::update_bfield, EnzoBfieldMethodCT.cpp:
+// 617-687): E_k at j -/+ 1/2 (ek_m, ek_p), E_j at k -/+ 1/2 (ej_m, ej_p).
+// Never contracted.
+__device__ __forceinline__ double face_b_update(double b0, double dtdj, double dtdk,
+                                                double ek_p, double ek_m, double ej_p,
+                                                double ej_m) {
+#pragma clang fp contract(off)
+  double E_k_term = dtdj * (ek_p - ek_m);
+  double E_j_term = dtdk * (ej_p - ej_m);
+  return b0 - E_k_term + E_j_term;
+}
 
 // One thread per position p of the box [cs, m - cs + 1) (faces reach one
 // index further); updates face X = p_i of bfieldi_i when
@@ -549,12 +646,11 @@ k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     // edges at i-position X - 1/2: cell-shaped index of p - e_i
     const long e = idx3(k, j, i, my, mx) - str[ax];
     const double *Ej = b.E[aj], *Ek = b.E[ak];
-    double E_k_term = dtdj * (Ek[e] - Ek[e - str[aj]]);
-    double E_j_term = dtdk * (Ej[e] - Ej[e - str[ak]]);
     const int bnx = mx + (ax == 0), bny = my + (ax == 1);
     const long cb = idx3(k, j, i, bny, bnx);
     double *out = sp.is_final ? b.bfi[ax] : b.bft[ax];
-    out[cb] = b.bfi[ax][cb] - E_k_term + E_j_term;
+    out[cb] = face_b_update(b.bfi[ax][cb], dtdj, dtdk, Ek[e], Ek[e - str[aj]], Ej[e],
+                            Ej[e - str[ak]]);
   }
 }
 
@@ -646,44 +742,38 @@ __device__ __forceinline__ void block_min_to(unsigned long long *result, double 
 // 16-92, called after each dim's divergence,
 // EnzoMHDIntegratorStageCommands.cpp:248-255), and the energies are
 // synchronised instead of floored.
-template <typename FluxT, bool MHD, bool DE = false>
-__global__ void __launch_bounds__(256)
-k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  int bx, by;
-  wg_coords(sp.xcd_remap & 8, bx, by);
-  if (zero_step(sp, by)) return;
-  const BlockDesc<FluxT> &b = blocks[by];
+// The per-cell work of k_cells for the cell (k, j, i) of the block `b`;
+// in_launch / in_box: the cell lies in the launch box / in [lo, m - lo).
+// face_b(k, j, i, f[6]) gives the cell's six face values of the stage's
+// face-centred B (low and high face along x, y, z).  Returns the cell's CFL
+// term (DBL_MAX when it takes none).
+template <typename FluxT, bool MHD, bool DE, typename FaceB>
+__device__ __forceinline__ double cell_stage(const BlockDesc<FluxT> &b, const StageParams &sp,
+                                             int by, int k, int j, int i, bool in_launch,
+                                             bool in_box, const FaceB &face_b) {
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
-  const int lo = sp.is_final ? cs : cs + 1;
-  int k, j, i;
-  const long t = (long)bx * blockDim.x + threadIdx.x;
-  // with the CFL minimum the launch covers the whole block (the cells
-  // outside [lo, m - lo) only take their CFL term) and every thread takes
-  // part in the reduction
   const bool cfl = sp.cfl != nullptr;
-  const int blo = cfl ? 0 : lo;
-  const bool in_launch = cell_coords(t, blo, mx - 2 * blo, my - 2 * blo, mz - 2 * blo,
-                                     (sp.xcd_remap & 8) ? b.tile_rows : 0, k, j, i);
-  if (!in_launch && !cfl) return;
-  const bool in_box = in_launch && k >= lo && j >= lo && i >= lo && k < mz - lo &&
-                      j < my - lo && i < mx - lo;
   double local = 1.7976931348623157e308;
+  // the mixed path keeps the half-step state as FP32 primitives (b.Up)
+  // plus the FP64 centre E of the next stage's edge EMFs
+  constexpr bool HALF_F32 = std::is_same<FluxT, float>::value && !DE;
+  const bool to_f32 = HALF_F32 && !sp.is_final;
   auto body = [&]() {
     const long c = idx3(k, j, i, my, mx);
     double *const *Uo = sp.is_final ? b.U : b.Ut;
     double bc[3] = {0., 0., 0.};
     if (MHD) {
-      double *const *bf = sp.is_final ? b.bfi : b.bft;
-      bc[0] = 0.5 * (bf[0][idx3(k, j, i, my, mx + 1)] +
-                     bf[0][idx3(k, j, i + 1, my, mx + 1)]);
-      bc[1] = 0.5 * (bf[1][idx3(k, j, i, my + 1, mx)] +
-                     bf[1][idx3(k, j + 1, i, my + 1, mx)]);
-      bc[2] = 0.5 * (bf[2][idx3(k, j, i, my, mx)] +
-                     bf[2][idx3(k + 1, j, i, my, mx)]);
-      Uo[S_BX][c] = bc[0];
-      Uo[S_BY][c] = bc[1];
-      Uo[S_BZ][c] = bc[2];
+      double f[6];
+      face_b(k, j, i, f);
+      bc[0] = 0.5 * (f[0] + f[1]);
+      bc[1] = 0.5 * (f[2] + f[3]);
+      bc[2] = 0.5 * (f[4] + f[5]);
+      if (!to_f32) {
+        Uo[S_BX][c] = bc[0];
+        Uo[S_BY][c] = bc[1];
+        Uo[S_BZ][c] = bc[2];
+      }
     }
     const int ul = cs + 1;
     if (k < ul || j < ul || i < ul || k >= mz - ul || j >= my - ul ||
@@ -780,6 +870,24 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
       if (DE) finite = finite && isfinite(eint);
       debug_count(sp.dbg, DBG_NONFINITE, !finite);
     }
+    if (to_f32) {
+      float *const *Up = b.Up;
+      gst(Up[S_RHO], c, (float)new_rho);
+      gst(Up[S_VX], c, (float)v[0]);
+      gst(Up[S_VY], c, (float)v[1]);
+      gst(Up[S_VZ], c, (float)v[2]);
+      gst(Up[S_E], c, (float)prim_pressure<MHD>(new_rho, v[0], v[1], v[2], etot, bc[0], bc[1],
+                                                bc[2], sp.gamma - 1.0));
+      if (MHD) {
+        gst(Up[S_BX], c, (float)bc[0]);
+        gst(Up[S_BY], c, (float)bc[1]);
+        gst(Up[S_BZ], c, (float)bc[2]);
+        gst(b.Ech[0], c, center_efield(v[1], v[2], bc[1], bc[2]));
+        gst(b.Ech[1], c, center_efield(v[2], v[0], bc[2], bc[0]));
+        gst(b.Ech[2], c, center_efield(v[0], v[1], bc[0], bc[1]));
+      }
+      return;
+    }
     Uo[S_RHO][c] = new_rho;
     Uo[S_VX][c] = v[0];
     Uo[S_VY][c] = v[1];
@@ -831,6 +939,228 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     }
     if (b.pressure) b.pressure[c] = p;
   }
+  return local;
+}
+
+// The centre-B faces of a cell from the stage's face field in memory.
+template <typename FluxT>
+struct FaceBGlobal {
+  const BlockDesc<FluxT> &b;
+  double *const *bf;
+  __device__ __forceinline__ void operator()(int k, int j, int i, double f[6]) const {
+    const int mx = b.mx, my = b.my;
+    f[0] = bf[0][idx3(k, j, i, my, mx + 1)];
+    f[1] = bf[0][idx3(k, j, i + 1, my, mx + 1)];
+    f[2] = bf[1][idx3(k, j, i, my + 1, mx)];
+    f[3] = bf[1][idx3(k, j + 1, i, my + 1, mx)];
+    f[4] = bf[2][idx3(k, j, i, my, mx)];
+    f[5] = bf[2][idx3(k + 1, j, i, my, mx)];
+  }
+};
+
+template <typename FluxT, bool MHD, bool DE = false>
+__global__ void __launch_bounds__(256)
+k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  int bx, by;
+  wg_coords(sp.xcd_remap & 8, bx, by);
+  if (zero_step(sp, by)) return;
+  const BlockDesc<FluxT> &b = blocks[by];
+  const int mx = b.mx, my = b.my, mz = b.mz;
+  const int cs = sp.cur_stale;
+  const int lo = sp.is_final ? cs : cs + 1;
+  int k, j, i;
+  const long t = (long)bx * blockDim.x + threadIdx.x;
+  // with the CFL minimum the launch covers the whole block (the cells
+  // outside [lo, m - lo) only take their CFL term) and every thread takes
+  // part in the reduction
+  const bool cfl = sp.cfl != nullptr;
+  const int blo = cfl ? 0 : lo;
+  const bool in_launch = cell_coords(t, blo, mx - 2 * blo, my - 2 * blo, mz - 2 * blo,
+                                     (sp.xcd_remap & 8) ? b.tile_rows : 0, k, j, i);
+  if (!in_launch && !cfl) return;
+  const bool in_box = in_launch && k >= lo && j >= lo && i >= lo && k < mz - lo &&
+                      j < my - lo && i < mx - lo;
+  const FaceBGlobal<FluxT> fb{b, sp.is_final ? b.bfi : b.bft};
+  const double local = cell_stage<FluxT, MHD, DE>(b, sp, by, k, j, i, in_launch, in_box, fb);
+  if (cfl) block_min_to(sp.cfl, local);
+}
+
+// ----------------------------------------------------------------- k_ctu
+
+// Constrained transport and the cell update of one stage in one launch:
+// k_edge, k_faces and k_cells fused over tiles of CTU_TX x CTU_TY x CTU_TZ
+// cells of the k_cells launch box, one cell per thread, with the same
+// expressions in the same order (the same bits).  A workgroup
+//  1. computes into LDS the edge EMFs its faces need: E_a over the tile
+//     along a, and over the tile plus one edge below across a;
+//  2. computes into LDS the face-centred B of every face of its cells (low
+//     and high faces), from the edges in LDS; it writes the updated faces
+//     it owns (the low faces of its cells and, in the last tile along an
+//     axis, the face past the last cell); a face outside the update region
+//     keeps its start-of-step value;
+//  3. updates its cells as k_cells does, the centre B from the faces in LDS.
+// Neighbouring tiles read the start-of-step value of a face that its owner
+// may already have overwritten in place in the final stage: the first
+// stage stores a copy (bf0) of the faces it updates, a region that holds
+// the final stage's, and the final stage reads its base values there.
+constexpr int CTU_TX = 16, CTU_TY = 4, CTU_TZ = 4;
+constexpr int CTU_THREADS = CTU_TX * CTU_TY * CTU_TZ;
+__host__ __device__ constexpr int ctu_t(int d) { return d == 0 ? CTU_TX : (d == 1 ? CTU_TY : CTU_TZ); }
+// extent of the LDS edge array along axis a, direction d
+__host__ __device__ constexpr int ctu_eext(int a, int d) { return ctu_t(d) + (d != a); }
+// extent of the LDS face array of B_a, direction d
+__host__ __device__ constexpr int ctu_fext(int a, int d) { return ctu_t(d) + (d == a); }
+__host__ __device__ constexpr int ctu_ecount(int a) {
+  return ctu_eext(a, 0) * ctu_eext(a, 1) * ctu_eext(a, 2);
+}
+__host__ __device__ constexpr int ctu_fcount(int a) {
+  return ctu_fext(a, 0) * ctu_fext(a, 1) * ctu_fext(a, 2);
+}
+constexpr int CTU_NE = ctu_ecount(0) + ctu_ecount(1) + ctu_ecount(2);
+constexpr int CTU_NF = ctu_fcount(0) + ctu_fcount(1) + ctu_fcount(2);
+
+// LDS index of the edge along a at cell-shaped index p (tile origin o)
+__device__ __forceinline__ int ctu_eidx(int a, const int p[3], const int o[3]) {
+  int base = 0;
+  for (int x = 0; x < a; x++) base += ctu_ecount(x);
+  const int l0 = p[0] - o[0] + (a != 0), l1 = p[1] - o[1] + (a != 1),
+            l2 = p[2] - o[2] + (a != 2);
+  return base + (l2 * ctu_eext(a, 1) + l1) * ctu_eext(a, 0) + l0;
+}
+// LDS index of face B_a at cell-shaped index p (tile origin o)
+__device__ __forceinline__ int ctu_fidx(int a, int px, int py, int pz, const int o[3]) {
+  int base = 0;
+  for (int x = 0; x < a; x++) base += ctu_fcount(x);
+  return base + ((pz - o[2]) * ctu_fext(a, 1) + (py - o[1])) * ctu_fext(a, 0) + (px - o[0]);
+}
+
+// Tiles of the launch box [blo, m - blo) along each axis.
+__host__ __device__ inline void ctu_tiles(int mx, int my, int mz, int blo, int nt[3]) {
+  const int n[3] = {mx - 2 * blo, my - 2 * blo, mz - 2 * blo};
+  for (int d = 0; d < 3; d++) nt[d] = n[d] > 0 ? (n[d] + ctu_t(d) - 1) / ctu_t(d) : 0;
+}
+
+template <typename FluxT>
+struct FaceBLds {
+  const double *sF;
+  const int *o;
+  __device__ __forceinline__ void operator()(int k, int j, int i, double f[6]) const {
+    f[0] = sF[ctu_fidx(0, i, j, k, o)];
+    f[1] = sF[ctu_fidx(0, i + 1, j, k, o)];
+    f[2] = sF[ctu_fidx(1, i, j, k, o)];
+    f[3] = sF[ctu_fidx(1, i, j + 1, k, o)];
+    f[4] = sF[ctu_fidx(2, i, j, k, o)];
+    f[5] = sF[ctu_fidx(2, i, j, k + 1, o)];
+  }
+};
+
+template <typename FluxT, bool DE = false>
+__global__ void __launch_bounds__(CTU_THREADS)
+k_ctu(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  int bx, by;
+  wg_coords(sp.xcd_remap & 8, bx, by);
+  if (zero_step(sp, by)) return;
+  const BlockDesc<FluxT> &b = blocks[by];
+  const int mx = b.mx, my = b.my, mz = b.mz;
+  const int m[3] = {mx, my, mz};
+  const int cs = sp.cur_stale;
+  const int lo = sp.is_final ? cs : cs + 1;
+  const bool cfl = sp.cfl != nullptr;
+  const int blo = cfl ? 0 : lo;
+  int nt[3];
+  ctu_tiles(mx, my, mz, blo, nt);
+  // padding workgroups (launch rounded up to the XCDs) take part in nothing
+  if (bx >= nt[0] * nt[1] * nt[2]) return;
+  const int tix = bx % nt[0], tiy = (bx / nt[0]) % nt[1], tiz = bx / (nt[0] * nt[1]);
+  const int o[3] = {blo + tix * CTU_TX, blo + tiy * CTU_TY, blo + tiz * CTU_TZ};
+  const int hi[3] = {mx - blo, my - blo, mz - blo};  // end of the launch box
+  const long str[3] = {1, (long)mx, (long)mx * my};
+  const double dt = sp.dts ? sp.dts[by] * sp.dt_factor : sp.dt;
+  const int tid = threadIdx.x;
+
+  __shared__ double sE[CTU_NE];
+  __shared__ double sF[CTU_NF];
+
+  // 1. edges
+  double *const *Ecs = center_e(b, sp);
+  for (int e = tid; e < CTU_NE; e += CTU_THREADS) {
+    int a = 0, r = e;
+    if (r >= ctu_ecount(0)) {
+      r -= ctu_ecount(0);
+      a = 1;
+      if (r >= ctu_ecount(1)) {
+        r -= ctu_ecount(1);
+        a = 2;
+      }
+    }
+    const int ex = ctu_eext(a, 0), ey = ctu_eext(a, 1);
+    const int p[3] = {o[0] - (a != 0) + r % ex, o[1] - (a != 1) + (r / ex) % ey,
+                      o[2] - (a != 2) + r / (ex * ey)};
+    double val = 0.;
+    if (edge_in_region(a, p, m, cs)) {
+      const long c = idx3(p[2], p[1], p[0], my, mx);
+      if (a == 0)
+        val = edge_emf<0>(b, Ecs, c, str[1], str[2]);
+      else if (a == 1)
+        val = edge_emf<1>(b, Ecs, c, str[2], str[0]);
+      else
+        val = edge_emf<2>(b, Ecs, c, str[0], str[1]);
+    }
+    sE[e] = val;
+  }
+  __syncthreads();
+
+  // 2. faces
+  for (int f = tid; f < CTU_NF; f += CTU_THREADS) {
+    int a = 0, r = f;
+    if (r >= ctu_fcount(0)) {
+      r -= ctu_fcount(0);
+      a = 1;
+      if (r >= ctu_fcount(1)) {
+        r -= ctu_fcount(1);
+        a = 2;
+      }
+    }
+    const int fx = ctu_fext(a, 0), fy = ctu_fext(a, 1);
+    const int p[3] = {o[0] + r % fx, o[1] + (r / fx) % fy, o[2] + r / (fx * fy)};
+    const int aj = (a + 1) % 3, ak = (a + 2) % 3;
+    const int bnx = mx + (a == 0), bny = my + (a == 1);
+    const bool upd = p[a] >= cs + 1 && p[a] < m[a] - cs && p[aj] >= cs + 1 &&
+                     p[aj] < m[aj] - cs - 1 && p[ak] >= cs + 1 && p[ak] < m[ak] - cs - 1;
+    double val = 0.;
+    if (upd) {
+      const long cb = idx3(p[2], p[1], p[0], bny, bnx);
+      const double b0 = sp.is_final ? b.bf0[a][cb] : b.bfi[a][cb];
+      // edges at a-position p_a - 1/2 (cell-shaped index p - e_a)
+      int q[3] = {p[0], p[1], p[2]};
+      q[a] -= 1;
+      int qj[3] = {q[0], q[1], q[2]}, qk[3] = {q[0], q[1], q[2]};
+      qj[aj] -= 1;
+      qk[ak] -= 1;
+      val = face_b_update(b0, dt / b.cw[aj], dt / b.cw[ak], sE[ctu_eidx(ak, q, o)],
+                          sE[ctu_eidx(ak, qj, o)], sE[ctu_eidx(aj, q, o)],
+                          sE[ctu_eidx(aj, qk, o)]);
+      if (p[a] < o[a] + ctu_t(a) || p[a] == hi[a]) {
+        (sp.is_final ? b.bfi : b.bft)[a][cb] = val;
+        if (!sp.is_final) b.bf0[a][cb] = b0;
+      }
+    } else if (sp.is_final && p[a] <= m[a] && p[aj] < m[aj] && p[ak] < m[ak]) {
+      // outside the update region: the start-of-step value, never written
+      // in this stage
+      val = b.bfi[a][idx3(p[2], p[1], p[0], bny, bnx)];
+    }
+    sF[f] = val;
+  }
+  __syncthreads();
+
+  // 3. cells
+  const int i = o[0] + tid % CTU_TX, j = o[1] + (tid / CTU_TX) % CTU_TY,
+            k = o[2] + tid / (CTU_TX * CTU_TY);
+  const bool in_launch = i < hi[0] && j < hi[1] && k < hi[2];
+  const bool in_box = in_launch && k >= lo && j >= lo && i >= lo && k < mz - lo &&
+                      j < my - lo && i < mx - lo;
+  const FaceBLds<FluxT> fb{sF, o};
+  const double local = cell_stage<FluxT, true, DE>(b, sp, by, k, j, i, in_launch, in_box, fb);
   if (cfl) block_min_to(sp.cfl, local);
 }
 
@@ -845,7 +1175,8 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
 // butterfly, then LDS across the workgroup, then one atomic per workgroup.
 // Dual-energy parameters of the CFL kernel's synchronisation.  time (device,
 // may be null) and t_stop: a device-driven cycle that starts at or past the
-// stop time is zero-length and leaves the energies alone.
+// stop time is zero-length and changes nothing: the kernel leaves the
+// energies and the pressure field alone and only takes the minimum.
 struct DualParams {
   double pressure_floor, eta, ggm1;
   const double *time;
@@ -859,7 +1190,7 @@ k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
   const BlockDesc<FluxT> &b = blocks[blockIdx.y];
   const long n = (long)b.mx * b.my * b.mz;
   double local = 1.7976931348623157e308;
-  const bool live = !DE || dp.time == nullptr || *dp.time < dp.t_stop;
+  const bool live = dp.time == nullptr || *dp.time < dp.t_stop;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (long)gridDim.x * blockDim.x) {
     double d = b.U[S_RHO][c];
@@ -886,7 +1217,7 @@ k_dt(const BlockDesc<FluxT> *__restrict__ blocks, double gamma,
     } else {
       p = cell_pressure<MHD>(d, vx, vy, vz, b.U[S_E][c], bx, by, bz, gamma);
     }
-    if (b.pressure) b.pressure[c] = p;
+    if (b.pressure && live) b.pressure[c] = p;
     local = dmin(local, cfl_speed<MHD>(d, vx, vy, vz, bx, by, bz, p, gamma, b.cw));
   }
   block_min_to(result, local);
@@ -927,6 +1258,8 @@ struct RefreshGrid {
   int bc[3];     // 0 periodic, 1 outflow
   int face0;     // table index of bfieldi_x (then _y, _z); every other
                  // field is cell-centred
+  const double *dt;  // device dt of the cycle, or null: a cycle whose dt is
+                     // 0 (past its stop time) refreshes nothing
 };
 
 __device__ __forceinline__ bool refresh_axis(const RefreshGrid &G, int a,
@@ -1008,6 +1341,7 @@ __device__ __forceinline__ bool ghost_coords(long t, const int ext[3],
 // grid: x = ghost elements, y = block, z = field.
 static __global__ void __launch_bounds__(256)
 k_refresh(double *const *__restrict__ table, RefreshGrid G) {
+  if (G.dt != nullptr && *G.dt == 0.0) return;
   const int nb = G.nb[0] * G.nb[1] * G.nb[2];
   const int bidx = blockIdx.y;
   const int f = blockIdx.z;

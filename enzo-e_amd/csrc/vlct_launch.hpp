@@ -11,6 +11,7 @@
 #include <map>
 #include <string>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/enzoe_vlct.h"
@@ -78,6 +79,10 @@ struct enzoe_vlct_method {
   int xcd_remap = 2;
   double theta = 1.5, courant = 0.3, gamma = 5.0 / 3.0;
   double dfloor = 1e-200, pfloor = 1e-200;
+  // MHD stages run constrained transport and the cell update as one fused
+  // launch (k_ctu) instead of k_edge, k_faces and k_cells (same bits); off
+  // by default: on the measured shapes the three launches are faster
+  int ctu = 0;
   // dual-energy formalism ("modern" formulation) and its eta
   int dual = 0;
   double eta = 0.001;
@@ -123,6 +128,10 @@ struct enzoe_vlct_method {
 namespace enzoe {
 
 
+// The fused CT update needs the two stages of the VL scheme (its final
+// stage reads the first stage's copy of the start-of-step faces).
+inline bool use_ctu(const enzoe_vlct_method *m) { return m->mhd && m->ctu && m->nstages == 2; }
+
 inline int total_staling(int r) { return r == RC_NN ? 1 : 2; }
 inline int immediate_staling(int r) { return r == RC_NN ? 0 : 1; }
 
@@ -133,17 +142,29 @@ inline int parse_recon(const std::string &n, int *out) {
   return ENZOE_ERR_CONFIG;
 }
 
+// The mixed path (FP32 fluxes, no dual energy) keeps its half-step state as
+// FP32 primitives.
+template <typename FluxT>
+inline constexpr bool half_f32(int dual) {
+  return std::is_same<FluxT, float>::value && !dual;
+}
+
 // bytes of scratch per block
 template <typename FluxT>
-inline size_t scratch_per_block(int mx, int my, int mz, int mhd, int dual = 0) {
+inline size_t scratch_per_block(int mx, int my, int mz, int mhd, int dual = 0, int ctu = 0) {
   size_t ncc = (size_t)mx * my * mz;
   size_t nq = mhd ? 8 : 5;
   size_t bytes = 0;
-  bytes += nq * ncc * sizeof(double);                  // Ut
+  // half-step state: FP64 integration quantities (Ut), or in the mixed path
+  // FP32 primitives (Up)
+  bytes += nq * ncc * (half_f32<FluxT>(dual) ? sizeof(float) : sizeof(double));
   if (mhd) {
-    bytes += ((size_t)mz * my * (mx + 1) + (size_t)mz * (my + 1) * mx +
-              (size_t)(mz + 1) * my * mx) * sizeof(double);  // bft
+    const size_t faces = ((size_t)mz * my * (mx + 1) + (size_t)mz * (my + 1) * mx +
+                          (size_t)(mz + 1) * my * mx) * sizeof(double);
+    bytes += faces;                                    // bft
+    if (ctu) bytes += faces;                           // bf0
     bytes += 6 * ncc * sizeof(double);                 // E, Ec
+    if (half_f32<FluxT>(dual)) bytes += 3 * ncc * sizeof(double);  // Ech
   }
   bytes += 3 * (mhd ? 7 : 5) * ncc * sizeof(FluxT);    // F
   if (dual) bytes += 7 * ncc * sizeof(double);         // eit, Fe, Vb
@@ -243,14 +264,24 @@ void fill_desc_at(const enzoe_vlct_method *m, char *base, const enzoe_block *blo
     const int nq = m->mhd ? 8 : 5;
     for (int q = 0; q < S_N; q++) {
       if (!m->mhd && q >= S_BX) continue;
-      d.Ut[q] = (double *)base;
-      base += ncc * sizeof(double);
+      if (half_f32<FluxT>(m->dual)) {
+        d.Up[q] = (float *)base;
+        base += ncc * sizeof(float);
+      } else {
+        d.Ut[q] = (double *)base;
+        base += ncc * sizeof(double);
+      }
     }
     (void)nq;
     if (m->mhd) {
       d.bft[0] = (double *)base; base += (size_t)b.mz * b.my * (b.mx + 1) * sizeof(double);
       d.bft[1] = (double *)base; base += (size_t)b.mz * (b.my + 1) * b.mx * sizeof(double);
       d.bft[2] = (double *)base; base += (size_t)(b.mz + 1) * b.my * b.mx * sizeof(double);
+      if (m->ctu) {
+        d.bf0[0] = (double *)base; base += (size_t)b.mz * b.my * (b.mx + 1) * sizeof(double);
+        d.bf0[1] = (double *)base; base += (size_t)b.mz * (b.my + 1) * b.mx * sizeof(double);
+        d.bf0[2] = (double *)base; base += (size_t)(b.mz + 1) * b.my * b.mx * sizeof(double);
+      }
       for (int a = 0; a < 3; a++) {
         d.E[a] = (double *)base;
         base += ncc * sizeof(double);
@@ -259,6 +290,11 @@ void fill_desc_at(const enzoe_vlct_method *m, char *base, const enzoe_block *blo
         d.Ec[a] = (double *)base;
         base += ncc * sizeof(double);
       }
+      if (half_f32<FluxT>(m->dual))
+        for (int a = 0; a < 3; a++) {
+          d.Ech[a] = (double *)base;
+          base += ncc * sizeof(double);
+        }
     }
     for (int dd = 0; dd < 3; dd++) {
       for (int q = 0; q < S_N; q++) {
@@ -292,33 +328,42 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
   fill_desc_at<FluxT>(m, (char *)m->scratch, blocks, b0, b1, out.data(), accel);
 }
 
-template <typename FluxT, int RECON, int RIEMANN, bool MHD, bool DE>
+template <typename FluxT, int RECON, int RIEMANN, bool MHD, bool DE, bool PRIMF>
 void launch_flux(const BlockDesc<FluxT> *d, int nb, long nthreads,
                  StageParams sp, hipStream_t s) {
   dim3 grid(nblk8(nthreads, 256), nb);
-  hipLaunchKernelGGL((k_flux<FluxT, FluxT, RECON, RIEMANN, MHD, DE>), grid,
+  hipLaunchKernelGGL((k_flux<FluxT, FluxT, RECON, RIEMANN, MHD, DE, PRIMF>), grid,
                      dim3(256), 0, s, d, sp);
 }
 
+template <typename FluxT, bool MHD, bool DE, bool PRIMF>
+void dispatch_flux_p(const enzoe_vlct_method *m, int recon,
+                     const BlockDesc<FluxT> *d, int nb, long nthreads,
+                     StageParams sp, hipStream_t s) {
+  constexpr int RS = MHD ? RS_HLLD : RS_HLLC;
+  if (MHD && m->riemann != RS_HLLD) {
+    if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
+    else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
+    else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
+    return;
+  }
+  if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
+  else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
+  else launch_flux<FluxT, RC_PLM_ATHENA, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
+}
+
+// primf: the stage reads the mixed path's FP32 primitive half-step state
 template <typename FluxT, bool MHD, bool DE>
-void dispatch_flux(const enzoe_vlct_method *m, int recon,
+void dispatch_flux(const enzoe_vlct_method *m, int recon, bool primf,
                    const BlockDesc<FluxT> *d, int nb, long nthreads,
                    StageParams sp, hipStream_t s) {
-  if (MHD) {
-    if (m->riemann == RS_HLLD) {
-      if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLD, MHD, DE>(d, nb, nthreads, sp, s);
-      else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLD, MHD, DE>(d, nb, nthreads, sp, s);
-      else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLD, MHD, DE>(d, nb, nthreads, sp, s);
-    } else {
-      if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLE, MHD, DE>(d, nb, nthreads, sp, s);
-      else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLE, MHD, DE>(d, nb, nthreads, sp, s);
-      else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD, DE>(d, nb, nthreads, sp, s);
+  if constexpr (half_f32<FluxT>(DE ? 1 : 0)) {
+    if (primf) {
+      dispatch_flux_p<FluxT, MHD, DE, true>(m, recon, d, nb, nthreads, sp, s);
+      return;
     }
-  } else {
-    if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLC, MHD, DE>(d, nb, nthreads, sp, s);
-    else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLC, MHD, DE>(d, nb, nthreads, sp, s);
-    else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLC, MHD, DE>(d, nb, nthreads, sp, s);
   }
+  dispatch_flux_p<FluxT, MHD, DE, false>(m, recon, d, nb, nthreads, sp, s);
 }
 
 // gamma (gamma - 1) as the reference's synchronisation holds it: a float
@@ -391,25 +436,33 @@ int launch_chunks(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int ch
       const int cs = sp.cur_stale;
       {
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm", timers);
-        dispatch_flux<FluxT, MHD, DE>(m, recon, d, nb, cube(mx, my, mz, cs, cs), sp, s);
+        dispatch_flux<FluxT, MHD, DE>(m, recon, stage > 0 && half_f32<FluxT>(DE ? 1 : 0), d,
+                                      nb, cube(mx, my, mz, cs, cs), sp, s);
       }
-      if (MHD) {
-        {
-          KernelTimer kt(s, "k_edge", timers);
-          hipLaunchKernelGGL((k_edge<FluxT>), dim3(nblk8(cube(mx, my, mz, cs, cs), 256), nb),
-                             dim3(256), 0, s, d, sp);
+      // with the fused CFL the final update's launch covers the block
+      const int lo = sp.cfl ? 0 : (is_final ? cs : cs + 1);
+      if (MHD && use_ctu(m)) {
+        KernelTimer kt(s, "k_ctu", timers);
+        int nt[3];
+        ctu_tiles(mx, my, mz, lo, nt);
+        const long tiles = (long)nt[0] * nt[1] * nt[2];
+        hipLaunchKernelGGL((k_ctu<FluxT, DE>), dim3(nblk8(tiles * CTU_THREADS, CTU_THREADS), nb),
+                           dim3(CTU_THREADS), 0, s, d, sp);
+      } else {
+        if (MHD) {
+          {
+            KernelTimer kt(s, "k_edge", timers);
+            hipLaunchKernelGGL((k_edge<FluxT>), dim3(nblk8(cube(mx, my, mz, cs, cs), 256), nb),
+                               dim3(256), 0, s, d, sp);
+          }
+          {
+            KernelTimer kt(s, "k_faces", timers);
+            hipLaunchKernelGGL((k_faces<FluxT>),
+                               dim3(nblk8(cube(mx, my, mz, cs + 1, cs), 256), nb), dim3(256),
+                               0, s, d, sp);
+          }
         }
-        {
-          KernelTimer kt(s, "k_faces", timers);
-          hipLaunchKernelGGL((k_faces<FluxT>),
-                             dim3(nblk8(cube(mx, my, mz, cs + 1, cs), 256), nb), dim3(256),
-                             0, s, d, sp);
-        }
-      }
-      {
         KernelTimer kt(s, "k_cells", timers);
-        // with the fused CFL the final update's launch covers the block
-        const int lo = sp.cfl ? 0 : (is_final ? cs : cs + 1);
         hipLaunchKernelGGL((k_cells<FluxT, MHD, DE>),
                            dim3(nblk8(cube(mx, my, mz, lo, lo), 256), nb), dim3(256), 0,
                            s, d, sp);
@@ -449,7 +502,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   unsigned long long *cfl = nullptr) {
   const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
   if (DE != (m->dual != 0)) return ENZOE_ERR_CONFIG;
-  const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd, m->dual);
+  const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd, m->dual, m->ctu);
   // chunk size: as many blocks as fit the scratch budget (at least 1); the
   // chunks run one after the other on the stream and share the pool
   if (m->scratch_budget == 0) {

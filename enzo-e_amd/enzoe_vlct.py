@@ -68,6 +68,12 @@ class Diagnostics(ctypes.Structure):
                 ("max_abs_bface", ctypes.c_double), ("nonfinite", ctypes.c_longlong)]
 
 
+class StateL1(ctypes.Structure):
+    _fields_ = [("diff", ctypes.c_double * 8), ("ref", ctypes.c_double * 8),
+                ("velocity_magnitude", ctypes.c_double),
+                ("bfield_magnitude", ctypes.c_double)]
+
+
 DEBUG_COUNTS = ("nonfinite", "density_floor", "energy_floor", "reconstruction_floor")
 
 _lib = None
@@ -153,9 +159,12 @@ def lib():
                                          ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), vp]
         L.enzoe_vlct_set_debug_checks.argtypes = [vp, ctypes.c_int]
+        L.enzoe_vlct_set_fused_ct.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_debug_counts.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_ulonglong)]
         L.enzoe_vlct_diagnostics.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int, vp,
                                              ctypes.POINTER(Diagnostics)]
+        L.enzoe_vlct_state_l1.argtypes = [vp, ctypes.POINTER(Block), ctypes.POINTER(Block),
+                                          ctypes.c_int, vp, ctypes.POINTER(StateL1)]
         L.enzoe_set_profiling.argtypes = [ctypes.c_int]
         L.enzoe_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp,
                                            ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -353,6 +362,33 @@ class VlctMethod:
                 "min_density": d.min_density, "min_pressure": d.min_pressure,
                 "max_abs_divb": d.max_abs_divb, "max_abs_bface": d.max_abs_bface,
                 "nonfinite": int(d.nonfinite)}
+
+    def set_fused_ct(self, on):
+        """One fused launch for the constrained transport and cell update of
+        an MHD stage (the default) or three launches through memory."""
+        check(lib().enzoe_vlct_set_fused_ct(self.handle, 1 if on else 0),
+              "enzoe_vlct_set_fused_ct")
+
+    def state_l1(self, blocks, ref_blocks, stream=None):
+        """L1 of the blocks' cell-centred fields against ref_blocks' (same
+        shapes), over the active cells, on the device.  Returns (per_field,
+        by_magnitude): per field sum|a - b| / sum|b|, and the same with the
+        denominator of a velocity or B component the sum of its vector's
+        magnitude |b_vec|."""
+        out = StateL1()
+        check(lib().enzoe_vlct_state_l1(self.handle, self._blocks(blocks),
+                                        self._blocks(ref_blocks), len(blocks), stream,
+                                        ctypes.byref(out)), "enzoe_vlct_state_l1")
+        names = CC_FIELDS if self.mhd else CC_FIELDS[:5]
+        per, mag = {}, {}
+        for q, nm in enumerate(names):
+            d, r = out.diff[q], out.ref[q]
+            per[nm] = d / r if r > 0 else d
+            vec = {"velocity": out.velocity_magnitude,
+                   "bfield": out.bfield_magnitude}.get(nm.rsplit("_", 1)[0])
+            den = vec if vec else r
+            mag[nm] = d / den if den > 0 else d
+        return per, mag
 
     def autotune(self, blocks, dt, stream=None):
         """Picks the launch order of the stage kernels for this block shape

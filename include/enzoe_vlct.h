@@ -154,8 +154,9 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * without a host round trip: reads the time from *d_time (device), writes
  * dt = min(courant * min_cfl, t_stop - t) into d_dts[0..ndts) (device) and
  * advances *d_time by dt.  For device-driven cycles (graph capture); a
- * cycle that starts at t_stop is zero-length and, with dual energy, leaves
- * the energies unsynchronised.                                              */
+ * cycle that starts at t_stop is zero-length and changes nothing: the
+ * pressure field is not written and, with dual energy, the energies are
+ * left unsynchronised.                                                      */
 int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                                double *d_time, double t_stop, double *d_dts,
                                int ndts, enzoe_stream_t stream);
@@ -611,6 +612,15 @@ int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
  * energy floor (EnzoPhysicsFluidProps.cpp:278-285) and the reconstruction's
  * density/pressure floors (EnzoReconstructorPLM.hpp:219-220).  The C++ layer
  * turns a nonzero count into the reference's ERROR.                         */
+/* Launch structure of the MHD stages (no reference counterpart: the
+ * reference runs every operation as its own loop).  on: the constrained
+ * transport and the cell update of a stage run as one fused launch over
+ * tiles of cells, the edge EMFs and face-centred B of a tile held in LDS;
+ * off (the default, faster on the measured shapes): three launches (edge
+ * EMFs, face update, cell update) through memory.  Both give the same bits.
+ * Not to be changed while work of the method is queued.                     */
+int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int on);
+
 #define ENZOE_DEBUG_NONFINITE 0
 #define ENZOE_DEBUG_DENSITY_FLOOR 1
 #define ENZOE_DEBUG_ENERGY_FLOOR 2
@@ -628,8 +638,10 @@ int enzoe_vlct_debug_counts(enzoe_vlct_t m, enzoe_stream_t stream,
  * density, momentum and total energy density (rho e_tot), the smallest
  * density and pressure, the largest |div B| of the face field times the
  * smallest cell width and the largest |face B|; over every cell, ghosts
- * included, the number of non-finite field values.  Sums are accumulated
- * with atomics (their last bits depend on the order).  Synchronises.       */
+ * included, the number of non-finite field values.  Sums are compensated
+ * per workgroup and added in a fixed order on the host (the same bits on
+ * every run, within a few units in the last place of the exact sum).
+ * Synchronises.                                                             */
 typedef struct {
   double mass, momentum[3], energy;
   double min_density, min_pressure;
@@ -638,6 +650,23 @@ typedef struct {
 } enzoe_diagnostics;
 int enzoe_vlct_diagnostics(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                            enzoe_stream_t stream, enzoe_diagnostics *out);
+
+/* Difference of two states of the same block list (a[i] against b[i], equal
+ * shapes): over the active cells, per cell-centred field (density,
+ * velocity_x..z, total_energy, bfield_x..z; the B slots are 0 for hydro)
+ * the sums of |a - b| and of |b|, and the sums of the magnitudes |v_b| and
+ * |B_b| of b's velocity and cell-centred B.  The L1 error of a field
+ * against b is diff/ref, or, for a vector component, diff over the sum of
+ * its vector's magnitude (a component that is zero in b has no scale of its
+ * own).  Compensated per workgroup, added in a fixed order on the host.
+ * Does not correspond to a reference function (the reference computes L1
+ * norms in Python, tools/l1_error_norm.py:195-234).  Synchronises.          */
+typedef struct {
+  double diff[8], ref[8];
+  double velocity_magnitude, bfield_magnitude;
+} enzoe_state_l1;
+int enzoe_vlct_state_l1(enzoe_vlct_t m, const enzoe_block *a, const enzoe_block *b,
+                        int n, enzoe_stream_t stream, enzoe_state_l1 *out);
 
 /* Per-kernel average duration (ms) of the launches made while profiling was
  * on (enzoe_set_profiling(1)), measured with HIP events recorded on the
