@@ -64,6 +64,33 @@ std::string lower(const char *s, const char *dflt) {
   return r;
 }
 
+// ------------------------------------------------- allocation-free compute
+// The workspace of enzoe_vlct_compute_noalloc: one fixed array per device
+// (each GPU the library runs on holds it until the process ends).  Host
+// code never uses the array's name as an address: noalloc_workspace takes
+// its device address once per device, with that device current.
+constexpr size_t NOALLOC_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+std::mutex g_noalloc_mtx;
+std::map<int, char *> g_noalloc_ptr;
+
+// The current device's address of the workspace.
+int noalloc_workspace(char **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ENZOE_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_noalloc_mtx);
+  auto it = g_noalloc_ptr.find(dev);
+  if (it == g_noalloc_ptr.end()) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return ENZOE_ERR_HIP;
+    it = g_noalloc_ptr.emplace(dev, (char *)p).first;
+  }
+  *out = it->second;
+  return ENZOE_OK;
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
 }
@@ -483,6 +510,21 @@ int enzoe_vlct_compute(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   if (!m || !blocks || n <= 0) return ENZOE_ERR_ARG;
   return compute_runs(m, blocks, n, dt, nullptr, (hipStream_t)stream);
 }
+
+int enzoe_vlct_compute_noalloc(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                               double dt, enzoe_stream_t stream) {
+  if (!m || !blocks || n <= 0) return ENZOE_ERR_ARG;
+  char *ws = nullptr;
+  if (int rc = noalloc_workspace(&ws)) return rc;
+  m->arena = ws;
+  m->arena_bytes = NOALLOC_BYTES;
+  const int rc = compute_runs(m, blocks, n, dt, nullptr, (hipStream_t)stream);
+  m->arena = nullptr;
+  m->arena_bytes = 0;
+  return rc;
+}
+
+size_t enzoe_vlct_noalloc_bytes(void) { return NOALLOC_BYTES; }
 
 int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                           const double *dt, int dt_on_device,

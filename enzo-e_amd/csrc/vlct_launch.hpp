@@ -103,6 +103,10 @@ struct enzoe_vlct_method {
   std::vector<double *> last_rtab;
   size_t scratch_budget = 0;       // bytes of the chunk scratch pool
   bool dry = false;                // prepare buffers and tables, launch nothing
+  // fixed workspace of the allocation-free entry point while it runs (the
+  // descriptor table, then the stage scratch), or null
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
   // launch-order mask chosen by enzoe_vlct_autotune per block shape, and a
   // mask forced while the autotuner times a candidate (-1: none)
   std::map<long long, int> tuned_mask;
@@ -503,6 +507,24 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
   const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
   if (DE != (m->dual != 0)) return ENZOE_ERR_CONFIG;
   const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd, m->dual, m->ctu);
+  if (m->arena) {
+    // the allocation-free path: descriptor table and scratch in the fixed
+    // workspace; nothing is allocated, nothing cached across calls
+    const size_t dbytes = ((size_t)n * sizeof(BlockDesc<FluxT>) + 255) & ~(size_t)255;
+    if (dbytes >= m->arena_bytes || m->arena_bytes - dbytes < per) return ENZOE_ERR_SIZE;
+    const int chunk =
+        (int)std::max<size_t>(1, std::min<size_t>(n, (m->arena_bytes - dbytes) / per));
+    std::vector<BlockDesc<FluxT>> desc(n);
+    for (int c0 = 0; c0 < n; c0 += chunk)
+      fill_desc_at<FluxT>(m, m->arena + dbytes, blocks, c0, std::min(n, c0 + chunk),
+                          desc.data() + c0, accel);
+    if (hipMemcpyAsync(m->arena, desc.data(), n * sizeof(BlockDesc<FluxT>),
+                       hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return ENZOE_ERR_HIP;
+    return launch_chunks<FluxT, MHD, DE>(m, blocks, n, chunk, (const BlockDesc<FluxT> *)m->arena,
+                                         dt, dts, s, st, cfl, false);
+  }
   // chunk size: as many blocks as fit the scratch budget (at least 1); the
   // chunks run one after the other on the stream and share the pool
   if (m->scratch_budget == 0) {

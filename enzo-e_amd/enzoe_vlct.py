@@ -160,6 +160,9 @@ def lib():
                                          ctypes.POINTER(ctypes.c_int), vp]
         L.enzoe_vlct_set_debug_checks.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_set_fused_ct.argtypes = [vp, ctypes.c_int]
+        L.enzoe_vlct_compute_noalloc.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                                 ctypes.c_double, vp]
+        L.enzoe_vlct_noalloc_bytes.restype = ctypes.c_size_t
         L.enzoe_vlct_debug_counts.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_ulonglong)]
         L.enzoe_vlct_diagnostics.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int, vp,
                                              ctypes.POINTER(Diagnostics)]

@@ -33,6 +33,7 @@ const char *status_text(int rc) {
     case ENZOE_ERR_HIP: return "HIP runtime error";
     case ENZOE_ERR_ALLOC: return "device allocation failed";
     case ENZOE_ERR_STATE: return "invalid state";
+    case ENZOE_ERR_SIZE: return "input larger than the fixed workspace";
     default: return "unknown status";
   }
 }

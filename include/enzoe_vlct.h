@@ -32,6 +32,8 @@ extern "C" {
 #define ENZOE_ERR_HIP 3     /* a HIP runtime call failed                   */
 #define ENZOE_ERR_ALLOC 4   /* device allocation failed                    */
 #define ENZOE_ERR_STATE 5   /* call out of order (CT state machine)        */
+#define ENZOE_ERR_SIZE 6    /* input larger than the fixed workspace of
+                               enzoe_vlct_compute_noalloc                  */
 
 /* stream handle: a hipStream_t passed as an opaque pointer (NULL = default) */
 typedef void *enzoe_stream_t;
@@ -129,6 +131,22 @@ int enzoe_vlct_compute_timestep(enzoe_vlct_t m, const enzoe_block *blocks,
  * transport and the conserved update.  Asynchronous on `stream`.           */
 int enzoe_vlct_compute(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                        double dt, enzoe_stream_t stream);
+
+/* enzoe_vlct_compute that allocates nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  The block descriptor
+ * table and the stage scratch (half-step state, fluxes, EMFs) live in one
+ * fixed device array of the library, enzoe_vlct_noalloc_bytes() bytes
+ * (64 MiB) on every GPU it runs on, held until the process ends; blocks
+ * are taken in chunks that fit it.  ENZOE_ERR_SIZE, before anything is
+ * launched, when one block's scratch plus the descriptor table does not
+ * fit (a 32^3 FP64 MHD block needs about 17 MB, a 64^3 one about 110 MB).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one
+ * GPU (two streams, two threads) race.  Results equal enzoe_vlct_compute's
+ * bit for bit.  Synchronises the stream (the descriptor upload).           */
+int enzoe_vlct_compute_noalloc(enzoe_vlct_t m, const enzoe_block *blocks, int n,
+                               double dt, enzoe_stream_t stream);
+size_t enzoe_vlct_noalloc_bytes(void);
 
 /* As enzoe_vlct_compute with one dt per block (blocks of different
  * refinement levels advanced together under sub-cycling, each by its own
