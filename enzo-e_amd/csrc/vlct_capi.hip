@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -598,19 +599,12 @@ namespace {
 // list changed, so the call makes no host round trip otherwise.
 // d_time / t_stop: the device time of a device-driven cycle and its stop
 // time (a cycle past it leaves the dual-energy fields alone), or null.
-int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_t s,
-              const double *d_time = nullptr, double t_stop = 0.0) {
-  for (int i = 0; i < n; i++)
-    if (int rc = check_block(m, blocks[i])) return rc;
-  if (!m->d_dt &&
-      hipMalloc((void **)&m->d_dt, sizeof(unsigned long long)) != hipSuccess) {
-    m->d_dt = nullptr;
-    return ENZOE_ERR_ALLOC;
-  }
-  // descriptors for the dt kernel: only U, pressure, shape and widths used
-  std::vector<BlockDesc<double>> desc(n);
+// Descriptors of the CFL kernel for blocks [0, n): only U, pressure, the
+// internal energy, shape and widths are used.
+void fill_dt_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int n,
+                  BlockDesc<double> *out) {
   for (int i = 0; i < n; i++) {
-    BlockDesc<double> &d = desc[i];
+    BlockDesc<double> &d = out[i];
     std::memset(&d, 0, sizeof(d));
     const enzoe_block &b = blocks[i];
     d.U[S_RHO] = b.density;
@@ -623,6 +617,46 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
     d.mx = b.mx; d.my = b.my; d.mz = b.mz;
     for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
   }
+}
+
+// The CFL reduction of n blocks whose descriptors are table[0 .. n) (device)
+// into *target: its initialisation, then the kernel.
+int launch_dt_on(const enzoe_vlct_method *m, const BlockDesc<double> *table, int n,
+                 long maxcells, unsigned long long *target, hipStream_t s,
+                 const double *d_time, double t_stop) {
+  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, target);
+  KernelTimer kt(s, "k_dt");
+  // enough workgroups to fill the device; each thread strides over cells
+  const unsigned wg = (unsigned)std::max<long>(1, std::min<long>(nblk(maxcells, 256),
+                                                                 std::max<long>(1, 2048 / n)));
+  dim3 grid(wg, n);
+  const DualParams dp = {m->pfloor, m->eta, sync_ggm1(m->gamma), d_time, t_stop};
+  if (m->mhd && m->dual)
+    hipLaunchKernelGGL((k_dt<double, true, true>), grid, dim3(256), 0, s, table, m->gamma,
+                       target, dp);
+  else if (m->mhd)
+    hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s, table, m->gamma, target,
+                       dp);
+  else if (m->dual)
+    hipLaunchKernelGGL((k_dt<double, false, true>), grid, dim3(256), 0, s, table, m->gamma,
+                       target, dp);
+  else
+    hipLaunchKernelGGL((k_dt<double, false>), grid, dim3(256), 0, s, table, m->gamma, target,
+                       dp);
+  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+}
+
+int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_t s,
+              const double *d_time = nullptr, double t_stop = 0.0) {
+  for (int i = 0; i < n; i++)
+    if (int rc = check_block(m, blocks[i])) return rc;
+  if (!m->d_dt &&
+      hipMalloc((void **)&m->d_dt, sizeof(unsigned long long)) != hipSuccess) {
+    m->d_dt = nullptr;
+    return ENZOE_ERR_ALLOC;
+  }
+  std::vector<BlockDesc<double>> desc(n);
+  fill_dt_desc(m, blocks, n, desc.data());
   const size_t dbytes = desc.size() * sizeof(BlockDesc<double>);
   const char *raw = (const char *)desc.data();
   if (m->last_dt_desc.size() != dbytes ||
@@ -637,30 +671,11 @@ int launch_dt(enzoe_vlct_method *m, const enzoe_block *blocks, int n, hipStream_
   // with dual energy the reduction also writes the energies: a dry call
   // (graph set-up) uploads the table only
   if (m->dry) return ENZOE_OK;
-  hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, m->d_dt);
   long maxcells = 0;
   for (int i = 0; i < n; i++)
     maxcells = std::max<long>(maxcells, (long)blocks[i].mx * blocks[i].my * blocks[i].mz);
-  KernelTimer kt(s, "k_dt");
-  // enough workgroups to fill the device; each thread strides over cells
-  const unsigned wg = (unsigned)std::max<long>(1, std::min<long>(nblk(maxcells, 256),
-                                                                 std::max<long>(1, 2048 / n)));
-  dim3 grid(wg, n);
-  const BlockDesc<double> *dd = (const BlockDesc<double> *)m->d_ptrtab;
-  const DualParams dp = {m->pfloor, m->eta, sync_ggm1(m->gamma), d_time, t_stop};
-  if (m->mhd && m->dual)
-    hipLaunchKernelGGL((k_dt<double, true, true>), grid, dim3(256), 0, s, dd, m->gamma,
-                       m->d_dt, dp);
-  else if (m->mhd)
-    hipLaunchKernelGGL((k_dt<double, true>), grid, dim3(256), 0, s, dd, m->gamma, m->d_dt,
-                       dp);
-  else if (m->dual)
-    hipLaunchKernelGGL((k_dt<double, false, true>), grid, dim3(256), 0, s, dd, m->gamma,
-                       m->d_dt, dp);
-  else
-    hipLaunchKernelGGL((k_dt<double, false>), grid, dim3(256), 0, s, dd, m->gamma, m->d_dt,
-                       dp);
-  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+  return launch_dt_on(m, (const BlockDesc<double> *)m->d_ptrtab, n, maxcells, m->d_dt, s,
+                      d_time, t_stop);
 }
 
 }  // namespace
@@ -776,11 +791,23 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     rc = ENZOE_ERR_HIP;
   const bool prof = g_profiling;
   g_profiling = false;
-  int best = m->xcd_remap;
+  int best = m->xcd_remap, best_ctu = 0;
   float best_ms = 1e30f;
-  // every combination of XCD-contiguous order over the four stage kernels
-  for (int mask = 0; mask < 16 && rc == ENZOE_OK; mask++) {
+  // the launch structures: the three CT/update launches with every
+  // combination of XCD-contiguous order over the four stage kernels, and
+  // (MHD, setting "auto") the fused launch with each tile shape and the
+  // orders of its two kernels (k_flux bit 0, k_ctu bit 3)
+  std::vector<std::pair<int, int>> cand;  // (fused-CT variant, mask)
+  const int fixed = (m->mhd && m->nstages == 2) ? m->ctu : 0;
+  for (int v = 0; v <= 3; v++) {
+    if (fixed >= 0 && v != fixed) continue;
+    for (int mask = 0; mask < 16; mask++)
+      if (v == 0 || (mask & 6) == 0) cand.push_back({v, mask});
+  }
+  for (size_t ci = 0; ci < cand.size() && rc == ENZOE_OK; ci++) {
+    const int mask = cand[ci].second;
     m->force_mask = mask;
+    m->force_ctu = cand[ci].first;
     rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);  // warm-up
     if (rc) break;
     (void)hipEventRecord(e0, s);
@@ -796,9 +823,11 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     if (ms < best_ms) {
       best_ms = ms;
       best = mask;
+      best_ctu = cand[ci].first;
     }
   }
   m->force_mask = -1;
+  m->force_ctu = -1;
   g_profiling = prof;
   (void)hipStreamSynchronize(s);
   if (e0) (void)hipEventDestroy(e0);
@@ -806,7 +835,15 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   (void)hipFree(pool);
   if (rc) return rc;
   m->tuned_mask[shape_key(mx, my, mz)] = best;
+  m->tuned_ctu[shape_key(mx, my, mz)] = best_ctu;
   if (mask_out) *mask_out = best;
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask, int *fused_ct) {
+  if (!m) return ENZOE_ERR_ARG;
+  if (mask) *mask = launch_mask(m, mx, my, mz);
+  if (fused_ct) *fused_ct = (m->mhd && ctu_possible(m)) ? ctu_variant(m, mx, my, mz) : 0;
   return ENZOE_OK;
 }
 
@@ -886,9 +923,9 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
   return grid_refresh(m, blocks, nbx, nby, nbz, bc, (hipStream_t)stream, nullptr);
 }
 
-int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int on) {
-  if (!m) return ENZOE_ERR_ARG;
-  m->ctu = on ? 1 : 0;
+int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int variant) {
+  if (!m || variant < -1 || variant > 3) return ENZOE_ERR_ARG;
+  m->ctu = variant;
   return ENZOE_OK;
 }
 
@@ -1126,6 +1163,42 @@ int enzoe_profile_report(char *names, int name_len, double *ms, int *count,
 // The device buffers a method's launches read, moved in and out of the
 // method so that a graph owns a private set (a later eager call may
 // re-upload or regrow the method's own ones while the graph is alive).
+// ------------------------------------------------------ sub-cycled levels
+
+// dt of a coarse step of sub-cycled levels on the device: dt_l = courant *
+// CFL minimum of level l (as enzoe_vlct_timestep takes it), dt_0 = min_l
+// 2^l dt_l, and every entry of every substep's block list gets dt_0 /
+// 2^(its level); the time state[0] advances by dt_0, state[1] holds it.
+__global__ void k_levels_dt(const unsigned long long *__restrict__ cfl, int nlev,
+                            double courant, const int *__restrict__ lev,
+                            double *__restrict__ dts, long ndts, double *state) {
+  double dt0 = 0.;
+  for (int l = 0; l < nlev; l++) {
+    const double d =
+        (double)(1LL << l) * (__longlong_as_double((long long)cfl[l]) * courant);
+    dt0 = (l == 0 || d < dt0) ? d : dt0;
+  }
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ndts) dts[t] = dt0 / (double)(1LL << lev[t]);
+  if (t == 0) {
+    state[1] = dt0;
+    state[0] = state[0] + dt0;
+  }
+}
+
+// One launch run of a substep: consecutive blocks of one shape.
+struct LevelRun {
+  int first = 0, n = 0, chunk = 1;
+  size_t desc_off = 0;  // byte offset of its descriptor table
+};
+// One substep: the blocks of the levels that step at it, coarsest first.
+struct LevelSub {
+  std::vector<enzoe_block> blocks;
+  std::vector<int> level;
+  long dts_off = 0;  // first entry of its per-block dt
+  std::vector<LevelRun> runs;
+};
+
 struct MethodBuffers {
   void *scratch = nullptr, *d_desc = nullptr, *d_dt = nullptr,
        *d_ptrtab = nullptr, *d_rtab = nullptr;
@@ -1155,6 +1228,83 @@ struct MethodBuffers {
     scratch = d_desc = d_dt = d_ptrtab = d_rtab = nullptr;
   }
 };
+
+struct enzoe_vlct_levels_s {
+  enzoe_vlct_t m = nullptr;
+  int nlev = 0;
+  std::vector<std::vector<enzoe_block>> lev;  // blocks of each level
+  std::vector<LevelSub> sub;                   // 2^(nlev-1) substeps
+  void *scratch = nullptr;                     // stage scratch (shared by the runs)
+  void *d_desc = nullptr;                      // descriptor tables of the runs
+  void *d_dtdesc = nullptr;                    // CFL-kernel tables of the levels
+  std::vector<long> dtdesc_first;              // first descriptor of each level
+  std::vector<long> level_maxcells;
+  double *d_dts = nullptr;                     // per-block dt of every substep
+  int *d_lev = nullptr;                        // the level of each of those entries
+  long ndts = 0;
+  unsigned long long *d_cfl = nullptr;         // CFL minimum of each level
+  double *d_state = nullptr;                   // [0] time, [1] dt_0 of the last step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void *p : {scratch, d_desc, d_dtdesc, (void *)d_dts, (void *)d_lev, (void *)d_cfl,
+                    (void *)d_state})
+      if (p) (void)hipFree(p);
+    exec = nullptr;
+    graph = nullptr;
+    scratch = d_desc = d_dtdesc = nullptr;
+    d_dts = nullptr;
+    d_lev = nullptr;
+    d_cfl = nullptr;
+    d_state = nullptr;
+  }
+};
+
+namespace {
+
+// One coarse step of the levels: the CFL reduction of every level, the dt
+// kernel, then every substep's runs; kernels only (capturable).
+int levels_launch(enzoe_vlct_levels_s *L, hipStream_t s) {
+  enzoe_vlct_method *m = L->m;
+  for (int l = 0; l < L->nlev; l++)
+    if (int rc = launch_dt_on(m, (const BlockDesc<double> *)L->d_dtdesc + L->dtdesc_first[l],
+                              (int)L->lev[l].size(), L->level_maxcells[l], L->d_cfl + l, s,
+                              nullptr, 0.0))
+      return rc;
+  hipLaunchKernelGGL(k_levels_dt, dim3(nblk(std::max<long>(1, L->ndts), 256)), dim3(256), 0, s,
+                     L->d_cfl, L->nlev, m->courant, L->d_lev, L->d_dts, L->ndts, L->d_state);
+  for (const LevelSub &sb : L->sub)
+    for (const LevelRun &r : sb.runs) {
+      const enzoe_block *bl = sb.blocks.data() + r.first;
+      const void *table = (const char *)L->d_desc + r.desc_off;
+      const double *dts = L->d_dts + sb.dts_off + r.first;
+      int rc;
+      if (m->precision == ENZOE_PREC_DOUBLE) {
+        if (m->mhd)
+          rc = m->dual ? launch_chunks<double, true, true>(m, bl, r.n, r.chunk,
+                                                           (const BlockDesc<double> *)table, 0.0,
+                                                           dts, s, nullptr, nullptr, true)
+                       : launch_chunks<double, true, false>(m, bl, r.n, r.chunk,
+                                                            (const BlockDesc<double> *)table,
+                                                            0.0, dts, s, nullptr, nullptr, true);
+        else
+          rc = m->dual ? launch_chunks<double, false, true>(m, bl, r.n, r.chunk,
+                                                            (const BlockDesc<double> *)table,
+                                                            0.0, dts, s, nullptr, nullptr, true)
+                       : launch_chunks<double, false, false>(m, bl, r.n, r.chunk,
+                                                             (const BlockDesc<double> *)table,
+                                                             0.0, dts, s, nullptr, nullptr, true);
+      } else {
+        rc = launch_prepared_mixed(m, bl, r.n, r.chunk, table, dts, s);
+      }
+      if (rc) return rc;
+    }
+  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+}
+
+}  // namespace
 
 struct enzoe_vlct_graph_s {
   hipGraph_t graph = nullptr;
@@ -1256,6 +1406,186 @@ int enzoe_vlct_graph_time(enzoe_vlct_graph_t g, enzoe_stream_t stream, double *t
           hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return ENZOE_ERR_HIP;
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_levels_create(enzoe_vlct_t m, const enzoe_block *blocks, const int *level_of,
+                             int n, enzoe_vlct_levels_t *out) {
+  if (!m || !blocks || !level_of || n <= 0 || !out) return ENZOE_ERR_ARG;
+  *out = nullptr;
+  int nlev = 0;
+  for (int i = 0; i < n; i++) {
+    if (level_of[i] < 0 || level_of[i] > 20) return ENZOE_ERR_ARG;
+    if (int rc = check_block(m, blocks[i])) return rc;
+    nlev = std::max(nlev, level_of[i] + 1);
+  }
+  std::unique_ptr<enzoe_vlct_levels_s> L(new enzoe_vlct_levels_s);
+  L->m = m;
+  L->nlev = nlev;
+  L->lev.resize(nlev);
+  for (int i = 0; i < n; i++) L->lev[level_of[i]].push_back(blocks[i]);
+  for (int l = 0; l < nlev; l++)
+    if (L->lev[l].empty()) return ENZOE_ERR_ARG;  // every level holds blocks
+  // substep s advances the levels l with s % 2^(L-1-l) == 0
+  const int nsub = 1 << (nlev - 1);
+  L->sub.resize(nsub);
+  std::vector<int> lev_of_entry;
+  for (int st = 0; st < nsub; st++) {
+    LevelSub &sb = L->sub[st];
+    for (int l = 0; l < nlev; l++)
+      if (st % (1 << (nlev - 1 - l)) == 0)
+        for (const enzoe_block &b : L->lev[l]) {
+          sb.blocks.push_back(b);
+          sb.level.push_back(l);
+        }
+    sb.dts_off = L->ndts;
+    L->ndts += (long)sb.blocks.size();
+    lev_of_entry.insert(lev_of_entry.end(), sb.level.begin(), sb.level.end());
+    for (int b0 = 0; b0 < (int)sb.blocks.size();) {
+      int b1 = b0 + 1;
+      while (b1 < (int)sb.blocks.size() && sb.blocks[b1].mx == sb.blocks[b0].mx &&
+             sb.blocks[b1].my == sb.blocks[b0].my && sb.blocks[b1].mz == sb.blocks[b0].mz)
+        b1++;
+      LevelRun r;
+      r.first = b0;
+      r.n = b1 - b0;
+      sb.runs.push_back(r);
+      b0 = b1;
+    }
+  }
+  // one scratch pool for every run (they run one after the other), each
+  // run in chunks that fit the method's budget
+  const bool mixed = m->precision != ENZOE_PREC_DOUBLE;
+  const size_t budget = scratch_budget(m);
+  size_t pool = 0;
+  for (LevelSub &sb : L->sub)
+    for (LevelRun &r : sb.runs) {
+      const enzoe_block &b = sb.blocks[r.first];
+      const size_t per = mixed ? scratch_per_block<float>(b.mx, b.my, b.mz, m->mhd, m->dual,
+                                                          ctu_possible(m))
+                               : scratch_per_block<double>(b.mx, b.my, b.mz, m->mhd, m->dual,
+                                                           ctu_possible(m));
+      r.chunk = (int)std::max<size_t>(1, std::min<size_t>(r.n, budget / per));
+      pool = std::max(pool, per * r.chunk);
+    }
+  auto fail = [&](int code) {
+    L->release();
+    return code;
+  };
+  if (hipMalloc(&L->scratch, pool) != hipSuccess) {
+    L->scratch = nullptr;
+    return fail(ENZOE_ERR_ALLOC);
+  }
+  std::vector<char> tables;
+  for (LevelSub &sb : L->sub)
+    for (LevelRun &r : sb.runs) {
+      r.desc_off = tables.size();
+      const enzoe_block *bl = sb.blocks.data() + r.first;
+      if (mixed) {
+        std::vector<BlockDesc<float>> d(r.n);
+        for (int c0 = 0; c0 < r.n; c0 += r.chunk)
+          fill_desc_at<float>(m, (char *)L->scratch, bl, c0, std::min(r.n, c0 + r.chunk),
+                              d.data() + c0);
+        tables.insert(tables.end(), (const char *)d.data(),
+                      (const char *)(d.data() + r.n));
+      } else {
+        std::vector<BlockDesc<double>> d(r.n);
+        for (int c0 = 0; c0 < r.n; c0 += r.chunk)
+          fill_desc_at<double>(m, (char *)L->scratch, bl, c0, std::min(r.n, c0 + r.chunk),
+                               d.data() + c0);
+        tables.insert(tables.end(), (const char *)d.data(),
+                      (const char *)(d.data() + r.n));
+      }
+      // 256-byte aligned tables
+      tables.resize((tables.size() + 255) & ~(size_t)255);
+    }
+  std::vector<BlockDesc<double>> dtdesc;
+  for (int l = 0; l < nlev; l++) {
+    L->dtdesc_first.push_back((long)dtdesc.size());
+    const size_t at = dtdesc.size();
+    dtdesc.resize(at + L->lev[l].size());
+    fill_dt_desc(m, L->lev[l].data(), (int)L->lev[l].size(), dtdesc.data() + at);
+    long mc = 0;
+    for (const enzoe_block &b : L->lev[l]) mc = std::max<long>(mc, (long)b.mx * b.my * b.mz);
+    L->level_maxcells.push_back(mc);
+  }
+  const double state0[2] = {0.0, 0.0};
+  if (hipMalloc(&L->d_desc, tables.size()) != hipSuccess ||
+      hipMalloc(&L->d_dtdesc, dtdesc.size() * sizeof(BlockDesc<double>)) != hipSuccess ||
+      hipMalloc((void **)&L->d_dts, L->ndts * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&L->d_lev, L->ndts * sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&L->d_cfl, nlev * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc((void **)&L->d_state, 2 * sizeof(double)) != hipSuccess)
+    return fail(ENZOE_ERR_ALLOC);
+  if (hipMemcpy(L->d_desc, tables.data(), tables.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(L->d_dtdesc, dtdesc.data(), dtdesc.size() * sizeof(BlockDesc<double>),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(L->d_lev, lev_of_entry.data(), L->ndts * sizeof(int),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(L->d_state, state0, sizeof(state0), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(ENZOE_ERR_HIP);
+  *out = L.release();
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_levels_step(enzoe_vlct_levels_t L, enzoe_stream_t stream) {
+  if (!L) return ENZOE_ERR_ARG;
+  return levels_launch(L, (hipStream_t)stream);
+}
+
+int enzoe_vlct_levels_capture(enzoe_vlct_levels_t L, int nsteps) {
+  if (!L || nsteps <= 0) return ENZOE_ERR_ARG;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return ENZOE_ERR_HIP;
+  if (L->exec) (void)hipGraphExecDestroy(L->exec);
+  if (L->graph) (void)hipGraphDestroy(L->graph);
+  L->exec = nullptr;
+  L->graph = nullptr;
+  const bool prof = g_profiling;
+  g_profiling = false;  // no timing events inside the graph
+  int rc = ENZOE_OK;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    rc = ENZOE_ERR_HIP;
+  } else {
+    for (int i = 0; i < nsteps && rc == ENZOE_OK; i++) rc = levels_launch(L, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    L->graph = graph;
+    if (!rc && (ec != hipSuccess || !graph)) rc = ENZOE_ERR_HIP;
+    if (!rc && hipGraphInstantiate(&L->exec, L->graph, nullptr, nullptr, 0) != hipSuccess) {
+      L->exec = nullptr;
+      rc = ENZOE_ERR_HIP;
+    }
+  }
+  g_profiling = prof;
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+int enzoe_vlct_levels_replay(enzoe_vlct_levels_t L, enzoe_stream_t stream) {
+  if (!L || !L->exec) return ENZOE_ERR_STATE;
+  return hipGraphLaunch(L->exec, (hipStream_t)stream) == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
+}
+
+int enzoe_vlct_levels_state(enzoe_vlct_levels_t L, enzoe_stream_t stream, double *time,
+                            double *dt0) {
+  if (!L) return ENZOE_ERR_ARG;
+  double st[2];
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(st, L->d_state, sizeof(st), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ENZOE_ERR_HIP;
+  if (time) *time = st[0];
+  if (dt0) *dt0 = st[1];
+  return ENZOE_OK;
+}
+
+int enzoe_vlct_levels_destroy(enzoe_vlct_levels_t L) {
+  if (!L) return ENZOE_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  L->release();
+  delete L;
   return ENZOE_OK;
 }
 

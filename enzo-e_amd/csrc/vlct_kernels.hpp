@@ -988,8 +988,8 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
 // ----------------------------------------------------------------- k_ctu
 
 // Constrained transport and the cell update of one stage in one launch:
-// k_edge, k_faces and k_cells fused over tiles of CTU_TX x CTU_TY x CTU_TZ
-// cells of the k_cells launch box, one cell per thread, with the same
+// k_edge, k_faces and k_cells fused over tiles of TX x TY x TZ cells of
+// the k_cells launch box, one cell per thread, with the same
 // expressions in the same order (the same bits).  A workgroup
 //  1. computes into LDS the edge EMFs its faces need: E_a over the tile
 //     along a, and over the tile plus one edge below across a;
@@ -1003,59 +1003,137 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
 // may already have overwritten in place in the final stage: the first
 // stage stores a copy (bf0) of the faces it updates, a region that holds
 // the final stage's, and the final stage reads its base values there.
-constexpr int CTU_TX = 16, CTU_TY = 4, CTU_TZ = 4;
-constexpr int CTU_THREADS = CTU_TX * CTU_TY * CTU_TZ;
-__host__ __device__ constexpr int ctu_t(int d) { return d == 0 ? CTU_TX : (d == 1 ? CTU_TY : CTU_TZ); }
-// extent of the LDS edge array along axis a, direction d
-__host__ __device__ constexpr int ctu_eext(int a, int d) { return ctu_t(d) + (d != a); }
-// extent of the LDS face array of B_a, direction d
-__host__ __device__ constexpr int ctu_fext(int a, int d) { return ctu_t(d) + (d == a); }
-__host__ __device__ constexpr int ctu_ecount(int a) {
-  return ctu_eext(a, 0) * ctu_eext(a, 1) * ctu_eext(a, 2);
-}
-__host__ __device__ constexpr int ctu_fcount(int a) {
-  return ctu_fext(a, 0) * ctu_fext(a, 1) * ctu_fext(a, 2);
-}
-constexpr int CTU_NE = ctu_ecount(0) + ctu_ecount(1) + ctu_ecount(2);
-constexpr int CTU_NF = ctu_fcount(0) + ctu_fcount(1) + ctu_fcount(2);
+// The tile: TX x TY x TZ cells, one thread each.
+template <int TX, int TY, int TZ>
+struct CtuTile {
+  static constexpr int NT = TX * TY * TZ;
+  __host__ __device__ static constexpr int t(int d) { return d == 0 ? TX : (d == 1 ? TY : TZ); }
+  // extent of the LDS edge array along axis a, direction d
+  __host__ __device__ static constexpr int eext(int a, int d) { return t(d) + (d != a); }
+  // extent of the LDS face array of B_a, direction d
+  __host__ __device__ static constexpr int fext(int a, int d) { return t(d) + (d == a); }
+  __host__ __device__ static constexpr int ecount(int a) {
+    return eext(a, 0) * eext(a, 1) * eext(a, 2);
+  }
+  __host__ __device__ static constexpr int fcount(int a) {
+    return fext(a, 0) * fext(a, 1) * fext(a, 2);
+  }
+  __host__ __device__ static constexpr int ebase(int a) {
+    return a == 0 ? 0 : (a == 1 ? ecount(0) : ecount(0) + ecount(1));
+  }
+  __host__ __device__ static constexpr int fbase(int a) {
+    return a == 0 ? 0 : (a == 1 ? fcount(0) : fcount(0) + fcount(1));
+  }
+  static constexpr int NE = ecount(0) + ecount(1) + ecount(2);
+  static constexpr int NF = fcount(0) + fcount(1) + fcount(2);
+  // LDS index of the edge along a at cell-shaped index p (tile origin o)
+  __device__ static int eidx(int a, const int p[3], const int o[3]) {
+    const int l0 = p[0] - o[0] + (a != 0), l1 = p[1] - o[1] + (a != 1),
+              l2 = p[2] - o[2] + (a != 2);
+    return ebase(a) + (l2 * eext(a, 1) + l1) * eext(a, 0) + l0;
+  }
+  // LDS index of face B_a at cell-shaped index (px, py, pz)
+  __device__ static int fidx(int a, int px, int py, int pz, const int o[3]) {
+    return fbase(a) + ((pz - o[2]) * fext(a, 1) + (py - o[1])) * fext(a, 0) + (px - o[0]);
+  }
+};
 
-// LDS index of the edge along a at cell-shaped index p (tile origin o)
-__device__ __forceinline__ int ctu_eidx(int a, const int p[3], const int o[3]) {
-  int base = 0;
-  for (int x = 0; x < a; x++) base += ctu_ecount(x);
-  const int l0 = p[0] - o[0] + (a != 0), l1 = p[1] - o[1] + (a != 1),
-            l2 = p[2] - o[2] + (a != 2);
-  return base + (l2 * ctu_eext(a, 1) + l1) * ctu_eext(a, 0) + l0;
-}
-// LDS index of face B_a at cell-shaped index p (tile origin o)
-__device__ __forceinline__ int ctu_fidx(int a, int px, int py, int pz, const int o[3]) {
-  int base = 0;
-  for (int x = 0; x < a; x++) base += ctu_fcount(x);
-  return base + ((pz - o[2]) * ctu_fext(a, 1) + (py - o[1])) * ctu_fext(a, 0) + (px - o[0]);
-}
+// The tile shapes of k_ctu the launch chooses from (enzoe_vlct_set_fused_ct
+// / the autotuner): 1 = 16 x 4 x 4, 2 = 32 x 4 x 2, 3 = 8 x 8 x 4.
+typedef CtuTile<16, 4, 4> CtuTile1;
+typedef CtuTile<32, 4, 2> CtuTile2;
+typedef CtuTile<8, 8, 4> CtuTile3;
 
 // Tiles of the launch box [blo, m - blo) along each axis.
+template <class Tile>
 __host__ __device__ inline void ctu_tiles(int mx, int my, int mz, int blo, int nt[3]) {
   const int n[3] = {mx - 2 * blo, my - 2 * blo, mz - 2 * blo};
-  for (int d = 0; d < 3; d++) nt[d] = n[d] > 0 ? (n[d] + ctu_t(d) - 1) / ctu_t(d) : 0;
+  for (int d = 0; d < 3; d++) nt[d] = n[d] > 0 ? (n[d] + Tile::t(d) - 1) / Tile::t(d) : 0;
 }
 
-template <typename FluxT>
+template <class Tile>
 struct FaceBLds {
   const double *sF;
   const int *o;
   __device__ __forceinline__ void operator()(int k, int j, int i, double f[6]) const {
-    f[0] = sF[ctu_fidx(0, i, j, k, o)];
-    f[1] = sF[ctu_fidx(0, i + 1, j, k, o)];
-    f[2] = sF[ctu_fidx(1, i, j, k, o)];
-    f[3] = sF[ctu_fidx(1, i, j + 1, k, o)];
-    f[4] = sF[ctu_fidx(2, i, j, k, o)];
-    f[5] = sF[ctu_fidx(2, i, j, k + 1, o)];
+    f[0] = sF[Tile::fidx(0, i, j, k, o)];
+    f[1] = sF[Tile::fidx(0, i + 1, j, k, o)];
+    f[2] = sF[Tile::fidx(1, i, j, k, o)];
+    f[3] = sF[Tile::fidx(1, i, j + 1, k, o)];
+    f[4] = sF[Tile::fidx(2, i, j, k, o)];
+    f[5] = sF[Tile::fidx(2, i, j, k + 1, o)];
   }
 };
 
-template <typename FluxT, bool DE = false>
-__global__ void __launch_bounds__(CTU_THREADS)
+// The edges along AX of the tile into LDS, each thread a fixed set of them
+// (the iterations unrolled, so their loads go out together).
+template <int AX, class Tile, typename FluxT>
+__device__ __forceinline__ void ctu_edges(const BlockDesc<FluxT> &b, double *const *Ecs,
+                                          const int o[3], const int m[3], int cs,
+                                          const long str[3], double *sE) {
+  constexpr int n = Tile::ecount(AX);
+  constexpr int ex = Tile::eext(AX, 0), ey = Tile::eext(AX, 1);
+  constexpr int aj = (AX + 1) % 3, ak = (AX + 2) % 3;
+#pragma unroll
+  for (int it = 0; it < (n + Tile::NT - 1) / Tile::NT; it++) {
+    const int r = threadIdx.x + it * Tile::NT;
+    if (r < n) {
+      const int p[3] = {o[0] - (AX != 0) + r % ex, o[1] - (AX != 1) + (r / ex) % ey,
+                        o[2] - (AX != 2) + r / (ex * ey)};
+      double val = 0.;
+      if (edge_in_region(AX, p, m, cs))
+        val = edge_emf<AX>(b, Ecs, idx3(p[2], p[1], p[0], m[1], m[0]), str[aj], str[ak]);
+      sE[Tile::ebase(AX) + r] = val;
+    }
+  }
+}
+
+// The faces B_AX of the tile into LDS (and the owned updated ones to memory).
+template <int AX, class Tile, typename FluxT>
+__device__ __forceinline__ void ctu_faces(const BlockDesc<FluxT> &b, const StageParams &sp,
+                                          double dt, const int o[3], const int m[3],
+                                          const int hi[3], int cs, const double *sE,
+                                          double *sF) {
+  constexpr int n = Tile::fcount(AX);
+  constexpr int fx = Tile::fext(AX, 0), fy = Tile::fext(AX, 1);
+  constexpr int aj = (AX + 1) % 3, ak = (AX + 2) % 3;
+  const int bnx = m[0] + (AX == 0), bny = m[1] + (AX == 1);
+  const double dtdj = dt / b.cw[aj], dtdk = dt / b.cw[ak];
+#pragma unroll
+  for (int it = 0; it < (n + Tile::NT - 1) / Tile::NT; it++) {
+    const int r = threadIdx.x + it * Tile::NT;
+    if (r < n) {
+      const int p[3] = {o[0] + r % fx, o[1] + (r / fx) % fy, o[2] + r / (fx * fy)};
+      const bool upd = p[AX] >= cs + 1 && p[AX] < m[AX] - cs && p[aj] >= cs + 1 &&
+                       p[aj] < m[aj] - cs - 1 && p[ak] >= cs + 1 && p[ak] < m[ak] - cs - 1;
+      double val = 0.;
+      if (upd) {
+        const long cb = idx3(p[2], p[1], p[0], bny, bnx);
+        const double b0 = sp.is_final ? b.bf0[AX][cb] : b.bfi[AX][cb];
+        // edges at AX-position p_AX - 1/2 (cell-shaped index p - e_AX)
+        int q[3] = {p[0], p[1], p[2]};
+        q[AX] -= 1;
+        int qj[3] = {q[0], q[1], q[2]}, qk[3] = {q[0], q[1], q[2]};
+        qj[aj] -= 1;
+        qk[ak] -= 1;
+        val = face_b_update(b0, dtdj, dtdk, sE[Tile::eidx(ak, q, o)], sE[Tile::eidx(ak, qj, o)],
+                            sE[Tile::eidx(aj, q, o)], sE[Tile::eidx(aj, qk, o)]);
+        if (p[AX] < o[AX] + Tile::t(AX) || p[AX] == hi[AX]) {
+          (sp.is_final ? b.bfi : b.bft)[AX][cb] = val;
+          if (!sp.is_final) b.bf0[AX][cb] = b0;
+        }
+      } else if (sp.is_final && p[AX] <= m[AX] && p[aj] < m[aj] && p[ak] < m[ak]) {
+        // outside the update region: the start-of-step value, never written
+        // in this stage
+        val = b.bfi[AX][idx3(p[2], p[1], p[0], bny, bnx)];
+      }
+      sF[Tile::fbase(AX) + r] = val;
+    }
+  }
+}
+
+template <typename FluxT, bool DE, class Tile>
+__global__ void __launch_bounds__(Tile::NT)
 k_ctu(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
   wg_coords(sp.xcd_remap & 8, bx, by);
@@ -1068,98 +1146,37 @@ k_ctu(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const bool cfl = sp.cfl != nullptr;
   const int blo = cfl ? 0 : lo;
   int nt[3];
-  ctu_tiles(mx, my, mz, blo, nt);
+  ctu_tiles<Tile>(mx, my, mz, blo, nt);
   // padding workgroups (launch rounded up to the XCDs) take part in nothing
   if (bx >= nt[0] * nt[1] * nt[2]) return;
   const int tix = bx % nt[0], tiy = (bx / nt[0]) % nt[1], tiz = bx / (nt[0] * nt[1]);
-  const int o[3] = {blo + tix * CTU_TX, blo + tiy * CTU_TY, blo + tiz * CTU_TZ};
+  const int o[3] = {blo + tix * Tile::t(0), blo + tiy * Tile::t(1), blo + tiz * Tile::t(2)};
   const int hi[3] = {mx - blo, my - blo, mz - blo};  // end of the launch box
   const long str[3] = {1, (long)mx, (long)mx * my};
   const double dt = sp.dts ? sp.dts[by] * sp.dt_factor : sp.dt;
-  const int tid = threadIdx.x;
 
-  __shared__ double sE[CTU_NE];
-  __shared__ double sF[CTU_NF];
+  __shared__ double sE[Tile::NE];
+  __shared__ double sF[Tile::NF];
 
   // 1. edges
   double *const *Ecs = center_e(b, sp);
-  for (int e = tid; e < CTU_NE; e += CTU_THREADS) {
-    int a = 0, r = e;
-    if (r >= ctu_ecount(0)) {
-      r -= ctu_ecount(0);
-      a = 1;
-      if (r >= ctu_ecount(1)) {
-        r -= ctu_ecount(1);
-        a = 2;
-      }
-    }
-    const int ex = ctu_eext(a, 0), ey = ctu_eext(a, 1);
-    const int p[3] = {o[0] - (a != 0) + r % ex, o[1] - (a != 1) + (r / ex) % ey,
-                      o[2] - (a != 2) + r / (ex * ey)};
-    double val = 0.;
-    if (edge_in_region(a, p, m, cs)) {
-      const long c = idx3(p[2], p[1], p[0], my, mx);
-      if (a == 0)
-        val = edge_emf<0>(b, Ecs, c, str[1], str[2]);
-      else if (a == 1)
-        val = edge_emf<1>(b, Ecs, c, str[2], str[0]);
-      else
-        val = edge_emf<2>(b, Ecs, c, str[0], str[1]);
-    }
-    sE[e] = val;
-  }
+  ctu_edges<0, Tile>(b, Ecs, o, m, cs, str, sE);
+  ctu_edges<1, Tile>(b, Ecs, o, m, cs, str, sE);
+  ctu_edges<2, Tile>(b, Ecs, o, m, cs, str, sE);
   __syncthreads();
-
   // 2. faces
-  for (int f = tid; f < CTU_NF; f += CTU_THREADS) {
-    int a = 0, r = f;
-    if (r >= ctu_fcount(0)) {
-      r -= ctu_fcount(0);
-      a = 1;
-      if (r >= ctu_fcount(1)) {
-        r -= ctu_fcount(1);
-        a = 2;
-      }
-    }
-    const int fx = ctu_fext(a, 0), fy = ctu_fext(a, 1);
-    const int p[3] = {o[0] + r % fx, o[1] + (r / fx) % fy, o[2] + r / (fx * fy)};
-    const int aj = (a + 1) % 3, ak = (a + 2) % 3;
-    const int bnx = mx + (a == 0), bny = my + (a == 1);
-    const bool upd = p[a] >= cs + 1 && p[a] < m[a] - cs && p[aj] >= cs + 1 &&
-                     p[aj] < m[aj] - cs - 1 && p[ak] >= cs + 1 && p[ak] < m[ak] - cs - 1;
-    double val = 0.;
-    if (upd) {
-      const long cb = idx3(p[2], p[1], p[0], bny, bnx);
-      const double b0 = sp.is_final ? b.bf0[a][cb] : b.bfi[a][cb];
-      // edges at a-position p_a - 1/2 (cell-shaped index p - e_a)
-      int q[3] = {p[0], p[1], p[2]};
-      q[a] -= 1;
-      int qj[3] = {q[0], q[1], q[2]}, qk[3] = {q[0], q[1], q[2]};
-      qj[aj] -= 1;
-      qk[ak] -= 1;
-      val = face_b_update(b0, dt / b.cw[aj], dt / b.cw[ak], sE[ctu_eidx(ak, q, o)],
-                          sE[ctu_eidx(ak, qj, o)], sE[ctu_eidx(aj, q, o)],
-                          sE[ctu_eidx(aj, qk, o)]);
-      if (p[a] < o[a] + ctu_t(a) || p[a] == hi[a]) {
-        (sp.is_final ? b.bfi : b.bft)[a][cb] = val;
-        if (!sp.is_final) b.bf0[a][cb] = b0;
-      }
-    } else if (sp.is_final && p[a] <= m[a] && p[aj] < m[aj] && p[ak] < m[ak]) {
-      // outside the update region: the start-of-step value, never written
-      // in this stage
-      val = b.bfi[a][idx3(p[2], p[1], p[0], bny, bnx)];
-    }
-    sF[f] = val;
-  }
+  ctu_faces<0, Tile>(b, sp, dt, o, m, hi, cs, sE, sF);
+  ctu_faces<1, Tile>(b, sp, dt, o, m, hi, cs, sE, sF);
+  ctu_faces<2, Tile>(b, sp, dt, o, m, hi, cs, sE, sF);
   __syncthreads();
-
   // 3. cells
-  const int i = o[0] + tid % CTU_TX, j = o[1] + (tid / CTU_TX) % CTU_TY,
-            k = o[2] + tid / (CTU_TX * CTU_TY);
+  const int tid = threadIdx.x;
+  const int i = o[0] + tid % Tile::t(0), j = o[1] + (tid / Tile::t(0)) % Tile::t(1),
+            k = o[2] + tid / (Tile::t(0) * Tile::t(1));
   const bool in_launch = i < hi[0] && j < hi[1] && k < hi[2];
   const bool in_box = in_launch && k >= lo && j >= lo && i >= lo && k < mz - lo &&
                       j < my - lo && i < mx - lo;
-  const FaceBLds<FluxT> fb{sF, o};
+  const FaceBLds<Tile> fb{sF, o};
   const double local = cell_stage<FluxT, true, DE>(b, sp, by, k, j, i, in_launch, in_box, fb);
   if (cfl) block_min_to(sp.cfl, local);
 }

@@ -79,10 +79,13 @@ struct enzoe_vlct_method {
   int xcd_remap = 2;
   double theta = 1.5, courant = 0.3, gamma = 5.0 / 3.0;
   double dfloor = 1e-200, pfloor = 1e-200;
-  // MHD stages run constrained transport and the cell update as one fused
-  // launch (k_ctu) instead of k_edge, k_faces and k_cells (same bits); off
-  // by default: on the measured shapes the three launches are faster
-  int ctu = 0;
+  // MHD stages: constrained transport and the cell update as three launches
+  // (0: k_edge, k_faces, k_cells) or one fused launch over tiles (k_ctu
+  // with tile 1..3), same bits either way; -1 (the default): the
+  // autotuner's choice per block shape, three launches until it has run
+  int ctu = -1;
+  int force_ctu = -1;                    // set while the autotuner times one
+  std::map<long long, int> tuned_ctu;    // the autotuner's choice per shape
   // dual-energy formalism ("modern" formulation) and its eta
   int dual = 0;
   double eta = 0.001;
@@ -132,9 +135,36 @@ struct enzoe_vlct_method {
 namespace enzoe {
 
 
+inline long long shape_key(int mx, int my, int mz) {
+  return ((long long)mx << 40) | ((long long)my << 20) | (long long)mz;
+}
+
+// The launch structure of the MHD stages for blocks of this shape: 0 =
+// k_edge + k_faces + k_cells, 1..3 = k_ctu with tile CtuTile1..3; the
+// method's setting, or the autotuner's choice for the shape when the
+// setting is "auto" (-1).
+inline int ctu_variant(const enzoe_vlct_method *m, int mx, int my, int mz) {
+  if (m->force_ctu >= 0) return m->force_ctu;
+  if (m->ctu >= 0) return m->ctu;
+  auto it = m->tuned_ctu.find(shape_key(mx, my, mz));
+  return it != m->tuned_ctu.end() ? it->second : 0;
+}
+
 // The fused CT update needs the two stages of the VL scheme (its final
-// stage reads the first stage's copy of the start-of-step faces).
-inline bool use_ctu(const enzoe_vlct_method *m) { return m->mhd && m->ctu && m->nstages == 2; }
+// stage reads the first stage's copy of the start-of-step faces); the
+// scratch holds that copy whenever the fused launch may be chosen.
+inline bool ctu_possible(const enzoe_vlct_method *m) {
+  return m->mhd && m->nstages == 2 && (m->ctu != 0 || m->force_ctu > 0);
+}
+template <typename FluxT, bool DE, class Tile>
+void launch_ctu(const BlockDesc<FluxT> *d, int nb, int mx, int my, int mz, int lo,
+                StageParams sp, hipStream_t s) {
+  int nt[3];
+  ctu_tiles<Tile>(mx, my, mz, lo, nt);
+  const long tiles = (long)nt[0] * nt[1] * nt[2];
+  hipLaunchKernelGGL((k_ctu<FluxT, DE, Tile>), dim3(nblk8(tiles * Tile::NT, Tile::NT), nb),
+                     dim3(Tile::NT), 0, s, d, sp);
+}
 
 inline int total_staling(int r) { return r == RC_NN ? 1 : 2; }
 inline int immediate_staling(int r) { return r == RC_NN ? 0 : 1; }
@@ -281,7 +311,7 @@ void fill_desc_at(const enzoe_vlct_method *m, char *base, const enzoe_block *blo
       d.bft[0] = (double *)base; base += (size_t)b.mz * b.my * (b.mx + 1) * sizeof(double);
       d.bft[1] = (double *)base; base += (size_t)b.mz * (b.my + 1) * b.mx * sizeof(double);
       d.bft[2] = (double *)base; base += (size_t)(b.mz + 1) * b.my * b.mx * sizeof(double);
-      if (m->ctu) {
+      if (ctu_possible(m)) {
         d.bf0[0] = (double *)base; base += (size_t)b.mz * b.my * (b.mx + 1) * sizeof(double);
         d.bf0[1] = (double *)base; base += (size_t)b.mz * (b.my + 1) * b.mx * sizeof(double);
         d.bf0[2] = (double *)base; base += (size_t)(b.mz + 1) * b.my * b.mx * sizeof(double);
@@ -377,10 +407,6 @@ inline double sync_ggm1(double gamma) {
 }
 
 
-inline long long shape_key(int mx, int my, int mz) {
-  return ((long long)mx << 40) | ((long long)my << 20) | (long long)mz;
-}
-
 // XCD-order mask of the stage kernels for blocks of this shape
 inline int launch_mask(const enzoe_vlct_method *m, int mx, int my, int mz) {
   if (m->force_mask >= 0) return m->force_mask;
@@ -400,6 +426,19 @@ struct StoreArgs {
   StoreSlots sl;
   double *const *d_fdtab;
 };
+
+// Bytes of the chunk scratch pool: the larger of 4 GiB and a third of the
+// free device memory, at most 16 GiB (taken once per method).
+inline size_t scratch_budget(enzoe_vlct_method *m) {
+  if (m->scratch_budget == 0) {
+    size_t budget = (size_t)4 << 30;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      budget = std::max(budget, std::min((size_t)16 << 30, free_b / 3));
+    m->scratch_budget = budget;
+  }
+  return m->scratch_budget;
+}
 
 // The launch sequence of compute_group over chunks of `chunk` blocks whose
 // descriptors are table[0 .. n) in device memory.  timers: bracket the
@@ -445,13 +484,15 @@ int launch_chunks(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int ch
       }
       // with the fused CFL the final update's launch covers the block
       const int lo = sp.cfl ? 0 : (is_final ? cs : cs + 1);
-      if (MHD && use_ctu(m)) {
+      const int variant = (MHD && ctu_possible(m)) ? ctu_variant(m, mx, my, mz) : 0;
+      if (variant > 0) {
         KernelTimer kt(s, "k_ctu", timers);
-        int nt[3];
-        ctu_tiles(mx, my, mz, lo, nt);
-        const long tiles = (long)nt[0] * nt[1] * nt[2];
-        hipLaunchKernelGGL((k_ctu<FluxT, DE>), dim3(nblk8(tiles * CTU_THREADS, CTU_THREADS), nb),
-                           dim3(CTU_THREADS), 0, s, d, sp);
+        if (variant == 2)
+          launch_ctu<FluxT, DE, CtuTile2>(d, nb, mx, my, mz, lo, sp, s);
+        else if (variant == 3)
+          launch_ctu<FluxT, DE, CtuTile3>(d, nb, mx, my, mz, lo, sp, s);
+        else
+          launch_ctu<FluxT, DE, CtuTile1>(d, nb, mx, my, mz, lo, sp, s);
       } else {
         if (MHD) {
           {
@@ -506,7 +547,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   unsigned long long *cfl = nullptr) {
   const int mx = blocks[0].mx, my = blocks[0].my, mz = blocks[0].mz;
   if (DE != (m->dual != 0)) return ENZOE_ERR_CONFIG;
-  const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd, m->dual, m->ctu);
+  const size_t per = scratch_per_block<FluxT>(mx, my, mz, m->mhd, m->dual, ctu_possible(m));
   if (m->arena) {
     // the allocation-free path: descriptor table and scratch in the fixed
     // workspace; nothing is allocated, nothing cached across calls
@@ -527,14 +568,7 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
   }
   // chunk size: as many blocks as fit the scratch budget (at least 1); the
   // chunks run one after the other on the stream and share the pool
-  if (m->scratch_budget == 0) {
-    size_t budget = (size_t)4 << 30;
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
-      budget = std::max(budget, std::min((size_t)16 << 30, free_b / 3));
-    m->scratch_budget = budget;
-  }
-  const size_t budget = m->scratch_budget;
+  const size_t budget = scratch_budget(m);
   int chunk = (int)std::max<size_t>(1, std::min<size_t>(n, budget / per));
   if (int rc = ensure_bytes(&m->scratch, &m->scratch_bytes, per * chunk)) return rc;
 
@@ -564,6 +598,12 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                                        (const BlockDesc<FluxT> *)m->d_desc, dt, dts, s, st,
                                        cfl, true);
 }
+
+// The launch sequence of n blocks (chunks of `chunk`) of the FP32 flux path
+// over a descriptor table prepared in device memory, per-block dt in dts
+// (device); compiled in vlct_mixed.hip.
+int launch_prepared_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int chunk,
+                          const void *table, const double *dts, hipStream_t s);
 
 // FP32 flux path, compiled in vlct_mixed.hip
 int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,

@@ -17,4 +17,14 @@ int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                 : compute_group<float, false>(m, blocks, n, dt, dts, s, st, accel, cfl);
 }
 
+int launch_prepared_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int chunk,
+                          const void *table, const double *dts, hipStream_t s) {
+  return m->mhd ? launch_chunks<float, true, false>(m, blocks, n, chunk,
+                                                    (const BlockDesc<float> *)table, 0.0, dts,
+                                                    s, nullptr, nullptr, true)
+                : launch_chunks<float, false, false>(m, blocks, n, chunk,
+                                                     (const BlockDesc<float> *)table, 0.0, dts,
+                                                     s, nullptr, nullptr, true);
+}
+
 }  // namespace enzoe

@@ -1506,6 +1506,260 @@ __device__ __forceinline__ void restrict_cell(double *__restrict__ vc,
   vc[i_c] = accumulate ? vc[i_c] + v : v;
 }
 
+// --------------------------------------------- EnzoProlong (3A, 2A, 2B)
+// EnzoProlong::apply_ (src/Enzo/mesh/EnzoProlong.cpp:112-190) calls
+// interpolate (src/Enzo/mesh/interpolate.F) with refinement 2, pstart 2 and
+// pend = nf + 1: the coarse region holds the parents of the fine region and
+// one cell of stencil on each side.  One thread per parent: it writes its 8
+// children, with the Fortran routines' operation order (tscint3d, interp3d,
+// int_lin3d in src/Enzo/hydro-mhd/ppm_fortran/).  pc points at the coarse
+// region's first cell, gf at the fine region's; Fortran indices are kept
+// 1-based (PRO_P, PRO_G).
+
+#define PRO_P(ip, jp, kp) pc[((ip) - 1) + (long)mc.v[0] * (((jp) - 1) + (long)mc.v[1] * ((kp) - 1))]
+#define PRO_G(ig, jg, kg) gf[((ig) - 1) + (long)mf.v[0] * (((jg) - 1) + (long)mf.v[1] * ((kg) - 1))]
+
+__device__ __forceinline__ double pro_min(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double pro_max(double a, double b) { return (b > a) ? b : a; }
+__device__ __forceinline__ double pro_sign(double a, double b) { return copysign(fabs(a), b); }
+
+// the corner average work(i, j, k) of interp3d / int_lin3d (pstart = 1)
+__device__ __forceinline__ double pro_corner(const double *__restrict__ pc, Int3 mc, int i, int j,
+                                             int k) {
+  return 0.125 * (PRO_P(i, j, k) + PRO_P(i, j, k + 1) + PRO_P(i, j + 1, k) +
+                  PRO_P(i, j + 1, k + 1) + PRO_P(i + 1, j, k) + PRO_P(i + 1, j, k + 1) +
+                  PRO_P(i + 1, j + 1, k) + PRO_P(i + 1, j + 1, k + 1));
+}
+
+// the children fbar + (i1 - 1/2)/2 fx + (j1 - 1/2)/2 fy + (k1 - 1/2)/2 fz of
+// parent (i, j, k)
+__device__ __forceinline__ void pro_children(double *__restrict__ gf, Int3 mf, int i, int j,
+                                             int k, double fbar, double fx, double fy,
+                                             double fz) {
+  for (int k1 = 0; k1 < 2; k1++)
+    for (int j1 = 0; j1 < 2; j1++)
+      for (int i1 = 0; i1 < 2; i1++)
+        PRO_G(1 + (i - 1) * 2 + i1, 1 + (j - 1) * 2 + j1, 1 + (k - 1) * 2 + k1) =
+            (fbar + ((double)i1 + 0.5 - 0.5 * 2.0) / 2.0 * fx +
+             ((double)j1 + 0.5 - 0.5 * 2.0) / 2.0 * fy +
+             ((double)k1 + 0.5 - 0.5 * 2.0) / 2.0 * fz);
+}
+
+// tscint3d ("3A"): triangular-shaped-cloud weights, the conservation
+// correction, then two passes of the routine's monotonicity step along x, y
+// and z; its second bound, max(max(min(d1right/d2right, 0), 1), fraction),
+// is at least 1 and the first leaves fraction <= 1, so every fraction is 1
+// (finite data) and each pass sets a parent's children to their mean along
+// the axis, all within one parent.
+__global__ void k_op_prolong_3a(double *__restrict__ gf, Int3 mf, const double *__restrict__ pc,
+                                Int3 mc, Int3 pd) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)pd.v[0] * pd.v[1] * pd.v[2]) return;
+  const int i = (int)(t % pd.v[0]) + 1, j = (int)((t / pd.v[0]) % pd.v[1]) + 1,
+            k = (int)(t / ((long)pd.v[0] * pd.v[1])) + 1;
+  double cm[3], cp[3], c0[3];
+  for (int q = 0; q < 2; q++) {
+    cm[q + 1] = (double)((2 - q) * (2 - q) * (2 - q) - (1 - q) * (1 - q) * (1 - q)) / 48.0;
+    cp[q + 1] = (double)(3 * q * q + 3 * q + 1) / 48.0;
+    c0[q + 1] = 1.0 / 2 - cm[q + 1] - cp[q + 1];
+  }
+  const int ip = i + 1, jp = j + 1, kp = k + 1;  // parent of the children
+  double g[2][2][2];  // [k1][j1][i1]
+  for (int k1 = 0; k1 < 2; k1++) {
+    const double wm3 = cm[k1 + 1], w03 = c0[k1 + 1], wp3 = cp[k1 + 1];
+    for (int j1 = 0; j1 < 2; j1++) {
+      const double wm2 = cm[j1 + 1], w02 = c0[j1 + 1], wp2 = cp[j1 + 1];
+      for (int i1 = 0; i1 < 2; i1++) {
+        const double wm1 = cm[i1 + 1], w01 = c0[i1 + 1], wp1 = cp[i1 + 1];
+        g[k1][j1][i1] =
+            wm1 * wm2 * wm3 * PRO_P(ip - 1, jp - 1, kp - 1) +
+            w01 * wm2 * wm3 * PRO_P(ip, jp - 1, kp - 1) +
+            wp1 * wm2 * wm3 * PRO_P(ip + 1, jp - 1, kp - 1) +
+            wm1 * w02 * wm3 * PRO_P(ip - 1, jp, kp - 1) +
+            w01 * w02 * wm3 * PRO_P(ip, jp, kp - 1) +
+            wp1 * w02 * wm3 * PRO_P(ip + 1, jp, kp - 1) +
+            wm1 * wp2 * wm3 * PRO_P(ip - 1, jp + 1, kp - 1) +
+            w01 * wp2 * wm3 * PRO_P(ip, jp + 1, kp - 1) +
+            wp1 * wp2 * wm3 * PRO_P(ip + 1, jp + 1, kp - 1) +
+            wm1 * wm2 * w03 * PRO_P(ip - 1, jp - 1, kp) +
+            w01 * wm2 * w03 * PRO_P(ip, jp - 1, kp) +
+            wp1 * wm2 * w03 * PRO_P(ip + 1, jp - 1, kp) +
+            wm1 * w02 * w03 * PRO_P(ip - 1, jp, kp) +
+            w01 * w02 * w03 * PRO_P(ip, jp, kp) +
+            wp1 * w02 * w03 * PRO_P(ip + 1, jp, kp) +
+            wm1 * wp2 * w03 * PRO_P(ip - 1, jp + 1, kp) +
+            w01 * wp2 * w03 * PRO_P(ip, jp + 1, kp) +
+            wp1 * wp2 * w03 * PRO_P(ip + 1, jp + 1, kp) +
+            wm1 * wm2 * wp3 * PRO_P(ip - 1, jp - 1, kp + 1) +
+            w01 * wm2 * wp3 * PRO_P(ip, jp - 1, kp + 1) +
+            wp1 * wm2 * wp3 * PRO_P(ip + 1, jp - 1, kp + 1) +
+            wm1 * w02 * wp3 * PRO_P(ip - 1, jp, kp + 1) +
+            w01 * w02 * wp3 * PRO_P(ip, jp, kp + 1) +
+            wp1 * w02 * wp3 * PRO_P(ip + 1, jp, kp + 1) +
+            wm1 * wp2 * wp3 * PRO_P(ip - 1, jp + 1, kp + 1) +
+            w01 * wp2 * wp3 * PRO_P(ip, jp + 1, kp + 1) +
+            wp1 * wp2 * wp3 * PRO_P(ip + 1, jp + 1, kp + 1);
+      }
+    }
+  }
+  double temp = 0.0;
+  for (int k1 = 0; k1 < 2; k1++)
+    for (int j1 = 0; j1 < 2; j1++)
+      for (int i1 = 0; i1 < 2; i1++) temp = temp + g[k1][j1][i1];
+  temp = (PRO_P(1 + i, 1 + j, 1 + k) - temp) / (double)(2 * 2 * 2);
+  for (int k1 = 0; k1 < 2; k1++)
+    for (int j1 = 0; j1 < 2; j1++)
+      for (int i1 = 0; i1 < 2; i1++) g[k1][j1][i1] = g[k1][j1][i1] + temp;
+  const double frac = 1.0;
+  for (int iter = 0; iter < 2; iter++) {
+    for (int k1 = 0; k1 < 2; k1++)
+      for (int j1 = 0; j1 < 2; j1++) {
+        double tm = 0.0;
+        for (int i1 = 0; i1 < 2; i1++) tm = tm + g[k1][j1][i1] / 2.0;
+        for (int i1 = 0; i1 < 2; i1++) g[k1][j1][i1] = frac * tm + (1.0 - frac) * g[k1][j1][i1];
+      }
+    for (int k1 = 0; k1 < 2; k1++)
+      for (int i1 = 0; i1 < 2; i1++) {
+        double tm = 0.0;
+        for (int j1 = 0; j1 < 2; j1++) tm = tm + g[k1][j1][i1] / 2.0;
+        for (int j1 = 0; j1 < 2; j1++) g[k1][j1][i1] = frac * tm + (1.0 - frac) * g[k1][j1][i1];
+      }
+    for (int j1 = 0; j1 < 2; j1++)
+      for (int i1 = 0; i1 < 2; i1++) {
+        double tm = 0.0;
+        for (int k1 = 0; k1 < 2; k1++) tm = tm + g[k1][j1][i1] / 2.0;
+        for (int k1 = 0; k1 < 2; k1++) g[k1][j1][i1] = frac * tm + (1.0 - frac) * g[k1][j1][i1];
+      }
+  }
+  for (int k1 = 0; k1 < 2; k1++)
+    for (int j1 = 0; j1 < 2; j1++)
+      for (int i1 = 0; i1 < 2; i1++)
+        PRO_G(1 + (i - 1) * 2 + i1, 1 + (j - 1) * 2 + j1, 1 + (k - 1) * 2 + k1) =
+            g[k1][j1][i1] * 2.0 * 2.0 * 2.0;
+}
+
+// interp3d's first loop over the (pd+1)^3 corners: flags[0] counts corners
+// or cells that are not positive (nneg), flags[1] a NaN coarse value
+__global__ void k_op_prolong_2a_scan(const double *__restrict__ pc, Int3 mc, Int3 pd,
+                                     int *__restrict__ flags) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n0 = pd.v[0] + 1, n1 = pd.v[1] + 1, n2 = pd.v[2] + 1;
+  if (t >= (long)n0 * n1 * n2) return;
+  const int i = (int)(t % n0) + 1, j = (int)((t / n0) % n1) + 1, k = (int)(t / ((long)n0 * n1)) + 1;
+  const double w = pro_corner(pc, mc, i, j, k);
+  const double pv = PRO_P(1 + i, 1 + j, 1 + k);
+  if (pv != pv) atomicOr(&flags[1], 1);
+  if (w <= 0.0 || pv <= 0.0) atomicAdd(&flags[0], 1);
+}
+
+// interp3d ("2A"): min-limited corner slopes along the four diagonals; with
+// flags from the scan (nothing is written after a NaN coarse value, as the
+// routine returns there); flags[2] is set where the routine would return
+// with its error (a non-positive child of a positive field, a NaN slope).
+__global__ void k_op_prolong_2a(double *__restrict__ gf, Int3 mf, const double *__restrict__ pc,
+                                Int3 mc, Int3 pd, int *__restrict__ flags) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)pd.v[0] * pd.v[1] * pd.v[2]) return;
+  if (flags[1]) return;
+  const int nneg = flags[0];
+  const int i = (int)(t % pd.v[0]) + 1, j = (int)((t / pd.v[0]) % pd.v[1]) + 1,
+            k = (int)(t / ((long)pd.v[0] * pd.v[1])) + 1;
+  const double w000 = pro_corner(pc, mc, i, j, k), w111 = pro_corner(pc, mc, i + 1, j + 1, k + 1),
+               w100 = pro_corner(pc, mc, i + 1, j, k), w011 = pro_corner(pc, mc, i, j + 1, k + 1),
+               w010 = pro_corner(pc, mc, i, j + 1, k), w101 = pro_corner(pc, mc, i + 1, j, k + 1),
+               w001 = pro_corner(pc, mc, i, j, k + 1), w110 = pro_corner(pc, mc, i + 1, j + 1, k);
+  const double fbar = PRO_P(1 + i, 1 + j, 1 + k);
+  double d0 = pro_min(fabs(fbar - w000), fabs(fbar - w111)) * pro_sign(1.0, fbar - w000);
+  if ((w111 - fbar) * (fbar - w000) <= 0.0) d0 = 0.0;
+  double d1 = pro_min(fabs(fbar - w100), fabs(fbar - w011)) * pro_sign(1.0, fbar - w100);
+  if ((w011 - fbar) * (fbar - w100) <= 0.0) d1 = 0.0;
+  double d2 = pro_min(fabs(fbar - w010), fabs(fbar - w101)) * pro_sign(1.0, fbar - w010);
+  if ((w101 - fbar) * (fbar - w010) <= 0.0) d2 = 0.0;
+  double d3 = pro_min(fabs(fbar - w001), fabs(fbar - w110)) * pro_sign(1.0, fbar - w001);
+  if ((w110 - fbar) * (fbar - w001) <= 0.0) d3 = 0.0;
+  double s, sp;
+  if (d0 == 0.0) {
+    d0 = 1.0e-5 * pro_sign(1.0e-20, (d1 + d2 + d3));
+    sp = 1.0;
+  } else {
+    s = (d1 + d2 + d3) / d0;
+    sp = pro_min(pro_max(1.0 * s, 0.0), 1.0);
+  }
+  double c1 = 1.0, c2 = 1.0, c3 = 1.0;
+  if (d1 / d0 > 0.0) c1 = 0.0;
+  if (d2 / d0 > 0.0) c2 = 0.0;
+  if (d3 / d0 > 0.0) c3 = 0.0;
+  c1 = (1.0 - sp) * c1 + sp * (1.0 - c1);
+  c2 = (1.0 - sp) * c2 + sp * (1.0 - c2);
+  c3 = (1.0 - sp) * c3 + sp * (1.0 - c3);
+  if (sp != 0.0 && sp != 1.0) {
+    c1 = 0.0;
+    c2 = 0.0;
+    c3 = 0.0;
+  }
+  double frac = -((-d0 * sp) + (1.0 - c1) * d1 + (1.0 - c2) * d2 + (1.0 - c3) * d3) /
+                (c1 * d1 + c2 * d2 + c3 * d3 + 1.0e-35);
+  frac = pro_min(frac, 1.0);
+  if (c1 + c2 + c3 == 0) frac = 0.0;
+  frac = pro_max(frac, 0.0);
+  const double d1n = frac * c1 * d1 + (1.0 - c1) * d1;
+  const double d2n = frac * c2 * d2 + (1.0 - c2) * d2;
+  const double d3n = frac * c3 * d3 + (1.0 - c3) * d3;
+  s = d1n + d2n + d3n;
+  if ((nneg == 0 && (fbar + s <= 0.0 || fbar - s <= 0.0)) || s != s) atomicOr(&flags[2], 1);
+  pro_children(gf, mf, i, j, k, fbar, d2n + d3n, d1n + d3n, d1n + d2n);
+}
+
+// int_lin3d ("2B"): corner-difference slopes, positivity (iposflag = 1)
+// limiting them to 0.8 of the parent value
+__global__ void k_op_prolong_2b(double *__restrict__ gf, Int3 mf, const double *__restrict__ pc,
+                                Int3 mc, Int3 pd, int positive) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)pd.v[0] * pd.v[1] * pd.v[2]) return;
+  const int i = (int)(t % pd.v[0]) + 1, j = (int)((t / pd.v[0]) % pd.v[1]) + 1,
+            k = (int)(t / ((long)pd.v[0] * pd.v[1])) + 1;
+  const double fbar = PRO_P(1 + i, 1 + j, 1 + k);
+  double d0 = 0.5 * (pro_corner(pc, mc, i + 1, j + 1, k + 1) - pro_corner(pc, mc, i, j, k));
+  double d1 = 0.5 * (pro_corner(pc, mc, i, j + 1, k + 1) - pro_corner(pc, mc, i + 1, j, k));
+  double d2 = 0.5 * (pro_corner(pc, mc, i + 1, j, k + 1) - pro_corner(pc, mc, i, j + 1, k));
+  double d3 = 0.5 * (pro_corner(pc, mc, i + 1, j + 1, k) - pro_corner(pc, mc, i, j, k + 1));
+  if (positive == 1) {
+    if (fbar + d0 < 0.0) d0 = -0.8 * fbar;
+    if (fbar - d0 < 0.0) d0 = 0.8 * fbar;
+    if (fbar + d1 < 0.0) d1 = -0.8 * fbar;
+    if (fbar - d1 < 0.0) d1 = 0.8 * fbar;
+    if (fbar + d2 < 0.0) d2 = -0.8 * fbar;
+    if (fbar - d2 < 0.0) d2 = 0.8 * fbar;
+    if (fbar + d3 < 0.0) d3 = -0.8 * fbar;
+    if (fbar - d3 < 0.0) d3 = 0.8 * fbar;
+  }
+  if (d0 == 0.0) d0 = pro_sign(1.0e-20, d0 + d1 + d2);
+  const double s = (d1 + d2 + d3) / d0;
+  const double sp = pro_min(pro_max(s, 0.0), 1.0);
+  double c1 = 1.0, c2 = 1.0, c3 = 1.0;
+  if (d1 / d0 >= 0.0) c1 = 0.0;
+  if (d2 / d0 >= 0.0) c2 = 0.0;
+  if (d3 / d0 >= 0.0) c3 = 0.0;
+  c1 = (1.0 - sp) * c1 + sp * (1.0 - c1);
+  c2 = (1.0 - sp) * c2 + sp * (1.0 - c2);
+  c3 = (1.0 - sp) * c3 + sp * (1.0 - c3);
+  if (sp != 0.0 && sp != 1.0) {
+    c1 = 0.0;
+    c2 = 0.0;
+    c3 = 0.0;
+  }
+  double frac = -((-d0 * sp) + (1.0 - c1) * d1 + (1.0 - c2) * d2 + (1.0 - c3) * d3) /
+                (c1 * d1 + c2 * d2 + c3 * d3 + 1.0e-20);
+  frac = pro_min(frac, 1.0);
+  const double d1n = frac * c1 * d1 + (1.0 - c1) * d1;
+  const double d2n = frac * c2 * d2 + (1.0 - c2) * d2;
+  const double d3n = frac * c3 * d3 + (1.0 - c3) * d3;
+  pro_children(gf, mf, i, j, k, fbar, d2n + d3n, d1n + d3n, d1n + d2n);
+}
+
+#undef PRO_P
+#undef PRO_G
+
 __global__ void k_op_prolong_linear(double *__restrict__ vf, Int3 mf, Int3 of,
                                     Int3 nf, const double *__restrict__ vc,
                                     Int3 mc, Int3 oc, Int3 gc, int rank,
@@ -1747,6 +2001,63 @@ int enzoe_prolong_linear(double *values_f, const int mf3[3], const int of3[3],
                      (hipStream_t)stream, values_f, mf, of, nf, values_c, mc, oc, gc,
                      rank, accumulate ? 1 : 0);
   return launched();
+}
+
+int enzoe_enzo_prolong(int method, int positive, double *values_f, const int mf3[3],
+                       const int of3[3], const int nf3[3], const double *values_c,
+                       const int mc3[3], const int oc3[3], const int nc3[3], int *ierror,
+                       enzoe_stream_t stream) {
+  if (!values_f || !values_c || !mf3 || !of3 || !nf3 || !mc3 || !oc3 || !nc3)
+    return ENZOE_ERR_ARG;
+  if (method < 0 || method > 2) return ENZOE_ERR_CONFIG;
+  for (int a = 0; a < 3; a++) {
+    if (nf3[a] < 2 || nf3[a] % 2 != 0 || nc3[a] != nf3[a] / 2 + 2) return ENZOE_ERR_ARG;
+    if (oc3[a] < 0 || oc3[a] + nc3[a] > mc3[a]) return ENZOE_ERR_ARG;
+    if (of3[a] < 0 || of3[a] + nf3[a] > mf3[a]) return ENZOE_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Int3 mf, mc, pd;
+  for (int a = 0; a < 3; a++) {
+    mf.v[a] = mf3[a];
+    mc.v[a] = mc3[a];
+    pd.v[a] = nf3[a] / 2;
+  }
+  const double *pc = values_c + oc3[0] + (long)mc3[0] * (oc3[1] + (long)mc3[1] * oc3[2]);
+  double *gf = values_f + of3[0] + (long)mf3[0] * (of3[1] + (long)mf3[1] * of3[2]);
+  const long parents = (long)pd.v[0] * pd.v[1] * pd.v[2];
+  if (ierror) *ierror = 0;
+  if (method == 0) {
+    hipLaunchKernelGGL(k_op_prolong_3a, dim3(nblk(parents)), dim3(256), 0, s, gf, mf, pc, mc, pd);
+    return launched();
+  }
+  if (method == 2) {
+    hipLaunchKernelGGL(k_op_prolong_2b, dim3(nblk(parents)), dim3(256), 0, s, gf, mf, pc, mc, pd,
+                       positive == 1 ? 1 : 0);
+    return launched();
+  }
+  // 2A: a scan of the corners first (the routine's count of non-positive
+  // values decides its error test; a NaN coarse value stops it)
+  int *flags = nullptr;
+  if (hipMalloc((void **)&flags, 3 * sizeof(int)) != hipSuccess) return ENZOE_ERR_ALLOC;
+  int rc = ENZOE_OK;
+  if (hipMemsetAsync(flags, 0, 3 * sizeof(int), s) != hipSuccess) rc = ENZOE_ERR_HIP;
+  if (!rc) {
+    const long corners = (long)(pd.v[0] + 1) * (pd.v[1] + 1) * (pd.v[2] + 1);
+    hipLaunchKernelGGL(k_op_prolong_2a_scan, dim3(nblk(corners)), dim3(256), 0, s, pc, mc, pd,
+                       flags);
+    hipLaunchKernelGGL(k_op_prolong_2a, dim3(nblk(parents)), dim3(256), 0, s, gf, mf, pc, mc, pd,
+                       flags);
+    rc = launched();
+  }
+  int host_flags[3] = {0, 0, 0};
+  if (!rc && (hipMemcpyAsync(host_flags, flags, sizeof(host_flags), hipMemcpyDeviceToHost, s) !=
+                  hipSuccess ||
+              hipStreamSynchronize(s) != hipSuccess))
+    rc = ENZOE_ERR_HIP;
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(flags);
+  if (!rc && ierror) *ierror = (host_flags[1] || host_flags[2]) ? 1 : 0;
+  return rc;
 }
 
 int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,

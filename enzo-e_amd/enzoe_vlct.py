@@ -152,6 +152,14 @@ def lib():
                                               ctypes.POINTER(ctypes.c_int), ctypes.c_double,
                                               ctypes.c_double, ctypes.c_int,
                                               ctypes.POINTER(vp)]
+        L.enzoe_vlct_levels_create.argtypes = [vp, ctypes.POINTER(Block),
+                                               ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                               ctypes.POINTER(vp)]
+        L.enzoe_vlct_levels_step.argtypes = [vp, vp]
+        L.enzoe_vlct_levels_capture.argtypes = [vp, ctypes.c_int]
+        L.enzoe_vlct_levels_replay.argtypes = [vp, vp]
+        L.enzoe_vlct_levels_state.argtypes = [vp, vp, _dp, _dp]
+        L.enzoe_vlct_levels_destroy.argtypes = [vp]
         L.enzoe_vlct_graph_launch.argtypes = [vp, vp]
         L.enzoe_vlct_graph_time.argtypes = [vp, vp, _dp]
         L.enzoe_vlct_graph_destroy.argtypes = [vp]
@@ -160,6 +168,9 @@ def lib():
                                          ctypes.POINTER(ctypes.c_int), vp]
         L.enzoe_vlct_set_debug_checks.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_set_fused_ct.argtypes = [vp, ctypes.c_int]
+        L.enzoe_vlct_launch_choice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_int)]
         L.enzoe_vlct_compute_noalloc.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                                  ctypes.c_double, vp]
         L.enzoe_vlct_noalloc_bytes.restype = ctypes.c_size_t
@@ -366,11 +377,20 @@ class VlctMethod:
                 "max_abs_divb": d.max_abs_divb, "max_abs_bface": d.max_abs_bface,
                 "nonfinite": int(d.nonfinite)}
 
-    def set_fused_ct(self, on):
-        """One fused launch for the constrained transport and cell update of
-        an MHD stage (the default) or three launches through memory."""
-        check(lib().enzoe_vlct_set_fused_ct(self.handle, 1 if on else 0),
-              "enzoe_vlct_set_fused_ct")
+    def set_fused_ct(self, variant):
+        """Launch structure of the MHD stages: 0 (or False) three launches
+        through memory, 1..3 (True = 1) one fused CT + update launch with
+        tile 16x4x4 / 32x4x2 / 8x8x4, -1 the autotuner's choice."""
+        v = int(variant) if not isinstance(variant, bool) else (1 if variant else 0)
+        check(lib().enzoe_vlct_set_fused_ct(self.handle, v), "enzoe_vlct_set_fused_ct")
+
+    def launch_choice(self, shape):
+        """(XCD mask, fused-CT variant) used for blocks of shape (mx, my, mz)."""
+        mask, ctu = ctypes.c_int(), ctypes.c_int()
+        check(lib().enzoe_vlct_launch_choice(self.handle, shape[0], shape[1], shape[2],
+                                             ctypes.byref(mask), ctypes.byref(ctu)),
+              "enzoe_vlct_launch_choice")
+        return mask.value, ctu.value
 
     def state_l1(self, blocks, ref_blocks, stream=None):
         """L1 of the blocks' cell-centred fields against ref_blocks' (same
@@ -505,6 +525,52 @@ class DeviceGrid:
         return n, log
 
 
+class Levels:
+    """enzoe_vlct_levels_*: sub-cycled refinement levels stepped on the
+    device (per-level CFL dt, dt_0 = min_l 2^l dt_l, 2^(L-1) batched
+    substeps), eagerly or from a captured HIP graph.  levels[l] is the list
+    of DeviceBlocks of level l (0 = coarsest)."""
+
+    def __init__(self, method, levels):
+        self.method = method
+        self.levels = [list(lv) for lv in levels]
+        blocks = [b for lv in self.levels for b in lv]
+        level_of = [l for l, lv in enumerate(self.levels) for _ in lv]
+        arr = (Block * len(blocks))(*[b.desc for b in blocks])
+        lev = (ctypes.c_int * len(blocks))(*level_of)
+        self.handle = ctypes.c_void_p()
+        check(lib().enzoe_vlct_levels_create(method.handle, arr, lev, len(blocks),
+                                             ctypes.byref(self.handle)),
+              "enzoe_vlct_levels_create")
+
+    def step(self, stream=None):
+        check(lib().enzoe_vlct_levels_step(self.handle, stream), "enzoe_vlct_levels_step")
+
+    def capture(self, nsteps):
+        check(lib().enzoe_vlct_levels_capture(self.handle, nsteps), "enzoe_vlct_levels_capture")
+
+    def replay(self, stream=None):
+        check(lib().enzoe_vlct_levels_replay(self.handle, stream), "enzoe_vlct_levels_replay")
+
+    def state(self, stream=None):
+        """(device time, dt_0 of the last step)."""
+        t, d = ctypes.c_double(), ctypes.c_double()
+        check(lib().enzoe_vlct_levels_state(self.handle, stream, ctypes.byref(t),
+                                            ctypes.byref(d)), "enzoe_vlct_levels_state")
+        return t.value, d.value
+
+    def destroy(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().enzoe_vlct_levels_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class CycleGraph:
     """enzoe_vlct_graph_*: cycles of a DeviceGrid captured in a HIP graph."""
 
@@ -629,6 +695,8 @@ def ops():
         L.enzoe_flux_correct.argtypes = [ctypes.POINTER(vp), i, i, PI, vp, i, i, i, i,
                                          PI, vp]
         L.enzoe_prolong_linear.argtypes = [vp, PI, PI, PI, vp, PI, PI, PI, i, vp]
+        L.enzoe_enzo_prolong.argtypes = [i, i, vp, PI, PI, PI, vp, PI, PI, PI,
+                                         ctypes.POINTER(i), vp]
         L.enzoe_vlct_transfer_batch.argtypes = [vp, ctypes.POINTER(Transfer), i, vp]
         L.enzoe_restrict_linear.argtypes = [vp, PI, PI, PI, vp, PI, PI, PI, i, vp]
         L.enzoe_vlct_compute_gravity.argtypes = [vp, ctypes.POINTER(Block), i, d,
@@ -851,6 +919,9 @@ def host_lib():
                                                        _dp]
         H.enzoe_host_initial_shock_tube.argtypes = [ctypes.c_char_p, ctypes.POINTER(Block),
                                                     _dp, _dp]
+        H.enzoe_host_enzo_prolong_apply.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                                    ctypes.c_int, vp, PI, PI, PI, vp, PI,
+                                                    PI, PI, ctypes.c_int, vp]
         H.enzoe_host_prolong_apply.argtypes = [ctypes.c_char_p, ctypes.c_int, vp, PI, PI, PI,
                                                vp, PI, PI, PI, ctypes.c_int, vp]
         H.enzoe_host_restrict_apply.argtypes = [ctypes.c_char_p, vp, PI, PI, PI, vp, PI, PI,
@@ -962,6 +1033,22 @@ def prolong_linear(values_f, mf3, of3, nf3, values_c, mc3, oc3, nc3, accumulate=
     check(ops().enzoe_prolong_linear(values_f, _ints(mf3), _ints(of3), _ints(nf3), values_c,
                                      _ints(mc3), _ints(oc3), _ints(nc3), int(accumulate),
                                      stream), "enzoe_prolong_linear")
+
+
+ENZO_PROLONG_METHODS = {"3A": 0, "2A": 1, "2B": 2}
+
+
+def enzo_prolong(method, values_f, mf3, of3, nf3, values_c, mc3, oc3, nc3, positive=False,
+                 stream=None):
+    """EnzoProlong's interpolate() ("3A", "2A", "2B") on device pointers
+    (dims x, y, z; nc3 = nf3/2 + 2).  Returns interp3d's error flag (2A;
+    0 otherwise)."""
+    ierr = ctypes.c_int(0)
+    check(ops().enzoe_enzo_prolong(ENZO_PROLONG_METHODS[method], int(bool(positive)), values_f,
+                                   _ints(mf3), _ints(of3), _ints(nf3), values_c, _ints(mc3),
+                                   _ints(oc3), _ints(nc3), ctypes.byref(ierr), stream),
+          "enzoe_enzo_prolong")
+    return ierr.value
 
 
 def restrict_linear(values_c, nd3_c, im3_c, n3_c, values_f, nd3_f, im3_f, n3_f,

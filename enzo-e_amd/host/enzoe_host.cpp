@@ -1674,11 +1674,20 @@ void EnzoProlong::apply(precision_type precision, void *values_f, int m3_f[3],
                         int o3_f[3], int n3_f[3], const void *values_c, int m3_c[3],
                         int o3_c[3], int n3_c[3], bool accumulate) {
   ProlongLinear prolong_linear;
-  if (!accumulate) {
+  if (!accumulate && !use_linear_) {
+    // EnzoProlong.cpp:64-74 -> apply_ (:112-190), interpolate() on the device
+    const int rank = (m3_f[1] == 1) ? 1 : ((m3_f[2] == 1) ? 2 : 3);
     ENZOE_ASSERT("EnzoProlong::apply()",
-                 "Enzo's interpolate() (use_linear = false) is not provided by the "
-                 "device library",
-                 use_linear_);
+                 "Enzo's interpolate() is provided for rank 3 only", rank == 3);
+    ENZOE_ASSERT("EnzoProlong::apply()",
+                 "Unknown precision " + std::to_string(precision) +
+                     " (double precision only)",
+                 precision == precision_double || precision == precision_default);
+    check_rc(enzoe_enzo_prolong(method_, positive_, static_cast<double *>(values_f), m3_f,
+                                o3_f, n3_f, static_cast<const double *>(values_c), m3_c,
+                                o3_c, n3_c, nullptr, g_stream),
+             "EnzoProlong::apply_");
+  } else if (!accumulate) {
     // EnzoProlong.cpp:62-78: the coarse region without its padding
     const int rank = (m3_f[1] == 1) ? 1 : ((m3_f[2] == 1) ? 2 : 3);
     int n3[3] = {n3_c[0], n3_c[1], n3_c[2]}, o3[3] = {o3_c[0], o3_c[1], o3_c[2]};

@@ -788,9 +788,10 @@ class ProlongLinear : public Prolong {
 };
 
 // src/Enzo/mesh/EnzoProlong.cpp: types "3A", "2A", "2B"; with use_linear
-// (and for accumulate) it is ProlongLinear on the coarse region shrunk by
-// one cell per side (:49-97).  Enzo's Fortran interpolate() behind
-// use_linear = false is not provided (ERROR).
+// it is ProlongLinear on the coarse region shrunk by one cell per side, for
+// accumulate ProlongLinear as given (:49-97); otherwise Enzo's
+// interpolate() (apply_, :112-190) on the device (enzoe_enzo_prolong,
+// rank 3).
 class EnzoProlong : public Prolong {
  public:
   EnzoProlong(std::string type, bool positive, bool use_linear);

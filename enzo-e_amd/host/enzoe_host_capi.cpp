@@ -215,6 +215,23 @@ int enzoe_host_prolong_apply(const char *name, int use_linear, double *values_f,
   return ENZOE_OK;
 }
 
+int enzoe_host_enzo_prolong_apply(const char *type, int positive, int use_linear,
+                                  double *values_f, const int m3_f[3], const int o3_f[3],
+                                  const int n3_f[3], const double *values_c,
+                                  const int m3_c[3], const int o3_c[3], const int n3_c[3],
+                                  int accumulate, enzoe_stream_t stream) {
+  if (!type) return ENZOE_ERR_ARG;
+  enzoe_host_set_stream(stream);
+  EnzoProlong p(type, positive != 0, use_linear != 0);
+  int mf[3], of[3], nf[3], mc[3], oc[3], nc[3];
+  for (int a = 0; a < 3; a++) {
+    mf[a] = m3_f[a]; of[a] = o3_f[a]; nf[a] = n3_f[a];
+    mc[a] = m3_c[a]; oc[a] = o3_c[a]; nc[a] = n3_c[a];
+  }
+  p.apply(precision_double, values_f, mf, of, nf, values_c, mc, oc, nc, accumulate != 0);
+  return ENZOE_OK;
+}
+
 int enzoe_host_restrict_apply(const char *name, double *values_c, const int nd3_c[3],
                               const int im3_c[3], const int n3_c[3], const double *values_f,
                               const int nd3_f[3], const int im3_f[3], const int n3_f[3],

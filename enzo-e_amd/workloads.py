@@ -61,6 +61,11 @@ class SubcycledLevels:
     def launch_sizes(self):
         return [sum(len(self.levels[lev]) for lev in act) for act in self.active]
 
+    def device_levels(self):
+        """The same levels as an enzoe_vlct_levels object: coarse steps with
+        the dt on the device, eagerly or replayed from a HIP graph."""
+        return ev.Levels(self.method, self.levels)
+
     def coarse_step(self):
         m = self.method
         cfl = [m.timestep(blks) for blks in self.levels]

@@ -158,15 +158,22 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                           const double *dt, int dt_on_device,
                           enzoe_stream_t stream);
 
-/* Launch-shape autotuner (no counterpart in the reference): times every
- * combination of XCD-contiguous workgroup order over the four stage kernels
- * on a scratch copy of the leading blocks (at most ~2 GiB; the caller's
- * fields are not touched), advancing them by dt, and keeps the fastest for
- * this block shape; later compute calls on blocks of that shape use it.
- * *mask_out (optional) gets the chosen mask: bit 0 k_flux, 1 k_edge,
- * 2 k_faces, 3 k_cells.                                                     */
+/* Launch-shape autotuner (no counterpart in the reference): times the
+ * launch structures on a scratch copy of the leading blocks (at most ~2 GiB;
+ * the caller's fields are not touched), advancing them by dt, and keeps the
+ * fastest for this block shape; later compute calls on blocks of that shape
+ * use it.  The candidates: every combination of XCD-contiguous workgroup
+ * order over the four stage kernels (k_flux, k_edge, k_faces, k_cells)
+ * and, for MHD with the fused-CT setting "auto" (enzoe_vlct_set_fused_ct),
+ * the fused CT + update launch with each of its tile shapes and the orders
+ * of its two kernels.  *mask_out (optional) gets the chosen mask: bit 0
+ * k_flux, 1 k_edge, 2 k_faces, 3 k_cells (or the fused launch).
+ * enzoe_vlct_launch_choice: the mask and the fused-CT variant (0 = three
+ * launches, 1..3 = tile) calls on blocks of this shape use.               */
 int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                         double dt, enzoe_stream_t stream, int *mask_out);
+int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask,
+                             int *fused_ct);
 
 /* EnzoMethodMHDVlct::timestep + the clip of control_stopping.cpp:83-86
  * without a host round trip: reads the time from *d_time (device), writes
@@ -178,6 +185,31 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
 int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                                double *d_time, double t_stop, double *d_dts,
                                int ndts, enzoe_stream_t stream);
+
+/* Sub-cycled refinement levels advanced together on the device (no
+ * counterpart in the reference, which takes one global dt): blocks[i] lies
+ * on level level_of[i] (0 = coarsest; every level below the finest must
+ * hold blocks).  One coarse step: the CFL minimum of every level (the
+ * pressure field written, as enzoe_vlct_timestep), dt_0 = min_l 2^l
+ * courant * min_l, then 2^(L-1) substeps; substep s advances the levels l
+ * with s % 2^(L-1-l) == 0, each block with dt_0 / 2^l, as one batched
+ * launch sequence per run of equal block shapes (each block equal bit for
+ * bit to enzoe_vlct_compute_dt of the same list with those dt).  Every
+ * table and buffer is prepared at create, so a step launches kernels only:
+ * enzoe_vlct_levels_capture records `nsteps` steps into a HIP graph that
+ * enzoe_vlct_levels_replay launches with no host work.  The blocks must
+ * stay valid and unmoved; the ghost zones are not refreshed (each block is
+ * advanced as a self-contained Method input).  levels_state: the device
+ * time (the sum of the steps' dt_0) and the last dt_0; synchronises.      */
+typedef struct enzoe_vlct_levels_s *enzoe_vlct_levels_t;
+int enzoe_vlct_levels_create(enzoe_vlct_t m, const enzoe_block *blocks, const int *level_of,
+                             int n, enzoe_vlct_levels_t *out);
+int enzoe_vlct_levels_step(enzoe_vlct_levels_t levels, enzoe_stream_t stream);
+int enzoe_vlct_levels_capture(enzoe_vlct_levels_t levels, int nsteps);
+int enzoe_vlct_levels_replay(enzoe_vlct_levels_t levels, enzoe_stream_t stream);
+int enzoe_vlct_levels_state(enzoe_vlct_levels_t levels, enzoe_stream_t stream, double *time,
+                            double *dt0);
+int enzoe_vlct_levels_destroy(enzoe_vlct_levels_t levels);
 
 /* A HIP graph of `ncycles` Cello cycles of a uniform periodic/outflow grid
  * of blocks (device dt + refresh + compute), captured once and replayed
@@ -585,6 +617,29 @@ int enzoe_prolong_linear(double *values_f, const int mf3[3], const int of3[3],
                          const int mc3[3], const int oc3[3], const int nc3[3],
                          int accumulate, enzoe_stream_t stream);
 
+/* EnzoProlong::apply with use_linear = false and accumulate = false
+ * (src/Enzo/mesh/EnzoProlong.cpp:64-74, apply_ :112-190): Enzo's
+ * interpolate() (src/Enzo/mesh/interpolate.F) with refinement 2, rank 3.
+ * method 0 = "3A" (tscint3d: triangular-shaped-cloud weights, a
+ * conservation correction, then the routine's monotonicity passes, which
+ * for finite data leave every child at its parent's value), 1 = "2A"
+ * (interp3d: min-limited slopes along the four cell diagonals from the
+ * corner averages), 2 = "2B" (int_lin3d: corner-difference slopes;
+ * positive = 1 limits them to 0.8 of the parent value).  Region (of3, nf3)
+ * of the fine array from the coarse region (oc3, nc3) that holds the
+ * parents and one cell of stencil on each side: nf3 even, nc3 = nf3/2 + 2
+ * (ENZOE_ERR_ARG otherwise; the reference checks less and reads past the
+ * region).  ierror (may be NULL; 2A only, which then synchronises the
+ * stream): 1 where interp3d sets its error flag -- a NaN coarse value (no
+ * child written, as the routine returns first) or a child of an
+ * all-positive field that would not be positive (every child is written
+ * here, where the routine stops at that row); EnzoProlong ignores the flag.
+ * The other cases of EnzoProlong::apply are enzoe_prolong_linear.           */
+int enzoe_enzo_prolong(int method, int positive, double *values_f, const int mf3[3],
+                       const int of3[3], const int nf3[3], const double *values_c,
+                       const int mc3[3], const int oc3[3], const int nc3[3], int *ierror,
+                       enzoe_stream_t stream);
+
 /* RestrictLinear::apply (src/Cello/problem_RestrictLinear.cpp:20-198):
  * region (im3_c, n3_c) of values_c (dims nd3_c) = mean of the 2^rank fine
  * children starting at im3_f of values_f (dims nd3_f); accumulate adds.    */
@@ -631,13 +686,15 @@ int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
  * density/pressure floors (EnzoReconstructorPLM.hpp:219-220).  The C++ layer
  * turns a nonzero count into the reference's ERROR.                         */
 /* Launch structure of the MHD stages (no reference counterpart: the
- * reference runs every operation as its own loop).  on: the constrained
- * transport and the cell update of a stage run as one fused launch over
- * tiles of cells, the edge EMFs and face-centred B of a tile held in LDS;
- * off (the default, faster on the measured shapes): three launches (edge
- * EMFs, face update, cell update) through memory.  Both give the same bits.
- * Not to be changed while work of the method is queued.                     */
-int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int on);
+ * reference runs every operation as its own loop).  variant 0: three
+ * launches (edge EMFs, face update, cell update) through memory; 1, 2, 3:
+ * the constrained transport and the cell update of a stage as one fused
+ * launch over tiles of 16x4x4, 32x4x2 or 8x8x4 cells, the edge EMFs and
+ * face-centred B of a tile held in LDS; -1 (the default): the autotuner's
+ * choice per block shape (three launches until it has run).  Every variant
+ * gives the same bits.  Not to be changed while work of the method is
+ * queued.                                                                   */
+int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int variant);
 
 #define ENZOE_DEBUG_NONFINITE 0
 #define ENZOE_DEBUG_DENSITY_FLOOR 1

@@ -122,16 +122,22 @@ int enzoe_host_initial_shock_tube(const char *parameters,
                                   const double lower[3], const double upper[3]);
 
 /* Prolong::apply of the C++ layer: name "linear" (ProlongLinear,
- * src/Cello/problem_ProlongLinear.cpp:25-370) or "enzo" (EnzoProlong "3A",
- * src/Enzo/mesh/EnzoProlong.cpp:49-97; only use_linear = 1 is provided: the
- * coarse region given with one cell of padding per side).  Device arrays,
- * dims (x, y, z).                                                          */
+ * src/Cello/problem_ProlongLinear.cpp:25-370) or "enzo" (EnzoProlong "3A"
+ * with positivity, src/Enzo/mesh/EnzoProlong.cpp:49-190; the coarse region
+ * given with one cell of padding per side).  Device arrays, dims (x, y, z).
+ * enzoe_host_enzo_prolong_apply: EnzoProlong(type, positive, use_linear)
+ * for any of the reference's types ("3A", "2A", "2B").                     */
 int enzoe_host_prolong_apply(const char *name, int use_linear, double *values_f,
                              const int m3_f[3], const int o3_f[3],
                              const int n3_f[3], const double *values_c,
                              const int m3_c[3], const int o3_c[3],
                              const int n3_c[3], int accumulate,
                              enzoe_stream_t stream);
+int enzoe_host_enzo_prolong_apply(const char *type, int positive, int use_linear,
+                                  double *values_f, const int m3_f[3], const int o3_f[3],
+                                  const int n3_f[3], const double *values_c,
+                                  const int m3_c[3], const int o3_c[3], const int n3_c[3],
+                                  int accumulate, enzoe_stream_t stream);
 /* Restrict::apply: "linear" (RestrictLinear, problem_RestrictLinear.cpp:
  * 20-198) or "enzo" (EnzoRestrict, which writes nothing,
  * src/Enzo/mesh/EnzoRestrict.cpp:36-45).  Returns the bytes written.       */

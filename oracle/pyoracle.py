@@ -147,6 +147,9 @@ def lib():
         L.o_prolong_linear.restype = ctypes.c_int
         L.o_restrict_linear.argtypes = [dp, ip, ip, ip, dp, ip, ip, ip, ctypes.c_int]
         L.o_restrict_linear.restype = ctypes.c_int
+        L.o_enzo_prolong.argtypes = [ctypes.c_int, ctypes.c_int, dp, ip, ip, ip, dp, ip, ip,
+                                     ip]
+        L.o_enzo_prolong.restype = ctypes.c_int
         _lib = L
     return _lib
 

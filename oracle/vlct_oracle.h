@@ -130,6 +130,18 @@ int o_restrict_linear(double *values_c, const int nd3_c[3], const int im3_c[3],
                       const int nd3_f[3], const int im3_f[3],
                       const int n3_f[3], int accumulate);
 
+/* EnzoProlong (src/Enzo/mesh/EnzoProlong.cpp:112-190 -> interpolate.F),
+ * rank 3, refinement 2, accumulate = false, use_linear = false (the other
+ * cases call ProlongLinear): method 0 "3A" (tscint3d), 1 "2A" (interp3d),
+ * 2 "2B" (int_lin3d with iposflag = positive).  Region (of3, nf3) of the
+ * fine array from the coarse region (oc3, nc3), nc3 = nf3/2 + 2 (one coarse
+ * cell of stencil on each side).  Returns 0; -1 for sizes outside that
+ * contract; -2 where interp3d returns with its error flag
+ * (prolong_oracle.c).                                                       */
+int o_enzo_prolong(int method, int positive, double *values_f, const int mf3[3],
+                   const int of3[3], const int nf3[3], const double *values_c,
+                   const int mc3[3], const int oc3[3], const int nc3[3]);
+
 /* EnzoMethodMHDVlct::compute with acceleration fields (accel[0..2] =
  * acceleration_x..z, (mz,my,mx)): the gravity source of
  * EnzoSourceGravity::calculate_source (toolkit/EnzoSourceGravity.cpp:17-69),

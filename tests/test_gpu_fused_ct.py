@@ -2,10 +2,11 @@
 three-launch path (k_edge, k_faces, k_cells) on the GPU: every field, the
 pressure field and the next dt equal bit for bit, for FP64 (plain and
 dual energy), the mixed path (FP32 fluxes, FP32 primitive half-step state),
-HLLE, two blocks in one batched launch with per-block dt, and blocks whose
-extents are not multiples of the 16 x 4 x 4 tile (partial tiles on every
-axis); the three-launch path (the default) is the one the oracle parity
-tests run."""
+HLLE, two blocks in one batched launch with per-block dt, every tile shape
+(16x4x4, 32x4x2, 8x8x4), and blocks whose extents are not multiples of the
+tiles (partial tiles on every axis); the three-launch path (the default
+until the autotuner has chosen) is the one the oracle parity tests run; the
+autotuner's choice keeps the results."""
 import numpy as np
 import pytest
 
@@ -27,16 +28,17 @@ def _equal(blocks_a, blocks_b):
             np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("riemann", ["hlld", "hlle"])
-def test_fused_ct_equals_three_launches(precision, riemann):
+def test_fused_ct_equals_three_launches(precision, riemann, variant):
     ev = _ev()
     grid = random_periodic_grid(mhd=True, seed=77, n=(21, 13, 11), vamp=0.5,
                                 riemann=riemann)
     runs = []
-    for fused in (True, False):
+    for v in (variant, 0):
         m = method_for(grid.cfg, precision=precision)
-        m.set_fused_ct(fused)
+        m.set_fused_ct(v)
         runs.append((m, device_copy(grid, m)))
     (mf, gf), (mu, gu) = runs
     dtf = mf.timestep(gf.blocks)
@@ -67,9 +69,9 @@ def test_fused_ct_two_blocks_per_block_dt(dual):
             arr["internal_energy"][...] = (arr["total_energy"] - 0.5 * v2 -
                                            0.5 * b2 / arr["density"])
     out = []
-    for fused in (True, False):
+    for v in (2, 0):
         m = method_for(grids[0].cfg)
-        m.set_fused_ct(fused)
+        m.set_fused_ct(v)
         dgs = [device_copy(g, m) for g in grids]
         blocks = [b for dg in dgs for b in dg.blocks]
         for _ in range(2):
@@ -79,3 +81,25 @@ def test_fused_ct_two_blocks_per_block_dt(dual):
         out.append(blocks)
     ev.synchronize()
     _equal(out[0], out[1])
+
+
+def test_autotuned_launch_structure_keeps_results():
+    """The autotuner (fused-CT setting "auto") picks a launch structure for
+    the shape among the three launches and the tiles; cycles after it equal
+    the oracle's."""
+    ev = _ev()
+    grid = random_periodic_grid(mhd=True, seed=31, n=(20, 14, 12), vamp=0.5)
+    m = method_for(grid.cfg)
+    dg = device_copy(grid, m)
+    mask = m.autotune(dg.blocks, 1e-3)
+    b = dg.blocks[0]
+    assert m.launch_choice((b.mx, b.my, b.mz))[0] == mask
+    assert m.launch_choice((b.mx, b.my, b.mz))[1] in (0, 1, 2, 3)
+    t = 0.0
+    for _ in range(2):
+        dt_o = grid.cycle(t, 10.0)
+        assert dg.cycle(t, 10.0) == dt_o
+        t += dt_o
+    ev.synchronize()
+    from test_gpu_parity import compare_grids
+    compare_grids(dg, grid)

@@ -5,6 +5,11 @@
   with and without coarse ghost cells; accumulate).
 * The C++ layer's EnzoProlong with use_linear is ProlongLinear on the coarse
   region without its padding; EnzoRestrict writes nothing.
+* enzoe_enzo_prolong (EnzoProlong's interpolate(): "3A", "2A", "2B" with and
+  without positivity) equals the oracle bit for bit on random data, inside
+  a larger fine array (cells outside the region untouched), and through the
+  C++ layer's EnzoProlong(type, positive, use_linear = false); a NaN coarse
+  value gives 2A's error flag and writes nothing, as the oracle.
 * The ghost zones on both sides of the refinement face of the two-level
   composite, filled from device data by prolongation and restriction,
   equal a linear field exactly (dyadic data): the regions line up.
@@ -158,3 +163,57 @@ def test_composite_interface_ghosts_reproduce_a_linear_field(batched):
             # the slab over the fine blocks' y-z extent is restricted, the
             # rest of the array is untouched
             assert np.array_equal(got, f)
+
+
+@pytest.mark.parametrize("method,positive", [("3A", 0), ("2A", 0), ("2B", 0), ("2B", 1)])
+def test_enzo_prolong_equals_the_oracle(method, positive):
+    ev = _ev()
+    H = ev.host_lib()
+    rng = np.random.default_rng(7 + 3 * positive + {"3A": 0, "2A": 1, "2B": 2}[method])
+    mc3, oc3, nf3, of3 = [13, 11, 9], [2, 1, 1], [16, 12, 10], [3, 2, 1]
+    nc3 = [n // 2 + 2 for n in nf3]
+    mf3 = [nf3[a] + of3[a] + 2 for a in range(3)]
+    # a positive field with structure on every scale (a smooth part, steps,
+    # noise), so the limiters take every branch
+    z, y, x = np.meshgrid(np.arange(mc3[2]), np.arange(mc3[1]), np.arange(mc3[0]),
+                          indexing="ij")
+    vc = (2.0 + np.sin(0.7 * x) * np.cos(0.5 * y) + 0.3 * z + (x > 6) * 1.5 +
+          0.4 * rng.standard_normal((mc3[2], mc3[1], mc3[0])))
+    vc = np.ascontiguousarray(np.where(rng.random(vc.shape) < 0.1, 0.05, vc))
+    fill = np.full((mf3[2], mf3[1], mf3[0]), -3.0)
+    mcode = {"3A": 0, "2A": 1, "2B": 2}[method]
+    ref = fill.copy()
+    assert po.lib().o_enzo_prolong(mcode, positive, po.dptr(ref), po.ints(mf3), po.ints(of3),
+                                   po.ints(nf3), po.dptr(vc), po.ints(mc3), po.ints(oc3),
+                                   po.ints(nc3)) == 0
+    dc, df, dh = _dev(vc), _dev(fill), _dev(fill)
+    assert ev.enzo_prolong(method, df.ptr.value, mf3, of3, nf3, dc.ptr.value, mc3, oc3, nc3,
+                           positive=positive) == 0
+    assert H.enzoe_host_enzo_prolong_apply(method.encode(), positive, 0, dh.ptr.value,
+                                           po.ints(mf3), po.ints(of3), po.ints(nf3),
+                                           dc.ptr.value, po.ints(mc3), po.ints(oc3),
+                                           po.ints(nc3), 0, None) == 0
+    ev.synchronize()
+    got = ev.from_device(df.ptr.value, fill.shape)
+    host = ev.from_device(dh.ptr.value, fill.shape)
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(host, ref)
+
+
+def test_enzo_prolong_2a_nan_sets_the_error_flag_and_writes_nothing():
+    ev = _ev()
+    mc3, oc3, nf3, of3 = [10, 8, 8], [1, 1, 1], [8, 6, 6], [0, 0, 0]
+    nc3 = [n // 2 + 2 for n in nf3]
+    mf3 = list(nf3)
+    vc = np.full((mc3[2], mc3[1], mc3[0]), 2.0)
+    vc[2, 2, 3] = np.nan
+    fill = np.full((mf3[2], mf3[1], mf3[0]), -3.0)
+    ref = fill.copy()
+    assert po.lib().o_enzo_prolong(1, 0, po.dptr(ref), po.ints(mf3), po.ints(of3),
+                                   po.ints(nf3), po.dptr(vc), po.ints(mc3), po.ints(oc3),
+                                   po.ints(nc3)) == -2
+    dc, df = _dev(vc), _dev(fill)
+    assert ev.enzo_prolong("2A", df.ptr.value, mf3, of3, nf3, dc.ptr.value, mc3, oc3,
+                           nc3) == 1
+    ev.synchronize()
+    np.testing.assert_array_equal(ev.from_device(df.ptr.value, fill.shape), ref)
