@@ -156,8 +156,16 @@ STEP_LAUNCHES = {"k_refresh": 1, "k_flux_nn": 1, "k_flux_plm": 1, "k_edge": 2,
 
 
 def measured_step_traffic(workload, mhd=True):
-    """HBM bytes of one step from the committed per-kernel counter summary,
-    or None when a kernel of the step has no entry."""
+    """HBM bytes of one step from the committed counter summary: its
+    step_traffic line, else the sum of the step's launches, or None when a
+    kernel of the step has no entry."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 3 and p[0] == "step_traffic" and p[1] == workload:
+                    return float(p[2])
     tot = 0.0
     for k, n in STEP_LAUNCHES.items():
         if not mhd and k in ("k_edge", "k_faces"):
@@ -338,6 +346,17 @@ def _roofline(prof, steps, cells_per_step, prec, mhd, workload):
             "bytes_per_step": nbytes, "kernel_ms_per_step": per_step_ms}
 
 
+def _both_rooflines(prof, steps, elapsed, zones_per_step, cells_per_step, precision, mhd,
+                    workload):
+    """The step against the HBM roofline (algorithmic bytes per zone-update
+    over the wall time per step), with the flux kernel's own figure nested."""
+    prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
+    gpu_ms = sum(v[0] * v[1] for v in prof.values()) / steps
+    return dict(step_roofline(precision, zones_per_step, elapsed * 1e3 / steps, gpu_ms,
+                              workload, mhd),
+                dominant_kernel=_roofline(prof, steps, cells_per_step, prec, mhd, workload))
+
+
 def _profiled(fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -403,7 +422,9 @@ def run_fast_wave(a):
 def run_batched_amr(a):
     """Config 3: 512 / 1536 / 3000 independent 32^3 blocks on three levels,
     2:1 sub-cycling (launches of 5048, 3000, 4536, 3000 blocks per coarse
-    step), one batched launch sequence per substep with per-block dt."""
+    step), one batched launch sequence per substep with per-block dt, the
+    whole coarse step (dt included) on the device and replayed from a HIP
+    graph (enzoe_vlct_levels)."""
     import workloads
     res = {}
     for precision in (a.precision, "mixed" if a.precision == "fp64" else "fp64"):
@@ -412,16 +433,31 @@ def run_batched_amr(a):
                                theta=2.0, courant=0.4, precision=prec)
         lv = workloads.SubcycledLevels(method)
         method.autotune(lv.levels[-1], method.timestep(lv.levels[-1]) / 4)
-        el, prof = _profiled(lv.coarse_step, a.steps, a.warmup)
+        # the levels prepared on the device: a coarse step is kernels only
+        # (per-level CFL, the dt kernel, the substeps); per-kernel times from
+        # eager steps, the rate from a captured graph of `steps` coarse steps
+        plan = lv.device_levels()
+        _, prof = _profiled(plan.step, a.steps, a.warmup)
+        plan.capture(a.steps)
+        plan.replay()  # first replay: the graph's upload
+        ev.synchronize()
+        t0 = time.perf_counter()
+        plan.replay()
+        ev.synchronize()
+        el = time.perf_counter() - t0
         zones = lv.zones_per_coarse_step()
         cells = sum(34 ** 3 * c * 2 ** l for l, c in enumerate(lv.counts))
         res[precision] = {"value": zones * a.steps / el,
                           "ms_per_step": el * 1e3 / a.steps,
                           "kernels_ms": {k: round(v[0], 4) for k, v in sorted(prof.items())},
-                          "roofline": _roofline(prof, a.steps, cells, prec, True,
-                                                "batched_amr_%s" % precision),
-                          "launch_blocks": lv.launch_sizes()}
-        del lv, method
+                          "roofline": _both_rooflines(prof, a.steps, el, zones, cells,
+                                                      precision, True,
+                                                      "batched_amr_%s" % precision),
+                          "launch_blocks": lv.launch_sizes(),
+                          "launch_choice": method.launch_choice((38, 38, 38)),
+                          "driver": "HIP graph of %d device-driven coarse steps" % a.steps}
+        plan.destroy()
+        del lv, method, plan
         if a.no_compare:
             break
     r = res[a.precision]
@@ -489,8 +525,9 @@ def run_uniform(a):
         res[precision] = {"value": n ** 3 * nblk * a.steps / el,
                           "ms_per_step": el * 1e3 / a.steps,
                           "kernels_ms": {k: round(v[0], 4) for k, v in sorted(prof.items())},
-                          "roofline": _roofline(prof, a.steps, cells, prec, mhd,
-                                                "%s_%s" % (a.workload, precision))}
+                          "roofline": _both_rooflines(prof, a.steps, el,
+                                                      n ** 3 * nblk, cells, precision, mhd,
+                                                      "%s_%s" % (a.workload, precision))}
     r = res[a.precision]
     out = {
         "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1,

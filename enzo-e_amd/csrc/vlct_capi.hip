@@ -799,10 +799,10 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   // orders of its two kernels (k_flux bit 0, k_ctu bit 3)
   std::vector<std::pair<int, int>> cand;  // (fused-CT variant, mask)
   const int fixed = (m->mhd && m->nstages == 2) ? m->ctu : 0;
-  for (int v = 0; v <= 3; v++) {
+  for (int v = 0; v <= 4; v++) {
     if (fixed >= 0 && v != fixed) continue;
     for (int mask = 0; mask < 16; mask++)
-      if (v == 0 || (mask & 6) == 0) cand.push_back({v, mask});
+      if (v == 0 || (mask & (v == 4 ? 4 : 6)) == 0) cand.push_back({v, mask});
   }
   for (size_t ci = 0; ci < cand.size() && rc == ENZOE_OK; ci++) {
     const int mask = cand[ci].second;
@@ -828,6 +828,37 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   }
   m->force_mask = -1;
   m->force_ctu = -1;
+  // then the rows of the tiled cell order, for the chosen structure and
+  // order (when that order tiles at all)
+  const int rows0 = tile_rows_of(m, mx, my, mz);
+  int best_rows = rows0;
+  if (best != 0 && rc == ENZOE_OK) {
+    m->force_mask = best;
+    m->force_ctu = best_ctu;
+    for (int rows : {0, 8, 16, 32, 64}) {
+      if (rows == rows0 || rows >= my) continue;
+      m->force_rows = rows;
+      rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);  // warm-up
+      if (rc) break;
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < 3 && rc == ENZOE_OK; r++)
+        rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);
+      (void)hipEventRecord(e1, s);
+      if (rc || hipEventSynchronize(e1) != hipSuccess) {
+        rc = rc ? rc : ENZOE_ERR_HIP;
+        break;
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_rows = rows;
+      }
+    }
+    m->force_mask = -1;
+    m->force_ctu = -1;
+    m->force_rows = -1;
+  }
   g_profiling = prof;
   (void)hipStreamSynchronize(s);
   if (e0) (void)hipEventDestroy(e0);
@@ -836,13 +867,16 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   if (rc) return rc;
   m->tuned_mask[shape_key(mx, my, mz)] = best;
   m->tuned_ctu[shape_key(mx, my, mz)] = best_ctu;
+  m->tuned_rows[shape_key(mx, my, mz)] = best_rows;
   if (mask_out) *mask_out = best;
   return ENZOE_OK;
 }
 
-int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask, int *fused_ct) {
+int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask, int *fused_ct,
+                             int *tile_rows) {
   if (!m) return ENZOE_ERR_ARG;
   if (mask) *mask = launch_mask(m, mx, my, mz);
+  if (tile_rows) *tile_rows = tile_rows_of(m, mx, my, mz);
   if (fused_ct) *fused_ct = (m->mhd && ctu_possible(m)) ? ctu_variant(m, mx, my, mz) : 0;
   return ENZOE_OK;
 }
@@ -924,7 +958,7 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
 }
 
 int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int variant) {
-  if (!m || variant < -1 || variant > 3) return ENZOE_ERR_ARG;
+  if (!m || variant < -1 || variant > 4) return ENZOE_ERR_ARG;
   m->ctu = variant;
   return ENZOE_OK;
 }

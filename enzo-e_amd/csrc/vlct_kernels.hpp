@@ -1039,7 +1039,8 @@ struct CtuTile {
 };
 
 // The tile shapes of k_ctu the launch chooses from (enzoe_vlct_set_fused_ct
-// / the autotuner): 1 = 16 x 4 x 4, 2 = 32 x 4 x 2, 3 = 8 x 8 x 4.
+// / the autotuner): 1 = 16 x 4 x 4, 2 = 32 x 4 x 2, 3 = 8 x 8 x 4 (edges in
+// LDS); 4 = 16 x 4 x 4 after k_edge (faces and cells fused).
 typedef CtuTile<16, 4, 4> CtuTile1;
 typedef CtuTile<32, 4, 2> CtuTile2;
 typedef CtuTile<8, 8, 4> CtuTile3;
@@ -1088,8 +1089,10 @@ __device__ __forceinline__ void ctu_edges(const BlockDesc<FluxT> &b, double *con
   }
 }
 
-// The faces B_AX of the tile into LDS (and the owned updated ones to memory).
-template <int AX, class Tile, typename FluxT>
+// The faces B_AX of the tile into LDS (and the owned updated ones to
+// memory), the edge EMFs from LDS (EDGES_LDS) or from the edge arrays k_edge
+// wrote.
+template <int AX, class Tile, bool EDGES_LDS, typename FluxT>
 __device__ __forceinline__ void ctu_faces(const BlockDesc<FluxT> &b, const StageParams &sp,
                                           double dt, const int o[3], const int m[3],
                                           const int hi[3], int cs, const double *sE,
@@ -1116,8 +1119,17 @@ __device__ __forceinline__ void ctu_faces(const BlockDesc<FluxT> &b, const Stage
         int qj[3] = {q[0], q[1], q[2]}, qk[3] = {q[0], q[1], q[2]};
         qj[aj] -= 1;
         qk[ak] -= 1;
-        val = face_b_update(b0, dtdj, dtdk, sE[Tile::eidx(ak, q, o)], sE[Tile::eidx(ak, qj, o)],
-                            sE[Tile::eidx(aj, q, o)], sE[Tile::eidx(aj, qk, o)]);
+        if constexpr (EDGES_LDS) {
+          val = face_b_update(b0, dtdj, dtdk, sE[Tile::eidx(ak, q, o)],
+                              sE[Tile::eidx(ak, qj, o)], sE[Tile::eidx(aj, q, o)],
+                              sE[Tile::eidx(aj, qk, o)]);
+        } else {
+          const long e = idx3(q[2], q[1], q[0], m[1], m[0]);
+          const long str_j = aj == 0 ? 1 : (aj == 1 ? (long)m[0] : (long)m[0] * m[1]);
+          const long str_k = ak == 0 ? 1 : (ak == 1 ? (long)m[0] : (long)m[0] * m[1]);
+          const double *Ej = b.E[aj], *Ek = b.E[ak];
+          val = face_b_update(b0, dtdj, dtdk, Ek[e], Ek[e - str_j], Ej[e], Ej[e - str_k]);
+        }
         if (p[AX] < o[AX] + Tile::t(AX) || p[AX] == hi[AX]) {
           (sp.is_final ? b.bfi : b.bft)[AX][cb] = val;
           if (!sp.is_final) b.bf0[AX][cb] = b0;
@@ -1132,7 +1144,9 @@ __device__ __forceinline__ void ctu_faces(const BlockDesc<FluxT> &b, const Stage
   }
 }
 
-template <typename FluxT, bool DE, class Tile>
+// EDGES_LDS: the edge EMFs are computed here (k_edge + k_faces + k_cells
+// fused); otherwise k_edge has written them and this is k_faces + k_cells.
+template <typename FluxT, bool DE, class Tile, bool EDGES_LDS = true>
 __global__ void __launch_bounds__(Tile::NT)
 k_ctu(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
@@ -1155,19 +1169,21 @@ k_ctu(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const long str[3] = {1, (long)mx, (long)mx * my};
   const double dt = sp.dts ? sp.dts[by] * sp.dt_factor : sp.dt;
 
-  __shared__ double sE[Tile::NE];
+  __shared__ double sE[EDGES_LDS ? Tile::NE : 1];
   __shared__ double sF[Tile::NF];
 
   // 1. edges
-  double *const *Ecs = center_e(b, sp);
-  ctu_edges<0, Tile>(b, Ecs, o, m, cs, str, sE);
-  ctu_edges<1, Tile>(b, Ecs, o, m, cs, str, sE);
-  ctu_edges<2, Tile>(b, Ecs, o, m, cs, str, sE);
-  __syncthreads();
+  if constexpr (EDGES_LDS) {
+    double *const *Ecs = center_e(b, sp);
+    ctu_edges<0, Tile>(b, Ecs, o, m, cs, str, sE);
+    ctu_edges<1, Tile>(b, Ecs, o, m, cs, str, sE);
+    ctu_edges<2, Tile>(b, Ecs, o, m, cs, str, sE);
+    __syncthreads();
+  }
   // 2. faces
-  ctu_faces<0, Tile>(b, sp, dt, o, m, hi, cs, sE, sF);
-  ctu_faces<1, Tile>(b, sp, dt, o, m, hi, cs, sE, sF);
-  ctu_faces<2, Tile>(b, sp, dt, o, m, hi, cs, sE, sF);
+  ctu_faces<0, Tile, EDGES_LDS>(b, sp, dt, o, m, hi, cs, sE, sF);
+  ctu_faces<1, Tile, EDGES_LDS>(b, sp, dt, o, m, hi, cs, sE, sF);
+  ctu_faces<2, Tile, EDGES_LDS>(b, sp, dt, o, m, hi, cs, sE, sF);
   __syncthreads();
   // 3. cells
   const int tid = threadIdx.x;

@@ -86,6 +86,10 @@ struct enzoe_vlct_method {
   int ctu = -1;
   int force_ctu = -1;                    // set while the autotuner times one
   std::map<long long, int> tuned_ctu;    // the autotuner's choice per shape
+  // rows of the tiled cell order per block shape (the autotuner's choice;
+  // tile_rows_for otherwise) and the value forced while it times one
+  std::map<long long, int> tuned_rows;
+  int force_rows = -1;
   // dual-energy formalism ("modern" formulation) and its eta
   int dual = 0;
   double eta = 0.001;
@@ -140,7 +144,8 @@ inline long long shape_key(int mx, int my, int mz) {
 }
 
 // The launch structure of the MHD stages for blocks of this shape: 0 =
-// k_edge + k_faces + k_cells, 1..3 = k_ctu with tile CtuTile1..3; the
+// k_edge + k_faces + k_cells, 1..3 = k_ctu with tile CtuTile1..3, 4 =
+// k_edge + k_ctu without its edge phase (faces and cells fused); the
 // method's setting, or the autotuner's choice for the shape when the
 // setting is "auto" (-1).
 inline int ctu_variant(const enzoe_vlct_method *m, int mx, int my, int mz) {
@@ -156,14 +161,14 @@ inline int ctu_variant(const enzoe_vlct_method *m, int mx, int my, int mz) {
 inline bool ctu_possible(const enzoe_vlct_method *m) {
   return m->mhd && m->nstages == 2 && (m->ctu != 0 || m->force_ctu > 0);
 }
-template <typename FluxT, bool DE, class Tile>
+template <typename FluxT, bool DE, class Tile, bool EDGES_LDS = true>
 void launch_ctu(const BlockDesc<FluxT> *d, int nb, int mx, int my, int mz, int lo,
                 StageParams sp, hipStream_t s) {
   int nt[3];
   ctu_tiles<Tile>(mx, my, mz, lo, nt);
   const long tiles = (long)nt[0] * nt[1] * nt[2];
-  hipLaunchKernelGGL((k_ctu<FluxT, DE, Tile>), dim3(nblk8(tiles * Tile::NT, Tile::NT), nb),
-                     dim3(Tile::NT), 0, s, d, sp);
+  hipLaunchKernelGGL((k_ctu<FluxT, DE, Tile, EDGES_LDS>),
+                     dim3(nblk8(tiles * Tile::NT, Tile::NT), nb), dim3(Tile::NT), 0, s, d, sp);
 }
 
 inline int total_staling(int r) { return r == RC_NN ? 1 : 2; }
@@ -268,6 +273,13 @@ inline int tile_rows_for(int mx, int my) {
 }
 
 
+// Rows of the tiled cell order for blocks of this shape.
+inline int tile_rows_of(const enzoe_vlct_method *m, int mx, int my, int mz) {
+  if (m->force_rows >= 0) return m->force_rows;
+  auto it = m->tuned_rows.find(shape_key(mx, my, mz));
+  return it != m->tuned_rows.end() ? it->second : tile_rows_for(mx, my);
+}
+
 // Fills descriptors for blocks [b0, b1) into out[0 .. b1-b0), their scratch
 // laid out from `base`.
 // accel: 3 acceleration pointers per block of the call (x, y, z), or null
@@ -290,7 +302,7 @@ void fill_desc_at(const enzoe_vlct_method *m, char *base, const enzoe_block *blo
     d.pressure = b.pressure;
     d.mx = b.mx; d.my = b.my; d.mz = b.mz;
     d.g = b.ghost;
-    d.tile_rows = tile_rows_for(b.mx, b.my);
+    d.tile_rows = tile_rows_of(m, b.mx, b.my, b.mz);
     if (accel)
       for (int a = 0; a < 3; a++) d.acc[a] = accel[3 * bi + a];
     for (int a = 0; a < 3; a++) d.cw[a] = b.cell_width[a];
@@ -485,7 +497,15 @@ int launch_chunks(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int ch
       // with the fused CFL the final update's launch covers the block
       const int lo = sp.cfl ? 0 : (is_final ? cs : cs + 1);
       const int variant = (MHD && ctu_possible(m)) ? ctu_variant(m, mx, my, mz) : 0;
-      if (variant > 0) {
+      if (variant == 4) {
+        {
+          KernelTimer kt(s, "k_edge", timers);
+          hipLaunchKernelGGL((k_edge<FluxT>), dim3(nblk8(cube(mx, my, mz, cs, cs), 256), nb),
+                             dim3(256), 0, s, d, sp);
+        }
+        KernelTimer kt(s, "k_fc", timers);
+        launch_ctu<FluxT, DE, CtuTile1, false>(d, nb, mx, my, mz, lo, sp, s);
+      } else if (variant > 0) {
         KernelTimer kt(s, "k_ctu", timers);
         if (variant == 2)
           launch_ctu<FluxT, DE, CtuTile2>(d, nb, mx, my, mz, lo, sp, s);

@@ -170,6 +170,7 @@ def lib():
         L.enzoe_vlct_set_fused_ct.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_launch_choice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_int),
                                                ctypes.POINTER(ctypes.c_int)]
         L.enzoe_vlct_compute_noalloc.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                                  ctypes.c_double, vp]
@@ -380,17 +381,20 @@ class VlctMethod:
     def set_fused_ct(self, variant):
         """Launch structure of the MHD stages: 0 (or False) three launches
         through memory, 1..3 (True = 1) one fused CT + update launch with
-        tile 16x4x4 / 32x4x2 / 8x8x4, -1 the autotuner's choice."""
+        tile 16x4x4 / 32x4x2 / 8x8x4, 4 the edge EMFs then one face + cell
+        launch, -1 the autotuner's choice."""
         v = int(variant) if not isinstance(variant, bool) else (1 if variant else 0)
         check(lib().enzoe_vlct_set_fused_ct(self.handle, v), "enzoe_vlct_set_fused_ct")
 
     def launch_choice(self, shape):
-        """(XCD mask, fused-CT variant) used for blocks of shape (mx, my, mz)."""
-        mask, ctu = ctypes.c_int(), ctypes.c_int()
+        """(XCD mask, fused-CT variant, tile rows) used for blocks of shape
+        (mx, my, mz)."""
+        mask, ctu, rows = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().enzoe_vlct_launch_choice(self.handle, shape[0], shape[1], shape[2],
-                                             ctypes.byref(mask), ctypes.byref(ctu)),
+                                             ctypes.byref(mask), ctypes.byref(ctu),
+                                             ctypes.byref(rows)),
               "enzoe_vlct_launch_choice")
-        return mask.value, ctu.value
+        return mask.value, ctu.value, rows.value
 
     def state_l1(self, blocks, ref_blocks, stream=None):
         """L1 of the blocks' cell-centred fields against ref_blocks' (same

@@ -166,14 +166,16 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * order over the four stage kernels (k_flux, k_edge, k_faces, k_cells)
  * and, for MHD with the fused-CT setting "auto" (enzoe_vlct_set_fused_ct),
  * the fused CT + update launch with each of its tile shapes and the orders
- * of its two kernels.  *mask_out (optional) gets the chosen mask: bit 0
- * k_flux, 1 k_edge, 2 k_faces, 3 k_cells (or the fused launch).
- * enzoe_vlct_launch_choice: the mask and the fused-CT variant (0 = three
- * launches, 1..3 = tile) calls on blocks of this shape use.               */
+ * of its two kernels; then, for the chosen one, the rows of the tiled cell
+ * order (slabs of 8, 16, 32 or 64 rows, or the plain order).  *mask_out
+ * (optional) gets the chosen mask: bit 0 k_flux, 1 k_edge, 2 k_faces,
+ * 3 k_cells (or the fused launch).  enzoe_vlct_launch_choice: the mask, the
+ * fused-CT variant (0 = three launches, 1..4) and the tile rows calls on
+ * blocks of this shape use.                                                 */
 int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                         double dt, enzoe_stream_t stream, int *mask_out);
 int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask,
-                             int *fused_ct);
+                             int *fused_ct, int *tile_rows);
 
 /* EnzoMethodMHDVlct::timestep + the clip of control_stopping.cpp:83-86
  * without a host round trip: reads the time from *d_time (device), writes
@@ -690,7 +692,9 @@ int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
  * launches (edge EMFs, face update, cell update) through memory; 1, 2, 3:
  * the constrained transport and the cell update of a stage as one fused
  * launch over tiles of 16x4x4, 32x4x2 or 8x8x4 cells, the edge EMFs and
- * face-centred B of a tile held in LDS; -1 (the default): the autotuner's
+ * face-centred B of a tile held in LDS; 4: the edge EMFs through memory,
+ * then the face update and the cell update as one launch over 16x4x4
+ * tiles (face B in LDS); -1 (the default): the autotuner's
  * choice per block shape (three launches until it has run).  Every variant
  * gives the same bits.  Not to be changed while work of the method is
  * queued.                                                                   */

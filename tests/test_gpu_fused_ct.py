@@ -3,8 +3,9 @@ three-launch path (k_edge, k_faces, k_cells) on the GPU: every field, the
 pressure field and the next dt equal bit for bit, for FP64 (plain and
 dual energy), the mixed path (FP32 fluxes, FP32 primitive half-step state),
 HLLE, two blocks in one batched launch with per-block dt, every tile shape
-(16x4x4, 32x4x2, 8x8x4), and blocks whose extents are not multiples of the
-tiles (partial tiles on every axis); the three-launch path (the default
+(16x4x4, 32x4x2, 8x8x4) and the edge-EMFs-then-faces-and-cells variant, and
+blocks whose extents are not multiples of the tiles (partial tiles on every
+axis); the three-launch path (the default
 until the autotuner has chosen) is the one the oracle parity tests run; the
 autotuner's choice keeps the results."""
 import numpy as np
@@ -28,7 +29,7 @@ def _equal(blocks_a, blocks_b):
             np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("riemann", ["hlld", "hlle"])
 def test_fused_ct_equals_three_launches(precision, riemann, variant):
@@ -69,7 +70,7 @@ def test_fused_ct_two_blocks_per_block_dt(dual):
             arr["internal_energy"][...] = (arr["total_energy"] - 0.5 * v2 -
                                            0.5 * b2 / arr["density"])
     out = []
-    for v in (2, 0):
+    for v in (4, 0):
         m = method_for(grids[0].cfg)
         m.set_fused_ct(v)
         dgs = [device_copy(g, m) for g in grids]
@@ -93,8 +94,10 @@ def test_autotuned_launch_structure_keeps_results():
     dg = device_copy(grid, m)
     mask = m.autotune(dg.blocks, 1e-3)
     b = dg.blocks[0]
-    assert m.launch_choice((b.mx, b.my, b.mz))[0] == mask
-    assert m.launch_choice((b.mx, b.my, b.mz))[1] in (0, 1, 2, 3)
+    choice = m.launch_choice((b.mx, b.my, b.mz))
+    assert choice[0] == mask
+    assert choice[1] in (0, 1, 2, 3, 4)
+    assert choice[2] >= 0
     t = 0.0
     for _ in range(2):
         dt_o = grid.cycle(t, 10.0)
