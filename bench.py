@@ -37,13 +37,15 @@ STEP_BYTES_PER_ZONE_FP64 = 440.0
 # FP64 face fields (56 B instead of 88): 88 + 56 + 56 + 88 + 88 = 376 B.
 STEP_BYTES_PER_ZONE_MIXED = 376.0
 STEP_BYTES = {("mhd", "fp64"): STEP_BYTES_PER_ZONE_FP64, ("mhd", "mixed"): STEP_BYTES_PER_ZONE_MIXED,
+              ("mhd", "fp64_fma"): STEP_BYTES_PER_ZONE_FP64,
               # hydro: 5 words per state, FP64 25 words; mixed half-step 5 x 4 B
-              ("hd", "fp64"): 200.0, ("hd", "mixed"): 40.0 + 20.0 + 20.0 + 40.0 + 40.0}
+              ("hd", "fp64"): 200.0, ("hd", "mixed"): 40.0 + 20.0 + 20.0 + 40.0 + 40.0,
+              ("hd", "fp64_fma"): 200.0}
 # Algorithmic bytes per cell of one flux launch: it must read the 11-word
 # state of the cell and write 7 flux components on each of its 3 low faces.
-FLUX_BYTES_PER_CELL = {0: (11 + 21) * 8.0, 1: 11 * 8.0 + 21 * 4.0}
+FLUX_BYTES_PER_CELL = {0: (11 + 21) * 8.0, 1: 11 * 8.0 + 21 * 4.0, 2: (11 + 21) * 8.0}
 # hydro: read 5 words, write 5 fluxes on 3 faces; step: 25 words per zone
-FLUX_BYTES_PER_CELL_HD = {0: (5 + 15) * 8.0, 1: 5 * 8.0 + 15 * 4.0}
+FLUX_BYTES_PER_CELL_HD = {0: (5 + 15) * 8.0, 1: 5 * 8.0 + 15 * 4.0, 2: (5 + 15) * 8.0}
 STEP_BYTES_PER_ZONE_HD = 200.0
 WORKLOADS = ["orszag_tang", "fast_wave", "batched_amr", "turbulence", "smr_flux_correct",
              "kelvin_helmholtz", "output"]
@@ -55,7 +57,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="active cells per axis")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    # fp64_fma: FP64 with multiply-adds contracted (not bit-identical to the
+    # reference; held to 1e-12 relative per cell); Orszag-Tang only
+    ap.add_argument("--precision", choices=["fp64", "mixed", "fp64_fma"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the run at the other precision")
@@ -280,11 +284,15 @@ def cpu_baseline(n, nsteps):
 METRIC = "zone-updates/s/GPU (batched FP32-mixed vs plain FP64 VL+CT) + HBM GB/s"
 
 
-def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1):
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp64_fma": 2}  # ENZOE_PREC_*
+
+
+def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1, keep=False):
     """Warm-up, then `steps` timed steps with per-kernel HIP-event timing.
     dual_energy: the "modern" dual-energy formalism (eta 0.001) on.
-    launch_mask >= 0: that launch order instead of the autotuner's."""
-    prec = ev.ENZOE_PREC_DOUBLE if precision == "fp64" else ev.ENZOE_PREC_MIXED
+    launch_mask >= 0: that launch order instead of the autotuner's.
+    keep: the result holds the grid ("grid", "method") after the run."""
+    prec = PRECISIONS[precision]
     if launch_mask >= 0:
         os.environ["ENZOE_XCD_REMAP"] = str(launch_mask)
     method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
@@ -311,6 +319,7 @@ def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1):
     el = time.perf_counter() - t0
     prof = ev.profile_report()
     ev.profiling(False)
+    kept = {"grid": grid, "method": method} if keep else {}
     del grid, method
     zones = n ** 3
     m = n + 6
@@ -328,7 +337,7 @@ def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1):
             "step_gbs": STEP_BYTES_PER_ZONE_FP64 * zones / (gpu_step_ms * 1e-3) / 1e9,
             "workload": "orszag_tang_%d3_%s%s_vlct_plm_hlld" % (
                 n, precision, "_dual_energy" if dual_energy else ""),
-            "launch_mask": mask}
+            "launch_mask": mask, **kept}
 
 
 def _roofline(prof, steps, cells_per_step, prec, mhd, workload):
@@ -787,15 +796,25 @@ def main():
         out["copy_kernel_GBs"] = copy_bandwidth()["GBs"]
         print(json.dumps(out))
         return
-    if a.dual_energy and a.precision != "fp64":
+    if a.dual_energy and a.precision == "mixed":
         print("the dual-energy formalism runs in FP64 only", file=sys.stderr)
         sys.exit(2)
+    compare = not a.no_compare and a.precision != "fp64_fma"
     r = timed_run(a.precision, a.n, a.steps, a.warmup, dual_energy=a.dual_energy,
-                  launch_mask=a.launch_mask)
-    other = dual = None
-    if not a.no_compare:
+                  launch_mask=a.launch_mask, keep=compare and a.precision == "fp64")
+    other = dual = fma = None
+    if compare:
         other = timed_run("mixed" if a.precision == "fp64" else "fp64", a.n,
-                          a.steps, a.warmup)
+                          a.steps, a.warmup, keep=a.precision == "mixed")
+        fp64 = r if a.precision == "fp64" else other
+        # the contracted FP64 build on the same steps, its state against the
+        # reference-parity FP64 one cell by cell
+        fma = timed_run("fp64_fma", a.n, a.steps, a.warmup, keep=True)
+        per, mag, mx = fma["method"].state_l1(fma["grid"].blocks, fp64["grid"].blocks,
+                                              max_rel=True)
+        fma["vs_fp64"] = {"max_rel_per_cell": mx, "l1_by_vector_magnitude": mag}
+        for x in (fma, fp64):
+            del x["grid"], x["method"]
         dual = timed_run("fp64", a.n, a.steps, a.warmup, dual_energy=True)
     copy = copy_bandwidth()
     out = {
@@ -809,7 +828,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32-riemann/f64-update",
+        "dtype": {"fp64": "f64", "fp64_fma": "f64-fma"}.get(a.precision,
+                                                            "f32-riemann/f64-update"),
         "data": "synthetic (3D Orszag-Tang vortex, seeded 1e-3 v_z noise)",
         "config": {"workload": r["workload"],
                    "block": [a.n, a.n, a.n], "ghost": 3, "riemann": "hlld",
@@ -859,6 +879,18 @@ def main():
             "over_fp64": dual["value"] / fp64["value"],
             "ms_per_step": dual["ms_per_step"],
             "kernels_ms": {k: round(v[0], 4) for k, v in sorted(dual["prof"].items())},
+        }
+    if fma is not None:
+        fp64 = r if a.precision == "fp64" else other
+        out["contracted_fp64_comparison"] = {
+            "workload": fma["workload"], "value": fma["value"],
+            "over_fp64": fma["value"] / fp64["value"],
+            "ms_per_step": fma["ms_per_step"],
+            "kernels_ms": {k: round(v[0], 4) for k, v in sorted(fma["prof"].items())},
+            "bar": "1e-12 relative per cell (velocity and B components over the "
+                   "cell's vector magnitude)",
+            "vs_fp64_after_steps": a.warmup + a.steps,
+            **fma["vs_fp64"],
         }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)

@@ -277,9 +277,16 @@ k_diagnostics(const BlockDesc<double> *__restrict__ blocks, double gamma, DiagAc
 // Per-field L1 sums of a state against another (enzoe_vlct_state_l1):
 // DIAG_WG workgroups per block pair (grid.y), grid-stride over the active
 // cells, compensated per thread, then per wave and workgroup as the
-// diagnostics sums; part[((pair * DIAG_WG + wg) * L1_NSUM + q) * 2 + {0,1}]
-// with q = diff 0..7, ref 8..15, |v| 16, |B| 17.
+// diagnostics sums; part[(pair * DIAG_WG + wg) * L1_NPART + q * 2 + {0,1}]
+// with q = diff 0..7, ref 8..15, |v| 16, |B| 17; then at
+// L1_NSUM * 2 + q the largest per-cell relative difference of field q
+// (|a - b| over |b|, for a velocity or B component over the cell's |b_vec|;
+// a NaN difference is kept).
 constexpr int L1_NSUM = 18;
+constexpr int L1_NPART = L1_NSUM * 2 + 8;
+__device__ __forceinline__ double nan_max(double a, double b) {
+  return (b > a || b != b) ? b : a;
+}
 template <bool MHD>
 __global__ void __launch_bounds__(256)
 k_state_l1(const BlockDesc<double> *__restrict__ da, const BlockDesc<double> *__restrict__ db,
@@ -290,27 +297,43 @@ k_state_l1(const BlockDesc<double> *__restrict__ da, const BlockDesc<double> *__
   const int nx = b.mx - 2 * g, ny = b.my - 2 * g, nz = b.mz - 2 * g;
   const long nact = (long)nx * ny * nz;
   constexpr int NQ = MHD ? S_N : 5;
-  double s[L1_NSUM], cs[L1_NSUM];
+  double s[L1_NSUM], cs[L1_NSUM], mr[8];
 #pragma unroll
   for (int q = 0; q < L1_NSUM; q++) s[q] = cs[q] = 0.;
+#pragma unroll
+  for (int q = 0; q < 8; q++) mr[q] = 0.;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nact;
        t += (long)gridDim.x * blockDim.x) {
     const int i = (int)(t % nx) + g, j = (int)((t / nx) % ny) + g,
               k = (int)(t / ((long)nx * ny)) + g;
     const long c = idx3(k, j, i, b.my, b.mx);
-    double vb[S_N];
+    double vb[S_N], df[S_N];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       vb[q] = b.U[q][c];
-      comp_add(s[q], cs[q], fabs(a.U[q][c] - vb[q]));
+      df[q] = fabs(a.U[q][c] - vb[q]);
+      comp_add(s[q], cs[q], df[q]);
       comp_add(s[8 + q], cs[8 + q], fabs(vb[q]));
     }
-    comp_add(s[16], cs[16], sqrt(vb[S_VX] * vb[S_VX] + vb[S_VY] * vb[S_VY] +
-                                 vb[S_VZ] * vb[S_VZ]));
-    if (MHD)
-      comp_add(s[17], cs[17], sqrt(vb[S_BX] * vb[S_BX] + vb[S_BY] * vb[S_BY] +
-                                   vb[S_BZ] * vb[S_BZ]));
+    const double vmag = sqrt(vb[S_VX] * vb[S_VX] + vb[S_VY] * vb[S_VY] + vb[S_VZ] * vb[S_VZ]);
+    comp_add(s[16], cs[16], vmag);
+    double bmag = 0.;
+    if (MHD) {
+      bmag = sqrt(vb[S_BX] * vb[S_BX] + vb[S_BY] * vb[S_BY] + vb[S_BZ] * vb[S_BZ]);
+      comp_add(s[17], cs[17], bmag);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const double den = (q >= S_VX && q <= S_VZ) ? vmag
+                         : (q >= S_BX)            ? bmag
+                                                  : fabs(vb[q]);
+      mr[q] = nan_max(mr[q], df[q] == 0. ? 0. : df[q] / den);
+    }
   }
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mr[q] = nan_max(mr[q], __shfl_xor(mr[q], off, 64));
 #pragma unroll
   for (int q = 0; q < L1_NSUM; q++) {
 #pragma unroll
@@ -327,21 +350,29 @@ k_state_l1(const BlockDesc<double> *__restrict__ da, const BlockDesc<double> *__
     }
   }
   __shared__ double wsum[4][L1_NSUM][2];
+  __shared__ double wmax[4][8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0)
+  if (lane == 0) {
     for (int q = 0; q < L1_NSUM; q++) {
       wsum[wave][q][0] = s[q];
       wsum[wave][q][1] = cs[q];
     }
+    for (int q = 0; q < 8; q++) wmax[wave][q] = mr[q];
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double *out = part + ((long)blockIdx.y * gridDim.x + blockIdx.x) * L1_NSUM * 2;
+    double *out = part + ((long)blockIdx.y * gridDim.x + blockIdx.x) * L1_NPART;
     for (int q = 0; q < L1_NSUM; q++) {
       double ts = wsum[0][q][0], tc = wsum[0][q][1];
       for (int w = 1; w < (int)(blockDim.x >> 6); w++)
         comp_merge(ts, tc, wsum[w][q][0], wsum[w][q][1]);
       out[2 * q] = ts;
       out[2 * q + 1] = tc;
+    }
+    for (int q = 0; q < 8; q++) {
+      double mx = wmax[0][q];
+      for (int w = 1; w < (int)(blockDim.x >> 6); w++) mx = nan_max(mx, wmax[w][q]);
+      out[L1_NSUM * 2 + q] = mx;
     }
   }
 }
@@ -412,7 +443,8 @@ int enzoe_vlct_create(const enzoe_vlct_params *p, enzoe_vlct_t *out) {
   }
   m->dfloor = p->density_floor;
   m->pfloor = p->pressure_floor;
-  if (p->precision != ENZOE_PREC_DOUBLE && p->precision != ENZOE_PREC_MIXED) {
+  if (p->precision != ENZOE_PREC_DOUBLE && p->precision != ENZOE_PREC_MIXED &&
+      p->precision != ENZOE_PREC_DOUBLE_FMA) {
     delete m;
     return ENZOE_ERR_CONFIG;
   }
@@ -425,7 +457,7 @@ int enzoe_vlct_create(const enzoe_vlct_params *p, enzoe_vlct_t *out) {
       m->dual = 1;
       m->eta = (p->dual_energy_eta < 0) ? 0.001 : p->dual_energy_eta;
       // the FP32 flux path carries no internal-energy terms
-      if (m->precision != ENZOE_PREC_DOUBLE) { delete m; return ENZOE_ERR_CONFIG; }
+      if (m->precision == ENZOE_PREC_MIXED) { delete m; return ENZOE_ERR_CONFIG; }
     } else if (de != "disabled") {
       delete m;
       return ENZOE_ERR_CONFIG;
@@ -495,6 +527,8 @@ int compute_runs(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                                                 cfl)
                   : compute_group<double, false>(m, blocks + b0, b1 - b0, dt, dts, s, rs,
                                                  ra, cfl);
+    } else if (m->precision == ENZOE_PREC_DOUBLE_FMA) {
+      rc = compute_contracted(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra, cfl);
     } else {
       rc = compute_mixed(m, blocks + b0, b1 - b0, dt, dts, s, rs, ra, cfl);
     }
@@ -1118,7 +1152,7 @@ int enzoe_vlct_state_l1(enzoe_vlct_t m, const enzoe_block *a, const enzoe_block 
   hipStream_t s = (hipStream_t)stream;
   void *d_desc = nullptr, *d_part = nullptr;
   const size_t dbytes = desc.size() * sizeof(BlockDesc<double>);
-  const size_t nparts = (size_t)n * DIAG_WG * L1_NSUM * 2;
+  const size_t nparts = (size_t)n * DIAG_WG * L1_NPART;
   if (hipMalloc(&d_desc, dbytes) != hipSuccess) return ENZOE_ERR_ALLOC;
   if (hipMalloc(&d_part, nparts * sizeof(double)) != hipSuccess) {
     (void)hipFree(d_desc);
@@ -1147,15 +1181,21 @@ int enzoe_vlct_state_l1(enzoe_vlct_t m, const enzoe_block *a, const enzoe_block 
   (void)hipFree(d_desc);
   (void)hipFree(d_part);
   if (rc) return rc;
-  double tot[L1_NSUM], comp[L1_NSUM];
+  double tot[L1_NSUM], comp[L1_NSUM], mx[8];
   for (int q = 0; q < L1_NSUM; q++) tot[q] = comp[q] = 0.;
-  for (size_t w = 0; w < (size_t)n * DIAG_WG; w++)
-    for (int q = 0; q < L1_NSUM; q++)
-      comp_merge(tot[q], comp[q], parts[(w * L1_NSUM + q) * 2],
-                 parts[(w * L1_NSUM + q) * 2 + 1]);
+  for (int q = 0; q < 8; q++) mx[q] = 0.;
+  for (size_t w = 0; w < (size_t)n * DIAG_WG; w++) {
+    const double *pw = parts.data() + w * L1_NPART;
+    for (int q = 0; q < L1_NSUM; q++) comp_merge(tot[q], comp[q], pw[q * 2], pw[q * 2 + 1]);
+    for (int q = 0; q < 8; q++) {
+      const double v = pw[L1_NSUM * 2 + q];
+      if (v > mx[q] || v != v) mx[q] = v;
+    }
+  }
   for (int q = 0; q < 8; q++) {
     out->diff[q] = tot[q] + comp[q];
     out->ref[q] = tot[8 + q] + comp[8 + q];
+    out->max_rel[q] = mx[q];
   }
   out->velocity_magnitude = tot[16] + comp[16];
   out->bfield_magnitude = tot[17] + comp[17];
@@ -1330,6 +1370,8 @@ int levels_launch(enzoe_vlct_levels_s *L, hipStream_t s) {
                        : launch_chunks<double, false, false>(m, bl, r.n, r.chunk,
                                                              (const BlockDesc<double> *)table,
                                                              0.0, dts, s, nullptr, nullptr, true);
+      } else if (m->precision == ENZOE_PREC_DOUBLE_FMA) {
+        rc = launch_prepared_contracted(m, bl, r.n, r.chunk, table, dts, s);
       } else {
         rc = launch_prepared_mixed(m, bl, r.n, r.chunk, table, dts, s);
       }
@@ -1489,7 +1531,7 @@ int enzoe_vlct_levels_create(enzoe_vlct_t m, const enzoe_block *blocks, const in
   }
   // one scratch pool for every run (they run one after the other), each
   // run in chunks that fit the method's budget
-  const bool mixed = m->precision != ENZOE_PREC_DOUBLE;
+  const bool mixed = m->precision == ENZOE_PREC_MIXED;
   const size_t budget = scratch_budget(m);
   size_t pool = 0;
   for (LevelSub &sb : L->sub)

@@ -24,7 +24,7 @@
 
 #include "vlct_physics.hpp"
 
-namespace enzoe {
+namespace ENZOE_KNS {
 
 // Device descriptor of one block: permanent fields + the scratch the stage
 // kernels use for it.  FluxT is double (FP64 path) or float (mixed path).
@@ -1493,4 +1493,4 @@ k_store_fluxes(const BlockDesc<FluxT> *__restrict__ blocks,
   gst(fd[by], t, v);
 }
 
-}  // namespace enzoe
+}  // namespace ENZOE_KNS

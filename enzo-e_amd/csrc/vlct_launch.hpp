@@ -70,8 +70,8 @@ inline unsigned nblk8(long n, int bs) { return (nblk(n, bs) + 7u) & ~7u; }
 
 struct enzoe_vlct_method {
   int mhd = 1;
-  int riemann = enzoe::RS_HLLD;
-  int recon[2] = {enzoe::RC_NN, enzoe::RC_PLM};
+  int riemann = ENZOE_KNS::RS_HLLD;
+  int recon[2] = {ENZOE_KNS::RC_NN, ENZOE_KNS::RC_PLM};
   int nstages = 2;
   int precision = ENZOE_PREC_DOUBLE;
   // kernels launched in XCD-contiguous workgroup order (StageParams mask);
@@ -136,7 +136,9 @@ struct enzoe_vlct_method {
   std::vector<char> last_trtab[2];
 };
 
-namespace enzoe {
+namespace ENZOE_KNS {
+// the profiling helpers above stay in `enzoe` (one set for the library)
+using namespace ::enzoe;
 
 
 inline long long shape_key(int mx, int my, int mz) {
@@ -631,5 +633,19 @@ int compute_mixed(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
                   const StoreArgs *st = nullptr,
                   const double *const *accel = nullptr,
                   unsigned long long *cfl = nullptr);
+
+}  // namespace ENZOE_KNS
+
+namespace enzoe {
+
+// FP64 step with multiply-adds contracted into FMAs, compiled in
+// vlct_fp64c.hip: compute_runs' and the prepared launch sequence's
+// counterparts (st: the StoreArgs of the flux store, or null)
+int compute_contracted(enzoe_vlct_method *m, const enzoe_block *blocks, int n, double dt,
+                       const double *dts, hipStream_t s, const void *st,
+                       const double *const *accel, unsigned long long *cfl);
+int launch_prepared_contracted(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
+                               int chunk, const void *table, const double *dts,
+                               hipStream_t s);
 
 }  // namespace enzoe

@@ -9,7 +9,15 @@
 
 #include <hip/hip_runtime.h>
 
-namespace enzoe {
+// The namespace of the device code and of its launch templates: `enzoe` in
+// the default build; a unit that compiles the same templates under other
+// floating-point flags (the contracted FP64 step) names another one before
+// including, so that the two sets of instantiations never share a symbol.
+#ifndef ENZOE_KNS
+#define ENZOE_KNS enzoe
+#endif
+
+namespace ENZOE_KNS {
 
 // field slots used for cell-centred state, primitives and fluxes
 enum : int {
@@ -529,4 +537,4 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T gamma,
   flux[H_E] = sl * efl + sr * efr + (sm * cp * cw);
 }
 
-}  // namespace enzoe
+}  // namespace ENZOE_KNS

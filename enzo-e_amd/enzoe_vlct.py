@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libenzoe_vlct.so")
 
 ENZOE_PREC_DOUBLE = 0
 ENZOE_PREC_MIXED = 1
+ENZOE_PREC_DOUBLE_FMA = 2  # FP64, multiply-adds contracted (not bitwise)
 BC = {"periodic": 0, "outflow": 1}
 
 CC_FIELDS = ["density", "velocity_x", "velocity_y", "velocity_z",
@@ -71,7 +72,8 @@ class Diagnostics(ctypes.Structure):
 class StateL1(ctypes.Structure):
     _fields_ = [("diff", ctypes.c_double * 8), ("ref", ctypes.c_double * 8),
                 ("velocity_magnitude", ctypes.c_double),
-                ("bfield_magnitude", ctypes.c_double)]
+                ("bfield_magnitude", ctypes.c_double),
+                ("max_rel", ctypes.c_double * 8)]
 
 
 DEBUG_COUNTS = ("nonfinite", "density_floor", "energy_floor", "reconstruction_floor")
@@ -396,12 +398,14 @@ class VlctMethod:
               "enzoe_vlct_launch_choice")
         return mask.value, ctu.value, rows.value
 
-    def state_l1(self, blocks, ref_blocks, stream=None):
+    def state_l1(self, blocks, ref_blocks, stream=None, max_rel=False):
         """L1 of the blocks' cell-centred fields against ref_blocks' (same
         shapes), over the active cells, on the device.  Returns (per_field,
         by_magnitude): per field sum|a - b| / sum|b|, and the same with the
         denominator of a velocity or B component the sum of its vector's
-        magnitude |b_vec|."""
+        magnitude |b_vec|.  max_rel=True: also, third, per field the largest
+        per-cell relative difference |a - b| / |b| (a velocity or B
+        component over the cell's |b_vec|)."""
         out = StateL1()
         check(lib().enzoe_vlct_state_l1(self.handle, self._blocks(blocks),
                                         self._blocks(ref_blocks), len(blocks), stream,
@@ -415,6 +419,8 @@ class VlctMethod:
                    "bfield": out.bfield_magnitude}.get(nm.rsplit("_", 1)[0])
             den = vec if vec else r
             mag[nm] = d / den if den > 0 else d
+        if max_rel:
+            return per, mag, {nm: out.max_rel[q] for q, nm in enumerate(names)}
         return per, mag
 
     def autotune(self, blocks, dt, stream=None):

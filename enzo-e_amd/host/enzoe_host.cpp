@@ -1347,8 +1347,13 @@ EnzoMethodMHDVlct::EnzoMethodMHDVlct(ParameterGroup p, bool store_fluxes_for_cor
     ENZOE_ASSERT("EnzoMethodMHDVlct",
                  "mixed precision is only provided by the fused device path", fused_);
     precision_ = ENZOE_PREC_MIXED;
+  } else if (prec == "double_fma") {
+    ENZOE_ASSERT("EnzoMethodMHDVlct",
+                 "contracted FP64 is only provided by the fused device path", fused_);
+    precision_ = ENZOE_PREC_DOUBLE_FMA;
   } else {
-    ENZOE_ERROR("EnzoMethodMHDVlct", p.full_name("precision") + " must be \"double\" or \"mixed\"");
+    ENZOE_ERROR("EnzoMethodMHDVlct", p.full_name("precision") +
+                                         " must be \"double\", \"double_fma\" or \"mixed\"");
   }
 
   // the fused library handle (validation only; device memory on first use)
@@ -1370,7 +1375,7 @@ EnzoMethodMHDVlct::EnzoMethodMHDVlct(ParameterGroup p, bool store_fluxes_for_cor
     // the FP32 flux path has no internal-energy terms
     ENZOE_ASSERT("EnzoMethodMHDVlct",
                  "the dual-energy formalism needs " + p.full_name("precision") +
-                     " = \"double\"", precision_ == ENZOE_PREC_DOUBLE);
+                     " = \"double\" or \"double_fma\"", precision_ != ENZOE_PREC_MIXED);
     prm.dual_energy = "modern";
     prm.dual_energy_eta = fp->dual_energy_eta();
   } else {

@@ -42,6 +42,11 @@ typedef void *enzoe_stream_t;
 #define ENZOE_PREC_DOUBLE 0 /* everything in FP64 (reference parity)       */
 #define ENZOE_PREC_MIXED 1  /* FP32 reconstruction + Riemann fluxes; FP64
                                flux divergence, CT EMF/face-B update and dt */
+#define ENZOE_PREC_DOUBLE_FMA 2 /* FP64 with multiply-adds contracted into
+                               fused multiply-adds in the stage kernels: not
+                               bit-identical to the reference, fewer FP64
+                               instructions; the CT terms and the CFL
+                               expressions round as in ENZOE_PREC_DOUBLE  */
 
 /* Parameters of the "Method:mhd_vlct" group (EnzoMethodMHDVlct.cpp:38-101)
  * plus the fluid properties the method reads (gamma, floors:
@@ -58,12 +63,13 @@ typedef struct {
   double gamma;                   /* adiabatic index, > 1                   */
   double density_floor;           /* > 0                                    */
   double pressure_floor;          /* > 0                                    */
-  int precision;                  /* ENZOE_PREC_DOUBLE | ENZOE_PREC_MIXED   */
+  int precision;                  /* ENZOE_PREC_DOUBLE | ENZOE_PREC_MIXED |
+                                     ENZOE_PREC_DOUBLE_FMA                  */
   /* Physics:fluid_props:dual_energy (EnzoConfig.cpp:952-1040,
    * EnzoDualEnergyConfig.hpp:17-80): "disabled" (NULL) or "modern"; the
    * VL+CT method rejects "bryan95" (EnzoMHDIntegratorStageCommands.cpp:
    * 31-34).  With "modern" every block must carry internal_energy, and the
-   * precision must be ENZOE_PREC_DOUBLE.                                   */
+   * precision must be ENZOE_PREC_DOUBLE or ENZOE_PREC_DOUBLE_FMA.          */
   const char *dual_energy;
   double dual_energy_eta;         /* "modern" eta; negative = 0.001         */
 } enzoe_vlct_params;
@@ -738,11 +744,14 @@ int enzoe_vlct_diagnostics(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * against b is diff/ref, or, for a vector component, diff over the sum of
  * its vector's magnitude (a component that is zero in b has no scale of its
  * own).  Compensated per workgroup, added in a fixed order on the host.
- * Does not correspond to a reference function (the reference computes L1
+ * max_rel: per field the largest per-cell relative difference |a - b| / |b|,
+ * for a vector component over the cell's |b_vec| (0 where a equals b, NaN if
+ * any difference is NaN).  Does not correspond to a reference function (the reference computes L1
  * norms in Python, tools/l1_error_norm.py:195-234).  Synchronises.          */
 typedef struct {
   double diff[8], ref[8];
   double velocity_magnitude, bfield_magnitude;
+  double max_rel[8];
 } enzoe_state_l1;
 int enzoe_vlct_state_l1(enzoe_vlct_t m, const enzoe_block *a, const enzoe_block *b,
                         int n, enzoe_stream_t stream, enzoe_state_l1 *out);

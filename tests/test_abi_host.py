@@ -73,6 +73,14 @@ def test_abi_rejects_configurations_the_reference_rejects(kw):
     assert rc == ERR_CONFIG and not h.value
 
 
+@pytest.mark.parametrize("kw", [dict(precision=2), dict(precision=2, dual_energy=b"modern"),
+                                dict(precision=2, mhd_choice=b"no_bfield", riemann=b"hllc")])
+def test_abi_accepts_contracted_fp64(kw):
+    rc, h = _create(**kw)
+    assert rc == 0 and h.value
+    ev.lib().enzoe_vlct_destroy(h)
+
+
 def test_abi_accepts_and_defaults_courant():
     rc, h = _create()
     assert rc == 0
@@ -168,6 +176,10 @@ def test_host_method_defaults():
      "num_partial_timesteps!=2"),
     ("mhd_choice = constrained_transport; device_fused = false; precision = mixed",
      "mixed precision is only provided by the fused device path"),
+    ("mhd_choice = constrained_transport; device_fused = false; precision = double_fma",
+     "contracted FP64 is only provided by the fused device path"),
+    ("mhd_choice = constrained_transport; precision = single",
+     "must be \"double\", \"double_fma\" or \"mixed\""),
 ])
 def test_host_method_errors_exit_like_the_reference(params, message):
     r = _run_host(params)
