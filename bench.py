@@ -60,6 +60,10 @@ def parse():
     # fp64_fma: FP64 with multiply-adds contracted (not bit-identical to the
     # reference; held to 1e-12 relative per cell); Orszag-Tang only
     ap.add_argument("--precision", choices=["fp64", "mixed", "fp64_fma"], default="fp64")
+    # launch structure of the stages for the fast-wave and batched-AMR
+    # workloads (enzoe_vlct_set_fused_ct: 0 separate launches, 1-4 fused CT,
+    # 5 one persistent launch per stage); -1: the autotuner's choice
+    ap.add_argument("--structure", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the run at the other precision")
@@ -395,6 +399,10 @@ def run_fast_wave(a):
         grid.upload_from([blk])
         grid.refresh()
         ev.synchronize()
+        if a.structure >= 0:
+            method.set_fused_ct(a.structure)
+        else:
+            method.autotune(grid.blocks, method.timestep(grid.blocks))
         # eager cycles (host dt) on a copy, then the captured-graph cycles
         eager = ev.DeviceGrid(method, (1, 1, 1), (n, n, n), [cw])
         eager.upload_from([blk])
@@ -407,12 +415,27 @@ def run_fast_wave(a):
         t0 = time.perf_counter()
         nrep, t_end = grid.evolve_graph(0.0, period, ncycles=8)
         ev.synchronize()
-        el = time.perf_counter() - t0
+        el_graph = time.perf_counter() - t0
+        # the same device-driven cycles enqueued directly (no graph)
+        direct = ev.DeviceGrid(method, (1, 1, 1), (n, n, n), [cw])
+        direct.upload_from([blk])
+        direct.refresh()
+        ev.synchronize()
+        t0 = time.perf_counter()
+        ncalls, t_end_direct = direct.evolve_device(0.0, period, ncycles=8)
+        ev.synchronize()
+        el_direct = time.perf_counter() - t0
+        assert t_end_direct == t_end
+        el = min(el_graph, el_direct)
         fin = grid.blocks[0].download(["density", "velocity_x", "total_energy"])
         s = (slice(3, -3),) * 3
         l1 = {k: float(np.abs(fin[k][s] - blk[k][s]).mean()) for k in fin}
+        method.check_persistent()
         out[precision] = {"value": n ** 3 * ncycles / el, "cycles": ncycles,
                           "seconds": el, "eager_seconds": el_eager,
+                          "graph_seconds": el_graph, "direct_device_seconds": el_direct,
+                          "driver": "graph" if el_graph <= el_direct else "direct",
+                          "launch_choice": method.launch_choice((n + 6,) * 3),
                           "graph_replays_of_8_cycles": nrep, "t_end": t_end,
                           "l1_vs_initial": l1}
     return {
@@ -441,7 +464,10 @@ def run_batched_amr(a):
         method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm",
                                theta=2.0, courant=0.4, precision=prec)
         lv = workloads.SubcycledLevels(method)
-        method.autotune(lv.levels[-1], method.timestep(lv.levels[-1]) / 4)
+        if a.structure >= 0:
+            method.set_fused_ct(a.structure)
+        else:
+            method.autotune(lv.levels[-1], method.timestep(lv.levels[-1]) / 4)
         # the levels prepared on the device: a coarse step is kernels only
         # (per-level CFL, the dt kernel, the substeps); per-kernel times from
         # eager steps, the rate from a captured graph of `steps` coarse steps
@@ -466,6 +492,7 @@ def run_batched_amr(a):
                           "launch_choice": method.launch_choice((38, 38, 38)),
                           "driver": "HIP graph of %d device-driven coarse steps" % a.steps}
         plan.destroy()
+        method.check_persistent()
         del lv, method, plan
         if a.no_compare:
             break

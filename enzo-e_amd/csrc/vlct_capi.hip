@@ -484,7 +484,7 @@ int enzoe_vlct_destroy(enzoe_vlct_t m) {
   if (m->d_dts) (void)hipFree(m->d_dts);
   if (m->d_rtab) (void)hipFree(m->d_rtab);
   for (void *p : {m->d_fdtab, m->d_fltab, m->d_fbtab, m->d_trtab[0], m->d_trtab[1],
-                  (void *)m->d_dbg})
+                  (void *)m->d_dbg, m->d_bar})
     if (p) (void)hipFree(p);
   delete m;
   return ENZOE_OK;
@@ -831,29 +831,62 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   // combination of XCD-contiguous order over the four stage kernels, and
   // (MHD, setting "auto") the fused launch with each tile shape and the
   // orders of its two kernels (k_flux bit 0, k_ctu bit 3)
-  std::vector<std::pair<int, int>> cand;  // (fused-CT variant, mask)
-  const int fixed = (m->mhd && m->nstages == 2) ? m->ctu : 0;
-  for (int v = 0; v <= 4; v++) {
+  std::vector<std::pair<int, int>> cand;  // (launch structure, mask)
+  const int fixed = m->ctu;
+  const bool ct_fused = m->mhd && m->nstages == 2;
+  for (int v = 0; v <= 5; v++) {
     if (fixed >= 0 && v != fixed) continue;
+    if (v >= 1 && v <= 4 && !ct_fused) continue;
+    if (v == 5) {
+      // the persistent stage launch walks its own order
+      if (persistent_ok(m)) cand.push_back({5, 0});
+      continue;
+    }
     for (int mask = 0; mask < 16; mask++)
       if (v == 0 || (mask & (v == 4 ? 4 : 6)) == 0) cand.push_back({v, mask});
   }
+  if (persistent_ok(m) && (fixed == -1 || fixed == 5))
+    if (int rc2 = ensure_barrier(m)) rc = rc2;
+  // the time of one step of the sample under the forced settings: a timed
+  // warm-up run sets the repetitions (about 2 ms of work, 3 to 40 runs, so
+  // that small blocks are not timed on launch noise), then their average
+  auto time_runs = [&](float *ms_per_run) -> int {
+    (void)hipEventRecord(e0, s);
+    if (int r = compute_runs(m, copy.data(), ns, dt, nullptr, s)) return r;
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) return ENZOE_ERR_HIP;
+    float warm = 0.f;
+    (void)hipEventElapsedTime(&warm, e0, e1);
+    const int reps = (int)std::min(40.f, std::max(3.f, std::ceil(2.f / std::max(warm, 1e-3f))));
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; r++)
+      if (int rr = compute_runs(m, copy.data(), ns, dt, nullptr, s)) return rr;
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) return ENZOE_ERR_HIP;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_run = ms / reps;
+    return ENZOE_OK;
+  };
   for (size_t ci = 0; ci < cand.size() && rc == ENZOE_OK; ci++) {
     const int mask = cand[ci].second;
     m->force_mask = mask;
     m->force_ctu = cand[ci].first;
-    rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);  // warm-up
-    if (rc) break;
-    (void)hipEventRecord(e0, s);
-    for (int r = 0; r < 3 && rc == ENZOE_OK; r++)
-      rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);
-    (void)hipEventRecord(e1, s);
-    if (rc || hipEventSynchronize(e1) != hipSuccess) {
-      rc = rc ? rc : ENZOE_ERR_HIP;
-      break;
-    }
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
+    rc = time_runs(&ms);
+    if (rc) break;
+    if (cand[ci].first == 5) {
+      // a persistent launch whose barrier wait ran out is no candidate
+      GridBarrier gb;
+      if (hipMemcpy(&gb, m->d_bar, sizeof(gb), hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = ENZOE_ERR_HIP;
+        break;
+      }
+      if (gb.abort != 0u || gb.count != 0u) {
+        (void)hipMemset(m->d_bar, 0, sizeof(GridBarrier));
+        continue;
+      }
+    }
     if (ms < best_ms) {
       best_ms = ms;
       best = mask;
@@ -872,18 +905,9 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     for (int rows : {0, 8, 16, 32, 64}) {
       if (rows == rows0 || rows >= my) continue;
       m->force_rows = rows;
-      rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);  // warm-up
-      if (rc) break;
-      (void)hipEventRecord(e0, s);
-      for (int r = 0; r < 3 && rc == ENZOE_OK; r++)
-        rc = compute_runs(m, copy.data(), ns, dt, nullptr, s);
-      (void)hipEventRecord(e1, s);
-      if (rc || hipEventSynchronize(e1) != hipSuccess) {
-        rc = rc ? rc : ENZOE_ERR_HIP;
-        break;
-      }
       float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
+      rc = time_runs(&ms);
+      if (rc) break;
       if (ms < best_ms) {
         best_ms = ms;
         best_rows = rows;
@@ -911,7 +935,11 @@ int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask, 
   if (!m) return ENZOE_ERR_ARG;
   if (mask) *mask = launch_mask(m, mx, my, mz);
   if (tile_rows) *tile_rows = tile_rows_of(m, mx, my, mz);
-  if (fused_ct) *fused_ct = (m->mhd && ctu_possible(m)) ? ctu_variant(m, mx, my, mz) : 0;
+  if (fused_ct) {
+    const int v = ctu_variant(m, mx, my, mz);
+    *fused_ct = (v == 5) ? (persistent_ok(m) ? 5 : 0)
+                         : ((m->mhd && ctu_possible(m)) ? v : 0);
+  }
   return ENZOE_OK;
 }
 
@@ -992,9 +1020,24 @@ int enzoe_grid_refresh(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
 }
 
 int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int variant) {
-  if (!m || variant < -1 || variant > 4) return ENZOE_ERR_ARG;
+  if (!m || variant < -1 || variant > 5) return ENZOE_ERR_ARG;
   m->ctu = variant;
   return ENZOE_OK;
+}
+
+int enzoe_vlct_check_persistent(enzoe_vlct_t m) {
+  if (!m) return ENZOE_ERR_ARG;
+  if (!m->d_bar) return ENZOE_OK;
+  GridBarrier gb;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(&gb, m->d_bar, sizeof(gb), hipMemcpyDeviceToHost) != hipSuccess)
+    return ENZOE_ERR_HIP;
+  if (gb.abort == 0u && gb.count == 0u) return ENZOE_OK;
+  // a barrier wait ran out: the launch's results are not valid; the barrier
+  // starts afresh for the next launch
+  (void)hipMemset(m->d_bar, 0, sizeof(GridBarrier));
+  (void)hipDeviceSynchronize();
+  return ENZOE_ERR_HIP;
 }
 
 int enzoe_vlct_set_debug_checks(enzoe_vlct_t m, int on) {
@@ -1469,6 +1512,23 @@ int enzoe_vlct_graph_create(enzoe_vlct_t m, const enzoe_block *blocks, int nbx,
   return ENZOE_OK;
 }
 
+int enzoe_vlct_run_cycles(enzoe_vlct_t m, const enzoe_block *blocks, int nbx, int nby,
+                          int nbz, const int bc[3], double *d_state, double t_stop,
+                          int ncycles, enzoe_stream_t stream) {
+  if (!m || !blocks || !bc || !d_state || nbx <= 0 || nby <= 0 || nbz <= 0 || ncycles <= 0)
+    return ENZOE_ERR_ARG;
+  const int n = nbx * nby * nbz;
+  hipStream_t s = (hipStream_t)stream;
+  // the launches of a captured cycle (enzoe_vlct_graph_create), enqueued
+  for (int c = 0; c < ncycles; c++) {
+    if (int rc = enzoe_vlct_timestep_device(m, blocks, n, d_state, t_stop, d_state + 1, n, s))
+      return rc;
+    if (int rc = grid_refresh(m, blocks, nbx, nby, nbz, bc, s, d_state + 1)) return rc;
+    if (int rc = compute_runs(m, blocks, n, 0.0, d_state + 1, s)) return rc;
+  }
+  return ENZOE_OK;
+}
+
 int enzoe_vlct_graph_launch(enzoe_vlct_graph_t g, enzoe_stream_t stream) {
   if (!g || !g->exec) return ENZOE_ERR_ARG;
   return hipGraphLaunch(g->exec, (hipStream_t)stream) == hipSuccess ? ENZOE_OK
@@ -1495,6 +1555,9 @@ int enzoe_vlct_levels_create(enzoe_vlct_t m, const enzoe_block *blocks, const in
     if (int rc = check_block(m, blocks[i])) return rc;
     nlev = std::max(nlev, level_of[i] + 1);
   }
+  // the persistent stage launch's barrier (the autotuner may choose it)
+  if (persistent_ok(m))
+    if (int rc = ensure_barrier(m)) return rc;
   std::unique_ptr<enzoe_vlct_levels_s> L(new enzoe_vlct_levels_s);
   L->m = m;
   L->nlev = nlev;

@@ -449,13 +449,11 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
 
 // PRIMF: the stage reads the FP32 primitive half-step state of the mixed
 // path (b.Up), whose cell update also stored the centre E.
-template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD,
-          bool DE = false, bool PRIMF = false>
-__global__ void __launch_bounds__(256, RECON == RC_NN ? ENZOE_FLUX_NN_WAVES
-                                                      : ENZOE_FLUX_MIN_WAVES)
-k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  int bx, by;
-  wg_coords(sp.xcd_remap & 1, bx, by);
+// flux_cells: the work of the flux kernel's workgroup (bx, by); k_flux
+// and the persistent stage kernel call it.
+template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD, bool DE, bool PRIMF>
+__device__ __forceinline__ void flux_cells(const BlockDesc<FluxT> *__restrict__ blocks,
+                                           const StageParams &sp, int bx, int by) {
   if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
@@ -503,6 +501,16 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     flux_face<T, FluxT, RECON, RIEMANN, MHD, 0, DE, PRIMF>(b, src, bfr[0], k, j, i, c, wc, fc);
   if (j < my - 1 - cs)
     flux_face<T, FluxT, RECON, RIEMANN, MHD, 1, DE, PRIMF>(b, src, bfr[1], k, j, i, c, wc, fc);
+}
+
+template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD,
+          bool DE = false, bool PRIMF = false>
+__global__ void __launch_bounds__(256, RECON == RC_NN ? ENZOE_FLUX_NN_WAVES
+                                                      : ENZOE_FLUX_MIN_WAVES)
+k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  int bx, by;
+  wg_coords(sp.xcd_remap & 1, bx, by);
+  flux_cells<T, FluxT, RECON, RIEMANN, MHD, DE, PRIMF>(blocks, sp, bx, by);
 }
 
 // ---------------------------------------------------------------- k_edge
@@ -572,10 +580,8 @@ __device__ __forceinline__ bool edge_in_region(int ax, const int p[3], const int
 // (compute_center_efield, :267-292) was written by k_flux (or, for the mixed
 // path's final stage, by its first stage's cell update).
 template <typename FluxT>
-__global__ void __launch_bounds__(256)
-k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  int bx, by;
-  wg_coords(sp.xcd_remap & 2, bx, by);
+__device__ __forceinline__ void edge_cells(const BlockDesc<FluxT> *__restrict__ blocks,
+                                           const StageParams &sp, int bx, int by) {
   if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
@@ -594,6 +600,13 @@ k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   if (edge_in_region(0, p, m, cs)) b.E[0][c] = edge_emf<0>(b, Ecs, c, str[1], str[2]);
   if (edge_in_region(1, p, m, cs)) b.E[1][c] = edge_emf<1>(b, Ecs, c, str[2], str[0]);
   if (edge_in_region(2, p, m, cs)) b.E[2][c] = edge_emf<2>(b, Ecs, c, str[0], str[1]);
+}
+template <typename FluxT>
+__global__ void __launch_bounds__(256)
+k_edge(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  int bx, by;
+  wg_coords(sp.xcd_remap & 2, bx, by);
+  edge_cells<FluxT>(blocks, sp, bx, by);
 }
 
 // --------------------------------------------------------------- k_faces
@@ -617,10 +630,8 @@ __device__ __forceinline__ double face_b_update(double b0, double dtdj, double d
 // (EnzoBfieldMethodCT::update_bfield, EnzoBfieldMethodCT.cpp:617-687).  The
 // starting value is always the start-of-step face field.
 template <typename FluxT>
-__global__ void __launch_bounds__(256)
-k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  int bx, by;
-  wg_coords(sp.xcd_remap & 4, bx, by);
+__device__ __forceinline__ void face_cells(const BlockDesc<FluxT> *__restrict__ blocks,
+                                           const StageParams &sp, int bx, int by) {
   if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
@@ -652,6 +663,13 @@ k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
     out[cb] = face_b_update(b.bfi[ax][cb], dtdj, dtdk, Ek[e], Ek[e - str[aj]], Ej[e],
                             Ej[e - str[ak]]);
   }
+}
+template <typename FluxT>
+__global__ void __launch_bounds__(256)
+k_faces(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  int bx, by;
+  wg_coords(sp.xcd_remap & 4, bx, by);
+  face_cells<FluxT>(blocks, sp, bx, by);
 }
 
 // ------------------------------------------------------- CFL of one cell
@@ -958,11 +976,11 @@ struct FaceBGlobal {
   }
 };
 
-template <typename FluxT, bool MHD, bool DE = false>
-__global__ void __launch_bounds__(256)
-k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
-  int bx, by;
-  wg_coords(sp.xcd_remap & 8, bx, by);
+// update_cells: the work of the cell-update kernel's workgroup (bx, by);
+// with the CFL minimum every thread of the workgroup reaches its barrier.
+template <typename FluxT, bool MHD, bool DE>
+__device__ __forceinline__ void update_cells(const BlockDesc<FluxT> *__restrict__ blocks,
+                                             const StageParams &sp, int bx, int by) {
   if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
@@ -983,6 +1001,114 @@ k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   const FaceBGlobal<FluxT> fb{b, sp.is_final ? b.bfi : b.bft};
   const double local = cell_stage<FluxT, MHD, DE>(b, sp, by, k, j, i, in_launch, in_box, fb);
   if (cfl) block_min_to(sp.cfl, local);
+}
+template <typename FluxT, bool MHD, bool DE = false>
+__global__ void __launch_bounds__(256)
+k_cells(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
+  int bx, by;
+  wg_coords(sp.xcd_remap & 8, bx, by);
+  update_cells<FluxT, MHD, DE>(blocks, sp, bx, by);
+}
+
+// ------------------------------------------------ persistent stage kernel
+
+// A barrier across every workgroup of a launch whose workgroups are all
+// resident together (the launch holds no more workgroups than the device
+// runs at once).  count: arrivals at the current barrier; gen: barriers
+// passed so far; abort: set once a wait exceeds its poll limit, after which
+// every barrier returns at once, so that the launch ends (the host reads
+// the flag and reports the failure, enzoe_vlct_check_persistent).  The
+// agent-scope fences around the arrival make every store of the launch
+// before the barrier visible to every load after it, on every XCD.
+struct GridBarrier {
+  unsigned count;
+  unsigned gen;
+  unsigned abort;
+  unsigned pad;
+};
+
+// thread 0 of the workgroup keeps `gen`, the barriers it has passed
+__device__ __forceinline__ void grid_barrier(GridBarrier *gb, unsigned nwg, unsigned &gen,
+                                             unsigned limit) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (__hip_atomic_load(&gb->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+      if (atomicAdd(&gb->count, 1u) == nwg - 1u) {
+        // the last arrival: the count starts again from 0 before any
+        // workgroup can see the barrier passed
+        __hip_atomic_store(&gb->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&gb->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        unsigned polls = 0;
+        while (__hip_atomic_load(&gb->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+          if (++polls > limit) {
+            atomicExch(&gb->abort, 1u);
+            break;
+          }
+          if (__hip_atomic_load(&gb->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+            break;
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+    }
+    gen++;
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+// The phases of one persistent stage launch: per block, the workgroups of
+// the flux, edge, face and cell kernels (each phase's box as its own
+// launch would cover it).
+struct PersistParams {
+  int nb;         // blocks of the table
+  int nwg[4];     // workgroups per block: flux, edge EMF, face B, cells
+  GridBarrier *bar;
+  unsigned limit; // polls a barrier wait may take before it aborts
+};
+
+// One stage of the VL+CT step (k_flux, then for MHD k_edge and k_faces,
+// then k_cells) as one launch: the launch's workgroups stay resident and
+// take each phase's workgroups in turn, with a grid barrier between the
+// phases where the separate launches had a kernel boundary.  The same
+// per-workgroup work in the same cell order as the separate kernels (their
+// plain order), so the same bits.  Workgroup w takes, in every phase, the
+// virtual workgroups of one contiguous eighth of the phase (w % 8: the XCD
+// the hardware deals it to), strided by the launch's workgroups per XCD.
+template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD, bool PRIMF>
+__global__ void __launch_bounds__(256, RECON == RC_NN ? ENZOE_FLUX_NN_WAVES
+                                                      : ENZOE_FLUX_MIN_WAVES)
+k_stage(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp, PersistParams pp) {
+  unsigned gen = 0;
+  if (threadIdx.x == 0)
+    gen = __hip_atomic_load(&pp.bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned G = gridDim.x;  // a multiple of 8
+  const unsigned x = blockIdx.x & 7u, r = blockIdx.x >> 3, G8 = G >> 3;
+  auto phase = [&](int nwg, auto &&body) {
+    const unsigned nv = (unsigned)nwg * (unsigned)pp.nb;
+    const unsigned per = (nv + 7u) >> 3;
+    const unsigned v0 = x * per;
+    const unsigned v1 = (v0 + per < nv) ? v0 + per : nv;
+    for (unsigned v = v0 + r; v < v1; v += G8) body((int)(v % (unsigned)nwg), (int)(v / (unsigned)nwg));
+  };
+  phase(pp.nwg[0], [&](int bx, int by) {
+    flux_cells<T, FluxT, RECON, RIEMANN, MHD, false, PRIMF>(blocks, sp, bx, by);
+  });
+  grid_barrier(pp.bar, G, gen, pp.limit);
+  if (MHD) {
+    phase(pp.nwg[1], [&](int bx, int by) { edge_cells<FluxT>(blocks, sp, bx, by); });
+    grid_barrier(pp.bar, G, gen, pp.limit);
+    phase(pp.nwg[2], [&](int bx, int by) { face_cells<FluxT>(blocks, sp, bx, by); });
+    grid_barrier(pp.bar, G, gen, pp.limit);
+  }
+  const bool cfl = sp.cfl != nullptr;
+  phase(pp.nwg[3], [&](int bx, int by) {
+    update_cells<FluxT, MHD, false>(blocks, sp, bx, by);
+    // the CFL reduction's LDS words are read by thread 0 before the next
+    // virtual workgroup writes them
+    if (cfl) __syncthreads();
+  });
 }
 
 // ----------------------------------------------------------------- k_ctu

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <string>
@@ -79,10 +80,12 @@ struct enzoe_vlct_method {
   int xcd_remap = 2;
   double theta = 1.5, courant = 0.3, gamma = 5.0 / 3.0;
   double dfloor = 1e-200, pfloor = 1e-200;
-  // MHD stages: constrained transport and the cell update as three launches
-  // (0: k_edge, k_faces, k_cells) or one fused launch over tiles (k_ctu
-  // with tile 1..3), same bits either way; -1 (the default): the
-  // autotuner's choice per block shape, three launches until it has run
+  // launch structure of the stages: for MHD constrained transport and the
+  // cell update as three launches (0: k_edge, k_faces, k_cells) or fused
+  // launches over tiles (1..4, ctu_variant); 5: each stage as one
+  // persistent launch (k_stage, MHD or hydro); same bits every way; -1
+  // (the default): the autotuner's choice per block shape, three launches
+  // until it has run
   int ctu = -1;
   int force_ctu = -1;                    // set while the autotuner times one
   std::map<long long, int> tuned_ctu;    // the autotuner's choice per shape
@@ -90,6 +93,9 @@ struct enzoe_vlct_method {
   // tile_rows_for otherwise) and the value forced while it times one
   std::map<long long, int> tuned_rows;
   int force_rows = -1;
+  // the persistent stage kernel's grid barrier (a GridBarrier in device
+  // memory, zeroed once), or null until first needed
+  void *d_bar = nullptr;
   // dual-energy formalism ("modern" formulation) and its eta
   int dual = 0;
   double eta = 0.001;
@@ -145,11 +151,12 @@ inline long long shape_key(int mx, int my, int mz) {
   return ((long long)mx << 40) | ((long long)my << 20) | (long long)mz;
 }
 
-// The launch structure of the MHD stages for blocks of this shape: 0 =
-// k_edge + k_faces + k_cells, 1..3 = k_ctu with tile CtuTile1..3, 4 =
-// k_edge + k_ctu without its edge phase (faces and cells fused); the
-// method's setting, or the autotuner's choice for the shape when the
-// setting is "auto" (-1).
+// The launch structure of the stages for blocks of this shape: 0 =
+// k_flux, then k_edge + k_faces + k_cells; for MHD 1..3 = k_ctu with tile
+// CtuTile1..3, 4 = k_edge + k_ctu without its edge phase (faces and cells
+// fused); 5 = the persistent k_stage (persistent_ok); the method's
+// setting, or the autotuner's choice for the shape when the setting is
+// "auto" (-1).
 inline int ctu_variant(const enzoe_vlct_method *m, int mx, int my, int mz) {
   if (m->force_ctu >= 0) return m->force_ctu;
   if (m->ctu >= 0) return m->ctu;
@@ -162,6 +169,87 @@ inline int ctu_variant(const enzoe_vlct_method *m, int mx, int my, int mz) {
 // scratch holds that copy whenever the fused launch may be chosen.
 inline bool ctu_possible(const enzoe_vlct_method *m) {
   return m->mhd && m->nstages == 2 && (m->ctu != 0 || m->force_ctu > 0);
+}
+
+// The persistent stage launch (structure 5) exists for HLLD MHD and for
+// hydro without the dual-energy formalism, and not on the allocation-free
+// path (its barrier is a buffer of the method).
+inline bool persistent_ok(const enzoe_vlct_method *m) {
+  return !m->dual && (!m->mhd || m->riemann == RS_HLLD) && m->arena == nullptr;
+}
+
+// The method's grid barrier, allocated and zeroed on first use (not while
+// a graph is being captured: compute_group takes it before any launch,
+// also in the dry run that precedes a capture).
+inline int ensure_barrier(enzoe_vlct_method *m) {
+  if (m->d_bar) return 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, sizeof(GridBarrier)) != hipSuccess) return ENZOE_ERR_ALLOC;
+  if (hipMemset(p, 0, sizeof(GridBarrier)) != hipSuccess) {
+    (void)hipFree(p);
+    return ENZOE_ERR_HIP;
+  }
+  m->d_bar = p;
+  return 0;
+}
+
+// Workgroups of a persistent stage kernel the current device holds at once
+// (occupancy x CUs), cached per device; 0 on failure.
+template <typename FluxT, int RECON, int RIEMANN, bool MHD, bool PRIMF>
+int stage_capacity() {
+  static std::atomic<int> cap[16];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 0;
+  int c = cap[dev].load(std::memory_order_relaxed);
+  if (c == 0) {
+    int per = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, k_stage<FluxT, FluxT, RECON, RIEMANN, MHD, PRIMF>, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    c = per * cus;
+    cap[dev].store(c, std::memory_order_relaxed);
+  }
+  return c;
+}
+
+// One stage as one persistent launch: as many workgroups as the largest
+// phase needs, at most what the device holds at once (a multiple of 8).
+template <typename FluxT, int RECON, int RIEMANN, bool MHD, bool PRIMF>
+int launch_stage(const enzoe_vlct_method *m, const BlockDesc<FluxT> *d, int nb,
+                 const int nwg[4], StageParams sp, hipStream_t s) {
+  const int cap = stage_capacity<FluxT, RECON, RIEMANN, MHD, PRIMF>() & ~7;
+  if (cap < 8 || !m->d_bar) return ENZOE_ERR_HIP;
+  long need = 0;
+  for (int q = 0; q < 4; q++) need = std::max(need, (long)nwg[q] * nb);
+  const int G = (int)std::min<long>(cap, std::max<long>(8, (need + 7) & ~7L));
+  PersistParams pp;
+  pp.nb = nb;
+  for (int q = 0; q < 4; q++) pp.nwg[q] = nwg[q];
+  pp.bar = (GridBarrier *)m->d_bar;
+  pp.limit = 1u << 22;
+  sp.xcd_remap = 0;  // the phases walk their boxes in the plain order
+  hipLaunchKernelGGL((k_stage<FluxT, FluxT, RECON, RIEMANN, MHD, PRIMF>), dim3(G), dim3(256),
+                     0, s, d, sp, pp);
+  return 0;
+}
+
+template <typename FluxT, bool MHD>
+int dispatch_stage(const enzoe_vlct_method *m, int recon, bool primf,
+                   const BlockDesc<FluxT> *d, int nb, const int nwg[4],
+                   const StageParams &sp, hipStream_t s) {
+  constexpr int RS = MHD ? RS_HLLD : RS_HLLC;
+  if constexpr (std::is_same<FluxT, float>::value) {
+    if (primf) {
+      if (recon == RC_NN) return launch_stage<FluxT, RC_NN, RS, MHD, true>(m, d, nb, nwg, sp, s);
+      if (recon == RC_PLM)
+        return launch_stage<FluxT, RC_PLM, RS, MHD, true>(m, d, nb, nwg, sp, s);
+      return launch_stage<FluxT, RC_PLM_ATHENA, RS, MHD, true>(m, d, nb, nwg, sp, s);
+    }
+  }
+  if (recon == RC_NN) return launch_stage<FluxT, RC_NN, RS, MHD, false>(m, d, nb, nwg, sp, s);
+  if (recon == RC_PLM) return launch_stage<FluxT, RC_PLM, RS, MHD, false>(m, d, nb, nwg, sp, s);
+  return launch_stage<FluxT, RC_PLM_ATHENA, RS, MHD, false>(m, d, nb, nwg, sp, s);
 }
 template <typename FluxT, bool DE, class Tile, bool EDGES_LDS = true>
 void launch_ctu(const BlockDesc<FluxT> *d, int nb, int mx, int my, int mz, int lo,
@@ -491,14 +579,29 @@ int launch_chunks(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int ch
       sp.cfl = is_final ? cfl : nullptr;
       sp.dbg = m->d_dbg;
       const int cs = sp.cur_stale;
-      {
-        KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm", timers);
-        dispatch_flux<FluxT, MHD, DE>(m, recon, stage > 0 && half_f32<FluxT>(DE ? 1 : 0), d,
-                                      nb, cube(mx, my, mz, cs, cs), sp, s);
-      }
+      const bool primf = stage > 0 && half_f32<FluxT>(DE ? 1 : 0);
       // with the fused CFL the final update's launch covers the block
       const int lo = sp.cfl ? 0 : (is_final ? cs : cs + 1);
-      const int variant = (MHD && ctu_possible(m)) ? ctu_variant(m, mx, my, mz) : 0;
+      const int structure = ctu_variant(m, mx, my, mz);
+      if constexpr (!DE) {
+        if (structure == 5 && persistent_ok(m) && m->d_bar) {
+          // the stage's four kernels as one persistent launch
+          const int nwg[4] = {(int)nblk(cube(mx, my, mz, cs, cs), 256),
+                              (int)nblk(cube(mx, my, mz, cs, cs), 256),
+                              (int)nblk(cube(mx, my, mz, cs + 1, cs), 256),
+                              (int)nblk(cube(mx, my, mz, lo, lo), 256)};
+          KernelTimer kt(s, recon == RC_NN ? "k_stage_nn" : "k_stage_plm", timers);
+          if (int rc = dispatch_stage<FluxT, MHD>(m, recon, primf, d, nb, nwg, sp, s)) return rc;
+          stale += total_staling(recon);
+          continue;
+        }
+      }
+      {
+        KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm", timers);
+        dispatch_flux<FluxT, MHD, DE>(m, recon, primf, d, nb, cube(mx, my, mz, cs, cs), sp, s);
+      }
+      const int variant =
+          (MHD && ctu_possible(m) && structure >= 1 && structure <= 4) ? structure : 0;
       if (variant == 4) {
         {
           KernelTimer kt(s, "k_edge", timers);
@@ -615,6 +718,8 @@ int compute_group(enzoe_vlct_method *m, const enzoe_block *blocks, int n,
       m->last_desc.assign(raw, raw + dbytes);
     }
   }
+  if (!DE && persistent_ok(m) && (m->ctu == -1 || m->ctu == 5 || m->force_ctu == 5))
+    if (int rc = ensure_barrier(m)) return rc;
   if (m->dry) return 0;
   return launch_chunks<FluxT, MHD, DE>(m, blocks, n, chunk,
                                        (const BlockDesc<FluxT> *)m->d_desc, dt, dts, s, st,

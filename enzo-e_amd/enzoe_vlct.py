@@ -170,6 +170,11 @@ def lib():
                                          ctypes.POINTER(ctypes.c_int), vp]
         L.enzoe_vlct_set_debug_checks.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_set_fused_ct.argtypes = [vp, ctypes.c_int]
+        L.enzoe_vlct_check_persistent.argtypes = [vp]
+        L.enzoe_vlct_run_cycles.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int), vp, ctypes.c_double,
+                                            ctypes.c_int, vp]
         L.enzoe_vlct_launch_choice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.POINTER(ctypes.c_int),
                                                ctypes.POINTER(ctypes.c_int),
@@ -381,12 +386,18 @@ class VlctMethod:
                 "nonfinite": int(d.nonfinite)}
 
     def set_fused_ct(self, variant):
-        """Launch structure of the MHD stages: 0 (or False) three launches
-        through memory, 1..3 (True = 1) one fused CT + update launch with
-        tile 16x4x4 / 32x4x2 / 8x8x4, 4 the edge EMFs then one face + cell
-        launch, -1 the autotuner's choice."""
+        """Launch structure of the stages: 0 (or False) the flux launch and
+        (MHD) three launches through memory, 1..3 (True = 1) one fused CT +
+        update launch with tile 16x4x4 / 32x4x2 / 8x8x4, 4 the edge EMFs then
+        one face + cell launch, 5 each stage as one persistent launch, -1 the
+        autotuner's choice."""
         v = int(variant) if not isinstance(variant, bool) else (1 if variant else 0)
         check(lib().enzoe_vlct_set_fused_ct(self.handle, v), "enzoe_vlct_set_fused_ct")
+
+    def check_persistent(self):
+        """Raises if a barrier wait of a persistent stage launch ran out
+        since the last check (synchronises)."""
+        check(lib().enzoe_vlct_check_persistent(self.handle), "enzoe_vlct_check_persistent")
 
     def launch_choice(self, shape):
         """(XCD mask, fused-CT variant, tile rows) used for blocks of shape
@@ -522,6 +533,27 @@ class DeviceGrid:
             t = g.time()
             k += 1
         g.destroy()
+        return k, t
+
+    def evolve_device(self, t0, t_stop, ncycles=16, max_calls=1 << 20):
+        """evolve() with device-driven cycles enqueued without a graph
+        (enzoe_vlct_run_cycles): `ncycles` per call, the device time read
+        back after each call.  Returns the number of calls and the final
+        time."""
+        n = len(self.blocks)
+        state = DeviceBuffer((n + 1) * 8)
+        to_device(state.ptr.value, np.array([t0] + [0.0] * n))
+        arr = (Block * n)(*[b.desc for b in self.blocks])
+        bcs = (ctypes.c_int * 3)(*[BC[b] if isinstance(b, str) else b for b in self.bc])
+        t, k = t0, 0
+        while t < t_stop and k < max_calls:
+            check(lib().enzoe_vlct_run_cycles(self.method.handle, arr, self.nblocks[0],
+                                              self.nblocks[1], self.nblocks[2], bcs,
+                                              state.ptr, t_stop, ncycles, None),
+                  "enzoe_vlct_run_cycles")
+            t = float(from_device(state.ptr.value, (1,))[0])
+            k += 1
+        state.free()
         return k, t
 
     def evolve(self, t0, t_stop, max_cycles=1 << 30):

@@ -170,14 +170,16 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * fastest for this block shape; later compute calls on blocks of that shape
  * use it.  The candidates: every combination of XCD-contiguous workgroup
  * order over the four stage kernels (k_flux, k_edge, k_faces, k_cells)
- * and, for MHD with the fused-CT setting "auto" (enzoe_vlct_set_fused_ct),
- * the fused CT + update launch with each of its tile shapes and the orders
- * of its two kernels; then, for the chosen one, the rows of the tiled cell
- * order (slabs of 8, 16, 32 or 64 rows, or the plain order).  *mask_out
- * (optional) gets the chosen mask: bit 0 k_flux, 1 k_edge, 2 k_faces,
- * 3 k_cells (or the fused launch).  enzoe_vlct_launch_choice: the mask, the
- * fused-CT variant (0 = three launches, 1..4) and the tile rows calls on
- * blocks of this shape use.                                                 */
+ * and, with the launch-structure setting "auto" (enzoe_vlct_set_fused_ct),
+ * for MHD the fused CT + update launch with each of its tile shapes and the
+ * orders of its two kernels, and (HLLD MHD or hydro, no dual energy) the
+ * persistent launch of a whole stage; then, for the chosen one, the rows of
+ * the tiled cell order (slabs of 8, 16, 32 or 64 rows, or the plain
+ * order).  *mask_out (optional) gets the chosen mask: bit 0 k_flux,
+ * 1 k_edge, 2 k_faces, 3 k_cells (or the fused launch).
+ * enzoe_vlct_launch_choice: the mask, the launch structure (0 = four
+ * launches per stage, 1..5 as enzoe_vlct_set_fused_ct) and the tile rows
+ * calls on blocks of this shape use.                                       */
 int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                         double dt, enzoe_stream_t stream, int *mask_out);
 int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask,
@@ -235,6 +237,15 @@ int enzoe_vlct_graph_launch(enzoe_vlct_graph_t g, enzoe_stream_t stream);
 int enzoe_vlct_graph_time(enzoe_vlct_graph_t g, enzoe_stream_t stream,
                           double *t);
 int enzoe_vlct_graph_destroy(enzoe_vlct_graph_t g);
+
+/* The same `ncycles` cycles enqueued on `stream` without a graph (no host
+ * synchronisation either): d_state is n + 1 doubles of device memory, [0]
+ * the time (read and advanced by each cycle), [1 .. n] the per-block dt
+ * the cycles write.  On this platform a captured graph of these short
+ * kernels replays more slowly than the same launches enqueued directly.    */
+int enzoe_vlct_run_cycles(enzoe_vlct_t m, const enzoe_block *blocks, int nbx, int nby,
+                          int nbz, const int bc[3], double *d_state, double t_stop,
+                          int ncycles, enzoe_stream_t stream);
 
 /* EnzoMethodMHDVlct::compute on blocks with acceleration fields
  * (EnzoMethodMHDVlct.cpp:219-231, 410-414): the step of enzoe_vlct_compute
@@ -693,18 +704,27 @@ int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
  * energy floor (EnzoPhysicsFluidProps.cpp:278-285) and the reconstruction's
  * density/pressure floors (EnzoReconstructorPLM.hpp:219-220).  The C++ layer
  * turns a nonzero count into the reference's ERROR.                         */
-/* Launch structure of the MHD stages (no reference counterpart: the
- * reference runs every operation as its own loop).  variant 0: three
- * launches (edge EMFs, face update, cell update) through memory; 1, 2, 3:
- * the constrained transport and the cell update of a stage as one fused
- * launch over tiles of 16x4x4, 32x4x2 or 8x8x4 cells, the edge EMFs and
- * face-centred B of a tile held in LDS; 4: the edge EMFs through memory,
- * then the face update and the cell update as one launch over 16x4x4
- * tiles (face B in LDS); -1 (the default): the autotuner's
- * choice per block shape (three launches until it has run).  Every variant
- * gives the same bits.  Not to be changed while work of the method is
- * queued.                                                                   */
+/* Launch structure of the stages (no reference counterpart: the reference
+ * runs every operation as its own loop).  variant 0: the flux launch, then
+ * (MHD) three launches (edge EMFs, face update, cell update) through
+ * memory; for MHD, 1, 2, 3: the constrained transport and the cell update
+ * of a stage as one fused launch over tiles of 16x4x4, 32x4x2 or 8x8x4
+ * cells, the edge EMFs and face-centred B of a tile held in LDS; 4: the
+ * edge EMFs through memory, then the face update and the cell update as
+ * one launch over 16x4x4 tiles (face B in LDS); 5: each stage as one
+ * persistent launch whose resident workgroups run the flux, edge, face and
+ * cell phases in turn with a device-wide barrier between them (HLLD MHD or
+ * hydro without the dual-energy formalism; elsewhere, and on the
+ * allocation-free entry point, the four launches); -1 (the default): the
+ * autotuner's choice per block shape (the four launches until it has run).
+ * Every variant gives the same bits.  Not to be changed while work of the
+ * method is queued.
+ * enzoe_vlct_check_persistent: synchronises and reports ENZOE_ERR_HIP if a
+ * barrier wait of a persistent launch ran out (the workgroups of a launch
+ * could not all be resident: results since the last check are not valid;
+ * the barrier is reset), ENZOE_OK otherwise.                               */
 int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int variant);
+int enzoe_vlct_check_persistent(enzoe_vlct_t m);
 
 #define ENZOE_DEBUG_NONFINITE 0
 #define ENZOE_DEBUG_DENSITY_FLOOR 1

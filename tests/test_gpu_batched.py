@@ -214,6 +214,29 @@ def test_graph_replays_match_eager_cycles():
         np.testing.assert_array_equal(a[k], b[k])
 
 
+def test_device_cycles_without_a_graph_match_eager_cycles():
+    """enzoe_vlct_run_cycles (the graph's launches enqueued directly, device
+    dt, no host synchronisation) equals host-driven eager cycles bit for
+    bit, the last cycle clipped to t_stop and the cycles after it
+    zero-length."""
+    ev = _ev()
+    from _states import random_periodic_grid
+    grid = random_periodic_grid(mhd=True, seed=13, n=(11, 10, 9), vamp=0.4)
+    method = method_for(grid.cfg)
+    eager = device_copy(grid, method)
+    dev = device_copy(grid, method)
+    n_cyc, log = eager.evolve(0.0, 10.0, max_cycles=5)
+    t_stop = sum(log) - 0.3 * log[-1]
+    eager2 = device_copy(grid, method)
+    n2, _ = eager2.evolve(0.0, t_stop)
+    ncalls, t_end = dev.evolve_device(0.0, t_stop, ncycles=3)
+    ev.synchronize()
+    assert t_end == t_stop and n2 == 5 and ncalls == 2
+    a, b = dev.blocks[0].download(), eager2.blocks[0].download()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_graph_clipped_cycle_that_rounds_past_t_stop():
     """A stop time whose clipped cycle ends, in floating point, just above
     t_stop: t0 + (t_stop - t0) > t_stop for this pair (found by search;

@@ -12,8 +12,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _make(makefile):
-    r = subprocess.run(["make", "-s", "-f", os.path.join(ROOT, makefile)],
-                       capture_output=True, text=True, timeout=600)
+    # one build at a time (tests of this module may run in parallel workers)
+    import fcntl
+    import tempfile
+    with open(os.path.join(tempfile.gettempdir(), "enzoe_sanitize_build.lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        r = subprocess.run(["make", "-s", "-f", os.path.join(ROOT, makefile)],
+                           capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
 
 
