@@ -505,14 +505,16 @@ def run_batched_amr(a):
         "data": "synthetic (seeded random smooth MHD blocks: lognormal rho, Mach 2, beta 1)",
         "config": {"workload": "batched_amr_32_512_1536_3000_%s" % a.precision,
                    "blocks_per_level": [512, 1536, 3000], "block": [32, 32, 32],
-                   "subcycling": "2:1", "launch_blocks": r["launch_blocks"]},
+                   "subcycling": "2:1", "launch_blocks": r["launch_blocks"],
+                   "launch_choice": r["launch_choice"]},
         "roofline": r["roofline"], "kernels_ms": r["kernels_ms"],
     }
     if len(res) == 2:
         f, m = res["fp64"], res["mixed"]
         out["precision_comparison"] = {"fp64_value": f["value"], "mixed_value": m["value"],
                                        "mixed_over_fp64": m["value"] / f["value"],
-                                       "mixed_kernels_ms": m["kernels_ms"]}
+                                       "mixed_kernels_ms": m["kernels_ms"],
+                                       "mixed_launch_choice": m["launch_choice"]}
     return out
 
 
