@@ -450,10 +450,14 @@ __device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
 // PRIMF: the stage reads the FP32 primitive half-step state of the mixed
 // path (b.Up), whose cell update also stored the centre E.
 // flux_cells: the work of the flux kernel's workgroup (bx, by); k_flux
-// and the persistent stage kernel call it.
+// and the persistent stage kernel call it.  dir: -1, the three low faces of
+// each cell; 0, 1, 2, only the face along that axis (the face-split launch
+// of small blocks: three workgroups per cell range, one face each; the
+// first one also writes the centre E).
 template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD, bool DE, bool PRIMF>
 __device__ __forceinline__ void flux_cells(const BlockDesc<FluxT> *__restrict__ blocks,
-                                           const StageParams &sp, int bx, int by) {
+                                           const StageParams &sp, int bx, int by,
+                                           int dir = -1) {
   if (zero_step(sp, by)) return;
   const BlockDesc<FluxT> &b = blocks[by];
   const int mx = b.mx, my = b.my, mz = b.mz;
@@ -481,7 +485,7 @@ __device__ __forceinline__ void flux_cells(const BlockDesc<FluxT> *__restrict__ 
   } else {
     double wd[S_N];
     load_prim<MHD, DE>(U, c, fc.gm1, wd, ei);
-    if (MHD) {
+    if (MHD && dir <= 0) {
       // centre E_i = -v_j B_k + v_k B_j of this cell for the edge kernel
       // (EnzoBfieldMethodCT::compute_center_efield, EnzoBfieldMethodCT.cpp:
       // 267-292); the box of k_flux covers every cell the edges read
@@ -495,11 +499,11 @@ __device__ __forceinline__ void flux_cells(const BlockDesc<FluxT> *__restrict__ 
   // the z face first: in the tiled order the plane above is the one not yet
   // in L2, so its loads go out with this cell's and the two latencies
   // overlap (each face writes its own arrays; the order changes no value)
-  if (k < mz - 1 - cs)
+  if ((dir < 0 || dir == 2) && k < mz - 1 - cs)
     flux_face<T, FluxT, RECON, RIEMANN, MHD, 2, DE, PRIMF>(b, src, bfr[2], k, j, i, c, wc, fc);
-  if (i < mx - 1 - cs)
+  if ((dir < 0 || dir == 0) && i < mx - 1 - cs)
     flux_face<T, FluxT, RECON, RIEMANN, MHD, 0, DE, PRIMF>(b, src, bfr[0], k, j, i, c, wc, fc);
-  if (j < my - 1 - cs)
+  if ((dir < 0 || dir == 1) && j < my - 1 - cs)
     flux_face<T, FluxT, RECON, RIEMANN, MHD, 1, DE, PRIMF>(b, src, bfr[1], k, j, i, c, wc, fc);
 }
 
@@ -510,7 +514,9 @@ __global__ void __launch_bounds__(256, RECON == RC_NN ? ENZOE_FLUX_NN_WAVES
 k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   int bx, by;
   wg_coords(sp.xcd_remap & 1, bx, by);
-  flux_cells<T, FluxT, RECON, RIEMANN, MHD, DE, PRIMF>(blocks, sp, bx, by);
+  // a launch with three z slices is the face-split one
+  flux_cells<T, FluxT, RECON, RIEMANN, MHD, DE, PRIMF>(blocks, sp, bx, by,
+                                                      gridDim.z == 3 ? (int)blockIdx.z : -1);
 }
 
 // ---------------------------------------------------------------- k_edge
@@ -769,6 +775,10 @@ template <typename FluxT, bool MHD, bool DE, typename FaceB>
 __device__ __forceinline__ double cell_stage(const BlockDesc<FluxT> &b, const StageParams &sp,
                                              int by, int k, int j, int i, bool in_launch,
                                              bool in_box, const FaceB &face_b) {
+  // the FP64 update is never contracted into FMAs, also where the including
+  // unit allows it: every kernel that runs it (k_cells, k_ctu, k_stage)
+  // rounds alike
+#pragma clang fp contract(off)
   const int mx = b.mx, my = b.my, mz = b.mz;
   const int cs = sp.cur_stale;
   const bool cfl = sp.cfl != nullptr;
@@ -778,6 +788,7 @@ __device__ __forceinline__ double cell_stage(const BlockDesc<FluxT> &b, const St
   constexpr bool HALF_F32 = std::is_same<FluxT, float>::value && !DE;
   const bool to_f32 = HALF_F32 && !sp.is_final;
   auto body = [&]() {
+#pragma clang fp contract(off)
     const long c = idx3(k, j, i, my, mx);
     double *const *Uo = sp.is_final ? b.U : b.Ut;
     double bc[3] = {0., 0., 0.};

@@ -93,6 +93,12 @@ struct enzoe_vlct_method {
   // tile_rows_for otherwise) and the value forced while it times one
   std::map<long long, int> tuned_rows;
   int force_rows = -1;
+  // the flux launch with one face per thread (three workgroups per cell
+  // range) for blocks of this shape: the autotuner's choice, and the value
+  // forced while it times one
+  std::map<long long, int> tuned_split;
+  int force_split = -1;
+  int face_split = -1;  // the method's setting: 0, 1, or -1 (the autotuner's)
   // the persistent stage kernel's grid barrier (a GridBarrier in device
   // memory, zeroed once), or null until first needed
   void *d_bar = nullptr;
@@ -363,6 +369,15 @@ inline int tile_rows_for(int mx, int my) {
 }
 
 
+// Whether the flux launch for blocks of this shape puts one face per
+// thread (the autotuner's choice; three faces per thread otherwise).
+inline bool face_split_of(const enzoe_vlct_method *m, int mx, int my, int mz) {
+  if (m->force_split >= 0) return m->force_split != 0;
+  if (m->face_split >= 0) return m->face_split != 0;
+  auto it = m->tuned_split.find(shape_key(mx, my, mz));
+  return it != m->tuned_split.end() && it->second != 0;
+}
+
 // Rows of the tiled cell order for blocks of this shape.
 inline int tile_rows_of(const enzoe_vlct_method *m, int mx, int my, int mz) {
   if (m->force_rows >= 0) return m->force_rows;
@@ -466,8 +481,8 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
 
 template <typename FluxT, int RECON, int RIEMANN, bool MHD, bool DE, bool PRIMF>
 void launch_flux(const BlockDesc<FluxT> *d, int nb, long nthreads,
-                 StageParams sp, hipStream_t s) {
-  dim3 grid(nblk8(nthreads, 256), nb);
+                 StageParams sp, hipStream_t s, bool split) {
+  dim3 grid(nblk8(nthreads, 256), nb, split ? 3 : 1);
   hipLaunchKernelGGL((k_flux<FluxT, FluxT, RECON, RIEMANN, MHD, DE, PRIMF>), grid,
                      dim3(256), 0, s, d, sp);
 }
@@ -475,31 +490,38 @@ void launch_flux(const BlockDesc<FluxT> *d, int nb, long nthreads,
 template <typename FluxT, bool MHD, bool DE, bool PRIMF>
 void dispatch_flux_p(const enzoe_vlct_method *m, int recon,
                      const BlockDesc<FluxT> *d, int nb, long nthreads,
-                     StageParams sp, hipStream_t s) {
+                     StageParams sp, hipStream_t s, bool split) {
   constexpr int RS = MHD ? RS_HLLD : RS_HLLC;
   if (MHD && m->riemann != RS_HLLD) {
-    if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
-    else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
-    else launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
+    if (recon == RC_NN)
+      launch_flux<FluxT, RC_NN, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+    else if (recon == RC_PLM)
+      launch_flux<FluxT, RC_PLM, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+    else
+      launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
     return;
   }
-  if (recon == RC_NN) launch_flux<FluxT, RC_NN, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
-  else if (recon == RC_PLM) launch_flux<FluxT, RC_PLM, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
-  else launch_flux<FluxT, RC_PLM_ATHENA, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s);
+  if (recon == RC_NN)
+    launch_flux<FluxT, RC_NN, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+  else if (recon == RC_PLM)
+    launch_flux<FluxT, RC_PLM, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+  else
+    launch_flux<FluxT, RC_PLM_ATHENA, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
 }
 
-// primf: the stage reads the mixed path's FP32 primitive half-step state
+// primf: the stage reads the mixed path's FP32 primitive half-step state;
+// split: one face per thread (face_split_of)
 template <typename FluxT, bool MHD, bool DE>
 void dispatch_flux(const enzoe_vlct_method *m, int recon, bool primf,
                    const BlockDesc<FluxT> *d, int nb, long nthreads,
-                   StageParams sp, hipStream_t s) {
+                   StageParams sp, hipStream_t s, bool split) {
   if constexpr (half_f32<FluxT>(DE ? 1 : 0)) {
     if (primf) {
-      dispatch_flux_p<FluxT, MHD, DE, true>(m, recon, d, nb, nthreads, sp, s);
+      dispatch_flux_p<FluxT, MHD, DE, true>(m, recon, d, nb, nthreads, sp, s, split);
       return;
     }
   }
-  dispatch_flux_p<FluxT, MHD, DE, false>(m, recon, d, nb, nthreads, sp, s);
+  dispatch_flux_p<FluxT, MHD, DE, false>(m, recon, d, nb, nthreads, sp, s, split);
 }
 
 // gamma (gamma - 1) as the reference's synchronisation holds it: a float
@@ -598,7 +620,8 @@ int launch_chunks(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int ch
       }
       {
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm", timers);
-        dispatch_flux<FluxT, MHD, DE>(m, recon, primf, d, nb, cube(mx, my, mz, cs, cs), sp, s);
+        dispatch_flux<FluxT, MHD, DE>(m, recon, primf, d, nb, cube(mx, my, mz, cs, cs), sp, s,
+                                      face_split_of(m, mx, my, mz));
       }
       const int variant =
           (MHD && ctu_possible(m) && structure >= 1 && structure <= 4) ? structure : 0;

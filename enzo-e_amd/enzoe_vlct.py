@@ -171,11 +171,13 @@ def lib():
         L.enzoe_vlct_set_debug_checks.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_set_fused_ct.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_check_persistent.argtypes = [vp]
+        L.enzoe_vlct_set_face_split.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_run_cycles.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int), vp, ctypes.c_double,
                                             ctypes.c_int, vp]
         L.enzoe_vlct_launch_choice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_int),
                                                ctypes.POINTER(ctypes.c_int),
                                                ctypes.POINTER(ctypes.c_int),
                                                ctypes.POINTER(ctypes.c_int)]
@@ -394,20 +396,26 @@ class VlctMethod:
         v = int(variant) if not isinstance(variant, bool) else (1 if variant else 0)
         check(lib().enzoe_vlct_set_fused_ct(self.handle, v), "enzoe_vlct_set_fused_ct")
 
+    def set_face_split(self, split):
+        """Force the flux launch with one face per thread (1), three (0), or
+        leave it to the autotuner (-1).  Every setting gives the same bits."""
+        check(lib().enzoe_vlct_set_face_split(self.handle, int(split)),
+              "enzoe_vlct_set_face_split")
+
     def check_persistent(self):
         """Raises if a barrier wait of a persistent stage launch ran out
         since the last check (synchronises)."""
         check(lib().enzoe_vlct_check_persistent(self.handle), "enzoe_vlct_check_persistent")
 
     def launch_choice(self, shape):
-        """(XCD mask, fused-CT variant, tile rows) used for blocks of shape
-        (mx, my, mz)."""
-        mask, ctu, rows = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        """(XCD mask, launch structure, tile rows, face split) used for blocks
+        of shape (mx, my, mz)."""
+        mask, ctu, rows, split = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().enzoe_vlct_launch_choice(self.handle, shape[0], shape[1], shape[2],
                                              ctypes.byref(mask), ctypes.byref(ctu),
-                                             ctypes.byref(rows)),
+                                             ctypes.byref(rows), ctypes.byref(split)),
               "enzoe_vlct_launch_choice")
-        return mask.value, ctu.value, rows.value
+        return mask.value, ctu.value, rows.value, split.value
 
     def state_l1(self, blocks, ref_blocks, stream=None, max_rel=False):
         """L1 of the blocks' cell-centred fields against ref_blocks' (same

@@ -175,15 +175,16 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * orders of its two kernels, and (HLLD MHD or hydro, no dual energy) the
  * persistent launch of a whole stage; then, for the chosen one, the rows of
  * the tiled cell order (slabs of 8, 16, 32 or 64 rows, or the plain
- * order).  *mask_out (optional) gets the chosen mask: bit 0 k_flux,
- * 1 k_edge, 2 k_faces, 3 k_cells (or the fused launch).
+ * order), and the flux launch with one face per thread (three times the
+ * workgroups: for small blocks).  *mask_out (optional) gets the chosen
+ * mask: bit 0 k_flux, 1 k_edge, 2 k_faces, 3 k_cells (or the fused launch).
  * enzoe_vlct_launch_choice: the mask, the launch structure (0 = four
- * launches per stage, 1..5 as enzoe_vlct_set_fused_ct) and the tile rows
- * calls on blocks of this shape use.                                       */
+ * launches per stage, 1..5 as enzoe_vlct_set_fused_ct), the tile rows and
+ * the face split (0 or 1) calls on blocks of this shape use.               */
 int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                         double dt, enzoe_stream_t stream, int *mask_out);
 int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask,
-                             int *fused_ct, int *tile_rows);
+                             int *fused_ct, int *tile_rows, int *face_split);
 
 /* EnzoMethodMHDVlct::timestep + the clip of control_stopping.cpp:83-86
  * without a host round trip: reads the time from *d_time (device), writes
@@ -725,6 +726,13 @@ int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
  * the barrier is reset), ENZOE_OK otherwise.                               */
 int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int variant);
 int enzoe_vlct_check_persistent(enzoe_vlct_t m);
+
+/* The flux launch's work per thread (no reference counterpart): 0, the
+ * three low faces of a cell; 1, one face (three times the workgroups, for
+ * blocks too small to fill the device); -1 (the default), the autotuner's
+ * choice per block shape (three faces until it has run).  Every setting
+ * gives the same bits.                                                     */
+int enzoe_vlct_set_face_split(enzoe_vlct_t m, int split);
 
 #define ENZOE_DEBUG_NONFINITE 0
 #define ENZOE_DEBUG_DENSITY_FLOOR 1

@@ -55,6 +55,53 @@ def test_fused_ct_equals_three_launches(precision, riemann, variant):
     _equal(gf.blocks, gu.blocks)
 
 
+FACE_SPLIT_CASES = [
+    dict(mhd=True, riemann="hlld", recon=("nn", "plm"), theta=2.0, dual=False),
+    dict(mhd=True, riemann="hlle", recon=("nn", "plm_athena"), theta=1.5, dual=False),
+    dict(mhd=True, riemann="hlld", recon=("nn", "plm"), theta=2.0, dual=True),
+    dict(mhd=False, riemann="hllc", recon=("nn", "plm"), theta=2.0, dual=False),
+]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("case", FACE_SPLIT_CASES, ids=lambda c: "%s-%s-%s%s" % (
+    "mhd" if c["mhd"] else "hd", c["riemann"], c["recon"][1], "-de" if c["dual"] else ""))
+def test_face_split_flux_launch_equals_three_faces_per_thread(case, precision):
+    """The flux launch with one face per thread (the small-block launch
+    shape) against three faces per thread: every field and the next dt equal
+    bit for bit."""
+    if case["dual"] and precision:
+        pytest.skip("the dual-energy formalism runs in FP64 only")
+    ev = _ev()
+    grid = random_periodic_grid(mhd=case["mhd"], seed=53, n=(17, 12, 10), vamp=0.5,
+                                riemann=case["riemann"], recon=case["recon"],
+                                theta=case["theta"])
+    if case["dual"]:
+        grid.cfg.dual_energy = 1
+        grid.cfg.eta = 0.001
+        arr = grid.block_arrays(0)
+        v2 = sum(np.square(arr[k]) for k in ("velocity_x", "velocity_y", "velocity_z"))
+        b2 = sum(np.square(arr[k]) for k in ("bfield_x", "bfield_y", "bfield_z"))
+        arr["internal_energy"][...] = (arr["total_energy"] - 0.5 * v2 -
+                                       0.5 * b2 / arr["density"])
+    runs = []
+    for split in (1, 0):
+        m = method_for(grid.cfg, precision=precision)
+        m.set_face_split(split)
+        runs.append((m, device_copy(grid, m)))
+    (ma, ga), (mb, gb) = runs
+    assert ma.launch_choice((23, 18, 16))[3] == 1
+    dta, dtb = ma.timestep(ga.blocks), mb.timestep(gb.blocks)
+    for _ in range(3):
+        ga.refresh()
+        gb.refresh()
+        dta = ma.compute_timestep(ga.blocks, dta)
+        dtb = mb.compute_timestep(gb.blocks, dtb)
+        assert dta == dtb
+    ev.synchronize()
+    _equal(ga.blocks, gb.blocks)
+
+
 @pytest.mark.parametrize("dual", [False, True], ids=["plain", "dual_energy"])
 def test_fused_ct_two_blocks_per_block_dt(dual):
     ev = _ev()
@@ -98,6 +145,7 @@ def test_autotuned_launch_structure_keeps_results():
     assert choice[0] == mask
     assert choice[1] in (0, 1, 2, 3, 4)
     assert choice[2] >= 0
+    assert choice[3] in (0, 1)
     t = 0.0
     for _ in range(2):
         dt_o = grid.cycle(t, 10.0)
