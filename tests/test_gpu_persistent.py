@@ -225,3 +225,31 @@ def test_autotuner_may_choose_the_persistent_stage():
     m.check_persistent()
     from test_gpu_parity import compare_grids
     compare_grids(dg, grid)
+
+
+def test_persistent_stage_many_launches_in_a_row():
+    """40 persistent launches back to back (20 steps of 16 blocks of 26^3,
+    more workgroups per phase than the launch holds) equal the separate
+    launches bit for bit; the barrier's generation word carries across the
+    launches and no wait runs out."""
+    import problems
+    ev = _ev()
+    n, nb = 26, 16
+    arrays = [problems.random_mhd_block(1300 + i, n=n, width=1.0 / n)[0] for i in range(nb)]
+    dts = [7e-4] * nb
+    out = []
+    for v in (PERSISTENT, 0):
+        m = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm", theta=2.0,
+                          courant=0.4)
+        m.set_fused_ct(v)
+        blocks = []
+        for a in arrays:
+            b = ev.DeviceBlock(n + 6, n + 6, n + 6, 3, (1.0 / n,) * 3, mhd=True)
+            b.upload(a)
+            blocks.append(b)
+        for _ in range(20):
+            m.compute_dt(blocks, dts)
+        ev.synchronize()
+        m.check_persistent()
+        out.append((m, blocks))
+    _equal(out[0][1], out[1][1])
