@@ -60,9 +60,10 @@ def parse():
     # fp64_fma: FP64 with multiply-adds contracted (not bit-identical to the
     # reference; held to 1e-12 relative per cell); Orszag-Tang only
     ap.add_argument("--precision", choices=["fp64", "mixed", "fp64_fma"], default="fp64")
-    # launch structure of the stages for the fast-wave and batched-AMR
-    # workloads (enzoe_vlct_set_fused_ct: 0 separate launches, 1-4 fused CT,
-    # 5 one persistent launch per stage); -1: the autotuner's choice
+    # launch structure of the stages (enzoe_vlct_set_fused_ct: 0 separate
+    # launches, 1-4 fused CT, 5 one persistent launch per stage) for the
+    # Orszag-Tang block (its primary run), the fast-wave and batched-AMR
+    # workloads; -1: the autotuner's choice
     ap.add_argument("--structure", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
@@ -291,11 +292,14 @@ METRIC = "zone-updates/s/GPU (batched FP32-mixed vs plain FP64 VL+CT) + HBM GB/s
 PRECISIONS = {"fp64": 0, "mixed": 1, "fp64_fma": 2}  # ENZOE_PREC_*
 
 
-def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1, keep=False):
+def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1, keep=False,
+              structure=-1):
     """Warm-up, then `steps` timed steps with per-kernel HIP-event timing.
     dual_energy: the "modern" dual-energy formalism (eta 0.001) on.
     launch_mask >= 0: that launch order instead of the autotuner's.
-    keep: the result holds the grid ("grid", "method") after the run."""
+    keep: the result holds the grid ("grid", "method") after the run.
+    structure >= 0: that launch structure (enzoe_vlct_set_fused_ct) instead
+    of the autotuner's."""
     prec = PRECISIONS[precision]
     if launch_mask >= 0:
         os.environ["ENZOE_XCD_REMAP"] = str(launch_mask)
@@ -303,6 +307,8 @@ def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1, ke
                            theta=2.0, courant=0.4, gamma=5.0 / 3.0,
                            precision=prec,
                            dual_energy_eta=0.001 if dual_energy else None)
+    if structure >= 0:
+        method.set_fused_ct(structure)
     grid = make_state(n, method)
     dt = method.timestep(grid.blocks)  # the first cycle's CFL reduction
     for _ in range(warmup):
@@ -314,6 +320,7 @@ def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1, ke
         mask = launch_mask
     else:
         mask = method.autotune(grid.blocks, dt)
+    choice = method.launch_choice((n + 6,) * 3)
     ev.profiling(True)
     ev.synchronize()
     t0 = time.perf_counter()
@@ -341,7 +348,7 @@ def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1, ke
             "step_gbs": STEP_BYTES_PER_ZONE_FP64 * zones / (gpu_step_ms * 1e-3) / 1e9,
             "workload": "orszag_tang_%d3_%s%s_vlct_plm_hlld" % (
                 n, precision, "_dual_energy" if dual_energy else ""),
-            "launch_mask": mask, **kept}
+            "launch_mask": mask, "launch_choice": choice, **kept}
 
 
 def _roofline(prof, steps, cells_per_step, prec, mhd, workload):
@@ -830,7 +837,8 @@ def main():
         sys.exit(2)
     compare = not a.no_compare and a.precision != "fp64_fma"
     r = timed_run(a.precision, a.n, a.steps, a.warmup, dual_energy=a.dual_energy,
-                  launch_mask=a.launch_mask, keep=compare and a.precision == "fp64")
+                  launch_mask=a.launch_mask, keep=compare and a.precision == "fp64",
+                  structure=a.structure)
     other = dual = fma = None
     if compare:
         other = timed_run("mixed" if a.precision == "fp64" else "fp64", a.n,
@@ -864,6 +872,7 @@ def main():
                    "block": [a.n, a.n, a.n], "ghost": 3, "riemann": "hlld",
                    "reconstruct": "plm", "theta": 2.0, "courant": 0.4,
                    "precision": a.precision, "launch_mask": r["launch_mask"],
+                   "launch_choice": r["launch_choice"],
                    "dual_energy": a.dual_energy},
         # the step against the roofline as the metric defines it (440 B per
         # zone-update, fully fused); the dominant kernel's own figure beside it
