@@ -65,6 +65,10 @@ def parse():
     # Orszag-Tang block (its primary run), the fast-wave and batched-AMR
     # workloads; -1: the autotuner's choice
     ap.add_argument("--structure", type=int, default=-1)
+    # shape of the flux launch (enzoe_vlct_set_flux_shape: 0 one thread per
+    # cell, 1 one face per thread, 2 z pencils for the nearest-neighbour
+    # stage) for the Orszag-Tang run; -1: the autotuner's choice
+    ap.add_argument("--flux-shape", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the run at the other precision")
@@ -293,7 +297,7 @@ PRECISIONS = {"fp64": 0, "mixed": 1, "fp64_fma": 2}  # ENZOE_PREC_*
 
 
 def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1, keep=False,
-              structure=-1):
+              structure=-1, flux_shape=-1):
     """Warm-up, then `steps` timed steps with per-kernel HIP-event timing.
     dual_energy: the "modern" dual-energy formalism (eta 0.001) on.
     launch_mask >= 0: that launch order instead of the autotuner's.
@@ -309,6 +313,8 @@ def timed_run(precision, n, steps, warmup, dual_energy=False, launch_mask=-1, ke
                            dual_energy_eta=0.001 if dual_energy else None)
     if structure >= 0:
         method.set_fused_ct(structure)
+    if flux_shape >= 0:
+        method.set_flux_shape(flux_shape)
     grid = make_state(n, method)
     dt = method.timestep(grid.blocks)  # the first cycle's CFL reduction
     for _ in range(warmup):
@@ -838,7 +844,7 @@ def main():
     compare = not a.no_compare and a.precision != "fp64_fma"
     r = timed_run(a.precision, a.n, a.steps, a.warmup, dual_energy=a.dual_energy,
                   launch_mask=a.launch_mask, keep=compare and a.precision == "fp64",
-                  structure=a.structure)
+                  structure=a.structure, flux_shape=a.flux_shape)
     other = dual = fma = None
     if compare:
         other = timed_run("mixed" if a.precision == "fp64" else "fp64", a.n,

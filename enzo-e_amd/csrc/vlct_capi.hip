@@ -917,24 +917,27 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     m->force_ctu = -1;
     m->force_rows = -1;
   }
-  // then the flux launch with one face per thread (more workgroups for
-  // small blocks), for the chosen structure, order and rows
-  int best_split = 0;
-  if (best_ctu != 5 && m->face_split < 0 && rc == ENZOE_OK) {
+  // then the other shapes of the flux launch (one face per thread: more
+  // workgroups for small blocks; z pencils for the nearest-neighbour
+  // stage), for the chosen structure, order and rows
+  int best_shape = 0;
+  if (best_ctu != 5 && m->flux_shape < 0 && rc == ENZOE_OK) {
     m->force_mask = best;
     m->force_ctu = best_ctu;
     m->force_rows = best_rows;
-    m->force_split = 1;
-    float ms = 0.f;
-    rc = time_runs(&ms);
-    if (rc == ENZOE_OK && ms < best_ms) {
-      best_ms = ms;
-      best_split = 1;
+    for (int shape = 1; shape <= 2 && rc == ENZOE_OK; shape++) {
+      m->force_shape = shape;
+      float ms = 0.f;
+      rc = time_runs(&ms);
+      if (rc == ENZOE_OK && ms < best_ms) {
+        best_ms = ms;
+        best_shape = shape;
+      }
     }
     m->force_mask = -1;
     m->force_ctu = -1;
     m->force_rows = -1;
-    m->force_split = -1;
+    m->force_shape = -1;
   }
   g_profiling = prof;
   (void)hipStreamSynchronize(s);
@@ -945,15 +948,15 @@ int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
   m->tuned_mask[shape_key(mx, my, mz)] = best;
   m->tuned_ctu[shape_key(mx, my, mz)] = best_ctu;
   m->tuned_rows[shape_key(mx, my, mz)] = best_rows;
-  m->tuned_split[shape_key(mx, my, mz)] = best_split;
+  m->tuned_shape[shape_key(mx, my, mz)] = best_shape;
   if (mask_out) *mask_out = best;
   return ENZOE_OK;
 }
 
 int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask, int *fused_ct,
-                             int *tile_rows, int *face_split) {
+                             int *tile_rows, int *flux_shape) {
   if (!m) return ENZOE_ERR_ARG;
-  if (face_split) *face_split = face_split_of(m, mx, my, mz) ? 1 : 0;
+  if (flux_shape) *flux_shape = flux_shape_of(m, mx, my, mz);
   if (mask) *mask = launch_mask(m, mx, my, mz);
   if (tile_rows) *tile_rows = tile_rows_of(m, mx, my, mz);
   if (fused_ct) {
@@ -1046,9 +1049,9 @@ int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int variant) {
   return ENZOE_OK;
 }
 
-int enzoe_vlct_set_face_split(enzoe_vlct_t m, int split) {
-  if (!m || split < -1 || split > 1) return ENZOE_ERR_ARG;
-  m->face_split = split;
+int enzoe_vlct_set_flux_shape(enzoe_vlct_t m, int shape) {
+  if (!m || shape < -1 || shape > 2) return ENZOE_ERR_ARG;
+  m->flux_shape = shape;
   return ENZOE_OK;
 }
 

@@ -329,9 +329,8 @@ struct FluxConsts {
 // sides, the face-centred normal B (EnzoBfieldMethodCT.cpp:122-166) and the
 // Riemann solve, written at index c of the flux arrays of dim D.
 template <typename T, typename FluxT, int RECON, int RIEMANN, bool MHD, int D,
-          bool DE, bool PRIMF>
-__device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b,
-                                          const PrimSource<T, MHD, DE, PRIMF> &src,
+          bool DE, bool PRIMF, class Src = PrimSource<T, MHD, DE, PRIMF>>
+__device__ __forceinline__ void flux_face(const BlockDesc<FluxT> &b, const Src &src,
                                           const double *bfi, int k, int j,
                                           int i, int c, const T wc[S_N],
                                           const FluxConsts &fc) {
@@ -517,6 +516,129 @@ k_flux(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp) {
   // a launch with three z slices is the face-split one
   flux_cells<T, FluxT, RECON, RIEMANN, MHD, DE, PRIMF>(blocks, sp, bx, by,
                                                       gridDim.z == 3 ? (int)blockIdx.z : -1);
+}
+
+// The primitive state of the neighbour cell a face needs, already at hand
+// (the z-pencil kernel's LDS plane or its next-plane load).
+template <typename T>
+struct GivenPrim {
+  const T *w;
+  __device__ __forceinline__ void operator()(int, T out[S_N]) const {
+#pragma unroll
+    for (int q = 0; q < S_N; q++) out[q] = w[q];
+  }
+};
+
+// Tile of the z-pencil flux kernel: 16 x 16 cells of a plane per
+// workgroup (a wave takes 4 rows of 16: four 128-byte lines per FP64 load,
+// as a 64-cell row would; square, so that the +x column and +y row past it
+// add only 1/8 and partial tiles at the box edges stay small).
+constexpr int ZP_TX = 16, ZP_TY = 16;
+
+// The nearest-neighbour flux stage as z pencils: a workgroup takes a 16 x 16
+// tile of (x, y) columns of the flux box and marches up through a segment
+// of its planes.  Per plane the tile's primitive states (and the +x column
+// and +y row past it) sit in LDS, so a cell's x- and y-face neighbours are
+// read from there and its z-face neighbour is the thread's own load of the
+// next plane, which is its own state at the next step of the march: every
+// cell's state is loaded and converted to primitives about 1.1 times
+// instead of 4 times (k_flux).  The faces and their arithmetic are
+// k_flux's (flux_face, the same neighbour values), so the same bits.
+// Launch: grid.x = tiles_x * tiles_y * segments, grid.y = block.
+template <typename T, typename FluxT, int RIEMANN, bool MHD, bool DE, bool PRIMF>
+__global__ void __launch_bounds__(256, ENZOE_FLUX_NN_WAVES)
+k_flux_zp(const BlockDesc<FluxT> *__restrict__ blocks, StageParams sp, int zplanes) {
+  const int by = blockIdx.y;
+  if (zero_step(sp, by)) return;
+  const BlockDesc<FluxT> &b = blocks[by];
+  const int mx = b.mx, my = b.my, mz = b.mz;
+  const int cs = sp.cur_stale;
+  const int nx = mx - 2 * cs, ny = my - 2 * cs, nz = mz - 2 * cs;
+  const int ntx = (nx + ZP_TX - 1) / ZP_TX, nty = (ny + ZP_TY - 1) / ZP_TY;
+  const int tile = (int)blockIdx.x % (ntx * nty), seg = (int)blockIdx.x / (ntx * nty);
+  const int tx = (int)threadIdx.x % ZP_TX, ty = (int)threadIdx.x / ZP_TX;
+  const int i = cs + (tile % ntx) * ZP_TX + tx, j = cs + (tile / ntx) * ZP_TY + ty;
+  const int k0 = cs + seg * zplanes;
+  const int k1 = (k0 + zplanes < mz - cs) ? k0 + zplanes : mz - cs;
+  const bool mine = i < mx - cs && j < my - cs;  // a cell of the flux box
+  double *const *U = (sp.stage == 0) ? b.U : b.Ut;
+  const double *const *bfr = (sp.stage == 0) ? b.bfi : b.bft;
+  FluxConsts fc;
+  fc.gm1 = sp.gamma - 1.0;
+  fc.gamma = sp.gamma;
+  fc.theta = sp.theta;
+  fc.dfloor = sp.density_floor;
+  fc.pfloor = sp.pressure_floor;
+  fc.dbg = sp.dbg;
+  const double *ei = DE ? ((sp.stage == 0) ? b.ei : b.eit) : nullptr;
+  const PrimSource<T, MHD, DE, PRIMF> src{U, ei, b.Up, fc.gm1};
+  // the halo cell this thread loads per plane: the +y row past the tile
+  // (threads 0 .. ZP_TX-1) or the +x column past it (the next ZP_TY)
+  const int t = (int)threadIdx.x;
+  const int hi_ = (t < ZP_TX) ? cs + (tile % ntx) * ZP_TX + t : cs + (tile % ntx) * ZP_TX + ZP_TX;
+  const int hj_ = (t < ZP_TX) ? cs + (tile / ntx) * ZP_TY + ZP_TY
+                              : cs + (tile / ntx) * ZP_TY + (t - ZP_TX);
+  const bool halo = t < ZP_TX + ZP_TY && hi_ < mx - cs && hj_ < my - cs;
+  // a cell's state as primitives, FP64 (kept for the centre E) and as T,
+  // exactly as k_flux forms them
+  auto load = [&](int cc, double d[S_N], T w[S_N]) {
+    if constexpr (PRIMF) {
+      src(cc, w);
+    } else {
+      load_prim<MHD, DE>(U, cc, fc.gm1, d, ei);
+#pragma unroll
+      for (int q = 0; q < S_N; q++) w[q] = (T)d[q];
+    }
+  };
+  // LDS: the tile's plane with the +x column and +y row
+  __shared__ T sw[ZP_TY + 1][ZP_TX + 1][S_N];
+  T wc[S_N];
+  double wd[S_N];
+  if (mine && k0 < k1) load((k0 * my + j) * mx + i, wd, wc);
+  for (int k = k0; k < k1; k++) {
+    const int c = (k * my + j) * mx + i;
+    if (mine) {
+#pragma unroll
+      for (int q = 0; q < S_N; q++) sw[ty][tx][q] = wc[q];
+      if (MHD && !PRIMF) {
+        // centre E of this cell for the edge kernel (as k_flux)
+        gst(b.Ec[0], c, center_efield(wd[S_VY], wd[S_VZ], wd[S_BY], wd[S_BZ]));
+        gst(b.Ec[1], c, center_efield(wd[S_VZ], wd[S_VX], wd[S_BZ], wd[S_BX]));
+        gst(b.Ec[2], c, center_efield(wd[S_VX], wd[S_VY], wd[S_BX], wd[S_BY]));
+      }
+    }
+    if (halo) {
+      T h[S_N];
+      double hd[S_N];
+      load((k * my + hj_) * mx + hi_, hd, h);
+      const int ly = (t < ZP_TX) ? ZP_TY : t - ZP_TX;
+      const int lx = (t < ZP_TX) ? t : ZP_TX;
+#pragma unroll
+      for (int q = 0; q < S_N; q++) sw[ly][lx][q] = h[q];
+    }
+    __syncthreads();
+    if (mine) {
+      if (i < mx - 1 - cs)
+        flux_face<T, FluxT, RC_NN, RIEMANN, MHD, 0, DE, PRIMF>(
+            b, GivenPrim<T>{sw[ty][tx + 1]}, bfr[0], k, j, i, c, wc, fc);
+      if (j < my - 1 - cs)
+        flux_face<T, FluxT, RC_NN, RIEMANN, MHD, 1, DE, PRIMF>(
+            b, GivenPrim<T>{sw[ty + 1][tx]}, bfr[1], k, j, i, c, wc, fc);
+      if (k < mz - 1 - cs) {
+        T up[S_N];
+        double upd[S_N];
+        load(c + mx * my, upd, up);
+        flux_face<T, FluxT, RC_NN, RIEMANN, MHD, 2, DE, PRIMF>(
+            b, GivenPrim<T>{up}, bfr[2], k, j, i, c, wc, fc);
+#pragma unroll
+        for (int q = 0; q < S_N; q++) {
+          wc[q] = up[q];
+          wd[q] = upd[q];
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------- k_edge

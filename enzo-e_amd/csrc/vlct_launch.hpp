@@ -93,12 +93,13 @@ struct enzoe_vlct_method {
   // tile_rows_for otherwise) and the value forced while it times one
   std::map<long long, int> tuned_rows;
   int force_rows = -1;
-  // the flux launch with one face per thread (three workgroups per cell
-  // range) for blocks of this shape: the autotuner's choice, and the value
-  // forced while it times one
-  std::map<long long, int> tuned_split;
-  int force_split = -1;
-  int face_split = -1;  // the method's setting: 0, 1, or -1 (the autotuner's)
+  // the shape of the flux launch for blocks of this shape (flux_shape_of:
+  // 0 three faces per thread, 1 one face per thread, 2 z pencils for the
+  // nearest-neighbour stage): the autotuner's choice, the value forced
+  // while it times one, and the method's setting (-1: the autotuner's)
+  std::map<long long, int> tuned_shape;
+  int force_shape = -1;
+  int flux_shape = -1;
   // the persistent stage kernel's grid barrier (a GridBarrier in device
   // memory, zeroed once), or null until first needed
   void *d_bar = nullptr;
@@ -369,13 +370,16 @@ inline int tile_rows_for(int mx, int my) {
 }
 
 
-// Whether the flux launch for blocks of this shape puts one face per
-// thread (the autotuner's choice; three faces per thread otherwise).
-inline bool face_split_of(const enzoe_vlct_method *m, int mx, int my, int mz) {
-  if (m->force_split >= 0) return m->force_split != 0;
-  if (m->face_split >= 0) return m->face_split != 0;
-  auto it = m->tuned_split.find(shape_key(mx, my, mz));
-  return it != m->tuned_split.end() && it->second != 0;
+// The flux launch for blocks of this shape: 0, three faces per thread
+// (k_flux); 1, one face per thread (k_flux over three z slices, for blocks
+// too small to fill the device); 2, the nearest-neighbour stage as z
+// pencils with LDS-staged planes (k_flux_zp; the other stages as 0).  The
+// setting, else the autotuner's choice, else 0.
+inline int flux_shape_of(const enzoe_vlct_method *m, int mx, int my, int mz) {
+  if (m->force_shape >= 0) return m->force_shape;
+  if (m->flux_shape >= 0) return m->flux_shape;
+  auto it = m->tuned_shape.find(shape_key(mx, my, mz));
+  return it != m->tuned_shape.end() ? it->second : 0;
 }
 
 // Rows of the tiled cell order for blocks of this shape.
@@ -479,49 +483,72 @@ void fill_desc(const enzoe_vlct_method *m, const enzoe_block *blocks, int b0,
   fill_desc_at<FluxT>(m, (char *)m->scratch, blocks, b0, b1, out.data(), accel);
 }
 
+// cells of the box [lo, m - hi_trim) of a block
+inline long cube(int mx, int my, int mz, int lo, int hi_trim) {
+  long a = mx - lo - hi_trim, b = my - lo - hi_trim, c = mz - lo - hi_trim;
+  if (a <= 0 || b <= 0 || c <= 0) return 0;
+  return a * b * c;
+}
+
+// flux launch of one stage; shape as flux_shape_of (2 applies to the
+// nearest-neighbour reconstruction only)
 template <typename FluxT, int RECON, int RIEMANN, bool MHD, bool DE, bool PRIMF>
-void launch_flux(const BlockDesc<FluxT> *d, int nb, long nthreads,
-                 StageParams sp, hipStream_t s, bool split) {
-  dim3 grid(nblk8(nthreads, 256), nb, split ? 3 : 1);
+void launch_flux(const BlockDesc<FluxT> *d, int nb, int mx, int my, int mz,
+                 StageParams sp, hipStream_t s, int shape) {
+  const int cs = sp.cur_stale;
+  if (RECON == RC_NN && shape == 2) {
+    // z pencils: ZP_TX x ZP_TY tiles, each marching through a segment of
+    // planes; about 2048 workgroups per block
+    const int nx = mx - 2 * cs, ny = my - 2 * cs, nz = mz - 2 * cs;
+    const int tiles = ((nx + ZP_TX - 1) / ZP_TX) * ((ny + ZP_TY - 1) / ZP_TY);
+    const int want = std::max(1, std::min(nz, (2048 + tiles - 1) / tiles));
+    const int zplanes = (nz + want - 1) / want;
+    const int nseg = (nz + zplanes - 1) / zplanes;
+    hipLaunchKernelGGL((k_flux_zp<FluxT, FluxT, RIEMANN, MHD, DE, PRIMF>),
+                       dim3(tiles * nseg, nb), dim3(256), 0, s, d, sp, zplanes);
+    return;
+  }
+  const long nthreads = cube(mx, my, mz, cs, cs);
+  dim3 grid(nblk8(nthreads, 256), nb, shape == 1 ? 3 : 1);
   hipLaunchKernelGGL((k_flux<FluxT, FluxT, RECON, RIEMANN, MHD, DE, PRIMF>), grid,
                      dim3(256), 0, s, d, sp);
 }
 
 template <typename FluxT, bool MHD, bool DE, bool PRIMF>
 void dispatch_flux_p(const enzoe_vlct_method *m, int recon,
-                     const BlockDesc<FluxT> *d, int nb, long nthreads,
-                     StageParams sp, hipStream_t s, bool split) {
+                     const BlockDesc<FluxT> *d, int nb, int mx, int my, int mz,
+                     StageParams sp, hipStream_t s, int shape) {
   constexpr int RS = MHD ? RS_HLLD : RS_HLLC;
   if (MHD && m->riemann != RS_HLLD) {
     if (recon == RC_NN)
-      launch_flux<FluxT, RC_NN, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+      launch_flux<FluxT, RC_NN, RS_HLLE, MHD, DE, PRIMF>(d, nb, mx, my, mz, sp, s, shape);
     else if (recon == RC_PLM)
-      launch_flux<FluxT, RC_PLM, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+      launch_flux<FluxT, RC_PLM, RS_HLLE, MHD, DE, PRIMF>(d, nb, mx, my, mz, sp, s, shape);
     else
-      launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+      launch_flux<FluxT, RC_PLM_ATHENA, RS_HLLE, MHD, DE, PRIMF>(d, nb, mx, my, mz, sp, s, shape);
     return;
   }
   if (recon == RC_NN)
-    launch_flux<FluxT, RC_NN, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+    launch_flux<FluxT, RC_NN, RS, MHD, DE, PRIMF>(d, nb, mx, my, mz, sp, s, shape);
   else if (recon == RC_PLM)
-    launch_flux<FluxT, RC_PLM, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+    launch_flux<FluxT, RC_PLM, RS, MHD, DE, PRIMF>(d, nb, mx, my, mz, sp, s, shape);
   else
-    launch_flux<FluxT, RC_PLM_ATHENA, RS, MHD, DE, PRIMF>(d, nb, nthreads, sp, s, split);
+    launch_flux<FluxT, RC_PLM_ATHENA, RS, MHD, DE, PRIMF>(d, nb, mx, my, mz, sp, s, shape);
 }
 
 // primf: the stage reads the mixed path's FP32 primitive half-step state;
-// split: one face per thread (face_split_of)
+// shape: flux_shape_of
 template <typename FluxT, bool MHD, bool DE>
 void dispatch_flux(const enzoe_vlct_method *m, int recon, bool primf,
-                   const BlockDesc<FluxT> *d, int nb, long nthreads,
-                   StageParams sp, hipStream_t s, bool split) {
+                   const BlockDesc<FluxT> *d, int nb, int mx, int my, int mz,
+                   StageParams sp, hipStream_t s, int shape) {
   if constexpr (half_f32<FluxT>(DE ? 1 : 0)) {
     if (primf) {
-      dispatch_flux_p<FluxT, MHD, DE, true>(m, recon, d, nb, nthreads, sp, s, split);
+      dispatch_flux_p<FluxT, MHD, DE, true>(m, recon, d, nb, mx, my, mz, sp, s, shape);
       return;
     }
   }
-  dispatch_flux_p<FluxT, MHD, DE, false>(m, recon, d, nb, nthreads, sp, s, split);
+  dispatch_flux_p<FluxT, MHD, DE, false>(m, recon, d, nb, mx, my, mz, sp, s, shape);
 }
 
 // gamma (gamma - 1) as the reference's synchronisation holds it: a float
@@ -536,12 +563,6 @@ inline int launch_mask(const enzoe_vlct_method *m, int mx, int my, int mz) {
   if (m->force_mask >= 0) return m->force_mask;
   auto it = m->tuned_mask.find(shape_key(mx, my, mz));
   return it != m->tuned_mask.end() ? it->second : m->xcd_remap;
-}
-
-inline long cube(int mx, int my, int mz, int lo, int hi_trim) {
-  long a = mx - lo - hi_trim, b = my - lo - hi_trim, c = mz - lo - hi_trim;
-  if (a <= 0 || b <= 0 || c <= 0) return 0;
-  return a * b * c;
 }
 
 // Saving the final stage's face fluxes for flux corrections: the fields and
@@ -620,8 +641,8 @@ int launch_chunks(enzoe_vlct_method *m, const enzoe_block *blocks, int n, int ch
       }
       {
         KernelTimer kt(s, recon == RC_NN ? "k_flux_nn" : "k_flux_plm", timers);
-        dispatch_flux<FluxT, MHD, DE>(m, recon, primf, d, nb, cube(mx, my, mz, cs, cs), sp, s,
-                                      face_split_of(m, mx, my, mz));
+        dispatch_flux<FluxT, MHD, DE>(m, recon, primf, d, nb, mx, my, mz, sp, s,
+                                      flux_shape_of(m, mx, my, mz));
       }
       const int variant =
           (MHD && ctu_possible(m) && structure >= 1 && structure <= 4) ? structure : 0;

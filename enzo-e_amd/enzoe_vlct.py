@@ -171,7 +171,7 @@ def lib():
         L.enzoe_vlct_set_debug_checks.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_set_fused_ct.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_check_persistent.argtypes = [vp]
-        L.enzoe_vlct_set_face_split.argtypes = [vp, ctypes.c_int]
+        L.enzoe_vlct_set_flux_shape.argtypes = [vp, ctypes.c_int]
         L.enzoe_vlct_run_cycles.argtypes = [vp, ctypes.POINTER(Block), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int), vp, ctypes.c_double,
@@ -396,11 +396,12 @@ class VlctMethod:
         v = int(variant) if not isinstance(variant, bool) else (1 if variant else 0)
         check(lib().enzoe_vlct_set_fused_ct(self.handle, v), "enzoe_vlct_set_fused_ct")
 
-    def set_face_split(self, split):
-        """Force the flux launch with one face per thread (1), three (0), or
-        leave it to the autotuner (-1).  Every setting gives the same bits."""
-        check(lib().enzoe_vlct_set_face_split(self.handle, int(split)),
-              "enzoe_vlct_set_face_split")
+    def set_flux_shape(self, shape):
+        """Shape of the flux launch: 0 three faces per thread, 1 one face per
+        thread, 2 z pencils for the nearest-neighbour stage, -1 the
+        autotuner's choice.  Every shape gives the same bits."""
+        check(lib().enzoe_vlct_set_flux_shape(self.handle, int(shape)),
+              "enzoe_vlct_set_flux_shape")
 
     def check_persistent(self):
         """Raises if a barrier wait of a persistent stage launch ran out
@@ -408,8 +409,8 @@ class VlctMethod:
         check(lib().enzoe_vlct_check_persistent(self.handle), "enzoe_vlct_check_persistent")
 
     def launch_choice(self, shape):
-        """(XCD mask, launch structure, tile rows, face split) used for blocks
-        of shape (mx, my, mz)."""
+        """(XCD mask, launch structure, tile rows, flux-launch shape) used for
+        blocks of shape (mx, my, mz)."""
         mask, ctu, rows, split = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().enzoe_vlct_launch_choice(self.handle, shape[0], shape[1], shape[2],
                                              ctypes.byref(mask), ctypes.byref(ctu),

@@ -175,16 +175,16 @@ int enzoe_vlct_compute_dt(enzoe_vlct_t m, const enzoe_block *blocks, int n,
  * orders of its two kernels, and (HLLD MHD or hydro, no dual energy) the
  * persistent launch of a whole stage; then, for the chosen one, the rows of
  * the tiled cell order (slabs of 8, 16, 32 or 64 rows, or the plain
- * order), and the flux launch with one face per thread (three times the
- * workgroups: for small blocks).  *mask_out (optional) gets the chosen
- * mask: bit 0 k_flux, 1 k_edge, 2 k_faces, 3 k_cells (or the fused launch).
- * enzoe_vlct_launch_choice: the mask, the launch structure (0 = four
- * launches per stage, 1..5 as enzoe_vlct_set_fused_ct), the tile rows and
- * the face split (0 or 1) calls on blocks of this shape use.               */
+ * order), and the other shapes of the flux launch (enzoe_vlct_set_flux_shape).
+ * *mask_out (optional) gets the chosen mask: bit 0 k_flux, 1 k_edge,
+ * 2 k_faces, 3 k_cells (or the fused launch).  enzoe_vlct_launch_choice:
+ * the mask, the launch structure (0 = four launches per stage, 1..5 as
+ * enzoe_vlct_set_fused_ct), the tile rows and the flux-launch shape (0..2)
+ * calls on blocks of this shape use.                                       */
 int enzoe_vlct_autotune(enzoe_vlct_t m, const enzoe_block *blocks, int n,
                         double dt, enzoe_stream_t stream, int *mask_out);
 int enzoe_vlct_launch_choice(enzoe_vlct_t m, int mx, int my, int mz, int *mask,
-                             int *fused_ct, int *tile_rows, int *face_split);
+                             int *fused_ct, int *tile_rows, int *flux_shape);
 
 /* EnzoMethodMHDVlct::timestep + the clip of control_stopping.cpp:83-86
  * without a host round trip: reads the time from *d_time (device), writes
@@ -718,8 +718,10 @@ int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
  * hydro without the dual-energy formalism; elsewhere, and on the
  * allocation-free entry point, the four launches); -1 (the default): the
  * autotuner's choice per block shape (the four launches until it has run).
- * Every variant gives the same bits.  Not to be changed while work of the
- * method is queued.
+ * Every variant gives the same bits in FP64 (the mixed and contracted
+ * builds: up to the rounding of their contracted flux arithmetic, which the
+ * compiler may fuse differently in different kernels).  Not to be changed
+ * while work of the method is queued.
  * enzoe_vlct_check_persistent: synchronises and reports ENZOE_ERR_HIP if a
  * barrier wait of a persistent launch ran out (the workgroups of a launch
  * could not all be resident: results since the last check are not valid;
@@ -727,12 +729,18 @@ int enzoe_vlct_transfer_batch(enzoe_vlct_t m, const enzoe_transfer *items,
 int enzoe_vlct_set_fused_ct(enzoe_vlct_t m, int variant);
 int enzoe_vlct_check_persistent(enzoe_vlct_t m);
 
-/* The flux launch's work per thread (no reference counterpart): 0, the
- * three low faces of a cell; 1, one face (three times the workgroups, for
- * blocks too small to fill the device); -1 (the default), the autotuner's
- * choice per block shape (three faces until it has run).  Every setting
- * gives the same bits.                                                     */
-int enzoe_vlct_set_face_split(enzoe_vlct_t m, int split);
+/* The shape of the flux launch (no reference counterpart): 0, one thread
+ * per cell solving its three faces; 1, one face per thread (three times
+ * the workgroups, for blocks too small to fill the device); 2, the
+ * nearest-neighbour stage as z pencils: 16 x 16 tiles of columns march
+ * through the planes with each plane's primitive states staged in LDS, so
+ * a cell's state is loaded and converted about 1.1 times instead of 4
+ * (the reconstructing stage as 0); -1 (the default), the autotuner's
+ * choice per block shape (0 until it has run).  Every shape gives the same
+ * bits in FP64; the mixed and contracted builds fuse their flux
+ * multiply-adds as the compiler finds them in each kernel, so there the
+ * shapes agree to the rounding of that arithmetic.                         */
+int enzoe_vlct_set_flux_shape(enzoe_vlct_t m, int shape);
 
 #define ENZOE_DEBUG_NONFINITE 0
 #define ENZOE_DEBUG_DENSITY_FLOOR 1

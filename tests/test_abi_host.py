@@ -239,7 +239,7 @@ def test_fused_entry_points_reject_bad_arguments_without_a_device():
 
 
 def test_launch_settings_and_device_cycles_reject_bad_arguments_without_a_device():
-    """The launch-structure, face-split and device-cycle entry points check
+    """The launch-structure, flux-shape and device-cycle entry points check
     their arguments before any device call; a method that never launched a
     persistent stage has nothing to report."""
     L = ev.lib()
@@ -250,19 +250,19 @@ def test_launch_settings_and_device_cycles_reject_bad_arguments_without_a_device
     for v in (-2, 6):
         assert L.enzoe_vlct_set_fused_ct(h, v) == ERR_ARG
     assert L.enzoe_vlct_set_fused_ct(None, 0) == ERR_ARG
-    for v in (-1, 0, 1):
-        assert L.enzoe_vlct_set_face_split(h, v) == 0
-    for v in (-2, 2):
-        assert L.enzoe_vlct_set_face_split(h, v) == ERR_ARG
-    assert L.enzoe_vlct_set_face_split(None, 0) == ERR_ARG
+    for v in (-1, 0, 1, 2):
+        assert L.enzoe_vlct_set_flux_shape(h, v) == 0
+    for v in (-2, 3):
+        assert L.enzoe_vlct_set_flux_shape(h, v) == ERR_ARG
+    assert L.enzoe_vlct_set_flux_shape(None, 0) == ERR_ARG
     assert L.enzoe_vlct_check_persistent(h) == 0
     assert L.enzoe_vlct_check_persistent(None) == ERR_ARG
     # the launch choice of a shape the autotuner has not seen: the settings
     L.enzoe_vlct_set_fused_ct(h, 5)
-    L.enzoe_vlct_set_face_split(h, 1)
+    L.enzoe_vlct_set_flux_shape(h, 2)
     out = [ctypes.c_int(-9) for _ in range(4)]
     assert L.enzoe_vlct_launch_choice(h, 20, 20, 20, *[ctypes.byref(o) for o in out]) == 0
-    assert out[1].value == 5 and out[3].value == 1
+    assert out[1].value == 5 and out[3].value == 2
     assert L.enzoe_vlct_launch_choice(None, 20, 20, 20, None, None, None, None) == ERR_ARG
     empty = (ev.Block * 1)()
     bcs = (ctypes.c_int * 3)(0, 0, 0)

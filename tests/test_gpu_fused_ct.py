@@ -59,21 +59,28 @@ FACE_SPLIT_CASES = [
     dict(mhd=True, riemann="hlld", recon=("nn", "plm"), theta=2.0, dual=False),
     dict(mhd=True, riemann="hlle", recon=("nn", "plm_athena"), theta=1.5, dual=False),
     dict(mhd=True, riemann="hlld", recon=("nn", "plm"), theta=2.0, dual=True),
+    dict(mhd=True, riemann="hlld", recon=("nn", "nn"), theta=2.0, dual=False),
     dict(mhd=False, riemann="hllc", recon=("nn", "plm"), theta=2.0, dual=False),
 ]
 
 
+@pytest.mark.parametrize("shape", [1, 2], ids=["face_split", "z_pencils"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("case", FACE_SPLIT_CASES, ids=lambda c: "%s-%s-%s%s" % (
     "mhd" if c["mhd"] else "hd", c["riemann"], c["recon"][1], "-de" if c["dual"] else ""))
-def test_face_split_flux_launch_equals_three_faces_per_thread(case, precision):
-    """The flux launch with one face per thread (the small-block launch
-    shape) against three faces per thread: every field and the next dt equal
-    bit for bit."""
+def test_flux_launch_shapes_equal_three_faces_per_thread(case, precision, shape):
+    """The flux launch with one face per thread (the small-block shape) and
+    the nearest-neighbour stage as z pencils with LDS-staged planes, each
+    against one thread per cell with its three faces: every field and the
+    next dt equal bit for bit in FP64.  The mixed path contracts its FP32
+    flux arithmetic, which the compiler may fuse differently in the pencil
+    kernel: there the shapes agree to 1e-6 in relative L1 (the mixed bar
+    against FP64 is 1e-5) and the dt to 1e-6."""
+    bitwise = precision == 0 or shape == 1
     if case["dual"] and precision:
         pytest.skip("the dual-energy formalism runs in FP64 only")
     ev = _ev()
-    grid = random_periodic_grid(mhd=case["mhd"], seed=53, n=(17, 12, 10), vamp=0.5,
+    grid = random_periodic_grid(mhd=case["mhd"], seed=53, n=(71, 12, 10), vamp=0.5,
                                 riemann=case["riemann"], recon=case["recon"],
                                 theta=case["theta"])
     if case["dual"]:
@@ -85,21 +92,61 @@ def test_face_split_flux_launch_equals_three_faces_per_thread(case, precision):
         arr["internal_energy"][...] = (arr["total_energy"] - 0.5 * v2 -
                                        0.5 * b2 / arr["density"])
     runs = []
-    for split in (1, 0):
+    for sh in (shape, 0):
         m = method_for(grid.cfg, precision=precision)
-        m.set_face_split(split)
+        m.set_flux_shape(sh)
         runs.append((m, device_copy(grid, m)))
     (ma, ga), (mb, gb) = runs
-    assert ma.launch_choice((23, 18, 16))[3] == 1
+    assert ma.launch_choice((77, 18, 16))[3] == shape
     dta, dtb = ma.timestep(ga.blocks), mb.timestep(gb.blocks)
     for _ in range(3):
         ga.refresh()
         gb.refresh()
         dta = ma.compute_timestep(ga.blocks, dta)
         dtb = mb.compute_timestep(gb.blocks, dtb)
-        assert dta == dtb
+        if bitwise:
+            assert dta == dtb
+        else:
+            assert dta == pytest.approx(dtb, rel=1e-6, abs=0)
     ev.synchronize()
-    _equal(ga.blocks, gb.blocks)
+    if bitwise:
+        _equal(ga.blocks, gb.blocks)
+    else:
+        _, mag = ma.state_l1(ga.blocks, gb.blocks)
+        print("relative L1 by vector magnitude", mag)
+        assert max(mag.values()) <= 1e-6, mag
+
+
+def test_z_pencils_on_a_batch_of_large_blocks():
+    """The z-pencil launch over two blocks of a shape with several tiles and
+    plane segments along every axis (partial tiles included), per-block dt:
+    equal to one thread per cell, bit for bit in FP64, to 1e-6 in relative
+    L1 in the mixed path."""
+    import problems
+    ev = _ev()
+    n = 70
+    arrays = [problems.random_mhd_block(400 + i, n=n, width=1.0 / n)[0] for i in range(2)]
+    for precision in (0, 1):
+        out = []
+        for sh in (2, 0):
+            m = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm", theta=2.0,
+                              courant=0.4, precision=precision)
+            m.set_flux_shape(sh)
+            blocks = []
+            for a in arrays:
+                b = ev.DeviceBlock(n + 6, n + 6, n + 6, 3, (1.0 / n,) * 3, mhd=True)
+                b.upload(a)
+                blocks.append(b)
+            for _ in range(2):
+                m.compute_dt(blocks, [6e-4, 5e-4])
+            ev.synchronize()
+            out.append((m, blocks))
+        if precision == 0:
+            _equal(out[0][1], out[1][1])
+        else:
+            _, mag = out[0][0].state_l1(out[0][1], out[1][1])
+            print("relative L1 by vector magnitude", mag)
+            assert max(mag.values()) <= 1e-6, mag
 
 
 @pytest.mark.parametrize("dual", [False, True], ids=["plain", "dual_energy"])
@@ -145,7 +192,7 @@ def test_autotuned_launch_structure_keeps_results():
     assert choice[0] == mask
     assert choice[1] in (0, 1, 2, 3, 4)
     assert choice[2] >= 0
-    assert choice[3] in (0, 1)
+    assert choice[3] in (0, 1, 2)
     t = 0.0
     for _ in range(2):
         dt_o = grid.cycle(t, 10.0)

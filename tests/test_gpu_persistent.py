@@ -3,8 +3,9 @@ face and cell phases of a stage in one launch with a device-wide barrier
 between them) against the separate launches on the GPU, bit for bit: every
 field, the pressure field and the next dt, for HLLD MHD (PLM, PLM Athena,
 nearest neighbour) and HLLC hydro, FP64, mixed and contracted FP64 (the
-last within 1e-12 per active cell of the separate launches: the compiler
-may fuse its contracted flux arithmetic differently in the two kernels); a
+last two contract their flux arithmetic, which the compiler may fuse
+differently in the two kernels: mixed within 1e-6 in relative L1, the
+contracted FP64 within 1e-12 per active cell); a
 batch whose phases hold more workgroups than the launch (each workgroup
 takes several), with per-block dt and a zero-length block; a captured
 graph of cycles; the sub-cycled levels driven on the device and replayed
@@ -87,6 +88,8 @@ def test_persistent_stage_equals_separate_launches(case, precision):
         dts = ms.compute_timestep(gs.blocks, dts)
         if precision == 2:
             assert dtp == pytest.approx(dts, rel=1e-13, abs=0)
+        elif precision == 1:
+            assert dtp == pytest.approx(dts, rel=1e-6, abs=0)
         else:
             assert dtp == dts
     ev.synchronize()
@@ -94,6 +97,10 @@ def test_persistent_stage_equals_separate_launches(case, precision):
     if precision == 2:
         print("largest difference (relative to the field's scale)",
               _close(gp.blocks, gs.blocks, 1e-12))
+    elif precision == 1:
+        _, mag = mp.state_l1(gp.blocks, gs.blocks)
+        print("relative L1 by vector magnitude", mag)
+        assert max(mag.values()) <= 1e-6, mag
     else:
         _equal(gp.blocks, gs.blocks)
 
@@ -125,7 +132,11 @@ def test_persistent_stage_batch_larger_than_the_launch(precision):
         if v == PERSISTENT:
             m.check_persistent()
         out.append((m, blocks))
-    _equal(out[0][1], out[1][1])
+    if precision == 0:
+        _equal(out[0][1], out[1][1])
+    else:
+        _, mag = out[0][0].state_l1(out[0][1], out[1][1])
+        assert max(mag.values()) <= 1e-6, mag
     # the zero-length block is unchanged
     got = out[0][1][5].download()
     for k, v in arrays[5].items():
