@@ -767,9 +767,8 @@ int enzoe_vlct_timestep_device(enzoe_vlct_t m, const enzoe_block *blocks, int n,
     return ENZOE_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   if (int rc = launch_dt(m, blocks, n, s, d_time, t_stop)) return rc;
-  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s,
-                     (const unsigned long long *)m->d_dt, m->courant, d_time, t_stop,
-                     d_dts, ndts);
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s, m->d_dt, m->courant, d_time,
+                     t_stop, d_dts, ndts, 0);
   return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
 }
 
@@ -1549,14 +1548,19 @@ int enzoe_vlct_run_cycles(enzoe_vlct_t m, const enzoe_block *blocks, int nbx, in
     return ENZOE_ERR_ARG;
   const int n = nbx * nby * nbz;
   hipStream_t s = (hipStream_t)stream;
-  // the launches of a captured cycle (enzoe_vlct_graph_create), enqueued
+  // the first cycle's CFL minimum from the state as it is; then per cycle
+  // its dt (clipped at t_stop, the time advanced, the minimum word reset),
+  // the refresh, and the compute whose final cell update takes the next
+  // cycle's minimum (the dt a CFL pass over its result would give)
+  if (int rc = launch_dt(m, blocks, n, s, d_state, t_stop)) return rc;
   for (int c = 0; c < ncycles; c++) {
-    if (int rc = enzoe_vlct_timestep_device(m, blocks, n, d_state, t_stop, d_state + 1, n, s))
-      return rc;
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s, m->d_dt, m->courant, d_state,
+                       t_stop, d_state + 1, n, 1);
     if (int rc = grid_refresh(m, blocks, nbx, nby, nbz, bc, s, d_state + 1)) return rc;
-    if (int rc = compute_runs(m, blocks, n, 0.0, d_state + 1, s)) return rc;
+    if (int rc = compute_runs(m, blocks, n, 0.0, d_state + 1, s, nullptr, nullptr, m->d_dt))
+      return rc;
   }
-  return ENZOE_OK;
+  return hipGetLastError() == hipSuccess ? ENZOE_OK : ENZOE_ERR_HIP;
 }
 
 int enzoe_vlct_graph_launch(enzoe_vlct_graph_t g, enzoe_stream_t stream) {

@@ -1525,9 +1525,10 @@ static __global__ void k_dt_init(unsigned long long *result) {
 // cycle that starts at or past t_stop is zero-length: the clipped cycle's
 // t + (t_stop - t) can round above t_stop, and the difference of the next
 // one would then be a (tiny) negative step.
-static __global__ void k_dt_finalize(const unsigned long long *bits,
-                                     double courant, double *time,
-                                     double t_stop, double *dts, int n) {
+// reset: the minimum word starts again from the largest double once read
+// (the cycle's compute then takes the next cycle's minimum into it).
+static __global__ void k_dt_finalize(unsigned long long *bits, double courant, double *time,
+                                     double t_stop, double *dts, int n, int reset) {
   const double t = *time;
   double dt = __longlong_as_double((long long)*bits) * courant;
   if (t < t_stop && t_stop < t + dt) dt = t_stop - t;
@@ -1535,7 +1536,10 @@ static __global__ void k_dt_finalize(const unsigned long long *bits,
   dt = (rem > 0.0) ? (dt < rem ? dt : rem) : 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) dts[i] = dt;
   __syncthreads();
-  if (threadIdx.x == 0) *time = t + dt;
+  if (threadIdx.x == 0) {
+    *time = t + dt;
+    if (reset) *bits = 0x7FEFFFFFFFFFFFFFull;
+  }
 }
 
 // -------------------------------------------------------------- refresh

@@ -240,10 +240,14 @@ int enzoe_vlct_graph_time(enzoe_vlct_graph_t g, enzoe_stream_t stream,
 int enzoe_vlct_graph_destroy(enzoe_vlct_graph_t g);
 
 /* The same `ncycles` cycles enqueued on `stream` without a graph (no host
- * synchronisation either): d_state is n + 1 doubles of device memory, [0]
- * the time (read and advanced by each cycle), [1 .. n] the per-block dt
- * the cycles write.  On this platform a captured graph of these short
- * kernels replays more slowly than the same launches enqueued directly.    */
+ * synchronisation either), with the CFL minimum of each next cycle taken
+ * by the final cell update (as enzoe_vlct_compute_timestep) instead of a
+ * separate pass: the same fields and dt as the graph's cycles, except
+ * that the pressure field ends as the pressure of the final state (a CFL
+ * pass over it would write the same).  d_state is n + 1 doubles of device
+ * memory, [0] the time (read and advanced by each cycle), [1 .. n] the
+ * per-block dt the cycles write.  On this platform a captured graph of
+ * these short kernels replays more slowly than launches enqueued directly. */
 int enzoe_vlct_run_cycles(enzoe_vlct_t m, const enzoe_block *blocks, int nbx, int nby,
                           int nbz, const int bc[3], double *d_state, double t_stop,
                           int ncycles, enzoe_stream_t stream);

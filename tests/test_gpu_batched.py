@@ -215,10 +215,11 @@ def test_graph_replays_match_eager_cycles():
 
 
 def test_device_cycles_without_a_graph_match_eager_cycles():
-    """enzoe_vlct_run_cycles (the graph's launches enqueued directly, device
-    dt, no host synchronisation) equals host-driven eager cycles bit for
-    bit, the last cycle clipped to t_stop and the cycles after it
-    zero-length."""
+    """enzoe_vlct_run_cycles (the cycles enqueued directly, device dt with
+    the next cycle's CFL minimum taken by the final cell update, no host
+    synchronisation) equals host-driven eager cycles bit for bit, the last
+    cycle clipped to t_stop and the cycles after it zero-length; its
+    pressure field is the final state's, what a CFL pass writes."""
     ev = _ev()
     from _states import random_periodic_grid
     grid = random_periodic_grid(mhd=True, seed=13, n=(11, 10, 9), vamp=0.4)
@@ -230,6 +231,7 @@ def test_device_cycles_without_a_graph_match_eager_cycles():
     eager2 = device_copy(grid, method)
     n2, _ = eager2.evolve(0.0, t_stop)
     ncalls, t_end = dev.evolve_device(0.0, t_stop, ncycles=3)
+    eager2.method.timestep(eager2.blocks)  # the pressure of the final state
     ev.synchronize()
     assert t_end == t_stop and n2 == 5 and ncalls == 2
     a, b = dev.blocks[0].download(), eager2.blocks[0].download()
