@@ -55,6 +55,15 @@ template <typename T> __device__ __forceinline__ T dmin3(T a, T b, T c) {
   if (a < b) return (c < a) ? c : a;
   return (c < b) ? c : b;
 }
+// The same minimum of the magnitudes |a|, |b|, |c| with the hardware
+// minimum (IEEE minNum): the two agree on every input, NaN included, except
+// where |b| is NaN (the reference's comparisons then return |b|, minNum
+// would drop it), which the final select restores.  Used by the PLM limiter,
+// whose arguments are differences (never signalling NaNs).
+template <typename T> __device__ __forceinline__ T dmin3_abs(T a, T b, T c) {
+  const T m = __builtin_fmin(__builtin_fmin(fabs(a), fabs(b)), fabs(c));
+  return (b == b) ? m : fabs(b);
+}
 template <typename T> __device__ __forceinline__ T dsign(T v) {
   return (T)((T(0) < v) - (v < T(0)));
 }
@@ -65,8 +74,7 @@ __device__ __forceinline__ T limiter_enzo(T vm1, T v, T vp1, T theta) {
   T dv_c = T(0.5) * (vp1 - vm1);
   T dv_l = (v - vm1) * theta;
   T dv_r = (vp1 - v) * theta;
-  return (T(0.5) * (dsign(dv_l) + dsign(dv_r))) *
-         dmin3(fabs(dv_l), fabs(dv_r), fabs(dv_c));
+  return (T(0.5) * (dsign(dv_l) + dsign(dv_r))) * dmin3_abs(dv_l, dv_r, dv_c);
 }
 
 // PLM_AthenaLimiter (EnzoReconstructorPLM.hpp:337-345)
