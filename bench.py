@@ -290,6 +290,40 @@ def cpu_baseline(n, nsteps):
                          "sample": "same at %d^3, %d steps, %.1f s" % (n1, s1, el1)}}
 
 
+def cpu_baseline_sample(builder, n, mhd, gamma, riemann, nsteps):
+    """The oracle on a sample of a batched workload's own blocks: the first
+    2x2x2 blocks of the domain as a periodic grid of their own (ghost zones
+    refreshed within the sample), `nsteps` cycles with the CFL dt, OpenMP
+    over the z loops of each block on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cfg = po.make_config(gamma=gamma, riemann=riemann, recon=("nn", "plm"), theta=2.0,
+                         mhd=mhd, courant=0.4)
+    order = [(bx, by, bz) for bz in range(2) for by in range(2) for bx in range(2)]
+    blk0, cw = builder(order[0])
+    grid = po.Grid(cfg, (2, 2, 2), (n, n, n), (0.0, 0.0, 0.0),
+                   tuple(2 * n * c for c in cw))
+    for i, ib in enumerate(order):
+        arrs = blk0 if i == 0 else builder(ib)[0]
+        view = grid.block_arrays(i)
+        for k, v in arrs.items():
+            if k in view:
+                view[k][...] = v
+    grid.refresh()
+    grid.cycle(0.0, 1e9, nthreads=threads)  # warm the scratch
+    t0 = time.perf_counter()
+    t = 0.0
+    for _ in range(nsteps):
+        t += grid.cycle(t, 1e9, nthreads=threads)
+    el = time.perf_counter() - t0
+    return {"value": 8 * n ** 3 * nsteps / el, "unit": "zone-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": "oracle (C restatement of the reference loops, OpenMP over z) on "
+                      "2x2x2 of the workload's %d^3 blocks, %d steps, %.1f s"
+                      % (n, nsteps, el)}
+
+
 METRIC = "zone-updates/s/GPU (batched FP32-mixed vs plain FP64 VL+CT) + HBM GB/s"
 
 
@@ -510,6 +544,9 @@ def run_batched_amr(a):
         if a.no_compare:
             break
     r = res[a.precision]
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline_sample(builder, n, mhd, gamma, rs, 3 if n > 64 else 5)
     out = {
         "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
@@ -604,6 +641,8 @@ def run_uniform(a):
             "relative_l1_mixed_vs_fp64": l1, "max_relative_l1": max(l1.values()),
             "l1_mixed_vs_fp64_by_vector_magnitude": l1_mag,
             "max_l1_by_vector_magnitude": max(l1_mag.values())}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     return out
 
 
@@ -749,13 +788,14 @@ def run_output(a):
     }
 
 
-def batched_mixed_leg(steps):
+def batched_mixed_leg(steps, cpu=True):
     """Config 4 (512 blocks of 64^3, decaying Mach-5 MHD turbulence, beta
     0.1): `steps` fused steps in FP64 (its own CFL dt) and in mixed
     precision (the FP64 run's dt sequence, so both states stay comparable;
     the same fused step with its CFL reduction, whose dt is not used), both
     timed over the whole run; then the relative L1 of the mixed state
-    against the FP64 one per field, on the device."""
+    against the FP64 one per field, on the device.  cpu: the oracle timed
+    on a sample of the same blocks beside it."""
     import workloads
     nb, n = (8, 8, 8), 64
     spec = problems.turbulence_spectrum()
@@ -774,6 +814,7 @@ def batched_mixed_leg(steps):
                               bc=("periodic",) * 3)
             workloads.copy_grid_state(grids["fp64"], g)
         grids[precision] = g
+    cpu_rate = cpu_baseline_sample(builder, n, True, 5.0 / 3.0, "hlld", 5) if cpu else None
     del vel
     res, dts = {}, []
     zones = n ** 3 * nb[0] * nb[1] * nb[2]
@@ -808,14 +849,17 @@ def batched_mixed_leg(steps):
                                                     wl)}
     per, mag = grids["mixed"].method.state_l1(grids["mixed"].blocks, grids["fp64"].blocks)
     del grids
-    return {"workload": "turbulence_512x64^3_mach5_beta0.1_vlct_plm_hlld", "steps": steps,
-            "fp64": res["fp64"], "mixed": res["mixed"],
-            "mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
-            "relative_l1_mixed_vs_fp64": per,
-            "l1_mixed_vs_fp64_by_vector_magnitude": mag,
-            "max_l1_by_vector_magnitude": max(mag.values()),
-            "l1_definition": "sum|a-b| / sum|b| over the active cells; for a velocity or "
-                             "B component the denominator is the sum of |b_vec|"}
+    out = {"workload": "turbulence_512x64^3_mach5_beta0.1_vlct_plm_hlld", "steps": steps,
+           "fp64": res["fp64"], "mixed": res["mixed"],
+           "mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
+           "relative_l1_mixed_vs_fp64": per,
+           "l1_mixed_vs_fp64_by_vector_magnitude": mag,
+           "max_l1_by_vector_magnitude": max(mag.values()),
+           "l1_definition": "sum|a-b| / sum|b| over the active cells; for a velocity or "
+                            "B component the denominator is the sum of |b_vec|"}
+    if cpu_rate is not None:
+        out["cpu_baseline"] = cpu_rate
+    return out
 
 
 def main():
@@ -915,7 +959,7 @@ def main():
         # the metric's batched leg: the Mach-5 turbulence workload (512 blocks
         # of 64^3) for its 50 steps in FP64 and in mixed precision, with the
         # mixed state's L1 against the FP64 one per field
-        out["batched_mixed"] = batched_mixed_leg(a.batched_steps)
+        out["batched_mixed"] = batched_mixed_leg(a.batched_steps, cpu=not a.no_cpu_baseline)
     if dual is not None:
         fp64 = r if a.precision == "fp64" else other
         out["dual_energy_comparison"] = {
