@@ -544,9 +544,6 @@ def run_batched_amr(a):
         if a.no_compare:
             break
     r = res[a.precision]
-    cpu = None
-    if not a.no_cpu_baseline:
-        cpu = cpu_baseline_sample(builder, n, mhd, gamma, rs, 3 if n > 64 else 5)
     out = {
         "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
@@ -617,6 +614,9 @@ def run_uniform(a):
                                                       n ** 3 * nblk, cells, precision, mhd,
                                                       "%s_%s" % (a.workload, precision))}
     r = res[a.precision]
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline_sample(builder, n, mhd, gamma, rs, 3 if n > 64 else 5)
     out = {
         "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
