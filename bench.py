@@ -48,7 +48,7 @@ FLUX_BYTES_PER_CELL = {0: (11 + 21) * 8.0, 1: 11 * 8.0 + 21 * 4.0, 2: (11 + 21) 
 FLUX_BYTES_PER_CELL_HD = {0: (5 + 15) * 8.0, 1: 5 * 8.0 + 15 * 4.0, 2: (5 + 15) * 8.0}
 STEP_BYTES_PER_ZONE_HD = 200.0
 WORKLOADS = ["orszag_tang", "fast_wave", "batched_amr", "turbulence", "smr_flux_correct",
-             "kelvin_helmholtz", "output"]
+             "kelvin_helmholtz", "output", "drop_in"]
 
 
 def parse():
@@ -84,7 +84,9 @@ def parse():
                          "smr_flux_correct: 64 x (1 coarse + 4 fine) 32^3 hydro "
                          "blocks with coarse/fine ghost transfers and flux "
                          "corrections; output: HDF5 block output and restart "
-                         "input of the 256^3 Orszag-Tang state")
+                         "input of the 256^3 Orszag-Tang state; drop_in: the C++ "
+                         "layer's Method::timestep + Method::compute on the "
+                         "Orszag-Tang block, fused and stage by stage")
     ap.add_argument("--batched-steps", type=int, default=50,
                     help="steps of the batched mixed-precision leg (turbulence, 512 "
                          "blocks of 64^3) of the default run")
@@ -646,6 +648,62 @@ def run_uniform(a):
     return out
 
 
+def run_drop_in(a):
+    """The C++ drop-in layer (EnzoMethodMHDVlct of libenzoe_vlct_host.so)
+    driven as Cello drives it, per cycle: the ghost refresh,
+    Method::timestep on the block (its CFL pass, dt read back), then
+    Method::compute.  device_fused = true runs the fused launch sequence;
+    false runs the reference's own composition stage by stage through the
+    plugin objects (reconstructor, Riemann solver, CT, integration update),
+    every call one single-operation entry of the C ABI.  Beside them the
+    library's own cycle, whose final cell update also takes the next dt.
+    FP64, PLM + HLLD + CT, the Orszag-Tang block of --n cells."""
+    n = a.n
+    params = ("mhd_choice = constrained_transport; riemann_solver = hlld; "
+              "reconstruct_method = plm; theta_limiter = 2.0; courant = 0.4; "
+              "device_fused = %s")
+    res = {}
+    for mode in ("library", "host_fused", "host_staged"):
+        method = ev.VlctMethod(mhd=True, riemann="hlld", reconstruct="plm", theta=2.0,
+                               courant=0.4, gamma=5.0 / 3.0)
+        grid = make_state(n, method)
+        state = {"dt": method.timestep(grid.blocks)}
+        if mode == "library":
+            def one(grid=grid, state=state):
+                state["dt"] = fused_step(grid, state["dt"])
+        else:
+            hm = ev.HostMethod(params % ("true" if mode == "host_fused" else "false"),
+                               gamma=5.0 / 3.0)
+
+            def one(grid=grid, hm=hm):
+                grid.refresh()
+                hm.compute(grid.blocks, hm.timestep(grid.blocks[0]))
+        for _ in range(a.warmup):
+            one()
+        ev.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            one()
+        ev.synchronize()
+        el = time.perf_counter() - t0
+        res[mode] = {"value": n ** 3 * a.steps / el, "ms_per_step": el * 1e3 / a.steps}
+        print("%s: %.3e zone-updates/s" % (mode, res[mode]["value"]), file=sys.stderr,
+              flush=True)
+        del grid, method, one
+    lib = res["library"]["value"]
+    return {
+        "metric": METRIC, "value": res["host_fused"]["value"], "unit": "zone-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": res["host_fused"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (3D Orszag-Tang vortex, seeded 1e-3 v_z noise)",
+        "config": {"workload": "drop_in_orszag_tang_%d3_fp64_vlct_plm_hlld" % n,
+                   "block": [n, n, n], "driver": "Method::timestep + Method::compute "
+                   "per cycle (C++ layer)"},
+        "drop_in": {mode: dict(r, over_library=r["value"] / lib) for mode, r in res.items()},
+    }
+
+
 def run_smr_flux_correct(a):
     """Static mesh refinement with flux corrections (hydro, the reference's
     VL+CT + flux_correct pairing): 64 units of one coarse 32^3 block and the
@@ -877,7 +935,8 @@ def main():
     if a.workload != "orszag_tang":
         run = {"fast_wave": run_fast_wave, "batched_amr": run_batched_amr,
                "turbulence": run_uniform, "kelvin_helmholtz": run_uniform,
-               "smr_flux_correct": run_smr_flux_correct, "output": run_output}[a.workload]
+               "smr_flux_correct": run_smr_flux_correct, "output": run_output,
+               "drop_in": run_drop_in}[a.workload]
         out = run(a)
         out["copy_kernel_GBs"] = copy_bandwidth()["GBs"]
         print(json.dumps(out))
