@@ -76,6 +76,21 @@ __device__ __forceinline__ T limiter_enzo(T vm1, T v, T vp1, T theta) {
   T dv_r = (vp1 - v) * theta;
   return (T(0.5) * (dsign(dv_l) + dsign(dv_r))) * dmin3_abs(dv_l, dv_r, dv_c);
 }
+// The same limiter in FP32 (the mixed path, judged by its L1 distance to
+// the FP64 result, not bit for bit), with the sign factor taken from the
+// sign bits: equal signs give copysign(m, dv_l), unequal ones 0, and where
+// a difference is zero m is 0 as well.  The result equals the form above
+// but for the sign of a zero and for how a NaN difference spreads; it
+// takes about 12 instructions instead of 20 in the issue-bound FP32 flux
+// kernel.
+__device__ __forceinline__ float limiter_enzo(float vm1, float v, float vp1, float theta) {
+  const float dv_c = 0.5f * (vp1 - vm1);
+  const float dv_l = (v - vm1) * theta;
+  const float dv_r = (vp1 - v) * theta;
+  const float m = __builtin_fminf(__builtin_fminf(fabsf(dv_l), fabsf(dv_r)), fabsf(dv_c));
+  const bool same = ((__float_as_uint(dv_l) ^ __float_as_uint(dv_r)) & 0x80000000u) == 0u;
+  return same ? copysignf(m, dv_l) : 0.0f;
+}
 
 // PLM_AthenaLimiter (EnzoReconstructorPLM.hpp:337-345)
 template <typename T>
